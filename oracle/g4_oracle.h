@@ -1,0 +1,136 @@
+/*
+ * FP64 CPU oracle for the GENESIS 1.3 v4 undulator-step hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is a clean-room restatement of the
+ * reference algorithms, in the reference's own loop and operation order, used
+ * as the checker for the HIP implementation.  Only tests/, the smoke() entry
+ * of __graft_entry__.py and the cpu_baseline leg of bench.py may load it.
+ * Nothing in the product (the genesis-1.3-version4_amd package) links to it.
+ *
+ * Storage is structure-of-arrays with CSR slice offsets; arithmetic per
+ * particle follows the reference's array-of-structs code statement by
+ * statement, so the storage order does not change any result.
+ */
+#ifndef G4_ORACLE_H
+#define G4_ORACLE_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* One integration step of the unrolled lattice (the per-step arrays of
+ * include/Undulator.h:73-77 in the reference, one record per step). */
+typedef struct {
+    double aw, ax, ay, ku, kx, ky, gradx, grady;
+    double qf, qx, qy, cx, cy;
+    double chic_angle, chic_lb, chic_ld, chic_lt;
+    double slip, phaseshift, z, dz;
+    int helical, marker;
+} orc_step;
+
+/* Simulation state.  All arrays are owned by the caller. */
+typedef struct {
+    int nslice;
+    const long long *off;            /* [nslice+1] particle offsets          */
+    double *gamma, *theta, *x, *y, *px, *py;
+    double *current;                 /* [nslice] A                           */
+    double *longESC;                 /* [nslice] eV/m (long-range LSC)       */
+    double *eloss;                   /* [nslice] wake loss (zero here)       */
+    /* radiation field, fundamental only */
+    int ngrid;
+    double gridmax, dgrid, xks, xlambda, slicelength;
+    double *field;                   /* [nslice][ngrid*ngrid] complex (re,im) */
+    int first;
+    double accuslip;
+    /* space-charge solver (EFieldSolver) */
+    int sc_nz, sc_nphi, sc_ngrid, longrange;
+    double sc_rmax, sc_ks;
+    double *sc_efield;               /* [nslice] |celm[0]| of l=1 mode        */
+    double *sc_ez;                   /* scratch [max particles per slice]     */
+    /* beam / control */
+    double reflength;                /* lambda0, length of 2 pi in theta      */
+    double sample;                   /* slice length / reflength              */
+    int timerun, periodic;
+    int ndiag_threads;               /* OpenMP threads over slices (1=serial) */
+} orc_state;
+
+/* Per-slice input of the quiet loader (QuietLoading.h BeamSlice). */
+typedef struct {
+    double gamma, delgam, current, ex, ey, xcen, ycen, pxcen, pycen;
+    double betax, betay, alphax, alphay, bunch, bunchphase, emod, emodphase;
+} orc_slice_params;
+
+/* Per-slice diagnostics written by orc_diag (one output record). */
+typedef struct {
+    double *power, *bunching, *bunchingphase;     /* [nslice] each */
+    double *energy, *energyspread;
+    double *xposition, *yposition, *xsize, *ysize;
+    double *f_xposition, *f_yposition, *f_xsize, *f_ysize;
+    double *nf_intensity, *ff_intensity;
+    double *global_energy;           /* [1] field energy (J), time runs  */
+} orc_diag_out;
+
+/* ---------------- leaf numerics (src/Util) ---------------- */
+void   orc_ran2(unsigned int seed, int n, double *out);
+void   orc_hammersley(int base_index, unsigned int start, int n, double *out);
+double orc_inverfc(double y);
+double orc_besselj(int n, double x);
+
+/* ---------------- undulator helpers (Undulator.cpp) ---------------- */
+double orc_fc(const orc_step *st, int harm);
+double orc_faw(const orc_step *st, double x, double y);
+double orc_faw2(const orc_step *st, double x, double y);
+
+/* ---------------- loaders ---------------- */
+void orc_quiet_load(double *gamma, double *theta, double *x, double *y,
+                    double *px, double *py, const orc_slice_params *sp,
+                    int npart, int nbins, double theta0, int islice);
+/* A shot-noise stream as ShotNoise::init(seed, rank); returns an opaque
+ * handle that orc_shotnoise consumes sequentially. */
+void *orc_shotnoise_new(int seed, int rank);
+void  orc_shotnoise_free(void *h);
+void  orc_shotnoise(void *h, double *theta, int npart, int nbins, double ne);
+void  orc_load_gauss(double *field, int ngrid, double gridmax, double lambda,
+                     double power, double w0, double z0, double phase,
+                     double xcen, double ycen);
+
+/* ---------------- lattice (Lattice.cpp unroll + slippage) -------------- */
+/* Element records laid out along z: type 0=undulator 1=drift 2=quad
+ * 3=corrector 4=chicane 5=marker 6=phaseshifter.  par[8] per element:
+ * undulator: aw, lambdau, nwig, kx, ky, ax, ay, helical(0/1) (+gradx/grady 0)
+ * quad: k1, dx, dy; corrector: cx, cy; chicane: delay, lb, ld;
+ * marker: action; phaseshifter: phi.  Returns number of steps written (<=
+ * max_steps), or -1 if max_steps is too small. */
+int orc_unroll(int nele, const int *type, const double *z, const double *l,
+               const double *par, double delz, double lambda, double gamma,
+               orc_step *out, int max_steps);
+
+/* ---------------- one integration step (Gencore.cpp:102-160) ---------- */
+void orc_track_transverse(orc_state *s, const orc_step *st, double gammaref,
+                          double delz, int last);
+void orc_lsc_long(orc_state *s, double gammaref, double aw);
+void orc_advance(orc_state *s, const orc_step *st, double gammaref, double delz);
+void orc_adi_factors(int ngrid, double delz, double dgrid, double xks,
+                     double *c, double *cbet, double *cwet, double *cstep_im);
+void orc_deposit_slice(const orc_state *s, const orc_step *st, int ii,
+                       double delz, double *src);
+void orc_adi_slice(int ngrid, const double *c, const double *cbet,
+                   const double *cwet, double cstep_im, const double *src,
+                   double *u, double *r);
+void orc_field_advance(orc_state *s, const orc_step *st, double delz);
+void orc_slippage(orc_state *s, double slip);
+void orc_beam_track(orc_state *s, const orc_step *st, double gammaref, double delz);
+void orc_step_all(orc_state *s, const orc_step *st, double gammaref);
+void orc_diag(const orc_state *s, orc_diag_out *d);
+int  orc_sort_index(const double *theta, int n, double slen, int *atar);
+
+/* Whole run: initial diagnostics, then every step of the lattice with one
+ * output record per step (output_step=1).  Diagnostic arrays are
+ * [nsteps+1][nslice]. */
+void orc_run(orc_state *s, const orc_step *steps, int nsteps, double gammaref,
+             orc_diag_out *per_record /* array of nsteps+1 records */);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
