@@ -1,0 +1,183 @@
+#!/usr/bin/env python3
+"""Benchmark of the undulator integration step on one MI355X.
+
+A step is one pass of the hot path over the whole time window: transverse
+half step, space charge, longitudinal RK4 push, transverse half step, source
+deposition, ADI field advance, slippage and the per-step P(z)/|b(z)|
+diagnostics -- Gencore::run's loop body with output_step=1.  The workload
+is config 1 (time-dependent SASE, FP64 plain form, 1024 slices x 8192
+macroparticles, 151x151 field, long- and short-range space charge), with
+beam and field already resident in HBM when the timed region starts.
+
+Prints one JSON line.  The CPU baseline is the FP64 oracle (a clean-room
+C++ restatement of the reference's loops, OpenMP over slices like the
+reference's MPI slice blocks) timed on this host on a bounded sample.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "genesis-1.3-version4_amd", "python"))
+
+import numpy as np  # noqa: E402
+
+import g4mi  # noqa: E402
+import g4mi_workloads as W  # noqa: E402
+
+HBM_PEAK_GBS = 8000.0
+
+
+def build_workload(cfg):
+    ns, npart = cfg["nslice"], cfg["npart"]
+    off = np.arange(ns + 1, dtype=np.int64) * npart
+    slices = [W.slice_beam() for _ in range(ns)]
+    beam = g4mi.load_beam(off, slices, cfg["nbins"], cfg["shotnoise"], cfg["seed"], W.LAMBDA_R,
+                          1.0, cfg["timerun"])
+    steps = g4mi.parse_lattice(W.lattice_text(), "FEL", W.DELZ, W.LAMBDA_R, W.GAMMA0)
+    ng = cfg["ngrid"]
+    field = np.zeros((ns, ng * ng, 2))
+    if cfg["seed_power"] > 0:
+        f0 = g4mi.load_gauss(ng, cfg["gridmax"], W.LAMBDA_R, cfg["seed_power"], 30e-6)
+        field[:] = f0
+    current = np.full(ns, W.CURRENT)
+    return off, beam, field, current, steps
+
+
+def make_context(cfg, off, beam, field, current):
+    ctx = g4mi.Context(off, cfg["ngrid"], cfg["gridmax"], W.LAMBDA_R, 1.0, cfg["timerun"],
+                       False, g4mi.FP64)
+    ctx.set_efield(cfg["longrange"], cfg["sc_nz"], cfg["sc_nphi"], cfg["sc_ngrid"], 0.0)
+    ctx.upload_beam(beam, current)
+    ctx.upload_field(field, 0)
+    return ctx
+
+
+def gpu_measure(cfg, steps_k, warmup):
+    off, beam, field, current, steps = build_workload(cfg)
+    ctx = make_context(cfg, off, beam, field, current)
+    nst = len(steps)
+    if warmup + steps_k > nst:
+        raise SystemExit(f"warmup+steps must be <= {nst} (lattice length)")
+    warm = (g4mi.StepParams * warmup)(*steps[:warmup])
+    timed = (g4mi.StepParams * steps_k)(*steps[warmup:warmup + steps_k])
+    rec_keys = ("power", "bunching")
+    ns = cfg["nslice"]
+    bufs = {k: np.zeros((steps_k, ns)) for k in rec_keys}
+    rows = (g4mi.DiagOut * steps_k)()
+    for i in range(steps_k):
+        rows[i] = g4mi.DiagOut(*[bufs[k][i].ctypes.data_as(g4mi._dp) if k in bufs else None
+                                 for k in g4mi.DIAG_FIELDS])
+    L = g4mi.lib()
+    if warmup:
+        g4mi._chk(L.g4mi_run(ctx.h, warm, warmup, W.GAMMA0, None), "warmup")
+    ctx.sync()
+    ctx.set_timing(True)
+    ctx.sync()
+    t0 = time.perf_counter()
+    done = g4mi._chk(L.g4mi_run(ctx.h, timed, steps_k, W.GAMMA0, rows), "timed run")
+    ctx.sync()
+    t1 = time.perf_counter()
+    kt = ctx.kernel_times()
+    ctx.set_timing(False)
+    assert done == steps_k
+    dz = sum(s.dz for s in timed)
+    ntot = int(off[-1])
+    ctx.close()
+    return dict(seconds=t1 - t0, ntot=ntot, dz=dz, kt=kt, power=bufs["power"],
+                bunching=bufs["bunching"])
+
+
+def roofline(cfg, kt, ntot):
+    """Dominant kernel and its algorithmic bytes per launch:
+    push: read gamma, theta, x, y, px, py + write gamma, theta = 64 B/particle;
+    ADI (x then y sweep of every cell): field read + write = 32 B/cell;
+    deposit: read x, y, theta, gamma = 32 B/particle;
+    transverse (two half steps): 2 x (read 5 + write 4 doubles) = 144 B/particle."""
+    cells = cfg["nslice"] * cfg["ngrid"] ** 2
+    per = {"push": 64.0 * ntot, "adi": 32.0 * cells, "deposit": 32.0 * ntot,
+           "transverse": 144.0 * ntot}
+    name = max(per, key=lambda k: kt[k])
+    ms = kt[name]
+    gbs = per[name] / (ms * 1e-3) / 1e9
+    return name, dict(bound="hbm", achieved=round(gbs, 1), peak=HBM_PEAK_GBS, unit="GB/s",
+                      frac=round(gbs / HBM_PEAK_GBS, 4), traffic=None,
+                      kernel=name, ms_per_launch=round(ms, 5),
+                      algorithmic_bytes=int(per[name]))
+
+
+def cpu_baseline(cfg, nsteps):
+    """The FP64 oracle on this host's cores over the first nsteps steps of
+    the same workload (all slices)."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import ctypes as C
+    import oracle_py as O
+    threads = min(16, os.cpu_count() or 1)
+    ns, npart = cfg["nslice"], cfg["npart"]
+    sim = O.Sim([npart] * ns, cfg["ngrid"], cfg["gridmax"], W.LAMBDA_R, timerun=cfg["timerun"],
+                sc=(cfg["sc_nz"], cfg["sc_nphi"], cfg["sc_ngrid"], 0.0),
+                longrange=cfg["longrange"], threads=threads)
+    O.load_beam(sim, [W.slice_beam()] * ns, None, cfg["nbins"],
+                cfg["seed"] if cfg["shotnoise"] and cfg["timerun"] else None, lambda0=W.LAMBDA_R)
+    steps = O.unroll(W.fodo_elements(), W.DELZ, W.LAMBDA_R, W.GAMMA0)
+    t0 = time.perf_counter()
+    for i in range(nsteps):
+        O.lib().orc_step_all(C.byref(sim.s), C.byref(steps[i]), W.GAMMA0)
+        sim.diag()
+    dt = time.perf_counter() - t0
+    return dict(value=round(ns * npart * nsteps / dt, 1), unit="macroparticle-steps/s",
+                cores=threads, kind="port",
+                sample=f"{nsteps} steps of {cfg['name']} (all {ns} slices x {npart}), "
+                       f"{dt:.2f} s on {threads} threads")
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--config", choices=("config1", "config0"), default="config1")
+    ap.add_argument("--cpu-steps", type=int, default=2)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    a = ap.parse_args()
+    if a.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU only; the slice-block sharding "
+                                   "across GPUs is a design, not an implementation",
+                          "n_gpus": a.gpus}))
+        return 2
+    cfg = W.CONFIG1 if a.config == "config1" else W.CONFIG0
+    m = gpu_measure(cfg, a.steps, a.warmup)
+    rate = m["ntot"] * a.steps / m["seconds"]
+    cells = cfg["nslice"] * cfg["ngrid"] ** 2
+    kname, roof = roofline(cfg, m["kt"], m["ntot"])
+    out = {
+        "metric": "macroparticle-steps/s",
+        "value": round(rate, 1),
+        "unit": "macroparticle-steps/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": round(1e3 * m["seconds"] / a.steps, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic: Hammersley quiet loading + shot noise (seed 7), as LoadBeam",
+        "config": {"workload": W.describe(cfg), "slices": cfg["nslice"],
+                   "macroparticles_per_slice": cfg["npart"], "ngrid": cfg["ngrid"],
+                   "precision": "FP64 plain form", "parallelism": "single GPU"},
+        "field_cell_updates_per_s": round(cells * a.steps / m["seconds"], 1),
+        "s_per_undulator_metre": round(m["seconds"] / m["dz"], 5),
+        "kernel_ms_per_step": {k: round(v, 5) for k, v in m["kt"].items() if k != "steps"},
+        "roofline": roof,
+    }
+    if not a.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline(cfg, a.cpu_steps)
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

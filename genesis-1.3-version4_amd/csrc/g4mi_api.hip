@@ -1,0 +1,654 @@
+// C-ABI of the g4mi library: device context and the step entry points.
+//
+// The host side here mirrors the reference's step sequencing
+// (Gencore::run, src/Core/Gencore.cpp:102-160): beam track, field track,
+// slippage, diagnostics.  All particle- and cell-sized work is done by the
+// kernels in g4mi_kernels.hip on the context's stream; only per-step scalars
+// are prepared on the host.
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "g4mi_host.h"
+#include "g4mi_kernels.h"
+
+namespace g4mi {
+
+thread_local std::string g_last_error;
+void set_error(const std::string &msg) { g_last_error = msg; }
+
+struct Ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    int precision = G4MI_FP64;
+    int ns = 0, ng = 0;
+    double gridmax = 0, dgrid = 0, xks = 0, xlambda = 0, lambda0 = 0, sample = 1, slicelength = 0;
+    int timerun = 1, periodic = 0;
+    std::vector<long long> off;
+    long long ntot = 0, maxn = 0;
+    // device arrays
+    long long *d_off = nullptr;
+    double *d_g = nullptr, *d_th = nullptr, *d_x = nullptr, *d_y = nullptr, *d_px = nullptr,
+           *d_py = nullptr;
+    double *d_cur = nullptr, *d_esc = nullptr, *d_ez = nullptr, *d_mom = nullptr,
+           *d_fsize = nullptr, *d_cen = nullptr, *d_rbound = nullptr, *d_dr = nullptr,
+           *d_rmax = nullptr, *d_efield = nullptr;
+    double2 *d_field = nullptr, *d_work = nullptr, *d_src = nullptr;
+    int4 *d_box = nullptr;
+    double2 *d_fac = nullptr;  // c | cbet | cwet, ngrid each
+    int nblk = 0;
+    int *d_blk_slice = nullptr;
+    long long *d_blk_begin = nullptr;
+    double *d_diag_b = nullptr, *d_diag_f = nullptr;
+    unsigned long long *d_count = nullptr;
+    int *d_atar = nullptr;
+    // state
+    int first = 0;
+    double accuslip = 0;
+    bool has_beam = false, has_field = false;
+    // space charge
+    int longrange = 0, sc_nz = 0, sc_nphi = 0, sc_ng = 100;
+    double sc_rmax = 0;
+    // ADI factor cache
+    double adi_delz = 0, cstep = 0;
+    // timing: kEv events per step, recorded into a pool and folded lazily
+    static constexpr int kEv = 9;
+    static constexpr int kPoolSteps = 512;
+    bool timing = false;
+    std::vector<hipEvent_t> pool;
+    int pool_used = 0;  // steps recorded in the pool
+    double t_phase[G4MI_NTIMERS] = {};
+    long long n_timed = 0;
+    hipEvent_t *ev = nullptr;  // events of the step being recorded
+    std::vector<void *> allocs;
+};
+
+template <class T>
+static int dalloc(Ctx *c, T **p, size_t n) {
+    if (n == 0) n = 1;
+    void *q = nullptr;
+    hipError_t e = hipMalloc(&q, n * sizeof(T));
+    if (e != hipSuccess) {
+        set_error(std::string("hipMalloc: ") + hipGetErrorString(e));
+        return G4MI_E_HIP;
+    }
+    c->allocs.push_back(q);
+    *p = (T *)q;
+    return G4MI_OK;
+}
+
+#define G4_TRY(x)                \
+    do {                         \
+        int r_ = (x);            \
+        if (r_ != G4MI_OK) return r_; \
+    } while (0)
+
+static TransParams transverse_params(const Ctx *c, const g4mi_step_params &st, double gamma0,
+                                     double delz, bool last) {
+    TransParams p{};
+    const bool inund = st.aw > 0;  // Undulator::getUndulatorParameters
+    const double aw = inund ? st.aw : 0, dax = inund ? st.ax : 0, day = inund ? st.ay : 0;
+    const double kx = inund ? st.kx : 0, ky = inund ? st.ky : 0;
+    const double qf = st.qf;
+    const double dqx = qf == 0 ? 0 : st.qx, dqy = qf == 0 ? 0 : st.qy;
+    const double betpar0 = sqrt(1 - (1 + aw * aw) / gamma0 / gamma0);
+    const double qquad = qf * gamma0;
+    const double qnatx = kx * aw * aw / gamma0 / betpar0;
+    const double qnaty = ky * aw * aw / gamma0 / betpar0;
+    p.aw = aw;
+    p.delz = delz;
+    p.qx = qquad + qnatx;
+    p.qy = -qquad + qnaty;
+    p.xoff = qquad * dqx + qnatx * dax;
+    p.yoff = -qquad * dqy + qnaty * day;
+    if (last) {
+        if (st.cx != 0 || st.cy != 0) {
+            p.corrector = 1;
+            p.ccx = st.cx * gamma0;
+            p.ccy = st.cy * gamma0;
+        }
+    } else if (st.chic_angle != 0) {
+        double m[16];
+        chicane_matrix(st.chic_angle, st.chic_lb, st.chic_ld, st.chic_lt, m);
+        p.chicane = 1;
+        p.m00 = m[0];
+        p.m01 = m[1];
+        p.m10 = m[4];
+        p.m11 = m[5];
+        p.m22 = m[10];
+        p.m23 = m[11];
+        p.m32 = m[14];
+        p.m33 = m[15];
+    }
+    if (p.qx != 0) {
+        p.xoff = p.xoff / p.qx;
+        p.kindx = p.qx > 0 ? 1 : -1;
+    }
+    if (p.qy != 0) {
+        p.yoff = p.yoff / p.qy;
+        p.kindy = p.qy > 0 ? 1 : -1;
+    }
+    (void)c;
+    return p;
+}
+
+static void build_block_table(const std::vector<long long> &off, std::vector<int> &bs,
+                              std::vector<long long> &bb) {
+    bs.clear();
+    bb.clear();
+    for (size_t is = 0; is + 1 < off.size(); is++)
+        for (long long a = off[is]; a < off[is + 1]; a += 256) {
+            bs.push_back((int)is);
+            bb.push_back(a);
+        }
+}
+
+static int ensure_factors(Ctx *c, double delz) {
+    if (delz == c->adi_delz) return G4MI_OK;  // FieldSolverADI::init caching on delz
+    c->adi_delz = delz;
+    std::vector<double> h(6 * (size_t)c->ng);
+    c->cstep = adi_factors(c->ng, delz, c->dgrid, c->xks, h.data(), h.data() + 2 * c->ng,
+                           h.data() + 4 * c->ng);
+    G4_HIP(hipMemcpyAsync(c->d_fac, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice,
+                          c->stream));
+    G4_HIP(hipStreamSynchronize(c->stream));
+    return G4MI_OK;
+}
+
+static int beam_track(Ctx *c, const g4mi_step_params &st, double gamma0, double delz) {
+    if (!c->has_beam) {
+        set_error("beam not uploaded");
+        return G4MI_E_STATE;
+    }
+    TransParams t1 = transverse_params(c, st, gamma0, delz * 0.5, false);
+    if (c->ev) G4_HIP(hipEventRecord(c->ev[0], c->stream));
+    launch_transverse(c->stream, c->ntot, c->d_g, c->d_x, c->d_y, c->d_px, c->d_py, t1);
+
+    // BeamSolver::advance
+    if (c->ev) G4_HIP(hipEventRecord(c->ev[1], c->stream));
+    const double aw = st.aw;
+    const bool sc = (c->sc_nz > 0) && (c->sc_ng > 2);
+    if (c->longrange || sc) {
+        launch_slice_moments(c->stream, c->ns, c->d_off, c->d_x, c->d_y, c->d_mom);
+        launch_slice_sizes(c->stream, c->ns, c->d_off, c->d_cur, c->d_mom, c->d_fsize, c->d_cen);
+    }
+    if (c->longrange) {
+        const double gamma = gamma0 / sqrt(1 + aw * aw);
+        const double scl = c->slicelength / 2. / asin(1) / 299792458.0 / 2 / 8.85e-12;
+        launch_lsc_long(c->stream, c->ns, c->d_cur, c->d_fsize, c->slicelength, gamma, scl,
+                        c->d_esc);
+    }
+    if (sc) {
+        launch_sc_rbound(c->stream, c->ns, c->d_off, c->d_x, c->d_y, c->d_cen, c->d_rbound);
+        launch_sc_scan(c->stream, c->ns, c->d_off, c->d_rbound, c->d_rmax, c->sc_ng, c->d_dr);
+        const double gz2 = gamma0 * gamma0 / (1 + aw * aw);
+        const double ks = 4 * asin(1) / c->lambda0;
+        launch_sc_solve(c->stream, c->ns, c->d_off, c->d_x, c->d_y, c->d_th, c->d_cur, c->d_cen,
+                        c->d_dr, c->sc_ng, c->sc_nz, c->sc_nphi, ks, gz2, c->d_ez, c->d_efield);
+    }
+    if (c->ev) G4_HIP(hipEventRecord(c->ev[2], c->stream));
+    PushParams p{};
+    p.delz = delz;
+    p.xks = c->xks;
+    p.rtmp = coupling(st, 1) / c->xks;
+    p.xku = st.ku;
+    if (p.xku == 0) p.xku = c->xks * 0.5 / gamma0 / gamma0;
+    p.aw = aw;
+    p.autophase = st.phaseshift;
+    p.ax = st.ax;
+    p.ay = st.ay;
+    p.kx = st.kx;
+    p.ky = st.ky;
+    p.gradx = st.gradx;
+    p.grady = st.grady;
+    p.gridmax = c->gridmax;
+    p.dgrid = c->dgrid;
+    p.ngrid = c->ng;
+    p.first = c->first;
+    p.nslice = c->ns;
+    p.use_sc = sc ? 1 : 0;
+    launch_push(c->stream, c->nblk, c->d_blk_slice, c->d_blk_begin, c->d_off, c->d_g, c->d_th,
+                c->d_x, c->d_y, c->d_px, c->d_py, c->d_field, c->d_esc, c->d_ez, p);
+    if (c->ev) G4_HIP(hipEventRecord(c->ev[3], c->stream));
+    if (st.chic_angle != 0) {
+        const double r56 = chicane_r56(st.chic_angle, st.chic_lb, st.chic_ld, c->lambda0, gamma0);
+        launch_r56(c->stream, c->ntot, c->d_g, c->d_th, r56, gamma0);
+    }
+    TransParams t2 = transverse_params(c, st, gamma0, delz * 0.5, true);
+    launch_transverse(c->stream, c->ntot, c->d_g, c->d_x, c->d_y, c->d_px, c->d_py, t2);
+    if (c->ev) G4_HIP(hipEventRecord(c->ev[4], c->stream));
+    G4_HIP(hipGetLastError());
+    return G4MI_OK;
+}
+
+static int field_track(Ctx *c, const g4mi_step_params &st, double delz) {
+    if (!c->has_field) {
+        set_error("field not uploaded");
+        return G4MI_E_STATE;
+    }
+    G4_TRY(ensure_factors(c, delz));
+    if (c->ev) G4_HIP(hipEventRecord(c->ev[5], c->stream));
+    DepositParams d{};
+    d.enabled = st.aw > 0 ? 1 : 0;  // in undulator, fundamental
+    d.fcoup = coupling(st, 1);
+    d.xks = c->xks;
+    d.delz = delz;
+    d.dgrid = c->dgrid;
+    d.gridmax = c->gridmax;
+    d.ax = st.ax;
+    d.ay = st.ay;
+    d.kx = st.kx;
+    d.ky = st.ky;
+    d.gradx = st.gradx;
+    d.grady = st.grady;
+    d.ngrid = c->ng;
+    launch_deposit(c->stream, c->ns, c->d_off, c->d_g, c->d_th, c->d_x, c->d_y, c->d_cur,
+                   c->d_src, c->d_box, d);
+    if (c->ev) G4_HIP(hipEventRecord(c->ev[6], c->stream));
+    AdiFactors f{c->d_fac, c->d_fac + c->ng, c->d_fac + 2 * c->ng, c->cstep, c->ng};
+    launch_adi(c->stream, c->ns, c->d_field, c->d_work, c->d_src, c->d_box, f, c->first);
+    G4_HIP(hipGetLastError());
+    if (c->ev) G4_HIP(hipEventRecord(c->ev[7], c->stream));
+    return G4MI_OK;
+}
+
+static int slippage(Ctx *c, double slip) {
+    if (!c->timerun) return G4MI_OK;  // Control::applySlippage :159
+    c->accuslip += slip;
+    int direction = 1;
+    const long long cells = (long long)c->ng * c->ng;
+    while (fabs(c->accuslip) > (c->sample * 0.8)) {
+        if (c->accuslip < 0) direction = -1;
+        c->accuslip -= c->sample * direction;
+        int last = (c->first + c->ns - 1) % c->ns;
+        if (direction < 0) last = (last + 1) % c->ns;
+        if (!c->periodic) launch_zero(c->stream, c->d_field + (long long)last * cells, cells);
+        c->first = last;
+        if (direction < 0) c->first = (last + 1) % c->ns;
+    }
+    return G4MI_OK;
+}
+
+static int diag(Ctx *c, g4mi_diag_out *o) {
+    const int ns = c->ns, ng = c->ng;
+    launch_diag_beam(c->stream, ns, c->d_off, c->d_g, c->d_th, c->d_x, c->d_y, c->d_diag_b);
+    launch_diag_field(c->stream, ns, c->d_field, ng, c->d_diag_f);
+    std::vector<double> hb(8 * (size_t)ns), hf(8 * (size_t)ns);
+    G4_HIP(hipMemcpyAsync(hb.data(), c->d_diag_b, hb.size() * 8, hipMemcpyDeviceToHost, c->stream));
+    G4_HIP(hipMemcpyAsync(hf.data(), c->d_diag_f, hf.size() * 8, hipMemcpyDeviceToHost, c->stream));
+    G4_HIP(hipStreamSynchronize(c->stream));
+    // DiagBeam::getValues normalisation (Diagnostic.cpp:452-489)
+    for (int is = 0; is < ns; is++) {
+        const long long n = c->off[is + 1] - c->off[is];
+        const double norm = n > 0 ? 1. / (double)n : 1.;
+        const double *v = &hb[8 * is];
+        const double g1 = v[0] * norm, g2 = v[1] * norm, x1 = v[2] * norm, x2 = v[3] * norm;
+        const double y1 = v[4] * norm, y2 = v[5] * norm, bre = v[6] * norm, bim = v[7] * norm;
+        if (o->energy) o->energy[is] = g1;
+        if (o->energyspread) o->energyspread[is] = sqrt(fabs(g2 - g1 * g1));
+        if (o->xposition) o->xposition[is] = x1;
+        if (o->xsize) o->xsize[is] = sqrt(fabs(x2 - x1 * x1));
+        if (o->yposition) o->yposition[is] = y1;
+        if (o->ysize) o->ysize[is] = sqrt(fabs(y2 - y1 * y1));
+        if (o->bunching) o->bunching[is] = hypot(bre, bim);
+        if (o->bunchingphase) o->bunchingphase[is] = atan2(bim, bre);
+    }
+    // DiagField::getValues (Diagnostic.cpp:754-969)
+    const double ks = 4. * asin(1) / c->xlambda;
+    const double scl = c->dgrid * kEev / ks;
+    double g_pow = 0;
+    for (int is0 = 0; is0 < ns; is0++) {
+        const int is = (ns + is0 - c->first) % ns;
+        const double *v = &hf[8 * is0];
+        double power = v[0], x1 = v[1], x2 = v[2], y1 = v[3], y2 = v[4];
+        g_pow += power;
+        if (power > 0) {
+            x1 /= power;
+            x2 /= power;
+            y1 /= power;
+            y2 /= power;
+        }
+        x2 = sqrt(fabs(x2 - x1 * x1));
+        y2 = sqrt(fabs(y2 - y1 * y1));
+        double farfield = v[5] * v[5] + v[6] * v[6];
+        farfield *= c->dgrid * c->dgrid;
+        double inten = v[7];
+        power *= scl * scl / kVacimp;
+        inten *= kEev * kEev / ks / ks / kVacimp;
+        if (o->power) o->power[is] = power;
+        if (o->f_xposition) o->f_xposition[is] = x1 * c->dgrid;
+        if (o->f_xsize) o->f_xsize[is] = x2 * c->dgrid;
+        if (o->f_yposition) o->f_yposition[is] = y1 * c->dgrid;
+        if (o->f_ysize) o->f_ysize[is] = y2 * c->dgrid;
+        if (o->nf_intensity) o->nf_intensity[is] = inten;
+        if (o->ff_intensity) o->ff_intensity[is] = farfield;
+    }
+    if (o->global_energy) {
+        g_pow *= scl * scl / kVacimp * c->slicelength / 299792458.0;
+        o->global_energy[0] = g_pow;
+    }
+    return G4MI_OK;
+}
+
+// Fold the recorded events of the pool into the per-phase totals.
+static int fold_timers(Ctx *c) {
+    if (c->pool_used == 0) return G4MI_OK;
+    G4_HIP(hipStreamSynchronize(c->stream));
+    for (int k = 0; k < c->pool_used; k++) {
+        hipEvent_t *e = &c->pool[(size_t)k * Ctx::kEv];
+        float d[8] = {};
+        for (int j = 0; j < 8; j++) G4_HIP(hipEventElapsedTime(&d[j], e[j], e[j + 1]));
+        // e0 trans1 e1 sc e2 push e3 trans2 e4 host e5 deposit e6 adi e7 slip+diag e8
+        c->t_phase[0] += d[0] + d[3];
+        c->t_phase[1] += d[1];
+        c->t_phase[2] += d[2];
+        c->t_phase[3] += d[5];
+        c->t_phase[4] += d[6];
+        c->t_phase[5] += d[7];
+        c->n_timed++;
+    }
+    c->pool_used = 0;
+    return G4MI_OK;
+}
+
+static int step_begin_timing(Ctx *c) {
+    c->ev = nullptr;
+    if (!c->timing) return G4MI_OK;
+    if (c->pool_used == Ctx::kPoolSteps) G4_TRY(fold_timers(c));
+    c->ev = &c->pool[(size_t)c->pool_used * Ctx::kEv];
+    return G4MI_OK;
+}
+
+static int step_end_timing(Ctx *c) {
+    if (!c->ev) return G4MI_OK;
+    G4_HIP(hipEventRecord(c->ev[8], c->stream));
+    c->ev = nullptr;
+    c->pool_used++;
+    return G4MI_OK;
+}
+
+static int step(Ctx *c, const g4mi_step_params &st, double gamma0) {
+    const double delz = st.dz;
+    G4_TRY(step_begin_timing(c));
+    G4_TRY(beam_track(c, st, gamma0, delz));
+    G4_TRY(field_track(c, st, delz));
+    G4_TRY(slippage(c, st.slip));
+    return G4MI_OK;
+}
+
+}  // namespace g4mi
+
+using namespace g4mi;
+
+struct g4mi_ctx {
+    Ctx c;
+};
+
+extern "C" {
+
+const char *g4mi_last_error(void) { return g_last_error.c_str(); }
+
+int g4mi_create(g4mi_ctx **out, int device, int nslice, const long long *off, int ngrid,
+                double gridmax, double lambda0, double sample, int timerun, int periodic,
+                int precision) {
+    if (!out || nslice < 1 || !off || ngrid < 3 || gridmax <= 0 || lambda0 <= 0 || sample <= 0) {
+        set_error("g4mi_create: invalid argument");
+        return G4MI_E_ARG;
+    }
+    if (ngrid > 4096) {
+        set_error("g4mi_create: ngrid above 4096 is not supported");
+        return G4MI_E_ARG;
+    }
+    if (precision != G4MI_FP64) {
+        set_error("g4mi_create: precision not available in this build");
+        return G4MI_E_ARG;
+    }
+    for (int i = 0; i < nslice; i++)
+        if (off[i + 1] < off[i] || off[0] != 0) {
+            set_error("g4mi_create: slice offsets must start at 0 and be non-decreasing");
+            return G4MI_E_ARG;
+        }
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1 || device < 0 || device >= ndev) {
+        set_error("g4mi_create: no usable HIP device");
+        return G4MI_E_NODEV;
+    }
+    G4_HIP(hipSetDevice(device));
+    g4mi_ctx *h = new g4mi_ctx;
+    Ctx *c = &h->c;
+    *out = nullptr;
+    c->device = device;
+    c->precision = precision;
+    c->ns = nslice;
+    c->ng = ngrid;
+    c->gridmax = gridmax;
+    c->dgrid = 2 * gridmax / (double)(ngrid - 1);  // Field::init :45-46
+    c->lambda0 = lambda0;
+    c->xlambda = lambda0 / 1.0;
+    c->xks = 4. * asin(1) / c->xlambda;
+    c->sample = sample;
+    c->slicelength = sample * lambda0;
+    c->timerun = timerun;
+    c->periodic = periodic;
+    c->off.assign(off, off + nslice + 1);
+    c->ntot = off[nslice];
+    for (int i = 0; i < nslice; i++) c->maxn = std::max(c->maxn, off[i + 1] - off[i]);
+    int rc = G4MI_OK;
+    auto fail = [&](int r) {
+        g4mi_destroy(h);
+        return r;
+    };
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) return fail(G4MI_E_HIP);
+    const size_t n = (size_t)c->ntot, ns = (size_t)nslice, cells = (size_t)ngrid * ngrid;
+    if ((rc = dalloc(c, &c->d_off, ns + 1)) || (rc = dalloc(c, &c->d_g, n)) ||
+        (rc = dalloc(c, &c->d_th, n)) || (rc = dalloc(c, &c->d_x, n)) ||
+        (rc = dalloc(c, &c->d_y, n)) || (rc = dalloc(c, &c->d_px, n)) ||
+        (rc = dalloc(c, &c->d_py, n)) || (rc = dalloc(c, &c->d_ez, n)) ||
+        (rc = dalloc(c, &c->d_cur, ns)) || (rc = dalloc(c, &c->d_esc, ns)) ||
+        (rc = dalloc(c, &c->d_mom, 4 * ns)) || (rc = dalloc(c, &c->d_fsize, ns)) ||
+        (rc = dalloc(c, &c->d_cen, 2 * ns)) || (rc = dalloc(c, &c->d_rbound, ns)) ||
+        (rc = dalloc(c, &c->d_dr, ns)) || (rc = dalloc(c, &c->d_rmax, 1)) ||
+        (rc = dalloc(c, &c->d_efield, ns)) || (rc = dalloc(c, &c->d_field, ns * cells)) ||
+        (rc = dalloc(c, &c->d_work, ns * cells)) || (rc = dalloc(c, &c->d_src, ns * cells)) ||
+        (rc = dalloc(c, &c->d_box, ns)) || (rc = dalloc(c, &c->d_fac, 3 * (size_t)ngrid)) ||
+        (rc = dalloc(c, &c->d_diag_b, 8 * ns)) || (rc = dalloc(c, &c->d_diag_f, 8 * ns)) ||
+        (rc = dalloc(c, &c->d_count, 1)))
+        return fail(rc);
+    std::vector<int> bs;
+    std::vector<long long> bb;
+    build_block_table(c->off, bs, bb);
+    c->nblk = (int)bs.size();
+    if ((rc = dalloc(c, &c->d_blk_slice, bs.size())) || (rc = dalloc(c, &c->d_blk_begin, bb.size())))
+        return fail(rc);
+    if (hipMemcpy(c->d_off, off, (ns + 1) * sizeof(long long), hipMemcpyHostToDevice) != hipSuccess ||
+        (c->nblk && hipMemcpy(c->d_blk_slice, bs.data(), bs.size() * sizeof(int), hipMemcpyHostToDevice) != hipSuccess) ||
+        (c->nblk && hipMemcpy(c->d_blk_begin, bb.data(), bb.size() * sizeof(long long), hipMemcpyHostToDevice) != hipSuccess) ||
+        hipMemset(c->d_esc, 0, ns * sizeof(double)) != hipSuccess ||
+        hipMemset(c->d_ez, 0, (n ? n : 1) * sizeof(double)) != hipSuccess ||
+        hipMemset(c->d_efield, 0, ns * sizeof(double)) != hipSuccess ||
+        hipMemset(c->d_rmax, 0, sizeof(double)) != hipSuccess) {
+        set_error("g4mi_create: initial copies failed");
+        return fail(G4MI_E_HIP);
+    }
+    *out = h;
+    return G4MI_OK;
+}
+
+int g4mi_destroy(g4mi_ctx *h) {
+    if (!h) return G4MI_OK;
+    Ctx *c = &h->c;
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    for (void *p : c->allocs) (void)hipFree(p);
+    for (auto &e : c->pool)
+        if (e) (void)hipEventDestroy(e);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete h;
+    return G4MI_OK;
+}
+
+int g4mi_set_efield(g4mi_ctx *h, int longrange, int nz, int nphi, int ngrid_r, double rmax) {
+    if (!h || nz < 0 || nphi < 0 || ngrid_r < 1 || ngrid_r > 256) {
+        set_error("g4mi_set_efield: invalid argument (radial grid limited to 256)");
+        return G4MI_E_ARG;
+    }
+    Ctx *c = &h->c;
+    c->longrange = longrange ? 1 : 0;
+    c->sc_nz = nz;
+    c->sc_nphi = nphi;
+    c->sc_ng = ngrid_r;
+    c->sc_rmax = rmax;
+    G4_HIP(hipMemcpyAsync(c->d_rmax, &c->sc_rmax, sizeof(double), hipMemcpyHostToDevice, c->stream));
+    G4_HIP(hipStreamSynchronize(c->stream));
+    if (!c->longrange) {
+        G4_HIP(hipMemsetAsync(c->d_esc, 0, c->ns * sizeof(double), c->stream));
+        G4_HIP(hipStreamSynchronize(c->stream));
+    }
+    return G4MI_OK;
+}
+
+int g4mi_beam_upload(g4mi_ctx *h, const double *g, const double *th, const double *x,
+                     const double *y, const double *px, const double *py, const double *cur) {
+    if (!h || !g || !th || !x || !y || !px || !py || !cur) {
+        set_error("g4mi_beam_upload: null pointer");
+        return G4MI_E_ARG;
+    }
+    Ctx *c = &h->c;
+    const size_t nb = (size_t)c->ntot * sizeof(double);
+    const double *src[6] = {g, th, x, y, px, py};
+    double *dst[6] = {c->d_g, c->d_th, c->d_x, c->d_y, c->d_px, c->d_py};
+    for (int k = 0; k < 6; k++)
+        if (nb) G4_HIP(hipMemcpyAsync(dst[k], src[k], nb, hipMemcpyHostToDevice, c->stream));
+    G4_HIP(hipMemcpyAsync(c->d_cur, cur, c->ns * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    G4_HIP(hipStreamSynchronize(c->stream));
+    c->has_beam = true;
+    return G4MI_OK;
+}
+
+int g4mi_beam_download(g4mi_ctx *h, double *g, double *th, double *x, double *y, double *px,
+                       double *py) {
+    if (!h) return G4MI_E_ARG;
+    Ctx *c = &h->c;
+    const size_t nb = (size_t)c->ntot * sizeof(double);
+    double *dst[6] = {g, th, x, y, px, py};
+    const double *src[6] = {c->d_g, c->d_th, c->d_x, c->d_y, c->d_px, c->d_py};
+    for (int k = 0; k < 6; k++)
+        if (dst[k] && nb) G4_HIP(hipMemcpyAsync(dst[k], src[k], nb, hipMemcpyDeviceToHost, c->stream));
+    G4_HIP(hipStreamSynchronize(c->stream));
+    return G4MI_OK;
+}
+
+int g4mi_field_upload(g4mi_ctx *h, const double *field, int first) {
+    if (!h || !field) return G4MI_E_ARG;
+    Ctx *c = &h->c;
+    if (first < 0 || first >= c->ns) {
+        set_error("g4mi_field_upload: first out of range");
+        return G4MI_E_ARG;
+    }
+    G4_HIP(hipMemcpyAsync(c->d_field, field, (size_t)c->ns * c->ng * c->ng * sizeof(double2),
+                          hipMemcpyHostToDevice, c->stream));
+    G4_HIP(hipStreamSynchronize(c->stream));
+    c->first = first;
+    c->has_field = true;
+    return G4MI_OK;
+}
+
+int g4mi_field_download(g4mi_ctx *h, double *field, int *first) {
+    if (!h) return G4MI_E_ARG;
+    Ctx *c = &h->c;
+    if (field)
+        G4_HIP(hipMemcpyAsync(field, c->d_field, (size_t)c->ns * c->ng * c->ng * sizeof(double2),
+                              hipMemcpyDeviceToHost, c->stream));
+    G4_HIP(hipStreamSynchronize(c->stream));
+    if (first) *first = c->first;
+    return G4MI_OK;
+}
+
+int g4mi_beam_track(g4mi_ctx *h, const g4mi_step_params *st, double gammaref, double delz) {
+    if (!h || !st) return G4MI_E_ARG;
+    return beam_track(&h->c, *st, gammaref, delz);
+}
+
+int g4mi_field_track(g4mi_ctx *h, const g4mi_step_params *st, double delz) {
+    if (!h || !st) return G4MI_E_ARG;
+    return field_track(&h->c, *st, delz);
+}
+
+int g4mi_slippage(g4mi_ctx *h, double slip) {
+    if (!h) return G4MI_E_ARG;
+    return slippage(&h->c, slip);
+}
+
+int g4mi_step(g4mi_ctx *h, const g4mi_step_params *st, double gammaref) {
+    if (!h || !st) return G4MI_E_ARG;
+    G4_TRY(step(&h->c, *st, gammaref));
+    return step_end_timing(&h->c);
+}
+
+int g4mi_run(g4mi_ctx *h, const g4mi_step_params *steps, int nsteps, double gammaref,
+             g4mi_diag_out *records) {
+    if (!h || (!steps && nsteps > 0) || nsteps < 0) return G4MI_E_ARG;
+    int done = 0;
+    for (int i = 0; i < nsteps; i++) {
+        if (steps[i].marker & 8) break;
+        G4_TRY(step(&h->c, steps[i], gammaref));
+        if (records) G4_TRY(diag(&h->c, &records[i]));
+        G4_TRY(step_end_timing(&h->c));
+        done++;
+    }
+    G4_HIP(hipStreamSynchronize(h->c.stream));
+    return done;
+}
+
+int g4mi_diag(g4mi_ctx *h, g4mi_diag_out *out) {
+    if (!h || !out) return G4MI_E_ARG;
+    return diag(&h->c, out);
+}
+
+int g4mi_sort_index(g4mi_ctx *h, double slen, int *atar, long long *moved) {
+    if (!h || slen == 0) return G4MI_E_ARG;
+    Ctx *c = &h->c;
+    if (!c->d_atar) G4_TRY(dalloc(c, &c->d_atar, (size_t)c->ntot));
+    G4_HIP(hipMemsetAsync(c->d_count, 0, sizeof(unsigned long long), c->stream));
+    launch_sort_index(c->stream, c->ntot, c->d_th, 1. / slen, c->d_atar, c->d_count);
+    unsigned long long m = 0;
+    if (atar && c->ntot)
+        G4_HIP(hipMemcpyAsync(atar, c->d_atar, c->ntot * sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    G4_HIP(hipMemcpyAsync(&m, c->d_count, sizeof(m), hipMemcpyDeviceToHost, c->stream));
+    G4_HIP(hipStreamSynchronize(c->stream));
+    if (moved) *moved = (long long)m;
+    return G4MI_OK;
+}
+
+int g4mi_sync(g4mi_ctx *h) {
+    if (!h) return G4MI_E_ARG;
+    G4_HIP(hipStreamSynchronize(h->c.stream));
+    return G4MI_OK;
+}
+
+int g4mi_set_timing(g4mi_ctx *h, int on) {
+    if (!h) return G4MI_E_ARG;
+    Ctx *c = &h->c;
+    G4_TRY(fold_timers(c));
+    if (on && c->pool.empty()) {
+        c->pool.resize((size_t)Ctx::kPoolSteps * Ctx::kEv);
+        for (auto &e : c->pool) G4_HIP(hipEventCreate(&e));
+    }
+    c->timing = on != 0;
+    for (double &t : c->t_phase) t = 0;
+    c->n_timed = 0;
+    return G4MI_OK;
+}
+
+int g4mi_kernel_times(g4mi_ctx *h, double *ms, int nslots, long long *nsteps) {
+    if (!h || nslots < 0 || nslots > G4MI_NTIMERS || (nslots > 0 && !ms)) return G4MI_E_ARG;
+    Ctx *c = &h->c;
+    G4_TRY(fold_timers(c));
+    const double n = c->n_timed > 0 ? (double)c->n_timed : 1.0;
+    for (int k = 0; k < nslots; k++) ms[k] = c->t_phase[k] / n;
+    if (nsteps) *nsteps = c->n_timed;
+    return G4MI_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,59 @@
+// Shared host/device definitions of the g4mi library.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <string>
+#include <vector>
+
+#include "g4mi.h"
+
+namespace g4mi {
+
+// Physical constants of the reference, src/Main/GenMain.cpp:63-68.
+constexpr double kVacimp = 376.73;
+constexpr double kEev = 510998.95069;
+constexpr double kCe = 4.8032045e-11;
+constexpr double kClight = 299792458.0;
+
+void set_error(const std::string &msg);
+
+#define G4_HIP(call)                                                              \
+    do {                                                                          \
+        hipError_t e_ = (call);                                                   \
+        if (e_ != hipSuccess) {                                                   \
+            g4mi::set_error(std::string(#call) + ": " + hipGetErrorString(e_));   \
+            return G4MI_E_HIP;                                                    \
+        }                                                                         \
+    } while (0)
+
+// Parameters of the transverse half-step (TrackBeam::track,
+// src/Core/TrackBeam.cpp:8-81), prepared on the host once per call.
+struct TransParams {
+    double aw, delz, qx, qy, xoff, yoff;
+    double ccx, ccy;               // corrector kicks (already times gamma0)
+    double m00, m01, m10, m11, m22, m23, m32, m33;  // chicane matrix
+    int kindx, kindy;              // 0 drift, +1 focusing, -1 defocusing
+    int corrector, chicane;
+};
+
+// Parameters of BeamSolver::advance for the fundamental.
+struct PushParams {
+    double delz, xks, xku, rtmp, aw, autophase;
+    double ax, ay, kx, ky, gradx, grady;   // faw(x,y)
+    double gridmax, dgrid;
+    int ngrid, first, nslice, use_sc;
+};
+
+// Parameters of the source deposition (FieldSolverADI::advance :13-51).
+struct DepositParams {
+    double fcoup, xks, delz, dgrid, gridmax;
+    double ax, ay, kx, ky, gradx, grady;
+    int ngrid, enabled;
+};
+
+// Device state and host mirror of one context.
+struct Ctx;
+
+}  // namespace g4mi

@@ -1,0 +1,68 @@
+// Host-side numerics of the g4mi library (compiled with g++, no HIP).
+//
+// These are the per-step scalars the device kernels consume: undulator
+// coupling, ADI Thomas factors, chicane matrices, and the input generators
+// (lattice, beam, field).  They run once per step or once per run on the
+// host, in FP64 with the reference's operation order.
+#pragma once
+
+#include <string>
+#include <vector>
+
+#include "g4mi.h"
+
+namespace g4mi {
+
+// Charge of the electron in the reference's units (src/Main/GenMain.cpp:68).
+inline double kCeH() { return 4.8032045e-11; }
+
+// Undulator::fc (src/Core/Undulator.cpp:141-167) for harmonic h.
+double coupling(const g4mi_step_params &st, int h);
+// BesselJ::value (src/Util/BesselJ.cpp:101-159).
+double bessel_j(int n, double x);
+
+// FieldSolverADI::init (src/Core/FieldSolverADI.cpp:125-182): interleaved
+// complex arrays c, cbet, cwet of ngrid entries; returns Im(cstep).
+double adi_factors(int ngrid, double delz, double dgrid, double xks, double *c, double *cbet,
+                   double *cwet);
+
+// Chicane transfer matrix of TrackBeam::applyChicane (:131-221), row-major 4x4.
+void chicane_matrix(double angle, double lb, double ld, double lt, double m[16]);
+// TrackBeam::applyR56 factor (:260-269), already scaled to theta units.
+double chicane_r56(double angle, double lb, double ld, double lambda0, double gamma0);
+
+// ---- input generation (Loading/, Lattice/) ----
+struct SliceBeam {  // QuietLoading.h BeamSlice
+    double gamma, delgam, current, ex, ey, xcen, ycen, pxcen, pycen;
+    double betax, betay, alphax, alphay, bunch, bunchphase, emod, emodphase;
+};
+// Sequential ran2 generator (src/Util/RandomU.cpp).
+class Ran2 {
+  public:
+    explicit Ran2(unsigned int seed);
+    double next();
+
+  private:
+    int iv_[32], iy_, s1_, s2_;
+};
+// Quiet loading of one slice (src/Loading/QuietLoading.cpp:50-260), SoA.
+void quiet_load(double *gamma, double *theta, double *x, double *y, double *px, double *py,
+                const SliceBeam &sl, int npart, int nbins, double theta0, int islice);
+// Shot noise (src/Loading/ShotNoise.cpp:29-68) consuming `stream`.
+void shot_noise(Ran2 &stream, double *theta, int npart, int nbins, double ne);
+// Seed stream of ShotNoise::init (:16-27).
+unsigned int shot_noise_seed(int seed, int rank);
+double inverse_erfc(double y);
+// Gaussian seed field (src/Util/GaussHermite.cpp:20-63), fundamental mode.
+void gauss_field(double *field, int ngrid, double gridmax, double lambda, double power,
+                 double w0, double z0, double phase, double xcen, double ycen);
+
+// Lattice file (LatticeParser.cpp) -> unrolled steps (Lattice.cpp).
+struct LatticeResult {
+    std::vector<g4mi_step_params> steps;
+    std::string error;
+};
+LatticeResult parse_and_unroll(const std::string &text, const std::string &beamline, double delz,
+                               double lambda0, double gamma0);
+
+}  // namespace g4mi

@@ -1,0 +1,49 @@
+// Launchers of the FP64 device kernels (defined in g4mi_kernels.hip).
+#pragma once
+
+#include "g4mi_common.h"
+
+namespace g4mi {
+
+// Thomas factors of the constant-coefficient ADI systems (complex, ngrid
+// entries each, device pointers) and cstep = (0, cstep).
+struct AdiFactors {
+    const double2 *c, *cbet, *cwet;
+    double cstep;
+    int ngrid;
+};
+
+void launch_transverse(hipStream_t s, long long n, const double *g, double *x, double *y,
+                       double *px, double *py, const TransParams &p);
+void launch_r56(hipStream_t s, long long n, const double *g, double *th, double r56, double g0);
+void launch_slice_moments(hipStream_t s, int ns, const long long *off, const double *x,
+                          const double *y, double *mom);
+void launch_slice_sizes(hipStream_t s, int ns, const long long *off, const double *cur,
+                        const double *mom, double *fsize, double *cen);
+void launch_lsc_long(hipStream_t s, int ns, const double *cur, const double *fsize, double sl,
+                     double gamma, double scl, double *esc);
+void launch_sc_rbound(hipStream_t s, int ns, const long long *off, const double *x,
+                      const double *y, const double *cen, double *rbound);
+void launch_sc_scan(hipStream_t s, int ns, const long long *off, const double *rbound,
+                    double *rmax_state, int ngr, double *dr);
+void launch_sc_solve(hipStream_t s, int ns, const long long *off, const double *x,
+                     const double *y, const double *th, const double *cur, const double *cen,
+                     const double *dr, int ngr, int nz, int nphi, double ks, double gz2,
+                     double *ez, double *efield);
+void launch_push(hipStream_t s, int nblocks, const int *blk_slice, const long long *blk_begin,
+                 const long long *off, double *g, double *th, const double *x, const double *y,
+                 const double *px, const double *py, const double2 *field, const double *esc,
+                 const double *ezsc, const PushParams &p);
+void launch_deposit(hipStream_t s, int ns, const long long *off, const double *g,
+                    const double *th, const double *x, const double *y, const double *cur,
+                    double2 *src, int4 *box, const DepositParams &p);
+void launch_adi(hipStream_t s, int ns, double2 *field, double2 *work, const double2 *src,
+                const int4 *box, const AdiFactors &f, int first);
+void launch_zero(hipStream_t s, double2 *p, long long n);
+void launch_diag_beam(hipStream_t s, int ns, const long long *off, const double *g,
+                      const double *th, const double *x, const double *y, double *out);
+void launch_diag_field(hipStream_t s, int ns, const double2 *field, int ng, double *out);
+void launch_sort_index(hipStream_t s, long long n, const double *th, double invslen, int *atar,
+                       unsigned long long *moved);
+
+}  // namespace g4mi

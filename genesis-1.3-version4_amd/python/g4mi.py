@@ -1,0 +1,268 @@
+"""ctypes binding of libg4mi.so, the MI355X undulator-step library.
+
+This module only marshals arrays across the C-ABI declared in
+include/g4mi.h; every computation happens in the library (HIP kernels on
+the GPU, host input generation in C++).  Loading fails loudly if the
+library has not been built.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+PKG = os.path.dirname(HERE)
+LIB_PATH = os.path.join(PKG, "lib", "libg4mi.so")
+HEADER = os.path.join(os.path.dirname(PKG), "include", "g4mi.h")
+
+FP64, FP32 = 0, 1
+CE = 4.8032045e-11
+
+_dp = C.POINTER(C.c_double)
+_lp = C.POINTER(C.c_longlong)
+_ip = C.POINTER(C.c_int)
+
+STEP_FIELDS = ("aw", "ax", "ay", "ku", "kx", "ky", "gradx", "grady", "qf", "qx", "qy", "cx",
+               "cy", "chic_angle", "chic_lb", "chic_ld", "chic_lt", "slip", "phaseshift", "z",
+               "dz")
+
+
+class StepParams(C.Structure):
+    _fields_ = [(n, C.c_double) for n in STEP_FIELDS] + [("helical", C.c_int),
+                                                          ("marker", C.c_int)]
+
+
+SLICE_FIELDS = ("gamma", "delgam", "current", "ex", "ey", "xcen", "ycen", "pxcen", "pycen",
+                "betax", "betay", "alphax", "alphay", "bunch", "bunchphase", "emod", "emodphase")
+
+
+class SliceBeam(C.Structure):
+    _fields_ = [(n, C.c_double) for n in SLICE_FIELDS]
+
+
+DIAG_FIELDS = ("power", "bunching", "bunchingphase", "energy", "energyspread", "xposition",
+               "yposition", "xsize", "ysize", "f_xposition", "f_yposition", "f_xsize",
+               "f_ysize", "nf_intensity", "ff_intensity", "global_energy")
+
+
+class DiagOut(C.Structure):
+    _fields_ = [(n, _dp) for n in DIAG_FIELDS]
+
+
+class G4miError(RuntimeError):
+    pass
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise G4miError(f"{LIB_PATH} is not built; run __graft_entry__.build()")
+        L = C.CDLL(LIB_PATH)
+        L.g4mi_last_error.restype = C.c_char_p
+        L.g4mi_create.argtypes = [C.POINTER(C.c_void_p), C.c_int, C.c_int, _lp, C.c_int,
+                                  C.c_double, C.c_double, C.c_double, C.c_int, C.c_int, C.c_int]
+        L.g4mi_destroy.argtypes = [C.c_void_p]
+        L.g4mi_set_efield.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_double]
+        L.g4mi_beam_upload.argtypes = [C.c_void_p] + [_dp] * 7
+        L.g4mi_beam_download.argtypes = [C.c_void_p] + [_dp] * 6
+        L.g4mi_field_upload.argtypes = [C.c_void_p, _dp, C.c_int]
+        L.g4mi_field_download.argtypes = [C.c_void_p, _dp, _ip]
+        L.g4mi_beam_track.argtypes = [C.c_void_p, C.POINTER(StepParams), C.c_double, C.c_double]
+        L.g4mi_field_track.argtypes = [C.c_void_p, C.POINTER(StepParams), C.c_double]
+        L.g4mi_slippage.argtypes = [C.c_void_p, C.c_double]
+        L.g4mi_step.argtypes = [C.c_void_p, C.POINTER(StepParams), C.c_double]
+        L.g4mi_run.argtypes = [C.c_void_p, C.POINTER(StepParams), C.c_int, C.c_double,
+                               C.POINTER(DiagOut)]
+        L.g4mi_diag.argtypes = [C.c_void_p, C.POINTER(DiagOut)]
+        L.g4mi_sort_index.argtypes = [C.c_void_p, C.c_double, _ip, _lp]
+        L.g4mi_sync.argtypes = [C.c_void_p]
+        L.g4mi_set_timing.argtypes = [C.c_void_p, C.c_int]
+        L.g4mi_kernel_times.argtypes = [C.c_void_p, _dp, C.c_int, _lp]
+        L.g4mi_parse_lattice.argtypes = [C.c_char_p, C.c_char_p, C.c_double, C.c_double,
+                                         C.c_double, C.POINTER(StepParams), C.c_int, _ip]
+        L.g4mi_load_beam.argtypes = [C.c_int, _lp, C.POINTER(SliceBeam), C.c_int, C.c_int,
+                                     C.c_int, C.c_int, C.c_double, C.c_double, C.c_int] + [_dp] * 6
+        L.g4mi_load_gauss.argtypes = [_dp, C.c_int] + [C.c_double] * 6
+        L.g4mi_run_deck.argtypes = [C.c_char_p, C.c_int, C.c_int]
+        _lib = L
+    return _lib
+
+
+def _chk(rc, what):
+    if rc < 0:
+        raise G4miError(f"{what} failed ({rc}): {lib().g4mi_last_error().decode()}")
+    return rc
+
+
+def _p(a):
+    return a.ctypes.data_as(_dp)
+
+
+# ---------------------------------------------------------------- input side
+def parse_lattice(text, line, delz, lambda0, gamma0):
+    n = C.c_int(0)
+    _chk(lib().g4mi_parse_lattice(text.encode(), line.encode(), delz, lambda0, gamma0, None, 0,
+                                  C.byref(n)), "g4mi_parse_lattice")
+    out = (StepParams * max(1, n.value))()
+    _chk(lib().g4mi_parse_lattice(text.encode(), line.encode(), delz, lambda0, gamma0, out,
+                                  n.value, C.byref(n)), "g4mi_parse_lattice")
+    return (StepParams * n.value)(*out[:n.value])
+
+
+def load_beam(off, slices, nbins, shotnoise, seed, lambda0, sample=1.0, timerun=True,
+              islice0=0):
+    """Quiet-load every slice (and apply shot noise) as LoadBeam::init.
+    slices: list of dicts with SLICE_FIELDS keys.  Returns dict of SoA."""
+    off = np.ascontiguousarray(off, dtype=np.int64)
+    ns = len(off) - 1
+    sb = (SliceBeam * ns)(*[SliceBeam(**{k: float(s.get(k, 0.0)) for k in SLICE_FIELDS})
+                            for s in slices])
+    n = int(off[-1])
+    arrs = {k: np.zeros(n) for k in ("gamma", "theta", "x", "y", "px", "py")}
+    _chk(lib().g4mi_load_beam(ns, off.ctypes.data_as(_lp), sb, nbins, int(shotnoise), seed,
+                              islice0, lambda0, sample, int(timerun),
+                              *[_p(arrs[k]) for k in ("gamma", "theta", "x", "y", "px", "py")]),
+         "g4mi_load_beam")
+    return arrs
+
+
+def load_gauss(ngrid, gridmax, lambda0, power, waist, waist_pos=0.0, phase=0.0):
+    f = np.zeros(ngrid * ngrid * 2)
+    _chk(lib().g4mi_load_gauss(_p(f), ngrid, gridmax, lambda0, power, waist, waist_pos, phase),
+         "g4mi_load_gauss")
+    return f.reshape(ngrid * ngrid, 2)
+
+
+# ---------------------------------------------------------------- device side
+class Context:
+    """A g4mi device context: one time window of slices on one GPU."""
+
+    def __init__(self, off, ngrid, gridmax, lambda0, sample=1.0, timerun=True, periodic=False,
+                 precision=FP64, device=0):
+        self.off = np.ascontiguousarray(off, dtype=np.int64)
+        self.ns = len(self.off) - 1
+        self.ngrid = ngrid
+        self.n = int(self.off[-1])
+        h = C.c_void_p()
+        _chk(lib().g4mi_create(C.byref(h), device, self.ns, self.off.ctypes.data_as(_lp), ngrid,
+                               gridmax, lambda0, sample, int(timerun), int(periodic), precision),
+             "g4mi_create")
+        self.h = h
+
+    def close(self):
+        if self.h:
+            lib().g4mi_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_efield(self, longrange=False, nz=0, nphi=0, ngrid_r=100, rmax=0.0):
+        _chk(lib().g4mi_set_efield(self.h, int(longrange), nz, nphi, ngrid_r, rmax),
+             "g4mi_set_efield")
+
+    def upload_beam(self, beam, current):
+        cur = np.ascontiguousarray(current, dtype=np.float64)
+        a = [np.ascontiguousarray(beam[k], dtype=np.float64)
+             for k in ("gamma", "theta", "x", "y", "px", "py")]
+        _chk(lib().g4mi_beam_upload(self.h, *[_p(v) for v in a], _p(cur)), "g4mi_beam_upload")
+
+    def download_beam(self):
+        out = {k: np.zeros(self.n) for k in ("gamma", "theta", "x", "y", "px", "py")}
+        _chk(lib().g4mi_beam_download(self.h, *[_p(out[k]) for k in
+                                                ("gamma", "theta", "x", "y", "px", "py")]),
+             "g4mi_beam_download")
+        return out
+
+    def upload_field(self, field, first=0):
+        f = np.ascontiguousarray(field, dtype=np.float64)
+        assert f.size == self.ns * self.ngrid * self.ngrid * 2
+        _chk(lib().g4mi_field_upload(self.h, _p(f), first), "g4mi_field_upload")
+
+    def download_field(self):
+        f = np.zeros((self.ns, self.ngrid * self.ngrid, 2))
+        first = C.c_int(0)
+        _chk(lib().g4mi_field_download(self.h, _p(f), C.byref(first)), "g4mi_field_download")
+        return f, first.value
+
+    def beam_track(self, st, gamma0, delz):
+        _chk(lib().g4mi_beam_track(self.h, C.byref(st), gamma0, delz), "g4mi_beam_track")
+
+    def field_track(self, st, delz):
+        _chk(lib().g4mi_field_track(self.h, C.byref(st), delz), "g4mi_field_track")
+
+    def slippage(self, slip):
+        _chk(lib().g4mi_slippage(self.h, slip), "g4mi_slippage")
+
+    def step(self, st, gamma0):
+        _chk(lib().g4mi_step(self.h, C.byref(st), gamma0), "g4mi_step")
+
+    def sync(self):
+        _chk(lib().g4mi_sync(self.h), "g4mi_sync")
+
+    def diag(self):
+        out = {k: np.zeros(self.ns) for k in DIAG_FIELDS}
+        d = DiagOut(*[_p(out[k]) for k in DIAG_FIELDS])
+        _chk(lib().g4mi_diag(self.h, C.byref(d)), "g4mi_diag")
+        out["global_energy"] = out["global_energy"][0]
+        return out
+
+    def run(self, steps, gamma0, record=("power", "bunching", "energy")):
+        """Initial diagnostics, then every step with a record after each."""
+        nst = len(steps)
+        recs = {k: np.zeros((nst + 1, self.ns)) for k in record}
+        d0 = self.diag()
+        for k in record:
+            recs[k][0] = d0[k]
+        rows = (DiagOut * max(1, nst))()
+        keep = []
+        for i in range(nst):
+            ptrs = []
+            for k in DIAG_FIELDS:
+                if k in recs:
+                    ptrs.append(recs[k][i + 1].ctypes.data_as(_dp))
+                else:
+                    ptrs.append(None)
+            rows[i] = DiagOut(*ptrs)
+        arr = (StepParams * max(1, nst))(*steps) if nst else (StepParams * 1)()
+        done = _chk(lib().g4mi_run(self.h, arr, nst, gamma0, rows), "g4mi_run")
+        keep.append(rows)
+        return {k: v[:done + 1] for k, v in recs.items()}
+
+    def sort_index(self, slen):
+        atar = np.zeros(self.n, dtype=np.int32)
+        moved = C.c_longlong(0)
+        _chk(lib().g4mi_sort_index(self.h, slen, atar.ctypes.data_as(_ip), C.byref(moved)),
+             "g4mi_sort_index")
+        return atar, moved.value
+
+    TIMER_NAMES = ("transverse", "space_charge", "push", "deposit", "adi", "slip_diag")
+
+    def set_timing(self, on=True):
+        _chk(lib().g4mi_set_timing(self.h, int(on)), "g4mi_set_timing")
+
+    def kernel_times(self):
+        """Average device ms per step of each phase since set_timing(True)."""
+        ms = np.zeros(len(self.TIMER_NAMES))
+        n = C.c_longlong()
+        _chk(lib().g4mi_kernel_times(self.h, _p(ms), len(ms), C.byref(n)), "g4mi_kernel_times")
+        out = dict(zip(self.TIMER_NAMES, ms.tolist()))
+        out["steps"] = n.value
+        return out
+
+
+def steps_from(seq):
+    """Convert any sequence of ctypes step records with the same field
+    names (e.g. the oracle's) to StepParams."""
+    out = (StepParams * len(seq))()
+    for i, s in enumerate(seq):
+        for k in STEP_FIELDS + ("helical", "marker"):
+            setattr(out[i], k, getattr(s, k))
+    return out

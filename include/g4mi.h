@@ -1,0 +1,182 @@
+/*
+ * g4mi -- MI355X-native (gfx950) undulator integration step for GENESIS 1.3 v4.
+ *
+ * C-ABI of the drop-in boundary.  Plain pointers and sizes only; every entry
+ * point returns 0 on success or a negative G4MI_E* code and never throws.
+ * Device memory is owned by the context; host pointers are borrowed for the
+ * duration of the call.  One host thread per context; all work is ordered on
+ * the context's HIP stream; download/diag calls synchronise that stream.
+ *
+ * The reference has no C-ABI for this path.  Each entry point below names
+ * the reference member function (file:line) whose contract it replaces; a
+ * maintainer binds them from FieldSolver/BeamSolver subclasses (shown in
+ * INTEGRATION.md).
+ */
+#ifndef G4MI_H
+#define G4MI_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define G4MI_OK 0
+#define G4MI_E_ARG -1      /* invalid argument or size                      */
+#define G4MI_E_HIP -2      /* HIP runtime error                             */
+#define G4MI_E_NODEV -3    /* no usable GPU                                 */
+#define G4MI_E_STATE -4    /* call out of order (e.g. no beam uploaded)     */
+#define G4MI_E_IO -5       /* input or output file error                    */
+#define G4MI_E_PARSE -6    /* namelist or lattice syntax error              */
+
+/* Arithmetic of a context. */
+#define G4MI_FP64 0        /* plain FP64 form, follows the reference order  */
+#define G4MI_FP32 1        /* FP32 push/field, FP64 slice base for theta,
+                              FP32 ADI + one FP64 residual-correction sweep */
+
+typedef struct g4mi_ctx g4mi_ctx;
+
+/* One integration step of the unrolled lattice: the per-step arrays of the
+ * reference's Undulator class (include/Undulator.h:73-77), one record per
+ * step, as Lattice::generateLattice (src/Lattice/Lattice.cpp:53-149) fills
+ * them. */
+typedef struct {
+    double aw, ax, ay, ku, kx, ky, gradx, grady;
+    double qf, qx, qy, cx, cy;
+    double chic_angle, chic_lb, chic_ld, chic_lt;
+    double slip, phaseshift, z, dz;
+    int helical, marker;
+} g4mi_step_params;
+
+/* Per-slice diagnostics of one output record (any pointer may be NULL).
+ * Arrays have nslice entries in time-window order, as DiagBeam::getValues
+ * (src/Core/Diagnostic.cpp:373-608) and DiagField::getValues (:718-996)
+ * store them. */
+typedef struct {
+    double *power, *bunching, *bunchingphase, *energy, *energyspread;
+    double *xposition, *yposition, *xsize, *ysize;
+    double *f_xposition, *f_yposition, *f_xsize, *f_ysize;
+    double *nf_intensity, *ff_intensity;
+    double *global_energy; /* [1] */
+} g4mi_diag_out;
+
+/* Context on HIP device `device` holding a time window of `nslice` slices
+ * with CSR particle offsets `off` (host, nslice+1 entries; uneven counts
+ * allowed), a field of ngrid x ngrid cells with half width `gridmax`,
+ * reference wavelength `lambda0` and slice length sample*lambda0.
+ * Replaces Beam::init (src/Core/Beam.cpp:22-39) + Field::init
+ * (src/Core/Field.cpp:34-70) + Control::init (src/Core/Control.cpp:108-154). */
+int g4mi_create(g4mi_ctx **ctx, int device, int nslice, const long long *off, int ngrid,
+                double gridmax, double lambda0, double sample, int timerun, int periodic,
+                int precision);
+int g4mi_destroy(g4mi_ctx *ctx);
+
+/* Longitudinal space-charge settings, EFieldSolver::init
+ * (src/Core/EFieldSolver.cpp:27-49) via Beam::initEField. */
+int g4mi_set_efield(g4mi_ctx *ctx, int longrange, int nz, int nphi, int ngrid_r, double rmax);
+
+/* Particle SoA and per-slice current (A), host -> device / device -> host.
+ * Replaces the particle fill of LoadBeam::init (src/Loading/LoadBeam.cpp:202-211)
+ * and the reads of Beam::beam by dumps. */
+int g4mi_beam_upload(g4mi_ctx *ctx, const double *gamma, const double *theta, const double *x,
+                     const double *y, const double *px, const double *py, const double *current);
+int g4mi_beam_download(g4mi_ctx *ctx, double *gamma, double *theta, double *x, double *y,
+                       double *px, double *py);
+
+/* Field slices, interleaved complex [nslice][ngrid*ngrid][2] in storage
+ * order, plus the ring offset `first` (src/Core/Field.h:57-59). */
+int g4mi_field_upload(g4mi_ctx *ctx, const double *field, int first);
+int g4mi_field_download(g4mi_ctx *ctx, double *field, int *first);
+
+/* Beam::track (src/Core/Beam.cpp:97-112): half transverse step,
+ * BeamSolver::advance (src/Core/BeamSolver.cpp:13-87) with long- and
+ * short-range space charge, R56, half transverse step. */
+int g4mi_beam_track(g4mi_ctx *ctx, const g4mi_step_params *st, double gammaref, double delz);
+
+/* Field::track (src/Core/Field.cpp:177-185): FieldSolverADI::init
+ * (src/Core/FieldSolverADI.cpp:125-182) when delz changes, then
+ * FieldSolverADI::advance (:9-57): source deposition + ADI per slice. */
+int g4mi_field_track(g4mi_ctx *ctx, const g4mi_step_params *st, double delz);
+
+/* Control::applySlippage (src/Core/Control.cpp:158-276) on one device:
+ * ring-offset rotation and zeroing of the inflowing slice. */
+int g4mi_slippage(g4mi_ctx *ctx, double slip);
+
+/* One step of Gencore::run (src/Core/Gencore.cpp:118-147): beam track,
+ * field track, slippage. */
+int g4mi_step(g4mi_ctx *ctx, const g4mi_step_params *st, double gammaref);
+
+/* Several consecutive steps with one diagnostics record after each
+ * (records array of nsteps entries, may be NULL); stops at a step whose
+ * marker has bit 8 set, as Undulator::advance (src/Core/Undulator.cpp:112-139).
+ * Returns the number of steps done (>= 0) or an error code. */
+int g4mi_run(g4mi_ctx *ctx, const g4mi_step_params *steps, int nsteps, double gammaref,
+             g4mi_diag_out *records);
+
+/* Diagnostics of the current state (synchronises the stream). */
+int g4mi_diag(g4mi_ctx *ctx, g4mi_diag_out *out);
+
+/* Sorting::localSort target index (src/Util/Sorting.cpp:96) for every
+ * particle: atar = floor(theta * (1/slen)); returns the count of particles
+ * with atar != 0 in *moved. atar is a host array of total-particle length. */
+int g4mi_sort_index(g4mi_ctx *ctx, double slen, int *atar, long long *moved);
+
+/* Wait for all work queued on the context. */
+int g4mi_sync(g4mi_ctx *ctx);
+
+/* Per-kernel timing with HIP events recorded on the context's stream into
+ * a pool (no host synchronisation inside a step).  g4mi_set_timing(1)
+ * starts recording and clears the totals; g4mi_kernel_times synchronises,
+ * folds the recorded events and returns the average device milliseconds per
+ * step of each phase: ms[0] transverse half steps (both), ms[1] space charge
+ * (moments, long range, short range), ms[2] longitudinal push, ms[3] source
+ * deposition, ms[4] ADI field advance (x and y sweeps), ms[5] slippage and
+ * diagnostics.  nslots <= G4MI_NTIMERS. */
+#define G4MI_NTIMERS 6
+int g4mi_set_timing(g4mi_ctx *ctx, int on);
+int g4mi_kernel_times(g4mi_ctx *ctx, double *ms, int nslots, long long *nsteps);
+
+/* Text of the last error on this thread. */
+const char *g4mi_last_error(void);
+
+/* ---- input generation (host only, no GPU needed) ---------------------- */
+
+/* Beam parameters of one slice, the BeamSlice record of the quiet loader
+ * (include/QuietLoading.h:16-35 in the reference). */
+typedef struct {
+    double gamma, delgam, current, ex, ey, xcen, ycen, pxcen, pycen;
+    double betax, betay, alphax, alphay, bunch, bunchphase, emod, emodphase;
+} g4mi_slice_beam;
+
+/* Parse a lattice file's text (LatticeParser::parse,
+ * src/Lattice/LatticeParser.cpp:51-240) and unroll beamline `line` into
+ * integration steps (Lattice::generateLattice, src/Lattice/Lattice.cpp:53-197).
+ * Writes at most `max_steps` records; *nsteps receives the full count. */
+int g4mi_parse_lattice(const char *text, const char *line, double delz, double lambda0,
+                       double gamma0, g4mi_step_params *out, int max_steps, int *nsteps);
+
+/* Generate the macroparticles of slices [0, nslice) as LoadBeam::init
+ * (src/Loading/LoadBeam.cpp:127-212) does for the non-one4one case: quiet
+ * Hammersley loading of each slice (global slice index islice0 + j), then,
+ * for time-dependent runs with shot noise, one sequential ran2 stream of
+ * ShotNoise (src/Loading/ShotNoise.cpp).  Output SoA arrays of off[nslice]
+ * particles; per-slice counts must be multiples of nbins. */
+int g4mi_load_beam(int nslice, const long long *off, const g4mi_slice_beam *slices, int nbins,
+                   int shotnoise, int seed, int islice0, double lambda0, double sample,
+                   int timerun, double *gamma, double *theta, double *x, double *y, double *px,
+                   double *py);
+
+/* Gaussian seed field of one slice (GaussHermite::loadGauss,
+ * src/Util/GaussHermite.cpp:20-63, fundamental TEM00), interleaved complex. */
+int g4mi_load_gauss(double *field, int ngrid, double gridmax, double lambda, double power,
+                    double waist, double waist_pos, double phase);
+
+/* Whole input deck: parse the namelist file (src/Main/GenMain.cpp:80-494
+ * subset: &setup &time &field &beam &efield &track) and lattice, generate
+ * beam and field, run every &track on the GPU and write <rootname>.out.h5
+ * in the reference layout (src/IO/Output.cpp) when HDF5 is available.
+ * Replaces genmain() for the supported commands. */
+int g4mi_run_deck(const char *input_file, int device, int quiet);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,209 @@
+"""GPU parity: the HIP FP64 plain form against the FP64 CPU oracle on the
+same inputs, through the C-ABI.  Tolerances: index work (slippage ring
+offset, sort index) bit-exact; FP64 floating point within 1e-10 relative
+(P(z), |b(z)|, final field), with a per-kernel check tighter than that.
+The only sources of difference are the device's sin/cos/log/exp (an ulp
+from glibc's) and reduction order in sums."""
+import ctypes as C
+import sys
+
+import numpy as np
+import pytest
+
+import cases
+import g4mi
+import harness as H
+import oracle_py as O
+
+pytestmark = pytest.mark.gpu
+
+BEAM = ("gamma", "theta", "x", "y", "px", "py")
+
+
+def _log(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def _beam_close(ctx, sim, rtol):
+    b = ctx.download_beam()
+    worst = {}
+    for k in BEAM:
+        ref = getattr(sim, k)
+        scale = np.maximum(np.abs(ref), 1e-3 * np.max(np.abs(ref)) + 1e-300)
+        worst[k] = float(np.max(np.abs(b[k] - ref) / scale))
+    assert max(worst.values()) <= rtol, worst
+    return worst
+
+
+@pytest.fixture(scope="module")
+def st():
+    return H.steps()
+
+
+def test_transverse_parity(st):
+    """Beam track at zero field and current: the transverse half steps
+    (drift, quadrupole, undulator focusing) and the free push."""
+    _log("transverse parity: 8 slices x 512, undulator/drift/quad/drift steps")
+    sim = H.make_state([512] * 8)
+    sim.current[:] = 0.0
+    ctx = H.to_gpu(sim)
+    gst = H.gsteps(st)
+    for idx in (0, 125, 126, 127):
+        O.lib().orc_beam_track(C.byref(sim.s), C.byref(st[idx]), cases.GAMMA0, st[idx].dz)
+        ctx.beam_track(gst[idx], cases.GAMMA0, st[idx].dz)
+    w = _beam_close(ctx, sim, 1e-14)
+    _log(f"  worst rel diff {w}")
+    ctx.close()
+
+
+def test_push_parity_with_field(st):
+    _log("push parity: 8 slices x 1024 with a 50 MW seed field, 5 undulator steps")
+    sim = H.make_state([1024] * 8, seed_power=5e7)
+    ctx = H.to_gpu(sim)
+    gst = H.gsteps(st)
+    for i in range(5):
+        O.lib().orc_beam_track(C.byref(sim.s), C.byref(st[i]), cases.GAMMA0, st[i].dz)
+        ctx.beam_track(gst[i], cases.GAMMA0, st[i].dz)
+    w = _beam_close(ctx, sim, 1e-13)
+    _log(f"  worst rel diff {w}")
+    ctx.close()
+
+
+def test_field_track_parity(st):
+    _log("field parity: deposit + ADI, 8 slices x 2048, 151^2, 3 steps")
+    sim = H.make_state([2048] * 8, seed_power=5e3)
+    ctx = H.to_gpu(sim)
+    gst = H.gsteps(st)
+    for i in range(3):
+        O.lib().orc_field_advance(C.byref(sim.s), C.byref(st[i]), st[i].dz)
+        ctx.field_track(gst[i], st[i].dz)
+    f, first = ctx.download_field()
+    assert first == sim.s.first
+    err = np.max(np.abs(f - sim.field)) / np.max(np.abs(sim.field))
+    _log(f"  field max diff / max |E| = {err:.3e}")
+    assert err < 1e-13
+
+
+def test_field_track_in_drift_has_no_source(st):
+    _log("field parity in a drift step (no source, pure diffraction)")
+    sim = H.make_state([256] * 4, seed_power=5e3)
+    ctx = H.to_gpu(sim)
+    gst = H.gsteps(st)
+    O.lib().orc_field_advance(C.byref(sim.s), C.byref(st[125]), st[125].dz)
+    ctx.field_track(gst[125], st[125].dz)
+    f, _ = ctx.download_field()
+    assert np.max(np.abs(f - sim.field)) <= 1e-14 * np.max(np.abs(sim.field))
+
+
+def test_space_charge_parity(st):
+    _log("space charge parity: long range + short range (nz=1, nphi=1, 100 radial)")
+    sim = H.make_state([1024] * 16, sc=(1, 1, 100, 0.0), longrange=True,
+                       currents=cases.gaussian_current(16, 3000.0))
+    ctx = H.to_gpu(sim)
+    gst = H.gsteps(st)
+    for i in range(3):
+        O.lib().orc_beam_track(C.byref(sim.s), C.byref(st[i]), cases.GAMMA0, st[i].dz)
+        ctx.beam_track(gst[i], cases.GAMMA0, st[i].dz)
+    w = _beam_close(ctx, sim, 1e-12)
+    _log(f"  worst rel diff {w}")
+
+
+def test_slippage_and_sort_index_bit_exact(st):
+    _log("slippage ring offset and sort index, bit-exact")
+    sim = H.make_state([64] * 7)
+    ctx = H.to_gpu(sim)
+    for s in (1.0000001, 0.3, 0.6, 9.0, 0.0, 2.5, -1.2):
+        O.lib().orc_slippage(C.byref(sim.s), s)
+        ctx.slippage(s)
+        f, first = ctx.download_field()
+        assert first == sim.s.first
+        assert np.array_equal(f, sim.field)
+    slen = 2 * np.pi
+    ctx2 = H.to_gpu(H.make_state([300, 500]))
+    b = ctx2.download_beam()
+    th = b["theta"]
+    ref = np.zeros(len(th), np.int32)
+    moved = O.lib().orc_sort_index(th.ctypes.data_as(C.POINTER(C.c_double)), len(th), slen,
+                                   ref.ctypes.data_as(C.POINTER(C.c_int)))
+    atar, m = ctx2.sort_index(slen)
+    assert np.array_equal(atar, ref) and m == moved
+
+
+def test_diag_parity():
+    _log("diagnostics parity on a loaded state")
+    sim = H.make_state([1024] * 8, seed_power=5e3)
+    sim.s.first = 3
+    ctx = H.to_gpu(sim)
+    d = ctx.diag()
+    r = sim.diag()
+    for k in ("power", "bunching", "energy", "xsize", "ysize", "f_xsize", "nf_intensity",
+              "ff_intensity"):
+        e = H.rel_err(d[k], r[k], 1e-12)
+        assert e < 1e-12, (k, e)
+    # energyspread = sqrt(|<g^2> - <g>^2|) (Diagnostic.cpp:478) cancels
+    # kappa = <g>^2 / spread^2 ~ 1e8 of the sums' relative rounding; the
+    # reference's sequential sum of n terms carries ~sqrt(n) eps of it, so
+    # a different (tree) order moves the result by up to sqrt(n) kappa eps.
+    kappa = np.max(r["energy"] ** 2 / r["energyspread"] ** 2)
+    e = H.rel_err(d["energyspread"], r["energyspread"])
+    assert e < np.sqrt(1024) * kappa * 2.2e-16, (e, kappa)
+    assert abs(d["global_energy"] / r["global_energy"] - 1) < 1e-13
+
+
+def test_steady_state_config0_end_to_end():
+    """Config 0: steady state, one slice of 8192 (nbins 4), 151^2, 5 kW
+    seed, four 125-period modules with FODO drifts (509 steps)."""
+    _log("config 0 end to end (509 steps) GPU vs oracle")
+    st = H.steps()
+    sim = H.make_state([8192], seed_power=5e3, timerun=False)
+    ctx = H.to_gpu(sim)
+    g = ctx.run(H.gsteps(st), cases.GAMMA0, record=("power", "bunching"))
+    r = sim.run(st, cases.GAMMA0, record=("power", "bunching"))
+    ep = H.rel_err(g["power"], r["power"])
+    # |b| of a quiet nbins=4 beam starts at ~1e-17: b = <exp(i theta)> is a
+    # cancellation with condition number 1/|b|, so an ulp-level difference
+    # in theta (device sin/cos) moves it by ~eps |theta| / sqrt(N) in
+    # absolute terms.  Tolerance: 1e-10 relative plus 1e-14 absolute.
+    db = np.abs(g["bunching"] - r["bunching"])
+    eb = float(np.max(db / (np.abs(r["bunching"]) + 1e-4)))
+    _log(f"  P(z) rel err {ep:.3e}  |b| abs err {db.max():.3e} at |b|="
+         f"{r['bunching'].ravel()[np.argmax(db)]:.3e}, max |b| {r['bunching'].max():.3e}, "
+         f"P_end {g['power'][-1, 0]:.6e} vs {r['power'][-1, 0]:.6e}")
+    assert ep < 1e-10
+    assert np.all(db <= 1e-10 * np.abs(r["bunching"]) + 1e-14), eb
+    f, _ = ctx.download_field()
+    assert np.max(np.abs(f - sim.field)) < 1e-10 * np.max(np.abs(sim.field))
+
+
+def test_sase_time_dependent_small_end_to_end():
+    """Config 1 physics on a reduced window: 24 slices x 1024, shot noise
+    seed 7, long-range space charge, slippage, 509 steps on 63^2."""
+    _log("time-dependent SASE, 24 slices, 509 steps, GPU vs oracle")
+    st = H.steps()
+    sim = H.make_state([1024] * 24, ngrid=63, gridmax=0.3e-3, longrange=True)
+    ctx = H.to_gpu(sim)
+    g = ctx.run(H.gsteps(st), cases.GAMMA0, record=("power", "bunching", "energy"))
+    r = sim.run(st, cases.GAMMA0, record=("power", "bunching", "energy"))
+    ep = H.rel_err(g["power"][1:], r["power"][1:])
+    eb = H.rel_err(g["bunching"], r["bunching"])
+    _log(f"  P(z) rel err {ep:.3e}  |b(z)| rel err {eb:.3e}  sum P_end {g['power'][-1].sum():.4e}")
+    assert ep < 1e-10 and eb < 1e-10
+    f, first = ctx.download_field()
+    assert first == sim.s.first
+    assert np.max(np.abs(f - sim.field)) < 1e-10 * np.max(np.abs(sim.field))
+
+
+def test_config1_full_size_first_steps():
+    """Config 1 at full size (1024 slices x 8192, 151^2, shot noise, LSC):
+    the first three steps against the oracle (threaded CPU run)."""
+    _log("config 1 full size, 3 steps GPU vs oracle (oracle multithreaded)")
+    st = H.steps()
+    sim = H.make_state([8192] * 1024, longrange=True, threads=16)
+    ctx = H.to_gpu(sim)
+    g = ctx.run(H.gsteps(st[:3]), cases.GAMMA0, record=("power", "bunching"))
+    r = sim.run(st, cases.GAMMA0, record=("power", "bunching"), nsteps=3)
+    ep = H.rel_err(g["power"][1:], r["power"][1:])
+    eb = H.rel_err(g["bunching"], r["bunching"])
+    _log(f"  P rel err {ep:.3e}  |b| rel err {eb:.3e}")
+    assert ep < 1e-10 and eb < 1e-10
