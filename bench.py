@@ -55,9 +55,11 @@ def make_context(cfg, off, beam, field, current):
     return ctx
 
 
-def gpu_measure(cfg, steps_k, warmup):
+def gpu_measure(cfg, steps_k, warmup, adi_kernel=-1):
     off, beam, field, current, steps = build_workload(cfg)
     ctx = make_context(cfg, off, beam, field, current)
+    if adi_kernel >= 0:
+        ctx.set_schedule("adi_kernel", adi_kernel)
     nst = len(steps)
     if warmup + steps_k > nst:
         raise SystemExit(f"warmup+steps must be <= {nst} (lattice length)")
@@ -82,12 +84,13 @@ def gpu_measure(cfg, steps_k, warmup):
     t1 = time.perf_counter()
     kt = ctx.kernel_times()
     ctx.set_timing(False)
+    sched = ctx.schedule()
     assert done == steps_k
     dz = sum(s.dz for s in timed)
     ntot = int(off[-1])
     ctx.close()
     return dict(seconds=t1 - t0, ntot=ntot, dz=dz, kt=kt, power=bufs["power"],
-                bunching=bufs["bunching"])
+                bunching=bufs["bunching"], schedule=sched)
 
 
 def roofline(cfg, kt, ntot):
@@ -141,6 +144,9 @@ def main():
     ap.add_argument("--config", choices=("config1", "config0"), default="config1")
     ap.add_argument("--cpu-steps", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--adi-kernel", type=int, default=-1,
+                    help="fix the ADI kernel (0 Thomas, 1 fused, 2 fused+prefetch, 3 banded); "
+                         "default: autotuned")
     a = ap.parse_args()
     if a.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; the slice-block sharding "
@@ -148,7 +154,7 @@ def main():
                           "n_gpus": a.gpus}))
         return 2
     cfg = W.CONFIG1 if a.config == "config1" else W.CONFIG0
-    m = gpu_measure(cfg, a.steps, a.warmup)
+    m = gpu_measure(cfg, a.steps, a.warmup, a.adi_kernel)
     rate = m["ntot"] * a.steps / m["seconds"]
     cells = cfg["nslice"] * cfg["ngrid"] ** 2
     kname, roof = roofline(cfg, m["kt"], m["ntot"])
@@ -172,6 +178,7 @@ def main():
         "s_per_undulator_metre": round(m["seconds"] / m["dz"], 5),
         "kernel_ms_per_step": {k: round(v, 5) for k, v in m["kt"].items() if k != "steps"},
         "roofline": roof,
+        "schedule": m["schedule"],
     }
     if not a.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(cfg, a.cpu_steps)

@@ -6,6 +6,7 @@
 // kernels in g4mi_kernels.hip on the context's stream; only per-step scalars
 // are prepared on the host.
 #include <algorithm>
+#include <cmath>
 #include <cstdio>
 #include <cstring>
 #include <string>
@@ -51,8 +52,22 @@ struct Ctx {
     // space charge
     int longrange = 0, sc_nz = 0, sc_nphi = 0, sc_ng = 100;
     double sc_rmax = 0;
-    // ADI factor cache
+    // ADI factor cache and the kernels eligible for it
     double adi_delz = 0, cstep = 0;
+    int adi_tile = 0;  // 8 or 16: window of the fused recurrence kernel; 0: not eligible
+    int adi_band_m = 0;  // 7, 8 or 12: half width of the banded-inverse kernel; 0: not eligible
+    int adi_klo = 0, adi_khi = -1;  // rows sharing the central band row
+    double2 *d_band = nullptr;
+    double *d_tilediag = nullptr;
+    // field diagnostics sums written by the fused ADI of the last field
+    // step, and the storage slices slippage has zeroed since then
+    bool field_diag_fresh = false;
+    std::vector<int> zeroed_since_adi;
+    // ADI kernel chosen by the autotuner (-1: not tuned yet) and its timings:
+    // 0 two-kernel Thomas, 1 fused recurrence, 2 fused recurrence with
+    // prefetch, 3 banded inverse
+    int adi_kernel = -1;
+    float adi_ms[4] = {-1.f, -1.f, -1.f, -1.f};
     // timing: kEv events per step, recorded into a pool and folded lazily
     static constexpr int kEv = 9;
     static constexpr int kPoolSteps = 512;
@@ -151,6 +166,58 @@ static int ensure_factors(Ctx *c, double delz) {
     std::vector<double> h(6 * (size_t)c->ng);
     c->cstep = adi_factors(c->ng, delz, c->dgrid, c->xks, h.data(), h.data() + 2 * c->ng,
                            h.data() + 4 * c->ng);
+    // Window of the fused kernel: a back-substitution started T cells early
+    // is off by at most max|cwet|^(T+1) relative; accept it below 2^-66.
+    double wmax = 0;
+    for (int i = 0; i < c->ng; i++)
+        wmax = std::max(wmax, std::hypot(h[4 * c->ng + 2 * i], h[4 * c->ng + 2 * i + 1]));
+    const double bound = std::ldexp(1.0, -66);
+    c->adi_tile = 0;
+    if (c->ng <= 256) {
+        if (std::pow(wmax, 9) < bound) c->adi_tile = 8;
+        else if (std::pow(wmax, 17) < bound) c->adi_tile = 16;
+    }
+    // banded inverse: accept the narrowest band whose dropped entries are
+    // all below 2^-60 of the diagonal (1/128 ulp of the result)
+    c->adi_band_m = 0;
+    for (int m : {7, 8, 12}) {
+        std::vector<double> band(2 * (size_t)c->ng * (2 * m + 1));
+        const double worst = adi_band(c->ng, h.data(), h.data() + 2 * c->ng,
+                                      h.data() + 4 * c->ng, m, band.data());
+        if (worst < std::ldexp(1.0, -60)) {
+            // interior rows whose band equals the central row's to 2^-60
+            const int W = 2 * m + 1, kc = c->ng / 2;
+            auto same = [&](int k) {
+                double d = 0, dia = std::hypot(band[2 * ((size_t)kc * W + m)], band[2 * ((size_t)kc * W + m) + 1]);
+                for (int q = 0; q < W; q++)
+                    d = std::max(d, std::hypot(band[2 * ((size_t)k * W + q)] - band[2 * ((size_t)kc * W + q)],
+                                               band[2 * ((size_t)k * W + q) + 1] - band[2 * ((size_t)kc * W + q) + 1]));
+                return d <= std::ldexp(1.0, -60) * dia;
+            };
+            int lo = kc, hi = kc;
+            while (lo > 0 && same(lo - 1)) lo--;
+            while (hi < c->ng - 1 && same(hi + 1)) hi++;
+            // the kernel takes the band of row (klo+khi)/2: keep the range centred
+            const int half = std::min(kc - lo, hi - kc);
+            c->adi_klo = kc - half;
+            c->adi_khi = kc + half;
+            if (c->d_band) (void)hipFree(c->d_band);
+            c->d_band = nullptr;
+            G4_HIP(hipMalloc(&c->d_band, band.size() * sizeof(double)));
+            G4_HIP(hipMemcpy(c->d_band, band.data(), band.size() * sizeof(double),
+                             hipMemcpyHostToDevice));
+            if (!c->d_tilediag)
+                G4_HIP(hipMalloc(&c->d_tilediag,
+                                 (size_t)c->ns * adi_band_tiles(c->ng) * 8 * sizeof(double)));
+            c->adi_band_m = m;
+            break;
+        }
+    }
+    if (c->adi_kernel == 1 || c->adi_kernel == 2) {
+        if (!c->adi_tile) c->adi_kernel = -1;      // re-tune if no longer eligible
+    } else if (c->adi_kernel == 3 && !c->adi_band_m) {
+        c->adi_kernel = -1;
+    }
     G4_HIP(hipMemcpyAsync(c->d_fac, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice,
                           c->stream));
     G4_HIP(hipStreamSynchronize(c->stream));
@@ -223,6 +290,63 @@ static int beam_track(Ctx *c, const g4mi_step_params &st, double gamma0, double 
     return G4MI_OK;
 }
 
+// One ADI launch of kernel `kind` on field buffer `fin` (the banded kernel
+// writes `fout`, the others work in place using `fout` as scratch).
+static void adi_launch(Ctx *c, int kind, const AdiFactors &f, double2 *fin, double2 *fout) {
+    if (kind == 3)
+        launch_adi_band(c->stream, c->ns, fin, fout, c->d_src, c->d_box, c->d_band, c->cstep,
+                        c->ng, c->first, c->adi_band_m, c->adi_klo, c->adi_khi, c->d_tilediag,
+                        c->d_diag_f);
+    else if (kind == 1 || kind == 2)
+        launch_adi_fused(c->stream, c->ns, fin, fout, c->d_src, c->d_box, f, c->first,
+                         c->d_diag_f, c->adi_tile, kind == 2);
+    else
+        launch_adi(c->stream, c->ns, fin, fout, c->d_src, c->d_box, f, c->first);
+}
+
+// Autotuner of the ADI: run each eligible kernel on a scratch copy of the
+// current field (same sources and boxes) and keep the fastest.  Runs
+// before the first field step of a context and after a step-size change
+// that alters eligibility.
+static int tune_adi(Ctx *c, const AdiFactors &f) {
+    const size_t bytes = (size_t)c->ns * c->ng * c->ng * sizeof(double2);
+    double2 *scratch = nullptr;
+    G4_HIP(hipMalloc(&scratch, bytes));
+    hipEvent_t e0, e1;
+    G4_HIP(hipEventCreate(&e0));
+    G4_HIP(hipEventCreate(&e1));
+    G4_HIP(hipMemcpyAsync(scratch, c->d_field, bytes, hipMemcpyDeviceToDevice, c->stream));
+    int best_kind = 0;
+    float best_ms = 1e30f;
+    for (int kind = 0; kind < 4; kind++) {
+        c->adi_ms[kind] = -1.f;
+        if ((kind == 1 || kind == 2) && !c->adi_tile) continue;
+        if (kind == 3 && !c->adi_band_m) continue;
+        adi_launch(c, kind, f, scratch, c->d_work);  // warm-up
+        float best = 1e30f;
+        for (int rep = 0; rep < 3; rep++) {
+            G4_HIP(hipEventRecord(e0, c->stream));
+            adi_launch(c, kind, f, scratch, c->d_work);
+            G4_HIP(hipEventRecord(e1, c->stream));
+            G4_HIP(hipEventSynchronize(e1));
+            float ms = 0;
+            G4_HIP(hipEventElapsedTime(&ms, e0, e1));
+            best = std::min(best, ms);
+        }
+        c->adi_ms[kind] = best;
+        if (best < best_ms) {
+            best_ms = best;
+            best_kind = kind;
+        }
+    }
+    G4_HIP(hipGetLastError());
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    G4_HIP(hipFree(scratch));
+    c->adi_kernel = best_kind;
+    return G4MI_OK;
+}
+
 static int field_track(Ctx *c, const g4mi_step_params &st, double delz) {
     if (!c->has_field) {
         set_error("field not uploaded");
@@ -248,7 +372,11 @@ static int field_track(Ctx *c, const g4mi_step_params &st, double delz) {
                    c->d_src, c->d_box, d);
     if (c->ev) G4_HIP(hipEventRecord(c->ev[6], c->stream));
     AdiFactors f{c->d_fac, c->d_fac + c->ng, c->d_fac + 2 * c->ng, c->cstep, c->ng};
-    launch_adi(c->stream, c->ns, c->d_field, c->d_work, c->d_src, c->d_box, f, c->first);
+    if (c->adi_kernel < 0) G4_TRY(tune_adi(c, f));
+    adi_launch(c, c->adi_kernel, f, c->d_field, c->d_work);
+    if (c->adi_kernel == 3) std::swap(c->d_field, c->d_work);  // result is in the other buffer
+    c->field_diag_fresh = c->adi_kernel != 0;
+    c->zeroed_since_adi.clear();
     G4_HIP(hipGetLastError());
     if (c->ev) G4_HIP(hipEventRecord(c->ev[7], c->stream));
     return G4MI_OK;
@@ -264,7 +392,10 @@ static int slippage(Ctx *c, double slip) {
         c->accuslip -= c->sample * direction;
         int last = (c->first + c->ns - 1) % c->ns;
         if (direction < 0) last = (last + 1) % c->ns;
-        if (!c->periodic) launch_zero(c->stream, c->d_field + (long long)last * cells, cells);
+        if (!c->periodic) {
+            launch_zero(c->stream, c->d_field + (long long)last * cells, cells);
+            c->zeroed_since_adi.push_back(last);
+        }
         c->first = last;
         if (direction < 0) c->first = (last + 1) % c->ns;
     }
@@ -274,11 +405,15 @@ static int slippage(Ctx *c, double slip) {
 static int diag(Ctx *c, g4mi_diag_out *o) {
     const int ns = c->ns, ng = c->ng;
     launch_diag_beam(c->stream, ns, c->d_off, c->d_g, c->d_th, c->d_x, c->d_y, c->d_diag_b);
-    launch_diag_field(c->stream, ns, c->d_field, ng, c->d_diag_f);
+    // the fused ADI already summed the field of this step; otherwise sum now
+    if (!c->field_diag_fresh) launch_diag_field(c->stream, ns, c->d_field, ng, c->d_diag_f);
     std::vector<double> hb(8 * (size_t)ns), hf(8 * (size_t)ns);
     G4_HIP(hipMemcpyAsync(hb.data(), c->d_diag_b, hb.size() * 8, hipMemcpyDeviceToHost, c->stream));
     G4_HIP(hipMemcpyAsync(hf.data(), c->d_diag_f, hf.size() * 8, hipMemcpyDeviceToHost, c->stream));
     G4_HIP(hipStreamSynchronize(c->stream));
+    if (c->field_diag_fresh)  // slices zeroed by slippage after the ADI sum to 0
+        for (int z : c->zeroed_since_adi)
+            for (int k = 0; k < 8; k++) hf[8 * (size_t)z + k] = 0;
     // DiagBeam::getValues normalisation (Diagnostic.cpp:452-489)
     for (int is = 0; is < ns; is++) {
         const long long n = c->off[is + 1] - c->off[is];
@@ -481,6 +616,8 @@ int g4mi_destroy(g4mi_ctx *h) {
     Ctx *c = &h->c;
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     for (void *p : c->allocs) (void)hipFree(p);
+    if (c->d_band) (void)hipFree(c->d_band);
+    if (c->d_tilediag) (void)hipFree(c->d_tilediag);
     for (auto &e : c->pool)
         if (e) (void)hipEventDestroy(e);
     if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -551,6 +688,7 @@ int g4mi_field_upload(g4mi_ctx *h, const double *field, int first) {
     G4_HIP(hipStreamSynchronize(c->stream));
     c->first = first;
     c->has_field = true;
+    c->field_diag_fresh = false;
     return G4MI_OK;
 }
 
@@ -618,6 +756,31 @@ int g4mi_sort_index(g4mi_ctx *h, double slen, int *atar, long long *moved) {
     G4_HIP(hipMemcpyAsync(&m, c->d_count, sizeof(m), hipMemcpyDeviceToHost, c->stream));
     G4_HIP(hipStreamSynchronize(c->stream));
     if (moved) *moved = (long long)m;
+    return G4MI_OK;
+}
+
+int g4mi_set_schedule(g4mi_ctx *h, const char *name, int value) {
+    if (!h || !name) return G4MI_E_ARG;
+    Ctx *c = &h->c;
+    if (std::string(name) == "adi_kernel" && value >= -1 && value <= 3) {
+        c->adi_kernel = value;
+        return G4MI_OK;
+    }
+    set_error(std::string("g4mi_set_schedule: unknown entry or value: ") + name);
+    return G4MI_E_ARG;
+}
+
+int g4mi_get_schedule(g4mi_ctx *h, char *buf, int len) {
+    if (!h || !buf || len < 1) return G4MI_E_ARG;
+    Ctx *c = &h->c;
+    char tmp[512];
+    static const char *names[4] = {"thomas", "fused", "fused-prefetch", "band"};
+    snprintf(tmp, sizeof(tmp),
+             "{\"adi_kernel\": %d, \"adi\": \"%s\", \"adi_window\": %d, \"adi_band_m\": %d, "
+             "\"adi_tuned_ms\": [%.4f, %.4f, %.4f, %.4f]}",
+             c->adi_kernel, c->adi_kernel >= 0 ? names[c->adi_kernel] : "untuned", c->adi_tile,
+             c->adi_band_m, c->adi_ms[0], c->adi_ms[1], c->adi_ms[2], c->adi_ms[3]);
+    snprintf(buf, (size_t)len, "%s", tmp);
     return G4MI_OK;
 }
 

@@ -152,6 +152,43 @@ double adi_factors(int ngrid, double delz, double dgrid, double xks, double *c, 
     return rtmp;
 }
 
+double adi_band(int ngrid, const double *c, const double *cbet, const double *cwet, int M,
+                double *band) {
+    typedef std::complex<double> cd;
+    const int n = ngrid, W = 2 * M + 1;
+    std::vector<cd> C(n), B(n), Wt(n), col(n);
+    for (int i = 0; i < n; i++) {
+        C[i] = cd(c[2 * i], c[2 * i + 1]);
+        B[i] = cd(cbet[2 * i], cbet[2 * i + 1]);
+        Wt[i] = cd(cwet[2 * i], cwet[2 * i + 1]);
+    }
+    std::vector<cd> inv((size_t)n * n);
+    for (int j = 0; j < n; j++) {  // column j of A^-1: Thomas on e_j
+        for (int i = 0; i < n; i++) {
+            const cd r = (i == j) ? cd(1, 0) : cd(0, 0);
+            col[i] = (i == 0) ? r * B[0] : (r - C[i] * col[i - 1]) * B[i];
+        }
+        for (int i = n - 2; i >= 0; i--) col[i] -= Wt[i + 1] * col[i + 1];
+        for (int i = 0; i < n; i++) inv[(size_t)i * n + j] = col[i];
+    }
+    double worst = 0;
+    for (int k = 0; k < n; k++) {
+        const double diag = std::abs(inv[(size_t)k * n + k]);
+        for (int j = 0; j < n; j++) {
+            const int m = j - k + M;
+            if (m >= 0 && m < W) continue;
+            worst = std::max(worst, std::abs(inv[(size_t)k * n + j]) / diag);
+        }
+        for (int m = 0; m < W; m++) {
+            const int j = k + m - M;
+            const cd v = (j >= 0 && j < n) ? inv[(size_t)k * n + j] : cd(0, 0);
+            band[2 * ((size_t)k * W + m)] = v.real();
+            band[2 * ((size_t)k * W + m) + 1] = v.imag();
+        }
+    }
+    return worst;
+}
+
 namespace {
 // t = e * m (left-multiplication), as TrackBeam::matmul.
 void left_mul(double m[16], const double e[16]) {

@@ -26,6 +26,14 @@ double bessel_j(int n, double x);
 double adi_factors(int ngrid, double delz, double dgrid, double xks, double *c, double *cbet,
                    double *cwet);
 
+// Band of the inverse of the ADI tridiagonal matrix: band[k][m] =
+// (A^-1)[k][k+m-M] (interleaved complex, ngrid*(2M+1) entries, zero where
+// k+m-M is off the grid), computed column by column with the Thomas factors
+// of adi_factors.  Returns the largest entry outside the band relative to
+// its row's diagonal entry.
+double adi_band(int ngrid, const double *c, const double *cbet, const double *cwet, int M,
+                double *band);
+
 // Chicane transfer matrix of TrackBeam::applyChicane (:131-221), row-major 4x4.
 void chicane_matrix(double angle, double lb, double ld, double lt, double m[16]);
 // TrackBeam::applyR56 factor (:260-269), already scaled to theta units.

@@ -39,6 +39,22 @@ void launch_deposit(hipStream_t s, int ns, const long long *off, const double *g
                     double2 *src, int4 *box, const DepositParams &p);
 void launch_adi(hipStream_t s, int ns, double2 *field, double2 *work, const double2 *src,
                 const int4 *box, const AdiFactors &f, int first);
+// Fused two-sweep ADI (one workgroup per slice, ngrid <= 256) with
+// register-window back substitution of `tile` cells (8 or 16), tiles
+// prefetched into registers (prefetch=1) or loaded when needed (0), and the
+// field diagnostics sums of every storage slice into diag[8*ns].
+int launch_adi_fused(hipStream_t s, int ns, double2 *field, double2 *work, const double2 *src,
+                     const int4 *box, const AdiFactors &f, int first, double *diag, int tile,
+                     int prefetch);
+// Banded-inverse ADI (both half steps, 32x32 output tiles, halo M+1):
+// reads field `fin`, writes `fout` (the host swaps the buffers), band =
+// [ngrid][2M+1] complex rows of A^-1 around the diagonal, a = Im(cstep).
+// m: 7, 8 or 12.  Rows [klo, khi] of the band equal row (klo+khi)/2 to
+// 2^-60.  tilediag: scratch [ns][adi_band_tiles(ngrid)][8].
+int adi_band_tiles(int ng);
+int launch_adi_band(hipStream_t s, int ns, const double2 *fin, double2 *fout, const double2 *src,
+                    const int4 *box, const double2 *band, double a, int ng, int first, int m,
+                    int klo, int khi, double *tilediag, double *diag);
 void launch_zero(hipStream_t s, double2 *p, long long n);
 void launch_diag_beam(hipStream_t s, int ns, const long long *off, const double *g,
                       const double *th, const double *x, const double *y, double *out);

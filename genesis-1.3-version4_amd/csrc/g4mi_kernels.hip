@@ -7,6 +7,8 @@
 // restates; expressions keep the reference's association order.
 #include "g4mi_kernels.h"
 
+#include <algorithm>
+
 namespace g4mi {
 
 // ---------------------------------------------------------------------------
@@ -193,34 +195,28 @@ __global__ void k_slice_sizes(int ns, const long long *__restrict__ off,
 
 // K3a: long-range space charge, EFieldSolver::longRange (:103-116) on one
 // device: E_i = sum_j sgn(i-j) (1 - sqrt(ds^2/(ds^2+size_j))) scl I_j/size_j.
-// The j loop runs in the reference's order, so the sum is bit-identical for
-// identical sizes.
-__global__ void __launch_bounds__(256) k_lsc_long(int ns, const double *__restrict__ cur,
-                                                  const double *__restrict__ fsize, double sl,
-                                                  double gamma, double scl,
-                                                  double *__restrict__ esc) {
-    __shared__ double sc[256], ss[256];
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
+// One wavefront per slice i: lanes take j = lane, lane+64, ... and the 64
+// partial sums are combined in a fixed butterfly order (deterministic).
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+    return v;
+}
+__global__ void __launch_bounds__(64) k_lsc_long(int ns, const double *__restrict__ cur,
+                                                 const double *__restrict__ fsize, double sl,
+                                                 double gamma, double scl,
+                                                 double *__restrict__ esc) {
+    const int i = blockIdx.x, lane = threadIdx.x;
     double e = 0;
-    for (int j0 = 0; j0 < ns; j0 += 256) {
-        __syncthreads();
-        if (j0 + threadIdx.x < ns) {
-            sc[threadIdx.x] = cur[j0 + threadIdx.x];
-            ss[threadIdx.x] = fsize[j0 + threadIdx.x];
-        }
-        __syncthreads();
-        int jn = min(256, ns - j0);
-        if (i < ns) {
-            for (int jj = 0; jj < jn; jj++) {
-                int j = j0 + jj;
-                double ds = (double)(j - i) * sl * gamma;
-                double coef = 1 - sqrt(ds * ds / (ds * ds + ss[jj]));
-                double sgn = (j > i) ? -1. : ((j < i) ? 1. : 0.);
-                e += sgn * coef * scl * sc[jj] / ss[jj];
-            }
-        }
+    for (int j = lane; j < ns; j += 64) {
+        const double sz = fsize[j];
+        const double ds = (double)(j - i) * sl * gamma;
+        const double coef = 1 - sqrt(ds * ds / (ds * ds + sz));
+        const double sgn = (j > i) ? -1. : ((j < i) ? 1. : 0.);
+        e += sgn * coef * scl * cur[j] / sz;
     }
-    if (i < ns) esc[i] = e;
+    e = wave_sum(e);
+    if (lane == 0) esc[i] = e;
 }
 
 // Largest radius from the slice centroid (EFieldSolver::analyseBeam
@@ -251,20 +247,37 @@ __global__ void __launch_bounds__(256) k_sc_rbound(const long long *__restrict__
 
 // The growing radial grid of the short-range solver: rmax is solver state
 // that grows to 1.5 rbound whenever a slice exceeds it, visited in slice
-// order (EFieldSolver.cpp:159-165).  One lane; ns is at most a few
-// thousand.
-__global__ void k_sc_scan(int ns, const long long *__restrict__ off,
-                          const double *__restrict__ rbound, double *__restrict__ rmax_state,
-                          int ngr, double *__restrict__ dr) {
-    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+// order (EFieldSolver.cpp:159-165).  One wavefront walks 64 slices at a
+// time; a ballot finds the next slice that raises rmax, so the serial part
+// is one step per growth event, not per slice.
+__global__ void __launch_bounds__(64) k_sc_scan(int ns, const long long *__restrict__ off,
+                                                const double *__restrict__ rbound,
+                                                double *__restrict__ rmax_state, int ngr,
+                                                double *__restrict__ dr) {
+    const int lane = threadIdx.x;
     double rmax = rmax_state[0];
-    for (int is = 0; is < ns; is++) {
-        if (off[is + 1] - off[is] >= 1) {
-            if (rbound[is] > rmax) rmax = rbound[is] * 1.5;
+    for (int base = 0; base < ns; base += 64) {
+        const int i = base + lane;
+        const bool valid = i < ns;
+        const bool nonempty = valid && (off[i + 1] - off[i] >= 1);
+        const double b = nonempty ? rbound[i] : -1.0;
+        double mine = rmax;
+        int start = 0;
+        for (;;) {
+            const unsigned long long m = __ballot(lane >= start && nonempty && b > rmax);
+            if (m == 0) {
+                if (lane >= start) mine = rmax;
+                break;
+            }
+            const int jl = __ffsll((long long)m) - 1;
+            if (lane >= start && lane < jl) mine = rmax;
+            rmax = __shfl(b, jl) * 1.5;
+            if (lane == jl) mine = rmax;
+            start = jl + 1;
         }
-        dr[is] = rmax / (double)(ngr - 1);
+        if (valid) dr[i] = mine / (double)(ngr - 1);
     }
-    rmax_state[0] = rmax;
+    if (lane == 0) rmax_state[0] = rmax;
 }
 
 // K3b: short-range space charge of one slice per workgroup,
@@ -661,6 +674,531 @@ __global__ void __launch_bounds__(64) k_adi_y(double2 *__restrict__ field,
     }
 }
 
+// ---------------------------------------------------------------------------
+// K5+K6 fused: both ADI half steps of one slice per workgroup.
+//
+// FieldSolverADI::ADI (:60-91) solves (1 - i a d_x^2) u' = S + (1 + i a d_y^2) u
+// row by row, then (1 - i a d_y^2) u'' = S + (1 + i a d_x^2) u' column by
+// column, with Thomas' algorithm (tridagx :94-104, tridagy :106-122).
+// Each lane owns one line and streams along it in tiles of T cells:
+//  * forward elimination y_k = (r_k - c_k y_{k-1}) cbet_k exactly as the
+//    reference;
+//  * back substitution u_k = y_k - cwet_{k+1} u_{k+1} run over a two-tile
+//    register window, started from y at the window's far end.  The start
+//    error reaches tile cells only through a product of >= T+1 factors
+//    |cwet| (the host checks max|cwet|^(T+1) < 2^-66 before choosing this
+//    kernel), so the result equals Thomas' to rounding.  At the line's end
+//    the start value is exact.
+// Row tiles of the x sweep are staged through LDS so every global access is
+// a contiguous run of T cells; the intermediate u' goes to `work` (it stays
+// in L2/MALL between the sweeps of a slice).  The y sweep also accumulates
+// the field diagnostics of DiagField::getValues (:763-861) on the final
+// values: diag[8*fs + {power, x1, x2, y1, y2, ff_re, ff_im, on-axis |E|^2}].
+// ---------------------------------------------------------------------------
+// Barrier that orders LDS traffic only: outstanding global stores are not
+// waited for (the sweeps' global results are published once, at the end of
+// the x sweep, with a full __syncthreads()).
+__device__ __forceinline__ void lds_barrier() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+    __builtin_amdgcn_s_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
+// Deterministic block sum of K doubles for any multiple of 64 threads:
+// butterfly within each wavefront, then the wave totals in wave order.
+template <int NT, int K>
+__device__ __forceinline__ void block_sum_waves(double (&v)[K], double *lds) {
+    const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
+#pragma unroll
+    for (int k = 0; k < K; k++) v[k] = wave_sum(v[k]);
+    __syncthreads();
+    if (lane == 0)
+#pragma unroll
+        for (int k = 0; k < K; k++) lds[wv * K + k] = v[k];
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < K; k++) {
+        double acc = lds[k];
+        for (int q = 1; q < NT / 64; q++) acc += lds[q * K + k];
+        v[k] = acc;
+    }
+    __syncthreads();
+}
+
+template <int T, int G, bool PREF>
+__global__ void __launch_bounds__(G <= 192 ? 192 : 256)
+    k_adi_fused(double2 *__restrict__ field, double2 *__restrict__ work,
+                const double2 *__restrict__ src, const int4 *__restrict__ box, AdiFactors f,
+                int first, int ns, double *__restrict__ diag) {
+    constexpr int NT = G <= 192 ? 192 : 256;
+    // prefetch registers per lane (ng <= G); without PREF tiles are loaded
+    // straight into LDS when they are needed
+    constexpr int PF = PREF ? (G * T + NT - 1) / NT : 1;
+    extern __shared__ double2 lds[];
+    const int ng = f.ngrid;
+    const int t = threadIdx.x;
+    const int ii = blockIdx.x;
+    const long long fs = (long long)((ii + first) % ns);
+    double2 *u = field + fs * ng * ng;
+    double2 *w = work + fs * ng * ng;
+    const double2 *S = src + (long long)ii * ng * ng;
+    const int4 bx = box[ii];
+    const double2 cst = make_double2(0.0, f.cstep);
+    const int ntile = (ng + T - 1) / T;
+    const int tcells = ng * T;
+    // x-sweep tiles are [row][T] with a row pitch of T+1 cells: lane iy
+    // reads row iy, and a pitch of 4 (mod 32) dwords keeps each group of 8
+    // lanes of a 16-byte LDS read on disjoint banks (pitch T would put the
+    // whole wavefront on the same four banks).
+    constexpr int TP = T + 1;
+    const int tstride = ng * TP;  // >= tcells, also holds the y-sweep tiles
+    // tile buffers addressed by index so every access stays an LDS access
+    auto buf = [&](int q) { return lds + (q & 1) * tstride; };
+    // Thomas factors in LDS: fc[k], fb[k], fw[k]
+    double2 *fc = lds + 2 * tstride, *fb = fc + ng, *fw = fb + ng;
+    for (int k = t; k < ng; k += NT) {
+        fc[k] = f.c[k];
+        fb[k] = f.cbet[k];
+        fw[k] = f.cwet[k];
+    }
+    double2 ya[T], yb[T], pf[PF], sv[T];
+    double2 yprev = make_double2(0, 0);
+
+    // ---------------- x sweep: lane = row iy, stream along ix -------------
+    // tile j holds columns [jT, jT+T) of every row: buf[.][row*T + col]
+    auto fetch_x = [&](int j) {
+#pragma unroll
+        for (int q = 0; q < PF; q++) {
+            const int e = t + q * NT;
+            const int row = e / T, col = j * T + e % T;
+            pf[q] = (e < tcells && col < ng) ? u[row * ng + col] : make_double2(0, 0);
+        }
+    };
+    auto stash_x = [&](double2 *dst) {
+#pragma unroll
+        for (int q = 0; q < PF; q++) {
+            const int e = t + q * NT;
+            if (e < tcells) dst[(e / T) * TP + e % T] = pf[q];
+        }
+    };
+    auto direct_x = [&](int j, double2 *dst) {
+        for (int e = t; e < tcells; e += NT) {
+            const int row = e / T, col = j * T + e % T;
+            dst[row * TP + e % T] = col < ng ? u[row * ng + col] : make_double2(0, 0);
+        }
+    };
+    if (PREF) {
+        fetch_x(0);
+        stash_x(buf(0));
+    } else {
+        direct_x(0, buf(0));
+    }
+    __syncthreads();
+    const int iy = t;
+    const bool srow = iy >= bx.y && iy <= bx.w;  // this row crosses the source box
+    for (int j = 0; j <= ntile; j++) {
+        double2 *cur = buf(j);
+        // source cells of this tile first, then the next tile's prefetch,
+        // so waiting for the former never waits for the latter
+#pragma unroll
+        for (int kk = 0; kk < T; kk++) {
+            const int k = j * T + kk;
+            sv[kk] = (j < ntile && iy < ng && srow && k >= bx.x && k <= bx.z && k < ng)
+                         ? S[iy * ng + k] : make_double2(0, 0);
+        }
+        if (PREF && j + 1 < ntile) fetch_x(j + 1);
+        if (iy < ng && j < ntile) {
+#pragma unroll
+            for (int kk = 0; kk < T; kk++) {
+                const int k = j * T + kk;
+                if (k < ng) {
+                    const double2 uc = cur[iy * TP + kk];
+                    double2 st;
+                    if (iy == 0) st = csub(cur[(iy + 1) * TP + kk], cscale(uc, 2.0));
+                    else if (iy == ng - 1) st = csub(cur[(iy - 1) * TP + kk], cscale(uc, 2.0));
+                    else st = cadd(csub(cur[(iy + 1) * TP + kk], cscale(uc, 2.0)), cur[(iy - 1) * TP + kk]);
+                    const double2 r = cadd(cadd(sv[kk], uc), cmul(cst, st));
+                    yb[kk] = (k == 0) ? cmul(r, fb[0]) : cmul(csub(r, cmul(fc[k], yprev)), fb[k]);
+                    yprev = yb[kk];
+                }
+            }
+        }
+        lds_barrier();  // every lane is done reading `cur`: reuse it for output
+        if (iy < ng && j > 0) {
+            const int kend = (j < ntile) ? min(j * T + T - 1, ng - 1) : (j * T - 1);
+            double2 v = make_double2(0, 0);
+            bool started = false;
+            if (j < ntile) {
+#pragma unroll
+                for (int kk = T - 1; kk >= 0; kk--) {
+                    const int k = j * T + kk;
+                    if (k <= kend) {
+                        v = started ? csub(yb[kk], cmul(fw[k + 1], v)) : yb[kk];
+                        started = true;
+                    }
+                }
+            }
+#pragma unroll
+            for (int kk = T - 1; kk >= 0; kk--) {
+                const int k = (j - 1) * T + kk;
+                if (k < ng) {
+                    v = started ? csub(ya[kk], cmul(fw[k + 1], v)) : ya[kk];
+                    started = true;
+                    cur[iy * TP + kk] = v;
+                }
+            }
+        }
+        lds_barrier();
+        if (j > 0) {
+            const int k0 = (j - 1) * T;
+            for (int e = t; e < tcells; e += NT) {
+                const int row = e / T, col = k0 + e % T;
+                if (col < ng) w[row * ng + col] = cur[row * TP + e % T];
+            }
+        }
+        if (j + 1 < ntile) {
+            if (PREF) stash_x(buf(j + 1));
+            else direct_x(j + 1, buf(j + 1));
+        }
+#pragma unroll
+        for (int kk = 0; kk < T; kk++) ya[kk] = yb[kk];
+        lds_barrier();
+    }
+    __syncthreads();  // publish the intermediate field of the whole slice
+
+    // ---------------- y sweep: lane = column ix, stream along iy ----------
+    // tile j holds rows [jT, jT+T) of every column: buf[.][row*ng + col]
+    auto fetch_y = [&](int j) {
+#pragma unroll
+        for (int q = 0; q < PF; q++) {
+            const int e = t + q * NT;
+            const int row = j * T + e / ng, col = e % ng;
+            pf[q] = (e < tcells && row < ng) ? w[row * ng + col] : make_double2(0, 0);
+        }
+    };
+    auto stash = [&](double2 *dst) {
+#pragma unroll
+        for (int q = 0; q < PF; q++) {
+            const int e = t + q * NT;
+            if (e < tcells) dst[e] = pf[q];
+        }
+    };
+    auto direct_y = [&](int j, double2 *dst) {
+        for (int e = t; e < tcells; e += NT) {
+            const int row = j * T + e / ng, col = e % ng;
+            dst[e] = row < ng ? w[row * ng + col] : make_double2(0, 0);
+        }
+    };
+    const int ix = t;
+    const bool scol = ix >= bx.x && ix <= bx.z;
+    const double shift = -0.5 * (double)(ng - 1);
+    double dg[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    const int icen = (ng / 2) * ng + (ng / 2);
+    yprev = make_double2(0, 0);
+    if (PREF) {
+        fetch_y(0);
+        stash(buf(0));
+    } else {
+        direct_y(0, buf(0));
+    }
+    lds_barrier();
+    for (int j = 0; j <= ntile; j++) {
+        const double2 *cur = buf(j);
+#pragma unroll
+        for (int kk = 0; kk < T; kk++) {
+            const int k = j * T + kk;
+            sv[kk] = (j < ntile && ix < ng && scol && k >= bx.y && k <= bx.w && k < ng)
+                         ? S[k * ng + ix] : make_double2(0, 0);
+        }
+        if (PREF && j + 1 < ntile) fetch_y(j + 1);
+        if (ix < ng && j < ntile) {
+#pragma unroll
+            for (int kk = 0; kk < T; kk++) {
+                const int k = j * T + kk;
+                if (k < ng) {
+                    const double2 wc = cur[kk * ng + ix];
+                    double2 st;
+                    if (ix == 0) st = csub(cur[kk * ng + ix + 1], cscale(wc, 2.0));
+                    else if (ix == ng - 1) st = csub(cur[kk * ng + ix - 1], cscale(wc, 2.0));
+                    else st = cadd(csub(cur[kk * ng + ix + 1], cscale(wc, 2.0)), cur[kk * ng + ix - 1]);
+                    const double2 r = cadd(cadd(sv[kk], wc), cmul(cst, st));
+                    yb[kk] = (k == 0) ? cmul(r, fb[0]) : cmul(csub(r, cmul(fc[k], yprev)), fb[k]);
+                    yprev = yb[kk];
+                }
+            }
+        }
+        if (ix < ng && j > 0) {
+            const int kend = (j < ntile) ? min(j * T + T - 1, ng - 1) : (j * T - 1);
+            double2 v = make_double2(0, 0);
+            bool started = false;
+            if (j < ntile) {
+#pragma unroll
+                for (int kk = T - 1; kk >= 0; kk--) {
+                    const int k = j * T + kk;
+                    if (k <= kend) {
+                        v = started ? csub(yb[kk], cmul(fw[k + 1], v)) : yb[kk];
+                        started = true;
+                    }
+                }
+            }
+#pragma unroll
+            for (int kk = T - 1; kk >= 0; kk--) {
+                const int k = (j - 1) * T + kk;
+                if (k < ng) {
+                    v = started ? csub(ya[kk], cmul(fw[k + 1], v)) : ya[kk];
+                    started = true;
+                    u[k * ng + ix] = v;
+                    const double wei = v.x * v.x + v.y * v.y;
+                    const double dx = (double)ix + shift, dy = (double)k + shift;
+                    dg[0] += wei;
+                    dg[1] += dx * wei;
+                    dg[2] += dx * dx * wei;
+                    dg[3] += dy * wei;
+                    dg[4] += dy * dy * wei;
+                    dg[5] += v.x;
+                    dg[6] += v.y;
+                    if (k * ng + ix == icen) dg[7] = wei;
+                }
+            }
+        }
+        if (j + 1 < ntile) {
+            if (PREF) stash(buf(j + 1));
+            else direct_y(j + 1, buf(j + 1));
+        }
+#pragma unroll
+        for (int kk = 0; kk < T; kk++) ya[kk] = yb[kk];
+        lds_barrier();
+    }
+    // block reduction of the diagnostics (fixed order); the on-axis value
+    // is carried by exactly one lane, others add 0
+    double *red = (double *)lds;
+    block_sum_waves<NT, 8>(dg, red);
+    if (t == 0)
+        for (int k = 0; k < 8; k++) diag[8 * fs + k] = dg[k];
+}
+
+// ---------------------------------------------------------------------------
+// K5+K6 as a banded operator: both ADI half steps for a 32x32 output tile
+// per workgroup.
+//
+// The x-implicit half step is u' = A^-1 (S + (1 + i a d_y^2) u) with A the
+// constant tridiagonal matrix (1 - i a d^2) of FieldSolverADI::init
+// (:125-182); the y half step is u'' = A^-1 (S + (1 + i a d_x^2) u').  The
+// entries of A^-1 fall off like |a/(1+2a)|^|m| away from the diagonal, so
+// A^-1 is, to rounding, the band |m| <= M whose coefficients band[k][m+M]
+// = (A^-1)[k][k+m] the host builds from the same Thomas factors the
+// reference uses and accepts only when every dropped entry is below 2^-60
+// of the diagonal (1/128 ulp).  A tile then needs u on a halo of M+1 cells, and the
+// update is a separable stencil: no sequential sweep, HBM traffic of one
+// read and one write of the field (the result goes to the second field
+// buffer; the host swaps them).  Complex multiply-adds use fused FMA here:
+// the rounding differs from Thomas' only at the last bits (the banded
+// product is a different, equally accurate, evaluation of A^-1 r).
+// Per-tile field-diagnostic sums go to tilediag[slice][tile][8] and are
+// combined in tile order by k_adi_tile_diag.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double2 cfma(double2 a, double2 b, double2 acc) {
+    acc.x = fma(a.x, b.x, fma(-a.y, b.y, acc.x));
+    acc.y = fma(a.x, b.y, fma(a.y, b.x, acc.y));
+    return acc;
+}
+
+template <int M>
+struct BandGeom {
+    static constexpr int TS = 32;                   // output tile edge
+    static constexpr int W = 2 * M + 1;             // band width
+    static constexpr int RE = TS + 2 * M;           // rows/cols of the E region
+    // LDS row pitches of 1 (mod 8) cells (4 mod 32 dwords): the 8 lanes of a
+    // 128-byte LDS access then fall on 8 distinct bank quads when they
+    // walk down a column
+    static constexpr int PE = ((RE + 6) / 8) * 8 + 1;
+    static constexpr int PG = TS + 1;
+    static constexpr int CX = 8;                    // outputs per lane in the x stage
+    static constexpr int SR = (RE + (256 / RE) - 1) / (256 / RE);  // rows per lane in the E stage
+    static constexpr size_t lds_cells = (size_t)RE * PE + (size_t)RE * PG + 2 * TS * W;
+};
+
+template <int M>
+__global__ void __launch_bounds__(256) k_adi_band(const double2 *__restrict__ fin,
+                                                  double2 *__restrict__ fout,
+                                                  const double2 *__restrict__ src,
+                                                  const int4 *__restrict__ box,
+                                                  const double2 *__restrict__ band, double a,
+                                                  int ng, int first, int ns, int ntx, int klo,
+                                                  int khi, double *__restrict__ tilediag) {
+    using Gm = BandGeom<M>;
+    constexpr int TS = Gm::TS, W = Gm::W, RE = Gm::RE, PE = Gm::PE, PG = Gm::PG, CX = Gm::CX;
+    constexpr int NT = 256;
+    extern __shared__ double2 sm[];
+    double2 *E = sm;                  // [RE][PE] E = S + (1 + i a d_y^2) u
+    double2 *G = E + RE * PE;         // [RE][PG] F = S + 2 u' - E
+    double2 *BX = G + RE * PG;        // [TS][W] band rows of x (edge tiles)
+    double2 *BY = BX + TS * W;        // [TS][W] band rows of y (edge tiles)
+    const int t = threadIdx.x;
+    const int ii = blockIdx.y;        // beam slice
+    const int tile = blockIdx.x;
+    const int tx0 = (tile % ntx) * TS, ty0 = (tile / ntx) * TS;
+    const long long fs = (long long)((ii + first) % ns);
+    const double2 *u = fin + fs * ng * ng;
+    double2 *uo = fout + fs * ng * ng;
+    const double2 *S = src + (long long)ii * ng * ng;
+    const int4 bx = box[ii];
+    auto sat = [&](int x, int y) {
+        return (x >= bx.x && x <= bx.z && y >= bx.y && y <= bx.w) ? S[y * ng + x]
+                                                                  : make_double2(0, 0);
+    };
+    // rows [klo, khi] of A^-1 share the band of row (klo+khi)/2 to 2^-60
+    const bool xin = tx0 >= klo && tx0 + TS - 1 <= khi;
+    const bool yin = ty0 >= klo && ty0 + TS - 1 <= khi;
+    const double2 *bc = band + (long long)((klo + khi) / 2) * W;
+    if (!xin)
+        for (int e = t; e < TS * W; e += NT) {
+            const int x = tx0 + e / W;
+            BX[e] = x < ng ? band[(long long)x * W + e % W] : make_double2(0, 0);
+        }
+    if (!yin)
+        for (int e = t; e < TS * W; e += NT) {
+            const int y = ty0 + e / W;
+            BY[e] = y < ng ? band[(long long)y * W + e % W] : make_double2(0, 0);
+        }
+    // explicit y stencil of the x half step (FieldSolverADI.cpp:64-72),
+    // zero outside the grid.  Lane -> one column of the E region and a strip
+    // of SR rows: the SR+2 field loads (and the source loads) are all issued
+    // before any is used, so a tile costs one memory round trip.
+    {
+        constexpr int NSTRIP = (RE + Gm::SR - 1) / Gm::SR;
+        constexpr int SR = Gm::SR;
+        if (t < RE * NSTRIP) {
+            const int c = t % RE, r0 = (t / RE) * SR;
+            const int x = tx0 - M + c;
+            const bool xok = x >= 0 && x < ng;
+            double2 uv[SR + 2], sv[SR];
+#pragma unroll
+            for (int q = 0; q < SR + 2; q++) {
+                const int y = ty0 - M + r0 + q - 1;
+                uv[q] = (xok && y >= 0 && y < ng && r0 + q - 1 < RE + 1) ? u[y * ng + x]
+                                                                         : make_double2(0, 0);
+            }
+#pragma unroll
+            for (int q = 0; q < SR; q++) sv[q] = sat(x, ty0 - M + r0 + q);
+#pragma unroll
+            for (int q = 0; q < SR; q++) {
+                const int r = r0 + q, y = ty0 - M + r;
+                if (r < RE) {
+                    double2 v = make_double2(0, 0);
+                    if (xok && y >= 0 && y < ng) {
+                        const double2 uc = uv[q + 1];
+                        const double2 st = cadd(csub(uv[q + 2], cscale(uc, 2.0)), uv[q]);
+                        v = cadd(cadd(sv[q], uc), make_double2(-a * st.y, a * st.x));
+                    }
+                    E[r * PE + c] = v;
+                }
+            }
+        }
+    }
+    __syncthreads();
+    // implicit x: u' = sum_m band[x][m] E(x+m-M, y) on rows [ty0-M, ty0+TS+M);
+    // lane -> (row r = t % RE, columns [CX q, CX q + CX)), CX outputs from one
+    // register window.  Then the y half step's right-hand side
+    // S + (1 + i a d_x^2) u' = S + 2 u' - E  (1 + i a d^2 = 2 - A, A u' = E):
+    // no x stencil on u'.
+    if (t < RE * (TS / CX)) {
+        const int r = t % RE, c0 = (t / RE) * CX;
+        const int y = ty0 - M + r;
+        double2 ev[CX + 2 * M];
+#pragma unroll
+        for (int q = 0; q < CX + 2 * M; q++) ev[q] = E[r * PE + c0 + q];
+        double2 acc[CX];
+#pragma unroll
+        for (int j = 0; j < CX; j++) acc[j] = make_double2(0, 0);
+        if (xin) {  // uniform: the shared band through the scalar cache
+#pragma unroll
+            for (int m = 0; m < W; m++) {
+                const double2 b = bc[m];
+#pragma unroll
+                for (int j = 0; j < CX; j++) acc[j] = cfma(b, ev[j + m], acc[j]);
+            }
+        } else {
+#pragma unroll
+            for (int j = 0; j < CX; j++)
+#pragma unroll
+                for (int m = 0; m < W; m++) acc[j] = cfma(BX[(c0 + j) * W + m], ev[j + m], acc[j]);
+        }
+#pragma unroll
+        for (int j = 0; j < CX; j++) {
+            const int x = tx0 + c0 + j;
+            double2 f = make_double2(0, 0);
+            if (x < ng && y >= 0 && y < ng)
+                f = csub(cadd(sat(x, y), cscale(acc[j], 2.0)), ev[j + M]);
+            G[r * PG + c0 + j] = f;
+        }
+    }
+    __syncthreads();
+    // implicit y: u'' = sum_m band[y][m] F(x, y+m-M), four consecutive y per lane
+    const double shift = -0.5 * (double)(ng - 1);
+    const int icen = (ng / 2) * ng + (ng / 2);
+    double dg[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    {
+        const int c = t % TS, r0 = (t / TS) * 4;
+        const int x = tx0 + c;
+        double2 fv[4 + 2 * M];
+#pragma unroll
+        for (int q = 0; q < 4 + 2 * M; q++) fv[q] = G[(r0 + q) * PG + c];
+        double2 acc4[4];
+#pragma unroll
+        for (int j = 0; j < 4; j++) acc4[j] = make_double2(0, 0);
+        if (yin) {
+#pragma unroll
+            for (int m = 0; m < W; m++) {
+                const double2 b = bc[m];
+#pragma unroll
+                for (int j = 0; j < 4; j++) acc4[j] = cfma(b, fv[j + m], acc4[j]);
+            }
+        } else {
+#pragma unroll
+            for (int j = 0; j < 4; j++)
+#pragma unroll
+                for (int m = 0; m < W; m++) acc4[j] = cfma(BY[(r0 + j) * W + m], fv[j + m], acc4[j]);
+        }
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            const int y = ty0 + r0 + j;
+            const double2 acc = acc4[j];
+            if (x < ng && y < ng) {
+                uo[y * ng + x] = acc;
+                const double wei = acc.x * acc.x + acc.y * acc.y;
+                const double dx = (double)x + shift, dy = (double)y + shift;
+                dg[0] += wei;
+                dg[1] += dx * wei;
+                dg[2] += dx * dx * wei;
+                dg[3] += dy * wei;
+                dg[4] += dy * dy * wei;
+                dg[5] += acc.x;
+                dg[6] += acc.y;
+                if (y * ng + x == icen) dg[7] = wei;
+            }
+        }
+    }
+    double *red = (double *)sm;
+    block_sum_waves<NT, 8>(dg, red);
+    if (t == 0) {
+        double *o = tilediag + ((long long)fs * gridDim.x + tile) * 8;
+        for (int k = 0; k < 8; k++) o[k] = dg[k];
+    }
+}
+
+// Combine the per-tile diagnostic sums of each storage slice in tile order.
+__global__ void __launch_bounds__(64) k_adi_tile_diag(const double *__restrict__ tilediag,
+                                                      int ntiles, int ns,
+                                                      double *__restrict__ diag) {
+    const int fs = blockIdx.x * 64 + threadIdx.x;
+    if (fs >= ns) return;
+    double v[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    for (int k = 0; k < ntiles; k++)
+        for (int q = 0; q < 8; q++) v[q] += tilediag[((long long)fs * ntiles + k) * 8 + q];
+    for (int q = 0; q < 8; q++) diag[8 * fs + q] = v[q];
+}
+
 // Zero one field slice (slippage inflow, Control.cpp:255-267).  A plain
 // store kernel rather than a memset so the step can be graph-captured.
 __global__ void __launch_bounds__(256) k_zero(double2 *__restrict__ p, long long n) {
@@ -769,7 +1307,7 @@ void launch_slice_sizes(hipStream_t s, int ns, const long long *off, const doubl
 }
 void launch_lsc_long(hipStream_t s, int ns, const double *cur, const double *fsize, double sl,
                      double gamma, double scl, double *esc) {
-    hipLaunchKernelGGL(k_lsc_long, dim3(nblk(ns, 256)), dim3(256), 0, s, ns, cur, fsize, sl, gamma, scl, esc);
+    hipLaunchKernelGGL(k_lsc_long, dim3(ns), dim3(64), 0, s, ns, cur, fsize, sl, gamma, scl, esc);
 }
 void launch_sc_rbound(hipStream_t s, int ns, const long long *off, const double *x,
                       const double *y, const double *cen, double *rbound) {
@@ -813,6 +1351,75 @@ void launch_adi(hipStream_t s, int ns, double2 *field, double2 *work, const doub
     hipLaunchKernelGGL(k_adi_x, grid, dim3(64), 0, s, field, work, src, box, f, first, ns);
     hipLaunchKernelGGL(k_adi_y, grid, dim3(64), 0, s, field, work, src, box, f, first, ns);
 }
+template <int T, int G, bool PREF>
+static void launch_adi_fused_t(hipStream_t s, int ns, double2 *field, double2 *work,
+                               const double2 *src, const int4 *box, const AdiFactors &f,
+                               int first, double *diag) {
+    static bool attr = false;
+    if (!attr) {
+        (void)hipFuncSetAttribute((const void *)k_adi_fused<T, G, PREF>,
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+        attr = true;
+    }
+    constexpr int NT = G <= 192 ? 192 : 256;
+    const size_t lds = (size_t)(2 * f.ngrid * (T + 1) + 3 * f.ngrid) * sizeof(double2);
+    hipLaunchKernelGGL((k_adi_fused<T, G, PREF>), dim3(ns), dim3(NT), lds, s, field, work, src,
+                       box, f, first, ns, diag);
+}
+
+int launch_adi_fused(hipStream_t s, int ns, double2 *field, double2 *work, const double2 *src,
+                     const int4 *box, const AdiFactors &f, int first, double *diag, int tile,
+                     int prefetch) {
+    if (f.ngrid > 256 || (tile != 8 && tile != 16)) return -1;
+    const bool small = f.ngrid <= 192;
+#define G4_ADI(T, G, P) launch_adi_fused_t<T, G, P>(s, ns, field, work, src, box, f, first, diag)
+    if (tile == 8) {
+        if (prefetch) { if (small) G4_ADI(8, 192, true); else G4_ADI(8, 256, true); }
+        else { if (small) G4_ADI(8, 192, false); else G4_ADI(8, 256, false); }
+    } else {
+        if (prefetch) { if (small) G4_ADI(16, 192, true); else G4_ADI(16, 256, true); }
+        else { if (small) G4_ADI(16, 192, false); else G4_ADI(16, 256, false); }
+    }
+#undef G4_ADI
+    return 0;
+}
+template <int M>
+static void launch_adi_band_t(hipStream_t s, int ns, const double2 *fin, double2 *fout,
+                              const double2 *src, const int4 *box, const double2 *band, double a,
+                              int ng, int first, int klo, int khi, double *tilediag,
+                              double *diag) {
+    constexpr int TS = BandGeom<M>::TS;
+    static bool attr = false;
+    const size_t lds = BandGeom<M>::lds_cells * sizeof(double2);
+    if (!attr) {
+        (void)hipFuncSetAttribute((const void *)k_adi_band<M>,
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        attr = true;
+    }
+    const int ntx = (ng + TS - 1) / TS;
+    hipLaunchKernelGGL(k_adi_band<M>, dim3(ntx * ntx, ns), dim3(256), lds, s, fin, fout, src, box,
+                       band, a, ng, first, ns, ntx, klo, khi, tilediag);
+    hipLaunchKernelGGL(k_adi_tile_diag, dim3((ns + 63) / 64), dim3(64), 0, s, tilediag,
+                       ntx * ntx, ns, diag);
+}
+
+int adi_band_tiles(int ng) {
+    const int ntx = (ng + 31) / 32;
+    return ntx * ntx;
+}
+
+int launch_adi_band(hipStream_t s, int ns, const double2 *fin, double2 *fout, const double2 *src,
+                    const int4 *box, const double2 *band, double a, int ng, int first, int m,
+                    int klo, int khi, double *tilediag, double *diag) {
+#define G4_BAND(MM) launch_adi_band_t<MM>(s, ns, fin, fout, src, box, band, a, ng, first, klo, khi, tilediag, diag)
+    if (m == 7) G4_BAND(7);
+    else if (m == 8) G4_BAND(8);
+    else if (m == 12) G4_BAND(12);
+    else return -1;
+#undef G4_BAND
+    return 0;
+}
+
 void launch_zero(hipStream_t s, double2 *p, long long n) {
     if (n > 0) hipLaunchKernelGGL(k_zero, dim3(nblk(n, 256)), dim3(256), 0, s, p, n);
 }

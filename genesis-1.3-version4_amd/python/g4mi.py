@@ -81,6 +81,8 @@ def lib():
         L.g4mi_sort_index.argtypes = [C.c_void_p, C.c_double, _ip, _lp]
         L.g4mi_sync.argtypes = [C.c_void_p]
         L.g4mi_set_timing.argtypes = [C.c_void_p, C.c_int]
+        L.g4mi_set_schedule.argtypes = [C.c_void_p, C.c_char_p, C.c_int]
+        L.g4mi_get_schedule.argtypes = [C.c_void_p, C.c_char_p, C.c_int]
         L.g4mi_kernel_times.argtypes = [C.c_void_p, _dp, C.c_int, _lp]
         L.g4mi_parse_lattice.argtypes = [C.c_char_p, C.c_char_p, C.c_double, C.c_double,
                                          C.c_double, C.POINTER(StepParams), C.c_int, _ip]
@@ -242,6 +244,15 @@ class Context:
         _chk(lib().g4mi_sort_index(self.h, slen, atar.ctypes.data_as(_ip), C.byref(moved)),
              "g4mi_sort_index")
         return atar, moved.value
+
+    def set_schedule(self, name, value):
+        _chk(lib().g4mi_set_schedule(self.h, name.encode(), int(value)), "g4mi_set_schedule")
+
+    def schedule(self):
+        import json
+        buf = C.create_string_buffer(1024)
+        _chk(lib().g4mi_get_schedule(self.h, buf, 1024), "g4mi_get_schedule")
+        return json.loads(buf.value.decode())
 
     TIMER_NAMES = ("transverse", "space_charge", "push", "deposit", "adi", "slip_diag")
 

@@ -119,6 +119,16 @@ int g4mi_diag(g4mi_ctx *ctx, g4mi_diag_out *out);
  * with atar != 0 in *moved. atar is a host array of total-particle length. */
 int g4mi_sort_index(g4mi_ctx *ctx, double slen, int *atar, long long *moved);
 
+/* Kernel schedule.  The first field step of a context times every ADI
+ * kernel eligible for its grid and step (two-kernel Thomas, fused
+ * recurrence with and without prefetch, banded inverse) on a scratch copy
+ * of its field and keeps the fastest (the library's autotuner).
+ * g4mi_set_schedule fixes an entry instead ("adi_kernel": 0..3, -1
+ * re-enables tuning); g4mi_get_schedule writes the current choice and the
+ * measured timings as a JSON object into buf. */
+int g4mi_set_schedule(g4mi_ctx *ctx, const char *name, int value);
+int g4mi_get_schedule(g4mi_ctx *ctx, char *buf, int len);
+
 /* Wait for all work queued on the context. */
 int g4mi_sync(g4mi_ctx *ctx);
 

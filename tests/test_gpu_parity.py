@@ -85,6 +85,46 @@ def test_field_track_parity(st):
     assert err < 1e-13
 
 
+@pytest.mark.parametrize("kernel", [0, 1, 2, 3])
+@pytest.mark.parametrize("delz", [0.015, 0.15, 5.0])
+def test_adi_kernels_and_regimes(delz, kernel):
+    """rtmp = dz/(4 ks dx^2) = 3e-3, 3e-2 and 1 on the 151^2 grid: the fused
+    recurrence kernels use an 8-cell window, a 16-cell window, or are not
+    eligible; the banded-inverse kernel a half width 7, 12, or is not
+    eligible.  Every eligible kernel agrees with the oracle's Thomas; an
+    ineligible request falls back to the autotuner's choice."""
+    _log(f"ADI regime delz={delz} kernel={kernel}")
+    sim = H.make_state([1024] * 4, seed_power=5e7)
+    ctx = H.to_gpu(sim)
+    ctx.set_schedule("adi_kernel", kernel)
+    st = O.Step(aw=cases.AW, ku=2 * np.pi / cases.LAMBDAU, ky=(2 * np.pi / cases.LAMBDAU) ** 2,
+                dz=delz)
+    for _ in range(3):
+        O.lib().orc_field_advance(C.byref(sim.s), C.byref(st), delz)
+        ctx.field_track(g4mi.steps_from([st])[0], delz)
+    f, _ = ctx.download_field()
+    err = np.max(np.abs(f - sim.field)) / np.max(np.abs(sim.field))
+    sch = ctx.schedule()
+    _log(f"  field max diff / max |E| = {err:.3e}  schedule {sch}")
+    assert err < 1e-13
+    assert sch["adi_window"] == {0.015: 8, 0.15: 16, 5.0: 0}[delz]
+    assert sch["adi_band_m"] == {0.015: 7, 0.15: 12, 5.0: 0}[delz]
+
+
+def test_adi_autotuner_picks_a_measured_variant():
+    sim = H.make_state([512] * 16, seed_power=5e3)
+    ctx = H.to_gpu(sim)
+    st = H.steps()
+    ctx.field_track(H.gsteps(st)[0], st[0].dz)
+    sch = ctx.schedule()
+    _log(f"  autotuned schedule {sch}")
+    ms = sch["adi_tuned_ms"]
+    assert sch["adi_kernel"] in (0, 1, 2, 3) and ms[sch["adi_kernel"]] == min(m for m in ms if m > 0)
+    O.lib().orc_field_advance(C.byref(sim.s), C.byref(st[0]), st[0].dz)
+    f, _ = ctx.download_field()
+    assert np.max(np.abs(f - sim.field)) < 1e-13 * np.max(np.abs(sim.field))
+
+
 def test_field_track_in_drift_has_no_source(st):
     _log("field parity in a drift step (no source, pure diffraction)")
     sim = H.make_state([256] * 4, seed_power=5e3)
