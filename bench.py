@@ -31,9 +31,11 @@ HBM_PEAK_GBS = 8000.0
 
 
 def build_workload(cfg):
-    ns, npart = cfg["nslice"], cfg["npart"]
-    off = np.arange(ns + 1, dtype=np.int64) * npart
-    slices = [W.slice_beam() for _ in range(ns)]
+    ns = cfg["nslice"]
+    counts, currents = W.counts_and_currents(cfg)
+    off = np.zeros(ns + 1, dtype=np.int64)
+    off[1:] = np.cumsum(counts)
+    slices = [W.slice_beam(current=c) for c in currents]
     beam = g4mi.load_beam(off, slices, cfg["nbins"], cfg["shotnoise"], cfg["seed"], W.LAMBDA_R,
                           1.0, cfg["timerun"])
     steps = g4mi.parse_lattice(W.lattice_text(), "FEL", W.DELZ, W.LAMBDA_R, W.GAMMA0)
@@ -42,27 +44,33 @@ def build_workload(cfg):
     if cfg["seed_power"] > 0:
         f0 = g4mi.load_gauss(ng, cfg["gridmax"], W.LAMBDA_R, cfg["seed_power"], 30e-6)
         field[:] = f0
-    current = np.full(ns, W.CURRENT)
+    current = np.asarray(currents, dtype=np.float64)
     return off, beam, field, current, steps
 
 
-def make_context(cfg, off, beam, field, current):
+def make_context(cfg, off, beam, field, current, precision):
     ctx = g4mi.Context(off, cfg["ngrid"], cfg["gridmax"], W.LAMBDA_R, 1.0, cfg["timerun"],
-                       False, g4mi.FP64)
+                       False, precision)
     ctx.set_efield(cfg["longrange"], cfg["sc_nz"], cfg["sc_nphi"], cfg["sc_ngrid"], 0.0)
     ctx.upload_beam(beam, current)
     ctx.upload_field(field, 0)
     return ctx
 
 
-def gpu_measure(cfg, steps_k, warmup, adi_kernel=-1):
+def gpu_measure(cfg, steps_k, warmup, adi_kernel=-1, precision=g4mi.FP64, breakdown=20, refine=1):
+    t_load = time.perf_counter()
     off, beam, field, current, steps = build_workload(cfg)
-    ctx = make_context(cfg, off, beam, field, current)
-    if adi_kernel >= 0:
+    ctx = make_context(cfg, off, beam, field, current, precision)
+    t_load = time.perf_counter() - t_load
+    del beam, field
+    if adi_kernel >= 0 and precision == g4mi.FP64:
         ctx.set_schedule("adi_kernel", adi_kernel)
+    if precision == g4mi.FP32:
+        ctx.set_schedule("adi_refine", refine)
     nst = len(steps)
-    if warmup + steps_k > nst:
-        raise SystemExit(f"warmup+steps must be <= {nst} (lattice length)")
+    extra = breakdown if precision == g4mi.FP32 else 0
+    if warmup + steps_k + extra > nst:
+        raise SystemExit(f"warmup+steps must be <= {nst - extra} (lattice length)")
     warm = (g4mi.StepParams * warmup)(*steps[:warmup])
     timed = (g4mi.StepParams * steps_k)(*steps[warmup:warmup + steps_k])
     rec_keys = ("power", "bunching")
@@ -76,12 +84,21 @@ def gpu_measure(cfg, steps_k, warmup, adi_kernel=-1):
     if warmup:
         g4mi._chk(L.g4mi_run(ctx.h, warm, warmup, W.GAMMA0, None), "warmup")
     ctx.sync()
-    ctx.set_timing(True)
+    # FP64: kernel groups timed by HIP events inside the timed region.  FP32:
+    # the timed region replays the captured step graphs; the kernel groups
+    # are then timed on `breakdown` further steps launched directly.
+    if precision == g4mi.FP64:
+        ctx.set_timing(True)
     ctx.sync()
     t0 = time.perf_counter()
     done = g4mi._chk(L.g4mi_run(ctx.h, timed, steps_k, W.GAMMA0, rows), "timed run")
     ctx.sync()
     t1 = time.perf_counter()
+    if precision == g4mi.FP32:
+        more = (g4mi.StepParams * extra)(*steps[warmup + steps_k:warmup + steps_k + extra])
+        ctx.set_timing(True)
+        g4mi._chk(L.g4mi_run(ctx.h, more, extra, W.GAMMA0, None), "breakdown run")
+        ctx.sync()
     kt = ctx.kernel_times()
     ctx.set_timing(False)
     sched = ctx.schedule()
@@ -90,18 +107,27 @@ def gpu_measure(cfg, steps_k, warmup, adi_kernel=-1):
     ntot = int(off[-1])
     ctx.close()
     return dict(seconds=t1 - t0, ntot=ntot, dz=dz, kt=kt, power=bufs["power"],
-                bunching=bufs["bunching"], schedule=sched)
+                bunching=bufs["bunching"], schedule=sched, load_s=t_load)
 
 
-def roofline(cfg, kt, ntot):
-    """Dominant kernel and its algorithmic bytes per launch:
-    push: read gamma, theta, x, y, px, py + write gamma, theta = 64 B/particle;
-    ADI (x then y sweep of every cell): field read + write = 32 B/cell;
-    deposit: read x, y, theta, gamma = 32 B/particle;
-    transverse (two half steps): 2 x (read 5 + write 4 doubles) = 144 B/particle."""
+def roofline(cfg, kt, ntot, precision=g4mi.FP64):
+    """Dominant kernel and its algorithmic bytes per launch.
+    FP64 plain form:
+      push: read gamma, theta, x, y, px, py + write gamma, theta = 64 B/particle;
+      ADI (x then y sweep of every cell): field read + write = 32 B/cell;
+      deposit: read x, y, theta, gamma = 32 B/particle;
+      transverse (two half steps): 2 x (read 5 + write 4 doubles) = 144 B/particle.
+    Mixed FP32 form:
+      main (space charge + push + R56 + transverse + deposit): read and write the
+      6 FP32 coordinates = 48 B/particle;
+      ADI: field read + write = 16 B/cell;
+      front (transverse + moments): read 5 + write 4 floats = 36 B/particle."""
     cells = cfg["nslice"] * cfg["ngrid"] ** 2
-    per = {"push": 64.0 * ntot, "adi": 32.0 * cells, "deposit": 32.0 * ntot,
-           "transverse": 144.0 * ntot}
+    if precision == g4mi.FP64:
+        per = {"push": 64.0 * ntot, "adi": 32.0 * cells, "deposit": 32.0 * ntot,
+               "transverse": 144.0 * ntot}
+    else:
+        per = {"push": 48.0 * ntot, "adi": 16.0 * cells, "transverse": 36.0 * ntot}
     name = max(per, key=lambda k: kt[k])
     ms = kt[name]
     gbs = per[name] / (ms * 1e-3) / 1e9
@@ -118,11 +144,12 @@ def cpu_baseline(cfg, nsteps):
     import ctypes as C
     import oracle_py as O
     threads = min(16, os.cpu_count() or 1)
-    ns, npart = cfg["nslice"], cfg["npart"]
-    sim = O.Sim([npart] * ns, cfg["ngrid"], cfg["gridmax"], W.LAMBDA_R, timerun=cfg["timerun"],
+    ns = cfg["nslice"]
+    counts, currents = W.counts_and_currents(cfg)
+    sim = O.Sim(counts, cfg["ngrid"], cfg["gridmax"], W.LAMBDA_R, timerun=cfg["timerun"],
                 sc=(cfg["sc_nz"], cfg["sc_nphi"], cfg["sc_ngrid"], 0.0),
                 longrange=cfg["longrange"], threads=threads)
-    O.load_beam(sim, [W.slice_beam()] * ns, None, cfg["nbins"],
+    O.load_beam(sim, [W.slice_beam(current=c) for c in currents], None, cfg["nbins"],
                 cfg["seed"] if cfg["shotnoise"] and cfg["timerun"] else None, lambda0=W.LAMBDA_R)
     steps = O.unroll(W.fodo_elements(), W.DELZ, W.LAMBDA_R, W.GAMMA0)
     t0 = time.perf_counter()
@@ -130,10 +157,25 @@ def cpu_baseline(cfg, nsteps):
         O.lib().orc_step_all(C.byref(sim.s), C.byref(steps[i]), W.GAMMA0)
         sim.diag()
     dt = time.perf_counter() - t0
-    return dict(value=round(ns * npart * nsteps / dt, 1), unit="macroparticle-steps/s",
+    ntot = int(sum(counts))
+    return dict(value=round(ntot * nsteps / dt, 1), unit="macroparticle-steps/s",
                 cores=threads, kind="port",
-                sample=f"{nsteps} steps of {cfg['name']} (all {ns} slices x {npart}), "
-                       f"{dt:.2f} s on {threads} threads")
+                sample=f"{nsteps} steps of {cfg['name']} in FP64 (all {ns} slices, {ntot} "
+                       f"macroparticles), {dt:.2f} s on {threads} threads")
+
+
+def copy_probe(mib):
+    """A plain device-to-device copy of a known size: its FETCH_SIZE and
+    WRITE_SIZE per dispatch, against mib MiB, give the counters' scale."""
+    import torch
+    n = mib * (1 << 20) // 4
+    src = torch.ones(n, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(5):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_probe_mib": mib, "ok": bool(dst[-1].item() == 1.0)}), flush=True)
+    return 0
 
 
 def main():
@@ -141,23 +183,36 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=100)
     ap.add_argument("--warmup", type=int, default=10)
-    ap.add_argument("--config", choices=("config1", "config0"), default="config1")
+    ap.add_argument("--config", choices=("config1", "config0", "config2", "config3"),
+                    default="config1")
+    ap.add_argument("--precision", choices=("fp64", "fp32"), default=None,
+                    help="override the configuration's precision")
     ap.add_argument("--cpu-steps", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--adi-kernel", type=int, default=-1,
                     help="fix the ADI kernel (0 Thomas, 1 fused, 2 fused+prefetch, 3 banded); "
                          "default: autotuned")
+    ap.add_argument("--adi-refine", type=int, default=1, choices=(0, 1),
+                    help="mixed form: FP64 residual correction in the ADI (default on)")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="only run a plain device copy of MIB MiB five times (the reference "
+                         "point that scales the FETCH_SIZE/WRITE_SIZE counters) and exit")
     a = ap.parse_args()
+    if a.copy_probe:
+        return copy_probe(a.copy_probe)
     if a.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; the slice-block sharding "
                                    "across GPUs is a design, not an implementation",
                           "n_gpus": a.gpus}))
         return 2
-    cfg = W.CONFIG1 if a.config == "config1" else W.CONFIG0
-    m = gpu_measure(cfg, a.steps, a.warmup, a.adi_kernel)
+    cfg = dict(W.CONFIGS[a.config])
+    if a.precision:
+        cfg["precision"] = a.precision
+    prec = g4mi.FP32 if cfg["precision"] == "fp32" else g4mi.FP64
+    m = gpu_measure(cfg, a.steps, a.warmup, a.adi_kernel, prec, refine=a.adi_refine)
     rate = m["ntot"] * a.steps / m["seconds"]
     cells = cfg["nslice"] * cfg["ngrid"] ** 2
-    kname, roof = roofline(cfg, m["kt"], m["ntot"])
+    kname, roof = roofline(cfg, m["kt"], m["ntot"], prec)
     out = {
         "metric": "macroparticle-steps/s",
         "value": round(rate, 1),
@@ -169,16 +224,17 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "f64" if prec == g4mi.FP64 else "f32 (f64 phase base, ADI correction, sums)",
         "data": "synthetic: Hammersley quiet loading + shot noise (seed 7), as LoadBeam",
         "config": {"workload": W.describe(cfg), "slices": cfg["nslice"],
-                   "macroparticles_per_slice": cfg["npart"], "ngrid": cfg["ngrid"],
-                   "precision": "FP64 plain form", "parallelism": "single GPU"},
+                   "macroparticles": m["ntot"], "ngrid": cfg["ngrid"],
+                   "precision": cfg["precision"], "parallelism": "single GPU"},
         "field_cell_updates_per_s": round(cells * a.steps / m["seconds"], 1),
         "s_per_undulator_metre": round(m["seconds"] / m["dz"], 5),
         "kernel_ms_per_step": {k: round(v, 5) for k, v in m["kt"].items() if k != "steps"},
         "roofline": roof,
         "schedule": m["schedule"],
+        "setup_s": round(m["load_s"], 2),
     }
     if not a.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(cfg, a.cpu_steps)

@@ -12,8 +12,12 @@
 #include <string>
 #include <vector>
 
+#include <map>
+#include <tuple>
+
 #include "g4mi_host.h"
 #include "g4mi_kernels.h"
+#include "g4mi_mixed.h"
 
 namespace g4mi {
 
@@ -78,6 +82,29 @@ struct Ctx {
     long long n_timed = 0;
     hipEvent_t *ev = nullptr;  // events of the step being recorded
     std::vector<void *> allocs;
+    // ---- mixed form (precision G4MI_FP32)
+    double gref = 0;                       // reference energy of the FP32 offsets
+    float *m_dg = nullptr, *m_dth = nullptr, *m_x = nullptr, *m_y = nullptr, *m_px = nullptr,
+          *m_py = nullptr;
+    double *m_base = nullptr;              // [ns] FP64 phase base
+    int4 *m_pbox = nullptr;                // [ns] footprint after the first half step
+    float2 *m_field[2] = {nullptr, nullptr}, *m_src = nullptr;
+    double *m_beamsum = nullptr, *m_tilediag = nullptr, *m_rec = nullptr;
+    int m_nrec = 0;
+    MixStep *m_steps = nullptr;            // device step table
+    int m_steps_cap = 0;
+    MixState *m_st = nullptr;
+    float2 *m_bands = nullptr;             // device band table
+    MixBand *m_binfo = nullptr;
+    int m_binfo_cap = 0;
+    std::vector<float2> h_bands;           // host copy of the band table
+    std::vector<MixBand> h_binfo;
+    std::map<double, int> band_of_dz;      // step size -> band index
+    std::vector<int> band_tmpl;            // template half width of each band
+    bool m_refine = true;                  // FP64 residual correction in the ADI
+    bool m_graphs = true;                  // replay captured step graphs
+    std::map<std::tuple<int, int, int, int, int>, hipGraphExec_t> m_graph;
+    std::vector<double> m_rec_host;
 };
 
 template <class T>
@@ -402,18 +429,12 @@ static int slippage(Ctx *c, double slip) {
     return G4MI_OK;
 }
 
-static int diag(Ctx *c, g4mi_diag_out *o) {
-    const int ns = c->ns, ng = c->ng;
-    launch_diag_beam(c->stream, ns, c->d_off, c->d_g, c->d_th, c->d_x, c->d_y, c->d_diag_b);
-    // the fused ADI already summed the field of this step; otherwise sum now
-    if (!c->field_diag_fresh) launch_diag_field(c->stream, ns, c->d_field, ng, c->d_diag_f);
-    std::vector<double> hb(8 * (size_t)ns), hf(8 * (size_t)ns);
-    G4_HIP(hipMemcpyAsync(hb.data(), c->d_diag_b, hb.size() * 8, hipMemcpyDeviceToHost, c->stream));
-    G4_HIP(hipMemcpyAsync(hf.data(), c->d_diag_f, hf.size() * 8, hipMemcpyDeviceToHost, c->stream));
-    G4_HIP(hipStreamSynchronize(c->stream));
-    if (c->field_diag_fresh)  // slices zeroed by slippage after the ADI sum to 0
-        for (int z : c->zeroed_since_adi)
-            for (int k = 0; k < 8; k++) hf[8 * (size_t)z + k] = 0;
+// Normalisation of the per-slice sums: hb[8*is] beam sums per beam slice,
+// hf[8*is0] field sums per storage slice, mapped to beam slices through the
+// ring offset `first`.
+static void finish_diag(const Ctx *c, const std::vector<double> &hb, const std::vector<double> &hf,
+                        int first, g4mi_diag_out *o) {
+    const int ns = c->ns;
     // DiagBeam::getValues normalisation (Diagnostic.cpp:452-489)
     for (int is = 0; is < ns; is++) {
         const long long n = c->off[is + 1] - c->off[is];
@@ -435,7 +456,7 @@ static int diag(Ctx *c, g4mi_diag_out *o) {
     const double scl = c->dgrid * kEev / ks;
     double g_pow = 0;
     for (int is0 = 0; is0 < ns; is0++) {
-        const int is = (ns + is0 - c->first) % ns;
+        const int is = (ns + is0 - first) % ns;
         const double *v = &hf[8 * is0];
         double power = v[0], x1 = v[1], x2 = v[2], y1 = v[3], y2 = v[4];
         g_pow += power;
@@ -464,6 +485,21 @@ static int diag(Ctx *c, g4mi_diag_out *o) {
         g_pow *= scl * scl / kVacimp * c->slicelength / 299792458.0;
         o->global_energy[0] = g_pow;
     }
+}
+
+static int diag(Ctx *c, g4mi_diag_out *o) {
+    const int ns = c->ns, ng = c->ng;
+    launch_diag_beam(c->stream, ns, c->d_off, c->d_g, c->d_th, c->d_x, c->d_y, c->d_diag_b);
+    // the fused ADI already summed the field of this step; otherwise sum now
+    if (!c->field_diag_fresh) launch_diag_field(c->stream, ns, c->d_field, ng, c->d_diag_f);
+    std::vector<double> hb(8 * (size_t)ns), hf(8 * (size_t)ns);
+    G4_HIP(hipMemcpyAsync(hb.data(), c->d_diag_b, hb.size() * 8, hipMemcpyDeviceToHost, c->stream));
+    G4_HIP(hipMemcpyAsync(hf.data(), c->d_diag_f, hf.size() * 8, hipMemcpyDeviceToHost, c->stream));
+    G4_HIP(hipStreamSynchronize(c->stream));
+    if (c->field_diag_fresh)  // slices zeroed by slippage after the ADI sum to 0
+        for (int z : c->zeroed_since_adi)
+            for (int k = 0; k < 8; k++) hf[8 * (size_t)z + k] = 0;
+    finish_diag(c, hb, hf, c->first, o);
     return G4MI_OK;
 }
 
@@ -513,6 +549,466 @@ static int step(Ctx *c, const g4mi_step_params &st, double gamma0) {
     return G4MI_OK;
 }
 
+// ===========================================================================
+// Mixed form (precision G4MI_FP32): device step table, banded FP32 inverse
+// per step size, one graph per launch signature replayed for every step.
+// ===========================================================================
+constexpr int kMixMaxBands = 64;
+constexpr int kMixBandRowMax = 2 * 12 + 1;
+constexpr int kMixRecRows = 32;
+
+static MixArrays mix_arrays(const Ctx *c) {
+    MixArrays a{};
+    a.dg = c->m_dg;
+    a.dth = c->m_dth;
+    a.x = c->m_x;
+    a.y = c->m_y;
+    a.px = c->m_px;
+    a.py = c->m_py;
+    a.base = c->m_base;
+    a.off = c->d_off;
+    a.cur = c->d_cur;
+    a.field[0] = c->m_field[0];
+    a.field[1] = c->m_field[1];
+    a.src = c->m_src;
+    a.box = c->d_box;
+    a.pbox = c->m_pbox;
+    a.mom_fsize = c->d_fsize;
+    a.cen = c->d_cen;
+    a.rbound = c->d_rbound;
+    a.dr = c->d_dr;
+    a.rmax = c->d_rmax;
+    a.esc = c->d_esc;
+    a.efield = c->d_efield;
+    a.beamsum = c->m_beamsum;
+    a.tilediag = c->m_tilediag;
+    a.rec = c->m_rec;
+    a.nrec = c->m_nrec;
+    a.steps = c->m_steps;
+    a.st = c->m_st;
+    a.bands = c->m_bands;
+    a.binfo = c->m_binfo;
+    a.ns = c->ns;
+    a.ng = c->ng;
+    a.sc_ng = c->sc_ng;
+    a.sc_nz = c->sc_nz;
+    a.sc_nphi = c->sc_nphi;
+    a.gref = c->gref;
+    a.gridmax = c->gridmax;
+    a.dgrid = c->dgrid;
+    a.sample = c->sample;
+    a.timerun = c->timerun;
+    a.periodic = c->periodic;
+    return a;
+}
+
+static void mix_clear_graphs(Ctx *c) {
+    for (auto &kv : c->m_graph) (void)hipGraphExecDestroy(kv.second);
+    c->m_graph.clear();
+}
+
+// FP32 banded inverse for step size delz: the narrowest half width whose
+// dropped entries are below 2^-28 of the diagonal, stored with the width of
+// the kernel instantiation that runs it; cached per step size.
+static int mix_band(Ctx *c, double delz, int *band) {
+    auto it = c->band_of_dz.find(delz);
+    if (it != c->band_of_dz.end()) {
+        *band = it->second;
+        return G4MI_OK;
+    }
+    if ((int)c->h_binfo.size() >= kMixMaxBands) {
+        set_error("mixed form: more distinct step sizes than the band table holds (64)");
+        return G4MI_E_ARG;
+    }
+    const int ng = c->ng;
+    std::vector<double> h(6 * (size_t)ng);
+    const double rtmp = adi_factors(ng, delz, c->dgrid, c->xks, h.data(), h.data() + 2 * ng,
+                                    h.data() + 4 * ng);
+    int need = -1, tm = 0;
+    for (int m = 1; m <= 12 && need < 0; m++) {
+        std::vector<double> tmp(2 * (size_t)ng * (2 * m + 1));
+        if (adi_band(ng, h.data(), h.data() + 2 * ng, h.data() + 4 * ng, m, tmp.data()) <
+            std::ldexp(1.0, -28))
+            need = m;
+    }
+    if (need < 0 || !mix_band_supported(need, &tm)) {
+        set_error("mixed form: the step size needs a band wider than 12 cells");
+        return G4MI_E_ARG;
+    }
+    const int W = 2 * tm + 1;
+    std::vector<double> bd(2 * (size_t)ng * W);
+    (void)adi_band(ng, h.data(), h.data() + 2 * ng, h.data() + 4 * ng, tm, bd.data());
+    // interior rows whose band equals the central row's to 2^-30 (FP32)
+    const int kc = ng / 2;
+    auto same = [&](int k) {
+        const double dia = std::hypot(bd[2 * ((size_t)kc * W + tm)], bd[2 * ((size_t)kc * W + tm) + 1]);
+        double d = 0;
+        for (int q = 0; q < W; q++)
+            d = std::max(d, std::hypot(bd[2 * ((size_t)k * W + q)] - bd[2 * ((size_t)kc * W + q)],
+                                       bd[2 * ((size_t)k * W + q) + 1] - bd[2 * ((size_t)kc * W + q) + 1]));
+        return d <= std::ldexp(1.0, -30) * dia;
+    };
+    int lo = kc, hi = kc;
+    while (lo > 0 && same(lo - 1)) lo--;
+    while (hi < ng - 1 && same(hi + 1)) hi++;
+    const int half = std::min(kc - lo, hi - kc);
+    MixBand bi{};
+    bi.off = (long long)c->h_bands.size();
+    bi.a = rtmp;
+    bi.klo = kc - half;
+    bi.khi = kc + half;
+    bi.m = need;
+    for (size_t k = 0; k < bd.size() / 2; k++)
+        c->h_bands.push_back(make_float2((float)bd[2 * k], (float)bd[2 * k + 1]));
+    const int idx = (int)c->h_binfo.size();
+    c->h_binfo.push_back(bi);
+    c->band_tmpl.push_back(tm);
+    G4_HIP(hipMemcpyAsync(c->m_bands + bi.off, c->h_bands.data() + bi.off, bd.size() / 2 * sizeof(float2),
+                          hipMemcpyHostToDevice, c->stream));
+    G4_HIP(hipMemcpyAsync(c->m_binfo + idx, &c->h_binfo[idx], sizeof(MixBand), hipMemcpyHostToDevice,
+                          c->stream));
+    G4_HIP(hipStreamSynchronize(c->stream));
+    c->band_of_dz[delz] = idx;
+    *band = idx;
+    return G4MI_OK;
+}
+
+static int mix_fill_step(Ctx *c, const g4mi_step_params &st, double gamma0, MixStep &m) {
+    const double delz = st.dz;
+    m = MixStep{};
+    m.t1 = transverse_params(c, st, gamma0, delz * 0.5, false);
+    m.t2 = transverse_params(c, st, gamma0, delz * 0.5, true);
+    PushParams &p = m.p;
+    p.delz = delz;
+    p.xks = c->xks;
+    p.rtmp = coupling(st, 1) / c->xks;
+    p.xku = st.ku;
+    if (p.xku == 0) p.xku = c->xks * 0.5 / gamma0 / gamma0;
+    p.aw = st.aw;
+    p.autophase = st.phaseshift;
+    p.ax = st.ax;
+    p.ay = st.ay;
+    p.kx = st.kx;
+    p.ky = st.ky;
+    p.gradx = st.gradx;
+    p.grady = st.grady;
+    p.gridmax = c->gridmax;
+    p.dgrid = c->dgrid;
+    p.ngrid = c->ng;
+    DepositParams &d = m.d;
+    d.enabled = st.aw > 0 ? 1 : 0;
+    d.fcoup = coupling(st, 1);
+    d.xks = c->xks;
+    d.delz = delz;
+    d.dgrid = c->dgrid;
+    d.gridmax = c->gridmax;
+    d.ax = st.ax;
+    d.ay = st.ay;
+    d.kx = st.kx;
+    d.ky = st.ky;
+    d.gradx = st.gradx;
+    d.grady = st.grady;
+    d.ngrid = c->ng;
+    m.gamma0 = gamma0;
+    m.chicane = st.chic_angle != 0 ? 1 : 0;
+    if (m.chicane) m.r56 = chicane_r56(st.chic_angle, st.chic_lb, st.chic_ld, c->lambda0, gamma0);
+    // reference particle: gamma = gref, on axis, no transverse momentum, no field
+    m.eps_ref = (1 + st.aw * st.aw) / (c->gref * c->gref);
+    // xks (1 - 1/beta) + xku with 1 - 1/beta = -eps / (beta (1 + beta)), no cancellation
+    const double bref = sqrt(1. - m.eps_ref);
+    m.kappa = -c->xks * m.eps_ref / (bref * (1. + bref)) + p.xku;
+    m.longrange = c->longrange;
+    if (c->longrange) {
+        const double gamma = gamma0 / sqrt(1 + st.aw * st.aw);
+        m.lsc_ds = c->slicelength * gamma;
+        m.lsc_scl = c->slicelength / 2. / asin(1) / 299792458.0 / 2 / 8.85e-12;
+    }
+    m.use_sc = (c->sc_nz > 0 && c->sc_ng > 2) ? 1 : 0;
+    m.sc_gz2 = gamma0 * gamma0 / (1 + st.aw * st.aw);
+    m.sc_ks = 4 * asin(1) / c->lambda0;
+    m.slip = st.slip;
+    G4_TRY(mix_band(c, delz, &m.band));
+    return G4MI_OK;
+}
+
+typedef std::tuple<int, int, int, int, int> MixKey;
+
+static int mix_launch(Ctx *c, const MixKey &k, hipEvent_t *ev) {
+    const MixArrays a = mix_arrays(c);
+    if (launch_mix_step(c->stream, a, std::get<0>(k), std::get<1>(k) != 0, std::get<2>(k) != 0,
+                        std::get<3>(k) != 0, std::get<4>(k), ev) != 0) {
+        set_error("mixed form: no ADI instantiation for this band width");
+        return G4MI_E_ARG;
+    }
+    G4_HIP(hipGetLastError());
+    return G4MI_OK;
+}
+
+static int mix_graph(Ctx *c, const MixKey &k, hipGraphExec_t *out) {
+    auto it = c->m_graph.find(k);
+    if (it != c->m_graph.end()) {
+        *out = it->second;
+        return G4MI_OK;
+    }
+    hipGraph_t g = nullptr;
+    G4_HIP(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    const int rc = mix_launch(c, k, nullptr);
+    hipError_t e = hipStreamEndCapture(c->stream, &g);
+    if (rc != G4MI_OK) {
+        if (g) (void)hipGraphDestroy(g);
+        return rc;
+    }
+    G4_HIP(e);
+    hipGraphExec_t ex = nullptr;
+    e = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    G4_HIP(e);
+    c->m_graph[k] = ex;
+    *out = ex;
+    return G4MI_OK;
+}
+
+// DiagBeam/DiagField normalisation of one record row (sums per beam slice).
+static void mix_finish_row(const Ctx *c, const double *row, g4mi_diag_out *o) {
+    const int ns = c->ns;
+    std::vector<double> hb(8 * (size_t)ns), hf(8 * (size_t)ns);
+    for (int is = 0; is < ns; is++)
+        for (int q = 0; q < 8; q++) {
+            hb[8 * (size_t)is + q] = row[(size_t)is * kMixRecord + q];
+            hf[8 * (size_t)is + q] = row[(size_t)is * kMixRecord + 8 + q];
+        }
+    finish_diag(c, hb, hf, 0, o);
+}
+
+static int mix_run(Ctx *c, const g4mi_step_params *steps, int nsteps, double gamma0,
+                   g4mi_diag_out *records, int *done) {
+    *done = 0;
+    if (!c->has_beam || !c->has_field) {
+        set_error("beam and field must be uploaded before stepping");
+        return G4MI_E_STATE;
+    }
+    int n = 0;
+    while (n < nsteps && !(steps[n].marker & 8)) n++;
+    if (n == 0) return G4MI_OK;
+    std::vector<MixStep> tab(n);
+    std::vector<MixKey> keys(n);
+    const bool sc = (c->sc_nz > 0 && c->sc_ng > 2);
+    const bool moments = sc || c->longrange;
+    int maxzero = 0;
+    if (c->timerun && !c->periodic)
+        for (int i = 0; i < n; i++)
+            maxzero = std::max(maxzero, (int)std::ceil(std::fabs(steps[i].slip) / (0.8 * c->sample)) + 1);
+    if (sc && (2 * c->sc_nphi + 1) * c->sc_nz * c->sc_ng > mix_sc_mode_cells()) {
+        set_error("mixed form: (2 nphi + 1) nz x radial cells of the space-charge solver exceed " +
+                  std::to_string(mix_sc_mode_cells()));
+        return G4MI_E_ARG;
+    }
+    if (maxzero > kMixMaxZero) {
+        set_error("mixed form: a step slips more slices than the device zero list holds");
+        return G4MI_E_ARG;
+    }
+    for (int i = 0; i < n; i++) {
+        G4_TRY(mix_fill_step(c, steps[i], gamma0, tab[i]));
+        const int tm = c->band_tmpl[tab[i].band];
+        const int refine = (c->m_refine && tm <= 9) ? 1 : 0;
+        keys[i] = MixKey(tm, refine, sc ? 1 : 0, moments ? 1 : 0, maxzero);
+    }
+    if (n > c->m_steps_cap) {
+        if (c->m_steps) (void)hipFree(c->m_steps);
+        c->m_steps = nullptr;
+        c->m_steps_cap = 0;
+        G4_HIP(hipMalloc(&c->m_steps, (size_t)std::max(n, 1024) * sizeof(MixStep)));
+        c->m_steps_cap = std::max(n, 1024);
+        mix_clear_graphs(c);  // the table pointer is part of every captured launch
+    }
+    G4_HIP(hipMemcpyAsync(c->m_steps, tab.data(), (size_t)n * sizeof(MixStep), hipMemcpyHostToDevice,
+                          c->stream));
+    const int zero = 0;
+    G4_HIP(hipMemcpyAsync(&c->m_st->istep, &zero, sizeof(int), hipMemcpyHostToDevice, c->stream));
+    G4_HIP(hipStreamSynchronize(c->stream));
+    const size_t rowd = (size_t)c->ns * kMixRecord;
+    int pending = 0;  // first step whose record is not yet copied back
+    for (int i = 0; i < n; i++) {
+        if (c->timing) {
+            // timed step: direct launches with the kernel groups bracketed by events
+            G4_TRY(step_begin_timing(c));
+            G4_TRY(mix_launch(c, keys[i], c->ev));
+            c->ev = nullptr;
+            c->pool_used++;
+        } else if (c->m_graphs) {
+            hipGraphExec_t ex = nullptr;
+            G4_TRY(mix_graph(c, keys[i], &ex));
+            G4_HIP(hipGraphLaunch(ex, c->stream));
+        } else {
+            G4_TRY(mix_launch(c, keys[i], nullptr));
+        }
+        if (records && ((i + 1 - pending) == c->m_nrec || i == n - 1)) {
+            const int cnt = i + 1 - pending;
+            c->m_rec_host.resize((size_t)c->m_nrec * rowd);
+            G4_HIP(hipMemcpyAsync(c->m_rec_host.data(), c->m_rec, (size_t)c->m_nrec * rowd * sizeof(double),
+                                  hipMemcpyDeviceToHost, c->stream));
+            G4_HIP(hipStreamSynchronize(c->stream));
+            for (int k = 0; k < cnt; k++) {
+                const int stp = pending + k;
+                mix_finish_row(c, c->m_rec_host.data() + (size_t)(stp % c->m_nrec) * rowd, &records[stp]);
+            }
+            pending = i + 1;
+        }
+    }
+    MixState hs{};
+    G4_HIP(hipMemcpyAsync(&hs, c->m_st, sizeof(MixState), hipMemcpyDeviceToHost, c->stream));
+    G4_HIP(hipStreamSynchronize(c->stream));
+    c->first = hs.first;
+    c->accuslip = hs.accuslip;
+    *done = n;
+    return G4MI_OK;
+}
+
+static int mix_diag(Ctx *c, g4mi_diag_out *o) {
+    const MixArrays a = mix_arrays(c);
+    launch_mix_diag(c->stream, a, c->d_diag_b, c->d_diag_f);
+    const int ns = c->ns;
+    std::vector<double> hb(8 * (size_t)ns), hf(8 * (size_t)ns);
+    MixState hs{};
+    G4_HIP(hipMemcpyAsync(hb.data(), c->d_diag_b, hb.size() * 8, hipMemcpyDeviceToHost, c->stream));
+    G4_HIP(hipMemcpyAsync(hf.data(), c->d_diag_f, hf.size() * 8, hipMemcpyDeviceToHost, c->stream));
+    G4_HIP(hipMemcpyAsync(&hs, c->m_st, sizeof(MixState), hipMemcpyDeviceToHost, c->stream));
+    G4_HIP(hipStreamSynchronize(c->stream));
+    finish_diag(c, hb, hf, hs.first, o);
+    return G4MI_OK;
+}
+
+static int mix_split_call(const char *name) {
+    set_error(std::string(name) + ": the mixed form advances whole steps (g4mi_step, g4mi_run)");
+    return G4MI_E_STATE;
+}
+
+// Upload in offset form: gref = the beam's mean energy rounded to an FP32
+// value, base = 2 pi floor(<theta>_slice / 2 pi) per slice.
+static int mix_beam_upload(Ctx *c, const double *g, const double *th, const double *x,
+                           const double *y, const double *px, const double *py, const double *cur) {
+    const long long n = c->ntot;
+    const int ns = c->ns;
+    double gsum = 0;
+    for (long long i = 0; i < n; i++) gsum += g[i];
+    c->gref = n > 0 ? (double)(float)(gsum / (double)n) : 1.0;
+    const double twopi = 4. * asin(1.0);
+    std::vector<double> base(ns, 0.0);
+    std::vector<float> hv(6 * (size_t)n);
+    float *fg = hv.data(), *ft = fg + n, *fx = ft + n, *fy = fx + n, *fpx = fy + n, *fpy = fpx + n;
+    for (int is = 0; is < ns; is++) {
+        const long long a = c->off[is], b = c->off[is + 1];
+        double ts = 0;
+        for (long long i = a; i < b; i++) ts += th[i];
+        if (b > a) base[is] = twopi * floor(ts / (double)(b - a) / twopi);
+        for (long long i = a; i < b; i++) {
+            fg[i] = (float)(g[i] - c->gref);
+            ft[i] = (float)(th[i] - base[is]);
+            fx[i] = (float)x[i];
+            fy[i] = (float)y[i];
+            fpx[i] = (float)px[i];
+            fpy[i] = (float)py[i];
+        }
+    }
+    float *dst[6] = {c->m_dg, c->m_dth, c->m_x, c->m_y, c->m_px, c->m_py};
+    for (int k = 0; k < 6; k++)
+        if (n) G4_HIP(hipMemcpyAsync(dst[k], hv.data() + (size_t)k * n, (size_t)n * sizeof(float),
+                                     hipMemcpyHostToDevice, c->stream));
+    G4_HIP(hipMemcpyAsync(c->m_base, base.data(), ns * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    G4_HIP(hipMemcpyAsync(c->d_cur, cur, ns * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    G4_HIP(hipStreamSynchronize(c->stream));
+    mix_clear_graphs(c);  // gref is part of every captured launch
+    c->has_beam = true;
+    return G4MI_OK;
+}
+
+static int mix_beam_download(Ctx *c, double *g, double *th, double *x, double *y, double *px,
+                             double *py) {
+    const long long n = c->ntot;
+    const int ns = c->ns;
+    std::vector<float> hv(6 * (size_t)n);
+    std::vector<double> base(ns);
+    const float *src[6] = {c->m_dg, c->m_dth, c->m_x, c->m_y, c->m_px, c->m_py};
+    for (int k = 0; k < 6; k++)
+        if (n) G4_HIP(hipMemcpyAsync(hv.data() + (size_t)k * n, src[k], (size_t)n * sizeof(float),
+                                     hipMemcpyDeviceToHost, c->stream));
+    G4_HIP(hipMemcpyAsync(base.data(), c->m_base, ns * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    G4_HIP(hipStreamSynchronize(c->stream));
+    const float *fg = hv.data(), *ft = fg + n;
+    for (int is = 0; is < ns; is++)
+        for (long long i = c->off[is]; i < c->off[is + 1]; i++) {
+            if (g) g[i] = c->gref + (double)fg[i];
+            if (th) th[i] = base[is] + (double)ft[i];
+        }
+    double *dst[4] = {x, y, px, py};
+    for (int k = 0; k < 4; k++)
+        if (dst[k])
+            for (long long i = 0; i < n; i++) dst[k][i] = (double)hv[(size_t)(k + 2) * n + i];
+    return G4MI_OK;
+}
+
+static int mix_field_upload(Ctx *c, const double *field, int first) {
+    const size_t cells = (size_t)c->ns * c->ng * c->ng;
+    std::vector<float2> hf(cells);
+    for (size_t k = 0; k < cells; k++) hf[k] = make_float2((float)field[2 * k], (float)field[2 * k + 1]);
+    MixState hs{};
+    G4_HIP(hipMemcpyAsync(&hs, c->m_st, sizeof(MixState), hipMemcpyDeviceToHost, c->stream));
+    G4_HIP(hipStreamSynchronize(c->stream));
+    hs.first = first;
+    hs.cur = 0;
+    hs.nzero = 0;
+    G4_HIP(hipMemcpyAsync(c->m_field[0], hf.data(), cells * sizeof(float2), hipMemcpyHostToDevice, c->stream));
+    G4_HIP(hipMemcpyAsync(c->m_st, &hs, sizeof(MixState), hipMemcpyHostToDevice, c->stream));
+    G4_HIP(hipStreamSynchronize(c->stream));
+    c->first = first;
+    c->has_field = true;
+    return G4MI_OK;
+}
+
+static int mix_field_download(Ctx *c, double *field, int *first) {
+    MixState hs{};
+    G4_HIP(hipMemcpyAsync(&hs, c->m_st, sizeof(MixState), hipMemcpyDeviceToHost, c->stream));
+    G4_HIP(hipStreamSynchronize(c->stream));
+    if (field) {
+        const size_t cells = (size_t)c->ns * c->ng * c->ng;
+        std::vector<float2> hf(cells);
+        G4_HIP(hipMemcpyAsync(hf.data(), c->m_field[hs.cur], cells * sizeof(float2), hipMemcpyDeviceToHost,
+                              c->stream));
+        G4_HIP(hipStreamSynchronize(c->stream));
+        for (size_t k = 0; k < cells; k++) {
+            field[2 * k] = hf[k].x;
+            field[2 * k + 1] = hf[k].y;
+        }
+    }
+    if (first) *first = hs.first;
+    return G4MI_OK;
+}
+
+static int mix_alloc(Ctx *c) {
+    const size_t n = (size_t)c->ntot, ns = (size_t)c->ns, cells = (size_t)c->ng * c->ng;
+    int rc;
+    if ((rc = dalloc(c, &c->m_dg, n)) || (rc = dalloc(c, &c->m_dth, n)) || (rc = dalloc(c, &c->m_x, n)) ||
+        (rc = dalloc(c, &c->m_y, n)) || (rc = dalloc(c, &c->m_px, n)) || (rc = dalloc(c, &c->m_py, n)) ||
+        (rc = dalloc(c, &c->m_base, ns)) || (rc = dalloc(c, &c->m_pbox, ns)) ||
+        (rc = dalloc(c, &c->m_field[0], ns * cells)) ||
+        (rc = dalloc(c, &c->m_field[1], ns * cells)) || (rc = dalloc(c, &c->m_src, ns * cells)) ||
+        (rc = dalloc(c, &c->m_beamsum, 8 * ns)) ||
+        (rc = dalloc(c, &c->m_tilediag, ns * mix_band_tiles(c->ng) * 8)) ||
+        (rc = dalloc(c, &c->m_rec, (size_t)kMixRecRows * ns * kMixRecord)) ||
+        (rc = dalloc(c, &c->m_st, 1)) ||
+        (rc = dalloc(c, &c->m_bands, (size_t)kMixMaxBands * c->ng * kMixBandRowMax)) ||
+        (rc = dalloc(c, &c->m_binfo, (size_t)kMixMaxBands)))
+        return rc;
+    c->m_nrec = kMixRecRows;
+    if (mix_prepare() != 0) {
+        set_error("mixed form: setting the kernels' LDS limits failed");
+        return G4MI_E_HIP;
+    }
+    MixState hs{};
+    G4_HIP(hipMemcpy(c->m_st, &hs, sizeof(MixState), hipMemcpyHostToDevice));
+    return G4MI_OK;
+}
+
 }  // namespace g4mi
 
 using namespace g4mi;
@@ -536,8 +1032,8 @@ int g4mi_create(g4mi_ctx **out, int device, int nslice, const long long *off, in
         set_error("g4mi_create: ngrid above 4096 is not supported");
         return G4MI_E_ARG;
     }
-    if (precision != G4MI_FP64) {
-        set_error("g4mi_create: precision not available in this build");
+    if (precision != G4MI_FP64 && precision != G4MI_FP32) {
+        set_error("g4mi_create: precision must be G4MI_FP64 or G4MI_FP32");
         return G4MI_E_ARG;
     }
     for (int i = 0; i < nslice; i++)
@@ -576,7 +1072,10 @@ int g4mi_create(g4mi_ctx **out, int device, int nslice, const long long *off, in
         return r;
     };
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) return fail(G4MI_E_HIP);
-    const size_t n = (size_t)c->ntot, ns = (size_t)nslice, cells = (size_t)ngrid * ngrid;
+    const bool fp64 = precision == G4MI_FP64;
+    // the FP64 particle and field arrays exist only in the plain form
+    const size_t n = fp64 ? (size_t)c->ntot : 0, ns = (size_t)nslice,
+                 cells = fp64 ? (size_t)ngrid * ngrid : 0;
     if ((rc = dalloc(c, &c->d_off, ns + 1)) || (rc = dalloc(c, &c->d_g, n)) ||
         (rc = dalloc(c, &c->d_th, n)) || (rc = dalloc(c, &c->d_x, n)) ||
         (rc = dalloc(c, &c->d_y, n)) || (rc = dalloc(c, &c->d_px, n)) ||
@@ -591,6 +1090,7 @@ int g4mi_create(g4mi_ctx **out, int device, int nslice, const long long *off, in
         (rc = dalloc(c, &c->d_diag_b, 8 * ns)) || (rc = dalloc(c, &c->d_diag_f, 8 * ns)) ||
         (rc = dalloc(c, &c->d_count, 1)))
         return fail(rc);
+    if (!fp64 && (rc = mix_alloc(c))) return fail(rc);
     std::vector<int> bs;
     std::vector<long long> bb;
     build_block_table(c->off, bs, bb);
@@ -618,6 +1118,8 @@ int g4mi_destroy(g4mi_ctx *h) {
     for (void *p : c->allocs) (void)hipFree(p);
     if (c->d_band) (void)hipFree(c->d_band);
     if (c->d_tilediag) (void)hipFree(c->d_tilediag);
+    if (c->m_steps) (void)hipFree(c->m_steps);
+    mix_clear_graphs(c);
     for (auto &e : c->pool)
         if (e) (void)hipEventDestroy(e);
     if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -652,6 +1154,7 @@ int g4mi_beam_upload(g4mi_ctx *h, const double *g, const double *th, const doubl
         return G4MI_E_ARG;
     }
     Ctx *c = &h->c;
+    if (c->precision == G4MI_FP32) return mix_beam_upload(c, g, th, x, y, px, py, cur);
     const size_t nb = (size_t)c->ntot * sizeof(double);
     const double *src[6] = {g, th, x, y, px, py};
     double *dst[6] = {c->d_g, c->d_th, c->d_x, c->d_y, c->d_px, c->d_py};
@@ -667,6 +1170,7 @@ int g4mi_beam_download(g4mi_ctx *h, double *g, double *th, double *x, double *y,
                        double *py) {
     if (!h) return G4MI_E_ARG;
     Ctx *c = &h->c;
+    if (c->precision == G4MI_FP32) return mix_beam_download(c, g, th, x, y, px, py);
     const size_t nb = (size_t)c->ntot * sizeof(double);
     double *dst[6] = {g, th, x, y, px, py};
     const double *src[6] = {c->d_g, c->d_th, c->d_x, c->d_y, c->d_px, c->d_py};
@@ -683,6 +1187,7 @@ int g4mi_field_upload(g4mi_ctx *h, const double *field, int first) {
         set_error("g4mi_field_upload: first out of range");
         return G4MI_E_ARG;
     }
+    if (c->precision == G4MI_FP32) return mix_field_upload(c, field, first);
     G4_HIP(hipMemcpyAsync(c->d_field, field, (size_t)c->ns * c->ng * c->ng * sizeof(double2),
                           hipMemcpyHostToDevice, c->stream));
     G4_HIP(hipStreamSynchronize(c->stream));
@@ -695,6 +1200,7 @@ int g4mi_field_upload(g4mi_ctx *h, const double *field, int first) {
 int g4mi_field_download(g4mi_ctx *h, double *field, int *first) {
     if (!h) return G4MI_E_ARG;
     Ctx *c = &h->c;
+    if (c->precision == G4MI_FP32) return mix_field_download(c, field, first);
     if (field)
         G4_HIP(hipMemcpyAsync(field, c->d_field, (size_t)c->ns * c->ng * c->ng * sizeof(double2),
                               hipMemcpyDeviceToHost, c->stream));
@@ -705,21 +1211,28 @@ int g4mi_field_download(g4mi_ctx *h, double *field, int *first) {
 
 int g4mi_beam_track(g4mi_ctx *h, const g4mi_step_params *st, double gammaref, double delz) {
     if (!h || !st) return G4MI_E_ARG;
+    if (h->c.precision == G4MI_FP32) return mix_split_call("g4mi_beam_track");
     return beam_track(&h->c, *st, gammaref, delz);
 }
 
 int g4mi_field_track(g4mi_ctx *h, const g4mi_step_params *st, double delz) {
     if (!h || !st) return G4MI_E_ARG;
+    if (h->c.precision == G4MI_FP32) return mix_split_call("g4mi_field_track");
     return field_track(&h->c, *st, delz);
 }
 
 int g4mi_slippage(g4mi_ctx *h, double slip) {
     if (!h) return G4MI_E_ARG;
+    if (h->c.precision == G4MI_FP32) return mix_split_call("g4mi_slippage");
     return slippage(&h->c, slip);
 }
 
 int g4mi_step(g4mi_ctx *h, const g4mi_step_params *st, double gammaref) {
     if (!h || !st) return G4MI_E_ARG;
+    if (h->c.precision == G4MI_FP32) {
+        int done = 0;
+        return mix_run(&h->c, st, 1, gammaref, nullptr, &done);
+    }
     G4_TRY(step(&h->c, *st, gammaref));
     return step_end_timing(&h->c);
 }
@@ -728,6 +1241,10 @@ int g4mi_run(g4mi_ctx *h, const g4mi_step_params *steps, int nsteps, double gamm
              g4mi_diag_out *records) {
     if (!h || (!steps && nsteps > 0) || nsteps < 0) return G4MI_E_ARG;
     int done = 0;
+    if (h->c.precision == G4MI_FP32) {
+        G4_TRY(mix_run(&h->c, steps, nsteps, gammaref, records, &done));
+        return done;
+    }
     for (int i = 0; i < nsteps; i++) {
         if (steps[i].marker & 8) break;
         G4_TRY(step(&h->c, steps[i], gammaref));
@@ -741,6 +1258,7 @@ int g4mi_run(g4mi_ctx *h, const g4mi_step_params *steps, int nsteps, double gamm
 
 int g4mi_diag(g4mi_ctx *h, g4mi_diag_out *out) {
     if (!h || !out) return G4MI_E_ARG;
+    if (h->c.precision == G4MI_FP32) return mix_diag(&h->c, out);
     return diag(&h->c, out);
 }
 
@@ -749,7 +1267,10 @@ int g4mi_sort_index(g4mi_ctx *h, double slen, int *atar, long long *moved) {
     Ctx *c = &h->c;
     if (!c->d_atar) G4_TRY(dalloc(c, &c->d_atar, (size_t)c->ntot));
     G4_HIP(hipMemsetAsync(c->d_count, 0, sizeof(unsigned long long), c->stream));
-    launch_sort_index(c->stream, c->ntot, c->d_th, 1. / slen, c->d_atar, c->d_count);
+    if (c->precision == G4MI_FP32)
+        launch_mix_sort_index(c->stream, mix_arrays(c), 1. / slen, c->d_atar, c->d_count);
+    else
+        launch_sort_index(c->stream, c->ntot, c->d_th, 1. / slen, c->d_atar, c->d_count);
     unsigned long long m = 0;
     if (atar && c->ntot)
         G4_HIP(hipMemcpyAsync(atar, c->d_atar, c->ntot * sizeof(int), hipMemcpyDeviceToHost, c->stream));
@@ -766,6 +1287,14 @@ int g4mi_set_schedule(g4mi_ctx *h, const char *name, int value) {
         c->adi_kernel = value;
         return G4MI_OK;
     }
+    if (std::string(name) == "adi_refine" && (value == 0 || value == 1)) {
+        c->m_refine = value != 0;
+        return G4MI_OK;
+    }
+    if (std::string(name) == "graphs" && (value == 0 || value == 1)) {
+        c->m_graphs = value != 0;
+        return G4MI_OK;
+    }
     set_error(std::string("g4mi_set_schedule: unknown entry or value: ") + name);
     return G4MI_E_ARG;
 }
@@ -773,8 +1302,20 @@ int g4mi_set_schedule(g4mi_ctx *h, const char *name, int value) {
 int g4mi_get_schedule(g4mi_ctx *h, char *buf, int len) {
     if (!h || !buf || len < 1) return G4MI_E_ARG;
     Ctx *c = &h->c;
-    char tmp[512];
+    char tmp[1024];
     static const char *names[4] = {"thomas", "fused", "fused-prefetch", "band"};
+    if (c->precision == G4MI_FP32) {
+        std::string bands = "[";
+        for (size_t k = 0; k < c->h_binfo.size(); k++)
+            bands += (k ? ", " : "") + std::to_string(c->h_binfo[k].m) + "/" + std::to_string(c->band_tmpl[k]);
+        bands += "]";
+        snprintf(tmp, sizeof(tmp),
+                 "{\"precision\": \"fp32\", \"adi\": \"band32\", \"adi_refine\": %d, "
+                 "\"graphs\": %d, \"bands_m_over_template\": \"%s\", \"graphs_captured\": %d}",
+                 c->m_refine ? 1 : 0, c->m_graphs ? 1 : 0, bands.c_str(), (int)c->m_graph.size());
+        snprintf(buf, (size_t)len, "%s", tmp);
+        return G4MI_OK;
+    }
     snprintf(tmp, sizeof(tmp),
              "{\"adi_kernel\": %d, \"adi\": \"%s\", \"adi_window\": %d, \"adi_band_m\": %d, "
              "\"adi_tuned_ms\": [%.4f, %.4f, %.4f, %.4f]}",

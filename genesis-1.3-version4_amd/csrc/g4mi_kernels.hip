@@ -291,8 +291,9 @@ __global__ void __launch_bounds__(256) k_sc_solve(
     const double *__restrict__ dr_arr, int ngr, int nz, int nphi, double ks, double gz2,
     double *__restrict__ ez, double *__restrict__ efield) {
     __shared__ double2 hist[4][kScMaxNg];
-    __shared__ double2 celm[kScMaxNg], gam[kScMaxNg];
+    __shared__ double2 celm[kScMaxNg], gam[kScMaxNg], rhs[kScMaxNg];
     __shared__ double vol[kScMaxNg], ldig[kScMaxNg + 1], rlog[kScMaxNg];
+    __shared__ double cupp[kScMaxNg], clow[kScMaxNg], cmid[kScMaxNg];
     const int is = blockIdx.x, t = threadIdx.x, w = t >> 6;
     const long long a = off[is], b = off[is + 1];
     const int npart = (int)(b - a);
@@ -301,16 +302,19 @@ __global__ void __launch_bounds__(256) k_sc_solve(
     if (npart < 1) return;
     const double dr = dr_arr[is], xc = cen[2 * is], yc = cen[2 * is + 1];
     const double pi = 2 * asin(1.0);
-    if (t == 0) {
-        vol[0] = pi * dr * dr;
-        rlog[0] = 0.5;
-        ldig[0] = 0;
-        for (int j = 1; j < ngr; j++) {
+    // radial grid tables, one entry per lane (constructLaplaceOperator :176-190)
+    for (int j = t; j <= ngr; j += 256) {
+        if (j == ngr) {
+            ldig[ngr] = 0;
+        } else if (j == 0) {
+            vol[0] = pi * dr * dr;
+            rlog[0] = 0.5;
+            ldig[0] = 0;
+        } else {
             vol[j] = pi * dr * dr * (2 * j + 1);
             ldig[j] = 2 * pi * j;
             rlog[j] = log((double)(j + 1) / (double)j);
         }
-        ldig[ngr] = 0;
     }
     const double coef = -gz2 / ks / ks;
     const double econst = kVacimp / kEev * cur[is] / ks / (double)npart;
@@ -332,28 +336,27 @@ __global__ void __launch_bounds__(256) k_sc_solve(
                 unsafeAtomicAdd(&hist[w][bin].y, v.y);
             }
             __syncthreads();
-            if (t == 0) {
-                // lmid for this m (:221-223), coefficients (:240-245), tridiag
-                auto lmid = [&](int i) {
-                    double v = -ldig[i] - ldig[i + 1] - 2. * pi * m * m * rlog[i];
-                    if (i == ngr - 1) v -= 2 * pi * ngr;
-                    return v;
-                };
-                auto src = [&](int i) {
-                    double2 cs = cadd(cadd(cadd(hist[0][i], hist[1][i]), hist[2][i]), hist[3][i]);
-                    return cmul(cs, make_double2(0, econst / l / vol[i]));
-                };
-                double2 bet;
-                double cmid0 = 1 + coef * lmid(0) / l / l / vol[0];
-                bet = make_double2(cmid0, 0);
-                celm[0] = cdiv(src(0), bet);
+            // operator coefficients (:240-245) and source of every radial cell,
+            // one cell per lane; the sequential recurrence then only reads them
+            for (int i = t; i < ngr; i += 256) {
+                double lm = -ldig[i] - ldig[i + 1] - 2. * pi * m * m * rlog[i];
+                if (i == ngr - 1) lm -= 2 * pi * ngr;
+                cmid[i] = 1 + coef * lm / l / l / vol[i];
+                if (i > 0) {
+                    cupp[i] = coef * ldig[i] / l / l / vol[i - 1];
+                    clow[i] = coef * ldig[i] / l / l / vol[i];
+                }
+                const double2 cs = cadd(cadd(cadd(hist[0][i], hist[1][i]), hist[2][i]), hist[3][i]);
+                rhs[i] = cmul(cs, make_double2(0, econst / l / vol[i]));
+            }
+            __syncthreads();
+            if (t == 0) {  // tridiag (:260-272)
+                double2 bet = make_double2(cmid[0], 0);
+                celm[0] = cdiv(rhs[0], bet);
                 for (int i = 1; i < ngr; i++) {
-                    double cupp = coef * ldig[i] / l / l / vol[i - 1];
-                    double clow = coef * ldig[i] / l / l / vol[i];
-                    double cmid = 1 + coef * lmid(i) / l / l / vol[i];
-                    gam[i] = cdiv(make_double2(cupp, 0), bet);
-                    bet = csub(make_double2(cmid, 0), cmul(make_double2(clow, 0), gam[i]));
-                    celm[i] = cdiv(csub(src(i), cmul(make_double2(clow, 0), celm[i - 1])), bet);
+                    gam[i] = cdiv(make_double2(cupp[i], 0), bet);
+                    bet = csub(make_double2(cmid[i], 0), cmul(make_double2(clow[i], 0), gam[i]));
+                    celm[i] = cdiv(csub(rhs[i], cmul(make_double2(clow[i], 0), celm[i - 1])), bet);
                 }
                 for (int i = ngr - 2; i > -1; i--) celm[i] = csub(celm[i], cmul(gam[i + 1], celm[i + 1]));
                 if (l == 1) efield[is] = hypot(celm[0].x, celm[0].y);
@@ -1035,8 +1038,16 @@ __global__ void __launch_bounds__(256) k_adi_band(const double2 *__restrict__ fi
     double2 *BX = G + RE * PG;        // [TS][W] band rows of x (edge tiles)
     double2 *BY = BX + TS * W;        // [TS][W] band rows of y (edge tiles)
     const int t = threadIdx.x;
-    const int ii = blockIdx.y;        // beam slice
-    const int tile = blockIdx.x;
+    // XCD-aware order: workgroups are handed to the 8 XCDs round-robin, so
+    // block b runs on XCD b % 8.  Blocks of a group of 8 slices are ordered
+    // tile-major (b = group*8*ntiles + tile*8 + slice%8): all tiles of one
+    // slice land on the same XCD, close in time, and share their halo rows
+    // through that XCD's L2.
+    const int ntiles = ntx * ntx;
+    const int b = blockIdx.x;
+    const int ii = (b / (8 * ntiles)) * 8 + (b & 7);   // beam slice
+    const int tile = (b % (8 * ntiles)) >> 3;
+    if (ii >= ns) return;  // uniform per block; padding of the last group
     const int tx0 = (tile % ntx) * TS, ty0 = (tile / ntx) * TS;
     const long long fs = (long long)((ii + first) % ns);
     const double2 *u = fin + fs * ng * ng;
@@ -1182,7 +1193,7 @@ __global__ void __launch_bounds__(256) k_adi_band(const double2 *__restrict__ fi
     double *red = (double *)sm;
     block_sum_waves<NT, 8>(dg, red);
     if (t == 0) {
-        double *o = tilediag + ((long long)fs * gridDim.x + tile) * 8;
+        double *o = tilediag + ((long long)fs * ntiles + tile) * 8;
         for (int k = 0; k < 8; k++) o[k] = dg[k];
     }
 }
@@ -1397,7 +1408,8 @@ static void launch_adi_band_t(hipStream_t s, int ns, const double2 *fin, double2
         attr = true;
     }
     const int ntx = (ng + TS - 1) / TS;
-    hipLaunchKernelGGL(k_adi_band<M>, dim3(ntx * ntx, ns), dim3(256), lds, s, fin, fout, src, box,
+    const unsigned nblocks = (unsigned)(((ns + 7) / 8) * 8) * (unsigned)(ntx * ntx);
+    hipLaunchKernelGGL(k_adi_band<M>, dim3(nblocks), dim3(256), lds, s, fin, fout, src, box,
                        band, a, ng, first, ns, ntx, klo, khi, tilediag);
     hipLaunchKernelGGL(k_adi_tile_diag, dim3((ns + 63) / 64), dim3(64), 0, s, tilediag,
                        ntx * ntx, ns, diag);

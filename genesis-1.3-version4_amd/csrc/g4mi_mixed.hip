@@ -1,0 +1,1285 @@
+// Mixed-precision kernels of the undulator step: FP32 particles and field,
+// FP64 slice phase base, FP64 residual correction in the ADI and FP64
+// accumulation for every diagnostic sum.
+//
+// Each kernel restates the same reference loop as its FP64 counterpart in
+// g4mi_kernels.hip (cited there); the differences are the storage type and
+// the offset form of gamma and theta:
+//   gamma = gref + dg,  theta = base[slice] + dth.
+// In the longitudinal equation (BeamSolver::ODE, src/Core/BeamSolver.cpp:
+// 144-163) the phase rate xks (1 - 1/beta_par) + xku is split into the
+// reference particle's rate kappa (applied to the FP64 base) and the
+// particle's departure
+//   dth' = xks (g(eps_ref) - g(eps)),  g(e) = 1/sqrt(1-e) - 1,  eps = btper0/gamma^2,
+// evaluated as xks g'(eps_bar) (eps_ref - eps) with eps_ref - eps formed
+// from differences that are small in their own right (dg, px^2 + py^2,
+// aw^2 (awloc^2 - 1), the field term): no catastrophic cancellation in FP32.
+//
+// Per-step parameters come from the device step table steps[st->istep], so
+// a captured graph replays any step.
+#include <type_traits>
+
+#include "g4mi_mixed.h"
+
+namespace g4mi {
+
+namespace {
+
+constexpr double kTwoPi = 6.283185307179586476925286766559;
+constexpr double kPi = 3.14159265358979323846264338328;
+
+__device__ __forceinline__ float2 f2(float a, float b) { return make_float2(a, b); }
+__device__ __forceinline__ float2 fadd(float2 a, float2 b) { return f2(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ float2 fsub(float2 a, float2 b) { return f2(a.x - b.x, a.y - b.y); }
+__device__ __forceinline__ float2 fscale(float2 a, float s) { return f2(a.x * s, a.y * s); }
+__device__ __forceinline__ float2 fmul(float2 a, float2 b) {
+    return f2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+__device__ __forceinline__ float2 ffma(float2 a, float2 b, float2 acc) {
+    acc.x = fmaf(a.x, b.x, fmaf(-a.y, b.y, acc.x));
+    acc.y = fmaf(a.x, b.y, fmaf(a.y, b.x, acc.y));
+    return acc;
+}
+__device__ __forceinline__ double2 d2(double a, double b) { return make_double2(a, b); }
+__device__ __forceinline__ double2 tod(float2 a) { return d2((double)a.x, (double)a.y); }
+__device__ __forceinline__ float2 tof(double2 a) { return f2((float)a.x, (float)a.y); }
+__device__ __forceinline__ float2 tof(float2 a) { return a; }
+
+// phase in [0, 2 pi) of an FP64 angle, as FP32
+__device__ __forceinline__ float wrap_phase(double p) {
+    return (float)(p - kTwoPi * floor(p / kTwoPi));
+}
+
+// Deterministic block sum of K doubles over NT threads (fixed tree order);
+// lds holds K*NT doubles.
+template <int NT, int K>
+__device__ __forceinline__ void bsum(double (&v)[K], double *lds) {
+    const int t = threadIdx.x;
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < K; k++) lds[k * NT + t] = v[k];
+    __syncthreads();
+    for (int s = NT / 2; s > 0; s >>= 1) {
+        if (t < s) {
+#pragma unroll
+            for (int k = 0; k < K; k++) lds[k * NT + t] += lds[k * NT + t + s];
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int k = 0; k < K; k++) v[k] = lds[k * NT];
+    __syncthreads();
+}
+
+// transverse half step (TrackBeam::track, src/Core/TrackBeam.cpp:8-116), FP32.
+// The focusing angle per step is small (omg ~ 1e-3), so cos(omg) is 1 to
+// within a few FP32 ulp; the map is applied as x + (a1 - 1) x + ... with
+// a1 - 1 = -2 sin^2(omg/2) (cosh: +2 sinh^2(omg/2)), which keeps the
+// focusing term's full relative precision.
+__device__ __forceinline__ void line_map32(int kind, float delz, float q, float &x, float &px,
+                                           float gz, float d) {
+    if (kind == 0) {
+        x += px * delz / gz;
+    } else if (kind > 0) {
+        const float foc = sqrtf(q / gz), omg = foc * delz;
+        float s, c, sh, ch;
+        __sincosf(omg, &s, &c);
+        __sincosf(0.5f * omg, &sh, &ch);
+        const float a1m = -2.f * sh * sh, a2 = s / foc, a3 = -a2 * foc * foc;
+        const float xt = x - d;
+        x = xt + (a1m * xt + a2 * px / gz) + d;
+        px = px + (a3 * xt * gz + a1m * px);
+    } else {
+        const float foc = sqrtf(-q / gz), omg = foc * delz;
+        const float sh = sinhf(0.5f * omg);
+        const float a1m = 2.f * sh * sh, a2 = sinhf(omg) / foc, a3 = a2 * foc * foc;
+        const float xt = x - d;
+        x = xt + (a1m * xt + a2 * px / gz) + d;
+        px = px + (a3 * xt * gz + a1m * px);
+    }
+}
+
+__device__ __forceinline__ void transverse32(const TransParams &p, float gam, float &x, float &y,
+                                             float &px, float &py) {
+    if (p.corrector) {
+        px += (float)p.ccx;
+        py += (float)p.ccy;
+    }
+    if (p.chicane) {
+        const float gz = sqrtf(gam * gam - 1.f - px * px - py * py);
+        float t = x;
+        x = (float)p.m00 * t + (float)p.m01 * px / gz;
+        px = (float)p.m10 * t * gz + (float)p.m11 * px;
+        t = y;
+        y = (float)p.m22 * t + (float)p.m23 * py / gz;
+        py = (float)p.m32 * t * gz + (float)p.m33 * py;
+    }
+    const float aw = (float)p.aw;
+    const float gz = sqrtf(gam * gam - 1.f - aw * aw - px * px - py * py);
+    line_map32(p.kindx, (float)p.delz, (float)p.qx, x, px, gz, (float)p.xoff);
+    line_map32(p.kindy, (float)p.delz, (float)p.qy, y, py, gz, (float)p.yoff);
+}
+
+// ---------------------------------------------------------------------------
+// front: first transverse half step + x/y moments (Beam::getSize,
+// src/Core/Beam.cpp:42-67) + the short-range solver's centroid and largest
+// radius (EFieldSolver::analyseBeam, EFieldSolver.cpp:137-157).  One
+// workgroup per slice.
+// ---------------------------------------------------------------------------
+constexpr int kFrontNT = 256;
+__global__ void __launch_bounds__(kFrontNT) k_mix_front(MixArrays a, int moments, int sc) {
+    __shared__ double lds[4 * kFrontNT];
+    const MixStep &S = a.steps[a.st->istep];
+    const int is = blockIdx.x, t = threadIdx.x;
+    const long long b0 = a.off[is], b1 = a.off[is + 1];
+    const float gref = (float)a.gref;
+    __shared__ int bb[4][kFrontNT];
+    double v[4] = {0, 0, 0, 0};
+    const float gridmax = (float)S.p.gridmax, inv_dgrid = 1.f / (float)S.p.dgrid;
+    const int ng = a.ng;
+    int bx0 = 1 << 30, by0 = 1 << 30, bx1 = -1, by1 = -1;
+    for (long long i = b0 + t; i < b1; i += kFrontNT) {
+        float x = a.x[i], y = a.y[i], px = a.px[i], py = a.py[i];
+        const float gam = gref + a.dg[i];
+        transverse32(S.t1, gam, x, y, px, py);
+        a.x[i] = x;
+        a.y[i] = y;
+        a.px[i] = px;
+        a.py[i] = py;
+        v[0] += x;
+        v[1] += (double)x * x;
+        v[2] += y;
+        v[3] += (double)y * y;
+        if ((x > -gridmax) && (x < gridmax) && (y > -gridmax) && (y < gridmax)) {
+            const int ix = min((int)floorf((x + gridmax) * inv_dgrid), ng - 2);
+            const int iy = min((int)floorf((y + gridmax) * inv_dgrid), ng - 2);
+            bx0 = min(bx0, ix);
+            bx1 = max(bx1, ix + 1);
+            by0 = min(by0, iy);
+            by1 = max(by1, iy + 1);
+        }
+    }
+    // footprint of the slice for the main kernel's deposit tile
+    bb[0][t] = bx0;
+    bb[1][t] = by0;
+    bb[2][t] = -bx1;
+    bb[3][t] = -by1;
+    __syncthreads();
+    for (int s = kFrontNT / 2; s > 0; s >>= 1) {
+        if (t < s)
+            for (int k = 0; k < 4; k++) bb[k][t] = min(bb[k][t], bb[k][t + s]);
+        __syncthreads();
+    }
+    if (t == 0) a.pbox[is] = make_int4(bb[0][0], bb[1][0], -bb[2][0], -bb[3][0]);
+    if (!moments) return;  // uniform
+    bsum<kFrontNT, 4>(v, lds);
+    const long long cnt = b1 - b0;
+    double xc = 0, yc = 0;
+    if (cnt > 0) {
+        xc = v[0] / (double)cnt;
+        yc = v[2] / (double)cnt;
+    }
+    if (t == 0) {
+        double sz;
+        if (a.cur[is] <= 0 || cnt == 0) {
+            sz = 1;
+        } else {
+            const double c = (double)cnt;
+            const double x1 = v[0] / c, x2 = v[1] / c, y1 = v[2] / c, y2 = v[3] / c;
+            sz = sqrt(fabs(x2 - x1 * x1)) * sqrt(fabs(y2 - y1 * y1));
+        }
+        if (sz <= 0) sz = 1;
+        a.mom_fsize[is] = sz;
+        a.cen[2 * is] = xc;
+        a.cen[2 * is + 1] = yc;
+    }
+    if (!sc) return;  // uniform
+    // largest radius from the centroid; each lane rereads what it wrote
+    double m = 0;
+    for (long long i = b0 + t; i < b1; i += kFrontNT) {
+        const double tx = (double)a.x[i] - xc, ty = (double)a.y[i] - yc;
+        m = fmax(m, tx * tx + ty * ty);
+    }
+    lds[t] = m;
+    __syncthreads();
+    for (int s = kFrontNT / 2; s > 0; s >>= 1) {
+        if (t < s) lds[t] = fmax(lds[t], lds[t + s]);
+        __syncthreads();
+    }
+    if (t == 0) a.rbound[is] = sqrt(lds[0]);
+}
+
+// the growing radial grid of the short-range solver, in slice order
+// (EFieldSolver.cpp:159-165); the same ballot walk as the FP64 form
+__global__ void __launch_bounds__(64) k_mix_scan(MixArrays a) {
+    const int lane = threadIdx.x;
+    double rmax = a.rmax[0];
+    const int ngr = a.sc_ng;
+    for (int base = 0; base < a.ns; base += 64) {
+        const int i = base + lane;
+        const bool valid = i < a.ns;
+        const bool nonempty = valid && (a.off[i + 1] - a.off[i] >= 1);
+        const double b = nonempty ? a.rbound[i] : -1.0;
+        double mine = rmax;
+        int start = 0;
+        for (;;) {
+            const unsigned long long m = __ballot(lane >= start && nonempty && b > rmax);
+            if (m == 0) {
+                if (lane >= start) mine = rmax;
+                break;
+            }
+            const int jl = __ffsll((long long)m) - 1;
+            if (lane >= start && lane < jl) mine = rmax;
+            rmax = __shfl(b, jl) * 1.5;
+            if (lane == jl) mine = rmax;
+            start = jl + 1;
+        }
+        if (valid) a.dr[i] = mine / (double)(ngr - 1);
+    }
+    if (lane == 0) a.rmax[0] = rmax;
+}
+
+// ---------------------------------------------------------------------------
+// main: one workgroup per beam slice.
+//   long-range space charge of the slice (EFieldSolver::longRange :103-116)
+//   short-range space charge: radial histograms and the radial tridiagonal
+//     per mode (EFieldSolver::shortRange :197-258)
+//   per particle: field gather (BeamSolver.cpp:66-79), RK4 (:89-141) in
+//     offset form, R56 (TrackBeam::applyR56), second transverse half step,
+//     beam sums (DiagBeam::getValues), footprint box
+//   deposit of the slice's source (FieldSolverADI::advance :13-51) into an
+//     LDS tile, written to the source plane inside the box
+// One dynamic LDS region is reused: histograms + mode solutions during the
+// space-charge part and the particle pass, reduction scratch after it, the
+// deposit tile last.
+// ---------------------------------------------------------------------------
+constexpr int kMainNT = 384;
+constexpr int kScMax = 256;
+constexpr int kMixDyn = 8192;  // float2 cells of the dynamic region (64 KiB)
+
+struct OdeMix {
+    float xks, eps_ref, bref, gref, ztemp1, ez;
+    float2 rpart;
+    float btpar_m1;  // btpar - bref = px^2 + py^2 + aw^2 (awloc^2 - 1)
+    float kgg, kpp;
+    __device__ __forceinline__ void eval(float dgam, float phase) {
+        float s, c;
+        __sincosf(phase, &s, &c);
+        // rpart * (cos, -sin)
+        const float cre = rpart.x * c + rpart.y * s, cim = rpart.y * c - rpart.x * s;
+        const float gam = gref + dgam;
+        const float ig2 = 1.f / (gam * gam);
+        const float dbt = btpar_m1 + ztemp1 * cre;   // btper0 - bref
+        const float eps = (bref + dbt) * ig2;
+        const float beta = sqrtf(1.f - eps);
+        // eps_ref - eps = bref (gam^2 - gref^2) / (gref^2 gam^2) - (btper0 - bref) / gam^2
+        const float deps = (eps_ref * dgam * (2.f * gref + dgam) - dbt) * ig2;
+        kpp += xks * (0.5f + 0.375f * (eps + eps_ref)) * deps;
+        kgg += cim / beta / gam - ez;
+    }
+};
+
+// a + b rounded stochastically to one of the two neighbouring FP32 values,
+// with probability proportional to the distance (unbiased: E[result] = a + b
+// exactly).  The residual of round-to-nearest comes from TwoSum; u is a
+// uniform 24-bit draw.  Accumulating a small per-step increment into an
+// O(1) offset with round-to-nearest repeats the same rounding error every
+// step; stochastic rounding makes it a zero-mean random walk instead.
+__device__ __forceinline__ float add_sr(float a, float b, unsigned u) {
+    const float s = a + b;
+    const float bb = s - a;
+    const float err = (a - (s - bb)) + (b - bb);
+    if (err == 0.f) return s;
+    const float nb = __int_as_float(__float_as_int(s) + ((err > 0.f) == (s > 0.f) ? 1 : -1));
+    const float gap = fabsf(nb - s);
+    return ((float)(u >> 8) * 5.9604645e-08f) * gap < fabsf(err) ? nb : s;
+}
+
+// 32-bit mix of a particle index and a step counter (xorshift-multiply)
+__device__ __forceinline__ unsigned mix_hash(unsigned long long i, unsigned step, unsigned salt) {
+    unsigned x = (unsigned)i * 0x9E3779B9u ^ (unsigned)(i >> 32) * 0x85EBCA6Bu ^ step * 0xC2B2AE35u ^ salt;
+    x ^= x >> 16;
+    x *= 0x7FEB352Du;
+    x ^= x >> 15;
+    x *= 0x846CA68Bu;
+    x ^= x >> 16;
+    return x;
+}
+
+__device__ __forceinline__ int sc_bin(float radi, float dr, int ngr) {
+    int bin = (int)floorf(radi / dr);
+    return (bin >= 0 && bin < ngr) ? bin : (bin >= ngr ? ngr - 1 : 0);
+}
+
+__device__ __forceinline__ float2 cw_pow(float2 cw, int m) {
+    // cw^m for a unit complex cw: conj(cw)^|m| for m < 0
+    float2 p = f2(1, 0);
+    const int um = m < 0 ? -m : m;
+    const float2 b = m < 0 ? f2(cw.x, -cw.y) : cw;
+    for (int q = 0; q < um; q++) p = fmul(p, b);
+    return p;
+}
+
+// Block sum of K doubles: butterfly within each wave, then the per-wave
+// partials in wave order (deterministic).  lds holds K * NT/64 doubles.
+template <int NT, int K>
+__device__ __forceinline__ void bsum_w(double (&v)[K], double *lds) {
+    constexpr int NW = NT / 64;
+    const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+#pragma unroll
+    for (int k = 0; k < K; k++)
+        for (int o = 32; o > 0; o >>= 1) v[k] += __shfl_xor(v[k], o);
+    __syncthreads();
+    if (lane == 0)
+#pragma unroll
+        for (int k = 0; k < K; k++) lds[k * NW + wv] = v[k];
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < K; k++) {
+        double acc = 0;
+        for (int q = 0; q < NW; q++) acc += lds[k * NW + q];
+        v[k] = acc;
+    }
+    __syncthreads();
+}
+
+// Block minimum of K ints, same structure as bsum_w; lds holds K * NT/64 ints.
+template <int NT, int K>
+__device__ __forceinline__ void bmin_w(int (&v)[K], int *lds) {
+    constexpr int NW = NT / 64;
+    const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+#pragma unroll
+    for (int k = 0; k < K; k++)
+        for (int o = 32; o > 0; o >>= 1) v[k] = min(v[k], __shfl_xor(v[k], o));
+    __syncthreads();
+    if (lane == 0)
+#pragma unroll
+        for (int k = 0; k < K; k++) lds[k * NW + wv] = v[k];
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < K; k++) {
+        int acc = lds[k * NW];
+        for (int q = 1; q < NW; q++) acc = min(acc, lds[k * NW + q]);
+        v[k] = acc;
+    }
+    __syncthreads();
+}
+
+// LDS accumulation is FP64 (ds_add_f64): on gfx950 FP32 LDS atomics
+// (ds_add_f32) issue far slower than FP64 ones, and the sums gain precision.
+constexpr int kCelmCells = 2048;                          // space-charge mode solutions (float2)
+constexpr int kTileCells = (kMixDyn - kCelmCells) / 2;    // double2 cells: histograms, then the deposit tile
+
+__global__ void __launch_bounds__(kMainNT)
+k_mix_main(MixArrays a, int sc_on) {
+    extern __shared__ float2 dyn[];
+    __shared__ int bred[5 * (kMainNT / 64)];
+    __shared__ double wred[8 * (kMainNT / 64)];
+    __shared__ float gsol[kScMax], scupp[kScMax], sclow[kScMax], scmid[kScMax];
+    __shared__ float2 scrhs[kScMax];
+    const MixStep &S = a.steps[a.st->istep];
+    const int is = blockIdx.x, t = threadIdx.x, w = t >> 6;
+    const long long b0 = a.off[is], b1 = a.off[is + 1];
+    const int npart = (int)(b1 - b0);
+    const int ns = a.ns, ng = a.ng;
+    const int first = a.st->first;
+    const float2 *field = a.field[a.st->cur] + (long long)((is + first) % ns) * ng * ng;
+    float2 *celm = dyn;                                   // [nmodes][ngr]
+    double2 *work = (double2 *)(dyn + kCelmCells);        // histograms / deposit tile
+    double2(*hist)[kScMax] = (double2(*)[kScMax])work;    // [kMainNT/64][kScMax]
+
+    // ---- long-range space charge of this slice
+    double esc = 0;
+    if (S.longrange) {
+        double e = 0;
+        for (int j = t; j < ns; j += kMainNT) {
+            const double sz = a.mom_fsize[j];
+            const double ds = (double)(j - is) * S.lsc_ds;
+            const double coef = 1 - sqrt(ds * ds / (ds * ds + sz));
+            const double sgn = (j > is) ? -1. : ((j < is) ? 1. : 0.);
+            e += sgn * coef * S.lsc_scl * a.cur[j] / sz;
+        }
+        double v[1] = {e};
+        bsum_w<kMainNT, 1>(v, wred);
+        esc = v[0];
+    }
+    const float ez_long = (float)(-esc / kEev);
+
+    // ---- short-range space charge
+    const bool sc = sc_on && S.use_sc && npart > 0;
+    const double basep = a.base[is];
+    const PushParams &P = S.p;
+    const float ph0 = wrap_phase(basep);
+    const float phh = wrap_phase(basep + S.kappa * 0.5 * P.delz);
+    const float ph1 = wrap_phase(basep + S.kappa * P.delz);
+    const int ngr = a.sc_ng;
+    float xc = 0, yc = 0, dr = 1;
+    if (sc) {
+        xc = (float)a.cen[2 * is];
+        yc = (float)a.cen[2 * is + 1];
+        dr = (float)a.dr[is];
+        const double econst = kVacimp / kEev * a.cur[is] / S.sc_ks / (double)npart;
+        const double coef = -S.sc_gz2 / S.sc_ks / S.sc_ks;
+        const double drd = a.dr[is];
+        int mode = 0;
+        for (int m = -a.sc_nphi; m <= a.sc_nphi; m++) {
+            for (int l = 1; l <= a.sc_nz; l++, mode++) {
+                __syncthreads();
+                for (int k = t; k < (kMainNT / 64) * kScMax; k += kMainNT)
+                    hist[k / kScMax][k % kScMax] = make_double2(0, 0);
+                __syncthreads();
+                float nx = 0, ny = 0, nt = 0;
+                if (b0 + t < b1) {
+                    nx = a.x[b0 + t];
+                    ny = a.y[b0 + t];
+                    nt = a.dth[b0 + t];
+                }
+                for (long long i = b0 + t; i < b1; i += kMainNT) {
+                    const float tx = nx - xc, ty = ny - yc, th = nt;
+                    if (i + kMainNT < b1) {
+                        nx = a.x[i + kMainNT];
+                        ny = a.y[i + kMainNT];
+                        nt = a.dth[i + kMainNT];
+                    }
+                    const float radi = sqrtf(tx * tx + ty * ty);
+                    const int bin = sc_bin(radi, dr, ngr);
+                    float s, c;
+                    __sincosf((float)l * (ph0 + th), &s, &c);
+                    const float2 v = fmul(cw_pow(f2(tx / radi, ty / radi), -m), f2(c, -s));
+                    unsafeAtomicAdd(&hist[w][bin].x, (double)v.x);
+                    unsafeAtomicAdd(&hist[w][bin].y, (double)v.y);
+                }
+                __syncthreads();
+                // constructLaplaceOperator (:176-190): coefficients and source of
+                // every radial cell, one cell per lane; tridiag (:260-272) on
+                // lane 0.  The operator is real, the right-hand side complex.
+                for (int i = t; i < ngr; i += kMainNT) {
+                    const double voli = kPi * drd * drd * (i == 0 ? 1.0 : (double)(2 * i + 1));
+                    const double ldi = i > 0 ? 2 * kPi * i : 0.0;
+                    const double ldn = (i + 1 < ngr) ? 2 * kPi * (i + 1) : 0.0;
+                    const double rl = i == 0 ? 0.5 : log((double)(i + 1) / (double)i);
+                    double lm = -ldi - ldn - 2. * kPi * m * m * rl;
+                    if (i == ngr - 1) lm -= 2 * kPi * ngr;
+                    scmid[i] = (float)(1 + coef * lm / l / l / voli);
+                    if (i > 0) {
+                        const double volm = kPi * drd * drd * (i == 1 ? 1.0 : (double)(2 * i - 1));
+                        scupp[i] = (float)(coef * ldi / l / l / volm);
+                        sclow[i] = (float)(coef * ldi / l / l / voli);
+                    }
+                    double csx = 0, csy = 0;
+                    for (int q = 0; q < kMainNT / 64; q++) {
+                        csx += hist[q][i].x;
+                        csy += hist[q][i].y;
+                    }
+                    const double e = econst / l / voli;
+                    scrhs[i] = f2((float)(-csy * e), (float)(csx * e));
+                }
+                __syncthreads();
+                if (t == 0) {
+                    float2 *ce = celm + mode * ngr;
+                    float bet = scmid[0];
+                    ce[0] = fscale(scrhs[0], 1.f / bet);
+                    for (int i = 1; i < ngr; i++) {
+                        gsol[i] = scupp[i] / bet;
+                        bet = scmid[i] - sclow[i] * gsol[i];
+                        ce[i] = fscale(fsub(scrhs[i], fscale(ce[i - 1], sclow[i])), 1.f / bet);
+                    }
+                    for (int i = ngr - 2; i > -1; i--) ce[i] = fsub(ce[i], fscale(ce[i + 1], gsol[i + 1]));
+                    if (l == 1) a.efield[is] = hypot((double)ce[0].x, (double)ce[0].y);
+                }
+            }
+        }
+    }
+
+    // ---- deposit tile: the slice's footprint after the first transverse half
+    // step (front kernel) padded by 2 cells.  The second half step moves a
+    // particle by |px/gamma_z| dz/2, far below a cell; a particle that does
+    // leave the tile sets the overflow flag and the slice then takes the
+    // exact two-pass deposit below.
+    const DepositParams &D = S.d;
+    const bool dep = D.enabled != 0;
+    const int4 pb = a.pbox[is];
+    int tx0 = 0, ty0 = 0, tw = 0, th = 0;
+    bool fused = false;
+    if (dep && pb.z >= pb.x) {
+        tx0 = max(0, pb.x - 2);
+        ty0 = max(0, pb.y - 2);
+        tw = min(ng - 1, pb.z + 2) - tx0 + 1;
+        th = min(ng - 1, pb.w + 2) - ty0 + 1;
+        fused = tw * th <= kTileCells;
+    }
+    float scl = 0;
+    if (dep) {
+        const double scl0 = D.fcoup * kVacimp * a.cur[is] * D.xks * D.delz;
+        scl = npart > 0 ? (float)(scl0 / (4 * kEev * (double)npart * D.dgrid * D.dgrid)) : 0.f;
+    }
+    __syncthreads();  // histograms dead, tile takes their place
+    if (fused)
+        for (int k = t; k < tw * th; k += kMainNT) work[k] = make_double2(0, 0);
+    __syncthreads();
+
+    // ---- particle pass
+    const float gref = (float)a.gref;
+    const float xks = (float)P.xks;
+    const float bref = (float)(1 + P.aw * P.aw);
+    const float aw2 = (float)(P.aw * P.aw);
+    const float gridmax = (float)P.gridmax, dgrid = (float)P.dgrid;
+    const float rtmp = (float)P.rtmp;
+    const float delz = (float)P.delz;
+    const float dgam0 = (float)(a.gref - S.gamma0);  // gamma - gamma0 = dg + dgam0
+    const float r56 = (float)S.r56;
+    const float autoph = (float)P.autophase;
+    const float dax = (float)D.ax, day = (float)D.ay, dkx = (float)D.kx, dky = (float)D.ky;
+    const float dgx = (float)D.gradx, dgy = (float)D.grady;
+    // per-lane beam sums of at most a few dozen particles in FP32, energy as
+    // the offset dg; widened to FP64 for the block sum
+    float fv[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    int bx0 = 1 << 30, by0 = 1 << 30, bx1 = -1, by1 = -1, ovf = 0;
+    const unsigned stepno = a.st->seq;
+    const float inv_dgrid = 1.f / dgrid;
+    // the next particle's coordinates are loaded before this one's stores,
+    // so their latency overlaps the RK4 of the current particle
+    float n_th = 0, n_g = 0, n_x = 0, n_y = 0, n_px = 0, n_py = 0;
+    if (b0 + t < b1) {
+        const long long i = b0 + t;
+        n_th = a.dth[i];
+        n_g = a.dg[i];
+        n_x = a.x[i];
+        n_y = a.y[i];
+        n_px = a.px[i];
+        n_py = a.py[i];
+    }
+    for (long long i = b0 + t; i < b1; i += kMainNT) {
+        const float th_in = n_th;
+        const float g0 = n_g, t0 = th_in + autoph;
+        float x = n_x, y = n_y, px = n_px, py = n_py;
+        if (i + kMainNT < b1) {
+            const long long j = i + kMainNT;
+            n_th = a.dth[j];
+            n_g = a.dg[j];
+            n_x = a.x[j];
+            n_y = a.y[j];
+            n_px = a.px[j];
+            n_py = a.py[j];
+        }
+        const float dx = x - (float)P.ax, dy = y - (float)P.ay;
+        const float awm1 = 0.5f * ((float)P.kx * dx * dx + (float)P.ky * dy * dy) +
+                           (float)P.gradx * dx + (float)P.grady * dy;  // awloc - 1
+        OdeMix o;
+        o.xks = xks;
+        o.eps_ref = (float)S.eps_ref;
+        o.bref = bref;
+        o.gref = gref;
+        o.ztemp1 = -2.f / xks;
+        o.btpar_m1 = px * px + py * py + aw2 * awm1 * (awm1 + 2.f);
+        float ez = ez_long;
+        if (sc) {
+            const float tx = x - xc, ty = y - yc;
+            const float radi = sqrtf(tx * tx + ty * ty);
+            const float2 cw = f2(tx / radi, ty / radi);
+            const int bin = sc_bin(radi, dr, ngr);
+            int mode = 0;
+            for (int m = -a.sc_nphi; m <= a.sc_nphi; m++) {
+                for (int l = 1; l <= a.sc_nz; l++, mode++) {
+                    float s, c;
+                    __sincosf((float)l * (ph0 + th_in), &s, &c);
+                    const float2 ct = fmul(cw_pow(cw, m), f2(c, s));
+                    ez += 2.f * fmul(ct, celm[mode * ngr + bin]).x;
+                }
+            }
+        }
+        o.ez = ez;
+        o.rpart = f2(0, 0);
+        if ((x > -gridmax) && (x < gridmax) && (y > -gridmax) && (y < gridmax)) {
+            float wx = (x + gridmax) * inv_dgrid, wy = (y + gridmax) * inv_dgrid;
+            const int ix = min((int)floorf(wx), ng - 2), iy = min((int)floorf(wy), ng - 2);
+            wx = 1.f + floorf(wx) - wx;
+            wy = 1.f + floorf(wy) - wy;
+            const int idx = ix + iy * ng;
+            const float2 q0 = field[idx], q1 = field[idx + 1], q2 = field[idx + ng],
+                         q3 = field[idx + ng + 1];
+            float2 cp = fscale(q0, wx * wy);
+            cp = fadd(cp, fscale(q1, (1 - wx) * wy));
+            cp = fadd(cp, fscale(q2, wx * (1 - wy)));
+            cp = fadd(cp, fscale(q3, (1 - wx) * (1 - wy)));
+            const float r = rtmp * (1.f + awm1);
+            o.rpart = f2(r * cp.x, -r * cp.y);
+        }
+        // RungeKutta (BeamSolver.cpp:93-141): the reference updates the state
+        // in place between evaluations; here the increments (eg, et) follow
+        // that same sequence from zero, the stages are evaluated at
+        // (g0 + eg, t0 + et) on the base phases base, base + kappa dz/2
+        // (twice) and base + kappa dz, and the step's total increment is
+        // added to the stored offsets once, with stochastic rounding.
+        float k3gg, k3pp, eg = 0.f, et = 0.f;
+        o.kgg = 0;
+        o.kpp = 0;
+        o.eval(g0, ph0 + t0);
+        float stpz = 0.5f * delz;
+        eg += stpz * o.kgg;
+        et += stpz * o.kpp;
+        k3gg = o.kgg;
+        k3pp = o.kpp;
+        o.kgg = 0;
+        o.kpp = 0;
+        o.eval(g0 + eg, phh + (t0 + et));
+        eg += stpz * (o.kgg - k3gg);
+        et += stpz * (o.kpp - k3pp);
+        k3gg /= 6;
+        k3pp /= 6;
+        o.kgg *= -0.5f;
+        o.kpp *= -0.5f;
+        o.eval(g0 + eg, phh + (t0 + et));
+        stpz = delz;
+        eg += stpz * o.kgg;
+        et += stpz * o.kpp;
+        k3gg -= o.kgg;
+        k3pp -= o.kpp;
+        o.kgg *= 2;
+        o.kpp *= 2;
+        o.eval(g0 + eg, ph1 + (t0 + et));
+        eg += stpz * (k3gg + o.kgg / 6.0f);
+        et += stpz * (k3pp + o.kpp / 6.0f);
+        const unsigned hs = mix_hash((unsigned long long)i, stepno, 0x51u);
+        const float dg = add_sr(g0, eg, hs);
+        if (S.chicane) et += r56 * (dg + dgam0);  // TrackBeam::applyR56
+        const float dth = add_sr(t0, et, mix_hash((unsigned long long)i, stepno, 0xA7u));
+        transverse32(S.t2, gref + dg, x, y, px, py);
+        a.dg[i] = dg;
+        a.dth[i] = dth;
+        a.x[i] = x;
+        a.y[i] = y;
+        a.px[i] = px;
+        a.py[i] = py;
+        // beam sums of the updated state
+        fv[0] += dg;
+        fv[1] += dg * dg;
+        fv[2] += x;
+        fv[3] += x * x;
+        fv[4] += y;
+        fv[5] += y * y;
+        float s, c;
+        __sincosf(ph1 + dth, &s, &c);
+        fv[6] += c;
+        fv[7] += s;
+        if (dep && (x > -gridmax) && (x < gridmax) && (y > -gridmax) && (y < gridmax)) {
+            float wx = (x + gridmax) * inv_dgrid, wy = (y + gridmax) * inv_dgrid;
+            const int ix = min((int)floorf(wx), ng - 2), iy = min((int)floorf(wy), ng - 2);
+            bx0 = min(bx0, ix);
+            bx1 = max(bx1, ix + 1);
+            by0 = min(by0, iy);
+            by1 = max(by1, iy + 1);
+            if (fused) {
+                if (ix >= tx0 && ix + 1 < tx0 + tw && iy >= ty0 && iy + 1 < ty0 + th) {
+                    wx = 1.f + floorf(wx) - wx;
+                    wy = 1.f + floorf(wy) - wy;
+                    const float ddx = x - dax, ddy = y - day;
+                    const float faw2 = 1.f + dkx * ddx * ddx + dky * ddy * ddy + 2.f * (dgx * ddx + dgy * ddy);
+                    const float part = sqrtf(faw2) * scl / (gref + dg);
+                    const float2 cp = f2(s * part, c * part);
+                    const int lk = (iy - ty0) * tw + (ix - tx0);
+                    unsafeAtomicAdd(&work[lk].x, (double)(wx * wy * cp.x));
+                    unsafeAtomicAdd(&work[lk].y, (double)(wx * wy * cp.y));
+                    unsafeAtomicAdd(&work[lk + 1].x, (double)((1 - wx) * wy * cp.x));
+                    unsafeAtomicAdd(&work[lk + 1].y, (double)((1 - wx) * wy * cp.y));
+                    unsafeAtomicAdd(&work[lk + tw].x, (double)(wx * (1 - wy) * cp.x));
+                    unsafeAtomicAdd(&work[lk + tw].y, (double)(wx * (1 - wy) * cp.y));
+                    unsafeAtomicAdd(&work[lk + tw + 1].x, (double)((1 - wx) * (1 - wy) * cp.x));
+                    unsafeAtomicAdd(&work[lk + tw + 1].y, (double)((1 - wx) * (1 - wy) * cp.y));
+                } else {
+                    ovf = 1;
+                }
+            }
+        }
+    }
+    double v[8];
+#pragma unroll
+    for (int k = 0; k < 8; k++) v[k] = fv[k];
+    bsum_w<kMainNT, 8>(v, wred);
+    if (t == 0) {
+        // sum gamma = n gref + sum dg, sum gamma^2 = n gref^2 + 2 gref sum dg + sum dg^2
+        const double n = (double)npart, gr = a.gref;
+        a.beamsum[8 * is + 0] = n * gr + v[0];
+        a.beamsum[8 * is + 1] = n * gr * gr + 2.0 * gr * v[0] + v[1];
+        for (int k = 2; k < 8; k++) a.beamsum[8 * is + k] = v[k];
+        a.base[is] = basep + S.kappa * P.delz;
+    }
+    // ---- footprint box and overflow flag
+    int bv[5] = {bx0, by0, -bx1, -by1, -ovf};
+    bmin_w<kMainNT, 5>(bv, bred);
+    bx0 = bv[0];
+    by0 = bv[1];
+    bx1 = -bv[2];
+    by1 = -bv[3];
+    ovf = -bv[4];
+    if (!dep || bx1 < 0) {
+        if (t == 0) a.box[is] = make_int4(1, 1, 0, 0);
+        return;
+    }
+    float2 *plane = a.src + (long long)is * ng * ng;
+    if (fused && !ovf) {
+        for (int k = t; k < tw * th; k += kMainNT)
+            plane[(ty0 + k / tw) * ng + tx0 + k % tw] = tof(work[k]);
+        if (t == 0) a.box[is] = make_int4(tx0, ty0, tx0 + tw - 1, ty0 + th - 1);
+        return;
+    }
+    // ---- exact two-pass deposit over the footprint box (rows in bands of the tile)
+    bx1 = min(bx1, ng - 1);
+    by1 = min(by1, ng - 1);
+    if (t == 0) a.box[is] = make_int4(bx0, by0, bx1, by1);
+    const int ww = bx1 - bx0 + 1;
+    const int rows_per_band = max(1, kTileCells / ww);
+    double2 *tile = work;
+    for (int r0 = by0; r0 <= by1; r0 += rows_per_band) {
+        const int r1 = min(by1, r0 + rows_per_band - 1);
+        const int ncell = (r1 - r0 + 1) * ww;
+        for (int k = t; k < ncell; k += kMainNT) tile[k] = make_double2(0, 0);
+        __syncthreads();
+        for (long long i = b0 + t; i < b1; i += kMainNT) {
+            const float x = a.x[i], y = a.y[i];
+            if (!((x > -gridmax) && (x < gridmax) && (y > -gridmax) && (y < gridmax))) continue;
+            float wx = (x + gridmax) * inv_dgrid, wy = (y + gridmax) * inv_dgrid;
+            const int ix = min((int)floorf(wx), ng - 2), iy = min((int)floorf(wy), ng - 2);
+            if (iy + 1 < r0 || iy > r1) continue;
+            wx = 1.f + floorf(wx) - wx;
+            wy = 1.f + floorf(wy) - wy;
+            const float ddx = x - dax, ddy = y - day;
+            const float faw2 = 1.f + dkx * ddx * ddx + dky * ddy * ddy + 2.f * (dgx * ddx + dgy * ddy);
+            const float part = sqrtf(faw2) * scl / (gref + a.dg[i]);
+            float s, c;
+            __sincosf(ph1 + a.dth[i], &s, &c);
+            const float2 cp = f2(s * part, c * part);
+            const float wgt[4] = {wx * wy, (1 - wx) * wy, wx * (1 - wy), (1 - wx) * (1 - wy)};
+            const int cx[4] = {ix, ix + 1, ix, ix + 1};
+            const int cy[4] = {iy, iy, iy + 1, iy + 1};
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+                if (cy[k] < r0 || cy[k] > r1 || cx[k] < bx0 || cx[k] > bx1) continue;
+                const int lk = (cy[k] - r0) * ww + (cx[k] - bx0);
+                unsafeAtomicAdd(&tile[lk].x, (double)(wgt[k] * cp.x));
+                unsafeAtomicAdd(&tile[lk].y, (double)(wgt[k] * cp.y));
+            }
+        }
+        __syncthreads();
+        for (int k = t; k < ncell; k += kMainNT) {
+            const int ry = r0 + k / ww, rx = bx0 + k % ww;
+            plane[ry * ng + rx] = tof(tile[k]);
+        }
+        __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------------------
+// ADI of the mixed form: both half steps (FieldSolverADI::ADI :60-91) for a
+// 32x32 output tile, with the truncated banded inverse B ~ A^-1 in FP32
+// (dropped entries < 2^-28 of the diagonal).
+//
+// REF (FP64 residual correction): each half step solves A u = r as
+//   u1 = B r (FP32),  rho = r - A u1 (FP64),  u = u1 + B rho,
+// with r and u kept in FP64, so the step is accurate to ~(2^-24)^2 before
+// the result is rounded to the FP32 field.  The correction needs r on a
+// halo of 2M+1 cells instead of M.
+// Without REF the half steps are u = B r in FP32.
+// ---------------------------------------------------------------------------
+template <int M, bool REF>
+struct MixGeom {
+    static constexpr int TS = 32;
+    static constexpr int W = 2 * M + 1;
+    static constexpr int H = REF ? 2 * M + 1 : M;   // halo of the E region
+    static constexpr int RE = TS + 2 * H;
+    static constexpr int PE = ((RE + 6) / 8) * 8 + 1;
+    static constexpr int PG = TS + 1;
+    static constexpr int CX = REF ? (M <= 4 ? 4 : 2) : 8;   // x-stage outputs per task
+    static constexpr int RY = REF ? (M <= 4 ? 4 : 2) : 4;   // y-stage outputs per task
+    static constexpr int BR = REF ? M + 1 : 0;      // extra band rows each side
+    static constexpr int NB = TS + 2 * BR;          // band rows staged for edge tiles
+    using ET = typename std::conditional<REF, double2, float2>::type;
+    static constexpr int SR = (RE + (256 / RE) - 1) / (256 / RE);
+    static constexpr size_t lds_bytes =
+        (size_t)RE * PE * sizeof(ET) + (size_t)RE * PG * sizeof(ET) + 2 * (size_t)NB * W * sizeof(float2);
+};
+
+template <int M, bool REF>
+__global__ void __launch_bounds__(256) k_mix_adi(MixArrays a) {
+    using Gm = MixGeom<M, REF>;
+    using ET = typename Gm::ET;
+    constexpr int TS = Gm::TS, W = Gm::W, H = Gm::H, RE = Gm::RE, PE = Gm::PE, PG = Gm::PG;
+    constexpr int CX = Gm::CX, RY = Gm::RY, BR = Gm::BR, NB = Gm::NB;
+    constexpr int NT = 256;
+    extern __shared__ double2 smd[];
+    ET *E = (ET *)smd;                 // [RE][PE]
+    ET *G = E + RE * PE;               // [RE][PG]
+    float2 *BX = (float2 *)(G + RE * PG);  // [NB][W]
+    float2 *BY = BX + NB * W;          // [NB][W]
+    const int t = threadIdx.x;
+    const MixStep &S = a.steps[a.st->istep];
+    const MixBand bi = a.binfo[S.band];
+    const float2 *band = a.bands + bi.off;
+    const double aa = bi.a;
+    const int ng = a.ng, ns = a.ns;
+    const int ntx = (ng + TS - 1) / TS, ntiles = ntx * ntx;
+    // XCD-aware order (see k_adi_band): all tiles of a slice on one XCD
+    const int b = blockIdx.x;
+    const int ii = (b / (8 * ntiles)) * 8 + (b & 7);
+    const int tile = (b % (8 * ntiles)) >> 3;
+    if (ii >= ns) return;
+    const int tx0 = (tile % ntx) * TS, ty0 = (tile / ntx) * TS;
+    const int first = a.st->first;
+    const long long fs = (long long)((ii + first) % ns);
+    const int cur = a.st->cur;
+    const float2 *u = a.field[cur] + fs * ng * ng;
+    float2 *uo = a.field[cur ^ 1] + fs * ng * ng;
+    const float2 *Sp = a.src + (long long)ii * ng * ng;
+    const int4 bx = a.box[ii];
+    auto sat = [&](int x, int y) {
+        return (x >= bx.x && x <= bx.z && y >= bx.y && y <= bx.w) ? Sp[y * ng + x] : f2(0, 0);
+    };
+    const bool xin = tx0 - BR >= bi.klo && tx0 + TS - 1 + BR <= bi.khi;
+    const bool yin = ty0 - BR >= bi.klo && ty0 + TS - 1 + BR <= bi.khi;
+    const float2 *bc = band + (long long)((bi.klo + bi.khi) / 2) * W;
+    if (!xin)
+        for (int e = t; e < NB * W; e += NT) {
+            const int x = tx0 - BR + e / W;
+            BX[e] = (x >= 0 && x < ng) ? band[(long long)x * W + e % W] : f2(0, 0);
+        }
+    if (!yin)
+        for (int e = t; e < NB * W; e += NT) {
+            const int y = ty0 - BR + e / W;
+            BY[e] = (y >= 0 && y < ng) ? band[(long long)y * W + e % W] : f2(0, 0);
+        }
+    // ---- E = S + (1 + i a d_y^2) u on the RE x RE region (zero off the grid)
+    {
+        constexpr int SR = Gm::SR;
+        constexpr int NSTRIP = (RE + SR - 1) / SR;
+        if (t < RE * NSTRIP) {
+            const int c = t % RE, r0 = (t / RE) * SR;
+            const int x = tx0 - H + c;
+            const bool xok = x >= 0 && x < ng;
+            float2 uv[SR + 2], sv[SR];
+#pragma unroll
+            for (int q = 0; q < SR + 2; q++) {
+                const int y = ty0 - H + r0 + q - 1;
+                uv[q] = (xok && y >= 0 && y < ng && r0 + q - 1 < RE + 1) ? u[y * ng + x] : f2(0, 0);
+            }
+#pragma unroll
+            for (int q = 0; q < SR; q++) sv[q] = (r0 + q < RE) ? sat(x, ty0 - H + r0 + q) : f2(0, 0);
+#pragma unroll
+            for (int q = 0; q < SR; q++) {
+                const int r = r0 + q, y = ty0 - H + r;
+                if (r < RE) {
+                    ET v{};
+                    if (xok && y >= 0 && y < ng) {
+                        if constexpr (REF) {
+                            const double2 uc = tod(uv[q + 1]), up = tod(uv[q + 2]), dn = tod(uv[q]);
+                            const double sx = up.x - 2.0 * uc.x + dn.x, sy = up.y - 2.0 * uc.y + dn.y;
+                            v = d2((double)sv[q].x + uc.x - aa * sy, (double)sv[q].y + uc.y + aa * sx);
+                        } else {
+                            const float af = (float)aa;
+                            const float2 uc = uv[q + 1];
+                            const float sx = uv[q + 2].x - 2.f * uc.x + uv[q].x;
+                            const float sy = uv[q + 2].y - 2.f * uc.y + uv[q].y;
+                            v = f2(sv[q].x + uc.x - af * sy, sv[q].y + uc.y + af * sx);
+                        }
+                    }
+                    E[r * PE + c] = v;
+                }
+            }
+        }
+    }
+    __syncthreads();
+    // ---- implicit x on every row of the region, output columns of the tile;
+    // then F = S + (1 + i a d_x^2) u' = S + 2 u' - E into G.  The band
+    // accessor is chosen once per tile: interior tiles read the shared row
+    // (uniform, scalar loads), edge tiles the rows staged in LDS.
+    auto xstage = [&](auto bx_row) {
+    for (int task = t; task < RE * (TS / CX); task += NT) {
+        const int r = task % RE, g = task / RE;
+        const int y = ty0 - H + r;
+        const int x0 = tx0 + g * CX;
+        if (y < 0 || y >= ng) {
+#pragma unroll
+            for (int j = 0; j < CX; j++) G[r * PG + g * CX + j] = ET{};
+            continue;
+        }
+        if constexpr (!REF) {
+            float2 ev[CX + 2 * M];
+#pragma unroll
+            for (int k = 0; k < CX + 2 * M; k++) ev[k] = E[r * PE + g * CX + k];
+#pragma unroll
+            for (int j = 0; j < CX; j++) {
+                const int x = x0 + j;
+                float2 acc = f2(0, 0);
+#pragma unroll
+                for (int m = 0; m < W; m++) acc = ffma(bx_row(x, m), ev[j + m], acc);
+                float2 f = f2(0, 0);
+                if (x < ng) {
+                    const float2 sv = sat(x, y);
+                    f = f2(sv.x + 2.f * acc.x - ev[j + M].x, sv.y + 2.f * acc.y - ev[j + M].y);
+                }
+                G[r * PG + g * CX + j] = f;
+            }
+        } else {
+            constexpr int NE = CX + 4 * M + 2, NU = CX + 2 * M + 2, NR = CX + 2 * M;
+            double2 ev[NE];
+#pragma unroll
+            for (int k = 0; k < NE; k++) ev[k] = E[r * PE + g * CX + k];
+            float2 u1[NU];
+#pragma unroll
+            for (int k = 0; k < NU; k++) {
+                const int xp = x0 - (M + 1) + k;
+                float2 acc = f2(0, 0);
+                if (xp >= 0 && xp < ng) {
+#pragma unroll
+                    for (int m = 0; m < W; m++) acc = ffma(bx_row(xp, m), tof(ev[k + m]), acc);
+                }
+                u1[k] = acc;
+            }
+            float2 rho[NR];
+#pragma unroll
+            for (int k = 0; k < NR; k++) {
+                // rho = r - A u1, A u = (1 + 2 i a) u - i a (u_{+1} + u_{-1})
+                const double2 c0 = tod(u1[k + 1]), cl = tod(u1[k]), cr = tod(u1[k + 2]);
+                const double nbx = cl.x + cr.x, nby = cl.y + cr.y;
+                const double aux = c0.x - 2.0 * aa * c0.y + aa * nby;
+                const double auy = c0.y + 2.0 * aa * c0.x - aa * nbx;
+                rho[k] = tof(d2(ev[k + M + 1].x - aux, ev[k + M + 1].y - auy));
+            }
+#pragma unroll
+            for (int j = 0; j < CX; j++) {
+                const int x = x0 + j;
+                float2 d = f2(0, 0);
+#pragma unroll
+                for (int m = 0; m < W; m++) d = ffma(bx_row(x, m), rho[j + m], d);
+                double2 f = d2(0, 0);
+                if (x < ng) {
+                    const float2 sv = sat(x, y);
+                    const double upx = (double)u1[j + M + 1].x + (double)d.x;
+                    const double upy = (double)u1[j + M + 1].y + (double)d.y;
+                    f = d2((double)sv.x + 2.0 * upx - ev[j + H].x, (double)sv.y + 2.0 * upy - ev[j + H].y);
+                }
+                G[r * PG + g * CX + j] = f;
+            }
+        }
+    }
+    };
+    if (xin) xstage([&](int, int m) -> float2 { return bc[m]; });
+    else xstage([&](int x, int m) -> float2 { return BX[(x - (tx0 - BR)) * W + m]; });
+    __syncthreads();
+    // ---- implicit y, output rows of the tile; field sums in FP64
+    const double shift = -0.5 * (double)(ng - 1);
+    const int icen = (ng / 2) * ng + (ng / 2);
+    double dg[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    auto ystage = [&](auto by_row) {
+    for (int task = t; task < TS * (TS / RY); task += NT) {
+        const int c = task % TS, q = task / TS;
+        const int x = tx0 + c, y0 = ty0 + q * RY;
+        float2 out[RY];
+        if constexpr (!REF) {
+            float2 fv[RY + 2 * M];
+#pragma unroll
+            for (int k = 0; k < RY + 2 * M; k++) fv[k] = G[(q * RY + k) * PG + c];
+#pragma unroll
+            for (int j = 0; j < RY; j++) {
+                float2 acc = f2(0, 0);
+#pragma unroll
+                for (int m = 0; m < W; m++) acc = ffma(by_row(y0 + j, m), fv[j + m], acc);
+                out[j] = acc;
+            }
+        } else {
+            constexpr int NE = RY + 4 * M + 2, NU = RY + 2 * M + 2, NR = RY + 2 * M;
+            double2 gv[NE];
+#pragma unroll
+            for (int k = 0; k < NE; k++) gv[k] = G[(q * RY + k) * PG + c];
+            float2 u1[NU];
+#pragma unroll
+            for (int k = 0; k < NU; k++) {
+                const int yp = y0 - (M + 1) + k;
+                float2 acc = f2(0, 0);
+                if (yp >= 0 && yp < ng) {
+#pragma unroll
+                    for (int m = 0; m < W; m++) acc = ffma(by_row(yp, m), tof(gv[k + m]), acc);
+                }
+                u1[k] = acc;
+            }
+            float2 rho[NR];
+#pragma unroll
+            for (int k = 0; k < NR; k++) {
+                const double2 c0 = tod(u1[k + 1]), cl = tod(u1[k]), cr = tod(u1[k + 2]);
+                const double nbx = cl.x + cr.x, nby = cl.y + cr.y;
+                const double aux = c0.x - 2.0 * aa * c0.y + aa * nby;
+                const double auy = c0.y + 2.0 * aa * c0.x - aa * nbx;
+                rho[k] = tof(d2(gv[k + M + 1].x - aux, gv[k + M + 1].y - auy));
+            }
+#pragma unroll
+            for (int j = 0; j < RY; j++) {
+                float2 d = f2(0, 0);
+#pragma unroll
+                for (int m = 0; m < W; m++) d = ffma(by_row(y0 + j, m), rho[j + m], d);
+                out[j] = tof(d2((double)u1[j + M + 1].x + (double)d.x, (double)u1[j + M + 1].y + (double)d.y));
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < RY; j++) {
+            const int y = y0 + j;
+            if (x < ng && y < ng) {
+                const float2 o = out[j];
+                uo[y * ng + x] = o;
+                const double ox = o.x, oy = o.y;
+                const double wei = ox * ox + oy * oy;
+                const double dx = (double)x + shift, dy = (double)y + shift;
+                dg[0] += wei;
+                dg[1] += dx * wei;
+                dg[2] += dx * dx * wei;
+                dg[3] += dy * wei;
+                dg[4] += dy * dy * wei;
+                dg[5] += ox;
+                dg[6] += oy;
+                if (y * ng + x == icen) dg[7] = wei;
+            }
+        }
+    }
+    };
+    if (yin) ystage([&](int, int m) -> float2 { return bc[m]; });
+    else ystage([&](int y, int m) -> float2 { return BY[(y - (ty0 - BR)) * W + m]; });
+    double *red = (double *)smd;
+    bsum<NT, 8>(dg, red);
+    if (t == 0) {
+        double *o = a.tilediag + (fs * ntiles + tile) * 8;
+        for (int k = 0; k < 8; k++) o[k] = dg[k];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// slippage on the device (Control::applySlippage, Control.cpp:158-276): the
+// ring offset, the slices to zero, the live field buffer and the step index.
+// ---------------------------------------------------------------------------
+__global__ void k_mix_slip(MixArrays a) {
+    if (threadIdx.x != 0) return;
+    MixState &st = *a.st;
+    const MixStep &S = a.steps[st.istep];
+    const int ns = a.ns;
+    st.nzero = 0;
+    st.cur ^= 1;  // the ADI of this step wrote the other buffer
+    if (a.timerun) {
+        st.accuslip += S.slip;
+        int direction = 1;
+        while (fabs(st.accuslip) > (a.sample * 0.8)) {
+            if (st.accuslip < 0) direction = -1;
+            st.accuslip -= a.sample * direction;
+            int last = (st.first + ns - 1) % ns;
+            if (direction < 0) last = (last + 1) % ns;
+            if (!a.periodic && st.nzero < kMixMaxZero) st.zero[st.nzero++] = last;
+            st.first = last;
+            if (direction < 0) st.first = (last + 1) % ns;
+        }
+    }
+    st.rec_step = st.istep;
+    st.istep = st.istep + 1;
+    st.seq = st.seq + 1;
+}
+
+__global__ void __launch_bounds__(256) k_mix_zero(MixArrays a) {
+    const int z = blockIdx.y;
+    if (z >= a.st->nzero) return;
+    const long long cells = (long long)a.ng * a.ng;
+    const long long i = (long long)blockIdx.x * 256 + threadIdx.x;
+    if (i < cells) a.field[a.st->cur][(long long)a.st->zero[z] * cells + i] = f2(0, 0);
+}
+
+// Record of the completed step per beam slice: beam sums and the field sums
+// of its storage slice (zero for slices slippage has just cleared).
+__global__ void __launch_bounds__(64) k_mix_record(MixArrays a) {
+    const int is = blockIdx.x * 64 + threadIdx.x;
+    if (is >= a.ns) return;
+    const MixState &st = *a.st;
+    const int fs = (is + st.first) % a.ns;
+    bool zeroed = false;
+    for (int k = 0; k < st.nzero; k++) zeroed |= (st.zero[k] == fs);
+    const int ntx = (a.ng + 31) / 32, ntiles = ntx * ntx;
+    double f[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    if (!zeroed)
+        for (int k = 0; k < ntiles; k++)
+            for (int q = 0; q < 8; q++) f[q] += a.tilediag[((long long)fs * ntiles + k) * 8 + q];
+    double *o = a.rec + ((long long)(st.rec_step % a.nrec) * a.ns + is) * kMixRecord;
+    for (int q = 0; q < 8; q++) o[q] = a.beamsum[8 * is + q];
+    for (int q = 0; q < 8; q++) o[8 + q] = f[q];
+}
+
+// Standalone sums for a diagnostics call outside a run.
+__global__ void __launch_bounds__(256) k_mix_diag_beam(MixArrays a, double *out) {
+    __shared__ double lds[8 * 256];
+    const int is = blockIdx.x;
+    const long long b0 = a.off[is], b1 = a.off[is + 1];
+    const float ph = wrap_phase(a.base[is]);
+    double v[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    for (long long i = b0 + threadIdx.x; i < b1; i += 256) {
+        const double gd = a.gref + (double)a.dg[i];
+        const double x = a.x[i], y = a.y[i];
+        v[0] += gd;
+        v[1] += gd * gd;
+        v[2] += x;
+        v[3] += x * x;
+        v[4] += y;
+        v[5] += y * y;
+        float s, c;
+        __sincosf(ph + a.dth[i], &s, &c);
+        v[6] += c;
+        v[7] += s;
+    }
+    bsum<256, 8>(v, lds);
+    if (threadIdx.x == 0)
+        for (int k = 0; k < 8; k++) out[8 * is + k] = v[k];
+}
+
+__global__ void __launch_bounds__(256) k_mix_diag_field(MixArrays a, double *out) {
+    __shared__ double lds[8 * 256];
+    const int is0 = blockIdx.x, ng = a.ng;
+    const float2 *f = a.field[a.st->cur] + (long long)is0 * ng * ng;
+    const double shift = -0.5 * (double)(ng - 1);
+    double v[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    for (int i = threadIdx.x; i < ng * ng; i += 256) {
+        const int iy = i / ng, ix = i % ng;
+        const double dy = (double)iy + shift, dx = (double)ix + shift;
+        const double lx = f[i].x, ly = f[i].y;
+        const double wei = lx * lx + ly * ly;
+        v[0] += wei;
+        v[1] += dx * wei;
+        v[2] += dx * dx * wei;
+        v[3] += dy * wei;
+        v[4] += dy * dy * wei;
+        v[5] += lx;
+        v[6] += ly;
+    }
+    bsum<256, 8>(v, lds);
+    if (threadIdx.x == 0) {
+        const int ic = (ng / 2) * ng + (ng / 2);
+        const double cx = f[ic].x, cy = f[ic].y;
+        v[7] = cx * cx + cy * cy;
+        for (int k = 0; k < 8; k++) out[8 * is0 + k] = v[k];
+    }
+}
+
+// Sorting::localSort target slice (src/Util/Sorting.cpp:96) on the absolute
+// phase base + dth.
+__global__ void __launch_bounds__(256) k_mix_sort_index(MixArrays a, double invslen, int *atar,
+                                                        unsigned long long *moved) {
+    const int is = blockIdx.x;
+    const long long b0 = a.off[is], b1 = a.off[is + 1];
+    const double base = a.base[is];
+    for (long long i0 = b0; i0 < b1; i0 += 256) {
+        const long long i = i0 + threadIdx.x;
+        int m = 0;
+        if (i < b1) {
+            const int at = (int)floor((base + (double)a.dth[i]) * invslen);
+            atar[i] = at;
+            m = (at != 0);
+        }
+        const unsigned long long bal = __ballot(m);
+        if ((threadIdx.x & 63) == 0 && bal) atomicAdd(moved, (unsigned long long)__popcll(bal));
+    }
+}
+
+template <int M, bool REF>
+int set_adi_attr() {
+    using Gm = MixGeom<M, REF>;
+    if (Gm::lds_bytes > 160 * 1024) return 0;  // not launchable; never selected
+    return hipFuncSetAttribute((const void *)k_mix_adi<M, REF>,
+                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)Gm::lds_bytes) ==
+           hipSuccess;
+}
+
+template <int M, bool REF>
+void launch_adi_t(hipStream_t s, const MixArrays &a) {
+    using Gm = MixGeom<M, REF>;
+    const int ntx = (a.ng + Gm::TS - 1) / Gm::TS;
+    const unsigned nblocks = (unsigned)(((a.ns + 7) / 8) * 8) * (unsigned)(ntx * ntx);
+    hipLaunchKernelGGL((k_mix_adi<M, REF>), dim3(nblocks), dim3(256), Gm::lds_bytes, s, a);
+}
+
+constexpr int kTmplM[] = {3, 4, 6, 9, 12};
+
+}  // namespace
+
+bool mix_band_supported(int m, int *tmpl_m) {
+    for (int t : kTmplM)
+        if (m <= t) {
+            *tmpl_m = t;
+            return true;
+        }
+    return false;
+}
+
+int mix_band_tiles(int ng) {
+    const int ntx = (ng + 31) / 32;
+    return ntx * ntx;
+}
+
+size_t mix_band_lds(int tmpl_m, bool refine) {
+#define G4_L(MM) \
+    if (tmpl_m == MM) return refine ? MixGeom<MM, true>::lds_bytes : MixGeom<MM, false>::lds_bytes;
+    G4_L(3) G4_L(4) G4_L(6) G4_L(9) G4_L(12)
+#undef G4_L
+    return 0;
+}
+
+int mix_prepare() {
+    int ok = hipFuncSetAttribute((const void *)k_mix_main, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                 kMixDyn * (int)sizeof(float2)) == hipSuccess;
+#define G4_P(MM) ok &= set_adi_attr<MM, false>(); if (MixGeom<MM, true>::lds_bytes <= 160 * 1024) ok &= set_adi_attr<MM, true>();
+    G4_P(3) G4_P(4) G4_P(6) G4_P(9) G4_P(12)
+#undef G4_P
+    return ok ? 0 : -1;
+}
+
+int mix_sc_mode_cells() { return kCelmCells; }
+
+int launch_mix_step(hipStream_t s, const MixArrays &a, int tmpl_m, bool refine, bool sc,
+                    bool moments, int maxzero, hipEvent_t *ev) {
+    auto mark = [&](int k) {
+        if (ev) (void)hipEventRecord(ev[k], s);
+    };
+    if (refine && mix_band_lds(tmpl_m, true) > 160 * 1024) return -1;
+    mark(0);
+    hipLaunchKernelGGL(k_mix_front, dim3(a.ns), dim3(kFrontNT), 0, s, a, moments ? 1 : 0, sc ? 1 : 0);
+    if (sc) hipLaunchKernelGGL(k_mix_scan, dim3(1), dim3(64), 0, s, a);
+    mark(1);
+    mark(2);
+    hipLaunchKernelGGL(k_mix_main, dim3(a.ns), dim3(kMainNT), kMixDyn * sizeof(float2), s, a,
+                       sc ? 1 : 0);
+    mark(3);
+    mark(4);
+    mark(5);
+    mark(6);
+#define G4_A(MM)                                         \
+    else if (tmpl_m == MM) {                             \
+        if (refine) launch_adi_t<MM, true>(s, a);        \
+        else launch_adi_t<MM, false>(s, a);              \
+    }
+    if (false) {
+    }
+    G4_A(3) G4_A(4) G4_A(6) G4_A(9) G4_A(12)
+    else return -1;
+#undef G4_A
+    mark(7);
+    hipLaunchKernelGGL(k_mix_slip, dim3(1), dim3(64), 0, s, a);
+    if (maxzero > 0) {
+        const long long cells = (long long)a.ng * a.ng;
+        hipLaunchKernelGGL(k_mix_zero, dim3((unsigned)((cells + 255) / 256), (unsigned)maxzero),
+                           dim3(256), 0, s, a);
+    }
+    hipLaunchKernelGGL(k_mix_record, dim3((a.ns + 63) / 64), dim3(64), 0, s, a);
+    mark(8);
+    return 0;
+}
+
+void launch_mix_diag(hipStream_t s, const MixArrays &a, double *beam8, double *field8) {
+    hipLaunchKernelGGL(k_mix_diag_beam, dim3(a.ns), dim3(256), 0, s, a, beam8);
+    hipLaunchKernelGGL(k_mix_diag_field, dim3(a.ns), dim3(256), 0, s, a, field8);
+}
+
+void launch_mix_sort_index(hipStream_t s, const MixArrays &a, double invslen, int *atar,
+                           unsigned long long *moved) {
+    hipLaunchKernelGGL(k_mix_sort_index, dim3(a.ns), dim3(256), 0, s, a, invslen, atar, moved);
+}
+
+}  // namespace g4mi

@@ -12,7 +12,8 @@ import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 PKG = os.path.dirname(HERE)
-LIB_PATH = os.path.join(PKG, "lib", "libg4mi.so")
+# G4MI_LIBRARY names another build of the same library (e.g. a debug build)
+LIB_PATH = os.environ.get("G4MI_LIBRARY") or os.path.join(PKG, "lib", "libg4mi.so")
 HEADER = os.path.join(os.path.dirname(PKG), "include", "g4mi.h")
 
 FP64, FP32 = 0, 1

@@ -67,6 +67,34 @@ CONFIG0 = dict(name="config0", nslice=1, npart=8192, nbins=4, ngrid=NGRID, gridm
                seed_power=5e3, timerun=False)
 
 
+CONFIG2 = dict(CONFIG1, name="config2", precision="fp32")
+
+
+def config3_profile(ns=4096, peak=4000.0, sigma=1024.0, nmax=16384, nmin=256, nbins=4):
+    """Uneven beam of config 3: I_s = peak exp(-((s - (ns-1)/2) / sigma)^2 / 2) and
+    n_s = nmax I_s / peak macroparticles, clamped to [nmin, nmax] and rounded to
+    a multiple of nbins (the mirrored beamlets)."""
+    c = (ns - 1) / 2.0
+    cur = [peak * math.exp(-0.5 * ((i - c) / sigma) ** 2) for i in range(ns)]
+    cnt = [min(nmax, max(nmin, nbins * int(round(nmax * I / peak / nbins)))) for I in cur]
+    return cur, cnt
+
+
+CONFIG3 = dict(name="config3", nslice=4096, npart=None, nbins=4, ngrid=255, gridmax=GRIDMAX,
+               seed=7, shotnoise=True, longrange=True, sc_nz=1, sc_nphi=0, sc_ngrid=100,
+               seed_power=0.0, timerun=True, precision="fp32", uneven=True)
+for _c in (CONFIG0, CONFIG1):
+    _c["precision"] = "fp64"
+CONFIGS = {c["name"]: c for c in (CONFIG0, CONFIG1, CONFIG2, CONFIG3)}
+
+
+def counts_and_currents(cfg):
+    if cfg.get("uneven"):
+        cur, cnt = config3_profile(cfg["nslice"], nbins=cfg["nbins"])
+        return cnt, cur
+    return [cfg["npart"]] * cfg["nslice"], [CURRENT] * cfg["nslice"]
+
+
 def describe(cfg):
     lsc = []
     if cfg["longrange"]:
@@ -75,7 +103,15 @@ def describe(cfg):
         lsc.append(f"short-range LSC nz={cfg['sc_nz']} nphi={cfg['sc_nphi']} "
                    f"{cfg['sc_ngrid']} radial cells")
     mode = "time-dependent SASE" if cfg["timerun"] else "steady state"
-    return (f"{cfg['name']}: {mode} FP64, {cfg['nslice']} slices x {cfg['npart']} "
-            f"macroparticles (nbins {cfg['nbins']}), {cfg['ngrid']}x{cfg['ngrid']} field, "
+    prec = "FP64 plain form" if cfg.get("precision", "fp64") == "fp64" else \
+        "mixed FP32 form (FP64 slice phase base, FP64 ADI residual correction)"
+    if cfg.get("uneven"):
+        cnt, _ = counts_and_currents(cfg)
+        parts = (f"{cfg['nslice']} slices with a Gaussian current (peak 4 kA, sigma 1024 slices), "
+                 f"{min(cnt)}..{max(cnt)} macroparticles per slice, {sum(cnt)} in total")
+    else:
+        parts = f"{cfg['nslice']} slices x {cfg['npart']} macroparticles"
+    return (f"{cfg['name']}: {mode} {prec}, {parts} "
+            f"(nbins {cfg['nbins']}), {cfg['ngrid']}x{cfg['ngrid']} field, "
             f"planar aw={AW} lambda_u={LAMBDAU} m, 4x125 periods + FODO, "
             + (", ".join(lsc) if lsc else "no space charge"))

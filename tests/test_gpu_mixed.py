@@ -1,0 +1,194 @@
+"""GPU tests of the mixed-precision form (FP32 particles and field, FP64
+slice phase base, FP64 residual correction in the ADI, FP64 reductions).
+
+The mixed form is not bit-compared: each test states its error against the
+FP64 oracle on the same inputs and bounds it.  The end-to-end bar is the
+one the FP32 form is built to: P(z) and |b(z)| within 1 % of the FP64
+oracle."""
+import ctypes as C
+import sys
+
+import numpy as np
+import pytest
+
+import cases
+import g4mi
+import harness as H
+import oracle_py as O
+
+pytestmark = pytest.mark.gpu
+
+
+def _log(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def st():
+    return H.steps()
+
+
+def _field_err(ctx, sim):
+    f, first = ctx.download_field()
+    assert first == sim.s.first
+    return float(np.max(np.abs(f - sim.field)) / np.max(np.abs(sim.field)))
+
+
+@pytest.mark.parametrize("refine", [1, 0])
+def test_mixed_adi_against_oracle(st, refine):
+    """Field-only steps (zero current: no source) over undulator, quadrupole
+    and drift step sizes (band half widths 3, 4, 6 on 151^2): the FP32 band
+    ADI, with and without the FP64 residual correction, against the FP64
+    Thomas ADI of the oracle.  Bound: the FP32 rounding of the stored field
+    per step (2^-24 relative) times the step count, with margin."""
+    _log(f"mixed ADI vs oracle, refine={refine}: 4 slices x 256, 151^2, steps 120..130")
+    # steady state: no slippage, so the seed stays on the grid for every step
+    sim = H.make_state([256] * 4, seed_power=5e3, timerun=False)
+    sim.current[:] = 0.0
+    ctx = H.to_gpu(sim, precision=g4mi.FP32)
+    ctx.set_schedule("adi_refine", refine)
+    idx = list(range(120, 131))
+    errs = []
+    for i in idx:
+        O.lib().orc_step_all(C.byref(sim.s), C.byref(st[i]), cases.GAMMA0)
+        ctx.step(H.gsteps(st)[i], cases.GAMMA0)
+        errs.append(_field_err(ctx, sim))
+    sched = ctx.schedule()
+    _log(f"  dz {[round(st[i].dz, 3) for i in idx]}\n  field rel err per step {['%.2e' % e for e in errs]}"
+         f"\n  schedule {sched}")
+    bound = (2.0 if refine else 8.0) * 2.0 ** -24 * len(idx)
+    assert np.all(np.isfinite(errs)), errs
+    assert max(errs) < bound, (max(errs), bound)
+    ctx.close()
+
+
+def test_mixed_free_push_offset_form(st):
+    """Zero field and current: the FP32 offset push (gamma = gref + dg,
+    theta = base + dth, phase rate xks g'(eps) (eps_ref - eps)) against the
+    FP64 oracle over the first module (125 steps) plus the drift after it.
+    The stored offsets take one stochastically rounded update per step, so
+    the theta error is a random walk of ~ulp(2 pi)/2 = 2.4e-7 rad per step:
+    std ~ 2.4e-7 sqrt(131) ~ 3e-6, the largest of 4096 particles ~5 std;
+    bounds 5e-5 on the largest and 1.5e-5 on the mean.  Gamma: constant, to
+    the FP32 rounding of dg."""
+    _log("mixed free push vs oracle: 8 slices x 512, 131 steps")
+    sim = H.make_state([512] * 8)
+    sim.current[:] = 0.0
+    ctx = H.to_gpu(sim, precision=g4mi.FP32)
+    n = 131
+    ctx.run(H.gsteps(st[:n]), cases.GAMMA0, record=())
+    for i in range(n):
+        O.lib().orc_step_all(C.byref(sim.s), C.byref(st[i]), cases.GAMMA0)
+    b = ctx.download_beam()
+    dth = float(np.max(np.abs(b["theta"] - sim.theta)))
+    dg = float(np.max(np.abs(b["gamma"] - sim.gamma)))
+    dx = float(np.max(np.abs(b["x"] - sim.x)) / np.max(np.abs(sim.x)))
+    mth = float(np.mean(b["theta"] - sim.theta))
+    _log(f"  max |dtheta| {dth:.3e} rad (mean {mth:.3e}), max |dgamma| {dg:.3e}, x rel {dx:.3e}")
+    assert dth < 5e-5 and abs(mth) < 1.5e-5
+    assert dg < 1e-3          # 2 ulp of FP32 at |dg| ~ 5 (gamma spread 1e-4)
+    # x: 262 FP32 linear-map half steps, each rounding at 2^-24 relative
+    assert dx < 3e-5
+    ctx.close()
+
+
+def _mixed_vs_oracle(sim, st, n, record=("power", "bunching")):
+    ctx = H.to_gpu(sim, precision=g4mi.FP32)
+    g = ctx.run(H.gsteps(st[:n]), cases.GAMMA0, record=record)
+    r = sim.run(st, cases.GAMMA0, record=record, nsteps=n)
+    return ctx, g, r
+
+
+def test_mixed_steady_state_seeded_within_one_percent(st):
+    """Config 0 (one slice of 8192, 5 kW seed, 151^2) over all 509 steps in
+    the mixed form: P(z) and |b(z)| within 1 % of the FP64 oracle."""
+    _log("mixed config 0, 509 steps, vs FP64 oracle")
+    sim = H.make_state([8192], seed_power=5e3, timerun=False)
+    ctx, g, r = _mixed_vs_oracle(sim, st, len(st))
+    ep = np.abs(g["power"][:, 0] / r["power"][:, 0] - 1)
+    eb = np.abs(g["bunching"][:, 0] / r["bunching"][:, 0] - 1)
+    mask = r["bunching"][:, 0] > 1e-3  # |b| of the quiet start is ~1e-17 (noise floor)
+    _log(f"  P(z) max rel err {ep.max():.3e} (end {ep[-1]:.3e}, P_end {r['power'][-1, 0]:.4e}); "
+         f"|b(z)| max rel err {eb[mask].max():.3e} over {mask.sum()} steps with |b| > 1e-3")
+    assert ep.max() < 1e-2
+    assert eb[mask].max() < 1e-2
+    ctx.close()
+
+
+def test_mixed_sase_small_within_one_percent(st):
+    """Time-dependent SASE on a reduced window (24 slices x 1024, shot noise
+    seed 7, long-range space charge, slippage, 63^2): the slice-summed P(z)
+    and the mean |b(z)| within 1 % of the FP64 oracle over all 509 steps,
+    and the final field within 1 % (relative to its maximum)."""
+    _log("mixed time-dependent SASE, 24 slices, 509 steps, vs FP64 oracle")
+    sim = H.make_state([1024] * 24, ngrid=63, gridmax=0.3e-3, longrange=True)
+    ctx, g, r = _mixed_vs_oracle(sim, st, len(st), record=("power", "bunching"))
+    pg, pr = g["power"][1:].sum(axis=1), r["power"][1:].sum(axis=1)
+    bg, br = g["bunching"].mean(axis=1), r["bunching"].mean(axis=1)
+    ep = float(np.max(np.abs(pg / pr - 1)))
+    eb = float(np.max(np.abs(bg / br - 1)))
+    f, first = ctx.download_field()
+    ef = float(np.max(np.abs(f - sim.field)) / np.max(np.abs(sim.field)))
+    _log(f"  sum P(z) max rel err {ep:.3e}  mean |b(z)| max rel err {eb:.3e}  final field {ef:.3e}"
+         f"  (P_end {pr[-1]:.4e})")
+    assert first == sim.s.first
+    assert ep < 1e-2 and eb < 1e-2 and ef < 1e-2
+    ctx.close()
+
+
+def test_mixed_graph_replay_matches_direct_launch(st):
+    """Every replay of the captured step graphs (undulator and drift step
+    signatures) against direct launches from the same initial state.  The
+    deposit and the space-charge histograms add with FP32 LDS atomics, whose
+    order varies run to run, so the bound is FP32 rounding, not equality."""
+    _log("mixed graph replay vs direct launch, 140 steps, 6 slices x 512")
+    recs = []
+    for graphs in (1, 0):
+        sim = H.make_state([512] * 6, ngrid=63, gridmax=0.3e-3, longrange=True,
+                           sc=(1, 0, 100, 0.0))
+        ctx = H.to_gpu(sim, precision=g4mi.FP32)
+        ctx.set_schedule("graphs", graphs)
+        out = {"power": [], "bunching": []}
+        for k in range(0, 140, 35):  # eager work and synchronisation between runs
+            r = ctx.run(H.gsteps(st[k:k + 35]), cases.GAMMA0, record=("power", "bunching"))
+            ctx.sync()
+            out["power"].append(r["power"])
+            out["bunching"].append(r["bunching"])
+        sched = ctx.schedule()
+        recs.append({k: np.concatenate(v) for k, v in out.items()})
+        ctx.close()
+        _log(f"  graphs={graphs}: {sched}")
+    for k in ("power", "bunching"):
+        a, b = recs[0][k], recs[1][k]
+        e = float(np.max(np.abs(a - b)) / np.max(np.abs(b)))
+        _log(f"  {k}: max rel diff graph vs direct {e:.3e}")
+        assert e < 1e-4
+
+
+def test_mixed_upload_download_and_sort_index():
+    """Offset form round trip: gamma and theta come back to FP32 rounding of
+    their offsets; the sort target slice equals floor(theta / slen) of the
+    downloaded theta."""
+    sim = H.make_state([300, 16, 516, 64])
+    ctx = H.to_gpu(sim, precision=g4mi.FP32)
+    b = ctx.download_beam()
+    assert np.max(np.abs(b["gamma"] - sim.gamma)) <= 1e-3
+    assert np.max(np.abs(b["theta"] - sim.theta)) <= 1e-6
+    for k in ("x", "y", "px", "py"):
+        ref = getattr(sim, k)
+        assert np.max(np.abs(b[k] - ref) / (np.abs(ref) + 1e-300)) <= 6e-8
+    slen = 2 * np.pi
+    atar, moved = ctx.sort_index(slen)
+    want = np.floor(b["theta"] * (1.0 / slen)).astype(int)
+    assert np.array_equal(atar, want)
+    assert moved == int(np.count_nonzero(want))
+    ctx.close()
+
+
+def test_mixed_rejects_split_calls(st):
+    sim = H.make_state([64] * 2)
+    ctx = H.to_gpu(sim, precision=g4mi.FP32)
+    with pytest.raises(g4mi.G4miError):
+        ctx.beam_track(H.gsteps(st)[0], cases.GAMMA0, st[0].dz)
+    ctx.close()
