@@ -57,7 +57,8 @@ def make_context(cfg, off, beam, field, current, precision):
     return ctx
 
 
-def gpu_measure(cfg, steps_k, warmup, adi_kernel=-1, precision=g4mi.FP64, breakdown=20, refine=1):
+def gpu_measure(cfg, steps_k, warmup, adi_kernel=-1, precision=g4mi.FP64, breakdown=20, refine=0,
+                mixed_adi=1):
     t_load = time.perf_counter()
     off, beam, field, current, steps = build_workload(cfg)
     ctx = make_context(cfg, off, beam, field, current, precision)
@@ -67,6 +68,7 @@ def gpu_measure(cfg, steps_k, warmup, adi_kernel=-1, precision=g4mi.FP64, breakd
         ctx.set_schedule("adi_kernel", adi_kernel)
     if precision == g4mi.FP32:
         ctx.set_schedule("adi_refine", refine)
+        ctx.set_schedule("mixed_adi", mixed_adi)
     nst = len(steps)
     extra = breakdown if precision == g4mi.FP32 else 0
     if warmup + steps_k + extra > nst:
@@ -192,8 +194,10 @@ def main():
     ap.add_argument("--adi-kernel", type=int, default=-1,
                     help="fix the ADI kernel (0 Thomas, 1 fused, 2 fused+prefetch, 3 banded); "
                          "default: autotuned")
-    ap.add_argument("--adi-refine", type=int, default=1, choices=(0, 1),
-                    help="mixed form: FP64 residual correction in the ADI (default on)")
+    ap.add_argument("--adi-refine", type=int, default=0, choices=(0, 1),
+                    help="mixed form: FP64 residual correction in the ADI (default off)")
+    ap.add_argument("--mixed-adi", type=int, default=1, choices=(0, 1),
+                    help="mixed form: 0 tile ADI kernel, 1 streaming ADI kernel")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB five times (the reference "
                          "point that scales the FETCH_SIZE/WRITE_SIZE counters) and exit")
@@ -209,7 +213,8 @@ def main():
     if a.precision:
         cfg["precision"] = a.precision
     prec = g4mi.FP32 if cfg["precision"] == "fp32" else g4mi.FP64
-    m = gpu_measure(cfg, a.steps, a.warmup, a.adi_kernel, prec, refine=a.adi_refine)
+    m = gpu_measure(cfg, a.steps, a.warmup, a.adi_kernel, prec, refine=a.adi_refine,
+                    mixed_adi=a.mixed_adi)
     rate = m["ntot"] * a.steps / m["seconds"]
     cells = cfg["nslice"] * cfg["ngrid"] ** 2
     kname, roof = roofline(cfg, m["kt"], m["ntot"], prec)

@@ -101,9 +101,10 @@ struct Ctx {
     std::vector<MixBand> h_binfo;
     std::map<double, int> band_of_dz;      // step size -> band index
     std::vector<int> band_tmpl;            // template half width of each band
-    bool m_refine = true;                  // FP64 residual correction in the ADI
+    bool m_refine = false;                 // FP64 residual correction in the ADI (schedule "adi_refine")
     bool m_graphs = true;                  // replay captured step graphs
-    std::map<std::tuple<int, int, int, int, int>, hipGraphExec_t> m_graph;
+    bool m_stream = true;                  // streaming ADI (schedule "mixed_adi" 1) or the tile kernel (0)
+    std::map<std::tuple<int, int, int, int, int, int>, hipGraphExec_t> m_graph;
     std::vector<double> m_rec_host;
 };
 
@@ -731,12 +732,12 @@ static int mix_fill_step(Ctx *c, const g4mi_step_params &st, double gamma0, MixS
     return G4MI_OK;
 }
 
-typedef std::tuple<int, int, int, int, int> MixKey;
+typedef std::tuple<int, int, int, int, int, int> MixKey;  // band, refine, sc, moments, maxzero, stream
 
 static int mix_launch(Ctx *c, const MixKey &k, hipEvent_t *ev) {
     const MixArrays a = mix_arrays(c);
     if (launch_mix_step(c->stream, a, std::get<0>(k), std::get<1>(k) != 0, std::get<2>(k) != 0,
-                        std::get<3>(k) != 0, std::get<4>(k), ev) != 0) {
+                        std::get<3>(k) != 0, std::get<4>(k), ev, std::get<5>(k) != 0) != 0) {
         set_error("mixed form: no ADI instantiation for this band width");
         return G4MI_E_ARG;
     }
@@ -811,7 +812,7 @@ static int mix_run(Ctx *c, const g4mi_step_params *steps, int nsteps, double gam
         G4_TRY(mix_fill_step(c, steps[i], gamma0, tab[i]));
         const int tm = c->band_tmpl[tab[i].band];
         const int refine = (c->m_refine && tm <= 9) ? 1 : 0;
-        keys[i] = MixKey(tm, refine, sc ? 1 : 0, moments ? 1 : 0, maxzero);
+        keys[i] = MixKey(tm, refine, sc ? 1 : 0, moments ? 1 : 0, maxzero, c->m_stream ? 1 : 0);
     }
     if (n > c->m_steps_cap) {
         if (c->m_steps) (void)hipFree(c->m_steps);
@@ -1291,6 +1292,10 @@ int g4mi_set_schedule(g4mi_ctx *h, const char *name, int value) {
         c->m_refine = value != 0;
         return G4MI_OK;
     }
+    if (std::string(name) == "mixed_adi" && (value == 0 || value == 1)) {
+        c->m_stream = value == 1;
+        return G4MI_OK;
+    }
     if (std::string(name) == "graphs" && (value == 0 || value == 1)) {
         c->m_graphs = value != 0;
         return G4MI_OK;
@@ -1310,9 +1315,10 @@ int g4mi_get_schedule(g4mi_ctx *h, char *buf, int len) {
             bands += (k ? ", " : "") + std::to_string(c->h_binfo[k].m) + "/" + std::to_string(c->band_tmpl[k]);
         bands += "]";
         snprintf(tmp, sizeof(tmp),
-                 "{\"precision\": \"fp32\", \"adi\": \"band32\", \"adi_refine\": %d, "
+                 "{\"precision\": \"fp32\", \"adi\": \"%s\", \"adi_refine\": %d, "
                  "\"graphs\": %d, \"bands_m_over_template\": \"%s\", \"graphs_captured\": %d}",
-                 c->m_refine ? 1 : 0, c->m_graphs ? 1 : 0, bands.c_str(), (int)c->m_graph.size());
+                 (c->m_stream && !c->m_refine) ? "band32-stream" : "band32-tile", c->m_refine ? 1 : 0,
+                 c->m_graphs ? 1 : 0, bands.c_str(), (int)c->m_graph.size());
         snprintf(buf, (size_t)len, "%s", tmp);
         return G4MI_OK;
     }

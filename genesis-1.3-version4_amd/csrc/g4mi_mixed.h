@@ -105,8 +105,10 @@ size_t mix_band_lds(int tmpl_m, bool refine);
 // slippage, zeroing and the step record.  ev (optional, 9 events) brackets
 // the kernel groups: front [0,1], main [3 - 2], ADI [6,7], slippage + record
 // [7,8]; the other intervals are empty.
+// stream: the streaming ADI (one wavefront per column strip) instead of
+// the 32x32 tile kernel (no residual correction only).
 int launch_mix_step(hipStream_t s, const MixArrays &a, int tmpl_m, bool refine, bool sc,
-                    bool moments, int maxzero, hipEvent_t *ev);
+                    bool moments, int maxzero, hipEvent_t *ev, bool stream);
 // Standalone sums for a diagnostics call outside a run:
 // beam [ns][8] and field per storage slice [ns][8].
 void launch_mix_diag(hipStream_t s, const MixArrays &a, double *beam8, double *field8);

@@ -17,6 +17,7 @@
 //
 // Per-step parameters come from the device step table steps[st->istep], so
 // a captured graph replays any step.
+#include <algorithm>
 #include <type_traits>
 
 #include "g4mi_mixed.h"
@@ -255,7 +256,7 @@ __global__ void __launch_bounds__(64) k_mix_scan(MixArrays a) {
 // ---------------------------------------------------------------------------
 constexpr int kMainNT = 384;
 constexpr int kScMax = 256;
-constexpr int kMixDyn = 8192;  // float2 cells of the dynamic region (64 KiB)
+constexpr int kMixDyn = 8704;  // float2 cells of the dynamic region (68 KiB)
 
 struct OdeMix {
     float xks, eps_ref, bref, gref, ztemp1, ez;
@@ -367,7 +368,7 @@ __device__ __forceinline__ void bmin_w(int (&v)[K], int *lds) {
 
 // LDS accumulation is FP64 (ds_add_f64): on gfx950 FP32 LDS atomics
 // (ds_add_f32) issue far slower than FP64 ones, and the sums gain precision.
-constexpr int kCelmCells = 2048;                          // space-charge mode solutions (float2)
+constexpr int kCelmCells = 1024;                          // space-charge mode solutions (float2)
 constexpr int kTileCells = (kMixDyn - kCelmCells) / 2;    // double2 cells: histograms, then the deposit tile
 
 __global__ void __launch_bounds__(kMainNT)
@@ -1050,6 +1051,173 @@ __global__ void __launch_bounds__(256) k_mix_adi(MixArrays a) {
 }
 
 // ---------------------------------------------------------------------------
+// Streaming ADI of the mixed form (FP32, no residual correction): both half
+// steps of FieldSolverADI::ADI (:60-91) with the same FP32 banded inverse as
+// the tile kernel, organised as one wavefront per strip of columns walking
+// down the rows of a slice.  Lane l owns column x = x0 - M + l; lanes
+// [M, 64-M) produce output.  For each row r:
+//   E(r) = S + u + i a (u(r+1) - 2 u(r) + u(r-1))     (field rows prefetched)
+//   u'(r) = sum_m B[x][m] E(r, x+m-M)                   (E row through LDS)
+//   F(r) = S + 2 u'(r) - E(r)                            (kept in a register ring)
+//   u''(r-M) = sum_m B[r-M][m] F(r-2M+m)                 (ring of 2M+1 rows)
+// Each field value is read once (plus the 2M-column halo of the strip) and
+// written once.  The x band rows are per lane (registers); the y band rows
+// are uniform per row (scalar loads).
+// ---------------------------------------------------------------------------
+template <int M>
+struct StreamGeom {
+    static constexpr int W = 2 * M + 1;
+    static constexpr int SO = 64 - 2 * M;  // output columns per strip
+    static constexpr int PF = 3;           // field rows prefetched ahead
+};
+
+int stream_strips(int ng, int m) { return (ng + (64 - 2 * m) - 1) / (64 - 2 * m); }
+
+// a row of zeros the streaming kernel reads for source rows outside the box
+__device__ float2 g_zero_row[4096];
+
+template <int M>
+__global__ void __launch_bounds__(64) k_mix_adi_stream(MixArrays a) {
+    using G = StreamGeom<M>;
+    constexpr int W = G::W, SO = G::SO, PF = G::PF;
+    __shared__ float2 lrow[2][64];
+    const int lane = threadIdx.x;
+    const MixStep &S = a.steps[a.st->istep];
+    const MixBand bi = a.binfo[S.band];
+    const float2 *band = a.bands + bi.off;
+    const float af = (float)bi.a;
+    const int ng = a.ng, ns = a.ns;
+    const int nstrip = (ng + SO - 1) / SO;
+    const int b = blockIdx.x;
+    const int ii = (b / (8 * nstrip)) * 8 + (b & 7);   // XCD-aware: a slice's strips on one XCD
+    const int strip = (b % (8 * nstrip)) >> 3;
+    if (ii >= ns) return;
+    const int x = strip * SO - M + lane;
+    const bool xin = x >= 0 && x < ng;
+    const bool outl = lane >= M && lane < 64 - M && xin;
+    const long long fs = (long long)((ii + a.st->first) % ns);
+    const int cur = a.st->cur;
+    const float2 *u = a.field[cur] + fs * ng * ng;
+    float2 *uo = a.field[cur ^ 1] + fs * ng * ng;
+    const float2 *Sp = a.src + (long long)ii * ng * ng;
+    const int4 bx = a.box[ii];
+    const bool sxin = x >= bx.x && x <= bx.z;
+    // this column's band row
+    float2 bxr[W];
+#pragma unroll
+    for (int m = 0; m < W; m++) bxr[m] = xin ? band[(long long)x * W + m] : f2(0, 0);
+    const float2 *bc = band + (long long)((bi.klo + bi.khi) / 2) * W;
+    float2 bcr[W];  // the shared interior row of the y band, held in registers
+#pragma unroll
+    for (int m = 0; m < W; m++) bcr[m] = bc[m];
+    // field rows q[k] = u(r - 1 + k) and source rows sq[k] = S(r + k - 1),
+    // k = 0..PF+1, prefetched in the order they are consumed (the memory
+    // counter retires loads in issue order)
+    // Loads are unconditional from clamped (valid) addresses so they stay in
+    // flight; the grid and source-box masks are applied when a row is used.
+    const int xcl = min(max(x, 0), ng - 1);
+    const float2 *ucol = u + xcl, *scol = Sp + xcl;
+    auto uload = [&](int rr) -> float2 { return ucol[(long long)min(rr, ng - 1) * ng]; };
+    // source rows outside the box read the zero row instead (no HBM traffic)
+    auto sload = [&](int rr) -> float2 {
+        const bool in = sxin && rr >= bx.y && rr <= bx.w;
+        return *(in ? scol + (long long)rr * ng : g_zero_row + xcl);
+    };
+    // queues of QN = PF+2 rows, indexed modulo QN: the row loop is unrolled
+    // by QN so every slot is a fixed register (no register moves of loads in
+    // flight).  Slot (r + 1 + k) % QN holds row r + k, k = -1..PF.
+    constexpr int QN = PF + 2;
+    float2 q[QN], sq[QN];
+    q[0] = f2(0, 0);
+    sq[0] = f2(0, 0);
+#pragma unroll
+    for (int k = 1; k < QN; k++) {
+        q[k] = uload(k - 1);
+        sq[k] = sload(k - 1);
+    }
+    float2 ring[W];
+#pragma unroll
+    for (int m = 0; m < W; m++) ring[m] = f2(0, 0);
+    const double shift = -0.5 * (double)(ng - 1);
+    const int icen = (ng / 2) * ng + (ng / 2);
+    double dg[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    const int nrow = ng + M;
+    for (int r0 = 0; r0 < nrow; r0 += QN) {
+#pragma unroll
+        for (int j = 0; j < QN; j++) {
+            const int r = r0 + j;
+            if (r >= nrow) break;
+            // slots: row r-1 -> j, row r -> (j+1)%QN, row r+1 -> (j+2)%QN
+            float2 Fr = f2(0, 0);
+            if (r < ng) {
+                const float2 sv = sq[(j + 1) % QN];
+                const float2 uc = q[(j + 1) % QN];
+                const float2 un = (r + 1 < ng) ? q[(j + 2) % QN] : f2(0, 0);
+                const float2 ul = (r > 0) ? q[j] : f2(0, 0);
+                const float sx = un.x - 2.f * uc.x + ul.x, sy = un.y - 2.f * uc.y + ul.y;
+                const float2 E = xin ? f2(sv.x + uc.x - af * sy, sv.y + uc.y + af * sx) : f2(0, 0);
+                float2 *lr = lrow[j & 1];
+                lr[lane] = E;
+                // one wavefront per workgroup: LDS operations of a wave complete in
+                // issue order, so the row is visible to the reads below without a
+                // hardware barrier; a workgroup barrier would also wait for the
+                // prefetched field rows (its fence drains vmcnt).  Compiler ordering only.
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                __builtin_amdgcn_wave_barrier();
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+                float2 up = f2(0, 0);
+                if (lane >= M && lane < 64 - M) {
+#pragma unroll
+                    for (int m = 0; m < W; m++) up = ffma(bxr[m], lr[lane + m - M], up);
+                }
+                Fr = xin ? f2(sv.x + 2.f * up.x - E.x, sv.y + 2.f * up.y - E.y) : f2(0, 0);
+                // slot j (row r-1) is free: it takes row r + PF + 1, first used
+                // PF rows later
+                q[j] = uload(r + PF + 1);
+                sq[j] = sload(r + PF + 1);
+            }
+#pragma unroll
+            for (int m = 0; m < W - 1; m++) ring[m] = ring[m + 1];
+            ring[W - 1] = Fr;
+            const int y = r - M;
+            if (y >= 0) {
+                float2 o = f2(0, 0);
+                if (y >= bi.klo && y <= bi.khi) {
+#pragma unroll
+                    for (int m = 0; m < W; m++) o = ffma(bcr[m], ring[m], o);
+                } else {
+                    const float2 *by = band + (long long)y * W;
+#pragma unroll
+                    for (int m = 0; m < W; m++) o = ffma(by[m], ring[m], o);
+                }
+                if (outl) {
+                    uo[(long long)y * ng + x] = o;
+                    const double ox = o.x, oy = o.y;
+                    const double wei = ox * ox + oy * oy;
+                    const double dx = (double)x + shift, dy = (double)y + shift;
+                    dg[0] += wei;
+                    dg[1] += dx * wei;
+                    dg[2] += dx * dx * wei;
+                    dg[3] += dy * wei;
+                    dg[4] += dy * dy * wei;
+                    dg[5] += ox;
+                    dg[6] += oy;
+                    if (y * ng + x == icen) dg[7] = wei;
+                }
+            }
+        }
+    }
+    // strip sums (wave butterfly, fixed order)
+#pragma unroll
+    for (int k = 0; k < 8; k++)
+        for (int o = 32; o > 0; o >>= 1) dg[k] += __shfl_xor(dg[k], o);
+    if (lane == 0) {
+        double *dst = a.tilediag + (fs * nstrip + strip) * 8;
+        for (int k = 0; k < 8; k++) dst[k] = dg[k];
+    }
+}
+
+// ---------------------------------------------------------------------------
 // slippage on the device (Control::applySlippage, Control.cpp:158-276): the
 // ring offset, the slices to zero, the live field buffer and the step index.
 // ---------------------------------------------------------------------------
@@ -1088,14 +1256,13 @@ __global__ void __launch_bounds__(256) k_mix_zero(MixArrays a) {
 
 // Record of the completed step per beam slice: beam sums and the field sums
 // of its storage slice (zero for slices slippage has just cleared).
-__global__ void __launch_bounds__(64) k_mix_record(MixArrays a) {
+__global__ void __launch_bounds__(64) k_mix_record(MixArrays a, int ntiles) {
     const int is = blockIdx.x * 64 + threadIdx.x;
     if (is >= a.ns) return;
     const MixState &st = *a.st;
     const int fs = (is + st.first) % a.ns;
     bool zeroed = false;
     for (int k = 0; k < st.nzero; k++) zeroed |= (st.zero[k] == fs);
-    const int ntx = (a.ng + 31) / 32, ntiles = ntx * ntx;
     double f[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     if (!zeroed)
         for (int k = 0; k < ntiles; k++)
@@ -1211,7 +1378,8 @@ bool mix_band_supported(int m, int *tmpl_m) {
 
 int mix_band_tiles(int ng) {
     const int ntx = (ng + 31) / 32;
-    return ntx * ntx;
+    // the tile-diagnostic scratch also holds the strip sums of the streaming kernel
+    return std::max(ntx * ntx, stream_strips(ng, 12));
 }
 
 size_t mix_band_lds(int tmpl_m, bool refine) {
@@ -1234,7 +1402,7 @@ int mix_prepare() {
 int mix_sc_mode_cells() { return kCelmCells; }
 
 int launch_mix_step(hipStream_t s, const MixArrays &a, int tmpl_m, bool refine, bool sc,
-                    bool moments, int maxzero, hipEvent_t *ev) {
+                    bool moments, int maxzero, hipEvent_t *ev, bool stream) {
     auto mark = [&](int k) {
         if (ev) (void)hipEventRecord(ev[k], s);
     };
@@ -1255,7 +1423,18 @@ int launch_mix_step(hipStream_t s, const MixArrays &a, int tmpl_m, bool refine, 
         if (refine) launch_adi_t<MM, true>(s, a);        \
         else launch_adi_t<MM, false>(s, a);              \
     }
-    if (false) {
+    int ntiles = mix_band_tiles(a.ng);
+    if (!refine && stream) {
+        // streaming kernel: one wavefront per strip
+        ntiles = stream_strips(a.ng, tmpl_m);
+        const unsigned nblocks = (unsigned)(((a.ns + 7) / 8) * 8) * (unsigned)ntiles;
+#define G4_S(MM) else if (tmpl_m == MM) hipLaunchKernelGGL(k_mix_adi_stream<MM>, dim3(nblocks), dim3(64), 0, s, a);
+        if (false) {
+        }
+        G4_S(3) G4_S(4) G4_S(6) G4_S(9) G4_S(12)
+        else return -1;
+#undef G4_S
+    } else if (false) {
     }
     G4_A(3) G4_A(4) G4_A(6) G4_A(9) G4_A(12)
     else return -1;
@@ -1267,7 +1446,7 @@ int launch_mix_step(hipStream_t s, const MixArrays &a, int tmpl_m, bool refine, 
         hipLaunchKernelGGL(k_mix_zero, dim3((unsigned)((cells + 255) / 256), (unsigned)maxzero),
                            dim3(256), 0, s, a);
     }
-    hipLaunchKernelGGL(k_mix_record, dim3((a.ns + 63) / 64), dim3(64), 0, s, a);
+    hipLaunchKernelGGL(k_mix_record, dim3((a.ns + 63) / 64), dim3(64), 0, s, a, ntiles);
     mark(8);
     return 0;
 }
