@@ -17,6 +17,7 @@
 
 #include "g4mi_host.h"
 #include "g4mi_kernels.h"
+#include "g4mi_fno_api.h"
 #include "g4mi_mixed.h"
 
 namespace g4mi {
@@ -89,7 +90,7 @@ struct Ctx {
     double *m_base = nullptr;              // [ns] FP64 phase base
     int4 *m_pbox = nullptr;                // [ns] footprint after the first half step
     float2 *m_field[2] = {nullptr, nullptr}, *m_src = nullptr;
-    double *m_beamsum = nullptr, *m_tilediag = nullptr, *m_rec = nullptr;
+    double *m_beamsum = nullptr, *m_egsum = nullptr, *m_tilediag = nullptr, *m_rec = nullptr;
     int m_nrec = 0;
     MixStep *m_steps = nullptr;            // device step table
     int m_steps_cap = 0;
@@ -106,6 +107,13 @@ struct Ctx {
     bool m_stream = true;                  // streaming ADI (schedule "mixed_adi" 1) or the tile kernel (0)
     std::map<std::tuple<int, int, int, int, int, int>, hipGraphExec_t> m_graph;
     std::vector<double> m_rec_host;
+    std::vector<double> h_cur;             // currents [ns] (host copy)
+    // learned field step (mixed form): 0 detached, 1 capture, 2 replace the ADI
+    Fno *fno = nullptr;
+    int fno_mode = 0, fno_every = 1;
+    double fno_thr = 1e-3;
+    long long fno_steps = 0, fno_adi_steps = 0, fno_nres_adi = 0;
+    double fno_res_sum = 0, fno_res_adi_sum = 0;
 };
 
 template <class T>
@@ -122,11 +130,6 @@ static int dalloc(Ctx *c, T **p, size_t n) {
     return G4MI_OK;
 }
 
-#define G4_TRY(x)                \
-    do {                         \
-        int r_ = (x);            \
-        if (r_ != G4MI_OK) return r_; \
-    } while (0)
 
 static TransParams transverse_params(const Ctx *c, const g4mi_step_params &st, double gamma0,
                                      double delz, bool last) {
@@ -582,6 +585,7 @@ static MixArrays mix_arrays(const Ctx *c) {
     a.esc = c->d_esc;
     a.efield = c->d_efield;
     a.beamsum = c->m_beamsum;
+    a.egsum = c->m_egsum;
     a.tilediag = c->m_tilediag;
     a.rec = c->m_rec;
     a.nrec = c->m_nrec;
@@ -781,6 +785,99 @@ static void mix_finish_row(const Ctx *c, const double *row, g4mi_diag_out *o) {
     finish_diag(c, hb, hf, 0, o);
 }
 
+// One mixed step with the learned field step attached (direct launches):
+// capture mode appends the ADI's (E, S, E_next) pairs of every fno_every-th
+// slice to the network's training set; replace mode advances the field with
+// the network and runs the ADI instead on every fno_every-th step and when
+// the energy-balance residual of the network's step exceeds fno_thr.
+static int mix_step_fno(Ctx *c, const MixKey &k) {
+    const MixArrays a = mix_arrays(c);
+    const int tm = std::get<0>(k), maxzero = std::get<4>(k);
+    const bool refine = std::get<1>(k) != 0, sc = std::get<2>(k) != 0, moments = std::get<3>(k) != 0;
+    const bool stream = std::get<5>(k) != 0;
+    const int ns = c->ns;
+    const long long P = (long long)c->ng * c->ng;
+    int ntiles = 0;
+    MixState hs{};
+    if (c->fno_mode == 1) {
+        launch_mix_beam(c->stream, a, sc, moments, nullptr);
+        if (launch_mix_adi(c->stream, a, tm, refine, stream, &ntiles)) {
+            set_error("mixed form: no ADI instantiation for this band width");
+            return G4MI_E_ARG;
+        }
+        std::vector<int4> box(ns);
+        G4_HIP(hipMemcpyAsync(&hs, c->m_st, sizeof(MixState), hipMemcpyDeviceToHost, c->stream));
+        G4_HIP(hipMemcpyAsync(box.data(), c->d_box, ns * sizeof(int4), hipMemcpyDeviceToHost, c->stream));
+        G4_HIP(hipStreamSynchronize(c->stream));
+        for (int ii = 0; ii < ns; ii += c->fno_every) {
+            const long long fs = (ii + hs.first) % ns;
+            if (fno_capture(c->fno, c->stream, c->m_field[hs.cur] + fs * P, c->m_src + (long long)ii * P,
+                            c->m_field[hs.cur ^ 1] + fs * P, box[ii]) != G4MI_OK)
+                break;  // training set full
+        }
+        launch_mix_tail(c->stream, a, maxzero, ntiles, nullptr);
+        G4_HIP(hipGetLastError());
+        return G4MI_OK;
+    }
+    // replace mode
+    const double ks = 4. * asin(1) / c->xlambda;
+    const double scl = c->dgrid * kEev / ks, pw = scl * scl / kVacimp;
+    std::vector<double> hf0(8 * (size_t)ns), hf1(8 * (size_t)ns), eg(ns), ss(ns);
+    launch_mix_field_sums(c->stream, a, 0, c->d_diag_f);
+    G4_HIP(hipMemcpyAsync(hf0.data(), c->d_diag_f, hf0.size() * 8, hipMemcpyDeviceToHost, c->stream));
+    launch_mix_beam(c->stream, a, sc, moments, nullptr);
+    G4_HIP(hipMemcpyAsync(&hs, c->m_st, sizeof(MixState), hipMemcpyDeviceToHost, c->stream));
+    G4_HIP(hipStreamSynchronize(c->stream));
+    G4_TRY(fno_infer_slices(c->fno, c->stream, c->m_field[hs.cur], c->m_field[hs.cur ^ 1], c->m_src, c->d_box, ns,
+                            hs.first));
+    launch_mix_field_sums(c->stream, a, 1, c->d_diag_f);
+    G4_HIP(hipMemcpyAsync(hf1.data(), c->d_diag_f, hf1.size() * 8, hipMemcpyDeviceToHost, c->stream));
+    G4_HIP(hipMemcpyAsync(eg.data(), c->m_egsum, eg.size() * 8, hipMemcpyDeviceToHost, c->stream));
+    launch_mix_src_sums(c->stream, a, c->d_diag_b);
+    G4_HIP(hipMemcpyAsync(ss.data(), c->d_diag_b, ss.size() * 8, hipMemcpyDeviceToHost, c->stream));
+    G4_HIP(hipStreamSynchronize(c->stream));
+    // particle-to-field energy balance of the step.  A step's field change is
+    // dP = 2 Re<U E, V S> + |V S|^2 (U unitary, V ~ 2 to first order in the
+    // step): the beam's loss sum_s I_s eev d<gamma>_s (from the push's own
+    // increments, free of the storage rounding) balances the cross term, and
+    // the source's self-energy |2 S|^2 is subtracted.
+    double p0 = 0, p1 = 0, dpb = 0, pself = 0;
+    for (int i = 0; i < ns; i++) {
+        p0 += hf0[8 * (size_t)i] * pw;
+        p1 += hf1[8 * (size_t)i] * pw;
+        pself += 4.0 * ss[i] * pw;
+        const long long n = c->off[i + 1] - c->off[i];
+        if (n > 0) dpb += c->h_cur[i] * kEev * eg[i] / (double)n;
+    }
+    const double res = std::fabs(p1 - p0 + dpb - pself) / std::max(p1, 1e-300);
+    const long long stepno = c->fno_steps + c->fno_adi_steps;
+    const bool use_adi = (c->fno_every > 0 && stepno % c->fno_every == 0) || !(res <= c->fno_thr);
+    if (use_adi) {
+        if (launch_mix_adi(c->stream, a, tm, refine, stream, &ntiles)) {
+            set_error("mixed form: no ADI instantiation for this band width");
+            return G4MI_E_ARG;
+        }
+        // the ADI's own residual, for reference
+        std::vector<double> hfa(8 * (size_t)ns);
+        launch_mix_field_sums(c->stream, a, 1, c->d_diag_f);
+        G4_HIP(hipMemcpyAsync(hfa.data(), c->d_diag_f, hfa.size() * 8, hipMemcpyDeviceToHost, c->stream));
+        G4_HIP(hipStreamSynchronize(c->stream));
+        double pa = 0;
+        for (int i = 0; i < ns; i++) pa += hfa[8 * (size_t)i] * pw;
+        c->fno_res_adi_sum += std::fabs(pa - p0 + dpb - pself) / std::max(pa, 1e-300);
+        c->fno_nres_adi++;
+        c->fno_adi_steps++;
+    } else {
+        launch_mix_field_sums(c->stream, a, 1, c->m_tilediag);
+        ntiles = 1;
+        c->fno_steps++;
+        c->fno_res_sum += res;
+    }
+    launch_mix_tail(c->stream, a, maxzero, ntiles, nullptr);
+    G4_HIP(hipGetLastError());
+    return G4MI_OK;
+}
+
 static int mix_run(Ctx *c, const g4mi_step_params *steps, int nsteps, double gamma0,
                    g4mi_diag_out *records, int *done) {
     *done = 0;
@@ -830,7 +927,9 @@ static int mix_run(Ctx *c, const g4mi_step_params *steps, int nsteps, double gam
     const size_t rowd = (size_t)c->ns * kMixRecord;
     int pending = 0;  // first step whose record is not yet copied back
     for (int i = 0; i < n; i++) {
-        if (c->timing) {
+        if (c->fno && c->fno_mode) {
+            G4_TRY(mix_step_fno(c, keys[i]));
+        } else if (c->timing) {
             // timed step: direct launches with the kernel groups bracketed by events
             G4_TRY(step_begin_timing(c));
             G4_TRY(mix_launch(c, keys[i], c->ev));
@@ -918,6 +1017,7 @@ static int mix_beam_upload(Ctx *c, const double *g, const double *th, const doub
     G4_HIP(hipMemcpyAsync(c->m_base, base.data(), ns * sizeof(double), hipMemcpyHostToDevice, c->stream));
     G4_HIP(hipMemcpyAsync(c->d_cur, cur, ns * sizeof(double), hipMemcpyHostToDevice, c->stream));
     G4_HIP(hipStreamSynchronize(c->stream));
+    c->h_cur.assign(cur, cur + ns);
     mix_clear_graphs(c);  // gref is part of every captured launch
     c->has_beam = true;
     return G4MI_OK;
@@ -993,7 +1093,7 @@ static int mix_alloc(Ctx *c) {
         (rc = dalloc(c, &c->m_base, ns)) || (rc = dalloc(c, &c->m_pbox, ns)) ||
         (rc = dalloc(c, &c->m_field[0], ns * cells)) ||
         (rc = dalloc(c, &c->m_field[1], ns * cells)) || (rc = dalloc(c, &c->m_src, ns * cells)) ||
-        (rc = dalloc(c, &c->m_beamsum, 8 * ns)) ||
+        (rc = dalloc(c, &c->m_beamsum, 8 * ns)) || (rc = dalloc(c, &c->m_egsum, ns)) ||
         (rc = dalloc(c, &c->m_tilediag, ns * mix_band_tiles(c->ng) * 8)) ||
         (rc = dalloc(c, &c->m_rec, (size_t)kMixRecRows * ns * kMixRecord)) ||
         (rc = dalloc(c, &c->m_st, 1)) ||
@@ -1328,6 +1428,40 @@ int g4mi_get_schedule(g4mi_ctx *h, char *buf, int len) {
              c->adi_kernel, c->adi_kernel >= 0 ? names[c->adi_kernel] : "untuned", c->adi_tile,
              c->adi_band_m, c->adi_ms[0], c->adi_ms[1], c->adi_ms[2], c->adi_ms[3]);
     snprintf(buf, (size_t)len, "%s", tmp);
+    return G4MI_OK;
+}
+
+int g4mi_attach_fno(g4mi_ctx *h, g4mi_fno *fh, int mode, int every, double threshold) {
+    if (!h || mode < 0 || mode > 2 || (mode && (!fh || every < 1 || !(threshold >= 0)))) {
+        set_error("g4mi_attach_fno: invalid argument");
+        return G4MI_E_ARG;
+    }
+    Ctx *c = &h->c;
+    if (mode && c->precision != G4MI_FP32) {
+        set_error("g4mi_attach_fno: the learned field step needs a mixed-form (G4MI_FP32) context");
+        return G4MI_E_STATE;
+    }
+    if (mode && fno_ngrid(fno_of(fh)) != c->ng) {
+        set_error("g4mi_attach_fno: network grid differs from the context's");
+        return G4MI_E_ARG;
+    }
+    c->fno = mode ? fno_of(fh) : nullptr;
+    c->fno_mode = mode;
+    c->fno_every = every;
+    c->fno_thr = threshold;
+    c->fno_steps = c->fno_adi_steps = c->fno_nres_adi = 0;
+    c->fno_res_sum = c->fno_res_adi_sum = 0;
+    return G4MI_OK;
+}
+
+int g4mi_fno_stats(g4mi_ctx *h, long long *fno_steps, long long *adi_steps, double *mean_res_fno,
+                   double *mean_res_adi) {
+    if (!h) return G4MI_E_ARG;
+    Ctx *c = &h->c;
+    if (fno_steps) *fno_steps = c->fno_steps;
+    if (adi_steps) *adi_steps = c->fno_adi_steps;
+    if (mean_res_fno) *mean_res_fno = c->fno_steps ? c->fno_res_sum / (double)c->fno_steps : 0.0;
+    if (mean_res_adi) *mean_res_adi = c->fno_nres_adi ? c->fno_res_adi_sum / (double)c->fno_nres_adi : 0.0;
     return G4MI_OK;
 }
 

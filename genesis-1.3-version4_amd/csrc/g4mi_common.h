@@ -28,6 +28,12 @@ void set_error(const std::string &msg);
         }                                                                         \
     } while (0)
 
+#define G4_TRY(x)                     \
+    do {                              \
+        int r_ = (x);                 \
+        if (r_ != G4MI_OK) return r_; \
+    } while (0)
+
 // Parameters of the transverse half-step (TrackBeam::track,
 // src/Core/TrackBeam.cpp:8-81), prepared on the host once per call.
 struct TransParams {

@@ -73,6 +73,7 @@ struct MixArrays {
     // per-slice scratch
     double *mom_fsize, *cen, *rbound, *dr, *rmax, *esc, *efield;
     double *beamsum;                // [ns][8]
+    double *egsum;                  // [ns] sum of the push's gamma increments (before rounding)
     double *tilediag;               // [ns][tiles][8]
     double *rec;                    // [nrec][ns][kMixRecord]
     int nrec;
@@ -109,6 +110,18 @@ size_t mix_band_lds(int tmpl_m, bool refine);
 // the 32x32 tile kernel (no residual correction only).
 int launch_mix_step(hipStream_t s, const MixArrays &a, int tmpl_m, bool refine, bool sc,
                     bool moments, int maxzero, hipEvent_t *ev, bool stream);
+// The same step in three parts: the beam kernels (front, scan, main; ev
+// slots 0-6), the ADI (ntiles: the field-sum blocks per slice it wrote) and
+// the tail (slippage, zeroing, record of ntiles blocks; ev slots 7-8).
+void launch_mix_beam(hipStream_t s, const MixArrays &a, bool sc, bool moments, hipEvent_t *ev);
+int launch_mix_adi(hipStream_t s, const MixArrays &a, int tmpl_m, bool refine, bool stream, int *ntiles);
+void launch_mix_tail(hipStream_t s, const MixArrays &a, int maxzero, int ntiles, hipEvent_t *ev);
+// Field sums per storage slice ([ns][8], as the ADI's diagnostics with one
+// block per slice) of the live buffer (next = 0) or the other one (1).
+void launch_mix_field_sums(hipStream_t s, const MixArrays &a, int next, double *out8);
+// sum of |S|^2 over each beam slice's source box, out[ns]
+void launch_mix_src_sums(hipStream_t s, const MixArrays &a, double *out);
+
 // Standalone sums for a diagnostics call outside a run:
 // beam [ns][8] and field per storage slice [ns][8].
 void launch_mix_diag(hipStream_t s, const MixArrays &a, double *beam8, double *field8);

@@ -375,7 +375,7 @@ __global__ void __launch_bounds__(kMainNT)
 k_mix_main(MixArrays a, int sc_on) {
     extern __shared__ float2 dyn[];
     __shared__ int bred[5 * (kMainNT / 64)];
-    __shared__ double wred[8 * (kMainNT / 64)];
+    __shared__ double wred[9 * (kMainNT / 64)];
     __shared__ float gsol[kScMax], scupp[kScMax], sclow[kScMax], scmid[kScMax];
     __shared__ float2 scrhs[kScMax];
     const MixStep &S = a.steps[a.st->istep];
@@ -534,7 +534,7 @@ k_mix_main(MixArrays a, int sc_on) {
     const float dgx = (float)D.gradx, dgy = (float)D.grady;
     // per-lane beam sums of at most a few dozen particles in FP32, energy as
     // the offset dg; widened to FP64 for the block sum
-    float fv[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    float fv[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};  // [8]: the push's energy increments
     int bx0 = 1 << 30, by0 = 1 << 30, bx1 = -1, by1 = -1, ovf = 0;
     const unsigned stepno = a.st->seq;
     const float inv_dgrid = 1.f / dgrid;
@@ -643,6 +643,7 @@ k_mix_main(MixArrays a, int sc_on) {
         et += stpz * (k3pp + o.kpp / 6.0f);
         const unsigned hs = mix_hash((unsigned long long)i, stepno, 0x51u);
         const float dg = add_sr(g0, eg, hs);
+        fv[8] += eg;
         if (S.chicane) et += r56 * (dg + dgam0);  // TrackBeam::applyR56
         const float dth = add_sr(t0, et, mix_hash((unsigned long long)i, stepno, 0xA7u));
         transverse32(S.t2, gref + dg, x, y, px, py);
@@ -693,16 +694,17 @@ k_mix_main(MixArrays a, int sc_on) {
             }
         }
     }
-    double v[8];
+    double v[9];
 #pragma unroll
-    for (int k = 0; k < 8; k++) v[k] = fv[k];
-    bsum_w<kMainNT, 8>(v, wred);
+    for (int k = 0; k < 9; k++) v[k] = fv[k];
+    bsum_w<kMainNT, 9>(v, wred);
     if (t == 0) {
         // sum gamma = n gref + sum dg, sum gamma^2 = n gref^2 + 2 gref sum dg + sum dg^2
         const double n = (double)npart, gr = a.gref;
         a.beamsum[8 * is + 0] = n * gr + v[0];
         a.beamsum[8 * is + 1] = n * gr * gr + 2.0 * gr * v[0] + v[1];
         for (int k = 2; k < 8; k++) a.beamsum[8 * is + k] = v[k];
+        a.egsum[is] = v[8];
         a.base[is] = basep + S.kappa * P.delz;
     }
     // ---- footprint box and overflow flag
@@ -1298,10 +1300,10 @@ __global__ void __launch_bounds__(256) k_mix_diag_beam(MixArrays a, double *out)
         for (int k = 0; k < 8; k++) out[8 * is + k] = v[k];
 }
 
-__global__ void __launch_bounds__(256) k_mix_diag_field(MixArrays a, double *out) {
+__global__ void __launch_bounds__(256) k_mix_diag_field(MixArrays a, double *out, int next) {
     __shared__ double lds[8 * 256];
     const int is0 = blockIdx.x, ng = a.ng;
-    const float2 *f = a.field[a.st->cur] + (long long)is0 * ng * ng;
+    const float2 *f = a.field[a.st->cur ^ (next ? 1 : 0)] + (long long)is0 * ng * ng;
     const double shift = -0.5 * (double)(ng - 1);
     double v[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     for (int i = threadIdx.x; i < ng * ng; i += 256) {
@@ -1324,6 +1326,29 @@ __global__ void __launch_bounds__(256) k_mix_diag_field(MixArrays a, double *out
         v[7] = cx * cx + cy * cy;
         for (int k = 0; k < 8; k++) out[8 * is0 + k] = v[k];
     }
+}
+
+// Sum of |S|^2 over the source box of each beam slice (the self-energy term
+// of a field step's energy balance).
+__global__ void __launch_bounds__(256) k_mix_src_sums(MixArrays a, double *out) {
+    __shared__ double red[256];
+    const int is = blockIdx.x, ng = a.ng;
+    const int4 bx = a.box[is];
+    const float2 *S = a.src + (long long)is * ng * ng;
+    const int w = bx.z - bx.x + 1, h = bx.w - bx.y + 1;
+    double v = 0;
+    if (w > 0 && h > 0)
+        for (int k = threadIdx.x; k < w * h; k += 256) {
+            const float2 q = S[(bx.y + k / w) * ng + bx.x + k % w];
+            v += (double)q.x * q.x + (double)q.y * q.y;
+        }
+    red[threadIdx.x] = v;
+    __syncthreads();
+    for (int k = 128; k > 0; k >>= 1) {
+        if (threadIdx.x < k) red[threadIdx.x] += red[threadIdx.x + k];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[is] = red[0];
 }
 
 // Sorting::localSort target slice (src/Util/Sorting.cpp:96) on the absolute
@@ -1401,12 +1426,10 @@ int mix_prepare() {
 
 int mix_sc_mode_cells() { return kCelmCells; }
 
-int launch_mix_step(hipStream_t s, const MixArrays &a, int tmpl_m, bool refine, bool sc,
-                    bool moments, int maxzero, hipEvent_t *ev, bool stream) {
+void launch_mix_beam(hipStream_t s, const MixArrays &a, bool sc, bool moments, hipEvent_t *ev) {
     auto mark = [&](int k) {
         if (ev) (void)hipEventRecord(ev[k], s);
     };
-    if (refine && mix_band_lds(tmpl_m, true) > 160 * 1024) return -1;
     mark(0);
     hipLaunchKernelGGL(k_mix_front, dim3(a.ns), dim3(kFrontNT), 0, s, a, moments ? 1 : 0, sc ? 1 : 0);
     if (sc) hipLaunchKernelGGL(k_mix_scan, dim3(1), dim3(64), 0, s, a);
@@ -1418,16 +1441,20 @@ int launch_mix_step(hipStream_t s, const MixArrays &a, int tmpl_m, bool refine, 
     mark(4);
     mark(5);
     mark(6);
+}
+
+int launch_mix_adi(hipStream_t s, const MixArrays &a, int tmpl_m, bool refine, bool stream, int *ntiles) {
+    if (refine && mix_band_lds(tmpl_m, true) > 160 * 1024) return -1;
 #define G4_A(MM)                                         \
     else if (tmpl_m == MM) {                             \
         if (refine) launch_adi_t<MM, true>(s, a);        \
         else launch_adi_t<MM, false>(s, a);              \
     }
-    int ntiles = mix_band_tiles(a.ng);
+    *ntiles = (a.ng + 31) / 32 * ((a.ng + 31) / 32);
     if (!refine && stream) {
         // streaming kernel: one wavefront per strip
-        ntiles = stream_strips(a.ng, tmpl_m);
-        const unsigned nblocks = (unsigned)(((a.ns + 7) / 8) * 8) * (unsigned)ntiles;
+        *ntiles = stream_strips(a.ng, tmpl_m);
+        const unsigned nblocks = (unsigned)(((a.ns + 7) / 8) * 8) * (unsigned)*ntiles;
 #define G4_S(MM) else if (tmpl_m == MM) hipLaunchKernelGGL(k_mix_adi_stream<MM>, dim3(nblocks), dim3(64), 0, s, a);
         if (false) {
         }
@@ -1439,7 +1466,11 @@ int launch_mix_step(hipStream_t s, const MixArrays &a, int tmpl_m, bool refine, 
     G4_A(3) G4_A(4) G4_A(6) G4_A(9) G4_A(12)
     else return -1;
 #undef G4_A
-    mark(7);
+    return 0;
+}
+
+void launch_mix_tail(hipStream_t s, const MixArrays &a, int maxzero, int ntiles, hipEvent_t *ev) {
+    if (ev) (void)hipEventRecord(ev[7], s);
     hipLaunchKernelGGL(k_mix_slip, dim3(1), dim3(64), 0, s, a);
     if (maxzero > 0) {
         const long long cells = (long long)a.ng * a.ng;
@@ -1447,13 +1478,29 @@ int launch_mix_step(hipStream_t s, const MixArrays &a, int tmpl_m, bool refine, 
                            dim3(256), 0, s, a);
     }
     hipLaunchKernelGGL(k_mix_record, dim3((a.ns + 63) / 64), dim3(64), 0, s, a, ntiles);
-    mark(8);
+    if (ev) (void)hipEventRecord(ev[8], s);
+}
+
+int launch_mix_step(hipStream_t s, const MixArrays &a, int tmpl_m, bool refine, bool sc,
+                    bool moments, int maxzero, hipEvent_t *ev, bool stream) {
+    launch_mix_beam(s, a, sc, moments, ev);
+    int ntiles = 0;
+    if (launch_mix_adi(s, a, tmpl_m, refine, stream, &ntiles) != 0) return -1;
+    launch_mix_tail(s, a, maxzero, ntiles, ev);
     return 0;
+}
+
+void launch_mix_src_sums(hipStream_t s, const MixArrays &a, double *out) {
+    hipLaunchKernelGGL(k_mix_src_sums, dim3(a.ns), dim3(256), 0, s, a, out);
+}
+
+void launch_mix_field_sums(hipStream_t s, const MixArrays &a, int next, double *out8) {
+    hipLaunchKernelGGL(k_mix_diag_field, dim3(a.ns), dim3(256), 0, s, a, out8, next);
 }
 
 void launch_mix_diag(hipStream_t s, const MixArrays &a, double *beam8, double *field8) {
     hipLaunchKernelGGL(k_mix_diag_beam, dim3(a.ns), dim3(256), 0, s, a, beam8);
-    hipLaunchKernelGGL(k_mix_diag_field, dim3(a.ns), dim3(256), 0, s, a, field8);
+    hipLaunchKernelGGL(k_mix_diag_field, dim3(a.ns), dim3(256), 0, s, a, field8, 0);
 }
 
 void launch_mix_sort_index(hipStream_t s, const MixArrays &a, double invslen, int *atar,

@@ -91,6 +91,24 @@ def lib():
                                      C.c_int, C.c_int, C.c_double, C.c_double, C.c_int] + [_dp] * 6
         L.g4mi_load_gauss.argtypes = [_dp, C.c_int] + [C.c_double] * 6
         L.g4mi_run_deck.argtypes = [C.c_char_p, C.c_int, C.c_int]
+        _fp = C.POINTER(C.c_float)
+        L.g4mi_fno_create.argtypes = [C.POINTER(C.c_void_p), C.c_int, C.c_int, C.c_int, C.c_int,
+                                      C.c_int, C.c_int, C.c_int, C.c_uint]
+        L.g4mi_fno_destroy.argtypes = [C.c_void_p]
+        L.g4mi_fno_param_count.argtypes = [C.c_void_p]
+        L.g4mi_fno_param_count.restype = C.c_longlong
+        L.g4mi_fno_get_params.argtypes = [C.c_void_p, _fp]
+        L.g4mi_fno_set_params.argtypes = [C.c_void_p, _fp]
+        L.g4mi_fno_forward.argtypes = [C.c_void_p, _fp, _fp, C.c_int]
+        L.g4mi_fno_loss_grad.argtypes = [C.c_void_p, _fp, _fp, C.c_int, _dp, _fp]
+        L.g4mi_fno_reserve.argtypes = [C.c_void_p, C.c_longlong]
+        L.g4mi_fno_pair_count.argtypes = [C.c_void_p]
+        L.g4mi_fno_pair_count.restype = C.c_longlong
+        L.g4mi_fno_train.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_float, C.c_longlong, C.c_uint,
+                                     _dp, _dp]
+        L.g4mi_fno_eval.argtypes = [C.c_void_p, C.c_longlong, C.c_longlong, C.c_int, _dp]
+        L.g4mi_attach_fno.argtypes = [C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_double]
+        L.g4mi_fno_stats.argtypes = [C.c_void_p, _lp, _lp, _dp, _dp]
         _lib = L
     return _lib
 
@@ -255,6 +273,19 @@ class Context:
         _chk(lib().g4mi_get_schedule(self.h, buf, 1024), "g4mi_get_schedule")
         return json.loads(buf.value.decode())
 
+    def attach_fno(self, fno, mode, every=8, threshold=1e-3):
+        """mode 0 detach, 1 capture training pairs, 2 replace the ADI (with fallback)."""
+        _chk(lib().g4mi_attach_fno(self.h, fno.h if fno is not None else None, mode, every,
+                                   threshold), "g4mi_attach_fno")
+
+    def fno_stats(self):
+        a, b = C.c_longlong(), C.c_longlong()
+        r1, r2 = C.c_double(), C.c_double()
+        _chk(lib().g4mi_fno_stats(self.h, C.byref(a), C.byref(b), C.byref(r1), C.byref(r2)),
+             "g4mi_fno_stats")
+        return dict(fno_steps=a.value, adi_steps=b.value, mean_residual_fno=r1.value,
+                    mean_residual_adi=r2.value)
+
     TIMER_NAMES = ("transverse", "space_charge", "push", "deposit", "adi", "slip_diag")
 
     def set_timing(self, on=True):
@@ -267,6 +298,97 @@ class Context:
         _chk(lib().g4mi_kernel_times(self.h, _p(ms), len(ms), C.byref(n)), "g4mi_kernel_times")
         out = dict(zip(self.TIMER_NAMES, ms.tolist()))
         out["steps"] = n.value
+        return out
+
+
+class Fno:
+    """The learned field step (a Fourier neural operator on one slice's grid)."""
+
+    def __init__(self, ngrid, width=32, modes=16, layers=4, hidden=64, max_batch=16, seed=1,
+                 device=0):
+        self.ngrid, self.width, self.modes = ngrid, width, modes
+        self.layers, self.hidden, self.max_batch = layers, hidden, max_batch
+        h = C.c_void_p()
+        _chk(lib().g4mi_fno_create(C.byref(h), device, ngrid, width, modes, layers, hidden,
+                                   max_batch, seed), "g4mi_fno_create")
+        self.h = h
+
+    def close(self):
+        if self.h:
+            lib().g4mi_fno_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def param_count(self):
+        return int(lib().g4mi_fno_param_count(self.h))
+
+    def get_params(self):
+        p = np.zeros(self.param_count(), dtype=np.float32)
+        _chk(lib().g4mi_fno_get_params(self.h, p.ctypes.data_as(C.POINTER(C.c_float))), "get_params")
+        return p
+
+    def set_params(self, p):
+        p = np.ascontiguousarray(p, dtype=np.float32)
+        assert p.size == self.param_count()
+        _chk(lib().g4mi_fno_set_params(self.h, p.ctypes.data_as(C.POINTER(C.c_float))), "set_params")
+
+    def forward(self, inp):
+        inp = np.ascontiguousarray(inp, dtype=np.float32)
+        B = inp.shape[0]
+        out = np.zeros((B, self.ngrid * self.ngrid, 2), dtype=np.float32)
+        _chk(lib().g4mi_fno_forward(self.h, inp.ctypes.data_as(C.POINTER(C.c_float)),
+                                    out.ctypes.data_as(C.POINTER(C.c_float)), B), "fno_forward")
+        return out
+
+    def loss_grad(self, inp, tgt):
+        inp = np.ascontiguousarray(inp, dtype=np.float32)
+        tgt = np.ascontiguousarray(tgt, dtype=np.float32)
+        loss = C.c_double()
+        g = np.zeros(self.param_count(), dtype=np.float32)
+        _chk(lib().g4mi_fno_loss_grad(self.h, inp.ctypes.data_as(C.POINTER(C.c_float)),
+                                      tgt.ctypes.data_as(C.POINTER(C.c_float)), inp.shape[0],
+                                      C.byref(loss), g.ctypes.data_as(C.POINTER(C.c_float))),
+             "fno_loss_grad")
+        return loss.value, g
+
+    def reserve(self, n):
+        _chk(lib().g4mi_fno_reserve(self.h, n), "fno_reserve")
+
+    def pair_count(self):
+        return int(lib().g4mi_fno_pair_count(self.h))
+
+    def train(self, epochs, batch, lr, nholdout=0, seed=1):
+        tl = np.zeros(max(1, epochs))
+        hl = np.zeros(max(1, epochs))
+        _chk(lib().g4mi_fno_train(self.h, epochs, batch, lr, nholdout, seed, _p(tl), _p(hl)),
+             "fno_train")
+        return tl[:epochs], hl[:epochs]
+
+    def eval(self, first, last, batch):
+        loss = C.c_double()
+        _chk(lib().g4mi_fno_eval(self.h, first, last, batch, C.byref(loss)), "fno_eval")
+        return loss.value
+
+    def param_blocks(self):
+        """(name, offset, size) of every parameter block, in storage order."""
+        C_, H, nm2 = self.width, self.hidden, self.modes * self.modes
+        out, o = [], 0
+        for name, n in [("P", 4 * C_), ("bP", C_)]:
+            out.append((name, o, n))
+            o += n
+        for l in range(self.layers):
+            for name, n in [(f"Rr{l}", nm2 * C_ * C_), (f"Ri{l}", nm2 * C_ * C_),
+                            (f"W{l}", C_ * C_), (f"b{l}", C_)]:
+                out.append((name, o, n))
+                o += n
+        for name, n in [("Q1", C_ * H), ("bQ1", H), ("Q2", 2 * H), ("bQ2", 2)]:
+            out.append((name, o, n))
+            o += n
         return out
 
 

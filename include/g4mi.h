@@ -186,6 +186,59 @@ int g4mi_load_gauss(double *field, int ngrid, double gridmax, double lambda, dou
  * Replaces genmain() for the supported commands. */
 int g4mi_run_deck(const char *input_file, int device, int quiet);
 
+/* ------------------------------------------------------------------------
+ * Learned field step (no counterpart in the reference).  A Fourier neural
+ * operator maps a slice's field E and deposited source S to the field
+ * after the ADI step; it is trained on (E, S, E_next) triplets captured
+ * from the solver's own steps and can stand in for the ADI of a mixed-form
+ * context, with the ADI as fallback.  Every dense product runs on BF16
+ * MFMA with FP32 accumulation; weights and Adam state are FP32.
+ * ------------------------------------------------------------------------ */
+typedef struct g4mi_fno g4mi_fno;
+
+/* width channels, modes x modes retained Fourier modes (even), layers
+ * Fourier layers, hidden projection channels, max_batch slices per pass. */
+int g4mi_fno_create(g4mi_fno **fno, int device, int ngrid, int width, int modes, int layers,
+                    int hidden, int max_batch, unsigned seed);
+int g4mi_fno_destroy(g4mi_fno *fno);
+long long g4mi_fno_param_count(g4mi_fno *fno);
+/* Parameter vector, in order: lift P[4][width], bP[width]; per layer
+ * Rr[modes^2][width][width], Ri[...], W[width][width], b[width];
+ * Q1[width][hidden], bQ1[hidden], Q2[hidden][2], bQ2[2]. */
+int g4mi_fno_get_params(g4mi_fno *fno, float *params);
+int g4mi_fno_set_params(g4mi_fno *fno, const float *params);
+/* in [batch][ngrid^2][4] = (Re E, Im E, Re S, Im S) / scale,
+ * out [batch][ngrid^2][2] = (E_next - E) / scale (the network output). */
+int g4mi_fno_forward(g4mi_fno *fno, const float *in, float *out, int batch);
+/* Mean-squared loss against target [batch][ngrid^2][2] and its gradient
+ * with respect to every parameter (same order as g4mi_fno_get_params). */
+int g4mi_fno_loss_grad(g4mi_fno *fno, const float *in, const float *target, int batch,
+                       double *loss, float *grad);
+/* Training set on the device: capacity, pairs held. */
+int g4mi_fno_reserve(g4mi_fno *fno, long long npairs);
+long long g4mi_fno_pair_count(g4mi_fno *fno);
+/* Adam with a cosine learning-rate schedule over `epochs`; the last
+ * `nholdout` pairs are kept out of training; per-epoch mean losses. */
+int g4mi_fno_train(g4mi_fno *fno, int epochs, int batch, float lr, long long nholdout,
+                   unsigned seed, double *train_loss, double *holdout_loss);
+int g4mi_fno_eval(g4mi_fno *fno, long long first, long long last, int batch, double *loss);
+
+/* Attach a network to a mixed-form (G4MI_FP32) context:
+ *   mode 0: detach;
+ *   mode 1: capture -- after every step's ADI, the (E, S, E_next) pairs of
+ *           every `every`-th beam slice are appended to the training set;
+ *   mode 2: replace -- the network advances the field; the ADI runs
+ *           instead on every `every`-th step and whenever the
+ *           particle-to-field energy-balance residual of the network's
+ *           step exceeds `threshold`:
+ *             r = |dP_field + sum_s I_s eev d<gamma>_s| / P_field,
+ *           dP_field the change of the total field power over the step. */
+int g4mi_attach_fno(g4mi_ctx *ctx, g4mi_fno *fno, int mode, int every, double threshold);
+/* Steps advanced by the network and by the ADI since attaching, and the
+ * mean residual of the network's steps and of the ADI's steps. */
+int g4mi_fno_stats(g4mi_ctx *ctx, long long *fno_steps, long long *adi_steps,
+                   double *mean_residual_fno, double *mean_residual_adi);
+
 #ifdef __cplusplus
 }
 #endif
