@@ -166,6 +166,33 @@ def cpu_baseline(cfg, nsteps):
                        f"macroparticles), {dt:.2f} s on {threads} threads")
 
 
+def fno_probe(ngrid=151, batch=16, reps=5):
+    """Throughput of the learned field step at its full shape (width 32,
+    16x16 modes, 4 layers, hidden 64): forward and forward+backward passes
+    on random data."""
+    f = g4mi.Fno(ngrid, width=32, modes=16, layers=4, hidden=64, max_batch=batch, seed=1)
+    P = ngrid * ngrid
+    rng = np.random.default_rng(0)
+    inp = rng.normal(size=(batch, P, 4)).astype(np.float32)
+    tgt = rng.normal(size=(batch, P, 2)).astype(np.float32)
+    f.forward(inp)
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        f.forward(inp)
+    tf = (time.perf_counter() - t0) / reps
+    f.loss_grad(inp, tgt)
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        f.loss_grad(inp, tgt)
+    tb = (time.perf_counter() - t0) / reps
+    out = dict(fno_probe=dict(ngrid=ngrid, batch=batch, params=f.param_count(),
+                              forward_ms=round(tf * 1e3, 3), forward_backward_ms=round(tb * 1e3, 3),
+                              samples_per_s_train=round(batch / tb, 1)))
+    f.close()
+    print(json.dumps(out), flush=True)
+    return 0
+
+
 def copy_probe(mib):
     """A plain device-to-device copy of a known size: its FETCH_SIZE and
     WRITE_SIZE per dispatch, against mib MiB, give the counters' scale."""
@@ -201,9 +228,12 @@ def main():
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB five times (the reference "
                          "point that scales the FETCH_SIZE/WRITE_SIZE counters) and exit")
+    ap.add_argument("--fno-probe", action="store_true", help="only time the learned field step")
     a = ap.parse_args()
     if a.copy_probe:
         return copy_probe(a.copy_probe)
+    if a.fno_probe:
+        return fno_probe()
     if a.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; the slice-block sharding "
                                    "across GPUs is a design, not an implementation",

@@ -131,28 +131,45 @@ __global__ void k_gelu_back(float *d, const float *z, long long n) {
     if (i < n) d[i] *= gelu_d(z[i]);
 }
 
-// column sums of d [rows][ch] into out[ch] (+= when acc): one block per channel
-__global__ void __launch_bounds__(256) k_colsum(const float *d, long long rows, int ch, float *out) {
+// column sums of d [rows][ch] (ch <= 256): stage 1, block b sums rows
+// b, b + nblk, ... into part[b][ch] (coalesced: lane -> (row, channel));
+// stage 2 adds the nblk partials in block order
+constexpr int kColBlocks = 256;
+__global__ void __launch_bounds__(256) k_colsum_part(const float *d, long long rows, int ch, double *part) {
     __shared__ double red[256];
-    const int c = blockIdx.x;
+    const int t = threadIdx.x;
+    const int rpb = 256 / ch;  // rows per pass of the block
     double s = 0;
-    for (long long r = threadIdx.x; r < rows; r += 256) s += d[r * ch + c];
-    red[threadIdx.x] = s;
-    __syncthreads();
-    for (int k = 128; k > 0; k >>= 1) {
-        if (threadIdx.x < k) red[threadIdx.x] += red[threadIdx.x + k];
-        __syncthreads();
+    if (t < rpb * ch) {
+        const int c = t % ch;
+        for (long long r = (long long)blockIdx.x * rpb + t / ch; r < rows; r += (long long)kColBlocks * rpb)
+            s += d[r * ch + c];
     }
-    if (threadIdx.x == 0) out[c] = (float)red[0];
+    red[t] = s;
+    __syncthreads();
+    if (t < ch) {
+        double acc = 0;
+        for (int q = 0; q < rpb; q++) acc += red[q * ch + t];
+        part[(long long)blockIdx.x * ch + t] = acc;
+    }
+}
+__global__ void k_colsum_fin(const double *part, int ch, float *out) {
+    const int c = blockIdx.x * 64 + threadIdx.x;
+    if (c >= ch) return;
+    double s = 0;
+    for (int b = 0; b < kColBlocks; b++) s += part[(long long)b * ch + c];
+    out[c] = (float)s;
 }
 
-// sum of nz partial blocks of n floats into out
-__global__ void k_reduce_parts(const float *parts, int nz, long long n, float *out) {
+// sum of nz partial blocks of n floats: blockIdx.y = g adds blocks
+// [g zc, min((g + 1) zc, nz)) into out[g][n]
+__global__ void k_reduce_parts(const float *parts, int nz, int zc, long long n, float *out) {
     const long long i = (long long)blockIdx.x * 256 + threadIdx.x;
     if (i >= n) return;
+    const int z0 = blockIdx.y * zc, z1 = min(z0 + zc, nz);
     float s = 0;
-    for (int z = 0; z < nz; z++) s += parts[(long long)z * n + i];
-    out[i] = s;
+    for (int z = z0; z < z1; z++) s += parts[(long long)z * n + i];
+    out[blockIdx.y * n + i] = s;
 }
 
 // Ahat[b][ky][kx][2C] from T[b][2nm][2nm C] (y transform of the x transform):
@@ -437,9 +454,9 @@ struct Fno {
     std::vector<float *> a, z, Ah;
     float *X1 = nullptr, *T = nullptr, *Bh = nullptr, *Zb = nullptr, *Y1 = nullptr;
     float *zq = nullptr, *q = nullptr, *out = nullptr;
-    float *da = nullptr, *da2 = nullptr, *dq = nullptr, *dout = nullptr, *parts = nullptr;
+    float *da = nullptr, *da2 = nullptr, *dq = nullptr, *dout = nullptr, *parts = nullptr, *parts2 = nullptr;
     long long parts_n = 0;
-    double *d_loss = nullptr;
+    double *d_loss = nullptr, *colpart = nullptr;
     int *d_idx = nullptr;
     // pair set
     float *pairs = nullptr;
@@ -469,14 +486,22 @@ void gemm(Fno *f, const float *A, const float *B, float *C, int M, int N, int K,
 
 // weight gradient dW (M x N) = op(A)^T-style product with K = rows (large):
 // split K into chunks, partial products into parts, then a sum
+constexpr int kWgradK = 256;   // rows per split-K block
+constexpr int kWgradG = 32;    // first-stage groups of the partial sum
 void gemm_wgrad(Fno *f, const float *A, const float *B, float *dW, int M, int N, long long K, int lda,
                 int ldb) {
-    const int kc = 8192;
+    const int kc = kWgradK;
     const int nz = (int)((K + kc - 1) / kc);
-    gemm(f, A, B, f->parts, M, N, kc, lda, ldb, N, (long long)kc * lda, (long long)kc * ldb, (long long)M * N, nz,
-         1, 0, 0.f, K);
-    hipLaunchKernelGGL(k_reduce_parts, dim3(nb((long long)M * N)), dim3(256), 0, f->s, f->parts, nz,
-                       (long long)M * N, dW);
+    const long long n = (long long)M * N;
+    gemm(f, A, B, f->parts, M, N, kc, lda, ldb, N, (long long)kc * lda, (long long)kc * ldb, n, nz, 1, 0, 0.f, K);
+    const int zc = (nz + kWgradG - 1) / kWgradG, g = (nz + zc - 1) / zc;
+    hipLaunchKernelGGL(k_reduce_parts, dim3(nb(n), g), dim3(256), 0, f->s, f->parts, nz, zc, n, f->parts2);
+    hipLaunchKernelGGL(k_reduce_parts, dim3(nb(n), 1), dim3(256), 0, f->s, f->parts2, g, g, n, dW);
+}
+
+void colsum(Fno *f, const float *d, long long rows, int ch, float *out) {
+    hipLaunchKernelGGL(k_colsum_part, dim3(kColBlocks), dim3(256), 0, f->s, d, rows, ch, f->colpart);
+    hipLaunchKernelGGL(k_colsum_fin, dim3((ch + 63) / 64), dim3(64), 0, f->s, f->colpart, ch, out);
 }
 
 void build_wreal(Fno *f) {
@@ -535,11 +560,11 @@ void backward(Fno *f, int B) {
     float *prm = f->params, *gr = f->grads;
     // projection
     gemm_wgrad(f, f->q, f->dout, gr + f->o_Q2, H, 2, rows, H, 2);
-    hipLaunchKernelGGL(k_colsum, dim3(2), dim3(256), 0, f->s, f->dout, rows, 2, gr + f->o_bQ2);
+    colsum(f, f->dout, rows, 2, gr + f->o_bQ2);
     gemm(f, f->dout, prm + f->o_Q2, f->dq, (int)rows, H, 2, 2, 2, H, 0, 0, 0, 1, 0, 1);
     hipLaunchKernelGGL(k_gelu_back, dim3(nb(rows * H)), dim3(256), 0, f->s, f->dq, f->zq, rows * H);
     gemm_wgrad(f, f->a[f->L], f->dq, gr + f->o_Q1, C, H, rows, C, H);
-    hipLaunchKernelGGL(k_colsum, dim3(H), dim3(256), 0, f->s, f->dq, rows, H, gr + f->o_bQ1);
+    colsum(f, f->dq, rows, H, gr + f->o_bQ1);
     float *dA = f->da, *dA2 = f->da2;
     gemm(f, f->dq, prm + f->o_Q1, dA, (int)rows, C, H, H, H, C, 0, 0, 0, 1, 0, 1);
     for (int l = f->L - 1; l >= 0; l--) {
@@ -547,7 +572,7 @@ void backward(Fno *f, int B) {
         // dz = dA * GELU'(z) (no activation after the last layer)
         if (l < f->L - 1) hipLaunchKernelGGL(k_gelu_back, dim3(nb(rows * C)), dim3(256), 0, f->s, dA, zl, rows * C);
         float *dz = dA;
-        hipLaunchKernelGGL(k_colsum, dim3(C), dim3(256), 0, f->s, dz, rows, C, gr + f->o_b[l]);
+        colsum(f, dz, rows, C, gr + f->o_b[l]);
         gemm_wgrad(f, a, dz, gr + f->o_W[l], C, C, rows, C, C);
         // dA2 = dz W^T
         gemm(f, dz, prm + f->o_W[l], dA2, (int)rows, C, C, C, C, C, 0, 0, 0, 1, 0, 1);
@@ -577,7 +602,7 @@ void backward(Fno *f, int B) {
     }
     // lift
     gemm_wgrad(f, f->in, dA, gr + f->o_P, 4, C, rows, 4, C);
-    hipLaunchKernelGGL(k_colsum, dim3(C), dim3(256), 0, f->s, dA, rows, C, gr + f->o_bP);
+    colsum(f, dA, rows, C, gr + f->o_bP);
 }
 
 void init_params(Fno *f, unsigned seed) {
@@ -608,8 +633,8 @@ void init_params(Fno *f, unsigned seed) {
 // internal API (used by the C-ABI below and by the context's deployment)
 // ---------------------------------------------------------------------------
 Fno *fno_new(int device, const FnoShape &sh, unsigned seed) {
-    if (sh.ngrid < 8 || sh.width < 1 || sh.modes < 2 || sh.modes % 2 || sh.modes > sh.ngrid || sh.layers < 1 ||
-        sh.hidden < 1 || sh.max_batch < 1) {
+    if (sh.ngrid < 8 || sh.width < 1 || sh.width > 256 || sh.modes < 2 || sh.modes % 2 || sh.modes > sh.ngrid ||
+        sh.layers < 1 || sh.hidden < 1 || sh.hidden > 256 || sh.max_batch < 1) {
         set_error("fno: invalid shape");
         return nullptr;
     }
@@ -671,13 +696,17 @@ Fno *fno_new(int device, const FnoShape &sh, unsigned seed) {
     rc |= falloc(f, &f->da, act) | falloc(f, &f->da2, act) | falloc(f, &f->dq, (long long)B * P * H) |
           falloc(f, &f->dout, (long long)B * P * 2);
     const long long rows = (long long)B * P;
-    f->parts_n = ((rows + 8191) / 8192) * (long long)std::max({C * C, C * H, 2 * H, 4 * C});
+    const long long wmax = std::max({C * C, C * H, 2 * H, 4 * C});
+    f->parts_n = ((rows + kWgradK - 1) / kWgradK) * wmax;
     rc |= falloc(f, &f->parts, f->parts_n);
+    rc |= falloc(f, &f->parts2, kWgradG * wmax);
     if (rc) return fail();
     if (hipMalloc(&f->d_loss, sizeof(double)) != hipSuccess) return fail();
     f->allocs.push_back(f->d_loss);
     if (hipMalloc(&f->d_idx, (size_t)B * sizeof(int)) != hipSuccess) return fail();
     f->allocs.push_back(f->d_idx);
+    if (hipMalloc(&f->colpart, (size_t)kColBlocks * std::max(C, H) * sizeof(double)) != hipSuccess) return fail();
+    f->allocs.push_back(f->colpart);
     // DFT matrices over the retained modes k = -nm/2 .. nm/2 - 1
     std::vector<float> fx(2 * nm * N), gy2(2 * N * nm), gx(N * 2 * nm);
     const double tw = 2.0 * M_PI / N, inv = 1.0 / ((double)N * N);

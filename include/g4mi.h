@@ -197,7 +197,8 @@ int g4mi_run_deck(const char *input_file, int device, int quiet);
 typedef struct g4mi_fno g4mi_fno;
 
 /* width channels, modes x modes retained Fourier modes (even), layers
- * Fourier layers, hidden projection channels, max_batch slices per pass. */
+ * Fourier layers, hidden projection channels, max_batch slices per pass.
+ * width and hidden are at most 256. */
 int g4mi_fno_create(g4mi_fno **fno, int device, int ngrid, int width, int modes, int layers,
                     int hidden, int max_batch, unsigned seed);
 int g4mi_fno_destroy(g4mi_fno *fno);

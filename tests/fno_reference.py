@@ -11,7 +11,6 @@ gradient oracle for the GPU's hand-written backward pass.
 import math
 
 import numpy as np
-import torch
 
 
 def blocks(N, width, modes, layers, hidden):
@@ -32,12 +31,14 @@ def blocks(N, width, modes, layers, hidden):
 
 
 def gelu(x):
+    import torch
     c = math.sqrt(2.0 / math.pi)
     return 0.5 * x * (1.0 + torch.tanh(c * (x + 0.044715 * x ** 3)))
 
 
 def forward(params, inp, N, width, modes, layers, hidden):
     """params: torch float64 vector; inp: [B, N*N, 4] -> out [B, N*N, 2]."""
+    import torch
     C, H = width, hidden
     bl, _ = blocks(N, width, modes, layers, hidden)
 
@@ -67,6 +68,7 @@ def forward(params, inp, N, width, modes, layers, hidden):
 
 
 def loss_and_grad(params_np, inp_np, tgt_np, N, width, modes, layers, hidden):
+    import torch
     p = torch.tensor(params_np.astype(np.float64), requires_grad=True)
     inp = torch.tensor(inp_np.astype(np.float64))
     tgt = torch.tensor(tgt_np.astype(np.float64))
