@@ -29,9 +29,18 @@ struct GemmArgs {
     int transA, transB;     // A stored K x M / B stored N x K
     float alpha, beta;
     long long ktot;         // > 0: split-K, batch z covers k in [z K, min((z+1) K, ktot))
+    // epilogue (all optional): v = alpha acc + beta C (+ bias[col]);
+    // v *= GELU'(dmul) (dmul indexed like C); C = v; act_out = GELU(v) when
+    // act, else v (act_out indexed like C)
+    const float *bias = nullptr;
+    const float *dmul = nullptr;
+    float *act_out = nullptr;
+    int act = 0;
 };
 
-// C[z] = alpha op(A[z]) op(B[z]) + beta C[z] for z < batch.
+// C[z] = alpha op(A[z]) op(B[z]) + beta C[z] for z < batch, with the
+// epilogue above.  The output tile (32 WM x 32 WN) is chosen per call from
+// M and N.
 void launch_gemm_bf16(hipStream_t s, const GemmArgs &g, int batch);
 
 struct Fno;

@@ -3,6 +3,7 @@
 // steps, and batched inference over the slices of a context.
 #include <algorithm>
 #include <cmath>
+#include <cstdint>
 #include <cstring>
 #include <random>
 #include <string>
@@ -26,22 +27,57 @@ __device__ __forceinline__ unsigned short f2bf(float f) {
 }
 
 // ---------------------------------------------------------------------------
-// GEMM: 64x64 output tile per workgroup of 4 wavefronts (2x2 of 32x32), k in
-// chunks of 32 staged through LDS as BF16, v_mfma_f32_32x32x8_bf16 with FP32
-// accumulation.  MFMA operand layout (32x32x8): lane l holds A row l%32 and
-// B column l%32 at k = 4 (l/32) .. 4 (l/32) + 3; accumulator register v of
-// lane l is C row 8 (v/4) + 4 (l/32) + v%4, column l%32.
+// elementwise helpers
 // ---------------------------------------------------------------------------
-constexpr int GT = 64;
-constexpr int GK = 32;
-constexpr int GP = GK + 4;  // LDS row pitch (bf16), 8-byte aligned rows
+__device__ __forceinline__ float gelu(float x) {
+    const float c = 0.7978845608028654f;  // sqrt(2/pi)
+    return 0.5f * x * (1.f + tanhf(c * (x + 0.044715f * x * x * x)));
+}
+__device__ __forceinline__ float gelu_d(float x) {
+    const float c = 0.7978845608028654f;
+    const float u = c * (x + 0.044715f * x * x * x);
+    const float th = tanhf(u);
+    return 0.5f * (1.f + th) + 0.5f * x * (1.f - th * th) * c * (1.f + 3.f * 0.044715f * x * x);
+}
 
-__global__ void __launch_bounds__(256) k_gemm_bf16(GemmArgs g) {
-    __shared__ unsigned short As[GT * GP];   // [m][k]
-    __shared__ unsigned short Bs[GT * GP];   // [n][k]
+// ---------------------------------------------------------------------------
+// GEMM: (32 WM) x (32 WN) output tile per workgroup of WM x WN wavefronts,
+// each wavefront one 32x32 block; k in chunks of 32 staged through LDS as
+// BF16 (global reads as float4 where the contiguous dimension allows),
+// v_mfma_f32_32x32x8_bf16 with FP32 accumulation.  MFMA operand layout
+// (32x32x8): lane l holds A row l%32 and B column l%32 at k = 4 (l/32) ..
+// 4 (l/32) + 3; accumulator register v of lane l is C row 8 (v/4) +
+// 4 (l/32) + v%4, column l%32.
+// ---------------------------------------------------------------------------
+constexpr int GK = 32;
+constexpr int GP = GK + 4;  // LDS row pitch (bf16): 8-byte aligned rows
+
+__device__ __forceinline__ void st_bf4(unsigned short *p, float4 v) {
+    // 4 consecutive bf16 in one 8-byte LDS store
+    const unsigned lo = (unsigned)f2bf(v.x) | ((unsigned)f2bf(v.y) << 16);
+    const unsigned hi = (unsigned)f2bf(v.z) | ((unsigned)f2bf(v.w) << 16);
+    *(uint2 *)p = make_uint2(lo, hi);
+}
+
+// four consecutive floats p[0..3] of which the first `n` (0..4) exist
+__device__ __forceinline__ float4 ld4(const float *p, int n, bool vec) {
+    if (vec && n >= 4) return *(const float4 *)p;
+    float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (n > 0) v.x = p[0];
+    if (n > 1) v.y = p[1];
+    if (n > 2) v.z = p[2];
+    if (n > 3) v.w = p[3];
+    return v;
+}
+
+template <int WM, int WN>
+__global__ void __launch_bounds__(64 * WM * WN) k_gemm_bf16(GemmArgs g, int vecA, int vecB) {
+    constexpr int NT = 64 * WM * WN, TM = 32 * WM, TN = 32 * WN;
+    __shared__ __align__(16) unsigned short As[TM * GP];   // [m][k]
+    __shared__ __align__(16) unsigned short Bs[TN * GP];   // [n][k]
     const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
-    const int wm = wv >> 1, wn = wv & 1;
-    const int m0 = blockIdx.y * GT, n0 = blockIdx.x * GT;
+    const int wm = wv / WN, wn = wv % WN;
+    const int m0 = blockIdx.y * TM, n0 = blockIdx.x * TN;
     const long long z = blockIdx.z;
     const float *A = g.A + z * g.sA, *B = g.B + z * g.sB;
     float *C = g.C + z * g.sC;
@@ -51,33 +87,42 @@ __global__ void __launch_bounds__(256) k_gemm_bf16(GemmArgs g) {
 #pragma unroll
     for (int v = 0; v < 16; v++) acc[v] = 0.f;
     for (int k0 = 0; k0 < K; k0 += GK) {
-        for (int e = t; e < GT * GK; e += 256) {
-            int mm, kk;
-            if (g.transA) {
-                mm = e % GT;
-                kk = e / GT;
-            } else {
-                kk = e % GK;
-                mm = e / GK;
+        // A tile: TM x GK in units of 4 along the contiguous dimension
+        for (int e = t; e < TM * GK / 4; e += NT) {
+            if (!g.transA) {  // A[m][k]: contiguous along k
+                const int mm = e / (GK / 4), kk = (e % (GK / 4)) * 4;
+                const int m = m0 + mm, k = k0 + kk;
+                const float4 v = m < g.M ? ld4(A + (long long)m * g.lda + k, K - k, vecA)
+                                         : make_float4(0.f, 0.f, 0.f, 0.f);
+                st_bf4(&As[mm * GP + kk], v);
+            } else {  // A[k][m]: contiguous along m
+                const int kk = e / (TM / 4), mm = (e % (TM / 4)) * 4;
+                const int m = m0 + mm, k = k0 + kk;
+                const float4 v = k < K ? ld4(A + (long long)k * g.lda + m, g.M - m, vecA)
+                                       : make_float4(0.f, 0.f, 0.f, 0.f);
+                As[(mm + 0) * GP + kk] = f2bf(v.x);
+                As[(mm + 1) * GP + kk] = f2bf(v.y);
+                As[(mm + 2) * GP + kk] = f2bf(v.z);
+                As[(mm + 3) * GP + kk] = f2bf(v.w);
             }
-            const int m = m0 + mm, k = k0 + kk;
-            float v = 0.f;
-            if (m < g.M && k < K) v = g.transA ? A[(long long)k * g.lda + m] : A[(long long)m * g.lda + k];
-            As[mm * GP + kk] = f2bf(v);
         }
-        for (int e = t; e < GT * GK; e += 256) {
-            int nn, kk;
-            if (g.transB) {
-                kk = e % GK;
-                nn = e / GK;
-            } else {
-                nn = e % GT;
-                kk = e / GT;
+        for (int e = t; e < TN * GK / 4; e += NT) {
+            if (g.transB) {  // B[n][k]: contiguous along k
+                const int nn = e / (GK / 4), kk = (e % (GK / 4)) * 4;
+                const int n = n0 + nn, k = k0 + kk;
+                const float4 v = n < g.N ? ld4(B + (long long)n * g.ldb + k, K - k, vecB)
+                                         : make_float4(0.f, 0.f, 0.f, 0.f);
+                st_bf4(&Bs[nn * GP + kk], v);
+            } else {  // B[k][n]: contiguous along n
+                const int kk = e / (TN / 4), nn = (e % (TN / 4)) * 4;
+                const int n = n0 + nn, k = k0 + kk;
+                const float4 v = k < K ? ld4(B + (long long)k * g.ldb + n, g.N - n, vecB)
+                                       : make_float4(0.f, 0.f, 0.f, 0.f);
+                Bs[(nn + 0) * GP + kk] = f2bf(v.x);
+                Bs[(nn + 1) * GP + kk] = f2bf(v.y);
+                Bs[(nn + 2) * GP + kk] = f2bf(v.z);
+                Bs[(nn + 3) * GP + kk] = f2bf(v.w);
             }
-            const int n = n0 + nn, k = k0 + kk;
-            float v = 0.f;
-            if (n < g.N && k < K) v = g.transB ? B[(long long)n * g.ldb + k] : B[(long long)k * g.ldb + n];
-            Bs[nn * GP + kk] = f2bf(v);
         }
         __syncthreads();
         const int mr = wm * 32 + (lane & 31), nc = wn * 32 + (lane & 31);
@@ -92,43 +137,20 @@ __global__ void __launch_bounds__(256) k_gemm_bf16(GemmArgs g) {
     }
     const int col = n0 + wn * 32 + (lane & 31);
     if (col >= g.N) return;
+    const float bcol = g.bias ? g.bias[col] : 0.f;
 #pragma unroll
     for (int v = 0; v < 16; v++) {
         const int row = m0 + wm * 32 + 8 * (v >> 2) + 4 * (lane >> 5) + (v & 3);
         if (row < g.M) {
-            float *c = C + (long long)row * g.ldc + col;
-            *c = g.beta != 0.f ? g.alpha * acc[v] + g.beta * *c : g.alpha * acc[v];
+            const long long o = (long long)row * g.ldc + col;
+            float r = g.alpha * acc[v];
+            if (g.beta != 0.f) r += g.beta * C[o];
+            r += bcol;
+            if (g.dmul) r *= gelu_d(g.dmul[z * g.sC + o]);
+            C[o] = r;
+            if (g.act_out) g.act_out[z * g.sC + o] = g.act ? gelu(r) : r;
         }
     }
-}
-
-// ---------------------------------------------------------------------------
-// elementwise kernels
-// ---------------------------------------------------------------------------
-__device__ __forceinline__ float gelu(float x) {
-    const float c = 0.7978845608028654f;  // sqrt(2/pi)
-    return 0.5f * x * (1.f + tanhf(c * (x + 0.044715f * x * x * x)));
-}
-__device__ __forceinline__ float gelu_d(float x) {
-    const float c = 0.7978845608028654f;
-    const float u = c * (x + 0.044715f * x * x * x);
-    const float th = tanhf(u);
-    return 0.5f * (1.f + th) + 0.5f * x * (1.f - th * th) * c * (1.f + 3.f * 0.044715f * x * x);
-}
-
-// z += bias (per channel); a = GELU(z) when act, else a = z
-__global__ void k_bias_act(float *z, float *a, const float *bias, long long rows, int ch, int act) {
-    const long long i = (long long)blockIdx.x * 256 + threadIdx.x;
-    if (i >= rows * ch) return;
-    const float v = z[i] + bias[i % ch];
-    z[i] = v;
-    a[i] = act ? gelu(v) : v;
-}
-
-// d *= GELU'(z)
-__global__ void k_gelu_back(float *d, const float *z, long long n) {
-    const long long i = (long long)blockIdx.x * 256 + threadIdx.x;
-    if (i < n) d[i] *= gelu_d(z[i]);
 }
 
 // column sums of d [rows][ch] (ch <= 256): stage 1, block b sums rows
@@ -429,9 +451,37 @@ inline unsigned nb(long long n) { return (unsigned)((n + 255) / 256); }
 
 }  // namespace
 
+namespace {
+// waves along one output dimension: the fewest padded rows, ties to the larger tile
+int pick_waves(int n, int maxw) {
+    int best = 1;
+    long long pad = (long long)((n + 31) / 32) * 32;
+    for (int w = 2; w <= maxw; w *= 2) {
+        const long long p = (long long)((n + 32 * w - 1) / (32 * w)) * 32 * w;
+        if (p <= pad) {
+            pad = p;
+            best = w;
+        }
+    }
+    return best;
+}
+bool al16(const void *p) { return ((uintptr_t)p & 15) == 0; }
+}  // namespace
+
 void launch_gemm_bf16(hipStream_t s, const GemmArgs &g, int batch) {
-    dim3 grid((unsigned)((g.N + GT - 1) / GT), (unsigned)((g.M + GT - 1) / GT), (unsigned)batch);
-    hipLaunchKernelGGL(k_gemm_bf16, grid, dim3(256), 0, s, g);
+    const int wm = pick_waves(g.M, 4), wn = pick_waves(g.N, 4 / wm);
+    // float4 reads need 16-byte aligned rows and batch bases
+    const int vecA = al16(g.A) && g.lda % 4 == 0 && (batch == 1 || g.sA % 4 == 0);
+    const int vecB = al16(g.B) && g.ldb % 4 == 0 && (batch == 1 || g.sB % 4 == 0);
+    const dim3 grid((unsigned)((g.N + 32 * wn - 1) / (32 * wn)), (unsigned)((g.M + 32 * wm - 1) / (32 * wm)),
+                    (unsigned)batch);
+    const dim3 blk(64 * wm * wn);
+    if (wm == 1 && wn == 1) hipLaunchKernelGGL((k_gemm_bf16<1, 1>), grid, blk, 0, s, g, vecA, vecB);
+    else if (wm == 1 && wn == 2) hipLaunchKernelGGL((k_gemm_bf16<1, 2>), grid, blk, 0, s, g, vecA, vecB);
+    else if (wm == 1) hipLaunchKernelGGL((k_gemm_bf16<1, 4>), grid, blk, 0, s, g, vecA, vecB);
+    else if (wm == 2 && wn == 1) hipLaunchKernelGGL((k_gemm_bf16<2, 1>), grid, blk, 0, s, g, vecA, vecB);
+    else if (wm == 2) hipLaunchKernelGGL((k_gemm_bf16<2, 2>), grid, blk, 0, s, g, vecA, vecB);
+    else hipLaunchKernelGGL((k_gemm_bf16<4, 1>), grid, blk, 0, s, g, vecA, vecB);
 }
 
 // ===========================================================================
@@ -477,10 +527,19 @@ int falloc(Fno *f, float **p, long long n) {
     return G4MI_OK;
 }
 
+// epilogue options of one product
+struct Epi {
+    const float *bias = nullptr;
+    const float *dmul = nullptr;
+    float *act_out = nullptr;
+    int act = 0;
+};
+
 void gemm(Fno *f, const float *A, const float *B, float *C, int M, int N, int K, int lda, int ldb, int ldc,
           long long sA, long long sB, long long sC, int batch, int tA = 0, int tB = 0, float beta = 0.f,
-          long long ktot = 0) {
-    GemmArgs g{A, B, C, M, N, K, lda, ldb, ldc, sA, sB, sC, tA, tB, 1.f, beta, ktot};
+          long long ktot = 0, const Epi &ep = Epi()) {
+    GemmArgs g{A, B, C, M, N, K, lda, ldb, ldc, sA, sB, sC, tA, tB, 1.f, beta, ktot,
+               ep.bias, ep.dmul, ep.act_out, ep.act};
     launch_gemm_bf16(f->s, g, batch);
 }
 
@@ -517,9 +576,12 @@ void forward(Fno *f, int B) {
     const long long rows = (long long)B * P;
     float *prm = f->params;
     // lift
-    gemm(f, f->in, prm + f->o_P, f->z[0], (int)rows, C, 4, 4, C, C, 0, 0, 0, 1);
-    hipLaunchKernelGGL(k_bias_act, dim3(nb(rows * C)), dim3(256), 0, f->s, f->z[0], f->a[0], prm + f->o_bP, rows,
-                       C, 0);
+    {
+        Epi ep;
+        ep.bias = prm + f->o_bP;
+        ep.act_out = f->a[0];
+        gemm(f, f->in, prm + f->o_P, f->z[0], (int)rows, C, 4, 4, C, C, 0, 0, 0, 1, 0, 0, 0.f, 0, ep);
+    }
     for (int l = 0; l < f->L; l++) {
         float *a = f->a[l], *zl = f->z[l + 1];
         // x transform: X1[b][y] (2nm x C) = Fx (2nm x N) a[b][y] (N x C)
@@ -540,17 +602,21 @@ void forward(Fno *f, int B) {
         // inverse x: s[b][y] (N x C) = Gx (N x 2nm) Y1[b][y] (2nm x C)
         gemm(f, f->Gx, f->Y1, zl, N, C, 2 * nm, 2 * nm, C, C, 0, 2LL * nm * C, (long long)N * C, B * N);
         // + a W, + b, activation
-        gemm(f, a, prm + f->o_W[l], zl, (int)rows, C, C, C, C, C, 0, 0, 0, 1, 0, 0, 1.f);
-        hipLaunchKernelGGL(k_bias_act, dim3(nb(rows * C)), dim3(256), 0, f->s, zl, f->a[l + 1], prm + f->o_b[l],
-                           rows, C, l < f->L - 1 ? 1 : 0);
+        Epi ep;
+        ep.bias = prm + f->o_b[l];
+        ep.act_out = f->a[l + 1];
+        ep.act = l < f->L - 1 ? 1 : 0;
+        gemm(f, a, prm + f->o_W[l], zl, (int)rows, C, C, C, C, C, 0, 0, 0, 1, 0, 0, 1.f, 0, ep);
     }
     // projection
-    gemm(f, f->a[f->L], prm + f->o_Q1, f->zq, (int)rows, H, C, C, H, H, 0, 0, 0, 1);
-    hipLaunchKernelGGL(k_bias_act, dim3(nb(rows * H)), dim3(256), 0, f->s, f->zq, f->q, prm + f->o_bQ1, rows, H, 1);
-    gemm(f, f->q, prm + f->o_Q2, f->out, (int)rows, 2, H, H, 2, 2, 0, 0, 0, 1);
-    float *dummy = f->dout;  // bias only (z == a)
-    hipLaunchKernelGGL(k_bias_act, dim3(nb(rows * 2)), dim3(256), 0, f->s, f->out, f->out, prm + f->o_bQ2, rows, 2, 0);
-    (void)dummy;
+    Epi e1;
+    e1.bias = prm + f->o_bQ1;
+    e1.act_out = f->q;
+    e1.act = 1;
+    gemm(f, f->a[f->L], prm + f->o_Q1, f->zq, (int)rows, H, C, C, H, H, 0, 0, 0, 1, 0, 0, 0.f, 0, e1);
+    Epi e2;
+    e2.bias = prm + f->o_bQ2;
+    gemm(f, f->q, prm + f->o_Q2, f->out, (int)rows, 2, H, H, 2, 2, 0, 0, 0, 1, 0, 0, 0.f, 0, e2);
 }
 
 // backward from f->dout into f->grads
@@ -561,16 +627,19 @@ void backward(Fno *f, int B) {
     // projection
     gemm_wgrad(f, f->q, f->dout, gr + f->o_Q2, H, 2, rows, H, 2);
     colsum(f, f->dout, rows, 2, gr + f->o_bQ2);
-    gemm(f, f->dout, prm + f->o_Q2, f->dq, (int)rows, H, 2, 2, 2, H, 0, 0, 0, 1, 0, 1);
-    hipLaunchKernelGGL(k_gelu_back, dim3(nb(rows * H)), dim3(256), 0, f->s, f->dq, f->zq, rows * H);
+    {
+        Epi ep;
+        ep.dmul = f->zq;
+        gemm(f, f->dout, prm + f->o_Q2, f->dq, (int)rows, H, 2, 2, 2, H, 0, 0, 0, 1, 0, 1, 0.f, 0, ep);
+    }
     gemm_wgrad(f, f->a[f->L], f->dq, gr + f->o_Q1, C, H, rows, C, H);
     colsum(f, f->dq, rows, H, gr + f->o_bQ1);
     float *dA = f->da, *dA2 = f->da2;
     gemm(f, f->dq, prm + f->o_Q1, dA, (int)rows, C, H, H, H, C, 0, 0, 0, 1, 0, 1);
     for (int l = f->L - 1; l >= 0; l--) {
-        float *a = f->a[l], *zl = f->z[l + 1];
-        // dz = dA * GELU'(z) (no activation after the last layer)
-        if (l < f->L - 1) hipLaunchKernelGGL(k_gelu_back, dim3(nb(rows * C)), dim3(256), 0, f->s, dA, zl, rows * C);
+        float *a = f->a[l];
+        // dz = dA * GELU'(z[l+1]), applied by the epilogue of the product
+        // that completed dA (no activation after the last layer)
         float *dz = dA;
         colsum(f, dz, rows, C, gr + f->o_b[l]);
         gemm_wgrad(f, a, dz, gr + f->o_W[l], C, C, rows, C, C);
@@ -597,7 +666,10 @@ void backward(Fno *f, int B) {
         gemm(f, f->Fy, f->T, f->X1, N, 2 * nm * C, 2 * nm, N, 2 * nm * C, 2 * nm * C, 0, 4LL * nm * nm * C,
              (long long)N * 2 * nm * C, B, 1);
         // dA2[b][y] (N x C) += Fx^T (N x 2nm) dX1[b][y] (2nm x C)
-        gemm(f, f->Fx, f->X1, dA2, N, C, 2 * nm, N, C, C, 0, 2LL * nm * C, (long long)N * C, B * N, 1, 0, 1.f);
+        Epi ep;
+        if (l > 0) ep.dmul = f->z[l];  // a[l] = GELU(z[l]) for 0 < l < L
+        gemm(f, f->Fx, f->X1, dA2, N, C, 2 * nm, N, C, C, 0, 2LL * nm * C, (long long)N * C, B * N, 1, 0, 1.f, 0,
+             ep);
         std::swap(dA, dA2);
     }
     // lift
