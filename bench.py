@@ -193,6 +193,153 @@ def fno_probe(ngrid=151, batch=16, reps=5):
     return 0
 
 
+def _run_steps(ctx, steps):
+    """Advance ctx over `steps` without records."""
+    if not steps:
+        return
+    arr = (g4mi.StepParams * len(steps))(*steps)
+    g4mi._chk(g4mi.lib().g4mi_run(ctx.h, arr, len(steps), W.GAMMA0, None), "g4mi_run")
+
+
+def _run_power(ctx, steps):
+    """Advance ctx over `steps`; slice power after each step, [len(steps)][ns]."""
+    r = ctx.run(steps, W.GAMMA0, record=("power",))
+    return r["power"][1:]
+
+
+def config4(a):
+    """Learned field step (config 4).  Training pairs: mixed-form config 1
+    runs (1024 slices x 8192, 151^2) with shot-noise seeds seed0.. ; on every
+    `step_stride`-th step the ADI's (E, S, E_next) of every `slice_stride`-th
+    slice are captured until `pairs` are held.  Training: Adam with a cosine
+    schedule, the last 5 % of the pairs (the last seeds' pairs) held out.
+    Deployment: config 3's beam (4096 slices, Gaussian current, uneven
+    counts) on a 151^2 grid in the mixed form with the network in place of
+    the ADI, ADI on every 8th step and when the energy-balance residual
+    exceeds 1e-3; P(z) against the FP64 GPU form (the oracle's arithmetic to
+    1e-10) and against the mixed form with the ADI alone."""
+    t_start = time.perf_counter()
+    log = lambda m: print(m, file=sys.stderr, flush=True)  # noqa: E731
+    ng = 151
+    steps = g4mi.parse_lattice(W.lattice_text(), "FEL", W.DELZ, W.LAMBDA_R, W.GAMMA0)
+    nst = len(steps)
+    npairs, nseeds = a.fno_pairs, a.fno_seeds
+    per_seed = -(-npairs // nseeds)
+    cfg1 = dict(W.CONFIG1, precision="fp32")
+    slice_stride = 64
+    per_step = -(-cfg1["nslice"] // slice_stride)
+    step_stride = max(1, (nst * per_step) // per_seed)
+    fno = g4mi.Fno(ng, width=32, modes=16, layers=4, hidden=64, max_batch=a.fno_batch, seed=1)
+    fno.reserve(npairs)
+    t0 = time.perf_counter()
+    for k in range(nseeds):
+        if fno.pair_count() >= npairs:
+            break
+        cfg = dict(cfg1, seed=a.fno_seed0 + k)
+        off, beam, field, current, _ = build_workload(cfg)
+        ctx = make_context(cfg, off, beam, field, current, g4mi.FP32)
+        del beam, field
+        for i in range(0, nst, step_stride):
+            ctx.attach_fno(fno, 1, every=slice_stride)
+            _run_steps(ctx, steps[i:i + 1])
+            ctx.attach_fno(None, 0)
+            _run_steps(ctx, steps[i + 1:i + step_stride])
+        ctx.sync()
+        ctx.close()
+        log(f"capture seed {cfg['seed']}: {fno.pair_count()} pairs, {time.perf_counter() - t0:.1f} s")
+    t_capture = time.perf_counter() - t0
+    n = fno.pair_count()
+    nh = max(1, n // 20)
+    ntr = n - nh
+    base = fno.eval(ntr, n, a.fno_train_batch)
+    p0 = fno.get_params()
+    fno.set_params(np.zeros_like(p0))
+    zero = fno.eval(ntr, n, a.fno_train_batch)  # field-unchanged prediction
+    fno.set_params(p0)
+    t0 = time.perf_counter()
+    tls, hls = [], []
+    for e in range(a.fno_epochs):
+        tl, hl = fno.train(epochs=1, batch=a.fno_train_batch, lr=a.fno_lr, nholdout=nh, seed=11,
+                           first_epoch=e, total_epochs=a.fno_epochs)
+        tls.append(float(tl[0]))
+        hls.append(float(hl[0]))
+        log(f"epoch {e}: train loss {tl[0]:.4e}, hold-out loss {hl[0]:.4e}, "
+            f"{time.perf_counter() - t0:.1f} s")
+    t_train = time.perf_counter() - t0
+
+    # deployment on config 3's beam at 151^2
+    cfg3 = dict(W.CONFIG3, ngrid=ng)
+    nd = min(nst, a.fno_deploy_steps) if a.fno_deploy_steps > 0 else nst
+    dsteps = steps[:nd]
+    off, beam, field, current, _ = build_workload(cfg3)
+    runs = {}
+    for name, prec in (("fp64", g4mi.FP64), ("mixed_adi", g4mi.FP32)):
+        ctx = make_context(cfg3, off, beam, field, current, prec)
+        ctx.sync()
+        t0 = time.perf_counter()
+        runs[name] = _run_power(ctx, dsteps)
+        ctx.sync()
+        runs[name + "_s"] = time.perf_counter() - t0
+        ctx.close()
+        log(f"deploy {name}: {runs[name + '_s']:.2f} s")
+    ctx = make_context(cfg3, off, beam, field, current, g4mi.FP32)
+    del beam, field
+    ctx.attach_fno(fno, 2, every=8, threshold=1e-3)
+    ctx.sync()
+    t0 = time.perf_counter()
+    parts = []
+    for i in range(0, nd, 16):
+        parts.append(_run_power(ctx, dsteps[i:i + 16]))
+        log(f"deploy fno: step {min(nd, i + 16)}/{nd}, {time.perf_counter() - t0:.1f} s")
+    ctx.sync()
+    t_fno = time.perf_counter() - t0
+    stats, judge = ctx.fno_stats(), ctx.fno_judge()
+    ctx.close()
+    fno.close()
+    pf = np.concatenate(parts).sum(axis=1)
+    p64, pm = runs["fp64"].sum(axis=1), runs["mixed_adi"].sum(axis=1)
+    floor = 1e-6 * p64.max()  # relative errors where P is above 1e-6 of its maximum
+    mask = p64 > floor
+
+    def rel(x):
+        return float(np.max(np.abs(x[mask] / p64[mask] - 1))) if mask.any() else None
+
+    out = {
+        "metric": "learned field step: P(z) error and fallback rate",
+        "config": {"workload": "config4: FNO (4 layers, width 32, 16x16 modes, hidden 64) on "
+                               "151x151; pairs from mixed-form config 1 runs; deployed on config 3's "
+                               "beam (4096 slices, Gaussian current) at 151x151",
+                   "pairs": n, "holdout_pairs": nh, "seeds": f"{a.fno_seed0}..{a.fno_seed0 + nseeds - 1}",
+                   "capture": f"every {step_stride}th step, every {slice_stride}th slice",
+                   "epochs": a.fno_epochs, "train_batch": a.fno_train_batch, "lr": a.fno_lr,
+                   "deploy_steps": nd},
+        "dtype": "bf16 MFMA, f32 accumulate (network); f32 mixed form (beam, field)",
+        "data": "synthetic: Hammersley quiet loading + shot noise, as LoadBeam",
+        "training": {"capture_s": round(t_capture, 1), "train_s": round(t_train, 1),
+                     "samples_per_s": round(ntr * a.fno_epochs / t_train, 1),
+                     "train_loss": tls, "holdout_loss": hls,
+                     "holdout_loss_untrained": base, "holdout_loss_field_unchanged": zero},
+        "deployment": {"fno_steps": stats["fno_steps"], "adi_steps": stats["adi_steps"],
+                       "fallback_rate": round(stats["adi_steps"] / max(1, nd), 4),
+                       "forced_every_8": -(-nd // 8),
+                       "residual_fallbacks": stats["adi_steps"] - -(-nd // 8),
+                       "mean_residual_fno": stats["mean_residual_fno"],
+                       "mean_residual_adi": stats["mean_residual_adi"],
+                       "judge": judge,
+                       "p_rel_err_max_fno_vs_fp64": rel(pf),
+                       "p_rel_err_end_fno_vs_fp64": float(abs(pf[-1] / p64[-1] - 1)),
+                       "p_rel_err_max_mixed_adi_vs_fp64": rel(pm),
+                       "p_rel_err_end_mixed_adi_vs_fp64": float(abs(pm[-1] / p64[-1] - 1)),
+                       "p_end_w": {"fp64": float(p64[-1]), "mixed_adi": float(pm[-1]), "fno": float(pf[-1])},
+                       "s_per_step": {"fno_run": round(t_fno / nd, 5),
+                                      "mixed_adi_run": round(runs["mixed_adi_s"] / nd, 5),
+                                      "fp64_run": round(runs["fp64_s"] / nd, 5)}},
+        "total_s": round(time.perf_counter() - t_start, 1),
+    }
+    print(json.dumps(out), flush=True)
+    return 0
+
+
 def copy_probe(mib):
     """A plain device-to-device copy of a known size: its FETCH_SIZE and
     WRITE_SIZE per dispatch, against mib MiB, give the counters' scale."""
@@ -212,8 +359,9 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=100)
     ap.add_argument("--warmup", type=int, default=10)
-    ap.add_argument("--config", choices=("config1", "config0", "config2", "config3"),
-                    default="config1")
+    ap.add_argument("--config", choices=("config1", "config0", "config2", "config3", "config4"),
+                    default="config1",
+                    help="config4 runs the learned field step: capture, training, deployment")
     ap.add_argument("--precision", choices=("fp64", "fp32"), default=None,
                     help="override the configuration's precision")
     ap.add_argument("--cpu-steps", type=int, default=2)
@@ -229,11 +377,21 @@ def main():
                     help="only run a plain device copy of MIB MiB five times (the reference "
                          "point that scales the FETCH_SIZE/WRITE_SIZE counters) and exit")
     ap.add_argument("--fno-probe", action="store_true", help="only time the learned field step")
+    ap.add_argument("--fno-pairs", type=int, default=200000)
+    ap.add_argument("--fno-seeds", type=int, default=100)
+    ap.add_argument("--fno-seed0", type=int, default=100)
+    ap.add_argument("--fno-epochs", type=int, default=3)
+    ap.add_argument("--fno-lr", type=float, default=1e-3)
+    ap.add_argument("--fno-batch", type=int, default=64, help="slices per inference pass")
+    ap.add_argument("--fno-train-batch", type=int, default=16)
+    ap.add_argument("--fno-deploy-steps", type=int, default=0, help="0: the whole lattice")
     a = ap.parse_args()
     if a.copy_probe:
         return copy_probe(a.copy_probe)
     if a.fno_probe:
         return fno_probe()
+    if a.config == "config4" and a.gpus == 1:
+        return config4(a)
     if a.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; the slice-block sharding "
                                    "across GPUs is a design, not an implementation",

@@ -114,6 +114,12 @@ struct Ctx {
     double fno_thr = 1e-3;
     long long fno_steps = 0, fno_adi_steps = 0, fno_nres_adi = 0;
     double fno_res_sum = 0, fno_res_adi_sum = 0;
+    // judge steps (every fno_every-th step of replace mode): the network's
+    // step is kept in m_judge and compared with the ADI's
+    float2 *m_judge = nullptr;
+    double *d_judge = nullptr;
+    long long fno_njudge = 0, fno_nres_all = 0;
+    double fno_judge_field = 0, fno_judge_update = 0, fno_res_all_sum = 0;
 };
 
 template <class T>
@@ -850,12 +856,40 @@ static int mix_step_fno(Ctx *c, const MixKey &k) {
         if (n > 0) dpb += c->h_cur[i] * kEev * eg[i] / (double)n;
     }
     const double res = std::fabs(p1 - p0 + dpb - pself) / std::max(p1, 1e-300);
+    c->fno_res_all_sum += res;
+    c->fno_nres_all++;
     const long long stepno = c->fno_steps + c->fno_adi_steps;
-    const bool use_adi = (c->fno_every > 0 && stepno % c->fno_every == 0) || !(res <= c->fno_thr);
+    const bool judge = c->fno_every > 0 && stepno % c->fno_every == 0;
+    const bool use_adi = judge || !(res <= c->fno_thr);
     if (use_adi) {
+        if (judge) {
+            if (!c->m_judge) {
+                G4_TRY(dalloc(c, &c->m_judge, (size_t)ns * P));
+                G4_TRY(dalloc(c, &c->d_judge, (size_t)ns * 3));
+            }
+            G4_HIP(hipMemcpyAsync(c->m_judge, c->m_field[hs.cur ^ 1], (size_t)ns * P * sizeof(float2),
+                                  hipMemcpyDeviceToDevice, c->stream));
+        }
         if (launch_mix_adi(c->stream, a, tm, refine, stream, &ntiles)) {
             set_error("mixed form: no ADI instantiation for this band width");
             return G4MI_E_ARG;
+        }
+        if (judge) {
+            // the network's step against the ADI's, over every slice
+            fno_judge_sums(c->stream, c->m_judge, c->m_field[hs.cur ^ 1], c->m_field[hs.cur], ns, (int)P,
+                           c->d_judge);
+            std::vector<double> js(3 * (size_t)ns);
+            G4_HIP(hipMemcpyAsync(js.data(), c->d_judge, js.size() * 8, hipMemcpyDeviceToHost, c->stream));
+            G4_HIP(hipStreamSynchronize(c->stream));
+            double e = 0, u = 0, t = 0;
+            for (int j = 0; j < ns; j++) {
+                e += js[3 * (size_t)j];
+                u += js[3 * (size_t)j + 1];
+                t += js[3 * (size_t)j + 2];
+            }
+            if (t > 0) c->fno_judge_field += std::sqrt(e / t);
+            if (u > 0) c->fno_judge_update += std::sqrt(e / u);
+            c->fno_njudge++;
         }
         // the ADI's own residual, for reference
         std::vector<double> hfa(8 * (size_t)ns);
@@ -1449,8 +1483,21 @@ int g4mi_attach_fno(g4mi_ctx *h, g4mi_fno *fh, int mode, int every, double thres
     c->fno_mode = mode;
     c->fno_every = every;
     c->fno_thr = threshold;
-    c->fno_steps = c->fno_adi_steps = c->fno_nres_adi = 0;
-    c->fno_res_sum = c->fno_res_adi_sum = 0;
+    c->fno_steps = c->fno_adi_steps = c->fno_nres_adi = c->fno_njudge = c->fno_nres_all = 0;
+    c->fno_res_sum = c->fno_res_adi_sum = c->fno_judge_field = c->fno_judge_update = c->fno_res_all_sum = 0;
+    return G4MI_OK;
+}
+
+int g4mi_fno_judge(g4mi_ctx *h, long long *njudge, double *rel_field_err, double *rel_update_err,
+                   double *mean_residual_all) {
+    if (!h) return G4MI_E_ARG;
+    const Ctx *c = &h->c;
+    const double n = (double)std::max(1LL, c->fno_njudge);
+    if (njudge) *njudge = c->fno_njudge;
+    if (rel_field_err) *rel_field_err = c->fno_judge_field / n;
+    if (rel_update_err) *rel_update_err = c->fno_judge_update / n;
+    if (mean_residual_all)
+        *mean_residual_all = c->fno_nres_all ? c->fno_res_all_sum / (double)c->fno_nres_all : 0.0;
     return G4MI_OK;
 }
 

@@ -1,16 +1,971 @@
-// Input-deck driver (namelist + lattice -> GPU run).  Filled in below.
-#include <string>
+// Input-deck driver: a GENESIS 1.3 v4 namelist file and its lattice file ->
+// generated beam and field -> the undulator steps on the device ->
+// <rootname>.out.h5 in the reference's output layout.
+//
+// Covered elements (src/Main/GenMain.cpp:80-494 dispatch order, element by
+// element in file order): &setup, &lattice (zmatch), &time, &field, &beam,
+// &efield, &track and &stop.  Any other element, or a key an element does
+// not know, stops the run with an error, as the reference does for unknown
+// keys.  Profile references ("@name") are not supported.
+#include <sys/stat.h>
+#include <unistd.h>
+#include <pwd.h>
 
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <ctime>
+#include <fstream>
+#include <map>
+#include <sstream>
+#include <string>
+#include <vector>
+
+#include "g4mi_h5.h"
 #include "g4mi_host.h"
 
 namespace g4mi {
 void set_error(const std::string &msg);
+
+namespace {
+
+constexpr double kVacimpD = 376.73;          // GenMain.cpp:62
+constexpr double kEevD = 510998.95069;       // GenMain.cpp:66
+
+// ---------------------------------------------------------------------------
+// namelist parsing (Parser::parse, src/Main/Parser.cpp:40-92; fillMap :94-124)
+// ---------------------------------------------------------------------------
+std::string trim_st(const std::string &s) {  // spaces and tabs only
+    const size_t a = s.find_first_not_of(" \t"), b = s.find_last_not_of(" \t");
+    return (a == std::string::npos || b == std::string::npos) ? std::string() : s.substr(a, b - a + 1);
 }
 
+bool to_bool(std::string v) {  // StringProcessing::atob
+    std::transform(v.begin(), v.end(), v.begin(), [](unsigned char c) { return (char)std::tolower(c); });
+    return v == "1" || v == "true" || v == "t";
+}
+
+struct Namelist {
+    std::string name;                         // lower case, with the leading '&'
+    std::map<std::string, std::string> kv;    // first occurrence of a key wins
+};
+
+bool parse_namelists(const std::string &text, std::vector<Namelist> &out, std::string &err) {
+    std::istringstream in(text);
+    std::string line, list;
+    bool open = false;
+    Namelist cur;
+    while (std::getline(in, line)) {
+        if (!line.empty() && line.back() == '\r') line.pop_back();
+        line = trim_st(line);
+        if (line.empty() || line[0] == '#') continue;
+        if (line == "&end" || line == "&END" || line == "&End") {
+            if (!open) {
+                err = "termination string outside element definition in input file";
+                return false;
+            }
+            // split into key = value pairs
+            size_t pos;
+            while ((pos = list.find(';')) != std::string::npos) {
+                const std::string item = list.substr(0, pos);
+                list.erase(0, pos + 1);
+                if (item.empty()) continue;
+                const size_t eq = item.find('=');
+                if (eq == std::string::npos) {
+                    err = "invalid format " + item + " in input file";
+                    return false;
+                }
+                const std::string k = trim_st(item.substr(0, eq)), v = trim_st(item.substr(eq + 1));
+                cur.kv.emplace(k, v);
+            }
+            out.push_back(cur);
+            cur = Namelist();
+            list.clear();
+            open = false;
+            continue;
+        }
+        if (line[0] == '&') {
+            if (open) {
+                err = "nested elements in main input file";
+                return false;
+            }
+            open = true;
+            cur.name = line;
+            std::transform(cur.name.begin(), cur.name.end(), cur.name.begin(),
+                           [](unsigned char c) { return (char)std::tolower(c); });
+            continue;
+        }
+        if (open) {
+            list += line;
+            list += ";";
+        }
+    }
+    if (open) {
+        err = "reached end of input file while parsing namelist '" + cur.name + "'";
+        return false;
+    }
+    return true;
+}
+
+// typed key extraction; every key taken is erased so the leftovers are unknown keys
+struct Keys {
+    Namelist &nl;
+    std::string err;
+    bool has(const char *k) const { return nl.kv.count(k) > 0; }
+    std::string take(const char *k) {
+        std::string v = nl.kv.at(k);
+        nl.kv.erase(k);
+        return v;
+    }
+    void str(const char *k, std::string &v) {
+        if (has(k)) v = take(k);
+    }
+    void dbl(const char *k, double &v) {
+        if (!has(k)) return;
+        const std::string s = take(k);
+        if (s.find('@') != std::string::npos) {
+            if (err.empty()) err = "profile reference '" + s + "' for " + k + " in " + nl.name + " is not supported";
+            return;
+        }
+        v = atof(s.c_str());
+    }
+    void integer(const char *k, int &v) {
+        if (has(k)) v = atoi(take(k).c_str());
+    }
+    void boolean(const char *k, bool &v) {
+        if (has(k)) v = to_bool(take(k));
+    }
+    bool finish() {
+        if (!err.empty()) return false;
+        if (!nl.kv.empty()) {
+            err = "unknown elements in " + nl.name + ":";
+            for (auto &e : nl.kv) err += " " + e.first;
+            return false;
+        }
+        return true;
+    }
+};
+
+// ---------------------------------------------------------------------------
+// deck state
+// ---------------------------------------------------------------------------
+struct SetupNL {  // Setup::Setup defaults (src/Main/Setup.cpp:7-44)
+    std::string rootname, outputdir, lattice, beamline;
+    double gamma0 = 5800 / 0.511, lambda0 = 1e-10, delz = 0.015;
+    int seed = 123456789, npart = 8192, nbins = 4;
+    bool one4one = false, shotnoise = true, beam_global_stat = false, field_global_stat = false;
+    bool excl_spatial = false, excl_fft = false, excl_intensity = false, excl_energy = false,
+         excl_aux = false, excl_current = true, excl_twiss = true, excl_field_dump = false;
+    bool defined = false;
+};
+
+struct TimeNL {  // Time::Time / Time::init (src/Main/GenTime.cpp)
+    bool dotime = false;
+    double s0 = 0, slen = 0;
+    int sample = 1;
+    int nslice = 1;
+    double ds = 0;
+};
+
+struct EFieldNL {
+    double rmax = 0;
+    int ngrid = 100, nz = 0, nphi = 0;
+    bool longrange = false;
+};
+
+struct Deck {
+    std::string input_file, input_text, lattice_text, lattice_path;
+    SetupNL setup;
+    TimeNL time;
+    EFieldNL efield;
+    bool matched = false;
+    double mbetax = 0, malphax = 0, mbetay = 0, malphay = 0, mphix = 0, mphiy = 0, zmatch = 0;
+    // generated beam (host copy of the loaded state) and field
+    std::vector<long long> off;
+    std::vector<double> gamma, theta, x, y, px, py, current;
+    std::vector<double> field;  // [ns][ng^2][2]
+    int ngrid = 0;
+    double gridmax = 0, flambda = 0;
+    bool have_beam = false, have_field = false;
+    // device context (created by the first &track)
+    g4mi_ctx *ctx = nullptr;
+    int precision = G4MI_FP64;
+    bool periodic = false;
+    int runs = 0;
+    std::string summary;  // JSON of what was set up (check mode)
+};
+
+int fail(const std::string &msg, int code = G4MI_E_PARSE) {
+    set_error("g4mi_run_deck: " + msg);
+    return code;
+}
+
+bool read_file(const std::string &path, std::string &out) {
+    std::ifstream f(path.c_str(), std::ios::in | std::ios::binary);
+    if (!f) return false;
+    std::stringstream ss;
+    ss << f.rdbuf();
+    out = ss.str();
+    return true;
+}
+
+std::string dir_of(const std::string &path) {
+    const size_t p = path.find_last_of('/');
+    return p == std::string::npos ? std::string(".") : path.substr(0, p);
+}
+
+// ---------------------------------------------------------------------------
+// Lattice::match (src/Lattice/Lattice.cpp:240-288) with Optics
+// (src/Lattice/Optics.cpp): the lattice unrolled at element resolution (a
+// step length of 20 m), the transfer matrices of every piece up to z0
+// multiplied in order, and the periodic Twiss functions of the product.
+// ---------------------------------------------------------------------------
+struct Mat2 {
+    double a = 1, b = 0, c = 0, d = 1;
+};
+Mat2 piece(double k1, double L) {  // Optics::getQuad / getDrift
+    Mat2 m;
+    if (k1 == 0) {
+        m.b = L;
+        return m;
+    }
+    const double sk = sqrt(fabs(k1)), w = sk * L;
+    if (k1 > 0) {
+        m.a = cos(w);
+        m.b = sin(w) / sk;
+        m.c = -sin(w) * sk;
+        m.d = cos(w);
+    } else {
+        m.a = cosh(w);
+        m.b = sinh(w) / sk;
+        m.c = sinh(w) * sk;
+        m.d = cosh(w);
+    }
+    return m;
+}
+Mat2 left_mul(const Mat2 &m, const Mat2 &t) {  // m * t
+    Mat2 r;
+    r.a = m.a * t.a + m.b * t.c;
+    r.b = m.a * t.b + m.b * t.d;
+    r.c = m.c * t.a + m.d * t.c;
+    r.d = m.c * t.b + m.d * t.d;
+    return r;
+}
+
+bool match_lattice(Deck &dk, std::string &err) {
+    const SetupNL &st = dk.setup;
+    LatticeResult lr = parse_and_unroll(dk.lattice_text, st.beamline, 20.0, st.lambda0, st.gamma0);
+    if (!lr.error.empty()) {
+        err = lr.error;
+        return false;
+    }
+    const auto &s = lr.steps;
+    const double z0 = dk.zmatch, g = st.gamma0;
+    Mat2 mx, my;
+    size_t i = 0;
+    while (i < s.size() && s[i].z < z0) {
+        const double aw = s[i].aw;
+        const double gref2 = g * g - 1 - aw * aw;
+        const double qf = s[i].qf * g * g / gref2;
+        double qx = aw * aw / gref2;
+        const double qy = qx * s[i].ky;
+        qx *= s[i].kx;
+        double dz = s[i].dz;
+        if (s[i].z + s[i].dz > z0) dz = z0 - s[i].z;
+        mx = left_mul(piece((qf + qx) == 0 ? 0.0 : qf + qx, dz), mx);
+        my = left_mul(piece((-qf + qy) == 0 ? 0.0 : -qf + qy, dz), my);
+        i++;
+    }
+    if (i == s.size() && s.size() > 0 && s.back().z + s.back().dz < z0) {
+        dk.matched = false;  // matching position beyond the lattice: ignored
+        return true;
+    }
+    double arg = 2 - mx.a * mx.a - 2 * mx.b * mx.c - mx.d * mx.d;
+    if (arg <= 0) {
+        dk.matched = false;
+        return true;
+    }
+    dk.mbetax = 2 * mx.b / sqrt(arg);
+    dk.malphax = (mx.a - mx.d) / sqrt(arg);
+    dk.mphix = acos(0.5 * (mx.a + mx.d)) * 90 / asin(1);
+    arg = 2 - my.a * my.a - 2 * my.b * my.c - my.d * my.d;
+    if (arg <= 0) {
+        dk.matched = false;
+        return true;
+    }
+    dk.mbetay = 2 * my.b / sqrt(arg);
+    dk.malphay = (my.a - my.d) / sqrt(arg);
+    dk.mphiy = acos(0.5 * (my.a + my.d)) * 90. / asin(1.);
+    dk.matched = true;
+    return true;
+}
+
+// ---------------------------------------------------------------------------
+// elements
+// ---------------------------------------------------------------------------
+bool do_setup(Deck &dk, Namelist &nl, std::string &err) {
+    Keys k{nl, ""};
+    SetupNL &s = dk.setup;
+    k.str("rootname", s.rootname);
+    k.str("outputdir", s.outputdir);
+    k.str("lattice", s.lattice);
+    k.str("beamline", s.beamline);
+    k.dbl("gamma0", s.gamma0);
+    k.dbl("lambda0", s.lambda0);
+    k.dbl("delz", s.delz);
+    k.integer("seed", s.seed);
+    k.boolean("one4one", s.one4one);
+    k.integer("npart", s.npart);
+    k.integer("nbins", s.nbins);
+    k.boolean("shotnoise", s.shotnoise);
+    k.boolean("beam_global_stat", s.beam_global_stat);
+    k.boolean("field_global_stat", s.field_global_stat);
+    k.boolean("exclude_spatial_output", s.excl_spatial);
+    k.boolean("exclude_fft_output", s.excl_fft);
+    k.boolean("exclude_intensity_output", s.excl_intensity);
+    k.boolean("exclude_energy_output", s.excl_energy);
+    k.boolean("exclude_aux_output", s.excl_aux);
+    k.boolean("exclude_current_output", s.excl_current);
+    k.boolean("exclude_twiss_output", s.excl_twiss);
+    k.boolean("exclude_field_dump", s.excl_field_dump);
+    bool dummy = false;
+    k.boolean("write_meta_file", dummy);
+    if (!k.finish()) {
+        err = k.err;
+        return false;
+    }
+    if (s.one4one) {
+        err = "one4one = true is not supported by this build";
+        return false;
+    }
+    std::string path = s.lattice;
+    if (!read_file(path, dk.lattice_text)) {
+        path = dir_of(dk.input_file) + "/" + s.lattice;
+        if (s.lattice.empty() || s.lattice[0] == '/' || !read_file(path, dk.lattice_text)) {
+            err = "cannot open lattice file " + s.lattice;
+            return false;
+        }
+    }
+    dk.lattice_path = path;
+    LatticeResult lr = parse_and_unroll(dk.lattice_text, s.beamline, s.delz, s.lambda0, s.gamma0);
+    if (!lr.error.empty()) {
+        err = "lattice: " + lr.error;
+        return false;
+    }
+    s.defined = true;
+    return true;
+}
+
+bool do_lattice(Deck &dk, Namelist &nl, std::string &err) {
+    Keys k{nl, ""};
+    k.dbl("zmatch", dk.zmatch);
+    if (!k.finish()) {
+        err = k.err + " (only zmatch is supported)";
+        return false;
+    }
+    if (dk.zmatch > 0) return match_lattice(dk, err);
+    return true;
+}
+
+bool do_time(Deck &dk, Namelist &nl, std::string &err) {
+    if (dk.have_beam || dk.have_field) return true;  // ignored once beam or field exist (GenMain.cpp)
+    Keys k{nl, ""};
+    TimeNL &t = dk.time;
+    t.dotime = true;
+    k.dbl("s0", t.s0);
+    k.dbl("slen", t.slen);
+    k.integer("sample", t.sample);
+    k.boolean("time", t.dotime);
+    if (!k.finish()) {
+        err = k.err;
+        return false;
+    }
+    if (t.sample < 1) {
+        err = "sample rate must be at least 1";
+        return false;
+    }
+    // Time::finishInit (GenTime.cpp) on one rank
+    t.ds = dk.setup.lambda0 * t.sample;
+    t.nslice = std::max(1, (int)std::round(t.slen / t.ds));
+    t.slen = t.ds * t.nslice;
+    return true;
+}
+
+int time_nslice(const Deck &dk) { return dk.time.dotime || dk.time.ds > 0 ? dk.time.nslice : 1; }
+double time_ds(const Deck &dk) { return dk.time.ds > 0 ? dk.time.ds : dk.setup.lambda0 * dk.time.sample; }
+
+bool do_field(Deck &dk, Namelist &nl, std::string &err) {
+    Keys k{nl, ""};
+    double lambda = dk.setup.lambda0, power = 0, phase = 0, z0 = 0, w0 = 100e-6;
+    double xcen = 0, ycen = 0, xangle = 0, yangle = 0, dgrid = 1e-3;
+    int ngrid = 151, harm = 1, nx = 0, ny = 0;
+    bool add = false;
+    k.dbl("lambda", lambda);
+    k.dbl("power", power);
+    k.dbl("phase", phase);
+    k.dbl("waist_pos", z0);
+    k.dbl("waist_size", w0);
+    k.dbl("xcenter", xcen);
+    k.dbl("ycenter", ycen);
+    k.dbl("xangle", xangle);
+    k.dbl("yangle", yangle);
+    k.dbl("dgrid", dgrid);
+    k.integer("ngrid", ngrid);
+    k.integer("harm", harm);
+    k.integer("nx", nx);
+    k.integer("ny", ny);
+    k.boolean("accumulate", add);
+    if (!k.finish()) {
+        err = k.err;
+        return false;
+    }
+    if (harm != 1 || nx != 0 || ny != 0) {
+        err = "&field: only the fundamental (harm = 1) TEM00 mode (nx = ny = 0) is supported";
+        return false;
+    }
+    if (add && dk.have_field) {
+        dgrid = dk.gridmax;
+        ngrid = dk.ngrid;
+    } else {
+        add = false;
+    }
+    if (ngrid < 3 || dgrid <= 0) {
+        err = "&field: invalid grid";
+        return false;
+    }
+    const int ns = time_nslice(dk);
+    const double ds = time_ds(dk);
+    const double l0 = dk.setup.lambda0;
+    const double dlam = -(lambda - l0) / l0 / l0 * 4 * asin(1);  // LoadField.cpp
+    const size_t cells = (size_t)ngrid * ngrid;
+    if (!add) dk.field.assign((size_t)ns * cells * 2, 0.0);
+    std::vector<double> slice(cells * 2);
+    for (int i = 0; i < ns; i++) {
+        const double s = dk.time.s0 + i * ds;
+        gauss_field(slice.data(), ngrid, dgrid, lambda, power, w0, z0, fmod(phase + s * dlam, 4 * asin(1.)), xcen,
+                    ycen, xangle, yangle);
+        double *dst = dk.field.data() + (size_t)i * cells * 2;
+        if (add)
+            for (size_t q = 0; q < cells * 2; q++) dst[q] += slice[q];
+        else
+            std::memcpy(dst, slice.data(), cells * 2 * sizeof(double));
+    }
+    dk.ngrid = ngrid;
+    dk.gridmax = dgrid;
+    dk.flambda = lambda;
+    dk.have_field = true;
+    return true;
+}
+
+bool do_beam(Deck &dk, Namelist &nl, std::string &err) {
+    if (dk.have_beam) {
+        err = "cannot generate beam, because beam is already defined";
+        return false;
+    }
+    Keys k{nl, ""};
+    g4mi_slice_beam sb{};
+    sb.gamma = dk.setup.gamma0;
+    sb.delgam = 0;
+    sb.current = 1000;
+    sb.ex = sb.ey = 0.3e-6;
+    sb.betax = sb.betay = 15;
+    if (dk.matched) {
+        sb.betax = dk.mbetax;
+        sb.alphax = dk.malphax;
+        sb.betay = dk.mbetay;
+        sb.alphay = dk.malphay;
+    }
+    k.dbl("gamma", sb.gamma);
+    k.dbl("delgam", sb.delgam);
+    k.dbl("current", sb.current);
+    k.dbl("ex", sb.ex);
+    k.dbl("ey", sb.ey);
+    k.dbl("betax", sb.betax);
+    k.dbl("betay", sb.betay);
+    k.dbl("alphax", sb.alphax);
+    k.dbl("alphay", sb.alphay);
+    k.dbl("xcenter", sb.xcen);
+    k.dbl("ycenter", sb.ycen);
+    k.dbl("pxcenter", sb.pxcen);
+    k.dbl("pycenter", sb.pycen);
+    k.dbl("bunch", sb.bunch);
+    k.dbl("bunchphase", sb.bunchphase);
+    k.dbl("emod", sb.emod);
+    k.dbl("emodphase", sb.emodphase);
+    if (!k.finish()) {
+        err = k.err;
+        return false;
+    }
+    const SetupNL &st = dk.setup;
+    if (st.npart % st.nbins != 0) {
+        err = "NPART is not a multiple of NBINS";
+        return false;
+    }
+    const int ns = time_nslice(dk);
+    dk.off.assign(ns + 1, 0);
+    for (int i = 0; i < ns; i++) dk.off[i + 1] = dk.off[i] + st.npart;
+    const size_t n = (size_t)dk.off[ns];
+    for (auto *v : {&dk.gamma, &dk.theta, &dk.x, &dk.y, &dk.px, &dk.py}) v->assign(n, 0.0);
+    std::vector<g4mi_slice_beam> slices(ns, sb);
+    const int rc = g4mi_load_beam(ns, dk.off.data(), slices.data(), st.nbins, st.shotnoise ? 1 : 0, st.seed, 0,
+                                  st.lambda0, (double)dk.time.sample, dk.time.dotime ? 1 : 0, dk.gamma.data(),
+                                  dk.theta.data(), dk.x.data(), dk.y.data(), dk.px.data(), dk.py.data());
+    if (rc != G4MI_OK) {
+        err = "beam loading failed";
+        return false;
+    }
+    dk.current.assign(ns, sb.current);
+    dk.have_beam = true;
+    return true;
+}
+
+bool do_efield(Deck &dk, Namelist &nl, std::string &err) {
+    Keys k{nl, ""};
+    EFieldNL &e = dk.efield;
+    k.dbl("rmax", e.rmax);
+    k.integer("ngrid", e.ngrid);
+    k.integer("nz", e.nz);
+    k.integer("nphi", e.nphi);
+    k.boolean("longrange", e.longrange);
+    if (!k.finish()) {
+        err = k.err;
+        return false;
+    }
+    return true;
+}
+
+// per-slice twiss moments of the current beam state (DiagBeam::getValues,
+// src/Core/Diagnostic.cpp:415-545, the values stored once at iz = 0)
+void twiss_once(const Deck &dk, std::vector<double> *out /* ex, ey, bx, by, ax, ay */) {
+    const int ns = (int)dk.off.size() - 1;
+    for (int q = 0; q < 6; q++) out[q].assign(ns, 0.0);
+    for (int is = 0; is < ns; is++) {
+        double x1 = 0, x2 = 0, y1 = 0, y2 = 0, px1 = 0, px2 = 0, py1 = 0, py2 = 0, xpx = 0, ypy = 0, g1 = 0;
+        for (long long i = dk.off[is]; i < dk.off[is + 1]; i++) {
+            x1 += dk.x[i];
+            x2 += dk.x[i] * dk.x[i];
+            y1 += dk.y[i];
+            y2 += dk.y[i] * dk.y[i];
+            g1 += dk.gamma[i];
+            px1 += dk.px[i];
+            py1 += dk.py[i];
+            px2 += dk.px[i] * dk.px[i];
+            py2 += dk.py[i] * dk.py[i];
+            xpx += dk.x[i] * dk.px[i];
+            ypy += dk.y[i] * dk.py[i];
+        }
+        const long long n = dk.off[is + 1] - dk.off[is];
+        const double norm = n > 0 ? 1. / (double)n : 1.;
+        x1 *= norm, x2 *= norm, y1 *= norm, y2 *= norm, px1 *= norm, px2 *= norm, py1 *= norm, py2 *= norm;
+        g1 *= norm, xpx *= norm, ypy *= norm;
+        const double ex = sqrt(fabs((x2 - x1 * x1) * (px2 - px1 * px1) - (xpx - x1 * px1) * (xpx - x1 * px1)));
+        const double ey = sqrt(fabs((y2 - y1 * y1) * (py2 - py1 * py1) - (ypy - y1 * py1) * (ypy - y1 * py1)));
+        out[0][is] = ex;
+        out[1][is] = ey;
+        out[2][is] = (x2 - x1 * x1) / ex * g1;
+        out[3][is] = (y2 - y1 * y1) / ey * g1;
+        out[4][is] = -(xpx - x1 * px1) / ex;
+        out[5][is] = -(ypy - y1 * py1) / ey;
+    }
+}
+
+std::string env_or(const char *k, const char *dflt) {
+    const char *v = getenv(k);
+    return v ? std::string(v) : std::string(dflt);
+}
+
+// Writes <rootname>.out.h5: /Meta, /Global, /Lattice, /Beam, /Field
+// (Diagnostic::writeToOutputFile, src/Core/Diagnostic.cpp:135-245;
+// Output::writeMeta :67-160; HDF5Base writers).
+bool write_output(const Deck &dk, const std::string &fname, const std::vector<g4mi_step_params> &steps,
+                  const std::vector<double> &zout, const std::map<std::string, std::vector<double>> &rec,
+                  size_t nrec, std::string &err) {
+    H5File f(fname);
+    if (!f.ok()) {
+        err = "unable to open output file " + fname + ": " + f.error();
+        return false;
+    }
+    const SetupNL &st = dk.setup;
+    const int ns = (int)dk.off.size() - 1;
+    bool ok = true;
+    // Meta
+    ok &= f.write_single("Meta/Version/Major", " ", {4.0});
+    ok &= f.write_single("Meta/Version/Minor", " ", {6.0});
+    ok &= f.write_single("Meta/Version/Revision", " ", {0.0});
+    ok &= f.write_single("Meta/Version/Beta", " ", {0.0});
+    ok &= f.write_string("Meta/Version/Build_Info", "g4mi: undulator step on MI355X (HIP), output layout of GENESIS 1.3 v4");
+    {
+        time_t now = time(nullptr);
+        char buf[64];
+        ctime_r(&now, buf);
+        ok &= f.write_string("Meta/TimeStamp", buf);
+    }
+    ok &= f.write_string("Meta/HOST", env_or("HOST", "Undefined"));
+    {
+        struct passwd *pw = getpwuid(getuid());
+        ok &= f.write_string("Meta/User", pw ? pw->pw_name : "username lookup failed");
+        char cwd[4096];
+        ok &= f.write_string("Meta/cwd", getcwd(cwd, sizeof(cwd)) ? cwd : "getcwd call failed");
+    }
+    ok &= f.write_string("Meta/InputFile", dk.input_text);
+    ok &= f.write_string("Meta/LatticeFile", dk.lattice_text);
+    ok &= f.write_int("Meta/Fielddumps/ndumps", {0});
+    ok &= f.write_int("Meta/Beamdumps/ndumps", {0});
+    ok &= f.write_single("Meta/mpisize", " ", {1.0});
+    // Lattice: per-step arrays and the output positions
+    auto col = [&](double g4mi_step_params::*m) {
+        std::vector<double> v(steps.size());
+        for (size_t i = 0; i < steps.size(); i++) v[i] = steps[i].*m;
+        return v;
+    };
+    ok &= f.write_single("Lattice/zplot", "m", zout);
+    ok &= f.write_single("Lattice/z", "m", col(&g4mi_step_params::z));
+    ok &= f.write_single("Lattice/dz", "m", col(&g4mi_step_params::dz));
+    ok &= f.write_single("Lattice/aw", " ", col(&g4mi_step_params::aw));
+    ok &= f.write_single("Lattice/ax", "m", col(&g4mi_step_params::ax));
+    ok &= f.write_single("Lattice/ay", "m", col(&g4mi_step_params::ay));
+    ok &= f.write_single("Lattice/ku", "m^{-1}", col(&g4mi_step_params::ku));
+    ok &= f.write_single("Lattice/kx", " ", col(&g4mi_step_params::kx));
+    ok &= f.write_single("Lattice/ky", " ", col(&g4mi_step_params::ky));
+    ok &= f.write_single("Lattice/qf", "m^{-2}", col(&g4mi_step_params::qf));
+    ok &= f.write_single("Lattice/qx", "m", col(&g4mi_step_params::qx));
+    ok &= f.write_single("Lattice/qy", "m", col(&g4mi_step_params::qy));
+    ok &= f.write_single("Lattice/cx", "rad", col(&g4mi_step_params::cx));
+    ok &= f.write_single("Lattice/cy", "rad", col(&g4mi_step_params::cy));
+    ok &= f.write_single("Lattice/gradx", "m^{-1}", col(&g4mi_step_params::gradx));
+    ok &= f.write_single("Lattice/grady", "m^{-1}", col(&g4mi_step_params::grady));
+    ok &= f.write_single("Lattice/slippage", " ", col(&g4mi_step_params::slip));
+    ok &= f.write_single("Lattice/phaseshift", "rad", col(&g4mi_step_params::phaseshift));
+    ok &= f.write_single("Lattice/chic_angle", "degree", col(&g4mi_step_params::chic_angle));
+    ok &= f.write_single("Lattice/chic_lb", "m", col(&g4mi_step_params::chic_lb));
+    ok &= f.write_single("Lattice/chic_ld", "m", col(&g4mi_step_params::chic_ld));
+    ok &= f.write_single("Lattice/chic_lt", "m", col(&g4mi_step_params::chic_lt));
+    // Global
+    const double slicelength = st.lambda0 * dk.time.sample;
+    ok &= f.write_single("Global/gamma0", "mc^2", {st.gamma0});
+    ok &= f.write_single("Global/lambdaref", "m", {st.lambda0});
+    ok &= f.write_single("Global/sample", " ", {slicelength / st.lambda0});
+    ok &= f.write_single("Global/slen", "m", {slicelength * ns});
+    ok &= f.write_single("Global/one4one", " ", {0.0});
+    ok &= f.write_single("Global/time", " ", {dk.time.dotime ? 1.0 : 0.0});
+    ok &= f.write_single("Global/scan", " ", {0.0});
+    {
+        std::vector<double> s(ns), fr(ns);
+        const double e0 = 1239.842e-9 / st.lambda0;
+        double df = e0 * st.lambda0 / slicelength / (double)ns;
+        if (ns == 1) df = 0;
+        for (int i = 0; i < ns; i++) {
+            s[i] = (double)i * slicelength;
+            fr[i] = e0 + (double)i * df - 0.5 * df * (double)ns;
+        }
+        ok &= f.write_single("Global/s", "m", s);
+        ok &= f.write_single("Global/frequency", "eV", fr);
+    }
+    // Beam and Field: [nrec][ns] records; once-values [1][ns]
+    auto put = [&](const char *path, const char *unit, const char *key) {
+        auto it = rec.find(key);
+        if (it != rec.end()) ok &= f.write_buffer(path, unit, it->second, nrec, (size_t)ns);
+    };
+    if (!st.excl_energy) {
+        put("Beam/energy", "mc^2", "energy");
+        put("Beam/energyspread", "mc^2", "energyspread");
+    }
+    if (!st.excl_spatial) {
+        put("Beam/xposition", "m", "xposition");
+        put("Beam/xsize", "m", "xsize");
+        put("Beam/yposition", "m", "yposition");
+        put("Beam/ysize", "m", "ysize");
+    }
+    put("Beam/bunching", " ", "bunching");
+    put("Beam/bunchingphase", "rad", "bunchingphase");
+    ok &= f.write_buffer("Beam/current", "A", dk.current, 1, (size_t)ns);
+    {
+        std::vector<double> tw[6];
+        twiss_once(dk, tw);
+        const char *names[6] = {"Beam/emitx", "Beam/emity", "Beam/betax", "Beam/betay", "Beam/alphax", "Beam/alphay"};
+        const char *units[6] = {"m", "m", "m", "m", "rad", "rad"};
+        for (int q = 0; q < 6; q++) ok &= f.write_buffer(names[q], units[q], tw[q], 1, (size_t)ns);
+    }
+    put("Field/power", "W", "power");
+    if (!st.excl_spatial) {
+        put("Field/xposition", "m", "f_xposition");
+        put("Field/xsize", "m", "f_xsize");
+        put("Field/yposition", "m", "f_yposition");
+        put("Field/ysize", "m", "f_ysize");
+    }
+    if (!st.excl_intensity) {
+        put("Field/intensity-nearfield", "W/m^2", "nf_intensity");
+        put("Field/intensity-farfield", "W/rad^2", "ff_intensity");
+    }
+    if (st.field_global_stat && ns > 1) {
+        auto it = rec.find("global_energy");
+        if (it != rec.end()) ok &= f.write_single("Field/Global/energy", "J", it->second);
+    }
+    const double dgrid = 2 * dk.gridmax / (dk.ngrid - 1);
+    ok &= f.write_single("Field/gridspacing", "m", {dgrid});
+    ok &= f.write_single("Field/dgrid", "m", {dk.gridmax});
+    ok &= f.write_single("Field/ngrid", " ", {(double)dk.ngrid});
+    f.close();
+    if (!ok) err = "writing " + fname + " failed";
+    return ok;
+}
+
+const char *kRecKeys[] = {"power", "bunching", "bunchingphase", "energy", "energyspread", "xposition",
+                          "yposition", "xsize", "ysize", "f_xposition", "f_yposition", "f_xsize",
+                          "f_ysize", "nf_intensity", "ff_intensity"};
+
+bool do_track(Deck &dk, Namelist &nl, int device, bool quiet, bool dry, std::string &err) {
+    Keys k{nl, ""};
+    double zstop = 1e9, s0 = dk.time.s0, slen = dk.time.slen;
+    int output_step = 1, fdump = 0, bdump = 0, sort_step = 0, bunchharm = 1;
+    bool dump_ue = false, fft = false, exclharm = false, periodic = false, dbg1 = false, dbg2 = false,
+         filt = false;
+    double xc = 1, yc = 1, sig = 1;
+    std::string prec = "fp64";
+    k.dbl("zstop", zstop);
+    k.dbl("s0", s0);
+    k.dbl("slen", slen);
+    k.integer("output_step", output_step);
+    k.integer("field_dump_step", fdump);
+    k.boolean("field_dump_at_undexit", dump_ue);
+    k.integer("beam_dump_step", bdump);
+    k.integer("sort_step", sort_step);
+    k.integer("bunchharm", bunchharm);
+    k.boolean("fft_fieldsolver", fft);
+    k.boolean("exclusive_harmonics", exclharm);
+    k.boolean("periodic", periodic);
+    k.boolean("dbg_report_lattice", dbg1);
+    k.boolean("dbg_suppress_outfile", dbg2);
+    k.dbl("xcut", xc);
+    k.dbl("ycut", yc);
+    k.dbl("sigmoid", sig);
+    k.boolean("source_filter", filt);
+    k.str("precision", prec);  // this build's extension: fp64 (default) or fp32
+    if (!k.finish()) {
+        err = k.err;
+        return false;
+    }
+    if (fft) {
+        err = "&track: fft_fieldsolver is not supported by this build (ADI solver only)";
+        return false;
+    }
+    if (bunchharm > 1) {
+        err = "&track: bunching harmonics above 1 are not supported by this build";
+        return false;
+    }
+    if (prec != "fp64" && prec != "fp32") {
+        err = "&track: precision must be fp64 or fp32";
+        return false;
+    }
+    if (!dk.setup.defined || !dk.have_beam || !dk.have_field) {
+        err = "&track needs &setup, &beam and &field first";
+        return false;
+    }
+    if (output_step < 1) output_step = 1;
+    const SetupNL &st = dk.setup;
+    LatticeResult lr = parse_and_unroll(dk.lattice_text, st.beamline, st.delz, st.lambda0, st.gamma0);
+    if (!lr.error.empty()) {
+        err = "lattice: " + lr.error;
+        return false;
+    }
+    std::vector<g4mi_step_params> steps = lr.steps;
+    bool dumps = fdump > 0 || bdump > 0 || dump_ue;
+    for (size_t i = 0; i < steps.size(); i++) {  // Undulator::updateMarker
+        if (sort_step > 0 && i % (size_t)sort_step == 0) steps[i].marker |= 4;
+        if (steps[i].z > zstop) steps[i].marker |= 8;
+        if (steps[i].marker & 3) dumps = true;
+    }
+    if (dumps && !quiet)
+        printf("*** Warning: field and beam dumps are not written by this build; continuing without them\n");
+    // steps that run (up to the first stop marker) and the output records
+    // (Undulator::updateOutput: the initial state, then after step i (1-based)
+    // when i %% output_step == 0 and step i carries no stop marker)
+    size_t nrun = 0;
+    while (nrun < steps.size() && !(steps[nrun].marker & 8)) nrun++;
+    std::vector<size_t> outs;
+    for (size_t i = 1; i <= nrun; i++)
+        if (i % (size_t)output_step == 0) outs.push_back(i);
+    const int ns = (int)dk.off.size() - 1;
+    const size_t nrec = outs.size() + 1;
+    std::vector<double> zout(nrec, 0.0);
+    for (size_t r = 0; r < outs.size(); r++) zout[r + 1] = steps[outs[r] - 1].z + steps[outs[r] - 1].dz;
+    char tmp[512];
+    snprintf(tmp, sizeof(tmp),
+             "{\"nslice\": %d, \"npart\": %lld, \"ngrid\": %d, \"gridmax\": %.17g, \"steps\": %zu, \"run_steps\": "
+             "%zu, \"records\": %zu, \"matched\": %s, \"betax\": %.17g, \"alphax\": %.17g, \"phix\": %.17g, "
+             "\"betay\": %.17g, \"alphay\": %.17g, \"phiy\": %.17g, \"time\": %s}",
+             ns, (long long)dk.off[ns], dk.ngrid, dk.gridmax, steps.size(), nrun, nrec, dk.matched ? "true" : "false",
+             dk.mbetax, dk.malphax, dk.mphix, dk.mbetay, dk.malphay, dk.mphiy, dk.time.dotime ? "true" : "false");
+    dk.summary = tmp;
+    std::string root = st.rootname;
+    if (!st.outputdir.empty()) root = st.outputdir + "/" + root;
+    if (dk.runs > 0) root += ".Run" + std::to_string(dk.runs + 1);
+    const std::string fname = root + ".out.h5";
+
+    std::map<std::string, std::vector<double>> rec;
+    if (dry) {
+        // setup check: the output file without device records
+        dk.runs++;
+        if (st.rootname.empty()) return true;
+        return write_output(dk, fname, steps, zout, rec, 0, err);
+    }
+    // device context: created by the first &track, reused by later ones
+    const int precision = prec == "fp32" ? G4MI_FP32 : G4MI_FP64;
+    if (!dk.ctx) {
+        dk.precision = precision;
+        dk.periodic = periodic;
+        int rc = g4mi_create(&dk.ctx, device, ns, dk.off.data(), dk.ngrid, dk.gridmax, st.lambda0,
+                             (double)dk.time.sample, dk.time.dotime ? 1 : 0, periodic ? 1 : 0, precision);
+        if (rc != G4MI_OK) {
+            err = std::string("device context: ") + g4mi_last_error();
+            return false;
+        }
+        const EFieldNL &e = dk.efield;
+        if ((rc = g4mi_set_efield(dk.ctx, e.longrange ? 1 : 0, e.nz, e.nphi, e.ngrid, e.rmax)) != G4MI_OK ||
+            (rc = g4mi_beam_upload(dk.ctx, dk.gamma.data(), dk.theta.data(), dk.x.data(), dk.y.data(),
+                                   dk.px.data(), dk.py.data(), dk.current.data())) != G4MI_OK ||
+            (rc = g4mi_field_upload(dk.ctx, dk.field.data(), 0)) != G4MI_OK) {
+            err = g4mi_last_error();
+            return false;
+        }
+    } else {
+        if (precision != dk.precision || periodic != dk.periodic) {
+            err = "&track: precision and periodic cannot change between &track elements";
+            return false;
+        }
+        // the once-values of the output describe the beam at the start of this run
+        if (g4mi_beam_download(dk.ctx, dk.gamma.data(), dk.theta.data(), dk.x.data(), dk.y.data(), dk.px.data(),
+                               dk.py.data()) != G4MI_OK) {
+            err = g4mi_last_error();
+            return false;
+        }
+    }
+    for (const char *key : kRecKeys) rec[key].assign(nrec * (size_t)ns, 0.0);
+    rec["global_energy"].assign(nrec, 0.0);
+    auto record = [&](size_t r) {
+        g4mi_diag_out d{};
+        d.power = rec["power"].data() + r * ns;
+        d.bunching = rec["bunching"].data() + r * ns;
+        d.bunchingphase = rec["bunchingphase"].data() + r * ns;
+        d.energy = rec["energy"].data() + r * ns;
+        d.energyspread = rec["energyspread"].data() + r * ns;
+        d.xposition = rec["xposition"].data() + r * ns;
+        d.yposition = rec["yposition"].data() + r * ns;
+        d.xsize = rec["xsize"].data() + r * ns;
+        d.ysize = rec["ysize"].data() + r * ns;
+        d.f_xposition = rec["f_xposition"].data() + r * ns;
+        d.f_yposition = rec["f_yposition"].data() + r * ns;
+        d.f_xsize = rec["f_xsize"].data() + r * ns;
+        d.f_ysize = rec["f_ysize"].data() + r * ns;
+        d.nf_intensity = rec["nf_intensity"].data() + r * ns;
+        d.ff_intensity = rec["ff_intensity"].data() + r * ns;
+        d.global_energy = rec["global_energy"].data() + r;
+        return g4mi_diag(dk.ctx, &d);
+    };
+    if (record(0) != G4MI_OK) {
+        err = g4mi_last_error();
+        return false;
+    }
+    if (!quiet) printf("Running %zu steps on the device (%d slices, %lld macroparticles)...\n", nrun, ns,
+                       (long long)dk.off[ns]);
+    size_t done = 0;
+    for (size_t r = 0; r < outs.size(); r++) {
+        const size_t upto = outs[r];
+        const int rc = g4mi_run(dk.ctx, steps.data() + done, (int)(upto - done), st.gamma0, nullptr);
+        if (rc < 0) {
+            err = g4mi_last_error();
+            return false;
+        }
+        done = upto;
+        if (record(r + 1) != G4MI_OK) {
+            err = g4mi_last_error();
+            return false;
+        }
+    }
+    if (done < nrun && g4mi_run(dk.ctx, steps.data() + done, (int)(nrun - done), st.gamma0, nullptr) < 0) {
+        err = g4mi_last_error();
+        return false;
+    }
+    dk.runs++;
+    if (st.rootname.empty()) {
+        err = "no rootname in &setup";
+        return false;
+    }
+    if (!quiet) printf("Writing output file %s...\n", fname.c_str());
+    return write_output(dk, fname, steps, zout, rec, nrec, err);
+}
+
+int run_deck(const char *input_file, int device, int quiet, bool dry, std::string *summary) {
+    if (!input_file) return fail("no input file", G4MI_E_ARG);
+    Deck dk;
+    dk.input_file = input_file;
+    if (!read_file(dk.input_file, dk.input_text)) return fail("cannot open main input file " + dk.input_file);
+    std::vector<Namelist> nls;
+    std::string err;
+    if (!parse_namelists(dk.input_text, nls, err)) return fail(err);
+    int rc = G4MI_OK;
+    for (Namelist &nl : nls) {
+        bool ok = true;
+        const std::string &e = nl.name;
+        if (e == "&stop") break;
+        if (e == "&setup") ok = do_setup(dk, nl, err);
+        else if (e == "&lattice") ok = dk.setup.defined ? do_lattice(dk, nl, err)
+                                                      : (err = "&lattice before &setup", false);
+        else if (e == "&time") ok = do_time(dk, nl, err);
+        else if (e == "&field") ok = do_field(dk, nl, err);
+        else if (e == "&beam") ok = do_beam(dk, nl, err);
+        else if (e == "&efield") ok = do_efield(dk, nl, err);
+        else if (e == "&track") ok = do_track(dk, nl, device, quiet != 0, dry, err);
+        else {
+            err = "element " + e + " is not supported by this build";
+            ok = false;
+        }
+        if (!ok) {
+            rc = fail(e + ": " + err, err.rfind("device", 0) == 0 ? G4MI_E_HIP : G4MI_E_PARSE);
+            break;
+        }
+    }
+    if (summary) *summary = dk.summary;
+    if (dk.ctx) g4mi_destroy(dk.ctx);
+    return rc;
+}
+
+}  // namespace
+}  // namespace g4mi
+
+using namespace g4mi;
+
 extern "C" int g4mi_run_deck(const char *input_file, int device, int quiet) {
-    (void)input_file;
-    (void)device;
-    (void)quiet;
-    g4mi::set_error("g4mi_run_deck: not available in this build");
-    return G4MI_E_STATE;
+    return run_deck(input_file, device, quiet, false, nullptr);
+}
+
+extern "C" int g4mi_check_deck(const char *input_file, char *json, int len) {
+    std::string s;
+    const int rc = run_deck(input_file, 0, 1, true, &s);
+    if (json && len > 0) snprintf(json, (size_t)len, "%s", s.c_str());
+    return rc;
+}
+
+extern "C" long long g4mi_h5_read(const char *file, const char *dataset, double *out, long long cap,
+                                  long long *dims) {
+    if (!file || !dataset) {
+        set_error("g4mi_h5_read: invalid argument");
+        return G4MI_E_ARG;
+    }
+    std::vector<double> v;
+    long long d[2] = {0, 0};
+    std::string err;
+    const long long n = h5_read_doubles(file, dataset, v, d, err);
+    if (n < 0) {
+        set_error("g4mi_h5_read: " + err);
+        return G4MI_E_PARSE;
+    }
+    if (dims) {
+        dims[0] = d[0];
+        dims[1] = d[1];
+    }
+    if (out)
+        for (long long i = 0; i < std::min(n, cap); i++) out[i] = v[(size_t)i];
+    return n;
 }

@@ -447,9 +447,43 @@ __global__ void k_slice_out(const float2 *fin, float2 *fout, int ng, int ns, int
     fout[fo] = make_float2(e.x + scale[b] * out[i * 2], e.y + scale[b] * out[i * 2 + 1]);
 }
 
+// Error of the network's field step against the ADI's on one storage slice
+// per block: parts[j] = (sum |E_fno - E_adi|^2, sum |E_adi - E|^2,
+// sum |E_adi|^2) with E the field before the step.
+__global__ void __launch_bounds__(256) k_judge(const float2 *efno, const float2 *eadi, const float2 *ein, int P,
+                                               double *parts) {
+    __shared__ double red[3][256];
+    const long long j = blockIdx.x;
+    const float2 *a = efno + j * P, *b = eadi + j * P, *e = ein + j * P;
+    double s0 = 0, s1 = 0, s2 = 0;
+    for (int p = threadIdx.x; p < P; p += 256) {
+        const float2 u = a[p], v = b[p], w = e[p];
+        const double dx = (double)u.x - v.x, dy = (double)u.y - v.y;
+        const double ux = (double)v.x - w.x, uy = (double)v.y - w.y;
+        s0 += dx * dx + dy * dy;
+        s1 += ux * ux + uy * uy;
+        s2 += (double)v.x * v.x + (double)v.y * v.y;
+    }
+    red[0][threadIdx.x] = s0;
+    red[1][threadIdx.x] = s1;
+    red[2][threadIdx.x] = s2;
+    __syncthreads();
+    for (int k = 128; k > 0; k >>= 1) {
+        if (threadIdx.x < k)
+            for (int q = 0; q < 3; q++) red[q][threadIdx.x] += red[q][threadIdx.x + k];
+        __syncthreads();
+    }
+    if (threadIdx.x < 3) parts[j * 3 + threadIdx.x] = red[threadIdx.x][0];
+}
+
 inline unsigned nb(long long n) { return (unsigned)((n + 255) / 256); }
 
 }  // namespace
+
+void fno_judge_sums(hipStream_t s, const float2 *efno, const float2 *eadi, const float2 *ein, int nslices, int P,
+                    double *parts) {
+    hipLaunchKernelGGL(k_judge, dim3(nslices), dim3(256), 0, s, efno, eadi, ein, P, parts);
+}
 
 namespace {
 // waves along one output dimension: the fewest padded rows, ties to the larger tile
@@ -926,27 +960,34 @@ static int run_pairs(Fno *f, long long p0, long long p1, int batch, bool train, 
     return G4MI_OK;
 }
 
-int fno_train(Fno *f, int epochs, int batch, float lr, long long nholdout, unsigned seed, double *train_loss,
-              double *holdout_loss) {
-    if (batch < 1 || batch > f->B || epochs < 0 || nholdout < 0 || nholdout >= f->pairs_n) {
+int fno_train_span(Fno *f, int first_epoch, int epochs, int total_epochs, int batch, float lr,
+                   long long nholdout, unsigned seed, double *train_loss, double *holdout_loss) {
+    if (batch < 1 || batch > f->B || epochs < 0 || first_epoch < 0 || total_epochs < first_epoch + epochs ||
+        nholdout < 0 || nholdout >= f->pairs_n) {
         set_error("fno_train: invalid arguments (batch, epochs or hold-out size)");
         return G4MI_E_ARG;
     }
-    std::mt19937_64 rng(seed);
     const long long ntrain = f->pairs_n - nholdout;
-    for (int e = 0; e < epochs; e++) {
-        // cosine schedule over the epochs
-        const float lre = lr * 0.5f * (1.f + std::cos((float)M_PI * (float)e / (float)std::max(1, epochs)));
+    for (int k = 0; k < epochs; k++) {
+        const int e = first_epoch + k;
+        std::mt19937_64 rng(seed + 0x9E3779B97F4A7C15ULL * (unsigned long long)(e + 1));
+        // cosine schedule over all total_epochs
+        const float lre = lr * 0.5f * (1.f + std::cos((float)M_PI * (float)e / (float)std::max(1, total_epochs)));
         double l = 0;
         G4_TRY(run_pairs(f, 0, ntrain, batch, true, lre, rng, &l));
-        if (train_loss) train_loss[e] = l;
+        if (train_loss) train_loss[k] = l;
         if (holdout_loss) {
             double h = 0;
             if (nholdout > 0) G4_TRY(run_pairs(f, ntrain, f->pairs_n, batch, false, 0.f, rng, &h));
-            holdout_loss[e] = h;
+            holdout_loss[k] = h;
         }
     }
     return G4MI_OK;
+}
+
+int fno_train(Fno *f, int epochs, int batch, float lr, long long nholdout, unsigned seed, double *train_loss,
+              double *holdout_loss) {
+    return fno_train_span(f, 0, epochs, epochs, batch, lr, nholdout, seed, train_loss, holdout_loss);
 }
 
 int fno_eval(Fno *f, long long p0, long long p1, int batch, double *loss) {
@@ -1024,6 +1065,11 @@ long long g4mi_fno_pair_count(g4mi_fno *h) { return h ? fno_pairs(h->f) : G4MI_E
 int g4mi_fno_train(g4mi_fno *h, int epochs, int batch, float lr, long long nholdout, unsigned seed, double *tl,
                    double *hl) {
     return h ? fno_train(h->f, epochs, batch, lr, nholdout, seed, tl, hl) : G4MI_E_ARG;
+}
+int g4mi_fno_train_span(g4mi_fno *h, int first_epoch, int epochs, int total_epochs, int batch, float lr,
+                        long long nholdout, unsigned seed, double *tl, double *hl) {
+    return h ? fno_train_span(h->f, first_epoch, epochs, total_epochs, batch, lr, nholdout, seed, tl, hl)
+             : G4MI_E_ARG;
 }
 int g4mi_fno_eval(g4mi_fno *h, long long first, long long last, int batch, double *loss) {
     return (h && loss) ? fno_eval(h->f, first, last, batch, loss) : G4MI_E_ARG;

@@ -24,7 +24,13 @@ int fno_reserve(Fno *f, long long npairs);
 int fno_capture(Fno *f, hipStream_t s, const float2 *E, const float2 *S, const float2 *En, int4 box);
 int fno_train(Fno *f, int epochs, int batch, float lr, long long nholdout, unsigned seed, double *train_loss,
               double *holdout_loss);
+int fno_train_span(Fno *f, int first_epoch, int epochs, int total_epochs, int batch, float lr,
+                   long long nholdout, unsigned seed, double *train_loss, double *holdout_loss);
 int fno_eval(Fno *f, long long p0, long long p1, int batch, double *loss);
+// per storage slice j < nslices: parts[3 j ..] = (sum |efno - eadi|^2,
+// sum |eadi - ein|^2, sum |eadi|^2) over its P cells
+void fno_judge_sums(hipStream_t s, const float2 *efno, const float2 *eadi, const float2 *ein, int nslices, int P,
+                    double *parts);
 int fno_infer_slices(Fno *f, hipStream_t s, const float2 *fin, float2 *fout, const float2 *src, const int4 *box,
                      int ns, int first);
 Fno *fno_of(g4mi_fno *h);

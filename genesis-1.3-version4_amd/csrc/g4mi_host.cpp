@@ -487,7 +487,8 @@ void shot_noise(Ran2 &stream, double *th, int npart, int nbins, double ne) {
 }
 
 void gauss_field(double *field, int ngrid, double gridmax, double lambda, double power,
-                 double w0, double z0in, double phase, double xcen, double ycen) {
+                 double w0, double z0in, double phase, double xcen, double ycen, double xangle,
+                 double yangle) {
     typedef std::complex<double> cd;
     const double k = 4. * asin(1) / lambda * (double)1;
     const double z0 = -z0in;
@@ -500,7 +501,7 @@ void gauss_field(double *field, int ngrid, double gridmax, double lambda, double
     const cd zscale = unit * norm * cd(cos(phase), sin(phase));
     const double xmid = gridmax + xcen, ymid = gridmax + ycen;
     const double dxy = 2. * gridmax / (ngrid - 1.);
-    const double kx = k * 0, ky = k * 0;
+    const double kx = k * xangle, ky = k * yangle;
     for (int iy = 0; iy < ngrid; iy++) {
         const double yy = iy * dxy - ymid;
         const double y2 = yy * yy, phiy = ky * yy;

@@ -63,7 +63,8 @@ unsigned int shot_noise_seed(int seed, int rank);
 double inverse_erfc(double y);
 // Gaussian seed field (src/Util/GaussHermite.cpp:20-63), fundamental mode.
 void gauss_field(double *field, int ngrid, double gridmax, double lambda, double power,
-                 double w0, double z0, double phase, double xcen, double ycen);
+                 double w0, double z0, double phase, double xcen, double ycen, double xangle = 0.0,
+                 double yangle = 0.0);
 
 // Lattice file (LatticeParser.cpp) -> unrolled steps (Lattice.cpp).
 struct LatticeResult {

@@ -42,7 +42,9 @@ int g4mi_load_beam(int nslice, const long long *off, const g4mi_slice_beam *slic
             return G4MI_E_ARG;
         }
     const double theta0 = 4. * asin(1.);  // LoadBeam.cpp:127
-    Ran2 stream(shot_noise_seed(seed, 0));
+    // quiet loading: each slice from its own reseeded sequence, so slices
+    // load independently (threads over slices)
+#pragma omp parallel for schedule(dynamic, 4)
     for (int j = 0; j < nslice; j++) {
         const long long a = off[j];
         const int n = (int)(off[j + 1] - a);
@@ -50,8 +52,16 @@ int g4mi_load_beam(int nslice, const long long *off, const g4mi_slice_beam *slic
         std::memcpy(&sb, &slices[j], sizeof(sb));
         quiet_load(gamma + a, theta + a, x + a, y + a, px + a, py + a, sb, n, nbins, theta0,
                    j + islice0);
-        const double ne = sb.current * lambda0 * sample / kCeH();
-        if (shotnoise && timerun) shot_noise(stream, theta + a, n, nbins, ne);
+    }
+    // shot noise: one sequential stream across the slices, in slice order
+    if (shotnoise && timerun) {
+        Ran2 stream(shot_noise_seed(seed, 0));
+        for (int j = 0; j < nslice; j++) {
+            const long long a = off[j];
+            const int n = (int)(off[j + 1] - a);
+            const double ne = slices[j].current * lambda0 * sample / kCeH();
+            shot_noise(stream, theta + a, n, nbins, ne);
+        }
     }
     return G4MI_OK;
 }

@@ -109,6 +109,13 @@ def lib():
         L.g4mi_fno_eval.argtypes = [C.c_void_p, C.c_longlong, C.c_longlong, C.c_int, _dp]
         L.g4mi_attach_fno.argtypes = [C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_double]
         L.g4mi_fno_stats.argtypes = [C.c_void_p, _lp, _lp, _dp, _dp]
+        L.g4mi_fno_train_span.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_float,
+                                          C.c_longlong, C.c_uint, _dp, _dp]
+        L.g4mi_fno_judge.argtypes = [C.c_void_p, _lp, _dp, _dp, _dp]
+        L.g4mi_run_deck.argtypes = [C.c_char_p, C.c_int, C.c_int]
+        L.g4mi_check_deck.argtypes = [C.c_char_p, C.c_char_p, C.c_int]
+        L.g4mi_h5_read.argtypes = [C.c_char_p, C.c_char_p, _dp, C.c_longlong, _lp]
+        L.g4mi_h5_read.restype = C.c_longlong
         _lib = L
     return _lib
 
@@ -278,6 +285,14 @@ class Context:
         _chk(lib().g4mi_attach_fno(self.h, fno.h if fno is not None else None, mode, every,
                                    threshold), "g4mi_attach_fno")
 
+    def fno_judge(self):
+        n = C.c_longlong()
+        ef, eu, ra = C.c_double(), C.c_double(), C.c_double()
+        _chk(lib().g4mi_fno_judge(self.h, C.byref(n), C.byref(ef), C.byref(eu), C.byref(ra)),
+             "g4mi_fno_judge")
+        return dict(judge_steps=n.value, rel_field_err=ef.value, rel_update_err=eu.value,
+                    mean_residual_all=ra.value)
+
     def fno_stats(self):
         a, b = C.c_longlong(), C.c_longlong()
         r1, r2 = C.c_double(), C.c_double()
@@ -362,11 +377,18 @@ class Fno:
     def pair_count(self):
         return int(lib().g4mi_fno_pair_count(self.h))
 
-    def train(self, epochs, batch, lr, nholdout=0, seed=1):
+    def train(self, epochs, batch, lr, nholdout=0, seed=1, first_epoch=0, total_epochs=None):
+        """Adam over `epochs` epochs; with first_epoch/total_epochs, those
+        epochs of a cosine schedule spanning total_epochs."""
         tl = np.zeros(max(1, epochs))
         hl = np.zeros(max(1, epochs))
-        _chk(lib().g4mi_fno_train(self.h, epochs, batch, lr, nholdout, seed, _p(tl), _p(hl)),
-             "fno_train")
+        if total_epochs is None and first_epoch == 0:
+            _chk(lib().g4mi_fno_train(self.h, epochs, batch, lr, nholdout, seed, _p(tl), _p(hl)),
+                 "fno_train")
+        else:
+            total = total_epochs if total_epochs is not None else first_epoch + epochs
+            _chk(lib().g4mi_fno_train_span(self.h, first_epoch, epochs, total, batch, lr, nholdout,
+                                           seed, _p(tl), _p(hl)), "fno_train_span")
         return tl[:epochs], hl[:epochs]
 
     def eval(self, first, last, batch):
@@ -390,6 +412,31 @@ class Fno:
             out.append((name, o, n))
             o += n
         return out
+
+
+def run_deck(path, device=0, quiet=True):
+    """Run a GENESIS 1.3 v4 input deck on the device (writes <rootname>.out.h5)."""
+    _chk(lib().g4mi_run_deck(str(path).encode(), device, int(quiet)), "g4mi_run_deck")
+
+
+def check_deck(path):
+    """Parse and set up a deck without a device; summary of its last &track."""
+    import json
+    buf = C.create_string_buffer(4096)
+    _chk(lib().g4mi_check_deck(str(path).encode(), buf, 4096), "g4mi_check_deck")
+    return json.loads(buf.value.decode()) if buf.value else {}
+
+
+def h5_read(file, dataset):
+    """A double dataset of an HDF5 file as a numpy array (1-D or 2-D)."""
+    dims = np.zeros(2, dtype=np.int64)
+    n = lib().g4mi_h5_read(str(file).encode(), dataset.encode(), None, 0,
+                           dims.ctypes.data_as(_lp))
+    _chk(n, "g4mi_h5_read")
+    out = np.zeros(int(n))
+    _chk(lib().g4mi_h5_read(str(file).encode(), dataset.encode(), _p(out), int(n),
+                            dims.ctypes.data_as(_lp)), "g4mi_h5_read")
+    return out.reshape(int(dims[0]), int(dims[1])) if dims[1] > 0 else out
 
 
 def steps_from(seq):

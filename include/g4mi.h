@@ -185,6 +185,18 @@ int g4mi_load_gauss(double *field, int ngrid, double gridmax, double lambda, dou
  * in the reference layout (src/IO/Output.cpp) when HDF5 is available.
  * Replaces genmain() for the supported commands. */
 int g4mi_run_deck(const char *input_file, int device, int quiet);
+/* The same deck without a device: parses it, matches the lattice, generates
+ * beam and field and, when &setup names a rootname, writes the output file
+ * with the /Meta, /Global, /Lattice and once-per-run /Beam and /Field
+ * datasets.  `json` receives a summary of the last &track (slices,
+ * macroparticles, grid, steps, output records, matched Twiss).  No
+ * counterpart in the reference: the setup half of genmain() alone. */
+int g4mi_check_deck(const char *input_file, char *json, int len);
+/* A double dataset of an HDF5 file (rank 1 or 2): up to `cap` values into
+ * `out`, its shape into dims[2] (dims[1] = 0 for rank 1); returns the
+ * element count.  Reads the output files back without h5py. */
+long long g4mi_h5_read(const char *file, const char *dataset, double *out, long long cap,
+                       long long *dims);
 
 /* ------------------------------------------------------------------------
  * Learned field step (no counterpart in the reference).  A Fourier neural
@@ -222,6 +234,12 @@ long long g4mi_fno_pair_count(g4mi_fno *fno);
  * `nholdout` pairs are kept out of training; per-epoch mean losses. */
 int g4mi_fno_train(g4mi_fno *fno, int epochs, int batch, float lr, long long nholdout,
                    unsigned seed, double *train_loss, double *holdout_loss);
+/* Epochs [first_epoch, first_epoch + epochs) of a schedule of total_epochs
+ * (the same cosine schedule, split over several calls); losses per epoch
+ * of this call. */
+int g4mi_fno_train_span(g4mi_fno *fno, int first_epoch, int epochs, int total_epochs, int batch,
+                        float lr, long long nholdout, unsigned seed, double *train_loss,
+                        double *holdout_loss);
 int g4mi_fno_eval(g4mi_fno *fno, long long first, long long last, int batch, double *loss);
 
 /* Attach a network to a mixed-form (G4MI_FP32) context:
@@ -232,13 +250,24 @@ int g4mi_fno_eval(g4mi_fno *fno, long long first, long long last, int batch, dou
  *           instead on every `every`-th step and whenever the
  *           particle-to-field energy-balance residual of the network's
  *           step exceeds `threshold`:
- *             r = |dP_field + sum_s I_s eev d<gamma>_s| / P_field,
- *           dP_field the change of the total field power over the step. */
+ *             r = |dP_field + sum_s I_s eev d<gamma>_s - 4 sum |S|^2 pw| / P_field,
+ *           dP_field the change of the total field power over the step,
+ *           d<gamma>_s the push's mean energy change of slice s and the
+ *           last term the source's own energy (S the deposited source,
+ *           pw the power per |E|^2 of a cell).  On the every-th steps the
+ *           network's step is also computed and compared with the ADI's
+ *           (g4mi_fno_judge). */
 int g4mi_attach_fno(g4mi_ctx *ctx, g4mi_fno *fno, int mode, int every, double threshold);
 /* Steps advanced by the network and by the ADI since attaching, and the
  * mean residual of the network's steps and of the ADI's steps. */
 int g4mi_fno_stats(g4mi_ctx *ctx, long long *fno_steps, long long *adi_steps,
                    double *mean_residual_fno, double *mean_residual_adi);
+/* Judge steps since attaching and the mean over them of the network's
+ * relative field error ||E_fno - E_adi|| / ||E_adi|| and relative update
+ * error ||E_fno - E_adi|| / ||E_adi - E|| (norms over all slices); and the
+ * mean energy-balance residual of every network step, accepted or not. */
+int g4mi_fno_judge(g4mi_ctx *ctx, long long *njudge, double *rel_field_err,
+                   double *rel_update_err, double *mean_residual_all);
 
 #ifdef __cplusplus
 }

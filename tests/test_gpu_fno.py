@@ -145,12 +145,16 @@ def test_fno_replace_mode_falls_back_to_adi():
     g = ctx.run(H.gsteps(st[:n]), cases.GAMMA0, record=("power",))
     r = sim.run(st, cases.GAMMA0, record=("power",), nsteps=n)
     s = ctx.fno_stats()
+    j = ctx.fno_judge()
     pg, pr = g["power"][1:].sum(axis=1), r["power"][1:].sum(axis=1)
     e = float(np.max(np.abs(pg / pr - 1)))
-    _log(f"replace mode: {s}; fallback rate {s['adi_steps'] / n:.2f}; sum P(z) max rel err vs FP64 "
-         f"oracle {e:.3e}")
+    _log(f"replace mode: {s}; judge steps {j}; fallback rate {s['adi_steps'] / n:.2f}; sum P(z) max "
+         f"rel err vs FP64 oracle {e:.3e}")
     assert s["fno_steps"] + s["adi_steps"] == n
     assert s["adi_steps"] >= n // 4
+    # every 4th step (0, 4, ..., 56) compares the network's step with the ADI's
+    assert j["judge_steps"] == n // 4
+    assert 0 < j["rel_field_err"] < 1 and np.isfinite(j["rel_update_err"])
     assert np.all(np.isfinite(pg)) and e < 0.25
     ctx.close()
     f.close()
