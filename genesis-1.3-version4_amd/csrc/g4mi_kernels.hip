@@ -1018,7 +1018,12 @@ struct BandGeom {
     static constexpr int PG = TS + 1;
     static constexpr int CX = 8;                    // outputs per lane in the x stage
     static constexpr int SR = (RE + (256 / RE) - 1) / (256 / RE);  // rows per lane in the E stage
-    static constexpr size_t lds_cells = (size_t)RE * PE + (size_t)RE * PG + 2 * TS * W;
+    // E, then (after the x stage has read it) the y stage's right-hand side
+    // F over the same cells with the y band rows of edge tiles behind it;
+    // the x band rows of edge tiles in a region of their own
+    static constexpr size_t lds_cells = (size_t)RE * PE + (size_t)TS * W;
+    static_assert(PG <= PE, "F must fit in E's rows");
+    static_assert((size_t)RE * PG + (size_t)TS * W <= (size_t)RE * PE, "F and the y band must fit in E");
 };
 
 template <int M>
@@ -1034,9 +1039,9 @@ __global__ void __launch_bounds__(256) k_adi_band(const double2 *__restrict__ fi
     constexpr int NT = 256;
     extern __shared__ double2 sm[];
     double2 *E = sm;                  // [RE][PE] E = S + (1 + i a d_y^2) u
-    double2 *G = E + RE * PE;         // [RE][PG] F = S + 2 u' - E
-    double2 *BX = G + RE * PG;        // [TS][W] band rows of x (edge tiles)
-    double2 *BY = BX + TS * W;        // [TS][W] band rows of y (edge tiles)
+    double2 *G = sm;                  // [RE][PG] F = S + 2 u' - E, written over E
+    double2 *BY = sm + RE * PG;       // [TS][W] band rows of y (edge tiles), over E
+    double2 *BX = sm + RE * PE;       // [TS][W] band rows of x (edge tiles)
     const int t = threadIdx.x;
     // XCD-aware order: workgroups are handed to the 8 XCDs round-robin, so
     // block b runs on XCD b % 8.  Blocks of a group of 8 slices are ordered
@@ -1066,11 +1071,6 @@ __global__ void __launch_bounds__(256) k_adi_band(const double2 *__restrict__ fi
         for (int e = t; e < TS * W; e += NT) {
             const int x = tx0 + e / W;
             BX[e] = x < ng ? band[(long long)x * W + e % W] : make_double2(0, 0);
-        }
-    if (!yin)
-        for (int e = t; e < TS * W; e += NT) {
-            const int y = ty0 + e / W;
-            BY[e] = y < ng ? band[(long long)y * W + e % W] : make_double2(0, 0);
         }
     // explicit y stencil of the x half step (FieldSolverADI.cpp:64-72),
     // zero outside the grid.  Lane -> one column of the E region and a strip
@@ -1113,7 +1113,9 @@ __global__ void __launch_bounds__(256) k_adi_band(const double2 *__restrict__ fi
     // register window.  Then the y half step's right-hand side
     // S + (1 + i a d_x^2) u' = S + 2 u' - E  (1 + i a d^2 = 2 - A, A u' = E):
     // no x stencil on u'.
-    if (t < RE * (TS / CX)) {
+    const bool xact = t < RE * (TS / CX);
+    double2 fx[CX];
+    if (xact) {
         const int r = t % RE, c0 = (t / RE) * CX;
         const int y = ty0 - M + r;
         double2 ev[CX + 2 * M];
@@ -1141,9 +1143,20 @@ __global__ void __launch_bounds__(256) k_adi_band(const double2 *__restrict__ fi
             double2 f = make_double2(0, 0);
             if (x < ng && y >= 0 && y < ng)
                 f = csub(cadd(sat(x, y), cscale(acc[j], 2.0)), ev[j + M]);
-            G[r * PG + c0 + j] = f;
+            fx[j] = f;
         }
     }
+    __syncthreads();  // every read of E is done: F and the y band go over it
+    if (xact) {
+        const int r = t % RE, c0 = (t / RE) * CX;
+#pragma unroll
+        for (int j = 0; j < CX; j++) G[r * PG + c0 + j] = fx[j];
+    }
+    if (!yin)
+        for (int e = t; e < TS * W; e += NT) {
+            const int y = ty0 + e / W;
+            BY[e] = y < ng ? band[(long long)y * W + e % W] : make_double2(0, 0);
+        }
     __syncthreads();
     // implicit y: u'' = sum_m band[y][m] F(x, y+m-M), four consecutive y per lane
     const double shift = -0.5 * (double)(ng - 1);
