@@ -112,7 +112,29 @@ def gpu_measure(cfg, steps_k, warmup, adi_kernel=-1, precision=g4mi.FP64, breakd
                 bunching=bufs["bunching"], schedule=sched, load_s=t_load)
 
 
-def roofline(cfg, kt, ntot, precision=g4mi.FP64):
+# device kernel behind each timed kernel group (the traffic table names kernels)
+KERNEL_OF = {g4mi.FP64: {"adi": "k_adi_band", "push": "k_push", "transverse": "k_transverse",
+                         "deposit": "k_deposit"},
+             g4mi.FP32: {"adi": "k_mix_adi_stream", "push": "k_mix_main", "transverse": "k_mix_front"}}
+
+
+def measured_traffic(workload, kernel):
+    """HBM bytes per launch of `kernel` on `workload` from the committed
+    FETCH_SIZE/WRITE_SIZE summary (profiles/pmc_traffic.md), or None."""
+    path = os.path.join(ROOT, "profiles", "pmc_traffic.md")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        cols = [c.strip() for c in line.strip().strip("|").split("|")]
+        if len(cols) == 6 and cols[0] == workload and kernel in cols[1]:
+            try:
+                return float(cols[5])
+            except ValueError:
+                return None
+    return None
+
+
+def roofline(cfg, kt, ntot, precision=g4mi.FP64, schedule=None):
     """Dominant kernel and its algorithmic bytes per launch.
     FP64 plain form:
       push: read gamma, theta, x, y, px, py + write gamma, theta = 64 B/particle;
@@ -133,9 +155,14 @@ def roofline(cfg, kt, ntot, precision=g4mi.FP64):
     name = max(per, key=lambda k: kt[k])
     ms = kt[name]
     gbs = per[name] / (ms * 1e-3) / 1e9
+    dev = KERNEL_OF[precision].get(name)
+    if name == "adi" and precision == g4mi.FP64 and schedule and schedule.get("adi") != "band":
+        dev = None  # the traffic table holds the banded kernel only
+    traffic = measured_traffic(cfg["name"], dev) if dev else None
     return name, dict(bound="hbm", achieved=round(gbs, 1), peak=HBM_PEAK_GBS, unit="GB/s",
-                      frac=round(gbs / HBM_PEAK_GBS, 4), traffic=None,
-                      kernel=name, ms_per_launch=round(ms, 5),
+                      frac=round(gbs / HBM_PEAK_GBS, 4),
+                      traffic=int(traffic) if traffic else None,
+                      kernel=name, device_kernel=dev, ms_per_launch=round(ms, 5),
                       algorithmic_bytes=int(per[name]))
 
 
@@ -405,7 +432,7 @@ def main():
                     mixed_adi=a.mixed_adi)
     rate = m["ntot"] * a.steps / m["seconds"]
     cells = cfg["nslice"] * cfg["ngrid"] ** 2
-    kname, roof = roofline(cfg, m["kt"], m["ntot"], prec)
+    kname, roof = roofline(cfg, m["kt"], m["ntot"], prec, m["schedule"])
     out = {
         "metric": "macroparticle-steps/s",
         "value": round(rate, 1),
