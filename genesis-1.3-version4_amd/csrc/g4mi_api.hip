@@ -108,6 +108,7 @@ struct Ctx {
     std::map<std::tuple<int, int, int, int, int, int>, hipGraphExec_t> m_graph;
     std::vector<double> m_rec_host;
     std::vector<double> h_cur;             // currents [ns] (host copy)
+    double *d_aux = nullptr;               // auxiliary diagnostics [16 ns]
     // learned field step (mixed form): 0 detached, 1 capture, 2 replace the ADI
     Fno *fno = nullptr;
     int fno_mode = 0, fno_every = 1;
@@ -445,13 +446,23 @@ static int slippage(Ctx *c, double slip) {
 static void finish_diag(const Ctx *c, const std::vector<double> &hb, const std::vector<double> &hf,
                         int first, g4mi_diag_out *o) {
     const int ns = c->ns;
-    // DiagBeam::getValues normalisation (Diagnostic.cpp:452-489)
+    // DiagBeam::getValues normalisation (Diagnostic.cpp:452-489) and the
+    // current-weighted sums of the global statistics (:552-607)
+    double gc = 0, gg1 = 0, gg2 = 0, gx1 = 0, gx2 = 0, gy1 = 0, gy2 = 0;
     for (int is = 0; is < ns; is++) {
         const long long n = c->off[is + 1] - c->off[is];
         const double norm = n > 0 ? 1. / (double)n : 1.;
         const double *v = &hb[8 * is];
         const double g1 = v[0] * norm, g2 = v[1] * norm, x1 = v[2] * norm, x2 = v[3] * norm;
         const double y1 = v[4] * norm, y2 = v[5] * norm, bre = v[6] * norm, bim = v[7] * norm;
+        const double cu = is < (int)c->h_cur.size() ? c->h_cur[is] : 0.0;
+        gc += cu;
+        gg1 += cu * g1;
+        gg2 += cu * g2;
+        gx1 += cu * x1;
+        gx2 += cu * x2;
+        gy1 += cu * y1;
+        gy2 += cu * y2;
         if (o->energy) o->energy[is] = g1;
         if (o->energyspread) o->energyspread[is] = sqrt(fabs(g2 - g1 * g1));
         if (o->xposition) o->xposition[is] = x1;
@@ -461,15 +472,30 @@ static void finish_diag(const Ctx *c, const std::vector<double> &hb, const std::
         if (o->bunching) o->bunching[is] = hypot(bre, bim);
         if (o->bunchingphase) o->bunchingphase[is] = atan2(bim, bre);
     }
+    {
+        const double norm = gc > 0 ? 1. / gc : 1.;
+        gg1 *= norm, gg2 *= norm, gx1 *= norm, gx2 *= norm, gy1 *= norm, gy2 *= norm;
+        if (o->g_energy) o->g_energy[0] = gg1;
+        if (o->g_energyspread) o->g_energyspread[0] = sqrt(fabs(gg2 - gg1 * gg1));
+        if (o->g_xposition) o->g_xposition[0] = gx1;
+        if (o->g_xsize) o->g_xsize[0] = sqrt(fabs(gx2 - gx1 * gx1));
+        if (o->g_yposition) o->g_yposition[0] = gy1;
+        if (o->g_ysize) o->g_ysize[0] = sqrt(fabs(gy2 - gy1 * gy1));
+    }
     // DiagField::getValues (Diagnostic.cpp:754-969)
     const double ks = 4. * asin(1) / c->xlambda;
     const double scl = c->dgrid * kEev / ks;
-    double g_pow = 0;
+    double g_pow = 0, fx1 = 0, fx2 = 0, fy1 = 0, fy2 = 0, g_ff = 0, g_in = 0;
     for (int is0 = 0; is0 < ns; is0++) {
         const int is = (ns + is0 - first) % ns;
         const double *v = &hf[8 * is0];
         double power = v[0], x1 = v[1], x2 = v[2], y1 = v[3], y2 = v[4];
         g_pow += power;
+        fx1 += x1;
+        fx2 += x2;
+        fy1 += y1;
+        fy2 += y2;
+        if (o->ff_phase) o->ff_phase[is] = atan2(v[6], v[5]);
         if (power > 0) {
             x1 /= power;
             x2 /= power;
@@ -481,6 +507,8 @@ static void finish_diag(const Ctx *c, const std::vector<double> &hb, const std::
         double farfield = v[5] * v[5] + v[6] * v[6];
         farfield *= c->dgrid * c->dgrid;
         double inten = v[7];
+        g_ff += farfield;
+        g_in += inten;
         power *= scl * scl / kVacimp;
         inten *= kEev * kEev / ks / ks / kVacimp;
         if (o->power) o->power[is] = power;
@@ -491,10 +519,71 @@ static void finish_diag(const Ctx *c, const std::vector<double> &hb, const std::
         if (o->nf_intensity) o->nf_intensity[is] = inten;
         if (o->ff_intensity) o->ff_intensity[is] = farfield;
     }
+    {
+        const double norm = g_pow > 0 ? 1. / g_pow : 1.;
+        fx1 *= norm, fx2 *= norm, fy1 *= norm, fy2 *= norm;
+        if (o->g_f_xsize) o->g_f_xsize[0] = sqrt(fabs(fx2 - fx1 * fx1)) * c->dgrid;
+        if (o->g_f_ysize) o->g_f_ysize[0] = sqrt(fabs(fy2 - fy1 * fy1)) * c->dgrid;
+        if (o->g_f_xposition) o->g_f_xposition[0] = fx1 * c->dgrid;
+        if (o->g_f_yposition) o->g_f_yposition[0] = fy1 * c->dgrid;
+        const double ninv = 1. / (double)ns;
+        if (o->g_nf_intensity) o->g_nf_intensity[0] = g_in * ninv * kEev * kEev / ks / ks / kVacimp;
+        if (o->g_ff_intensity) o->g_ff_intensity[0] = g_ff * ninv;
+    }
     if (o->global_energy) {
         g_pow *= scl * scl / kVacimp * c->slicelength / 299792458.0;
         o->global_energy[0] = g_pow;
     }
+}
+
+static bool wants_aux(const g4mi_diag_out *o) {
+    return o->pxposition || o->pyposition || o->nf_phase || o->efield || o->wakefield || o->lscfield ||
+           o->sscfield || o->xmin || o->xmax || o->pxmin || o->pxmax || o->ymin || o->ymax || o->pymin ||
+           o->pymax || o->emin || o->emax;
+}
+
+// The auxiliary outputs (DiagBeam :415-445, :505-519; DiagField :855-858)
+// from aux[16 ns] of launch_diag_aux / launch_mix_diag_aux and the per-slice
+// long-range (esc) and short-range (ssc) space-charge fields.
+static void finish_aux(const Ctx *c, const std::vector<double> &aux, const std::vector<double> &esc,
+                       const std::vector<double> &ssc, g4mi_diag_out *o) {
+    double *dst[10] = {o->xmin, o->xmax, o->pxmin, o->pxmax, o->ymin, o->ymax, o->pymin, o->pymax, o->emin, o->emax};
+    for (int is = 0; is < c->ns; is++) {
+        const long long n = c->off[is + 1] - c->off[is];
+        const double norm = n > 0 ? 1. / (double)n : 1.;
+        const double *a = &aux[16 * (size_t)is];
+        if (o->pxposition) o->pxposition[is] = a[0] * norm;
+        if (o->pyposition) o->pyposition[is] = a[1] * norm;
+        for (int k = 0; k < 10; k++)
+            if (dst[k]) dst[k][is] = a[2 + k];
+        if (o->nf_phase) o->nf_phase[is] = atan2(a[13], a[12]);
+        const double eloss = 0.0;  // no wakes in this build
+        if (o->efield) o->efield[is] = eloss + esc[is];
+        if (o->wakefield) o->wakefield[is] = eloss;
+        if (o->lscfield) o->lscfield[is] = esc[is];
+        if (o->sscfield) o->sscfield[is] = ssc[is];
+    }
+}
+
+static MixArrays mix_arrays(const Ctx *c);
+
+// aux sums plus the space-charge fields, copied to the host
+static int fetch_aux(Ctx *c, std::vector<double> &aux, std::vector<double> &esc, std::vector<double> &ssc) {
+    const int ns = c->ns;
+    if (!c->d_aux) G4_TRY(dalloc(c, &c->d_aux, 16 * (size_t)ns));
+    if (c->precision == G4MI_FP32)
+        launch_mix_diag_aux(c->stream, mix_arrays(c), c->d_aux);
+    else
+        launch_diag_aux(c->stream, ns, c->d_off, c->d_g, c->d_x, c->d_y, c->d_px, c->d_py, c->d_field, c->ng,
+                        c->first, c->d_aux);
+    aux.resize(16 * (size_t)ns);
+    esc.resize(ns);
+    ssc.resize(ns);
+    G4_HIP(hipMemcpyAsync(aux.data(), c->d_aux, aux.size() * 8, hipMemcpyDeviceToHost, c->stream));
+    G4_HIP(hipMemcpyAsync(esc.data(), c->d_esc, esc.size() * 8, hipMemcpyDeviceToHost, c->stream));
+    G4_HIP(hipMemcpyAsync(ssc.data(), c->d_efield, ssc.size() * 8, hipMemcpyDeviceToHost, c->stream));
+    G4_HIP(hipStreamSynchronize(c->stream));
+    return G4MI_OK;
 }
 
 static int diag(Ctx *c, g4mi_diag_out *o) {
@@ -510,6 +599,11 @@ static int diag(Ctx *c, g4mi_diag_out *o) {
         for (int z : c->zeroed_since_adi)
             for (int k = 0; k < 8; k++) hf[8 * (size_t)z + k] = 0;
     finish_diag(c, hb, hf, c->first, o);
+    if (wants_aux(o)) {
+        std::vector<double> aux, esc, ssc;
+        G4_TRY(fetch_aux(c, aux, esc, ssc));
+        finish_aux(c, aux, esc, ssc, o);
+    }
     return G4MI_OK;
 }
 
@@ -1009,6 +1103,11 @@ static int mix_diag(Ctx *c, g4mi_diag_out *o) {
     G4_HIP(hipMemcpyAsync(&hs, c->m_st, sizeof(MixState), hipMemcpyDeviceToHost, c->stream));
     G4_HIP(hipStreamSynchronize(c->stream));
     finish_diag(c, hb, hf, hs.first, o);
+    if (wants_aux(o)) {
+        std::vector<double> aux, esc, ssc;
+        G4_TRY(fetch_aux(c, aux, esc, ssc));
+        finish_aux(c, aux, esc, ssc, o);
+    }
     return G4MI_OK;
 }
 
@@ -1297,6 +1396,7 @@ int g4mi_beam_upload(g4mi_ctx *h, const double *g, const double *th, const doubl
         if (nb) G4_HIP(hipMemcpyAsync(dst[k], src[k], nb, hipMemcpyHostToDevice, c->stream));
     G4_HIP(hipMemcpyAsync(c->d_cur, cur, c->ns * sizeof(double), hipMemcpyHostToDevice, c->stream));
     G4_HIP(hipStreamSynchronize(c->stream));
+    c->h_cur.assign(cur, cur + c->ns);
     c->has_beam = true;
     return G4MI_OK;
 }

@@ -30,8 +30,6 @@ void set_error(const std::string &msg);
 
 namespace {
 
-constexpr double kVacimpD = 376.73;          // GenMain.cpp:62
-constexpr double kEevD = 510998.95069;       // GenMain.cpp:66
 
 // ---------------------------------------------------------------------------
 // namelist parsing (Parser::parse, src/Main/Parser.cpp:40-92; fillMap :94-124)
@@ -676,9 +674,45 @@ bool write_output(const Deck &dk, const std::string &fname, const std::vector<g4
         put("Beam/xsize", "m", "xsize");
         put("Beam/yposition", "m", "yposition");
         put("Beam/ysize", "m", "ysize");
+        put("Beam/pxposition", "rad", "pxposition");
+        put("Beam/pyposition", "rad", "pyposition");
     }
     put("Beam/bunching", " ", "bunching");
     put("Beam/bunchingphase", "rad", "bunchingphase");
+    if (!st.excl_aux) {
+        put("Beam/efield", "eV/m", "efield");
+        put("Beam/wakefield", "eV/m", "wakefield");
+        put("Beam/LSCfield", "eV/m", "lscfield");
+        put("Beam/SSCfield", "eV/m", "sscfield");
+        put("Beam/xmin", "m", "xmin");
+        put("Beam/xmax", "m", "xmax");
+        put("Beam/pxmin", "rad", "pxmin");
+        put("Beam/pxmax", "rad", "pxmax");
+        put("Beam/ymin", "m", "ymin");
+        put("Beam/ymax", "m", "ymax");
+        put("Beam/pymin", "rad", "pymin");
+        put("Beam/pymax", "rad", "pymax");
+        put("Beam/emin", "mc^2", "emin");
+        put("Beam/emax", "mc^2", "emax");
+    }
+    // global statistics: only when asked for and when there is more than one
+    // slice (Track.cpp: a steady-state run has none)
+    auto put1 = [&](const char *path, const char *unit, const char *key) {
+        auto it = rec.find(key);
+        if (it != rec.end() && nrec > 0) ok &= f.write_single(path, unit, it->second);
+    };
+    if (st.beam_global_stat && ns > 1) {
+        if (!st.excl_energy) {
+            put1("Beam/Global/energy", "mc^2", "g_energy");
+            put1("Beam/Global/energyspread", "mc^2", "g_energyspread");
+        }
+        if (!st.excl_spatial) {
+            put1("Beam/Global/xposition", "m", "g_xposition");
+            put1("Beam/Global/xsize", "m", "g_xsize");
+            put1("Beam/Global/yposition", "m", "g_yposition");
+            put1("Beam/Global/ysize", "m", "g_ysize");
+        }
+    }
     ok &= f.write_buffer("Beam/current", "A", dk.current, 1, (size_t)ns);
     {
         std::vector<double> tw[6];
@@ -696,11 +730,22 @@ bool write_output(const Deck &dk, const std::string &fname, const std::vector<g4
     }
     if (!st.excl_intensity) {
         put("Field/intensity-nearfield", "W/m^2", "nf_intensity");
+        put("Field/phase-nearfield", "rad", "nf_phase");
         put("Field/intensity-farfield", "W/rad^2", "ff_intensity");
+        put("Field/phase-farfield", "rad", "ff_phase");
     }
     if (st.field_global_stat && ns > 1) {
-        auto it = rec.find("global_energy");
-        if (it != rec.end()) ok &= f.write_single("Field/Global/energy", "J", it->second);
+        put1("Field/Global/energy", "J", "global_energy");
+        if (!st.excl_spatial) {
+            put1("Field/Global/xposition", "m", "g_f_xposition");
+            put1("Field/Global/xsize", "m", "g_f_xsize");
+            put1("Field/Global/yposition", "m", "g_f_yposition");
+            put1("Field/Global/ysize", "m", "g_f_ysize");
+        }
+        if (!st.excl_intensity) {
+            put1("Field/Global/intensity-nearfield", "W/m^2", "g_nf_intensity");
+            put1("Field/Global/intensity-farfield", "W/rad^2", "g_ff_intensity");
+        }
     }
     const double dgrid = 2 * dk.gridmax / (dk.ngrid - 1);
     ok &= f.write_single("Field/gridspacing", "m", {dgrid});
@@ -713,7 +758,13 @@ bool write_output(const Deck &dk, const std::string &fname, const std::vector<g4
 
 const char *kRecKeys[] = {"power", "bunching", "bunchingphase", "energy", "energyspread", "xposition",
                           "yposition", "xsize", "ysize", "f_xposition", "f_yposition", "f_xsize",
-                          "f_ysize", "nf_intensity", "ff_intensity"};
+                          "f_ysize", "nf_intensity", "ff_intensity", "pxposition", "pyposition",
+                          "nf_phase", "ff_phase", "efield", "wakefield", "lscfield", "sscfield", "xmin",
+                          "xmax", "pxmin", "pxmax", "ymin", "ymax", "pymin", "pymax", "emin", "emax"};
+// one value per record
+const char *kRecScalars[] = {"global_energy", "g_energy", "g_energyspread", "g_xposition", "g_xsize",
+                             "g_yposition", "g_ysize", "g_f_xposition", "g_f_xsize", "g_f_yposition",
+                             "g_f_ysize", "g_nf_intensity", "g_ff_intensity"};
 
 bool do_track(Deck &dk, Namelist &nl, int device, bool quiet, bool dry, std::string &err) {
     Keys k{nl, ""};
@@ -842,9 +893,43 @@ bool do_track(Deck &dk, Namelist &nl, int device, bool quiet, bool dry, std::str
         }
     }
     for (const char *key : kRecKeys) rec[key].assign(nrec * (size_t)ns, 0.0);
-    rec["global_energy"].assign(nrec, 0.0);
+    for (const char *key : kRecScalars) rec[key].assign(nrec, 0.0);
     auto record = [&](size_t r) {
         g4mi_diag_out d{};
+        auto ps = [&](const char *k) { return rec[k].data() + r * ns; };
+        auto p1 = [&](const char *k) { return rec[k].data() + r; };
+        d.pxposition = ps("pxposition");
+        d.pyposition = ps("pyposition");
+        d.nf_phase = ps("nf_phase");
+        d.ff_phase = ps("ff_phase");
+        if (!st.excl_aux) {
+            d.efield = ps("efield");
+            d.wakefield = ps("wakefield");
+            d.lscfield = ps("lscfield");
+            d.sscfield = ps("sscfield");
+            d.xmin = ps("xmin");
+            d.xmax = ps("xmax");
+            d.pxmin = ps("pxmin");
+            d.pxmax = ps("pxmax");
+            d.ymin = ps("ymin");
+            d.ymax = ps("ymax");
+            d.pymin = ps("pymin");
+            d.pymax = ps("pymax");
+            d.emin = ps("emin");
+            d.emax = ps("emax");
+        }
+        d.g_energy = p1("g_energy");
+        d.g_energyspread = p1("g_energyspread");
+        d.g_xposition = p1("g_xposition");
+        d.g_xsize = p1("g_xsize");
+        d.g_yposition = p1("g_yposition");
+        d.g_ysize = p1("g_ysize");
+        d.g_f_xposition = p1("g_f_xposition");
+        d.g_f_xsize = p1("g_f_xsize");
+        d.g_f_yposition = p1("g_f_yposition");
+        d.g_f_ysize = p1("g_f_ysize");
+        d.g_nf_intensity = p1("g_nf_intensity");
+        d.g_ff_intensity = p1("g_ff_intensity");
         d.power = rec["power"].data() + r * ns;
         d.bunching = rec["bunching"].data() + r * ns;
         d.bunchingphase = rec["bunchingphase"].data() + r * ns;

@@ -58,6 +58,11 @@ int launch_adi_band(hipStream_t s, int ns, const double2 *fin, double2 *fout, co
 void launch_zero(hipStream_t s, double2 *p, long long n);
 void launch_diag_beam(hipStream_t s, int ns, const long long *off, const double *g,
                       const double *th, const double *x, const double *y, double *out);
+// px/py sums, coordinate extremes and the on-axis field value per beam
+// slice, out[16 ns]
+void launch_diag_aux(hipStream_t s, int ns, const long long *off, const double *g, const double *x,
+                     const double *y, const double *px, const double *py, const double2 *field, int ng,
+                     int first, double *out);
 void launch_diag_field(hipStream_t s, int ns, const double2 *field, int ng, double *out);
 void launch_sort_index(hipStream_t s, long long n, const double *th, double invslen, int *atar,
                        unsigned long long *moved);

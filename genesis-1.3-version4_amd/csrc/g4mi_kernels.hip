@@ -1264,6 +1264,76 @@ __global__ void __launch_bounds__(256) k_diag_beam(const long long *__restrict__
 
 // Field part of DiagField::getValues (:754-861) per storage slice:
 // out[8*is0 + ...] = power x1 x2 y1 y2 ffre ffim inten(on-axis |E|^2).
+// Auxiliary beam diagnostics of DiagBeam::getValues (Diagnostic.cpp:415-445,
+// 505-519) per beam slice: sums of px and py, the extremes of x, px, y, py and
+// gamma (starting from the reference's bounds 1e5 / -1e5, gamma 1e7 / 1),
+// and the on-axis value of the slice's field (DiagField, :855-858).
+// out[16 is + ..] = {sum px, sum py, xmin, xmax, pxmin, pxmax, ymin, ymax,
+// pymin, pymax, gmin, gmax, Re E0, Im E0}.
+__device__ __forceinline__ void minmax_block(double (&m)[10], double *lds) {
+    const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
+#pragma unroll
+    for (int k = 0; k < 10; k++)
+        for (int o = 32; o > 0; o >>= 1) {
+            const double u = __shfl_xor(m[k], o);
+            m[k] = (k & 1) ? fmax(m[k], u) : fmin(m[k], u);
+        }
+    __syncthreads();
+    if (lane == 0)
+#pragma unroll
+        for (int k = 0; k < 10; k++) lds[wv * 10 + k] = m[k];
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < 10; k++) {
+        double v = lds[k];
+        for (int q = 1; q < (int)(blockDim.x >> 6); q++)
+            v = (k & 1) ? fmax(v, lds[q * 10 + k]) : fmin(v, lds[q * 10 + k]);
+        m[k] = v;
+    }
+    __syncthreads();
+}
+
+__global__ void __launch_bounds__(256) k_diag_aux(const long long *__restrict__ off,
+                                                  const double *__restrict__ g,
+                                                  const double *__restrict__ x,
+                                                  const double *__restrict__ y,
+                                                  const double *__restrict__ px,
+                                                  const double *__restrict__ py,
+                                                  const double2 *__restrict__ field, int ng, int ns,
+                                                  int first, double *__restrict__ out) {
+    __shared__ double lds[10 * 256];
+    const int is = blockIdx.x;
+    const long long a = off[is], b = off[is + 1];
+    double v[2] = {0, 0};
+    double m[10] = {1e5, -1e5, 1e5, -1e5, 1e5, -1e5, 1e5, -1e5, 1e7, 1};
+    for (long long i = a + threadIdx.x; i < b; i += 256) {
+        const double xi = x[i], yi = y[i], pxi = px[i], pyi = py[i], gi = g[i];
+        v[0] += pxi;
+        v[1] += pyi;
+        m[0] = fmin(m[0], xi);
+        m[1] = fmax(m[1], xi);
+        m[2] = fmin(m[2], pxi);
+        m[3] = fmax(m[3], pxi);
+        m[4] = fmin(m[4], yi);
+        m[5] = fmax(m[5], yi);
+        m[6] = fmin(m[6], pyi);
+        m[7] = fmax(m[7], pyi);
+        m[8] = fmin(m[8], gi);
+        m[9] = fmax(m[9], gi);
+    }
+    block_sum<256, 2>(v, lds);
+    minmax_block(m, lds);
+    if (threadIdx.x == 0) {
+        double *o = out + 16 * (long long)is;
+        o[0] = v[0];
+        o[1] = v[1];
+        for (int k = 0; k < 10; k++) o[2 + k] = m[k];
+        const double2 c = field[(long long)((is + first) % ns) * ng * ng + (ng / 2) * ng + ng / 2];
+        o[12] = c.x;
+        o[13] = c.y;
+    }
+}
+
 __global__ void __launch_bounds__(256) k_diag_field(const double2 *__restrict__ field, int ng,
                                                     double *__restrict__ out) {
     __shared__ double lds[8 * 256];
@@ -1451,6 +1521,11 @@ void launch_zero(hipStream_t s, double2 *p, long long n) {
 void launch_diag_beam(hipStream_t s, int ns, const long long *off, const double *g,
                       const double *th, const double *x, const double *y, double *out) {
     hipLaunchKernelGGL(k_diag_beam, dim3(ns), dim3(256), 0, s, off, g, th, x, y, out);
+}
+void launch_diag_aux(hipStream_t s, int ns, const long long *off, const double *g, const double *x,
+                     const double *y, const double *px, const double *py, const double2 *field, int ng,
+                     int first, double *out) {
+    hipLaunchKernelGGL(k_diag_aux, dim3(ns), dim3(256), 0, s, off, g, x, y, px, py, field, ng, ns, first, out);
 }
 void launch_diag_field(hipStream_t s, int ns, const double2 *field, int ng, double *out) {
     hipLaunchKernelGGL(k_diag_field, dim3(ns), dim3(256), 0, s, field, ng, out);

@@ -124,6 +124,7 @@ void launch_mix_src_sums(hipStream_t s, const MixArrays &a, double *out);
 
 // Standalone sums for a diagnostics call outside a run:
 // beam [ns][8] and field per storage slice [ns][8].
+void launch_mix_diag_aux(hipStream_t s, const MixArrays &a, double *out);
 void launch_mix_diag(hipStream_t s, const MixArrays &a, double *beam8, double *field8);
 void launch_mix_sort_index(hipStream_t s, const MixArrays &a, double invslen, int *atar,
                            unsigned long long *moved);

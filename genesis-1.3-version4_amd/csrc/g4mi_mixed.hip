@@ -404,6 +404,7 @@ k_mix_main(MixArrays a, int sc_on) {
         bsum_w<kMainNT, 1>(v, wred);
         esc = v[0];
     }
+    if (t == 0) a.esc[is] = esc;  // the slice's LSC field for the diagnostics
     const float ez_long = (float)(-esc / kEev);
 
     // ---- short-range space charge
@@ -1496,6 +1497,62 @@ void launch_mix_src_sums(hipStream_t s, const MixArrays &a, double *out) {
 
 void launch_mix_field_sums(hipStream_t s, const MixArrays &a, int next, double *out8) {
     hipLaunchKernelGGL(k_mix_diag_field, dim3(a.ns), dim3(256), 0, s, a, out8, next);
+}
+
+// The auxiliary beam diagnostics and the on-axis field value per beam slice
+// (the FP64 form's k_diag_aux on the offset form), out[16 ns].
+__global__ void __launch_bounds__(256) k_mix_diag_aux(MixArrays a, double *out) {
+    __shared__ double lds[10 * 256];
+    const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
+    const int is = blockIdx.x;
+    const long long b0 = a.off[is], b1 = a.off[is + 1];
+    double v[2] = {0, 0};
+    double m[10] = {1e5, -1e5, 1e5, -1e5, 1e5, -1e5, 1e5, -1e5, 1e7, 1};
+    for (long long i = b0 + t; i < b1; i += 256) {
+        const double xi = a.x[i], yi = a.y[i], pxi = a.px[i], pyi = a.py[i];
+        const double gi = a.gref + (double)a.dg[i];
+        v[0] += pxi;
+        v[1] += pyi;
+        m[0] = fmin(m[0], xi);
+        m[1] = fmax(m[1], xi);
+        m[2] = fmin(m[2], pxi);
+        m[3] = fmax(m[3], pxi);
+        m[4] = fmin(m[4], yi);
+        m[5] = fmax(m[5], yi);
+        m[6] = fmin(m[6], pyi);
+        m[7] = fmax(m[7], pyi);
+        m[8] = fmin(m[8], gi);
+        m[9] = fmax(m[9], gi);
+    }
+    bsum<256, 2>(v, lds);
+#pragma unroll
+    for (int k = 0; k < 10; k++)
+        for (int o = 32; o > 0; o >>= 1) {
+            const double u = __shfl_xor(m[k], o);
+            m[k] = (k & 1) ? fmax(m[k], u) : fmin(m[k], u);
+        }
+    __syncthreads();
+    if (lane == 0)
+        for (int k = 0; k < 10; k++) lds[wv * 10 + k] = m[k];
+    __syncthreads();
+    if (t == 0) {
+        double *o = out + 16 * (long long)is;
+        o[0] = v[0];
+        o[1] = v[1];
+        for (int k = 0; k < 10; k++) {
+            double r = lds[k];
+            for (int q = 1; q < 4; q++) r = (k & 1) ? fmax(r, lds[q * 10 + k]) : fmin(r, lds[q * 10 + k]);
+            o[2 + k] = r;
+        }
+        const int ng = a.ng;
+        const float2 c = a.field[a.st->cur][(long long)((is + a.st->first) % a.ns) * ng * ng + (ng / 2) * ng + ng / 2];
+        o[12] = c.x;
+        o[13] = c.y;
+    }
+}
+
+void launch_mix_diag_aux(hipStream_t s, const MixArrays &a, double *out) {
+    hipLaunchKernelGGL(k_mix_diag_aux, dim3(a.ns), dim3(256), 0, s, a, out);
 }
 
 void launch_mix_diag(hipStream_t s, const MixArrays &a, double *beam8, double *field8) {

@@ -43,7 +43,17 @@ class SliceBeam(C.Structure):
 
 DIAG_FIELDS = ("power", "bunching", "bunchingphase", "energy", "energyspread", "xposition",
                "yposition", "xsize", "ysize", "f_xposition", "f_yposition", "f_xsize",
-               "f_ysize", "nf_intensity", "ff_intensity", "global_energy")
+               "f_ysize", "nf_intensity", "ff_intensity", "global_energy",
+               "pxposition", "pyposition", "nf_phase", "ff_phase", "efield", "wakefield",
+               "lscfield", "sscfield", "xmin", "xmax", "pxmin", "pxmax", "ymin", "ymax",
+               "pymin", "pymax", "emin", "emax",
+               "g_energy", "g_energyspread", "g_xposition", "g_xsize", "g_yposition", "g_ysize",
+               "g_f_xposition", "g_f_xsize", "g_f_yposition", "g_f_ysize", "g_nf_intensity",
+               "g_ff_intensity")
+# outputs of one value per record rather than one per slice
+DIAG_SCALARS = ("global_energy", "g_energy", "g_energyspread", "g_xposition", "g_xsize",
+                "g_yposition", "g_ysize", "g_f_xposition", "g_f_xsize", "g_f_yposition",
+                "g_f_ysize", "g_nf_intensity", "g_ff_intensity")
 
 
 class DiagOut(C.Structure):
@@ -236,10 +246,11 @@ class Context:
         _chk(lib().g4mi_sync(self.h), "g4mi_sync")
 
     def diag(self):
-        out = {k: np.zeros(self.ns) for k in DIAG_FIELDS}
+        out = {k: np.zeros(1 if k in DIAG_SCALARS else self.ns) for k in DIAG_FIELDS}
         d = DiagOut(*[_p(out[k]) for k in DIAG_FIELDS])
         _chk(lib().g4mi_diag(self.h, C.byref(d)), "g4mi_diag")
-        out["global_energy"] = out["global_energy"][0]
+        for k in DIAG_SCALARS:
+            out[k] = out[k][0]
         return out
 
     def run(self, steps, gamma0, record=("power", "bunching", "energy")):

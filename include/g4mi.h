@@ -56,6 +56,19 @@ typedef struct {
     double *f_xposition, *f_yposition, *f_xsize, *f_ysize;
     double *nf_intensity, *ff_intensity;
     double *global_energy; /* [1] */
+    /* DiagBeam: mean momenta and the auxiliary block (LSC/SSC fields in
+     * eV/m, wake field 0 in this build, coordinate extremes);
+     * DiagField: near- and far-field phases */
+    double *pxposition, *pyposition;
+    double *nf_phase, *ff_phase;
+    double *efield, *wakefield, *lscfield, *sscfield;
+    double *xmin, *xmax, *pxmin, *pxmax, *ymin, *ymax, *pymin, *pymax, *emin, *emax;
+    /* [1] each: current-weighted beam statistics over all slices
+     * (Diagnostic.cpp:565-607) and power-weighted field statistics
+     * (:906-969) */
+    double *g_energy, *g_energyspread, *g_xposition, *g_xsize, *g_yposition, *g_ysize;
+    double *g_f_xposition, *g_f_xsize, *g_f_yposition, *g_f_ysize;
+    double *g_nf_intensity, *g_ff_intensity;
 } g4mi_diag_out;
 
 /* Context on HIP device `device` holding a time window of `nslice` slices

@@ -186,6 +186,27 @@ def test_mixed_upload_download_and_sort_index():
     ctx.close()
 
 
+def test_mixed_diag_aux_and_global(st):
+    """The auxiliary and global diagnostics of the mixed form against numpy
+    on its own downloaded state (FP64 reductions of the same values), and
+    its space-charge fields against the FP64 oracle's after the same steps
+    (FP32 moments and solve: 1e-4 and 1e-3)."""
+    sim = H.make_state([1024, 512, 768, 256], seed_power=5e3, longrange=True, sc=(1, 0, 100, 0.0),
+                       currents=[3000.0, 2000.0, 2500.0, 1000.0])
+    ctx = H.to_gpu(sim, precision=g4mi.FP32)
+    ctx.run(H.gsteps(st[:3]), cases.GAMMA0, record=())
+    for i in range(3):
+        O.lib().orc_step_all(C.byref(sim.s), C.byref(st[i]), cases.GAMMA0)
+    d = ctx.diag()
+    b = ctx.download_beam()
+    f, first = ctx.download_field()
+    ref = H.aux_reference(b, f, first, sim.off, sim.current, sim.s.ngrid, sim.s.dgrid, cases.LAMBDA_R)
+    H.check_aux(d, ref, b, 1e-6)
+    assert H.rel_err(d["lscfield"], sim.longESC, 1e-12) < 1e-4
+    assert H.rel_err(d["sscfield"], sim.sc_efield, 1e-12) < 1e-3
+    ctx.close()
+
+
 def test_mixed_rejects_split_calls(st):
     sim = H.make_state([64] * 2)
     ctx = H.to_gpu(sim, precision=g4mi.FP32)

@@ -191,6 +191,32 @@ def test_diag_parity():
     assert abs(d["global_energy"] / r["global_energy"] - 1) < 1e-13
 
 
+def test_diag_aux_and_global():
+    """Mean momenta, coordinate extremes, near/far-field phases and the
+    global (current- and power-weighted) statistics against numpy on the
+    downloaded state after 3 steps; the long- and short-range space-charge
+    fields of the auxiliary block against the oracle's after the same steps."""
+    _log("auxiliary and global diagnostics, 4 uneven slices, 3 steps")
+    st = H.steps()
+    sim = H.make_state([1024, 512, 768, 256], seed_power=5e3, longrange=True, sc=(1, 0, 100, 0.0),
+                       currents=[3000.0, 2000.0, 2500.0, 1000.0])
+    ctx = H.to_gpu(sim)
+    ctx.run(H.gsteps(st[:3]), cases.GAMMA0, record=())
+    for i in range(3):
+        O.lib().orc_step_all(C.byref(sim.s), C.byref(st[i]), cases.GAMMA0)
+    d = ctx.diag()
+    b = ctx.download_beam()
+    f, first = ctx.download_field()
+    ref = H.aux_reference(b, f, first, sim.off, sim.current, sim.s.ngrid, sim.s.dgrid, cases.LAMBDA_R)
+    H.check_aux(d, ref, b, 1e-10)
+    # the energy spread's cancellation (see test_diag_parity)
+    assert abs(d["g_energyspread"] / ref["g_energyspread"] - 1) < 1e-6
+    assert H.rel_err(d["lscfield"], sim.longESC, 1e-12) < 1e-10
+    assert H.rel_err(d["sscfield"], sim.sc_efield, 1e-12) < 1e-10
+    assert np.array_equal(d["efield"], d["lscfield"]) and not np.any(d["wakefield"])
+    ctx.close()
+
+
 def test_steady_state_config0_end_to_end():
     """Config 0: steady state, one slice of 8192 (nbins 4), 151^2, 5 kW
     seed, four 125-period modules with FODO drifts (509 steps)."""
