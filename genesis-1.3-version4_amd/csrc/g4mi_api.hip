@@ -900,7 +900,7 @@ static int mix_step_fno(Ctx *c, const MixKey &k) {
     int ntiles = 0;
     MixState hs{};
     if (c->fno_mode == 1) {
-        launch_mix_beam(c->stream, a, sc, moments, nullptr);
+        launch_mix_beam(c->stream, a, sc, moments, nullptr, false);
         if (launch_mix_adi(c->stream, a, tm, refine, stream, &ntiles)) {
             set_error("mixed form: no ADI instantiation for this band width");
             return G4MI_E_ARG;
@@ -925,7 +925,7 @@ static int mix_step_fno(Ctx *c, const MixKey &k) {
     std::vector<double> hf0(8 * (size_t)ns), hf1(8 * (size_t)ns), eg(ns), ss(ns);
     launch_mix_field_sums(c->stream, a, 0, c->d_diag_f);
     G4_HIP(hipMemcpyAsync(hf0.data(), c->d_diag_f, hf0.size() * 8, hipMemcpyDeviceToHost, c->stream));
-    launch_mix_beam(c->stream, a, sc, moments, nullptr);
+    launch_mix_beam(c->stream, a, sc, moments, nullptr, true);
     G4_HIP(hipMemcpyAsync(&hs, c->m_st, sizeof(MixState), hipMemcpyDeviceToHost, c->stream));
     G4_HIP(hipStreamSynchronize(c->stream));
     G4_TRY(fno_infer_slices(c->fno, c->stream, c->m_field[hs.cur], c->m_field[hs.cur ^ 1], c->m_src, c->d_box, ns,

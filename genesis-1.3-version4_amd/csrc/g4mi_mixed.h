@@ -113,7 +113,8 @@ int launch_mix_step(hipStream_t s, const MixArrays &a, int tmpl_m, bool refine, 
 // The same step in three parts: the beam kernels (front, scan, main; ev
 // slots 0-6), the ADI (ntiles: the field-sum blocks per slice it wrote) and
 // the tail (slippage, zeroing, record of ntiles blocks; ev slots 7-8).
-void launch_mix_beam(hipStream_t s, const MixArrays &a, bool sc, bool moments, hipEvent_t *ev);
+// egsum: also sum the push's energy increments per slice (a.egsum)
+void launch_mix_beam(hipStream_t s, const MixArrays &a, bool sc, bool moments, hipEvent_t *ev, bool egsum);
 int launch_mix_adi(hipStream_t s, const MixArrays &a, int tmpl_m, bool refine, bool stream, int *ntiles);
 void launch_mix_tail(hipStream_t s, const MixArrays &a, int maxzero, int ntiles, hipEvent_t *ev);
 // Field sums per storage slice ([ns][8], as the ADI's diagnostics with one

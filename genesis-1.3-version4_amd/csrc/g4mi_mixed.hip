@@ -371,6 +371,10 @@ __device__ __forceinline__ void bmin_w(int (&v)[K], int *lds) {
 constexpr int kCelmCells = 1024;                          // space-charge mode solutions (float2)
 constexpr int kTileCells = (kMixDyn - kCelmCells) / 2;    // double2 cells: histograms, then the deposit tile
 
+// EG: also sum the push's energy increments per slice (egsum, the energy
+// balance of the learned field step).  A variant of its own because the
+// extra accumulator costs registers the common path does not pay for.
+template <bool EG>
 __global__ void __launch_bounds__(kMainNT)
 k_mix_main(MixArrays a, int sc_on) {
     extern __shared__ float2 dyn[];
@@ -535,7 +539,10 @@ k_mix_main(MixArrays a, int sc_on) {
     const float dgx = (float)D.gradx, dgy = (float)D.grady;
     // per-lane beam sums of at most a few dozen particles in FP32, energy as
     // the offset dg; widened to FP64 for the block sum
-    float fv[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};  // [8]: the push's energy increments
+    constexpr int NV = EG ? 9 : 8;
+    float fv[NV];  // beam sums; [8]: the push's energy increments
+#pragma unroll
+    for (int k = 0; k < NV; k++) fv[k] = 0.f;
     int bx0 = 1 << 30, by0 = 1 << 30, bx1 = -1, by1 = -1, ovf = 0;
     const unsigned stepno = a.st->seq;
     const float inv_dgrid = 1.f / dgrid;
@@ -644,7 +651,7 @@ k_mix_main(MixArrays a, int sc_on) {
         et += stpz * (k3pp + o.kpp / 6.0f);
         const unsigned hs = mix_hash((unsigned long long)i, stepno, 0x51u);
         const float dg = add_sr(g0, eg, hs);
-        fv[8] += eg;
+        if constexpr (EG) fv[NV - 1] += eg;
         if (S.chicane) et += r56 * (dg + dgam0);  // TrackBeam::applyR56
         const float dth = add_sr(t0, et, mix_hash((unsigned long long)i, stepno, 0xA7u));
         transverse32(S.t2, gref + dg, x, y, px, py);
@@ -695,17 +702,17 @@ k_mix_main(MixArrays a, int sc_on) {
             }
         }
     }
-    double v[9];
+    double v[NV];
 #pragma unroll
-    for (int k = 0; k < 9; k++) v[k] = fv[k];
-    bsum_w<kMainNT, 9>(v, wred);
+    for (int k = 0; k < NV; k++) v[k] = fv[k];
+    bsum_w<kMainNT, NV>(v, wred);
     if (t == 0) {
         // sum gamma = n gref + sum dg, sum gamma^2 = n gref^2 + 2 gref sum dg + sum dg^2
         const double n = (double)npart, gr = a.gref;
         a.beamsum[8 * is + 0] = n * gr + v[0];
         a.beamsum[8 * is + 1] = n * gr * gr + 2.0 * gr * v[0] + v[1];
         for (int k = 2; k < 8; k++) a.beamsum[8 * is + k] = v[k];
-        a.egsum[is] = v[8];
+        if constexpr (EG) a.egsum[is] = v[NV - 1];
         a.base[is] = basep + S.kappa * P.delz;
     }
     // ---- footprint box and overflow flag
@@ -1417,8 +1424,10 @@ size_t mix_band_lds(int tmpl_m, bool refine) {
 }
 
 int mix_prepare() {
-    int ok = hipFuncSetAttribute((const void *)k_mix_main, hipFuncAttributeMaxDynamicSharedMemorySize,
+    int ok = hipFuncSetAttribute((const void *)k_mix_main<true>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                  kMixDyn * (int)sizeof(float2)) == hipSuccess;
+    ok &= hipFuncSetAttribute((const void *)k_mix_main<false>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                              kMixDyn * (int)sizeof(float2)) == hipSuccess;
 #define G4_P(MM) ok &= set_adi_attr<MM, false>(); if (MixGeom<MM, true>::lds_bytes <= 160 * 1024) ok &= set_adi_attr<MM, true>();
     G4_P(3) G4_P(4) G4_P(6) G4_P(9) G4_P(12)
 #undef G4_P
@@ -1427,7 +1436,7 @@ int mix_prepare() {
 
 int mix_sc_mode_cells() { return kCelmCells; }
 
-void launch_mix_beam(hipStream_t s, const MixArrays &a, bool sc, bool moments, hipEvent_t *ev) {
+void launch_mix_beam(hipStream_t s, const MixArrays &a, bool sc, bool moments, hipEvent_t *ev, bool egsum) {
     auto mark = [&](int k) {
         if (ev) (void)hipEventRecord(ev[k], s);
     };
@@ -1436,8 +1445,12 @@ void launch_mix_beam(hipStream_t s, const MixArrays &a, bool sc, bool moments, h
     if (sc) hipLaunchKernelGGL(k_mix_scan, dim3(1), dim3(64), 0, s, a);
     mark(1);
     mark(2);
-    hipLaunchKernelGGL(k_mix_main, dim3(a.ns), dim3(kMainNT), kMixDyn * sizeof(float2), s, a,
-                       sc ? 1 : 0);
+    if (egsum)
+        hipLaunchKernelGGL(k_mix_main<true>, dim3(a.ns), dim3(kMainNT), kMixDyn * sizeof(float2), s, a,
+                           sc ? 1 : 0);
+    else
+        hipLaunchKernelGGL(k_mix_main<false>, dim3(a.ns), dim3(kMainNT), kMixDyn * sizeof(float2), s, a,
+                           sc ? 1 : 0);
     mark(3);
     mark(4);
     mark(5);
@@ -1484,7 +1497,7 @@ void launch_mix_tail(hipStream_t s, const MixArrays &a, int maxzero, int ntiles,
 
 int launch_mix_step(hipStream_t s, const MixArrays &a, int tmpl_m, bool refine, bool sc,
                     bool moments, int maxzero, hipEvent_t *ev, bool stream) {
-    launch_mix_beam(s, a, sc, moments, ev);
+    launch_mix_beam(s, a, sc, moments, ev, false);
     int ntiles = 0;
     if (launch_mix_adi(s, a, tmpl_m, refine, stream, &ntiles) != 0) return -1;
     launch_mix_tail(s, a, maxzero, ntiles, ev);
