@@ -4,7 +4,7 @@
 //
 // Covered elements (src/Main/GenMain.cpp:80-494 dispatch order, element by
 // element in file order): &setup, &lattice (zmatch), &time, &field, &beam,
-// &efield, &track and &stop.  Any other element, or a key an element does
+// &efield, &track, &write and &stop.  Any other element, or a key an element does
 // not know, stops the run with an error, as the reference does for unknown
 // keys.  Profile references ("@name") are not supported.
 #include <sys/stat.h>
@@ -30,6 +30,8 @@ void set_error(const std::string &msg);
 
 namespace {
 
+constexpr double kVacimpD = 376.73;         // vacuum impedance [Ohm] (src/Main/GenMain.cpp:63)
+constexpr double kEevD = 510998.95069;      // electron rest energy [eV] (src/Main/GenMain.cpp:67)
 
 // ---------------------------------------------------------------------------
 // namelist parsing (Parser::parse, src/Main/Parser.cpp:40-92; fillMap :94-124)
@@ -573,12 +575,175 @@ std::string env_or(const char *k, const char *dflt) {
     return v ? std::string(v) : std::string(dflt);
 }
 
+// ---------------------------------------------------------------------------
+// dumps
+// ---------------------------------------------------------------------------
+void write_version(H5File &f, bool &ok) {
+    ok &= f.write_single("Meta/Version/Major", " ", {4.0});
+    ok &= f.write_single("Meta/Version/Minor", " ", {6.0});
+    ok &= f.write_single("Meta/Version/Revision", " ", {0.0});
+    ok &= f.write_single("Meta/Version/Beta", " ", {0.0});
+    ok &= f.write_string("Meta/Version/Build_Info", "g4mi: undulator step on MI355X (HIP), output layout of GENESIS 1.3 v4");
+}
+
+// the state on the host: download from the device when a run has started
+bool host_state(Deck &dk, int *first, std::string &err) {
+    *first = 0;
+    if (!dk.ctx) return true;
+    if (g4mi_beam_download(dk.ctx, dk.gamma.data(), dk.theta.data(), dk.x.data(), dk.y.data(), dk.px.data(),
+                           dk.py.data()) != G4MI_OK ||
+        g4mi_field_download(dk.ctx, dk.field.data(), first) != G4MI_OK) {
+        err = g4mi_last_error();
+        return false;
+    }
+    return true;
+}
+
+// <basename>.fld.h5 (WriteFieldHDF5::writeMain, src/IO/writeFieldHDF5.cpp:42-171):
+// globals, one group per slice in window order with the field scaled to
+// sqrt(W) per cell, and the intensity projections int_xy, int_xz, int_yz.
+bool write_field_dump(const Deck &dk, const std::string &basename, int first, std::string &err) {
+    const std::string fname = basename + ".fld.h5";
+    H5File f(fname);
+    if (!f.ok()) {
+        err = "unable to write " + fname + ": " + f.error();
+        return false;
+    }
+    bool ok = true;
+    write_version(f, ok);
+    const int ns = (int)dk.off.size() - 1, ng = dk.ngrid;
+    const size_t cells = (size_t)ng * ng;
+    const double dgrid = 2 * dk.gridmax / (ng - 1);
+    ok &= f.write_single("wavelength", "m", {dk.flambda});
+    ok &= f.write_single("slicespacing", "m", {dk.flambda * dk.time.sample});
+    ok &= f.write_single("refposition", "m", {dk.time.s0});
+    ok &= f.write_single("gridsize", "m", {dgrid});
+    ok &= f.write_int("gridpoints", {ng});
+    ok &= f.write_int("slicecount", {ns});
+    const double ks = 4. * asin(1) / dk.flambda;
+    const double scl = dgrid * kEevD / ks / sqrt(kVacimpD);
+    std::vector<double> re(cells), im(cells), ixy(cells, 0.0), ixz((size_t)ng * ns, 0.0), iyz((size_t)ng * ns, 0.0);
+    char name[32];
+    for (int i = 0; i < ns; i++) {
+        const double *fs = dk.field.data() + (size_t)((i + first) % ns) * cells * 2;
+        for (size_t j = 0; j < cells; j++) {
+            re[j] = scl * fs[2 * j];
+            im[j] = scl * fs[2 * j + 1];
+        }
+        if (!dk.setup.excl_field_dump) {
+            snprintf(name, sizeof(name), "slice%6.6d", i + 1);
+            ok &= f.write_single(std::string(name) + "/field-real", " ", re);
+            ok &= f.write_single(std::string(name) + "/field-imag", " ", im);
+        }
+        for (int iy = 0; iy < ng; iy++)
+            for (int ix = 0; ix < ng; ix++) {
+                const size_t j = (size_t)iy * ng + ix;
+                const double w = fs[2 * j] * fs[2 * j] + fs[2 * j + 1] * fs[2 * j + 1];
+                ixy[j] += w;
+                ixz[(size_t)ns * ix + i] += w;
+                iyz[(size_t)ns * iy + i] += w;
+            }
+    }
+    ok &= f.write_single("int_xy", " ", ixy);
+    ok &= f.write_buffer("int_xz", " ", ixz, (size_t)ng, (size_t)ns);
+    ok &= f.write_buffer("int_yz", " ", iyz, (size_t)ng, (size_t)ns);
+    f.close();
+    if (!ok) err = "writing " + fname + " failed";
+    return ok;
+}
+
+// <basename>.par.h5 (WriteBeamHDF5::write, src/IO/writeBeamHDF5.cpp:14-148):
+// globals and one group per slice with the current and every stride-th
+// particle's coordinates.
+bool write_beam_dump(const Deck &dk, const std::string &basename, int stride, std::string &err) {
+    const std::string fname = basename + ".par.h5";
+    H5File f(fname);
+    if (!f.ok()) {
+        err = "unable to write " + fname + ": " + f.error();
+        return false;
+    }
+    bool ok = true;
+    write_version(f, ok);
+    const int ns = (int)dk.off.size() - 1;
+    ok &= f.write_single("slicelength", "m", {dk.setup.lambda0});
+    ok &= f.write_single("slicespacing", "m", {dk.setup.lambda0 * dk.time.sample});
+    ok &= f.write_single("refposition", "m", {dk.time.s0});
+    ok &= f.write_int("beamletsize", {dk.setup.nbins});
+    ok &= f.write_int("slicecount", {ns});
+    ok &= f.write_int("one4one", {0});
+    if (stride < 1) stride = 1;
+    char name[32];
+    std::vector<double> w;
+    for (int i = 0; i < ns; i++) {
+        snprintf(name, sizeof(name), "slice%6.6d", i + 1);
+        const std::string g(name);
+        ok &= f.write_single(g + "/current", "A", {dk.current[i]});
+        const long long a = dk.off[i], b = dk.off[i + 1];
+        auto put = [&](const char *n, const char *unit, const std::vector<double> &v) {
+            w.clear();
+            for (long long k = a; k < b; k += stride) w.push_back(v[(size_t)k]);
+            if (!w.empty()) ok &= f.write_single(g + "/" + n, unit, w);
+        };
+        put("gamma", " ", dk.gamma);
+        put("theta", "rad", dk.theta);
+        put("x", "m", dk.x);
+        put("y", "m", dk.y);
+        put("px", "rad", dk.px);
+        put("py", "rad", dk.py);
+    }
+    f.close();
+    if (!ok) err = "writing " + fname + " failed";
+    return ok;
+}
+
+std::string file_name(const SetupNL &st, std::string name) {  // Setup::RootName_to_FileName
+    size_t p;
+    while ((p = name.find('@')) != std::string::npos) name.replace(p, 1, st.rootname);
+    return st.outputdir.empty() ? name : st.outputdir + "/" + name;
+}
+
+// &write (src/Main/Dump.cpp:19-62)
+bool do_write(Deck &dk, Namelist &nl, std::string &err) {
+    Keys k{nl, ""};
+    std::string fieldname, beamname;
+    int stride = 1;
+    k.str("field", fieldname);
+    k.str("beam", beamname);
+    k.integer("stride", stride);
+    if (!k.finish()) {
+        err = k.err;
+        return false;
+    }
+    int first = 0;
+    if (!host_state(dk, &first, err)) return false;
+    if (!fieldname.empty()) {
+        if (!dk.have_field) {
+            err = "&write: no field defined";
+            return false;
+        }
+        if (!write_field_dump(dk, file_name(dk.setup, fieldname), first, err)) return false;
+    }
+    if (!beamname.empty()) {
+        if (!dk.have_beam) {
+            err = "&write: no beam defined";
+            return false;
+        }
+        if (!write_beam_dump(dk, file_name(dk.setup, beamname), stride < 1 ? 1 : stride, err)) return false;
+    }
+    return true;
+}
+
+struct DumpList {
+    std::vector<int> field_step, beam_step;
+    std::vector<std::string> field_file, beam_file;
+};
+
 // Writes <rootname>.out.h5: /Meta, /Global, /Lattice, /Beam, /Field
 // (Diagnostic::writeToOutputFile, src/Core/Diagnostic.cpp:135-245;
 // Output::writeMeta :67-160; HDF5Base writers).
 bool write_output(const Deck &dk, const std::string &fname, const std::vector<g4mi_step_params> &steps,
                   const std::vector<double> &zout, const std::map<std::string, std::vector<double>> &rec,
-                  size_t nrec, std::string &err) {
+                  size_t nrec, const DumpList &dl, std::string &err) {
     H5File f(fname);
     if (!f.ok()) {
         err = "unable to open output file " + fname + ": " + f.error();
@@ -588,11 +753,7 @@ bool write_output(const Deck &dk, const std::string &fname, const std::vector<g4
     const int ns = (int)dk.off.size() - 1;
     bool ok = true;
     // Meta
-    ok &= f.write_single("Meta/Version/Major", " ", {4.0});
-    ok &= f.write_single("Meta/Version/Minor", " ", {6.0});
-    ok &= f.write_single("Meta/Version/Revision", " ", {0.0});
-    ok &= f.write_single("Meta/Version/Beta", " ", {0.0});
-    ok &= f.write_string("Meta/Version/Build_Info", "g4mi: undulator step on MI355X (HIP), output layout of GENESIS 1.3 v4");
+    write_version(f, ok);
     {
         time_t now = time(nullptr);
         char buf[64];
@@ -608,8 +769,15 @@ bool write_output(const Deck &dk, const std::string &fname, const std::vector<g4
     }
     ok &= f.write_string("Meta/InputFile", dk.input_text);
     ok &= f.write_string("Meta/LatticeFile", dk.lattice_text);
-    ok &= f.write_int("Meta/Fielddumps/ndumps", {0});
-    ok &= f.write_int("Meta/Beamdumps/ndumps", {0});
+    // Output::reportDumps (src/IO/Output.cpp:200-247)
+    ok &= f.write_int("Meta/Fielddumps/ndumps", {(int)dl.field_step.size()});
+    if (!dl.field_step.empty()) ok &= f.write_int("Meta/Fielddumps/intstep", dl.field_step);
+    for (size_t k = 0; k < dl.field_file.size(); k++)
+        ok &= f.write_string("Meta/Fielddumps/filename" + std::to_string(k + 1), dl.field_file[k]);
+    ok &= f.write_int("Meta/Beamdumps/ndumps", {(int)dl.beam_step.size()});
+    if (!dl.beam_step.empty()) ok &= f.write_int("Meta/Beamdumps/intstep", dl.beam_step);
+    for (size_t k = 0; k < dl.beam_file.size(); k++)
+        ok &= f.write_string("Meta/Beamdumps/filename" + std::to_string(k + 1), dl.beam_file[k]);
     ok &= f.write_single("Meta/mpisize", " ", {1.0});
     // Lattice: per-step arrays and the output positions
     auto col = [&](double g4mi_step_params::*m) {
@@ -821,14 +989,20 @@ bool do_track(Deck &dk, Namelist &nl, int device, bool quiet, bool dry, std::str
         return false;
     }
     std::vector<g4mi_step_params> steps = lr.steps;
-    bool dumps = fdump > 0 || bdump > 0 || dump_ue;
-    for (size_t i = 0; i < steps.size(); i++) {  // Undulator::updateMarker
-        if (sort_step > 0 && i % (size_t)sort_step == 0) steps[i].marker |= 4;
-        if (steps[i].z > zstop) steps[i].marker |= 8;
-        if (steps[i].marker & 3) dumps = true;
+    // Undulator::updateMarker (src/Core/Undulator.cpp:33-64) and markUndExits:
+    // dump bits at positions 0..nsteps (before step i; nsteps = after the last)
+    const size_t nst = steps.size();
+    std::vector<int> dflag(nst + 1, 0);
+    for (size_t i = 0; i <= nst; i++) {
+        if (i < nst) {
+            if (sort_step > 0 && i % (size_t)sort_step == 0) steps[i].marker |= 4;
+            if (steps[i].z > zstop) steps[i].marker |= 8;
+            dflag[i] = steps[i].marker & 3;
+        }
+        if (fdump > 0 && i % (size_t)fdump == 0) dflag[i] |= 1;
+        if (bdump > 0 && i % (size_t)bdump == 0) dflag[i] |= 2;
+        if (dump_ue && i > 0 && i < nst && steps[i - 1].aw != 0 && steps[i].aw == 0) dflag[i] |= 1;
     }
-    if (dumps && !quiet)
-        printf("*** Warning: field and beam dumps are not written by this build; continuing without them\n");
     // steps that run (up to the first stop marker) and the output records
     // (Undulator::updateOutput: the initial state, then after step i (1-based)
     // when i %% output_step == 0 and step i carries no stop marker)
@@ -855,11 +1029,12 @@ bool do_track(Deck &dk, Namelist &nl, int device, bool quiet, bool dry, std::str
     const std::string fname = root + ".out.h5";
 
     std::map<std::string, std::vector<double>> rec;
+    DumpList dl;
     if (dry) {
         // setup check: the output file without device records
         dk.runs++;
         if (st.rootname.empty()) return true;
-        return write_output(dk, fname, steps, zout, rec, 0, err);
+        return write_output(dk, fname, steps, zout, rec, 0, dl, err);
     }
     // device context: created by the first &track, reused by later ones
     const int precision = prec == "fp32" ? G4MI_FP32 : G4MI_FP64;
@@ -954,19 +1129,45 @@ bool do_track(Deck &dk, Namelist &nl, int device, bool quiet, bool dry, std::str
     }
     if (!quiet) printf("Running %zu steps on the device (%d slices, %lld macroparticles)...\n", nrun, ns,
                        (long long)dk.off[ns]);
-    size_t done = 0;
-    for (size_t r = 0; r < outs.size(); r++) {
-        const size_t upto = outs[r];
-        const int rc = g4mi_run(dk.ctx, steps.data() + done, (int)(upto - done), st.gamma0, nullptr);
-        if (rc < 0) {
-            err = g4mi_last_error();
-            return false;
+    // Control::applyMarker's dumps at position p (before step p, file
+    // <root>.<p>), the output records after step p; positions 0..nrun
+    auto dump_at = [&](size_t p) {
+        if (!dflag[p]) return true;
+        int first = 0;
+        if (!host_state(dk, &first, err)) return false;
+        const std::string base = root + "." + std::to_string(p);
+        if (dflag[p] & 1) {
+            if (!quiet) printf("Writing field distribution to file: %s.fld.h5 ...\n", base.c_str());
+            if (!write_field_dump(dk, base, first, err)) return false;
+            dl.field_step.push_back((int)p);
+            dl.field_file.push_back(base + ".fld.h5");
         }
-        done = upto;
-        if (record(r + 1) != G4MI_OK) {
-            err = g4mi_last_error();
-            return false;
+        if (dflag[p] & 2) {
+            if (!quiet) printf("Writing particle distribution to file: %s.par.h5 ...\n", base.c_str());
+            if (!write_beam_dump(dk, base, 1, err)) return false;
+            dl.beam_step.push_back((int)p);
+            dl.beam_file.push_back(base + ".par.h5");
         }
+        return true;
+    };
+    size_t done = 0, r = 0;
+    for (size_t p = 0; p <= nrun; p++) {
+        const bool out_here = r < outs.size() && outs[r] == p;
+        if (!out_here && !dflag[p]) continue;
+        if (p > done) {
+            if (g4mi_run(dk.ctx, steps.data() + done, (int)(p - done), st.gamma0, nullptr) < 0) {
+                err = g4mi_last_error();
+                return false;
+            }
+            done = p;
+        }
+        if (out_here) {
+            if (record(++r) != G4MI_OK) {
+                err = g4mi_last_error();
+                return false;
+            }
+        }
+        if (!dump_at(p)) return false;
     }
     if (done < nrun && g4mi_run(dk.ctx, steps.data() + done, (int)(nrun - done), st.gamma0, nullptr) < 0) {
         err = g4mi_last_error();
@@ -978,7 +1179,7 @@ bool do_track(Deck &dk, Namelist &nl, int device, bool quiet, bool dry, std::str
         return false;
     }
     if (!quiet) printf("Writing output file %s...\n", fname.c_str());
-    return write_output(dk, fname, steps, zout, rec, nrec, err);
+    return write_output(dk, fname, steps, zout, rec, nrec, dl, err);
 }
 
 int run_deck(const char *input_file, int device, int quiet, bool dry, std::string *summary) {
@@ -1002,6 +1203,7 @@ int run_deck(const char *input_file, int device, int quiet, bool dry, std::strin
         else if (e == "&beam") ok = do_beam(dk, nl, err);
         else if (e == "&efield") ok = do_efield(dk, nl, err);
         else if (e == "&track") ok = do_track(dk, nl, device, quiet != 0, dry, err);
+        else if (e == "&write") ok = do_write(dk, nl, err);
         else {
             err = "element " + e + " is not supported by this build";
             ok = false;

@@ -318,3 +318,154 @@ def test_time_dependent_deck_outputs(tmp_path):
     ep = float(np.max(np.abs(res["fp32"].sum(axis=1) / res["fp64"].sum(axis=1) - 1)))
     print(f"  sum P(z) mixed vs FP64 form: max rel {ep:.2e} over {len(res['fp64'])} records", flush=True)
     assert ep < 1e-2
+
+
+DUMP_IN = """&setup
+rootname={root}
+lattice={lat}
+beamline=FEL
+lambda0=1e-10
+gamma0=11357.82
+delz=0.045000
+npart=256
+nbins=4
+shotnoise=0
+&end
+
+&time
+slen=4e-10
+&end
+
+&field
+power=5e3
+dgrid=2.000000e-04
+ngrid=63
+waist_size=30e-6
+&end
+
+&beam
+current=3000
+delgam=1.000000
+ex=4.000000e-07
+ey=4.000000e-07
+betax=8.0
+betay=16.0
+&end
+
+&write
+field = @.init
+beam = @.init
+stride = 2
+&end
+
+&track
+zstop=1
+&end
+"""
+
+
+def test_write_dumps_generated_state(tmp_path):
+    """&write (src/Main/Dump.cpp:19-62) in check mode: the .fld.h5 and
+    .par.h5 files of the generated field and beam.  The field's scaling to
+    sqrt(W) per cell is pinned by the Gaussian's power: the dumped
+    sum of |E|^2 over the grid is the &field power (the 30 um waist is
+    resolved by 3.2 um cells, so the sampled Gaussian sums to the integral
+    to far below 1e-9).  The layout and the particle arrays are compared
+    with the oracle's loaders on the same parameters, bit for bit."""
+    (tmp_path / "Example1.lat").write_text(EX1_LAT.format(extra=""))
+    path = tmp_path / "dump.in"
+    root = str(tmp_path / "dmp")
+    path.write_text(DUMP_IN.format(root=root, lat=str(tmp_path / "Example1.lat")))
+    g4mi.check_deck(path)
+    fld, par = root + ".init.fld.h5", root + ".init.par.h5"
+    assert os.path.exists(fld) and os.path.exists(par)
+    ns, ng = 4, 63
+    assert g4mi.h5_read(fld, "slicecount")[0] == ns
+    assert g4mi.h5_read(fld, "gridpoints")[0] == ng
+    dgrid = 4e-4 / (ng - 1)
+    assert g4mi.h5_read(fld, "gridsize")[0] == pytest.approx(dgrid, rel=1e-15)
+    assert g4mi.h5_read(fld, "wavelength")[0] == 1e-10
+    assert g4mi.h5_read(fld, "slicespacing")[0] == 1e-10
+    # the oracle's Gaussian on the same grid
+    sim = O.Sim([256] * ns, ng, 2e-4, 1e-10, timerun=True)
+    O.load_gauss_all(sim, 5e3, 30e-6, 1e-10)
+    ks = 2 * math.pi / 1e-10
+    scl = dgrid * 510998.95069 / ks / math.sqrt(376.73)
+    ixy = np.zeros(ng * ng)
+    for i in range(ns):
+        re = g4mi.h5_read(fld, f"slice{i + 1:06d}/field-real")
+        im = g4mi.h5_read(fld, f"slice{i + 1:06d}/field-imag")
+        assert abs(np.sum(re * re + im * im) / 5e3 - 1) < 1e-9
+        assert np.array_equal(re, scl * sim.field[i][:, 0]) and np.array_equal(im, scl * sim.field[i][:, 1])
+        ixy += sim.field[i][:, 0] ** 2 + sim.field[i][:, 1] ** 2
+    assert np.allclose(g4mi.h5_read(fld, "int_xy"), ixy, rtol=1e-14, atol=0)
+    ixz = g4mi.h5_read(fld, "int_xz")
+    iyz = g4mi.h5_read(fld, "int_yz")
+    assert ixz.shape == (ng, ns) and iyz.shape == (ng, ns)
+    w = (sim.field[..., 0] ** 2 + sim.field[..., 1] ** 2).reshape(ns, ng, ng)  # [slice][iy][ix]
+    assert np.allclose(ixz, w.sum(axis=1).T, rtol=1e-13, atol=0)
+    assert np.allclose(iyz, w.sum(axis=2).T, rtol=1e-13, atol=0)
+    # beam: every second particle of each slice
+    assert g4mi.h5_read(par, "slicecount")[0] == ns
+    assert g4mi.h5_read(par, "beamletsize")[0] == 4
+    assert g4mi.h5_read(par, "slicelength")[0] == 1e-10
+    O.load_beam(sim, dict(gamma=11357.82, delgam=1.0, current=3000.0, ex=4e-7, ey=4e-7, betax=8.0,
+                          betay=16.0), None, 4, None, lambda0=1e-10)
+    for i in range(ns):
+        g = f"slice{i + 1:06d}/"
+        assert g4mi.h5_read(par, g + "current")[0] == 3000.0
+        a, b = int(sim.off[i]), int(sim.off[i + 1])
+        for k in ("gamma", "theta", "x", "y", "px", "py"):
+            v = g4mi.h5_read(par, g + k)
+            assert v.shape == (128,), k
+            assert np.array_equal(v, getattr(sim, k)[a:b:2]), (i, k)
+    # the .out.h5 of the check lists no dumps
+    assert g4mi.h5_read(root + ".out.h5", "Meta/Fielddumps/ndumps")[0] == 0
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_track_dumps_match_records(tmp_path, prec):
+    """Dumps during &track (Control::applyMarker, src/Core/Control.cpp:23-90):
+    field_dump_step, beam_dump_step and field_dump_at_undexit mark positions
+    0..nsteps (Undulator::updateMarker/markUndExits), the dump at position p
+    holds the state before step p, and the file is <root>.<p>.fld.h5 /
+    .par.h5.  With a record after every step, the dumped field's sum of
+    |E|^2 per slice is that record's /Field/power and the dumped beam's mean
+    gamma per slice its /Beam/energy; the dump list in /Meta holds the
+    positions and file names."""
+    print(f"deck runner: dumps during &track ({prec})", flush=True)
+    (tmp_path / "Example1.lat").write_text(EX1_LAT.format(extra=""))
+    path = tmp_path / "dumps.in"
+    root = str(tmp_path / "dd")
+    extra = (f"precision = {prec}\nfield_dump_step = 40\nbeam_dump_step = 60\nfield_dump_at_undexit = true\n")
+    deck = TD_IN.replace("zstop=20\noutput_step=10", "zstop=6\noutput_step=1")
+    path.write_text(deck.format(root=root, lat=str(tmp_path / "Example1.lat"), extra=extra))
+    s = g4mi.check_deck(path)
+    g4mi.run_deck(path, device=0, quiet=True)
+    out = root + ".out.h5"
+    nrun = s["run_steps"]
+    aw = g4mi.h5_read(out, "Lattice/aw")
+    exits = [i for i in range(1, len(aw)) if aw[i - 1] != 0 and aw[i] == 0]
+    fsteps = sorted({p for p in range(0, nrun + 1, 40)} | {p for p in exits if p <= nrun})
+    bsteps = list(range(0, nrun + 1, 60))
+    assert len(exits) >= 1 and exits[0] <= nrun
+    assert g4mi.h5_read(out, "Meta/Fielddumps/ndumps")[0] == len(fsteps)
+    assert list(g4mi.h5_read(out, "Meta/Fielddumps/intstep")) == fsteps
+    assert list(g4mi.h5_read(out, "Meta/Beamdumps/intstep")) == bsteps
+    power = g4mi.h5_read(out, "Field/power")
+    energy = g4mi.h5_read(out, "Beam/energy")
+    assert power.shape == (nrun + 1, 16)
+    tol = 1e-12 if prec == "fp64" else 1e-5
+    for p in fsteps:
+        fld = f"{root}.{p}.fld.h5"
+        dp = np.array([np.sum(g4mi.h5_read(fld, f"slice{i + 1:06d}/field-real") ** 2 +
+                              g4mi.h5_read(fld, f"slice{i + 1:06d}/field-imag") ** 2) for i in range(16)])
+        assert np.allclose(dp, power[p], rtol=tol, atol=0), p
+    for p in bsteps:
+        par = f"{root}.{p}.par.h5"
+        mg = np.array([np.mean(g4mi.h5_read(par, f"slice{i + 1:06d}/gamma")) for i in range(16)])
+        assert np.allclose(mg, energy[p], rtol=1e-13 if prec == "fp64" else 1e-9, atol=0), p
+        assert g4mi.h5_read(par, "slice000001/gamma").shape == (1024,)
+    print(f"  {len(fsteps)} field dumps, {len(bsteps)} beam dumps over {nrun} steps agree with the records",
+          flush=True)
