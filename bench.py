@@ -58,12 +58,14 @@ def make_context(cfg, off, beam, field, current, precision):
 
 
 def gpu_measure(cfg, steps_k, warmup, adi_kernel=-1, precision=g4mi.FP64, breakdown=20, refine=0,
-                mixed_adi=1):
+                mixed_adi=1, fft=False):
     t_load = time.perf_counter()
     off, beam, field, current, steps = build_workload(cfg)
     ctx = make_context(cfg, off, beam, field, current, precision)
     t_load = time.perf_counter() - t_load
     del beam, field
+    if fft:
+        ctx.set_field_solver(fft=True)
     if adi_kernel >= 0 and precision == g4mi.FP64:
         ctx.set_schedule("adi_kernel", adi_kernel)
     if precision == g4mi.FP32:
@@ -134,6 +136,9 @@ def measured_traffic(workload, kernel):
     return None
 
 
+FP64_MFMA_PEAK_TFLOPS = 78.6
+
+
 def roofline(cfg, kt, ntot, precision=g4mi.FP64, schedule=None):
     """Dominant kernel and its algorithmic bytes per launch.
     FP64 plain form:
@@ -147,6 +152,17 @@ def roofline(cfg, kt, ntot, precision=g4mi.FP64, schedule=None):
       ADI: field read + write = 16 B/cell;
       front (transverse + moments): read 5 + write 4 floats = 36 B/particle."""
     cells = cfg["nslice"] * cfg["ngrid"] ** 2
+    top = max((k for k in kt if k != "steps"), key=kt.get)
+    if schedule and schedule.get("field_solver") == "fft" and top == "adi":
+        # spectral step: two complex n x n x n products per slice (E M^T, then
+        # M T), 8 real flops per complex multiply-add, on FP64 MFMA
+        flops = 16.0 * cfg["ngrid"] ** 3 * cfg["nslice"]
+        ms = kt["adi"]
+        tf = flops / (ms * 1e-3) / 1e12
+        return "adi", dict(bound="mfma", achieved=round(tf, 2), peak=FP64_MFMA_PEAK_TFLOPS, unit="TFLOP/s",
+                           frac=round(tf / FP64_MFMA_PEAK_TFLOPS, 4), traffic=None, kernel="field (fft)",
+                           device_kernel="k_cgemm", ms_per_launch=round(ms, 5),
+                           algorithmic_flops=int(flops))
     if precision == g4mi.FP64:
         per = {"push": 64.0 * ntot, "adi": 32.0 * cells, "deposit": 32.0 * ntot,
                "transverse": 144.0 * ntot}
@@ -396,6 +412,8 @@ def main():
     ap.add_argument("--adi-kernel", type=int, default=-1,
                     help="fix the ADI kernel (0 Thomas, 1 fused, 2 fused+prefetch, 3 banded); "
                          "default: autotuned")
+    ap.add_argument("--field-solver", choices=("adi", "fft"), default="adi",
+                    help="FP64 form: the ADI (default) or the spectral field solver")
     ap.add_argument("--adi-refine", type=int, default=0, choices=(0, 1),
                     help="mixed form: FP64 residual correction in the ADI (default off)")
     ap.add_argument("--mixed-adi", type=int, default=1, choices=(0, 1),
@@ -429,7 +447,7 @@ def main():
         cfg["precision"] = a.precision
     prec = g4mi.FP32 if cfg["precision"] == "fp32" else g4mi.FP64
     m = gpu_measure(cfg, a.steps, a.warmup, a.adi_kernel, prec, refine=a.adi_refine,
-                    mixed_adi=a.mixed_adi)
+                    mixed_adi=a.mixed_adi, fft=a.field_solver == "fft")
     rate = m["ntot"] * a.steps / m["seconds"]
     cells = cfg["nslice"] * cfg["ngrid"] ** 2
     kname, roof = roofline(cfg, m["kt"], m["ntot"], prec, m["schedule"])

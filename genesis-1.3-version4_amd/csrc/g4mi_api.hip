@@ -17,6 +17,7 @@
 
 #include "g4mi_host.h"
 #include "g4mi_kernels.h"
+#include "g4mi_fft.h"
 #include "g4mi_fno_api.h"
 #include "g4mi_mixed.h"
 
@@ -73,6 +74,15 @@ struct Ctx {
     // prefetch, 3 banded inverse
     int adi_kernel = -1;
     float adi_ms[4] = {-1.f, -1.f, -1.f, -1.f};
+    // spectral solver (FieldSolverFFT) instead of the ADI, with the source
+    // filter's settings; the propagator matrix is cached per delz
+    int fft = 0, fft_filter = 0;
+    double fft_xc = 1, fft_yc = 1, fft_sig = 1;
+    bool fft_ready = false;
+    double fft_delz = 0;
+    double2 *d_fft_m = nullptr, *d_fft_w = nullptr, *d_fft_wc = nullptr, *d_fft_u = nullptr,
+            *d_fft_v = nullptr;
+    double *d_fft_sig = nullptr, *d_fft_mom = nullptr;
     // timing: kEv events per step, recorded into a pool and folded lazily
     static constexpr int kEv = 9;
     static constexpr int kPoolSteps = 512;
@@ -385,12 +395,72 @@ static int tune_adi(Ctx *c, const AdiFactors &f) {
     return G4MI_OK;
 }
 
+// FieldSolverFFT::FFT for every slice (src/Core/FieldSolverFFT.cpp:62-119)
+// after the deposit: E' = M E M^T + 2 S, or with the source filter
+// E' = M E M^T + 2 IDFT(sigmoid DFT(S)) / n^2, as complex FP64 GEMMs.
+static int fft_field_step(Ctx *c, double delz) {
+    const int ng = c->ng;
+    const size_t nn = (size_t)ng * ng;
+    if (!c->d_fft_m) G4_TRY(dalloc(c, &c->d_fft_m, nn));
+    if (!c->fft_ready || delz != c->fft_delz) {  // FieldSolverFFT::init keeps delz per call
+        std::vector<double> m(2 * nn);
+        fft_propagator(ng, c->dgrid, c->xks, delz, m.data());
+        G4_HIP(hipMemcpyAsync(c->d_fft_m, m.data(), m.size() * sizeof(double), hipMemcpyHostToDevice,
+                              c->stream));
+        G4_HIP(hipStreamSynchronize(c->stream));
+        c->fft_delz = delz;
+        c->fft_ready = true;
+    }
+    CgemmArgs g{};
+    g.ng = ng;
+    g.ns = c->ns;
+    g.nrows = (long long)c->ns * ng;
+    g.box = c->d_box;
+    g.scale = 1.0;
+    if (c->fft_filter) {
+        g.a = c->d_src;  // DFT of the source planes: U = S W^T, V = sigmoid (W U)
+        g.mask_a = 1;
+        g.mat = c->d_fft_w;
+        g.c = c->d_fft_u;
+        G4_TRY(launch_cgemm_rows(c->stream, g));
+        g.mask_a = 0;
+        g.b = c->d_fft_u;
+        g.ptw = c->d_fft_sig;
+        g.c = c->d_fft_v;
+        G4_TRY(launch_cgemm_cols(c->stream, g));
+        g.ptw = nullptr;  // back transform: Y = W* (V W*^T) / n^2
+        g.a = c->d_fft_v;
+        g.mat = c->d_fft_wc;
+        g.c = c->d_fft_u;
+        G4_TRY(launch_cgemm_rows(c->stream, g));
+        g.b = c->d_fft_u;
+        g.c = c->d_fft_v;
+        g.scale = 1.0 / (double)nn;
+        G4_TRY(launch_cgemm_cols(c->stream, g));
+        g.scale = 1.0;
+    }
+    g.a = c->d_field;  // T = E M^T
+    g.mat = c->d_fft_m;
+    g.c = c->d_work;
+    G4_TRY(launch_cgemm_rows(c->stream, g));
+    g.b = c->d_work;  // E = M T + 2 (S or Y) of beam slice (s - first) mod Ns
+    g.c = c->d_field;
+    g.ring = 1;
+    g.first = c->first;
+    if (c->fft_filter)
+        g.add = c->d_fft_v;
+    else
+        g.src = c->d_src;
+    G4_TRY(launch_cgemm_cols(c->stream, g));
+    return G4MI_OK;
+}
+
 static int field_track(Ctx *c, const g4mi_step_params &st, double delz) {
     if (!c->has_field) {
         set_error("field not uploaded");
         return G4MI_E_STATE;
     }
-    G4_TRY(ensure_factors(c, delz));
+    if (!c->fft) G4_TRY(ensure_factors(c, delz));
     if (c->ev) G4_HIP(hipEventRecord(c->ev[5], c->stream));
     DepositParams d{};
     d.enabled = st.aw > 0 ? 1 : 0;  // in undulator, fundamental
@@ -409,6 +479,13 @@ static int field_track(Ctx *c, const g4mi_step_params &st, double delz) {
     launch_deposit(c->stream, c->ns, c->d_off, c->d_g, c->d_th, c->d_x, c->d_y, c->d_cur,
                    c->d_src, c->d_box, d);
     if (c->ev) G4_HIP(hipEventRecord(c->ev[6], c->stream));
+    if (c->fft) {
+        G4_TRY(fft_field_step(c, delz));
+        c->field_diag_fresh = false;
+        c->zeroed_since_adi.clear();
+        if (c->ev) G4_HIP(hipEventRecord(c->ev[7], c->stream));
+        return G4MI_OK;
+    }
     AdiFactors f{c->d_fac, c->d_fac + c->ng, c->d_fac + 2 * c->ng, c->cstep, c->ng};
     if (c->adi_kernel < 0) G4_TRY(tune_adi(c, f));
     adi_launch(c, c->adi_kernel, f, c->d_field, c->d_work);
@@ -586,6 +663,82 @@ static int fetch_aux(Ctx *c, std::vector<double> &aux, std::vector<double> &esc,
     return G4MI_OK;
 }
 
+// DFT matrices W, W* and two field-sized work buffers of the spectral
+// solver's filter and the spectral diagnostics, allocated on first use.
+static int ensure_dft(Ctx *c) {
+    if (c->d_fft_w) return G4MI_OK;
+    const int ng = c->ng;
+    const size_t nn = (size_t)ng * ng, cells = (size_t)c->ns * nn;
+    G4_TRY(dalloc(c, &c->d_fft_w, nn));
+    G4_TRY(dalloc(c, &c->d_fft_wc, nn));
+    G4_TRY(dalloc(c, &c->d_fft_u, cells));
+    G4_TRY(dalloc(c, &c->d_fft_v, cells));
+    G4_TRY(dalloc(c, &c->d_fft_mom, 5 * (size_t)c->ns));
+    std::vector<double> w(2 * nn), wc(2 * nn);
+    dft_matrix(ng, -1, w.data());
+    dft_matrix(ng, +1, wc.data());
+    G4_HIP(hipMemcpy(c->d_fft_w, w.data(), w.size() * sizeof(double), hipMemcpyHostToDevice));
+    G4_HIP(hipMemcpy(c->d_fft_wc, wc.data(), wc.size() * sizeof(double), hipMemcpyHostToDevice));
+    return G4MI_OK;
+}
+
+static bool wants_spectral(const g4mi_diag_out *o) {
+    return o->xdivergence || o->ydivergence || o->xpointing || o->ypointing || o->g_xdivergence ||
+           o->g_ydivergence || o->g_xpointing || o->g_ypointing;
+}
+
+// DiagField's spectral block (src/Core/Diagnostic.cpp:789-807, 839-846 and
+// the global values :952-963): F = W E W^T per slice as two GEMMs, moments
+// of |F|^2 per slice, normalised on the host.  `wide` is the FP64 field
+// ([ns][ng][ng], storage order), `first` the ring offset.
+static int spectral_diag(Ctx *c, const double2 *wide, int first, g4mi_diag_out *o) {
+    G4_TRY(ensure_dft(c));
+    const int ns = c->ns, ng = c->ng;
+    CgemmArgs g{};
+    g.ng = ng;
+    g.ns = ns;
+    g.nrows = (long long)ns * ng;
+    g.scale = 1.0;
+    // the products cannot run in place: the intermediate is the work buffer
+    // that does not hold the input, the result goes to the other one
+    double2 *inter = wide == c->d_fft_u ? c->d_fft_v : c->d_fft_u;
+    double2 *res = wide == c->d_fft_u ? c->d_fft_u : c->d_fft_v;
+    g.a = wide;
+    g.mat = c->d_fft_w;
+    g.c = inter;
+    G4_TRY(launch_cgemm_rows(c->stream, g));
+    g.b = inter;
+    g.c = res;
+    G4_TRY(launch_cgemm_cols(c->stream, g));
+    launch_fft_moments(c->stream, ns, res, ng, c->d_fft_mom);
+    std::vector<double> m(5 * (size_t)ns);
+    G4_HIP(hipMemcpyAsync(m.data(), c->d_fft_mom, m.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    G4_HIP(hipStreamSynchronize(c->stream));
+    const double scltheta = c->xlambda / ng / c->dgrid;
+    double f[5] = {0, 0, 0, 0, 0};
+    for (int k = 0; k < ns; k++) {  // storage slice k is output slice (ns + k - first) mod ns
+        const int is = (ns + k - first) % ns;
+        double v[5];
+        for (int q = 0; q < 5; q++) {
+            v[q] = m[5 * (size_t)k + q];
+            f[q] += v[q];
+        }
+        if (v[0] > 0)
+            for (int q = 1; q < 5; q++) v[q] /= v[0];
+        if (o->xdivergence) o->xdivergence[is] = sqrt(fabs(v[2] - v[1] * v[1])) * scltheta;
+        if (o->ydivergence) o->ydivergence[is] = sqrt(fabs(v[4] - v[3] * v[3])) * scltheta;
+        if (o->xpointing) o->xpointing[is] = v[1] * scltheta;
+        if (o->ypointing) o->ypointing[is] = v[3] * scltheta;
+    }
+    const double norm = f[0] > 0 ? 1. / f[0] : 1.;
+    for (int q = 1; q < 5; q++) f[q] *= norm;
+    if (o->g_xdivergence) o->g_xdivergence[0] = sqrt(fabs(f[2] - f[1] * f[1])) * scltheta;
+    if (o->g_ydivergence) o->g_ydivergence[0] = sqrt(fabs(f[4] - f[3] * f[3])) * scltheta;
+    if (o->g_xpointing) o->g_xpointing[0] = f[1] * scltheta;
+    if (o->g_ypointing) o->g_ypointing[0] = f[3] * scltheta;
+    return G4MI_OK;
+}
+
 static int diag(Ctx *c, g4mi_diag_out *o) {
     const int ns = c->ns, ng = c->ng;
     launch_diag_beam(c->stream, ns, c->d_off, c->d_g, c->d_th, c->d_x, c->d_y, c->d_diag_b);
@@ -604,6 +757,7 @@ static int diag(Ctx *c, g4mi_diag_out *o) {
         G4_TRY(fetch_aux(c, aux, esc, ssc));
         finish_aux(c, aux, esc, ssc, o);
     }
+    if (wants_spectral(o)) G4_TRY(spectral_diag(c, c->d_field, c->first, o));
     return G4MI_OK;
 }
 
@@ -1108,6 +1262,11 @@ static int mix_diag(Ctx *c, g4mi_diag_out *o) {
         G4_TRY(fetch_aux(c, aux, esc, ssc));
         finish_aux(c, aux, esc, ssc, o);
     }
+    if (wants_spectral(o)) {  // on an FP64 copy of the FP32 field
+        G4_TRY(ensure_dft(c));
+        launch_widen_field(c->stream, c->m_field[hs.cur], c->d_fft_u, (long long)ns * c->ng * c->ng);
+        G4_TRY(spectral_diag(c, c->d_fft_u, hs.first, o));
+    }
     return G4MI_OK;
 }
 
@@ -1381,6 +1540,35 @@ int g4mi_set_efield(g4mi_ctx *h, int longrange, int nz, int nphi, int ngrid_r, d
     return G4MI_OK;
 }
 
+int g4mi_set_field_solver(g4mi_ctx *h, int fft, int filter, double xcut, double ycut, double sigmoid) {
+    if (!h) {
+        set_error("g4mi_set_field_solver: null context");
+        return G4MI_E_ARG;
+    }
+    Ctx *c = &h->c;
+    if (fft && c->precision != G4MI_FP64) {
+        set_error("g4mi_set_field_solver: the spectral solver runs in the FP64 form only");
+        return G4MI_E_ARG;
+    }
+    c->fft = fft ? 1 : 0;
+    // FieldSolverFFT::initSourceFilter (:159-170): no filter for sig, xc or yc <= 0
+    c->fft_filter = (fft && filter && sigmoid > 0 && xcut > 0 && ycut > 0) ? 1 : 0;
+    c->fft_xc = xcut;
+    c->fft_yc = ycut;
+    c->fft_sig = sigmoid;
+    c->fft_ready = false;
+    c->field_diag_fresh = false;
+    if (c->fft_filter) {
+        const size_t nn = (size_t)c->ng * c->ng;
+        G4_TRY(ensure_dft(c));
+        if (!c->d_fft_sig) G4_TRY(dalloc(c, &c->d_fft_sig, nn));
+        std::vector<double> sg(nn);
+        fft_sigmoid(c->ng, xcut, ycut, sigmoid, sg.data());
+        G4_HIP(hipMemcpy(c->d_fft_sig, sg.data(), sg.size() * sizeof(double), hipMemcpyHostToDevice));
+    }
+    return G4MI_OK;
+}
+
 int g4mi_beam_upload(g4mi_ctx *h, const double *g, const double *th, const double *x,
                      const double *y, const double *px, const double *py, const double *cur) {
     if (!h || !g || !th || !x || !y || !px || !py || !cur) {
@@ -1556,9 +1744,16 @@ int g4mi_get_schedule(g4mi_ctx *h, char *buf, int len) {
         snprintf(buf, (size_t)len, "%s", tmp);
         return G4MI_OK;
     }
+    if (c->fft) {
+        snprintf(tmp, sizeof(tmp),
+                 "{\"field_solver\": \"fft\", \"source_filter\": %d, \"gemm_tile\": %d}",
+                 c->fft_filter, cgemm_tile(c->ng));
+        snprintf(buf, (size_t)len, "%s", tmp);
+        return G4MI_OK;
+    }
     snprintf(tmp, sizeof(tmp),
-             "{\"adi_kernel\": %d, \"adi\": \"%s\", \"adi_window\": %d, \"adi_band_m\": %d, "
-             "\"adi_tuned_ms\": [%.4f, %.4f, %.4f, %.4f]}",
+             "{\"field_solver\": \"adi\", \"adi_kernel\": %d, \"adi\": \"%s\", \"adi_window\": %d, "
+             "\"adi_band_m\": %d, \"adi_tuned_ms\": [%.4f, %.4f, %.4f, %.4f]}",
              c->adi_kernel, c->adi_kernel >= 0 ? names[c->adi_kernel] : "untuned", c->adi_tile,
              c->adi_band_m, c->adi_ms[0], c->adi_ms[1], c->adi_ms[2], c->adi_ms[3]);
     snprintf(buf, (size_t)len, "%s", tmp);

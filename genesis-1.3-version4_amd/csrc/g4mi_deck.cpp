@@ -902,8 +902,20 @@ bool write_output(const Deck &dk, const std::string &fname, const std::vector<g4
         put("Field/intensity-farfield", "W/rad^2", "ff_intensity");
         put("Field/phase-farfield", "rad", "ff_phase");
     }
+    if (!st.excl_fft) {
+        put("Field/xdivergence", "rad", "xdivergence");
+        put("Field/ydivergence", "rad", "ydivergence");
+        put("Field/xpointing", "rad", "xpointing");
+        put("Field/ypointing", "rad", "ypointing");
+    }
     if (st.field_global_stat && ns > 1) {
         put1("Field/Global/energy", "J", "global_energy");
+        if (!st.excl_fft) {
+            put1("Field/Global/xdivergence", "rad", "g_xdivergence");
+            put1("Field/Global/ydivergence", "rad", "g_ydivergence");
+            put1("Field/Global/xpointing", "rad", "g_xpointing");
+            put1("Field/Global/ypointing", "rad", "g_ypointing");
+        }
         if (!st.excl_spatial) {
             put1("Field/Global/xposition", "m", "g_f_xposition");
             put1("Field/Global/xsize", "m", "g_f_xsize");
@@ -928,11 +940,13 @@ const char *kRecKeys[] = {"power", "bunching", "bunchingphase", "energy", "energ
                           "yposition", "xsize", "ysize", "f_xposition", "f_yposition", "f_xsize",
                           "f_ysize", "nf_intensity", "ff_intensity", "pxposition", "pyposition",
                           "nf_phase", "ff_phase", "efield", "wakefield", "lscfield", "sscfield", "xmin",
-                          "xmax", "pxmin", "pxmax", "ymin", "ymax", "pymin", "pymax", "emin", "emax"};
+                          "xmax", "pxmin", "pxmax", "ymin", "ymax", "pymin", "pymax", "emin", "emax",
+                          "xdivergence", "ydivergence", "xpointing", "ypointing"};
 // one value per record
 const char *kRecScalars[] = {"global_energy", "g_energy", "g_energyspread", "g_xposition", "g_xsize",
                              "g_yposition", "g_ysize", "g_f_xposition", "g_f_xsize", "g_f_yposition",
-                             "g_f_ysize", "g_nf_intensity", "g_ff_intensity"};
+                             "g_f_ysize", "g_nf_intensity", "g_ff_intensity", "g_xdivergence",
+                             "g_ydivergence", "g_xpointing", "g_ypointing"};
 
 bool do_track(Deck &dk, Namelist &nl, int device, bool quiet, bool dry, std::string &err) {
     Keys k{nl, ""};
@@ -965,8 +979,8 @@ bool do_track(Deck &dk, Namelist &nl, int device, bool quiet, bool dry, std::str
         err = k.err;
         return false;
     }
-    if (fft) {
-        err = "&track: fft_fieldsolver is not supported by this build (ADI solver only)";
+    if (fft && prec == "fp32") {
+        err = "&track: fft_fieldsolver runs in the FP64 form only (precision = fp64)";
         return false;
     }
     if (bunchharm > 1) {
@@ -1067,6 +1081,11 @@ bool do_track(Deck &dk, Namelist &nl, int device, bool quiet, bool dry, std::str
             return false;
         }
     }
+    // Track::init -> Field::initSolver per &track (src/Main/Track.cpp:107-112)
+    if (g4mi_set_field_solver(dk.ctx, fft ? 1 : 0, filt ? 1 : 0, xc, yc, sig) != G4MI_OK) {
+        err = g4mi_last_error();
+        return false;
+    }
     for (const char *key : kRecKeys) rec[key].assign(nrec * (size_t)ns, 0.0);
     for (const char *key : kRecScalars) rec[key].assign(nrec, 0.0);
     auto record = [&](size_t r) {
@@ -1105,6 +1124,16 @@ bool do_track(Deck &dk, Namelist &nl, int device, bool quiet, bool dry, std::str
         d.g_f_ysize = p1("g_f_ysize");
         d.g_nf_intensity = p1("g_nf_intensity");
         d.g_ff_intensity = p1("g_ff_intensity");
+        if (!st.excl_fft) {  // DiagField's spectral block (filter "fft")
+            d.xdivergence = ps("xdivergence");
+            d.ydivergence = ps("ydivergence");
+            d.xpointing = ps("xpointing");
+            d.ypointing = ps("ypointing");
+            d.g_xdivergence = p1("g_xdivergence");
+            d.g_ydivergence = p1("g_ydivergence");
+            d.g_xpointing = p1("g_xpointing");
+            d.g_ypointing = p1("g_ypointing");
+        }
         d.power = rec["power"].data() + r * ns;
         d.bunching = rec["bunching"].data() + r * ns;
         d.bunchingphase = rec["bunchingphase"].data() + r * ns;

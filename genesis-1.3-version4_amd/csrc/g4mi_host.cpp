@@ -190,6 +190,64 @@ double adi_band(int ngrid, const double *c, const double *cbet, const double *cw
 }
 
 namespace {
+// FFT index k of grid point i: (i + (n+1)/2) mod n (FieldSolverFFT.cpp:137-138),
+// inverted; returns the point's offset dx = i - (n-1)/2 from the centre.
+double fft_offset(int n, int k) {
+    const int i = ((k - (n + 1) / 2) % n + n) % n;
+    return (double)i - 0.5 * (double)(n - 1);
+}
+}  // namespace
+
+void fft_propagator(int n, double dgrid, double xks, double delz, double *m) {
+    typedef std::complex<long double> cl;
+    const double dk = 4. * asin(1.) / ((double)n * dgrid);
+    std::vector<cl> p(n), md(n);
+    for (int k = 0; k < n; k++) {
+        const double dx = fft_offset(n, k);
+        const double ph = -(dx * dx) * dk * dk / 2. / xks * delz;
+        p[k] = cl(cos(ph), sin(ph));
+    }
+    const long double twopi = 8.0L * atanl(1.0L);
+    for (int d = 0; d < n; d++) {
+        cl acc(0, 0);
+        for (int k = 0; k < n; k++) {
+            const long double a = twopi * (long double)(((long long)k * d) % n) / (long double)n;
+            acc += p[k] * cl(cosl(a), sinl(a));
+        }
+        md[d] = acc / (long double)n;
+    }
+    for (int j = 0; j < n; j++)
+        for (int l = 0; l < n; l++) {
+            const cl v = md[((j - l) % n + n) % n];
+            m[2 * ((size_t)j * n + l)] = (double)v.real();
+            m[2 * ((size_t)j * n + l) + 1] = (double)v.imag();
+        }
+}
+
+void dft_matrix(int n, int sign, double *w) {
+    const long double twopi = 8.0L * atanl(1.0L);
+    for (int j = 0; j < n; j++)
+        for (int l = 0; l < n; l++) {
+            const long double a = twopi * (long double)(((long long)j * l) % n) / (long double)n;
+            w[2 * ((size_t)j * n + l)] = (double)cosl(a);
+            w[2 * ((size_t)j * n + l) + 1] = (double)(sign * sinl(a));
+        }
+}
+
+void fft_sigmoid(int n, double xc, double yc, double sig, double *s) {
+    const double shift = -0.5 * (double)(n - 1);
+    for (int iy = 0; iy < n; iy++) {
+        const double y = ((double)iy + shift) / (double)n / yc;
+        for (int ix = 0; ix < n; ix++) {
+            const double x = ((double)ix + shift) / (double)n / xc;
+            const int ii = ((iy + (n + 1) / 2) % n) * n + (ix + (n + 1) / 2) % n;
+            const double r = (sqrt(x * x + y * y) - 1) / sig;
+            s[ii] = 1. / (1 + exp(r));
+        }
+    }
+}
+
+namespace {
 // t = e * m (left-multiplication), as TrackBeam::matmul.
 void left_mul(double m[16], const double e[16]) {
     double t[16];

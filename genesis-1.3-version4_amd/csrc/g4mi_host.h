@@ -34,6 +34,19 @@ double adi_factors(int ngrid, double delz, double dgrid, double xks, double *c, 
 double adi_band(int ngrid, const double *c, const double *cbet, const double *cwet, int M,
                 double *band);
 
+// Spectral solver matrices (FieldSolverFFT, src/Core/FieldSolverFFT.cpp),
+// interleaved complex, row-major n x n.
+// The step's propagator: M[j][l] = m[(j - l) mod n] with
+// m[d] = (1/n) sum_k p_k exp(2 pi i k d / n) and p_k = exp(i K2_1(k) delz),
+// K2_1(k) = -dx(k)^2 dk^2 / (2 xks) the one-dimensional part of K2 (:139)
+// at FFT index k.  exp(K2 delz) = p_ky p_kx, so the 2-D step
+// IDFT(exp(K2 delz) DFT(E)) / n^2 equals M E M^T.
+void fft_propagator(int n, double dgrid, double xks, double delz, double *m);
+// W[j][l] = exp(sign 2 pi i j l / n), the exponent reduced mod n exactly.
+void dft_matrix(int n, int sign, double *w);
+// The source filter's sigmoid at FFT index [ky][kx] (:141-147), n*n reals.
+void fft_sigmoid(int n, double xc, double yc, double sig, double *s);
+
 // Chicane transfer matrix of TrackBeam::applyChicane (:131-221), row-major 4x4.
 void chicane_matrix(double angle, double lb, double ld, double lt, double m[16]);
 // TrackBeam::applyR56 factor (:260-269), already scaled to theta units.

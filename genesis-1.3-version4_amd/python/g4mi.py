@@ -49,11 +49,18 @@ DIAG_FIELDS = ("power", "bunching", "bunchingphase", "energy", "energyspread", "
                "pymin", "pymax", "emin", "emax",
                "g_energy", "g_energyspread", "g_xposition", "g_xsize", "g_yposition", "g_ysize",
                "g_f_xposition", "g_f_xsize", "g_f_yposition", "g_f_ysize", "g_nf_intensity",
-               "g_ff_intensity")
+               "g_ff_intensity",
+               "xdivergence", "ydivergence", "xpointing", "ypointing",
+               "g_xdivergence", "g_ydivergence", "g_xpointing", "g_ypointing")
 # outputs of one value per record rather than one per slice
 DIAG_SCALARS = ("global_energy", "g_energy", "g_energyspread", "g_xposition", "g_xsize",
                 "g_yposition", "g_ysize", "g_f_xposition", "g_f_xsize", "g_f_yposition",
-                "g_f_ysize", "g_nf_intensity", "g_ff_intensity")
+                "g_f_ysize", "g_nf_intensity", "g_ff_intensity",
+                "g_xdivergence", "g_ydivergence", "g_xpointing", "g_ypointing")
+
+
+SPECTRAL_FIELDS = ("xdivergence", "ydivergence", "xpointing", "ypointing",
+                   "g_xdivergence", "g_ydivergence", "g_xpointing", "g_ypointing")
 
 
 class DiagOut(C.Structure):
@@ -78,6 +85,8 @@ def lib():
                                   C.c_double, C.c_double, C.c_double, C.c_int, C.c_int, C.c_int]
         L.g4mi_destroy.argtypes = [C.c_void_p]
         L.g4mi_set_efield.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_double]
+        L.g4mi_set_field_solver.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_double, C.c_double,
+                                            C.c_double]
         L.g4mi_beam_upload.argtypes = [C.c_void_p] + [_dp] * 7
         L.g4mi_beam_download.argtypes = [C.c_void_p] + [_dp] * 6
         L.g4mi_field_upload.argtypes = [C.c_void_p, _dp, C.c_int]
@@ -137,7 +146,7 @@ def _chk(rc, what):
 
 
 def _p(a):
-    return a.ctypes.data_as(_dp)
+    return a.ctypes.data_as(_dp) if a is not None else None
 
 
 # ---------------------------------------------------------------- input side
@@ -206,6 +215,13 @@ class Context:
         _chk(lib().g4mi_set_efield(self.h, int(longrange), nz, nphi, ngrid_r, rmax),
              "g4mi_set_efield")
 
+    def set_field_solver(self, fft=False, source_filter=False, xcut=1.0, ycut=1.0, sigmoid=1.0):
+        """Field::initSolver (src/Core/Field.cpp:72-83): the ADI (fft=False)
+        or the spectral solver, with FieldSolverFFT::initSourceFilter's
+        settings."""
+        _chk(lib().g4mi_set_field_solver(self.h, int(fft), int(source_filter), xcut, ycut, sigmoid),
+             "g4mi_set_field_solver")
+
     def upload_beam(self, beam, current):
         cur = np.ascontiguousarray(current, dtype=np.float64)
         a = [np.ascontiguousarray(beam[k], dtype=np.float64)
@@ -245,12 +261,16 @@ class Context:
     def sync(self):
         _chk(lib().g4mi_sync(self.h), "g4mi_sync")
 
-    def diag(self):
-        out = {k: np.zeros(1 if k in DIAG_SCALARS else self.ns) for k in DIAG_FIELDS}
-        d = DiagOut(*[_p(out[k]) for k in DIAG_FIELDS])
+    def diag(self, spectral=False):
+        """One diagnostics record; spectral=True adds the divergence and
+        pointing block (two DFT products per slice)."""
+        keys = [k for k in DIAG_FIELDS if spectral or k not in SPECTRAL_FIELDS]
+        out = {k: np.zeros(1 if k in DIAG_SCALARS else self.ns) for k in keys}
+        d = DiagOut(*[_p(out.get(k)) for k in DIAG_FIELDS])
         _chk(lib().g4mi_diag(self.h, C.byref(d)), "g4mi_diag")
         for k in DIAG_SCALARS:
-            out[k] = out[k][0]
+            if k in out:
+                out[k] = out[k][0]
         return out
 
     def run(self, steps, gamma0, record=("power", "bunching", "energy")):

@@ -69,6 +69,12 @@ typedef struct {
     double *g_energy, *g_energyspread, *g_xposition, *g_xsize, *g_yposition, *g_ysize;
     double *g_f_xposition, *g_f_xsize, *g_f_yposition, *g_f_ysize;
     double *g_nf_intensity, *g_ff_intensity;
+    /* DiagField's spectral block (Diagnostic.cpp:789-807, 839-846): rms
+     * divergence and mean pointing angle per slice from |DFT(E)|^2, and
+     * their power-weighted global values (:952-963) ([1] each).  Computed
+     * only when one of these pointers is set (two DFT products per slice). */
+    double *xdivergence, *ydivergence, *xpointing, *ypointing;
+    double *g_xdivergence, *g_ydivergence, *g_xpointing, *g_ypointing;
 } g4mi_diag_out;
 
 /* Context on HIP device `device` holding a time window of `nslice` slices
@@ -106,8 +112,20 @@ int g4mi_beam_track(g4mi_ctx *ctx, const g4mi_step_params *st, double gammaref, 
 
 /* Field::track (src/Core/Field.cpp:177-185): FieldSolverADI::init
  * (src/Core/FieldSolverADI.cpp:125-182) when delz changes, then
- * FieldSolverADI::advance (:9-57): source deposition + ADI per slice. */
+ * FieldSolverADI::advance (:9-57): source deposition + ADI per slice; or,
+ * after g4mi_set_field_solver(fft = 1), FieldSolverFFT::init/advance
+ * (src/Core/FieldSolverFFT.cpp:9-60, 122-157): the same deposition, then
+ * every slice propagated by exp(K2 delz) in Fourier space plus twice the
+ * (optionally filtered) source. */
 int g4mi_field_track(g4mi_ctx *ctx, const g4mi_step_params *st, double delz);
+
+/* Field::initSolver (src/Core/Field.cpp:72-83) with
+ * FieldSolverFFT::initSourceFilter (src/Core/FieldSolverFFT.cpp:159-170):
+ * fft = 0 the ADI solver (default), 1 the spectral solver; filter = 1 with
+ * xcut, ycut, sigmoid > 0 filters the source by 1/(1 + exp((r - 1)/sigmoid)),
+ * r the radius in units of (xcut, ycut) of the normalised frequency.  The
+ * spectral solver runs in the FP64 form only (G4MI_E_ARG otherwise). */
+int g4mi_set_field_solver(g4mi_ctx *ctx, int fft, int filter, double xcut, double ycut, double sigmoid);
 
 /* Control::applySlippage (src/Core/Control.cpp:158-276) on one device:
  * ring-offset rotation and zeroing of the inflowing slice. */

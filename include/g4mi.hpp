@@ -65,6 +65,11 @@ class Context {
         return first;
     }
     void sync() { check(g4mi_sync(h_), "g4mi_sync"); }
+    /* Field::initSolver (src/Core/Field.cpp:72-83): the ADI or the spectral
+     * solver (FP64 form), with FieldSolverFFT::initSourceFilter's settings */
+    void initSolver(bool isFFT, bool filter = false, double xc = 1, double yc = 1, double sig = 1) {
+        check(g4mi_set_field_solver(h_, isFFT ? 1 : 0, filter ? 1 : 0, xc, yc, sig), "g4mi_set_field_solver");
+    }
 
     g4mi_ctx *handle() const { return h_; }
     int nslice() const { return ns_; }
@@ -80,7 +85,8 @@ class Context {
 };
 
 /* FieldSolver (include/FieldSolver.h:20-26) on the device: deposit + ADI
- * for every slice of the context, FieldSolverADI's arithmetic. */
+ * (FieldSolverADI) or the spectral step (FieldSolverFFT, after
+ * Context::initSolver(true)) for every slice of the context. */
 class FieldSolverMI {
   public:
     explicit FieldSolverMI(Context &ctx) : ctx_(ctx) {}

@@ -1102,8 +1102,109 @@ extern "C" void orc_adi_slice(int ng, const double *cd, const double *cbd, const
     }
 }
 
+/* 2-D discrete Fourier transform of one ngrid x ngrid slice in place, the
+ * transform FFTW's fftw_plan_dft_2d computes (FieldSolverFFT.cpp:155-156):
+ * X[ky][kx] = sum_{y,x} x[y][x] exp(sign 2 pi i (ky y + kx x)/n), unscaled,
+ * sign -1 forward and +1 backward.  Evaluated as row transforms followed by
+ * column transforms, each a direct sum with the exponent reduced to
+ * (j k) mod n exactly, so every twiddle is one of n table values. */
+extern "C" void orc_dft2(int n, double *data, int sign) {
+    typedef complex<double> C;
+    C *d = reinterpret_cast<C *>(data);
+    vector<C> tw(n), line(n), out(n);
+    const double twopi = 4. * asin(1.);
+    for (int m = 0; m < n; m++) tw[m] = C(cos(twopi * m / n), sign * sin(twopi * m / n));
+    for (int pass = 0; pass < 2; pass++) {
+        const long long step = pass == 0 ? 1 : n, stride = pass == 0 ? n : 1;
+        for (int l = 0; l < n; l++) {
+            for (int j = 0; j < n; j++) line[j] = d[l * stride + j * step];
+            for (int k = 0; k < n; k++) {
+                C acc(0, 0);
+                for (int j = 0; j < n; j++) acc += line[j] * tw[(long long)j * k % n];
+                out[k] = acc;
+            }
+            for (int k = 0; k < n; k++) d[l * stride + k * step] = out[k];
+        }
+    }
+}
+
+/* FieldSolverFFT::init tables (FieldSolverFFT.cpp:127-152): K2 (imaginary
+ * part) and the source filter's sigmoid, both at FFT index
+ * ((i + (n+1)/2) mod n) of grid point i. */
+extern "C" void orc_fft_tables(int ng, double dgrid, double xks, double xc, double yc, double sig,
+                               double *k2im, double *sigmoid) {
+    const double dk = 4. * asin(1.) / ((double)ng * dgrid);
+    const double shift = -0.5 * (double)(ng - 1);
+    for (int iy = 0; iy < ng; iy++) {
+        double dy = (double)iy + shift;
+        double y = dy / (double)ng / yc;
+        for (int ix = 0; ix < ng; ix++) {
+            double dx = (double)ix + shift;
+            double x = dx / (double)ng / xc;
+            int iiy = (iy + (ng + 1) / 2) % ng;
+            int iix = (ix + (ng + 1) / 2) % ng;
+            int ii = iiy * ng + iix;
+            k2im[ii] = -(dx * dx + dy * dy) * dk * dk / 2. / xks;
+            double r = (sqrt(x * x + y * y) - 1) / sig;
+            sigmoid[ii] = 1. / (1 + exp(r));
+        }
+    }
+}
+
+/* FieldSolverFFT::FFT on one slice (FieldSolverFFT.cpp:62-119): propagate
+ * the field by exp(K2 delz) in Fourier space and add twice the source,
+ * in real space, or filtered by the sigmoid in Fourier space. */
+extern "C" void orc_fft_slice(int ng, const double *k2im, const double *sigmoid, int filter, double delz,
+                      const double *srcd, double *ud, double *work) {
+    typedef complex<double> C;
+    C *u = reinterpret_cast<C *>(ud), *w = reinterpret_cast<C *>(work);
+    const C *S = reinterpret_cast<const C *>(srcd);
+    const int nn = ng * ng;
+    orc_dft2(ng, ud, -1);
+    if (filter) {
+        for (int i = 0; i < nn; i++) w[i] = S[i];
+        orc_dft2(ng, work, -1);
+        for (int i = 0; i < nn; i++) w[i] = w[i] * sigmoid[i];
+        for (int i = 0; i < nn; i++) u[i] = u[i] * exp(C(0, k2im[i]) * delz) + 2. * w[i];
+    } else {
+        for (int i = 0; i < nn; i++) u[i] = u[i] * exp(C(0, k2im[i]) * delz);
+    }
+    orc_dft2(ng, ud, +1);
+    const double norm = 1. / (double)nn;
+    if (filter)
+        for (int i = 0; i < nn; i++) u[i] = u[i] * norm;
+    else
+        for (int i = 0; i < nn; i++) u[i] = u[i] * norm + 2. * S[i];
+}
+
+/* Field::track -> FieldSolverFFT::init/advance (FieldSolverFFT.cpp:9-60):
+ * the same source as the ADI solver, then fft_slice on slice
+ * (ii + first) mod Ns. */
+static void field_advance_fft(orc_state *s, const orc_step *st, double delz) {
+    int ng = s->ngrid, ns = s->nslice;
+    vector<double> k2(ng * ng), sg(ng * ng);
+    const int filter = s->fft_filter && s->fft_sig > 0 && s->fft_xcut > 0 && s->fft_ycut > 0;
+    orc_fft_tables(ng, s->dgrid, s->xks, filter ? s->fft_xcut : 1, filter ? s->fft_ycut : 1,
+                   filter ? s->fft_sig : 1, k2.data(), sg.data());
+#pragma omp parallel num_threads(s->ndiag_threads)
+    {
+        vector<double> src(2 * ng * ng), w(2 * ng * ng);
+#pragma omp for schedule(dynamic, 1)
+        for (int ii = 0; ii < ns; ii++) {
+            orc_deposit_slice(s, st, ii, delz, src.data());
+            int i = (ii + s->first) % ns;
+            orc_fft_slice(ng, k2.data(), sg.data(), filter, delz, src.data(), s->field + 2LL * i * ng * ng,
+                      w.data());
+        }
+    }
+}
+
 /* Field::track -> FieldSolverADI::init/advance, Field.cpp:177-185. */
 extern "C" void orc_field_advance(orc_state *s, const orc_step *st, double delz) {
+    if (s->fft_solver) {
+        field_advance_fft(s, st, delz);
+        return;
+    }
     int ng = s->ngrid, ns = s->nslice;
     vector<double> c(2 * ng), cb(2 * ng), cw(2 * ng);
     double cim;

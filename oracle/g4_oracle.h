@@ -52,6 +52,10 @@ typedef struct {
     double sample;                   /* slice length / reflength              */
     int timerun, periodic;
     int ndiag_threads;               /* OpenMP threads over slices (1=serial) */
+    /* field solver: 0 = ADI, 1 = FFT (FieldSolverFFT) with optional source
+     * filter (initSourceFilter: sigmoid of width fft_sig at xcut/ycut) */
+    int fft_solver, fft_filter;
+    double fft_xcut, fft_ycut, fft_sig;
 } orc_state;
 
 /* Per-slice input of the quiet loader (QuietLoading.h BeamSlice). */
@@ -118,6 +122,14 @@ void orc_adi_slice(int ngrid, const double *c, const double *cbet,
                    const double *cwet, double cstep_im, const double *src,
                    double *u, double *r);
 void orc_field_advance(orc_state *s, const orc_step *st, double delz);
+/* FieldSolverFFT: unscaled 2-D DFT in place (sign -1 forward, +1 backward)
+ * and the K2 / sigmoid tables of FieldSolverFFT::init. */
+void orc_dft2(int n, double *data, int sign);
+void orc_fft_tables(int ng, double dgrid, double xks, double xc, double yc, double sig,
+                    double *k2im, double *sigmoid);
+/* FieldSolverFFT::FFT on one slice: u <- propagated field + source term */
+void orc_fft_slice(int ng, const double *k2im, const double *sigmoid, int filter, double delz,
+                   const double *src, double *u, double *work);
 void orc_slippage(orc_state *s, double slip);
 void orc_beam_track(orc_state *s, const orc_step *st, double gammaref, double delz);
 void orc_step_all(orc_state *s, const orc_step *st, double gammaref);

@@ -40,7 +40,9 @@ class State(C.Structure):
         ("sc_nz", C.c_int), ("sc_nphi", C.c_int), ("sc_ngrid", C.c_int), ("longrange", C.c_int),
         ("sc_rmax", C.c_double), ("sc_ks", C.c_double), ("sc_efield", _dp), ("sc_ez", _dp),
         ("reflength", C.c_double), ("sample", C.c_double), ("timerun", C.c_int),
-        ("periodic", C.c_int), ("ndiag_threads", C.c_int)]
+        ("periodic", C.c_int), ("ndiag_threads", C.c_int),
+        ("fft_solver", C.c_int), ("fft_filter", C.c_int),
+        ("fft_xcut", C.c_double), ("fft_ycut", C.c_double), ("fft_sig", C.c_double)]
 
 
 class SliceParams(C.Structure):

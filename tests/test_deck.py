@@ -469,3 +469,36 @@ def test_track_dumps_match_records(tmp_path, prec):
         assert g4mi.h5_read(par, "slice000001/gamma").shape == (1024,)
     print(f"  {len(fsteps)} field dumps, {len(bsteps)} beam dumps over {nrun} steps agree with the records",
           flush=True)
+
+
+@pytest.mark.gpu
+def test_example1_deck_with_fft_solver(tmp_path):
+    """The Example1 deck on a 63^2 grid with fft_fieldsolver = true
+    (Track.cpp:88, Field::initSolver): /Field/power and /Beam/bunching of
+    every record against the oracle's FieldSolverFFT restatement on the same
+    generated beam and field, within 1e-10 relative."""
+    print("deck runner: Example1 on 63^2 with the spectral field solver", flush=True)
+    deck = EX1_IN.replace("ngrid=255", "ngrid=63").replace("&track\n&end", "&track\nfft_fieldsolver = true\n&end")
+    path = _write(tmp_path, deck=deck, root=str(tmp_path / "ex1f"))
+    s = g4mi.check_deck(path)
+    g4mi.run_deck(path, device=0, quiet=True)
+    out = str(tmp_path / "ex1f.out.h5")
+    p = g4mi.h5_read(out, "Field/power")
+    b = g4mi.h5_read(out, "Beam/bunching")
+    els = []
+    for k1 in (2.0, -2.0) * 6:
+        els += [("undulator", dict(aw=0.84853, lambdau=0.015, nwig=266, helical=True)),
+                ("drift", dict(l=0.44)), ("quadrupole", dict(l=0.08, k1=k1)), ("drift", dict(l=0.24))]
+    st = O.unroll(els, 0.045, 1e-10, 11357.82)
+    sim = O.Sim([8192], 63, 2e-4, 1e-10, timerun=False, threads=16)
+    sim.s.fft_solver = 1
+    O.load_beam(sim, dict(gamma=11357.82, delgam=1.0, current=3000.0, ex=4e-7, ey=4e-7,
+                          betax=s["betax"], betay=s["betay"], alphax=s["alphax"], alphay=s["alphay"]),
+                None, 8, None, lambda0=1e-10)
+    O.load_gauss_all(sim, 5e3, 30e-6, 1e-10)
+    r = sim.run(st, 11357.82, record=("power", "bunching"))
+    ep = float(np.max(np.abs(p[:, 0] / r["power"][:, 0] - 1)))
+    mask = r["bunching"][:, 0] > 1e-6
+    eb = float(np.max(np.abs(b[mask, 0] / r["bunching"][mask, 0] - 1)))
+    print(f"  P_end {p[-1, 0]:.6e} W; P(z) max rel err vs oracle {ep:.3e}; |b(z)| {eb:.3e}", flush=True)
+    assert ep < 1e-10 and eb < 1e-10

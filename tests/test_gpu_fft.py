@@ -1,0 +1,181 @@
+"""GPU parity of the spectral field solver (FieldSolverFFT,
+src/Core/FieldSolverFFT.cpp) against the oracle's restatement (a direct
+2-D DFT per slice, pinned against numpy.fft and the analytic Gaussian
+beam).  The device runs the step as E' = M E M^T + 2 S with M the circulant
+1-D propagator, FP64 MFMA products; the oracle transforms, multiplies by
+exp(K2 delz) and transforms back.  The two differ only by rounding: field
+within 1e-12 of max |E| per step, P(z) and |b(z)| within 1e-10 over a run."""
+import ctypes as C
+import sys
+
+import numpy as np
+import pytest
+
+import cases
+import g4mi
+import harness as H
+import oracle_py as O
+
+pytestmark = pytest.mark.gpu
+
+
+def _log(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def _fft_state(counts, ngrid, gridmax=cases.GRIDMAX, filt=None, **kw):
+    sim = H.make_state(counts, ngrid=ngrid, gridmax=gridmax, **kw)
+    sim.s.fft_solver = 1
+    if filt:
+        sim.s.fft_filter = 1
+        sim.s.fft_xcut, sim.s.fft_ycut, sim.s.fft_sig = filt
+    ctx = H.to_gpu(sim)
+    if filt:
+        ctx.set_field_solver(fft=True, source_filter=True, xcut=filt[0], ycut=filt[1], sigmoid=filt[2])
+    else:
+        ctx.set_field_solver(fft=True)
+    return sim, ctx
+
+
+@pytest.mark.parametrize("ngrid", [31, 40, 63, 151])
+def test_fft_field_track_parity(ngrid):
+    """Deposit + spectral step on grids that take each output tile edge
+    (32, 48, 64, 80) of the complex GEMM, three undulator steps."""
+    _log(f"spectral field step parity: 6 slices x 1024, {ngrid}^2, 3 steps")
+    st = H.steps()
+    sim, ctx = _fft_state([1024] * 6, ngrid, seed_power=5e7)
+    gst = H.gsteps(st)
+    for i in range(3):
+        O.lib().orc_field_advance(C.byref(sim.s), C.byref(st[i]), st[i].dz)
+        ctx.field_track(gst[i], st[i].dz)
+    f, first = ctx.download_field()
+    assert first == sim.s.first
+    err = np.max(np.abs(f - sim.field)) / np.max(np.abs(sim.field))
+    _log(f"  field max diff / max |E| = {err:.3e}")
+    assert err < 1e-12
+    ctx.close()
+
+
+def test_fft_drift_step_is_pure_diffraction():
+    """A drift step: no source, the field is only propagated."""
+    st = H.steps()
+    sim, ctx = _fft_state([256] * 4, 63, gridmax=0.3e-3, seed_power=5e3)
+    O.lib().orc_field_advance(C.byref(sim.s), C.byref(st[125]), st[125].dz)
+    ctx.field_track(H.gsteps(st)[125], st[125].dz)
+    f, _ = ctx.download_field()
+    err = np.max(np.abs(f - sim.field)) / np.max(np.abs(sim.field))
+    _log(f"  drift step: field max diff / max |E| = {err:.3e}")
+    assert err < 1e-13
+
+
+def test_fft_source_filter_parity():
+    """Source filtered by a sigmoid at a third of the band (xcut = ycut =
+    0.3, width 0.05), two undulator steps."""
+    _log("spectral step with source filter: 4 slices x 1024, 63^2")
+    st = H.steps()
+    sim, ctx = _fft_state([1024] * 4, 63, gridmax=0.3e-3, seed_power=5e7, filt=(0.3, 0.3, 0.05))
+    gst = H.gsteps(st)
+    for i in range(2):
+        O.lib().orc_field_advance(C.byref(sim.s), C.byref(st[i]), st[i].dz)
+        ctx.field_track(gst[i], st[i].dz)
+    f, _ = ctx.download_field()
+    err = np.max(np.abs(f - sim.field)) / np.max(np.abs(sim.field))
+    _log(f"  field max diff / max |E| = {err:.3e}")
+    assert err < 1e-12
+
+
+def test_fft_sase_end_to_end():
+    """Time-dependent SASE with slippage on the spectral solver: 12 slices
+    x 1024, 63^2, the first 200 steps (two undulator modules and a FODO
+    drift), P(z) and |b(z)| within 1e-10."""
+    _log("spectral solver SASE, 12 slices, 200 steps, GPU vs oracle")
+    st = H.steps()
+    sim, ctx = _fft_state([1024] * 12, 63, gridmax=0.3e-3, longrange=True, threads=16)
+    g = ctx.run(H.gsteps(st)[:200], cases.GAMMA0, record=("power", "bunching"))
+    r = sim.run(st, cases.GAMMA0, record=("power", "bunching"), nsteps=200)
+    ep = H.rel_err(g["power"][1:], r["power"][1:])
+    eb = H.rel_err(g["bunching"], r["bunching"])
+    _log(f"  P(z) rel err {ep:.3e}  |b(z)| rel err {eb:.3e}  sum P_end {g['power'][-1].sum():.4e}")
+    assert ep < 1e-10 and eb < 1e-10
+    f, first = ctx.download_field()
+    assert first == sim.s.first
+    assert np.max(np.abs(f - sim.field)) < 1e-10 * np.max(np.abs(sim.field))
+
+
+def test_fft_solver_rejected_in_mixed_form():
+    sim = H.make_state([256] * 2, ngrid=31, gridmax=0.3e-3)
+    ctx = H.to_gpu(sim, precision=g4mi.FP32)
+    with pytest.raises(g4mi.G4miError) as e:
+        ctx.set_field_solver(fft=True)
+    assert "FP64" in str(e.value)
+
+
+def _tilted_fields(ng, gridmax, lam, w0, tilts):
+    x = np.linspace(-gridmax, gridmax, ng)
+    X, Y = np.meshgrid(x, x)  # [iy][ix]
+    k = 2 * np.pi / lam
+    out = []
+    for tx, ty, amp in tilts:
+        e = amp * np.exp(-(X ** 2 + Y ** 2) / w0 ** 2) * np.exp(1j * k * (tx * X + ty * Y))
+        out.append(np.stack([e.real, e.imag], axis=-1).reshape(-1, 2))
+    return np.ascontiguousarray(np.array(out))
+
+
+def _numpy_spectral(field, ng, first, lam, dgrid):
+    """DiagField's spectral block restated with numpy.fft: moments of
+    |DFT(E)|^2 at the grid offset of each FFT index."""
+    ns = field.shape[0]
+    q = (np.arange(ng) - (ng + 1) // 2) % ng - 0.5 * (ng - 1)  # offset of FFT index k
+    scl = lam / ng / dgrid
+    res = {k: np.zeros(ns) for k in ("xdivergence", "ydivergence", "xpointing", "ypointing")}
+    tot = np.zeros(5)
+    for k in range(ns):
+        e = (field[k, :, 0] + 1j * field[k, :, 1]).reshape(ng, ng)
+        w = np.abs(np.fft.fft2(e)) ** 2
+        m = np.array([w.sum(), (w * q[None, :]).sum(), (w * q[None, :] ** 2).sum(),
+                      (w * q[:, None]).sum(), (w * q[:, None] ** 2).sum()])
+        tot += m
+        v = m[1:] / m[0]
+        i = (ns + k - first) % ns
+        res["xpointing"][i] = v[0] * scl
+        res["xdivergence"][i] = np.sqrt(abs(v[1] - v[0] ** 2)) * scl
+        res["ypointing"][i] = v[2] * scl
+        res["ydivergence"][i] = np.sqrt(abs(v[3] - v[2] ** 2)) * scl
+    v = tot[1:] / tot[0]
+    res["g_xpointing"] = v[0] * scl
+    res["g_xdivergence"] = np.sqrt(abs(v[1] - v[0] ** 2)) * scl
+    res["g_ypointing"] = v[2] * scl
+    res["g_ydivergence"] = np.sqrt(abs(v[3] - v[2] ** 2)) * scl
+    return res
+
+
+@pytest.mark.parametrize("prec", [g4mi.FP64, g4mi.FP32])
+def test_spectral_diagnostics(prec):
+    """Divergence and pointing (Diagnostic.cpp:789-846, globals :952-963)
+    of three tilted Gaussian beams (w0 = 30 um on 121^2 cells of 5 um, ring
+    offset first = 1): against the same moments from numpy.fft (1e-12 in
+    the FP64 form, 1e-5 in the FP32 form) and against optics: the pointing
+    is the tilt and the rms divergence lambda / (2 pi w0), to 1e-6."""
+    ng, gm, w0 = 121, 0.3e-3, 30e-6
+    lam = cases.LAMBDA_R
+    sig = lam / (2 * np.pi * w0)
+    tilts = [(3 * sig, -2 * sig, 1e7), (0.0, 0.0, 2e7), (-1.5 * sig, 4 * sig, 5e6)]
+    _log(f"spectral diagnostics ({'FP64' if prec == g4mi.FP64 else 'FP32'} form), 3 tilted beams, {ng}^2")
+    sim = H.make_state([64] * 3, ngrid=ng, gridmax=gm)
+    sim.field[:] = _tilted_fields(ng, gm, lam, w0, tilts)
+    sim.s.first = 1
+    ctx = H.to_gpu(sim, precision=prec)
+    d = ctx.diag(spectral=True)
+    ref = _numpy_spectral(sim.field, ng, 1, lam, 2 * gm / (ng - 1))
+    tol = 1e-12 if prec == g4mi.FP64 else 1e-5
+    for k in ("xdivergence", "ydivergence", "xpointing", "ypointing"):
+        np.testing.assert_allclose(d[k], ref[k], rtol=tol, atol=tol * sig, err_msg=k)
+        np.testing.assert_allclose(d["g_" + k], ref["g_" + k], rtol=tol, atol=tol * sig, err_msg=k)
+    # optics: output slice i holds storage slice (i + first) mod ns
+    for i in range(3):
+        tx, ty, _ = tilts[(i + 1) % 3]
+        assert abs(d["xpointing"][i] - tx) < 1e-6 * sig and abs(d["ypointing"][i] - ty) < 1e-6 * sig
+        assert abs(d["xdivergence"][i] / sig - 1) < 1e-6 and abs(d["ydivergence"][i] / sig - 1) < 1e-6
+    _log(f"  divergence {d['xdivergence']} (optics {sig:.6e}); pointing {d['xpointing']}")
+    ctx.close()
