@@ -4,7 +4,7 @@
 //
 // Covered elements (src/Main/GenMain.cpp:80-494 dispatch order, element by
 // element in file order): &setup, &lattice (zmatch), &time, &field, &beam,
-// &efield, &track, &write and &stop.  Any other element, or a key an element does
+// &efield, &track, &write, &importbeam, &importfield and &stop.  Any other element, or a key an element does
 // not know, stops the run with an error, as the reference does for unknown
 // keys.  Profile references ("@name") are not supported.
 #include <sys/stat.h>
@@ -517,6 +517,221 @@ bool do_beam(Deck &dk, Namelist &nl, std::string &err) {
     }
     dk.current.assign(ns, sb.current);
     dk.have_beam = true;
+    return true;
+}
+
+// Time window taken from an imported file when no &time element set one
+// (ReadBeamHDF5/ReadFieldHDF5::readGlobal: Time::setSampleRate rounds to
+// an integer, then Time::init/finishInit).
+bool time_from_file(Deck &dk, double s0, double slicelen, double reflen, int count, bool dotime,
+                    std::string &err) {
+    TimeNL &t = dk.time;
+    if (t.ds <= 0 && count > 1) {
+        t.sample = (int)std::round(slicelen / reflen);
+        t.s0 = s0;
+        t.slen = slicelen * count;
+        t.dotime = dotime;
+        if (t.sample < 1) {
+            err = "sample rate must be at least 1";
+            return false;
+        }
+        t.ds = dk.setup.lambda0 * t.sample;
+        t.nslice = std::max(1, (int)std::round(t.slen / t.ds));
+        t.slen = t.ds * t.nslice;
+    }
+    const double sample = (double)t.sample;
+    if (fabs(slicelen - dk.setup.lambda0 * sample) / slicelen > 1e-6) {
+        err = "mismatch in sample rate of run and of input file";
+        return false;
+    }
+    return true;
+}
+
+bool read_scalar(const std::string &file, const char *name, double &v, std::string &err) {
+    std::vector<double> buf;
+    long long dims[2];
+    if (h5_read_doubles(file, name, buf, dims, err) < 1) {
+        if (err.empty()) err = std::string("dataset ") + name + " is empty";
+        err = file + ": " + err;
+        return false;
+    }
+    v = buf[0];
+    return true;
+}
+
+// &importbeam (ImportBeam::init, src/Loading/ImportBeam.cpp:25-85, with
+// ReadBeamHDF5::readGlobal/readSlice, src/IO/readBeamHDF5.cpp:23-185): the
+// particles of every slice of the time window from a .par.h5 file; a
+// window position outside the file's slices gets the file's first slice
+// with zero current and zero transverse coordinates, a position between
+// its slices stays empty.
+bool do_importbeam(Deck &dk, Namelist &nl, std::string &err) {
+    if (dk.have_beam) {
+        err = "cannot import beam, because beam is already defined";
+        return false;
+    }
+    Keys k{nl, ""};
+    std::string file;
+    bool dotime = true;
+    k.str("file", file);
+    k.boolean("time", dotime);
+    if (!k.finish()) {
+        err = k.err;
+        return false;
+    }
+    double s0f = 0, reflen = 0, slicelen = 0, countd = 0, nbinsd = 0, o4o = 0;
+    if (!read_scalar(file, "refposition", s0f, err) || !read_scalar(file, "slicelength", reflen, err) ||
+        !read_scalar(file, "slicespacing", slicelen, err) || !read_scalar(file, "slicecount", countd, err) ||
+        !read_scalar(file, "beamletsize", nbinsd, err) || !read_scalar(file, "one4one", o4o, err))
+        return false;
+    const SetupNL &st = dk.setup;
+    const int count = (int)countd;
+    if (fabs((st.lambda0 - reflen) / st.lambda0) > 1e-6) {
+        err = "mismatch between reference length of run and of input file";
+        return false;
+    }
+    if ((int)nbinsd != st.nbins) {
+        err = "mismatch between beamlet size (NBINS) of run and of input file";
+        return false;
+    }
+    if (((st.one4one) && (o4o == 0)) || ((!st.one4one) && (o4o != 0))) {
+        err = "mismatch between ONE4ONE flag of run and of input file";
+        return false;
+    }
+    if (!time_from_file(dk, s0f, slicelen, reflen, count, dotime, err)) return false;
+    const int ns = time_nslice(dk);
+    const double ds = time_ds(dk);
+    dk.off.assign(ns + 1, 0);
+    dk.current.assign(ns, 0.0);
+    std::vector<std::vector<double>> parts((size_t)ns * 6);
+    static const char *names[6] = {"gamma", "theta", "x", "y", "px", "py"};
+    for (int i = 0; i < ns; i++) {
+        const double s = dk.time.s0 + i * ds;
+        const double rslice = (s - s0f) / slicelen;
+        long long n = 0;
+        if (fabs(rslice - std::round(rslice)) <= 1e-3) {
+            int islice = (int)std::round(rslice) + 1;
+            double wei = 1.;
+            if (islice < 1 || islice > count) {
+                wei = 0;
+                islice = 1;
+            }
+            char g[32];
+            snprintf(g, sizeof(g), "slice%6.6d/", islice);
+            double cur = 0;
+            if (!read_scalar(file, (std::string(g) + "current").c_str(), cur, err)) return false;
+            dk.current[i] = cur * wei;
+            for (int q = 0; q < 6; q++) {
+                long long dims[2];
+                std::vector<double> &v = parts[(size_t)i * 6 + q];
+                if (h5_read_doubles(file, std::string(g) + names[q], v, dims, err) < 0) {
+                    err = file + ": " + err;
+                    return false;
+                }
+                if (q >= 2)
+                    for (double &x : v) x *= wei;
+                if (q == 0) n = (long long)v.size();
+                if ((long long)v.size() != n) {
+                    err = file + ": slice " + std::to_string(islice) + " has datasets of different sizes";
+                    return false;
+                }
+            }
+        }
+        dk.off[i + 1] = dk.off[i] + n;
+    }
+    const size_t ntot = (size_t)dk.off[ns];
+    std::vector<double> *dst[6] = {&dk.gamma, &dk.theta, &dk.x, &dk.y, &dk.px, &dk.py};
+    for (int q = 0; q < 6; q++) {
+        dst[q]->assign(ntot, 0.0);
+        for (int i = 0; i < ns; i++) {
+            const std::vector<double> &v = parts[(size_t)i * 6 + q];
+            std::copy(v.begin(), v.end(), dst[q]->begin() + dk.off[i]);
+        }
+    }
+    dk.have_beam = true;
+    return true;
+}
+
+// &importfield (ImportField::init, src/Loading/ImportField.cpp:23-91, with
+// ReadFieldHDF5::readGlobal/readSlice, src/IO/readFieldHDF5.cpp:23-120):
+// the grid and the slices of a .fld.h5 file, scaled back from sqrt(W) per
+// cell by dgrid eev / ks / sqrt(vacimp) and by `attenuation`, the file's
+// start shifted by `offset` rounded to whole slices.
+bool do_importfield(Deck &dk, Namelist &nl, std::string &err) {
+    Keys k{nl, ""};
+    std::string file;
+    bool dotime = true;
+    int harm = 1;
+    double attenuation = 1.0, offset = 0.0;
+    k.str("file", file);
+    k.boolean("time", dotime);
+    k.integer("harmonic", harm);
+    k.dbl("attenuation", attenuation);
+    k.dbl("offset", offset);
+    if (!k.finish()) {
+        err = k.err;
+        return false;
+    }
+    if (harm != 1) {
+        err = "&importfield: only the fundamental (harmonic = 1) is supported";
+        return false;
+    }
+    if (dk.have_field) {
+        err = "cannot import field, because field is already defined";
+        return false;
+    }
+    double s0f = 0, dgrid = 0, reflen = 0, slicelen = 0, countd = 0, ngd = 0;
+    if (!read_scalar(file, "refposition", s0f, err) || !read_scalar(file, "gridsize", dgrid, err) ||
+        !read_scalar(file, "wavelength", reflen, err) || !read_scalar(file, "slicespacing", slicelen, err) ||
+        !read_scalar(file, "slicecount", countd, err) || !read_scalar(file, "gridpoints", ngd, err))
+        return false;
+    const int count = (int)countd, ngrid = (int)ngd;
+    if (ngrid < 3) {
+        err = file + ": invalid grid";
+        return false;
+    }
+    s0f += slicelen * std::round(offset / slicelen);
+    const double ks = 4. * asin(1) / reflen;
+    const double scl = 1. / (dgrid * kEevD / ks / sqrt(kVacimpD)) * attenuation;
+    const double gridmax = 0.5 * dgrid * (double)(ngrid - 1);
+    if (fabs((dk.setup.lambda0 - reflen * (double)harm) / dk.setup.lambda0) > 1e-6) {
+        err = "mismatch between reference length of run and of input file";
+        return false;
+    }
+    if (!time_from_file(dk, s0f, slicelen, reflen, count, dotime, err)) return false;
+    const int ns = time_nslice(dk);
+    const double ds = time_ds(dk);
+    const size_t cells = (size_t)ngrid * ngrid;
+    dk.field.assign((size_t)ns * cells * 2, 0.0);
+    for (int i = 0; i < ns; i++) {
+        const double s = dk.time.s0 + i * ds;
+        const double rslice = (s - s0f) / slicelen;
+        if (fabs(rslice - std::round(rslice)) > 1e-3) continue;
+        const int islice = (int)std::round(rslice) + 1;
+        if (islice < 1 || islice > count) continue;
+        char g[32];
+        snprintf(g, sizeof(g), "slice%6.6d/", islice);
+        std::vector<double> re, im;
+        long long dims[2];
+        if (h5_read_doubles(file, std::string(g) + "field-real", re, dims, err) < 0 ||
+            h5_read_doubles(file, std::string(g) + "field-imag", im, dims, err) < 0) {
+            err = file + ": " + err;
+            return false;
+        }
+        if (re.size() != cells || im.size() != cells) {
+            err = file + ": slice " + std::to_string(islice) + " does not hold gridpoints^2 values";
+            return false;
+        }
+        double *f = dk.field.data() + (size_t)i * cells * 2;
+        for (size_t j = 0; j < cells; j++) {
+            f[2 * j] = scl * re[j];
+            f[2 * j + 1] = scl * im[j];
+        }
+    }
+    dk.ngrid = ngrid;
+    dk.gridmax = gridmax;
+    dk.flambda = reflen;
+    dk.have_field = true;
     return true;
 }
 
@@ -1233,6 +1448,8 @@ int run_deck(const char *input_file, int device, int quiet, bool dry, std::strin
         else if (e == "&efield") ok = do_efield(dk, nl, err);
         else if (e == "&track") ok = do_track(dk, nl, device, quiet != 0, dry, err);
         else if (e == "&write") ok = do_write(dk, nl, err);
+        else if (e == "&importbeam") ok = do_importbeam(dk, nl, err);
+        else if (e == "&importfield") ok = do_importfield(dk, nl, err);
         else {
             err = "element " + e + " is not supported by this build";
             ok = false;

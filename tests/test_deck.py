@@ -502,3 +502,118 @@ def test_example1_deck_with_fft_solver(tmp_path):
     eb = float(np.max(np.abs(b[mask, 0] / r["bunching"][mask, 0] - 1)))
     print(f"  P_end {p[-1, 0]:.6e} W; P(z) max rel err vs oracle {ep:.3e}; |b(z)| {eb:.3e}", flush=True)
     assert ep < 1e-10 and eb < 1e-10
+
+
+IMPORT_IN = """&setup
+rootname={root}
+lattice={lat}
+beamline=FEL
+lambda0=1e-10
+gamma0=11357.82
+delz=0.045000
+npart=256
+nbins=4
+&end
+
+&importfield
+file = {src}.fld.h5
+&end
+
+&importbeam
+file = {src}.par.h5
+&end
+
+&write
+field = @.back
+beam = @.back
+&end
+
+&track
+zstop=1
+&end
+"""
+
+
+def test_import_round_trip(tmp_path):
+    """&importfield / &importbeam (ImportField.cpp, ImportBeam.cpp with
+    readFieldHDF5/readBeamHDF5) of the dumps &write made: the time window
+    (4 slices) comes from the files, the particles come back bit for bit,
+    the field within one rounding of the sqrt(W) scaling and back."""
+    (tmp_path / "Example1.lat").write_text(EX1_LAT.format(extra=""))
+    src = str(tmp_path / "dmp")
+    (tmp_path / "dump.in").write_text(DUMP_IN.format(root=src, lat=str(tmp_path / "Example1.lat"))
+                                      .replace("stride = 2", "stride = 1"))
+    g4mi.check_deck(tmp_path / "dump.in")
+    root = str(tmp_path / "imp")
+    (tmp_path / "imp.in").write_text(IMPORT_IN.format(root=root, lat=str(tmp_path / "Example1.lat"),
+                                                       src=src + ".init"))
+    s = g4mi.check_deck(tmp_path / "imp.in")
+    assert s["nslice"] == 4 and s["npart"] == 4 * 256 and s["ngrid"] == 63 and s["time"] is True
+    a, b = src + ".init.par.h5", root + ".back.par.h5"
+    for i in range(4):
+        for k in ("current", "gamma", "theta", "x", "y", "px", "py"):
+            va = g4mi.h5_read(a, f"slice{i + 1:06d}/{k}")
+            assert np.array_equal(va, g4mi.h5_read(b, f"slice{i + 1:06d}/{k}")), (i, k)
+    a, b = src + ".init.fld.h5", root + ".back.fld.h5"
+    for k in ("gridsize", "slicespacing", "refposition", "wavelength", "gridpoints", "slicecount"):
+        assert np.array_equal(g4mi.h5_read(a, k), g4mi.h5_read(b, k)), k
+    for i in range(4):
+        for k in ("field-real", "field-imag"):
+            va = g4mi.h5_read(a, f"slice{i + 1:06d}/{k}")
+            vb = g4mi.h5_read(b, f"slice{i + 1:06d}/{k}")
+            assert np.max(np.abs(va - vb)) <= 4e-16 * np.max(np.abs(va)), (i, k)
+
+
+@pytest.mark.parametrize("bad, msg", [("lambda0=1e-10", "reference length"), ("nbins=4", "beamlet size")])
+def test_import_mismatch_errors(tmp_path, bad, msg):
+    (tmp_path / "Example1.lat").write_text(EX1_LAT.format(extra=""))
+    src = str(tmp_path / "dmp")
+    (tmp_path / "dump.in").write_text(DUMP_IN.format(root=src, lat=str(tmp_path / "Example1.lat")))
+    g4mi.check_deck(tmp_path / "dump.in")
+    repl = {"lambda0=1e-10": "lambda0=2e-10", "nbins=4": "nbins=8"}[bad]
+    deck = IMPORT_IN.replace(bad, repl).replace("npart=256", "npart=512")
+    if bad == "lambda0=1e-10":
+        deck = deck.replace("&importfield\nfile = {src}.fld.h5\n&end\n", "")
+    (tmp_path / "imp.in").write_text(deck.format(root=str(tmp_path / "imp"), lat=str(tmp_path / "Example1.lat"),
+                                                 src=src + ".init"))
+    with pytest.raises(g4mi.G4miError) as e:
+        g4mi.check_deck(tmp_path / "imp.in")
+    assert msg in str(e.value).lower()
+
+
+@pytest.mark.gpu
+def test_restart_from_dumps(tmp_path):
+    """Checkpoint and restart: the Example1 deck (63^2) dumps beam and field
+    after every FODO cell; a second deck imports the first cell's dumps and
+    tracks five cells.  Its P(z) and |b(z)| are the uninterrupted run's from
+    that cell on, within 1e-10 (the restart only rounds the field through
+    the dump's sqrt(W) scaling)."""
+    print("deck runner: restart from the dumps after the first FODO cell", flush=True)
+    deck = EX1_IN.replace("ngrid=255", "ngrid=63")
+    path = _write(tmp_path, deck=deck, root=str(tmp_path / "full"))
+    s = g4mi.check_deck(path)
+    nst = s["steps"]
+    assert nst % 6 == 0
+    k = nst // 6
+    deck_a = deck.replace("&track\n&end", f"&track\nfield_dump_step = {k}\nbeam_dump_step = {k}\n&end")
+    path = _write(tmp_path, deck=deck_a, root=str(tmp_path / "full"))
+    g4mi.run_deck(path, device=0, quiet=True)
+    pa = g4mi.h5_read(str(tmp_path / "full.out.h5"), "Field/power")[:, 0]
+    ba = g4mi.h5_read(str(tmp_path / "full.out.h5"), "Beam/bunching")[:, 0]
+    z = g4mi.h5_read(str(tmp_path / "full.out.h5"), "Lattice/z")
+    src = str(tmp_path / f"full.{k}")
+    deck_b = (deck.replace("&field\npower=5e3\ndgrid=2.000000e-04\nngrid=63\nwaist_size=30e-6\n&end",
+                           f"&importfield\nfile = {src}.fld.h5\n&end")
+              .replace("&beam\ncurrent=3000\ndelgam=1.000000\nex=4.000000e-07\ney=4.000000e-07\n&end",
+                       f"&importbeam\nfile = {src}.par.h5\n&end")
+              .replace("&track\n&end", f"&track\nzstop = {z[5 * k] - 1e-3}\n&end"))
+    assert "&importfield" in deck_b and "&importbeam" in deck_b
+    path = _write(tmp_path, deck=deck_b, root=str(tmp_path / "rest"))
+    g4mi.run_deck(path, device=0, quiet=True)
+    pb = g4mi.h5_read(str(tmp_path / "rest.out.h5"), "Field/power")[:, 0]
+    bb = g4mi.h5_read(str(tmp_path / "rest.out.h5"), "Beam/bunching")[:, 0]
+    assert len(pb) == 5 * k + 1
+    ep = float(np.max(np.abs(pb / pa[k:] - 1)))
+    eb = float(np.max(np.abs(bb / ba[k:] - 1)))
+    print(f"  restart after {k} steps: P(z) rel err {ep:.3e}, |b(z)| {eb:.3e}", flush=True)
+    assert ep < 1e-10 and eb < 1e-10
