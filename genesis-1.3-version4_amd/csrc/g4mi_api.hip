@@ -83,6 +83,18 @@ struct Ctx {
     double2 *d_fft_m = nullptr, *d_fft_w = nullptr, *d_fft_wc = nullptr, *d_fft_u = nullptr,
             *d_fft_v = nullptr;
     double *d_fft_sig = nullptr, *d_fft_mom = nullptr;
+    // incoherent synchrotron radiation (Incoherent): the host's uniform
+    // stream, one kick per beamlet staged through pinned memory
+    Ran2 *incoh = nullptr;
+    int incoh_loss = 0, incoh_spread = 0, incoh_nbins = 1;
+    long long incoh_nb = 0;                 // beamlets in the whole beam
+    long long *d_boff = nullptr;            // [ns+1] first beamlet of each slice
+    double *d_bdg = nullptr, *h_bdg = nullptr;
+    hipEvent_t incoh_ev = nullptr;          // the last kick upload has been read
+    // wake loss per slice [eV/m] (Collective: wakeext + wakeint) or unset
+    double *d_wake = nullptr, *d_wake_dg = nullptr, *d_wake_buf = nullptr;
+    std::vector<double> h_wake, wake_applied;
+    double wake_dz = 0;
     // timing: kEv events per step, recorded into a pool and folded lazily
     static constexpr int kEv = 9;
     static constexpr int kPoolSteps = 512;
@@ -272,6 +284,47 @@ static int ensure_factors(Ctx *c, double delz) {
     return G4MI_OK;
 }
 
+// Incoherent::apply (src/Core/Incoherent.cpp:35-82) and Collective::apply
+// (src/Core/Collective.cpp:107-128) after the push.  The incoherent kicks
+// of one step are drawn on the host from the run's single uniform stream,
+// one per beamlet in slice and particle order, so the device adds exactly
+// the reference's values; the wake kick is eloss * delz / eev per slice.
+static int energy_kicks(Ctx *c, const g4mi_step_params &st, double gamma0, double delz) {
+    const double *bdg = nullptr, *sdg = nullptr;
+    if (c->incoh && st.aw > 0 && (c->incoh_loss || c->incoh_spread)) {
+        const double gam0 = gamma0, awz = st.aw, xkw0 = st.ku;
+        double dgamsig = 1.015e-27 * xkw0 * xkw0 * awz * awz;
+        if (st.helical)
+            dgamsig *= 1.42 * awz + 1. / (1. + 1.5 * awz + 0.95 * awz * awz);
+        else
+            dgamsig *= 1.697 * awz + 1. / (1. + 1.88 * awz + 0.8 * awz * awz);
+        if (!c->incoh_spread) dgamsig = 0;
+        dgamsig = sqrt(dgamsig * gam0 * gam0 * gam0 * gam0 * xkw0 * delz) * sqrt(3.);
+        double dgamavg = xkw0 * gam0 * awz;
+        if (!c->incoh_loss) dgamavg = 0;
+        dgamavg = dgamavg * dgamavg * 1.88e-15 * delz;
+        G4_HIP(hipEventSynchronize(c->incoh_ev));  // the previous upload has been read
+        for (long long b = 0; b < c->incoh_nb; b++) c->h_bdg[b] = -dgamavg + dgamsig * (2 * c->incoh->next() - 1);
+        G4_HIP(hipMemcpyAsync(c->d_bdg, c->h_bdg, c->incoh_nb * sizeof(double), hipMemcpyHostToDevice, c->stream));
+        G4_HIP(hipEventRecord(c->incoh_ev, c->stream));
+        bdg = c->d_bdg;
+    }
+    if (c->d_wake) {
+        if (delz != c->wake_dz) {
+            std::vector<double> dg(c->ns);
+            for (int s = 0; s < c->ns; s++) dg[s] = c->h_wake[s] * delz / kEev;
+            G4_HIP(hipMemcpyAsync(c->d_wake_dg, dg.data(), dg.size() * sizeof(double), hipMemcpyHostToDevice,
+                                  c->stream));
+            G4_HIP(hipStreamSynchronize(c->stream));
+            c->wake_dz = delz;
+        }
+        sdg = c->d_wake_dg;
+        c->wake_applied = c->h_wake;
+    }
+    launch_kicks(c->stream, c->ns, c->d_off, c->d_g, bdg, c->d_boff, c->incoh_nbins, sdg);
+    return G4MI_OK;
+}
+
 static int beam_track(Ctx *c, const g4mi_step_params &st, double gamma0, double delz) {
     if (!c->has_beam) {
         set_error("beam not uploaded");
@@ -327,6 +380,7 @@ static int beam_track(Ctx *c, const g4mi_step_params &st, double gamma0, double 
     launch_push(c->stream, c->nblk, c->d_blk_slice, c->d_blk_begin, c->d_off, c->d_g, c->d_th,
                 c->d_x, c->d_y, c->d_px, c->d_py, c->d_field, c->d_esc, c->d_ez, p);
     if (c->ev) G4_HIP(hipEventRecord(c->ev[3], c->stream));
+    G4_TRY(energy_kicks(c, st, gamma0, delz));
     if (st.chic_angle != 0) {
         const double r56 = chicane_r56(st.chic_angle, st.chic_lb, st.chic_ld, c->lambda0, gamma0);
         launch_r56(c->stream, c->ntot, c->d_g, c->d_th, r56, gamma0);
@@ -634,7 +688,8 @@ static void finish_aux(const Ctx *c, const std::vector<double> &aux, const std::
         for (int k = 0; k < 10; k++)
             if (dst[k]) dst[k][is] = a[2 + k];
         if (o->nf_phase) o->nf_phase[is] = atan2(a[13], a[12]);
-        const double eloss = 0.0;  // no wakes in this build
+        // Beam::eloss: the wake loss of the last step that applied it
+        const double eloss = is < (int)c->wake_applied.size() ? c->wake_applied[is] : 0.0;
         if (o->efield) o->efield[is] = eloss + esc[is];
         if (o->wakefield) o->wakefield[is] = eloss;
         if (o->lscfield) o->lscfield[is] = esc[is];
@@ -1515,8 +1570,77 @@ int g4mi_destroy(g4mi_ctx *h) {
     mix_clear_graphs(c);
     for (auto &e : c->pool)
         if (e) (void)hipEventDestroy(e);
+    if (c->incoh_ev) (void)hipEventDestroy(c->incoh_ev);
+    if (c->h_bdg) (void)hipHostFree(c->h_bdg);
+    delete c->incoh;
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete h;
+    return G4MI_OK;
+}
+
+int g4mi_set_incoherent(g4mi_ctx *h, int seed, int nbins, int do_loss, int do_spread) {
+    if (!h || nbins < 1) {
+        set_error("g4mi_set_incoherent: invalid argument");
+        return G4MI_E_ARG;
+    }
+    Ctx *c = &h->c;
+    if (c->precision != G4MI_FP64) {
+        set_error("g4mi_set_incoherent: incoherent synchrotron radiation runs in the FP64 form only");
+        return G4MI_E_ARG;
+    }
+    delete c->incoh;
+    c->incoh = nullptr;
+    c->incoh_loss = do_loss ? 1 : 0;
+    c->incoh_spread = do_spread ? 1 : 0;
+    if (!c->incoh_loss && !c->incoh_spread) return G4MI_OK;
+    // Incoherent::init (:17-31) on rank 0: the stream is seeded with
+    // round(1e9 u), u the first draw of ran2(seed)
+    Ran2 rseed((unsigned int)seed);
+    const double val = rseed.next() * 1e9;
+    c->incoh = new Ran2((unsigned int)(int)std::round(val));
+    // beamlet offsets: ceil(n_s / nbins) kicks per slice
+    std::vector<long long> boff(c->ns + 1, 0);
+    for (int s = 0; s < c->ns; s++) boff[s + 1] = boff[s] + (c->off[s + 1] - c->off[s] + nbins - 1) / nbins;
+    if (c->d_bdg && (c->incoh_nbins != nbins)) {
+        set_error("g4mi_set_incoherent: the beamlet size cannot change within a context");
+        return G4MI_E_ARG;
+    }
+    c->incoh_nbins = nbins;
+    c->incoh_nb = boff[c->ns];
+    if (!c->d_bdg) {
+        G4_TRY(dalloc(c, &c->d_boff, (size_t)c->ns + 1));
+        G4_TRY(dalloc(c, &c->d_bdg, (size_t)std::max(1LL, c->incoh_nb)));
+        G4_HIP(hipHostMalloc((void **)&c->h_bdg, (size_t)std::max(1LL, c->incoh_nb) * sizeof(double)));
+        G4_HIP(hipEventCreateWithFlags(&c->incoh_ev, hipEventDisableTiming));
+        G4_HIP(hipEventRecord(c->incoh_ev, c->stream));
+    }
+    G4_HIP(hipMemcpy(c->d_boff, boff.data(), boff.size() * sizeof(long long), hipMemcpyHostToDevice));
+    return G4MI_OK;
+}
+
+int g4mi_set_wake(g4mi_ctx *h, const double *eloss) {
+    if (!h) {
+        set_error("g4mi_set_wake: null context");
+        return G4MI_E_ARG;
+    }
+    Ctx *c = &h->c;
+    if (c->precision != G4MI_FP64) {
+        set_error("g4mi_set_wake: wakes run in the FP64 form only");
+        return G4MI_E_ARG;
+    }
+    if (!eloss) {
+        c->d_wake = nullptr;  // buffers stay allocated for a later call
+        c->h_wake.clear();
+        return G4MI_OK;
+    }
+    if (!c->d_wake_dg) {
+        G4_TRY(dalloc(c, &c->d_wake_dg, (size_t)c->ns));
+        G4_TRY(dalloc(c, &c->d_wake_buf, (size_t)c->ns));
+    }
+    c->h_wake.assign(eloss, eloss + c->ns);
+    G4_HIP(hipMemcpy(c->d_wake_buf, eloss, c->ns * sizeof(double), hipMemcpyHostToDevice));
+    c->d_wake = c->d_wake_buf;
+    c->wake_dz = std::nan("");
     return G4MI_OK;
 }
 

@@ -4,7 +4,7 @@
 //
 // Covered elements (src/Main/GenMain.cpp:80-494 dispatch order, element by
 // element in file order): &setup, &lattice (zmatch), &time, &field, &beam,
-// &efield, &track, &write, &importbeam, &importfield and &stop.  Any other element, or a key an element does
+// &efield, &wake, &sponrad, &track, &write, &importbeam, &importfield and &stop.  Any other element, or a key an element does
 // not know, stops the run with an error, as the reference does for unknown
 // keys.  Profile references ("@name") are not supported.
 #include <sys/stat.h>
@@ -174,11 +174,26 @@ struct EFieldNL {
     bool longrange = false;
 };
 
+struct WakeNL {  // Wake::Wake defaults (src/Main/Wake.cpp:5-20)
+    bool set = false;
+    double loss = 0, radius = 2.5e-3, conductivity = 0, relaxation = 0, gap = 0, lgap = 1, hrough = 0,
+           lrough = 1;
+    bool roundpipe = true;
+};
+
+struct SponNL {  // SponRad (src/Main/SponRad.cpp)
+    bool set = false;
+    int seed = 1234;
+    bool loss = false, spread = false;
+};
+
 struct Deck {
     std::string input_file, input_text, lattice_text, lattice_path;
     SetupNL setup;
     TimeNL time;
     EFieldNL efield;
+    WakeNL wake;
+    SponNL spon;
     bool matched = false;
     double mbetax = 0, malphax = 0, mbetay = 0, malphay = 0, mphix = 0, mphiy = 0, zmatch = 0;
     // generated beam (host copy of the loaded state) and field
@@ -187,7 +202,7 @@ struct Deck {
     std::vector<double> field;  // [ns][ng^2][2]
     int ngrid = 0;
     double gridmax = 0, flambda = 0;
-    bool have_beam = false, have_field = false;
+    bool have_beam = false, have_field = false, spon_applied = false;
     // device context (created by the first &track)
     g4mi_ctx *ctx = nullptr;
     int precision = G4MI_FP64;
@@ -732,6 +747,84 @@ bool do_importfield(Deck &dk, Namelist &nl, std::string &err) {
     dk.gridmax = gridmax;
     dk.flambda = reflen;
     dk.have_field = true;
+    return true;
+}
+
+// &wake (Wake::init, src/Main/Wake.cpp:48-133): the wake parameters; the
+// loss per slice is computed when a &track starts, from the beam's current
+// (Collective::update runs at the first step, src/Core/Collective.cpp:110-113).
+bool do_wake(Deck &dk, Namelist &nl, std::string &err) {
+    Keys k{nl, ""};
+    WakeNL &w = dk.wake;
+    w = WakeNL();
+    std::string loss, material, output;
+    bool transient = false;
+    double ztrans = 0;
+    k.str("loss", loss);
+    k.dbl("radius", w.radius);
+    k.dbl("conductivity", w.conductivity);
+    k.dbl("relaxation", w.relaxation);
+    k.boolean("roundpipe", w.roundpipe);
+    k.str("material", material);
+    k.dbl("gap", w.gap);
+    k.dbl("lgap", w.lgap);
+    k.dbl("hrough", w.hrough);
+    k.dbl("lrough", w.lrough);
+    k.boolean("transient", transient);
+    k.dbl("ztrans", ztrans);
+    k.str("output", output);
+    if (!k.finish()) {
+        err = k.err;
+        return false;
+    }
+    if (!loss.empty()) {
+        if (loss[0] == '@') {
+            err = "&wake: profile references (loss = @...) are not supported by this build";
+            return false;
+        }
+        char *end = nullptr;
+        w.loss = strtod(loss.c_str(), &end);
+        if (end == loss.c_str()) {
+            err = "&wake: loss is not a number";
+            return false;
+        }
+    }
+    if (!output.empty()) {
+        err = "&wake: output of the wake potentials is not supported by this build";
+        return false;
+    }
+    if (material == "CU" || material == "Cu" || material == "cu") {
+        w.conductivity = 5.813e7;
+        w.relaxation = 8.1e-6;
+    }
+    if (material == "AL" || material == "Al" || material == "al") {
+        w.conductivity = 3.571e7;
+        w.relaxation = 2.4e-6;
+    }
+    if (!(w.radius > 0) || !(w.lgap > 0) || !(w.lrough > 0)) {
+        err = "&wake: radius, lgap and lrough must be positive";
+        return false;
+    }
+    w.set = true;  // transient/ztrans do not change the loss (Collective.cpp:162-191)
+    return true;
+}
+
+// &sponrad (SponRad::init, src/Main/SponRad.cpp:20-37)
+bool do_sponrad(Deck &dk, Namelist &nl, std::string &err) {
+    Keys k{nl, ""};
+    SponNL &sp = dk.spon;
+    k.integer("seed", sp.seed);
+    k.boolean("doLoss", sp.loss);
+    k.boolean("doSpread", sp.spread);
+    if (!k.finish()) {
+        err = k.err;
+        return false;
+    }
+    sp.set = true;
+    if (dk.ctx && g4mi_set_incoherent(dk.ctx, sp.seed, dk.setup.nbins, sp.loss ? 1 : 0, sp.spread ? 1 : 0) != G4MI_OK) {
+        err = g4mi_last_error();
+        return false;
+    }
     return true;
 }
 
@@ -1296,6 +1389,27 @@ bool do_track(Deck &dk, Namelist &nl, int device, bool quiet, bool dry, std::str
             return false;
         }
     }
+    // the incoherent stream is seeded once (it runs on across &track
+    // elements); the wake loss follows the current beam
+    if (!dry && dk.spon.set && !dk.spon_applied) {
+        if (g4mi_set_incoherent(dk.ctx, dk.spon.seed, dk.setup.nbins, dk.spon.loss ? 1 : 0,
+                                dk.spon.spread ? 1 : 0) != G4MI_OK) {
+            err = g4mi_last_error();
+            return false;
+        }
+        dk.spon_applied = true;
+    }
+    if (dk.wake.set) {
+        const WakeNL &w = dk.wake;
+        std::vector<double> eloss(ns), ext(ns, w.loss);
+        if (g4mi_wake_loss(ns, dk.time.sample, dk.setup.lambda0, dk.current.data(), ext.data(), w.radius,
+                           w.roundpipe ? 1 : 0, w.conductivity, w.relaxation, w.gap, w.lgap, w.hrough, w.lrough,
+                           eloss.data(), nullptr, nullptr, nullptr) != G4MI_OK ||
+            g4mi_set_wake(dk.ctx, eloss.data()) != G4MI_OK) {
+            err = g4mi_last_error();
+            return false;
+        }
+    }
     // Track::init -> Field::initSolver per &track (src/Main/Track.cpp:107-112)
     if (g4mi_set_field_solver(dk.ctx, fft ? 1 : 0, filt ? 1 : 0, xc, yc, sig) != G4MI_OK) {
         err = g4mi_last_error();
@@ -1450,6 +1564,8 @@ int run_deck(const char *input_file, int device, int quiet, bool dry, std::strin
         else if (e == "&write") ok = do_write(dk, nl, err);
         else if (e == "&importbeam") ok = do_importbeam(dk, nl, err);
         else if (e == "&importfield") ok = do_importfield(dk, nl, err);
+        else if (e == "&wake") ok = do_wake(dk, nl, err);
+        else if (e == "&sponrad") ok = do_sponrad(dk, nl, err);
         else {
             err = "element " + e + " is not supported by this build";
             ok = false;

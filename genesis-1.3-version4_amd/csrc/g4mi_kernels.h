@@ -16,6 +16,12 @@ struct AdiFactors {
 void launch_transverse(hipStream_t s, long long n, const double *g, double *x, double *y,
                        double *px, double *py, const TransParams &p);
 void launch_r56(hipStream_t s, long long n, const double *g, double *th, double r56, double g0);
+// Energy kicks after the push (Beam::track, src/Core/Beam.cpp:105-106), in
+// the reference's order: gamma += beamlet_dg[boff[s] + (i - off[s]) / nbins]
+// (Incoherent::apply) when beamlet_dg is set, then gamma += slice_dg[s]
+// (Collective::apply) when slice_dg is set.  One workgroup per slice.
+void launch_kicks(hipStream_t s, int ns, const long long *off, double *g, const double *beamlet_dg,
+                  const long long *boff, int nbins, const double *slice_dg);
 void launch_slice_moments(hipStream_t s, int ns, const long long *off, const double *x,
                           const double *y, double *mom);
 void launch_slice_sizes(hipStream_t s, int ns, const long long *off, const double *cur,

@@ -1388,6 +1388,27 @@ void launch_transverse(hipStream_t s, long long n, const double *g, double *x, d
                        double *px, double *py, const TransParams &p) {
     if (n > 0) hipLaunchKernelGGL(k_transverse, dim3(nblk(n, 256)), dim3(256), 0, s, n, g, x, y, px, py, p);
 }
+__global__ void __launch_bounds__(256) k_kicks(const long long *__restrict__ off, double *__restrict__ g,
+                                                const double *__restrict__ bdg, const long long *__restrict__ boff,
+                                                int nbins, const double *__restrict__ sdg) {
+    const int s = blockIdx.x;
+    const long long a = off[s], b = off[s + 1];
+    const double ds = sdg ? sdg[s] : 0.0;
+    const long long b0 = bdg ? boff[s] : 0;
+    for (long long i = a + threadIdx.x; i < b; i += blockDim.x) {
+        double v = g[i];
+        if (bdg) v += bdg[b0 + (i - a) / nbins];
+        if (sdg) v += ds;
+        g[i] = v;
+    }
+}
+
+void launch_kicks(hipStream_t s, int ns, const long long *off, double *g, const double *beamlet_dg,
+                  const long long *boff, int nbins, const double *slice_dg) {
+    if (ns > 0 && (beamlet_dg || slice_dg))
+        hipLaunchKernelGGL(k_kicks, dim3(ns), dim3(256), 0, s, off, g, beamlet_dg, boff, nbins, slice_dg);
+}
+
 void launch_r56(hipStream_t s, long long n, const double *g, double *th, double r56, double g0) {
     if (n > 0) hipLaunchKernelGGL(k_r56, dim3(nblk(n, 256)), dim3(256), 0, s, n, g, th, r56, g0);
 }

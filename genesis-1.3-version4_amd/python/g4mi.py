@@ -85,6 +85,8 @@ def lib():
                                   C.c_double, C.c_double, C.c_double, C.c_int, C.c_int, C.c_int]
         L.g4mi_destroy.argtypes = [C.c_void_p]
         L.g4mi_set_efield.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_double]
+        L.g4mi_set_incoherent.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int]
+        L.g4mi_set_wake.argtypes = [C.c_void_p, _dp]
         L.g4mi_set_field_solver.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_double, C.c_double,
                                             C.c_double]
         L.g4mi_beam_upload.argtypes = [C.c_void_p] + [_dp] * 7
@@ -133,6 +135,8 @@ def lib():
         L.g4mi_fno_judge.argtypes = [C.c_void_p, _lp, _dp, _dp, _dp]
         L.g4mi_run_deck.argtypes = [C.c_char_p, C.c_int, C.c_int]
         L.g4mi_check_deck.argtypes = [C.c_char_p, C.c_char_p, C.c_int]
+        L.g4mi_wake_loss.argtypes = ([C.c_int, C.c_int, C.c_double, _dp, _dp, C.c_double, C.c_int]
+                                     + [C.c_double] * 6 + [_dp] * 4)
         L.g4mi_h5_read.argtypes = [C.c_char_p, C.c_char_p, _dp, C.c_longlong, _lp]
         L.g4mi_h5_read.restype = C.c_longlong
         _lib = L
@@ -214,6 +218,21 @@ class Context:
     def set_efield(self, longrange=False, nz=0, nphi=0, ngrid_r=100, rmax=0.0):
         _chk(lib().g4mi_set_efield(self.h, int(longrange), nz, nphi, ngrid_r, rmax),
              "g4mi_set_efield")
+
+    def set_incoherent(self, seed=1234, nbins=4, do_loss=False, do_spread=False):
+        """Beam::initIncoherent (src/Core/Incoherent.cpp:17-31), rank 0."""
+        _chk(lib().g4mi_set_incoherent(self.h, seed, nbins, int(do_loss), int(do_spread)),
+             "g4mi_set_incoherent")
+
+    def set_wake(self, eloss):
+        """Per-slice wake loss in eV/m (Collective::apply), or None."""
+        if eloss is None:
+            _chk(lib().g4mi_set_wake(self.h, None), "g4mi_set_wake")
+            return
+        e = np.ascontiguousarray(eloss, dtype=np.float64)
+        if e.shape != (self.ns,):
+            raise ValueError("eloss needs one value per slice")
+        _chk(lib().g4mi_set_wake(self.h, _p(e)), "g4mi_set_wake")
 
     def set_field_solver(self, fft=False, source_filter=False, xcut=1.0, ycut=1.0, sigmoid=1.0):
         """Field::initSolver (src/Core/Field.cpp:72-83): the ADI (fft=False)
@@ -448,6 +467,22 @@ class Fno:
 def run_deck(path, device=0, quiet=True):
     """Run a GENESIS 1.3 v4 input deck on the device (writes <rootname>.out.h5)."""
     _chk(lib().g4mi_run_deck(str(path).encode(), device, int(quiet)), "g4mi_run_deck")
+
+
+def wake_loss(current, ds, sample=1, loss=None, radius=2.5e-3, roundpipe=True, conductivity=0.0,
+              relaxation=0.0, gap=0.0, lgap=1.0, hrough=0.0, lrough=1.0):
+    """Per-slice wake loss [eV/m] for a current profile (A per slice) and
+    the single-particle wakes (resistive, geometric, roughness) on the
+    len(current) * sample bins of length ds (g4mi_wake_loss)."""
+    cur = np.ascontiguousarray(current, dtype=np.float64)
+    ns = len(cur)
+    ext = None if loss is None else np.ascontiguousarray(np.broadcast_to(loss, (ns,)), dtype=np.float64)
+    eloss = np.zeros(ns)
+    w = [np.zeros(ns * sample) for _ in range(3)]
+    _chk(lib().g4mi_wake_loss(ns, sample, ds, _p(cur), _p(ext), radius, int(roundpipe), conductivity,
+                              relaxation, gap, lgap, hrough, lrough, _p(eloss), *[_p(a) for a in w]),
+         "g4mi_wake_loss")
+    return eloss, {"resistive": w[0], "geometric": w[1], "roughness": w[2]}
 
 
 def check_deck(path):

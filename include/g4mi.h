@@ -119,6 +119,35 @@ int g4mi_beam_track(g4mi_ctx *ctx, const g4mi_step_params *st, double gammaref, 
  * (optionally filtered) source. */
 int g4mi_field_track(g4mi_ctx *ctx, const g4mi_step_params *st, double delz);
 
+/* Incoherent synchrotron radiation, Beam::initIncoherent ->
+ * Incoherent::init (src/Core/Incoherent.cpp:17-31) for rank 0, applied by
+ * Incoherent::apply (:35-82) after the push of every undulator step: one
+ * energy kick -dgamavg + dgamsig (2u - 1) per beamlet of `nbins`
+ * particles, u from the single ran2 stream seeded with round(1e9 ran2(seed)),
+ * drawn on the host in slice and particle order.  do_loss = do_spread = 0
+ * switches it off.  FP64 form only. */
+int g4mi_set_incoherent(g4mi_ctx *ctx, int seed, int nbins, int do_loss, int do_spread);
+
+/* Wakefield energy loss, Collective::apply (src/Core/Collective.cpp:107-128):
+ * eloss[nslice] in eV/m (Collective's wakeext + wakeint for each beam
+ * slice), added as eloss * delz / eev to every particle's gamma after the
+ * push of every step and reported as the `wakefield` diagnostic.  NULL
+ * removes it.  FP64 form only. */
+int g4mi_set_wake(g4mi_ctx *ctx, const double *eloss);
+
+/* Wake potentials and the loss they give each slice (host): Wake::init's
+ * single-particle wakes (src/Main/Wake.cpp:136-434: resistive wall of a
+ * round or flat chamber with AC conductivity, geometric wake of periodic
+ * gaps, surface roughness) on nslice*sample bins of length ds, convolved
+ * with the current profile as Collective::initWake/update
+ * (src/Core/Collective.cpp:36-196); eloss[nslice] = loss_ext (or 0) + the
+ * convolution, in eV/m, ready for g4mi_set_wake.  wres, wgeo, wrou
+ * (nslice*sample each, may be NULL) receive the single-particle wakes. */
+int g4mi_wake_loss(int nslice, int sample, double ds, const double *current, const double *loss_ext,
+                   double radius, int roundpipe, double conductivity, double relaxation, double gap,
+                   double lgap, double hrough, double lrough, double *eloss, double *wres, double *wgeo,
+                   double *wrou);
+
 /* Field::initSolver (src/Core/Field.cpp:72-83) with
  * FieldSolverFFT::initSourceFilter (src/Core/FieldSolverFFT.cpp:159-170):
  * fft = 0 the ADI solver (default), 1 the spectral solver; filter = 1 with

@@ -1245,10 +1245,79 @@ extern "C" void orc_slippage(orc_state *s, double slip) {
     }
 }
 
-/* Beam::track, src/Core/Beam.cpp:97-112 (no wakes, no incoherent SR). */
+struct Incoh {
+    Ran2 *r;
+    int loss, spread;
+};
+
+extern "C" void *orc_incoherent_new(int seed, int rank, int do_loss, int do_spread) {
+    Ran2 rseed((unsigned int)seed);
+    double val = 0;
+    for (int i = 0; i <= rank; i++) val = rseed.next();
+    val *= 1e9;
+    int locseed = (int)round(val);
+    Incoh *h = new Incoh;
+    h->r = new Ran2((unsigned int)locseed);
+    h->loss = do_loss;
+    h->spread = do_spread;
+    return h;
+}
+
+extern "C" void orc_incoherent_free(void *hv) {
+    Incoh *h = (Incoh *)hv;
+    if (!h) return;
+    delete h->r;
+    delete h;
+}
+
+/* Incoherent::apply, src/Core/Incoherent.cpp:35-82: one uniform draw per
+ * beamlet (every nbins-th particle), in slice and particle order. */
+extern "C" void orc_incoherent_apply(orc_state *s, const orc_step *st, double gamma0, double delz) {
+    Incoh *h = (Incoh *)s->incoh;
+    if (!h) return;
+    if (!(st->aw > 0)) return;
+    if ((!h->loss) && (!h->spread)) return;
+    double gam0 = gamma0;
+    double awz = st->aw;
+    double xkw0 = st->ku;
+    double dgamsig = 1.015e-27 * xkw0 * xkw0 * awz * awz;
+    if (st->helical)
+        dgamsig *= 1.42 * awz + 1. / (1. + 1.5 * awz + 0.95 * awz * awz);
+    else
+        dgamsig *= 1.697 * awz + 1. / (1. + 1.88 * awz + 0.8 * awz * awz);
+    if (!h->spread) dgamsig = 0;
+    dgamsig = sqrt(dgamsig * gam0 * gam0 * gam0 * gam0 * xkw0 * delz) * sqrt(3.);
+    double dgamavg = xkw0 * gam0 * awz;
+    if (!h->loss) dgamavg = 0;
+    dgamavg = dgamavg * dgamavg * 1.88e-15 * delz;
+    int nbins = s->nbins > 0 ? s->nbins : 1;
+    double dg = 0;
+    for (int is = 0; is < s->nslice; is++) {
+        long long n0 = s->off[is], npart = s->off[is + 1] - n0;
+        for (long long ip = 0; ip < npart; ip++) {
+            if ((ip % nbins) == 0) dg = -dgamavg + dgamsig * (2 * h->r->next() - 1);
+            s->gamma[n0 + ip] += dg;
+        }
+    }
+}
+
+/* Collective::apply, src/Core/Collective.cpp:107-128 (the wake potential
+ * is already folded into s->wake). */
+static void wake_apply(orc_state *s, double delz) {
+    if (!s->wake) return;
+    for (int ic = 0; ic < s->nslice; ic++) {
+        s->eloss[ic] = s->wake[ic];
+        double dg = s->eloss[ic] * delz / k_eev;
+        for (long long ip = s->off[ic]; ip < s->off[ic + 1]; ip++) s->gamma[ip] += dg;
+    }
+}
+
+/* Beam::track, src/Core/Beam.cpp:97-112. */
 extern "C" void orc_beam_track(orc_state *s, const orc_step *st, double gamma0, double delz) {
     orc_track_transverse(s, st, gamma0, delz * 0.5, 0);
     orc_advance(s, st, gamma0, delz);
+    orc_incoherent_apply(s, st, gamma0, delz);
+    wake_apply(s, delz);
     apply_r56(s, st, gamma0);
     orc_track_transverse(s, st, gamma0, delz * 0.5, 1);
 }

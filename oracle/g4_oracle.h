@@ -56,6 +56,13 @@ typedef struct {
      * filter (initSourceFilter: sigmoid of width fft_sig at xcut/ycut) */
     int fft_solver, fft_filter;
     double fft_xcut, fft_ycut, fft_sig;
+    /* incoherent synchrotron radiation (Incoherent, orc_incoherent_new) or
+     * NULL, and the beamlet size its draws follow */
+    void *incoh;
+    int nbins;
+    /* wake loss per slice [eV/m] (Collective's wakeext + wakeint) or NULL;
+     * applied after the push and copied to eloss */
+    const double *wake;
 } orc_state;
 
 /* Per-slice input of the quiet loader (QuietLoading.h BeamSlice). */
@@ -122,6 +129,12 @@ void orc_adi_slice(int ngrid, const double *c, const double *cbet,
                    const double *cwet, double cstep_im, const double *src,
                    double *u, double *r);
 void orc_field_advance(orc_state *s, const orc_step *st, double delz);
+/* Incoherent::init (src/Core/Incoherent.cpp:17-31): the energy-kick
+ * stream of rank `rank` for seed `seed`; apply (:35-82) after the push of
+ * every undulator step of orc_beam_track while s->incoh is set. */
+void *orc_incoherent_new(int seed, int rank, int do_loss, int do_spread);
+void orc_incoherent_free(void *h);
+void orc_incoherent_apply(orc_state *s, const orc_step *st, double gamma0, double delz);
 /* FieldSolverFFT: unscaled 2-D DFT in place (sign -1 forward, +1 backward)
  * and the K2 / sigmoid tables of FieldSolverFFT::init. */
 void orc_dft2(int n, double *data, int sign);

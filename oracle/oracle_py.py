@@ -42,7 +42,8 @@ class State(C.Structure):
         ("reflength", C.c_double), ("sample", C.c_double), ("timerun", C.c_int),
         ("periodic", C.c_int), ("ndiag_threads", C.c_int),
         ("fft_solver", C.c_int), ("fft_filter", C.c_int),
-        ("fft_xcut", C.c_double), ("fft_ycut", C.c_double), ("fft_sig", C.c_double)]
+        ("fft_xcut", C.c_double), ("fft_ycut", C.c_double), ("fft_sig", C.c_double),
+        ("incoh", C.c_void_p), ("nbins", C.c_int), ("wake", _dp)]
 
 
 class SliceParams(C.Structure):
@@ -87,6 +88,10 @@ def lib():
         L.orc_shotnoise_new.argtypes = [C.c_int, C.c_int]
         L.orc_shotnoise_new.restype = C.c_void_p
         L.orc_shotnoise_free.argtypes = [C.c_void_p]
+        L.orc_incoherent_new.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int]
+        L.orc_incoherent_new.restype = C.c_void_p
+        L.orc_incoherent_free.argtypes = [C.c_void_p]
+        L.orc_incoherent_apply.argtypes = [C.POINTER(State), C.POINTER(Step), C.c_double, C.c_double]
         L.orc_shotnoise.argtypes = [C.c_void_p, _dp, C.c_int, C.c_int, C.c_double]
         L.orc_load_gauss.argtypes = [_dp, C.c_int] + [C.c_double] * 8
         L.orc_unroll.argtypes = [C.c_int, _ip, _dp, _dp, _dp, C.c_double, C.c_double,

@@ -4,6 +4,7 @@ the oracle and the reference's own known answer (GPU).
 
 The decks are the reference's Example1 (examples/Example1-SteadyState/
 Example1.in and Example1.lat), written out here as data, and variants."""
+import ctypes as C
 import math
 import os
 
@@ -171,7 +172,7 @@ def test_time_window_slices(tmp_path):
     ("&end\n", "outside"),
     ("&setup\nrootname\n&end\n", "invalid format"),
     ("&setup\nlattice={lat}\nbeamline=FEL\nfoo=1\n&end\n", "unknown elements in &setup"),
-    ("&setup\nlattice={lat}\nbeamline=FEL\n&end\n&wake\n&end\n", "not supported"),
+    ("&setup\nlattice={lat}\nbeamline=FEL\n&end\n&profile_gauss\n&end\n", "not supported"),
     ("&setup\nlattice={lat}\nbeamline=FEL\n&end\n&beam\ncurrent=@prof\n&end\n", "profile"),
     ("&setup\nlattice={lat}\nbeamline=FEL\n&end\n&field\nharm=3\n&end\n", "harm"),
     ("&setup\nlattice={lat}\nbeamline=FEL\none4one=true\n&end\n", "one4one"),
@@ -617,3 +618,65 @@ def test_restart_from_dumps(tmp_path):
     eb = float(np.max(np.abs(bb / ba[k:] - 1)))
     print(f"  restart after {k} steps: P(z) rel err {ep:.3e}, |b(z)| {eb:.3e}", flush=True)
     assert ep < 1e-10 and eb < 1e-10
+
+
+def test_wake_and_sponrad_elements_check(tmp_path):
+    """&wake and &sponrad are accepted with the reference's keys; profile
+    references and wake output files are refused by name."""
+    deck = EX1_IN.replace("&track\n&end", "&wake\nmaterial = Cu\nradius = 3e-3\ngap = 1e-2\nloss = -1e3\n&end\n"
+                          "&sponrad\nseed = 5\ndoLoss = 1\ndoSpread = 1\n&end\n&track\n&end")
+    path = _write(tmp_path, deck=deck, root=str(tmp_path / "w"))
+    g4mi.check_deck(path)
+    for bad, msg in (("loss = -1e3", "profile"), ("gap = 1e-2", "output")):
+        repl = {"loss = -1e3": "loss = @prof", "gap = 1e-2": "output = w.h5"}[bad]
+        path = _write(tmp_path, deck=deck.replace(bad, repl), root=str(tmp_path / "w"))
+        with pytest.raises(g4mi.G4miError) as e:
+            g4mi.check_deck(path)
+        assert msg in str(e.value)
+
+
+@pytest.mark.gpu
+def test_example1_deck_with_wake_and_sponrad(tmp_path):
+    """Example1 on 63^2 with &sponrad (loss and spread) and a copper-pipe
+    &wake with periodic gaps and an external loss: P(z), |b(z)| and
+    <gamma>(z) of every record against the oracle with the same stream and
+    the same per-slice loss, within 1e-10; /Beam/wakefield is the loss."""
+    print("deck runner: Example1 on 63^2 with incoherent SR and wakes", flush=True)
+    deck = EX1_IN.replace("ngrid=255", "ngrid=63").replace(
+        "&track\n&end", "&wake\nmaterial = Cu\nradius = 3e-3\ngap = 1e-2\nlgap = 0.5\nloss = -1e3\n&end\n"
+        "&sponrad\nseed = 5\ndoLoss = 1\ndoSpread = 1\n&end\n&track\n&end")
+    path = _write(tmp_path, deck=deck, root=str(tmp_path / "ws"))
+    s = g4mi.check_deck(path)
+    g4mi.run_deck(path, device=0, quiet=True)
+    out = str(tmp_path / "ws.out.h5")
+    p = g4mi.h5_read(out, "Field/power")[:, 0]
+    b = g4mi.h5_read(out, "Beam/bunching")[:, 0]
+    en = g4mi.h5_read(out, "Beam/energy")[:, 0]
+    wf = g4mi.h5_read(out, "Beam/wakefield")[:, 0]
+    eloss, _ = g4mi.wake_loss([3000.0], 1e-10, loss=-1e3, radius=3e-3, conductivity=5.813e7,
+                              relaxation=8.1e-6, gap=1e-2, lgap=0.5)
+    assert wf[0] == 0 and np.all(wf[1:] == eloss[0])
+    els = []
+    for k1 in (2.0, -2.0) * 6:
+        els += [("undulator", dict(aw=0.84853, lambdau=0.015, nwig=266, helical=True)),
+                ("drift", dict(l=0.44)), ("quadrupole", dict(l=0.08, k1=k1)), ("drift", dict(l=0.24))]
+    st = O.unroll(els, 0.045, 1e-10, 11357.82)
+    sim = O.Sim([8192], 63, 2e-4, 1e-10, timerun=False, threads=16)
+    O.load_beam(sim, dict(gamma=11357.82, delgam=1.0, current=3000.0, ex=4e-7, ey=4e-7,
+                          betax=s["betax"], betay=s["betay"], alphax=s["alphax"], alphay=s["alphay"]),
+                None, 8, None, lambda0=1e-10)
+    O.load_gauss_all(sim, 5e3, 30e-6, 1e-10)
+    h = O.lib().orc_incoherent_new(5, 0, 1, 1)
+    sim.s.incoh = h
+    sim.s.nbins = 8
+    sim.s.wake = eloss.ctypes.data_as(C.POINTER(C.c_double))
+    r = sim.run(st, 11357.82, record=("power", "bunching", "energy"))
+    O.lib().orc_incoherent_free(h)
+    sim.s.incoh = None
+    ep = float(np.max(np.abs(p / r["power"][:, 0] - 1)))
+    mask = r["bunching"][:, 0] > 1e-6
+    eb = float(np.max(np.abs(b[mask] / r["bunching"][mask, 0] - 1)))
+    ee = float(np.max(np.abs(en / r["energy"][:, 0] - 1)))
+    print(f"  P_end {p[-1]:.6e} W, <gamma> change {en[-1] - en[0]:.4f}, wake loss {eloss[0]:.4e} eV/m; "
+          f"P(z) rel err {ep:.3e}, |b(z)| {eb:.3e}, <gamma> {ee:.3e}", flush=True)
+    assert ep < 1e-10 and eb < 1e-10 and ee < 1e-13

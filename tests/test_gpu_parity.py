@@ -273,3 +273,58 @@ def test_config1_full_size_first_steps():
     eb = H.rel_err(g["bunching"], r["bunching"])
     _log(f"  P rel err {ep:.3e}  |b| rel err {eb:.3e}")
     assert ep < 1e-10 and eb < 1e-10
+
+
+def test_incoherent_and_wake_kicks(st):
+    """Incoherent SR (loss and spread, beamlets of 8 over slices of 512,
+    508, 512 particles: one ragged beamlet) and a per-slice wake loss after
+    the push, five undulator steps and a drift: the device adds the host
+    stream's kicks and eloss dz / eev exactly as the oracle (Beam.cpp:105-106);
+    the wakefield diagnostic is 0 before the first step and eloss after."""
+    _log("incoherent SR + wake kicks: 3 slices, 5 undulator steps + drift")
+    sim = H.make_state([512, 508, 512], seed_power=5e7)
+    ctx = H.to_gpu(sim)
+    h = O.lib().orc_incoherent_new(4321, 0, 1, 1)
+    sim.s.incoh = h
+    sim.s.nbins = 8
+    w = np.array([-2.0e6, 5.0e5, 1.0e6])
+    sim.s.wake = w.ctypes.data_as(C.POINTER(C.c_double))
+    ctx.set_incoherent(seed=4321, nbins=8, do_loss=True, do_spread=True)
+    ctx.set_wake(w)
+    assert np.all(ctx.diag()["wakefield"] == 0)
+    gst = H.gsteps(st)
+    for idx in (0, 1, 2, 3, 4, 125):
+        O.lib().orc_beam_track(C.byref(sim.s), C.byref(st[idx]), cases.GAMMA0, st[idx].dz)
+        ctx.beam_track(gst[idx], cases.GAMMA0, st[idx].dz)
+    O.lib().orc_incoherent_free(h)
+    sim.s.incoh = None
+    wk = _beam_close(ctx, sim, 1e-13)
+    _log(f"  worst rel diff {wk}")
+    assert np.array_equal(ctx.diag()["wakefield"], w)
+    ctx.close()
+
+
+def test_incoherent_sase_end_to_end():
+    """Whole steps with incoherent SR and wakes: 8 slices x 1024 SASE on
+    63^2 over 200 steps, P(z) and |b(z)| within 1e-10 of the oracle."""
+    _log("SASE with incoherent SR and wakes, 8 slices, 200 steps")
+    st = H.steps()
+    sim = H.make_state([1024] * 8, ngrid=63, gridmax=0.3e-3, threads=16)
+    ctx = H.to_gpu(sim)
+    h = O.lib().orc_incoherent_new(99, 0, 1, 1)
+    sim.s.incoh = h
+    sim.s.nbins = 4
+    w = np.linspace(-3e5, 3e5, 8)
+    sim.s.wake = w.ctypes.data_as(C.POINTER(C.c_double))
+    ctx.set_incoherent(seed=99, nbins=4, do_loss=True, do_spread=True)
+    ctx.set_wake(w)
+    g = ctx.run(H.gsteps(st)[:200], cases.GAMMA0, record=("power", "bunching", "energy"))
+    r = sim.run(st, cases.GAMMA0, record=("power", "bunching", "energy"), nsteps=200)
+    O.lib().orc_incoherent_free(h)
+    sim.s.incoh = None
+    ep = H.rel_err(g["power"][1:], r["power"][1:])
+    eb = H.rel_err(g["bunching"], r["bunching"])
+    ee = H.rel_err(g["energy"], r["energy"])
+    _log(f"  P(z) rel err {ep:.3e}  |b(z)| {eb:.3e}  <gamma> {ee:.3e}  "
+         f"<gamma> change {g['energy'][-1].mean() - g['energy'][0].mean():.4f}")
+    assert ep < 1e-10 and eb < 1e-10 and ee < 1e-12

@@ -521,3 +521,74 @@ def test_fft_source_filter():
     r = np.sqrt(q[:, None] ** 2 + q[None, :] ** 2) / 0.25
     assert np.max(np.abs(lp[r > 1.02])) < 1e-12 * np.max(np.abs(lp))
     assert np.max(np.abs(lp[r < 0.98] - 2 * np.fft.fft2(s)[r < 0.98])) < 1e-10 * np.max(np.abs(lp))
+
+
+# ------------------------------------------------------------ incoherent SR and wake kicks
+def _kick_state(nbins=4, counts=(64, 61, 64)):
+    sim = O.Sim(list(counts), 15, 0.5e-3, 1e-10, timerun=False)
+    sim.gamma[:] = 11357.82
+    sim.s.nbins = nbins
+    return sim
+
+
+def test_incoherent_loss_is_the_radiated_power():
+    """Loss only: every particle loses dgamavg = 1.88e-15 (ku gamma0 aw)^2 dz,
+    which is the classical radiation loss (2/3) r_e gamma^2 ku^2 aw^2 dz to
+    the three digits of the reference's constant."""
+    sim = _kick_state()
+    h = O.lib().orc_incoherent_new(1234, 0, 1, 0)
+    sim.s.incoh = h
+    st = O.Step(aw=1.7, ku=2 * np.pi / 0.015, helical=0, dz=0.015)
+    O.lib().orc_incoherent_apply(C.byref(sim.s), C.byref(st), 11357.82, 0.015)
+    O.lib().orc_incoherent_free(h)
+    d = sim.gamma - 11357.82
+    re = 2.8179403262e-15
+    ref = 2.0 / 3.0 * re * (11357.82 * st.ku * 1.7) ** 2 * 0.015
+    assert np.allclose(d, d[0], rtol=0, atol=1e-12) and abs(-d[0] / ref - 1) < 1e-3
+    st.aw = 0.0  # outside undulators nothing happens
+    g = sim.gamma.copy()
+    sim.s.incoh = O.lib().orc_incoherent_new(1234, 0, 1, 1)
+    O.lib().orc_incoherent_apply(C.byref(sim.s), C.byref(st), 11357.82, 0.015)
+    O.lib().orc_incoherent_free(sim.s.incoh)
+    assert np.array_equal(g, sim.gamma)
+
+
+def test_incoherent_spread_draws_and_beamlets():
+    """Spread only: the kicks are dgamsig (2u - 1) with u the ran2 stream
+    seeded by round(1e9 ran2(seed)), one per beamlet of nbins particles in
+    slice order (a slice of 61 particles has 16 beamlets); their variance
+    is the closed form dgamsig^2 / 3 = c gamma^4 ku dz."""
+    nb = 4
+    counts = (64, 61, 64)
+    sim = _kick_state(nb, counts)
+    h = O.lib().orc_incoherent_new(77, 0, 0, 1)
+    sim.s.incoh = h
+    st = O.Step(aw=1.7, ku=2 * np.pi / 0.015, helical=0, dz=0.015)
+    O.lib().orc_incoherent_apply(C.byref(sim.s), C.byref(st), 11357.82, 0.015)
+    O.lib().orc_incoherent_free(h)
+    seed0 = np.zeros(1)
+    O.lib().orc_ran2(77, 1, dp(seed0))
+    nbl = sum((n + nb - 1) // nb for n in counts)
+    u = np.zeros(nbl)
+    O.lib().orc_ran2(int(round(seed0[0] * 1e9)), nbl, dp(u))
+    aw, ku, g0 = 1.7, st.ku, 11357.82
+    c = 1.015e-27 * ku * ku * aw * aw * (1.697 * aw + 1. / (1. + 1.88 * aw + 0.8 * aw * aw))
+    sig = math.sqrt(c * g0 ** 4 * ku * 0.015) * math.sqrt(3.)
+    kicks = []
+    for n in counts:
+        kicks += [(b, min(nb, n - b * nb)) for b in range((n + nb - 1) // nb)]
+    expect = np.concatenate([np.full(m, sig * (2 * u[i] - 1)) for i, (_, m) in enumerate(kicks)])
+    np.testing.assert_allclose(sim.gamma - g0, expect, rtol=0, atol=1e-11)
+    assert abs(np.var(sig * (2 * u - 1)) / (c * g0 ** 4 * ku * 0.015) - 1) < 0.35
+
+
+def test_wake_kick_is_eloss_times_dz():
+    sim = _kick_state()
+    w = np.array([-1.0e6, 2.5e5, 0.0])
+    sim.s.wake = w.ctypes.data_as(C.POINTER(C.c_double))
+    st = O.Step(dz=0.05)
+    O.lib().orc_beam_track(C.byref(sim.s), C.byref(st), 11357.82, 0.05)
+    for i in range(3):
+        a, b = int(sim.off[i]), int(sim.off[i + 1])
+        np.testing.assert_allclose(sim.gamma[a:b] - 11357.82, w[i] * 0.05 / 510998.95069, rtol=1e-9, atol=1e-12)
+    assert np.array_equal(sim.eloss, w)
