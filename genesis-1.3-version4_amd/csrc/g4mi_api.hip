@@ -18,6 +18,7 @@
 #include "g4mi_host.h"
 #include "g4mi_kernels.h"
 #include "g4mi_fft.h"
+#include "g4mi_sort.h"
 #include "g4mi_fno_api.h"
 #include "g4mi_mixed.h"
 
@@ -45,7 +46,7 @@ struct Ctx {
     double2 *d_field = nullptr, *d_work = nullptr, *d_src = nullptr;
     int4 *d_box = nullptr;
     double2 *d_fac = nullptr;  // c | cbet | cwet, ngrid each
-    int nblk = 0;
+    int nblk = 0, blk_cap = 0;
     int *d_blk_slice = nullptr;
     long long *d_blk_begin = nullptr;
     double *d_diag_b = nullptr, *d_diag_f = nullptr;
@@ -77,6 +78,8 @@ struct Ctx {
     // spectral solver (FieldSolverFFT) instead of the ADI, with the source
     // filter's settings; the propagator matrix is cached per delz
     int fft = 0, fft_filter = 0;
+    int fft_variant = -1;  // complex GEMM variant chosen by the tuner (-1: not tuned yet)
+    float fft_ms[kCgemmVariants] = {-1.f, -1.f, -1.f, -1.f};
     double fft_xc = 1, fft_yc = 1, fft_sig = 1;
     bool fft_ready = false;
     double fft_delz = 0;
@@ -94,6 +97,10 @@ struct Ctx {
     // wake loss per slice [eV/m] (Collective: wakeext + wakeint) or unset
     double *d_wake = nullptr, *d_wake_dg = nullptr, *d_wake_buf = nullptr;
     std::vector<double> h_wake, wake_applied;
+    // one4one beams: Beam::sort at sort markers, with its work buffers
+    bool one4one = false;
+    SortWork sw{};
+    long long *d_off_new = nullptr;
     double wake_dz = 0;
     // timing: kEv events per step, recorded into a pool and folded lazily
     static constexpr int kEv = 9;
@@ -452,6 +459,48 @@ static int tune_adi(Ctx *c, const AdiFactors &f) {
 // FieldSolverFFT::FFT for every slice (src/Core/FieldSolverFFT.cpp:62-119)
 // after the deposit: E' = M E M^T + 2 S, or with the source filter
 // E' = M E M^T + 2 IDFT(sigmoid DFT(S)) / n^2, as complex FP64 GEMMs.
+// Tuner of the complex GEMM: each variant runs the row product E M^T of the
+// current field into the work buffer (which the step overwrites anyway)
+// and the fastest is kept for both products of every later step.
+static int tune_fft(Ctx *c) {
+    hipEvent_t e0, e1;
+    G4_HIP(hipEventCreate(&e0));
+    G4_HIP(hipEventCreate(&e1));
+    CgemmArgs g{};
+    g.ng = c->ng;
+    g.ns = c->ns;
+    g.nrows = (long long)c->ns * c->ng;
+    g.scale = 1.0;
+    g.a = c->d_field;
+    g.mat = c->d_fft_m;
+    g.c = c->d_work;
+    int best = 0;
+    float best_ms = 1e30f;
+    for (int v = 0; v < kCgemmVariants; v++) {
+        g.variant = v;
+        G4_TRY(launch_cgemm_rows(c->stream, g));  // warm-up
+        float b = 1e30f;
+        for (int rep = 0; rep < 3; rep++) {
+            G4_HIP(hipEventRecord(e0, c->stream));
+            G4_TRY(launch_cgemm_rows(c->stream, g));
+            G4_HIP(hipEventRecord(e1, c->stream));
+            G4_HIP(hipEventSynchronize(e1));
+            float ms = 0;
+            G4_HIP(hipEventElapsedTime(&ms, e0, e1));
+            b = std::min(b, ms);
+        }
+        c->fft_ms[v] = b;
+        if (b < best_ms) {
+            best_ms = b;
+            best = v;
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    c->fft_variant = best;
+    return G4MI_OK;
+}
+
 static int fft_field_step(Ctx *c, double delz) {
     const int ng = c->ng;
     const size_t nn = (size_t)ng * ng;
@@ -465,12 +514,14 @@ static int fft_field_step(Ctx *c, double delz) {
         c->fft_delz = delz;
         c->fft_ready = true;
     }
+    if (c->fft_variant < 0) G4_TRY(tune_fft(c));
     CgemmArgs g{};
     g.ng = ng;
     g.ns = c->ns;
     g.nrows = (long long)c->ns * ng;
     g.box = c->d_box;
     g.scale = 1.0;
+    g.variant = c->fft_variant;
     if (c->fft_filter) {
         g.a = c->d_src;  // DFT of the source planes: U = S W^T, V = sigmoid (W U)
         g.mask_a = 1;
@@ -853,10 +904,70 @@ static int step_end_timing(Ctx *c) {
     return G4MI_OK;
 }
 
+// Beam::sort (src/Core/Beam.cpp:81-94) of a one4one beam: migration on the
+// device, then the host mirror follows the new slice offsets: currents
+// np ce / slicelength, the push block table and the incoherent beamlets.
+static int sort_beam(Ctx *c) {
+    const size_t n = (size_t)std::max(1LL, c->ntot);
+    if (!c->sw.keys) {
+        SortWork &w = c->sw;
+        G4_TRY(dalloc(c, &w.keys, n));
+        G4_TRY(dalloc(c, &w.keys_out, n));
+        G4_TRY(dalloc(c, &w.vals, n));
+        G4_TRY(dalloc(c, &w.vals_out, n));
+        G4_TRY(dalloc(c, &w.th_new, n));
+        for (double **p : {&w.g, &w.th, &w.x, &w.y, &w.px, &w.py}) G4_TRY(dalloc(c, p, n));
+        G4_TRY(sort_work_bytes(c->ns, c->ntot, &w.temp_bytes));
+        G4_TRY(dalloc(c, (char **)&w.temp, std::max<size_t>(1, w.temp_bytes)));
+        G4_TRY(dalloc(c, &c->d_off_new, (size_t)c->ns + 1));
+    }
+    const double slen = 4. * asin(1.) * c->slicelength / c->lambda0;  // Beam::initSorting (:75-79)
+    G4_TRY(launch_sort_one4one(c->stream, c->ns, c->ntot, c->d_off, slen, c->d_g, c->d_th, c->d_x, c->d_y,
+                               c->d_px, c->d_py, c->sw, c->d_off_new));
+    std::swap(c->d_g, c->sw.g);
+    std::swap(c->d_th, c->sw.th);
+    std::swap(c->d_x, c->sw.x);
+    std::swap(c->d_y, c->sw.y);
+    std::swap(c->d_px, c->sw.px);
+    std::swap(c->d_py, c->sw.py);
+    std::swap(c->d_off, c->d_off_new);
+    G4_HIP(hipMemcpyAsync(c->off.data(), c->d_off, (c->ns + 1) * sizeof(long long), hipMemcpyDeviceToHost,
+                          c->stream));
+    G4_HIP(hipStreamSynchronize(c->stream));
+    c->ntot = c->off[c->ns];
+    c->h_cur.assign(c->ns, 0.0);
+    for (int s = 0; s < c->ns; s++) c->h_cur[s] = (double)(c->off[s + 1] - c->off[s]) * kCe / c->slicelength;
+    std::vector<int> bs;
+    std::vector<long long> bb;
+    build_block_table(c->off, bs, bb);
+    if ((int)bs.size() > c->blk_cap) {
+        set_error("sort: block table capacity exceeded");
+        return G4MI_E_STATE;
+    }
+    c->nblk = (int)bs.size();
+    G4_HIP(hipMemcpyAsync(c->d_cur, c->h_cur.data(), c->ns * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    if (c->nblk) {
+        G4_HIP(hipMemcpyAsync(c->d_blk_slice, bs.data(), bs.size() * sizeof(int), hipMemcpyHostToDevice, c->stream));
+        G4_HIP(hipMemcpyAsync(c->d_blk_begin, bb.data(), bb.size() * sizeof(long long), hipMemcpyHostToDevice,
+                              c->stream));
+    }
+    if (c->incoh) {
+        std::vector<long long> boff(c->ns + 1, 0);
+        for (int s = 0; s < c->ns; s++)
+            boff[s + 1] = boff[s] + (c->off[s + 1] - c->off[s] + c->incoh_nbins - 1) / c->incoh_nbins;
+        c->incoh_nb = boff[c->ns];
+        G4_HIP(hipMemcpyAsync(c->d_boff, boff.data(), boff.size() * sizeof(long long), hipMemcpyHostToDevice,
+                              c->stream));
+    }
+    G4_HIP(hipStreamSynchronize(c->stream));  // the host vectors above go out of scope
+    return G4MI_OK;
+}
+
 static int step(Ctx *c, const g4mi_step_params &st, double gamma0) {
     const double delz = st.dz;
     G4_TRY(step_begin_timing(c));
     G4_TRY(beam_track(c, st, gamma0, delz));
+    if ((st.marker & 4) && c->one4one) G4_TRY(sort_beam(c));  // Gencore.cpp:124-134
     G4_TRY(field_track(c, st, delz));
     G4_TRY(slippage(c, st.slip));
     return G4MI_OK;
@@ -1543,7 +1654,9 @@ int g4mi_create(g4mi_ctx **out, int device, int nslice, const long long *off, in
     std::vector<long long> bb;
     build_block_table(c->off, bs, bb);
     c->nblk = (int)bs.size();
-    if ((rc = dalloc(c, &c->d_blk_slice, bs.size())) || (rc = dalloc(c, &c->d_blk_begin, bb.size())))
+    // capacity for any redistribution of the particles over the slices
+    c->blk_cap = (int)(ns + (size_t)c->ntot / 256 + 1);
+    if ((rc = dalloc(c, &c->d_blk_slice, (size_t)c->blk_cap)) || (rc = dalloc(c, &c->d_blk_begin, (size_t)c->blk_cap)))
         return fail(rc);
     if (hipMemcpy(c->d_off, off, (ns + 1) * sizeof(long long), hipMemcpyHostToDevice) != hipSuccess ||
         (c->nblk && hipMemcpy(c->d_blk_slice, bs.data(), bs.size() * sizeof(int), hipMemcpyHostToDevice) != hipSuccess) ||
@@ -1608,13 +1721,61 @@ int g4mi_set_incoherent(g4mi_ctx *h, int seed, int nbins, int do_loss, int do_sp
     c->incoh_nbins = nbins;
     c->incoh_nb = boff[c->ns];
     if (!c->d_bdg) {
+        // capacity for any redistribution of the particles (one4one sorting)
+        const size_t cap = (size_t)(c->ntot / nbins + c->ns + 1);
         G4_TRY(dalloc(c, &c->d_boff, (size_t)c->ns + 1));
-        G4_TRY(dalloc(c, &c->d_bdg, (size_t)std::max(1LL, c->incoh_nb)));
-        G4_HIP(hipHostMalloc((void **)&c->h_bdg, (size_t)std::max(1LL, c->incoh_nb) * sizeof(double)));
+        G4_TRY(dalloc(c, &c->d_bdg, cap));
+        G4_HIP(hipHostMalloc((void **)&c->h_bdg, cap * sizeof(double)));
         G4_HIP(hipEventCreateWithFlags(&c->incoh_ev, hipEventDisableTiming));
         G4_HIP(hipEventRecord(c->incoh_ev, c->stream));
     }
     G4_HIP(hipMemcpy(c->d_boff, boff.data(), boff.size() * sizeof(long long), hipMemcpyHostToDevice));
+    return G4MI_OK;
+}
+
+int g4mi_set_sorting(g4mi_ctx *h, int one4one) {
+    if (!h) {
+        set_error("g4mi_set_sorting: null context");
+        return G4MI_E_ARG;
+    }
+    Ctx *c = &h->c;
+    if (one4one && c->precision != G4MI_FP64) {
+        set_error("g4mi_set_sorting: one4one beams run in the FP64 form only");
+        return G4MI_E_ARG;
+    }
+    c->one4one = one4one != 0;
+    return G4MI_OK;
+}
+
+int g4mi_sort(g4mi_ctx *h, long long *nkept) {
+    if (!h) {
+        set_error("g4mi_sort: null context");
+        return G4MI_E_ARG;
+    }
+    Ctx *c = &h->c;
+    if (!c->one4one) {  // Beam::sort does nothing for beams that are not one4one
+        if (nkept) *nkept = c->ntot;
+        return G4MI_OK;
+    }
+    G4_TRY(sort_beam(c));
+    if (nkept) *nkept = c->ntot;
+    return G4MI_OK;
+}
+
+int g4mi_get_slices(g4mi_ctx *h, long long *off, double *current) {
+    if (!h) {
+        set_error("g4mi_get_slices: null context");
+        return G4MI_E_ARG;
+    }
+    Ctx *c = &h->c;
+    if (off) std::copy(c->off.begin(), c->off.end(), off);
+    if (current) {
+        if ((int)c->h_cur.size() != c->ns) {
+            set_error("g4mi_get_slices: no beam uploaded");
+            return G4MI_E_STATE;
+        }
+        std::copy(c->h_cur.begin(), c->h_cur.end(), current);
+    }
     return G4MI_OK;
 }
 
@@ -1834,6 +1995,10 @@ int g4mi_set_schedule(g4mi_ctx *h, const char *name, int value) {
         c->adi_kernel = value;
         return G4MI_OK;
     }
+    if (std::string(name) == "fft_variant" && value >= -1 && value < kCgemmVariants) {
+        c->fft_variant = value;  // -1: tune again at the next spectral step
+        return G4MI_OK;
+    }
     if (std::string(name) == "adi_refine" && (value == 0 || value == 1)) {
         c->m_refine = value != 0;
         return G4MI_OK;
@@ -1870,8 +2035,10 @@ int g4mi_get_schedule(g4mi_ctx *h, char *buf, int len) {
     }
     if (c->fft) {
         snprintf(tmp, sizeof(tmp),
-                 "{\"field_solver\": \"fft\", \"source_filter\": %d, \"gemm_tile\": %d}",
-                 c->fft_filter, cgemm_tile(c->ng));
+                 "{\"field_solver\": \"fft\", \"source_filter\": %d, \"gemm_tile\": %d, \"gemm_variant\": %d, "
+                 "\"gemm_tuned_ms\": [%.4f, %.4f, %.4f, %.4f]}",
+                 c->fft_filter, cgemm_tile(c->ng), c->fft_variant, c->fft_ms[0], c->fft_ms[1], c->fft_ms[2],
+                 c->fft_ms[3]);
         snprintf(buf, (size_t)len, "%s", tmp);
         return G4MI_OK;
     }

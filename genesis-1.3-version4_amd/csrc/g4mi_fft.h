@@ -28,7 +28,10 @@ struct CgemmArgs {
     const int4 *box;     // {ix0, iy0, ix1, iy1} per beam slice
     int mask_a;          // row mode: rows are source planes, zero outside box[row / ng]
     int first, ring;     // column mode: ii = ring ? (s - first + ns) % ns : s
+    int variant;         // 0: four real products per complex one, k chunks of 16;
+                         // 1: three products, 16; 2: four, 32; 3: three, 32
 };
+constexpr int kCgemmVariants = 4;
 
 // row mode: c[r][j] = scale * sum_l a[r][l] mat[j][l]          (a . mat^T)
 // column mode: c[s][j][x] = scale * sum_l mat[j][l] b[s][l][x] (+ epilogue)

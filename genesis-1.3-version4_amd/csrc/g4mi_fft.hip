@@ -24,17 +24,18 @@ namespace g4mi {
 
 typedef double f64x4 __attribute__((ext_vector_type(4)));
 
-constexpr int KC = 16;  // k per LDS chunk
-
 __device__ __forceinline__ bool in_box(const int4 &bx, int x, int y) {
     return x >= bx.x && x <= bx.z && y >= bx.y && y <= bx.w;
 }
 
-template <int NW, bool ROWS>
+// GAUSS: three real products per complex one, P1 = ar br, P2 = ai bi,
+// P3 = (ar + ai)(br + bi), re = P1 - P2, im = P3 - P1 - P2 (a quarter fewer
+// MFMAs, one more accumulator); otherwise four.  KC: k per LDS chunk.
+template <int NW, bool ROWS, bool GAUSS, int KC>
 __global__ void __launch_bounds__(64 * NW) k_cgemm(CgemmArgs g, int ntile_c, int ntile_r) {
     constexpr int T = 16 * NW, NT = 64 * NW, NB = NW;
     constexpr int PA = T + 1;                 // LDS pitch (complex) of a k row
-    constexpr int LPT = (T * KC) / NT;        // loads per thread per operand (= 4)
+    constexpr int LPT = (T * KC) / NT;        // loads per thread per operand (KC / 4)
     static_assert((T * KC) % NT == 0, "tile/thread mismatch");
     __shared__ double2 As[KC * PA];
     __shared__ double2 Bs[KC * PA];
@@ -101,11 +102,12 @@ __global__ void __launch_bounds__(64 * NW) k_cgemm(CgemmArgs g, int ntile_c, int
         }
     };
 
-    f64x4 cr[NB], ci[NB];
+    f64x4 cr[NB], ci[NB], c3[GAUSS ? NB : 1];
 #pragma unroll
     for (int i = 0; i < NB; i++) {
         cr[i] = f64x4{0.0, 0.0, 0.0, 0.0};
         ci[i] = f64x4{0.0, 0.0, 0.0, 0.0};
+        if (GAUSS) c3[i] = f64x4{0.0, 0.0, 0.0, 0.0};
     }
     const int nk = (ng + KC - 1) / KC;
     fetch(0);
@@ -118,14 +120,25 @@ __global__ void __launch_bounds__(64 * NW) k_cgemm(CgemmArgs g, int ntile_c, int
         for (int ks = 0; ks < KC / 4; ks++) {
             const int kk = ks * 4 + (lane >> 4);
             const double2 av = As[kk * PA + w * 16 + (lane & 15)];
-            const double nai = -av.y;
+            if (GAUSS) {
+                const double asum = av.x + av.y;
 #pragma unroll
-            for (int i = 0; i < NB; i++) {
-                const double2 bv = Bs[kk * PA + i * 16 + (lane & 15)];
-                cr[i] = __builtin_amdgcn_mfma_f64_16x16x4f64(av.x, bv.x, cr[i], 0, 0, 0);
-                cr[i] = __builtin_amdgcn_mfma_f64_16x16x4f64(nai, bv.y, cr[i], 0, 0, 0);
-                ci[i] = __builtin_amdgcn_mfma_f64_16x16x4f64(av.x, bv.y, ci[i], 0, 0, 0);
-                ci[i] = __builtin_amdgcn_mfma_f64_16x16x4f64(av.y, bv.x, ci[i], 0, 0, 0);
+                for (int i = 0; i < NB; i++) {
+                    const double2 bv = Bs[kk * PA + i * 16 + (lane & 15)];
+                    cr[i] = __builtin_amdgcn_mfma_f64_16x16x4f64(av.x, bv.x, cr[i], 0, 0, 0);
+                    ci[i] = __builtin_amdgcn_mfma_f64_16x16x4f64(av.y, bv.y, ci[i], 0, 0, 0);
+                    c3[i] = __builtin_amdgcn_mfma_f64_16x16x4f64(asum, bv.x + bv.y, c3[i], 0, 0, 0);
+                }
+            } else {
+                const double nai = -av.y;
+#pragma unroll
+                for (int i = 0; i < NB; i++) {
+                    const double2 bv = Bs[kk * PA + i * 16 + (lane & 15)];
+                    cr[i] = __builtin_amdgcn_mfma_f64_16x16x4f64(av.x, bv.x, cr[i], 0, 0, 0);
+                    cr[i] = __builtin_amdgcn_mfma_f64_16x16x4f64(nai, bv.y, cr[i], 0, 0, 0);
+                    ci[i] = __builtin_amdgcn_mfma_f64_16x16x4f64(av.x, bv.y, ci[i], 0, 0, 0);
+                    ci[i] = __builtin_amdgcn_mfma_f64_16x16x4f64(av.y, bv.x, ci[i], 0, 0, 0);
+                }
             }
         }
     }
@@ -139,7 +152,14 @@ __global__ void __launch_bounds__(64 * NW) k_cgemm(CgemmArgs g, int ntile_c, int
 #pragma unroll
         for (int v = 0; v < 4; v++) {
             const int row = w * 16 + 4 * v + (lane >> 4), col = i * 16 + (lane & 15);
-            double re = cr[i][v] * g.scale, im = ci[i][v] * g.scale;
+            double re, im;
+            if (GAUSS) {
+                re = (cr[i][v] - ci[i][v]) * g.scale;
+                im = (c3[i][v] - cr[i][v] - ci[i][v]) * g.scale;
+            } else {
+                re = cr[i][v] * g.scale;
+                im = ci[i][v] * g.scale;
+            }
             if (ROWS) {
                 const long long r = rr0 + row;
                 const int j = c0 + col;
@@ -182,6 +202,16 @@ int cgemm_tile(int ng) {
     return best;
 }
 
+template <bool ROWS, bool GAUSS, int KC>
+static void launch_variant(hipStream_t st, const CgemmArgs &g, int T, dim3 grid, int ntc, int ntr) {
+    switch (T) {
+        case 80: hipLaunchKernelGGL((k_cgemm<5, ROWS, GAUSS, KC>), grid, dim3(320), 0, st, g, ntc, ntr); break;
+        case 64: hipLaunchKernelGGL((k_cgemm<4, ROWS, GAUSS, KC>), grid, dim3(256), 0, st, g, ntc, ntr); break;
+        case 48: hipLaunchKernelGGL((k_cgemm<3, ROWS, GAUSS, KC>), grid, dim3(192), 0, st, g, ntc, ntr); break;
+        default: hipLaunchKernelGGL((k_cgemm<2, ROWS, GAUSS, KC>), grid, dim3(128), 0, st, g, ntc, ntr); break;
+    }
+}
+
 template <bool ROWS>
 static int launch_cgemm(hipStream_t st, const CgemmArgs &g) {
     if (g.ng < 2 || !g.mat || !g.c || (ROWS ? !g.a : !g.b)) {
@@ -198,11 +228,11 @@ static int launch_cgemm(hipStream_t st, const CgemmArgs &g) {
         return G4MI_E_ARG;
     }
     const dim3 grid((unsigned)blocks);
-    switch (T) {
-        case 80: hipLaunchKernelGGL((k_cgemm<5, ROWS>), grid, dim3(320), 0, st, g, ntc, ntr); break;
-        case 64: hipLaunchKernelGGL((k_cgemm<4, ROWS>), grid, dim3(256), 0, st, g, ntc, ntr); break;
-        case 48: hipLaunchKernelGGL((k_cgemm<3, ROWS>), grid, dim3(192), 0, st, g, ntc, ntr); break;
-        default: hipLaunchKernelGGL((k_cgemm<2, ROWS>), grid, dim3(128), 0, st, g, ntc, ntr); break;
+    switch (g.variant) {
+        case 1: launch_variant<ROWS, true, 16>(st, g, T, grid, ntc, ntr); break;
+        case 2: launch_variant<ROWS, false, 32>(st, g, T, grid, ntc, ntr); break;
+        case 3: launch_variant<ROWS, true, 32>(st, g, T, grid, ntc, ntr); break;
+        default: launch_variant<ROWS, false, 16>(st, g, T, grid, ntc, ntr); break;
     }
     G4_HIP(hipGetLastError());
     return G4MI_OK;

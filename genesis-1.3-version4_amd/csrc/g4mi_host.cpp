@@ -414,6 +414,14 @@ double inverse_erfc(double y) {
     return (y > 1) ? -x : x;
 }
 
+namespace {
+// QuietLoading::loadQuiet after the raw draws (:95-260): exact moments,
+// x-px decorrelation, Twiss scaling, beamlet mirroring, phase scale,
+// bunching and energy modulation.
+void quiet_finish(double *g, double *th, double *x, double *y, double *px, double *py, const SliceBeam &sl,
+                  int npart, int nbins, double theta0);
+}  // namespace
+
 void quiet_load(double *g, double *th, double *x, double *y, double *px, double *py,
                 const SliceBeam &sl, int npart, int nbins, double theta0, int islice) {
     Ran2 r((unsigned int)islice);
@@ -430,6 +438,35 @@ void quiet_load(double *g, double *th, double *x, double *y, double *px, double 
         px[i] = inverse_erfc(2 * h[4].next());
         py[i] = inverse_erfc(2 * h[5].next());
     }
+    quiet_finish(g, th, x, y, px, py, sl, npart, nbins, theta0);
+}
+
+unsigned int one4one_seed(int seed, int rank) {
+    Ran2 r((unsigned int)seed);
+    double v = 0;
+    for (int i = 0; i <= rank; i++) v = r.next();
+    return (unsigned int)(int)round(v * 1e9);
+}
+
+void quiet_load_one4one(Ran2 &stream, double *g, double *th, double *x, double *y, double *px, double *py,
+                        const SliceBeam &sl, int npart, double theta0) {
+    // all six coordinates from the one stream, particle by particle
+    for (int i = 0; i < npart; i++) {
+        th[i] = stream.next();
+        g[i] = inverse_erfc(2 * stream.next());
+        x[i] = inverse_erfc(2 * stream.next());
+        y[i] = inverse_erfc(2 * stream.next());
+        px[i] = inverse_erfc(2 * stream.next());
+        py[i] = inverse_erfc(2 * stream.next());
+    }
+    quiet_finish(g, th, x, y, px, py, sl, npart, 1, theta0);
+}
+
+namespace {
+void quiet_finish(double *g, double *th, double *x, double *y, double *px, double *py, const SliceBeam &sl,
+                  int npart, int nbins, double theta0) {
+    const int mpart = npart / nbins;
+    const double dtheta = 1. / (double)nbins;
     const double norm = mpart > 0 ? 1. / (double)mpart : 0;
     {  // energy to exact mean and rms
         double m1 = 0, m2 = 0;
@@ -515,6 +552,7 @@ void quiet_load(double *g, double *th, double *x, double *y, double *px, double 
         }
     }
 }
+}  // namespace
 
 unsigned int shot_noise_seed(int seed, int rank) {
     Ran2 r((unsigned int)seed);

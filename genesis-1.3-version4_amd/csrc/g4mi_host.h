@@ -69,6 +69,13 @@ class Ran2 {
 // Quiet loading of one slice (src/Loading/QuietLoading.cpp:50-260), SoA.
 void quiet_load(double *gamma, double *theta, double *x, double *y, double *px, double *py,
                 const SliceBeam &sl, int npart, int nbins, double theta0, int islice);
+// one4one loading (QuietLoading::init/loadQuiet with one4one,
+// src/Loading/QuietLoading.cpp:15-40, 50-260): every coordinate of every
+// particle from one ran2 stream seeded with one4one_seed(seed, rank),
+// which runs on from slice to slice; no beamlets.
+unsigned int one4one_seed(int seed, int rank);
+void quiet_load_one4one(Ran2 &stream, double *gamma, double *theta, double *x, double *y, double *px,
+                        double *py, const SliceBeam &sl, int npart, double theta0);
 // Shot noise (src/Loading/ShotNoise.cpp:29-68) consuming `stream`.
 void shot_noise(Ran2 &stream, double *theta, int npart, int nbins, double ne);
 // Seed stream of ShotNoise::init (:16-27).

@@ -66,6 +66,42 @@ int g4mi_load_beam(int nslice, const long long *off, const g4mi_slice_beam *slic
     return G4MI_OK;
 }
 
+int g4mi_one4one_counts(int nslice, const g4mi_slice_beam *slices, double lambda0, double sample,
+                        long long *counts) {
+    if (nslice < 1 || !slices || !counts || !(lambda0 > 0) || !(sample > 0)) {
+        set_error("g4mi_one4one_counts: invalid argument");
+        return G4MI_E_ARG;
+    }
+    for (int j = 0; j < nslice; j++) {  // LoadBeam.cpp:172, 191
+        const double ne = slices[j].current * lambda0 * sample / kCeH();
+        counts[j] = (long long)(int)round(ne);
+        if (counts[j] < 0) {
+            set_error("g4mi_one4one_counts: negative current");
+            return G4MI_E_ARG;
+        }
+    }
+    return G4MI_OK;
+}
+
+int g4mi_load_beam_one4one(int nslice, const long long *off, const g4mi_slice_beam *slices, int seed,
+                           double sample, double *gamma, double *theta, double *x, double *y, double *px,
+                           double *py) {
+    if (nslice < 1 || !off || !slices || !(sample > 0) || !gamma || !theta || !x || !y || !px || !py) {
+        set_error("g4mi_load_beam_one4one: invalid argument");
+        return G4MI_E_ARG;
+    }
+    const double theta0 = 4. * asin(1.) * sample;  // LoadBeam.cpp:129-132
+    Ran2 stream(one4one_seed(seed, 0));            // one stream through all slices, in order
+    for (int j = 0; j < nslice; j++) {
+        const long long a = off[j];
+        SliceBeam sb;
+        std::memcpy(&sb, &slices[j], sizeof(sb));
+        quiet_load_one4one(stream, gamma + a, theta + a, x + a, y + a, px + a, py + a, sb,
+                           (int)(off[j + 1] - a), theta0);
+    }
+    return G4MI_OK;
+}
+
 int g4mi_load_gauss(double *field, int ngrid, double gridmax, double lambda, double power,
                     double waist, double waist_pos, double phase) {
     if (!field || ngrid < 2 || gridmax <= 0 || lambda <= 0 || waist <= 0) {

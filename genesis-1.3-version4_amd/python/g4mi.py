@@ -86,6 +86,11 @@ def lib():
         L.g4mi_destroy.argtypes = [C.c_void_p]
         L.g4mi_set_efield.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_double]
         L.g4mi_set_incoherent.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int]
+        L.g4mi_set_sorting.argtypes = [C.c_void_p, C.c_int]
+        L.g4mi_sort.argtypes = [C.c_void_p, C.POINTER(C.c_longlong)]
+        L.g4mi_get_slices.argtypes = [C.c_void_p, _lp, _dp]
+        L.g4mi_one4one_counts.argtypes = [C.c_int, C.c_void_p, C.c_double, C.c_double, _lp]
+        L.g4mi_load_beam_one4one.argtypes = [C.c_int, _lp, C.c_void_p, C.c_int, C.c_double] + [_dp] * 6
         L.g4mi_set_wake.argtypes = [C.c_void_p, _dp]
         L.g4mi_set_field_solver.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_double, C.c_double,
                                             C.c_double]
@@ -181,6 +186,28 @@ def load_beam(off, slices, nbins, shotnoise, seed, lambda0, sample=1.0, timerun=
     return arrs
 
 
+def one4one_counts(slices, lambda0, sample=1.0):
+    """Electrons per slice of a one4one beam (g4mi_one4one_counts)."""
+    ns = len(slices)
+    sb = (SliceBeam * ns)(*[SliceBeam(**{k: float(s.get(k, 0.0)) for k in SLICE_FIELDS}) for s in slices])
+    cnt = np.zeros(ns, dtype=np.int64)
+    _chk(lib().g4mi_one4one_counts(ns, sb, lambda0, sample, cnt.ctypes.data_as(_lp)), "g4mi_one4one_counts")
+    return cnt
+
+
+def load_beam_one4one(off, slices, seed, sample=1.0):
+    """one4one loading of every slice from one ran2 stream (g4mi_load_beam_one4one)."""
+    off = np.ascontiguousarray(off, dtype=np.int64)
+    ns = len(off) - 1
+    sb = (SliceBeam * ns)(*[SliceBeam(**{k: float(s.get(k, 0.0)) for k in SLICE_FIELDS}) for s in slices])
+    n = int(off[-1])
+    arrs = {k: np.zeros(n) for k in ("gamma", "theta", "x", "y", "px", "py")}
+    _chk(lib().g4mi_load_beam_one4one(ns, off.ctypes.data_as(_lp), sb, seed, sample,
+                                      *[_p(arrs[k]) for k in ("gamma", "theta", "x", "y", "px", "py")]),
+         "g4mi_load_beam_one4one")
+    return arrs
+
+
 def load_gauss(ngrid, gridmax, lambda0, power, waist, waist_pos=0.0, phase=0.0):
     f = np.zeros(ngrid * ngrid * 2)
     _chk(lib().g4mi_load_gauss(_p(f), ngrid, gridmax, lambda0, power, waist, waist_pos, phase),
@@ -224,6 +251,23 @@ class Context:
         _chk(lib().g4mi_set_incoherent(self.h, seed, nbins, int(do_loss), int(do_spread)),
              "g4mi_set_incoherent")
 
+    def set_sorting(self, one4one=True):
+        """Beam::initSorting: Beam::sort at sort markers for one4one beams."""
+        _chk(lib().g4mi_set_sorting(self.h, int(one4one)), "g4mi_set_sorting")
+
+    def sort(self):
+        """Beam::sort now; returns the particle count."""
+        n = C.c_longlong(0)
+        _chk(lib().g4mi_sort(self.h, C.byref(n)), "g4mi_sort")
+        return n.value
+
+    def slices(self):
+        """Current slice offsets (ns+1) and currents (ns)."""
+        off = np.zeros(self.ns + 1, dtype=np.int64)
+        cur = np.zeros(self.ns)
+        _chk(lib().g4mi_get_slices(self.h, off.ctypes.data_as(_lp), _p(cur)), "g4mi_get_slices")
+        return off, cur
+
     def set_wake(self, eloss):
         """Per-slice wake loss in eV/m (Collective::apply), or None."""
         if eloss is None:
@@ -248,7 +292,11 @@ class Context:
         _chk(lib().g4mi_beam_upload(self.h, *[_p(v) for v in a], _p(cur)), "g4mi_beam_upload")
 
     def download_beam(self):
-        out = {k: np.zeros(self.n) for k in ("gamma", "theta", "x", "y", "px", "py")}
+        """The particles in their current slice order (after one4one sorting
+        the counts per slice are those of slices())."""
+        off = np.zeros(self.ns + 1, dtype=np.int64)
+        _chk(lib().g4mi_get_slices(self.h, off.ctypes.data_as(_lp), None), "g4mi_get_slices")
+        out = {k: np.zeros(int(off[-1])) for k in ("gamma", "theta", "x", "y", "px", "py")}
         _chk(lib().g4mi_beam_download(self.h, *[_p(out[k]) for k in
                                                 ("gamma", "theta", "x", "y", "px", "py")]),
              "g4mi_beam_download")

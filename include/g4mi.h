@@ -128,6 +128,21 @@ int g4mi_field_track(g4mi_ctx *ctx, const g4mi_step_params *st, double delz);
  * switches it off.  FP64 form only. */
 int g4mi_set_incoherent(g4mi_ctx *ctx, int seed, int nbins, int do_loss, int do_spread);
 
+/* one4one beams, Beam::initSorting (src/Core/Beam.cpp:75-79): with
+ * one4one = 1 every step whose marker has bit 4 set runs Beam::sort
+ * (:81-94) after the beam step, as Gencore::run (src/Core/Gencore.cpp:
+ * 124-134): particles outside the time window [0, slen nslice] are dropped,
+ * the others move to slice a + floor(theta / slen) with theta -= slen atar
+ * (Sorting.cpp:74-145, 367-428), and each slice's current becomes
+ * np ce / slicelength.  The order of the particles within a slice is the
+ * source order (the reference's differs; the sets are the same).
+ * g4mi_sort sorts now and returns the particle count; g4mi_get_slices
+ * copies the current offsets (nslice+1) and currents (nslice).  FP64 form
+ * only. */
+int g4mi_set_sorting(g4mi_ctx *ctx, int one4one);
+int g4mi_sort(g4mi_ctx *ctx, long long *nkept);
+int g4mi_get_slices(g4mi_ctx *ctx, long long *off, double *current);
+
 /* Wakefield energy loss, Collective::apply (src/Core/Collective.cpp:107-128):
  * eloss[nslice] in eV/m (Collective's wakeext + wakeint for each beam
  * slice), added as eloss * delz / eev to every particle's gamma after the
@@ -185,7 +200,11 @@ int g4mi_sort_index(g4mi_ctx *ctx, double slen, int *atar, long long *moved);
  * of its field and keeps the fastest (the library's autotuner).
  * g4mi_set_schedule fixes an entry instead ("adi_kernel": 0..3, -1
  * re-enables tuning); g4mi_get_schedule writes the current choice and the
- * measured timings as a JSON object into buf. */
+ * measured timings as a JSON object into buf.  The spectral solver's first
+ * step likewise times its complex-GEMM variants ("fft_variant": 0 four
+ * real products per complex one and k chunks of 16, 1 three products, 2
+ * four products with chunks of 32, 3 three with 32; -1 tunes again).
+ * Mixed form: "adi_refine", "mixed_adi", "graphs" (0/1). */
 int g4mi_set_schedule(g4mi_ctx *ctx, const char *name, int value);
 int g4mi_get_schedule(g4mi_ctx *ctx, char *buf, int len);
 
@@ -233,6 +252,19 @@ int g4mi_load_beam(int nslice, const long long *off, const g4mi_slice_beam *slic
                    int shotnoise, int seed, int islice0, double lambda0, double sample,
                    int timerun, double *gamma, double *theta, double *x, double *y, double *px,
                    double *py);
+
+/* one4one beams (one macroparticle per electron), LoadBeam::init with
+ * one4one (src/Loading/LoadBeam.cpp:129-132, 157-199): counts[j] =
+ * round(current lambda0 sample / ce) electrons per slice; then every
+ * coordinate of every slice, in slice order, from one ran2 stream seeded
+ * with round(1e9 u), u the first draw of ran2(seed) (QuietLoading::init,
+ * src/Loading/QuietLoading.cpp:15-40), no beamlets, theta scaled by 2 pi
+ * sample, no shot noise. */
+int g4mi_one4one_counts(int nslice, const g4mi_slice_beam *slices, double lambda0, double sample,
+                        long long *counts);
+int g4mi_load_beam_one4one(int nslice, const long long *off, const g4mi_slice_beam *slices, int seed,
+                           double sample, double *gamma, double *theta, double *x, double *y, double *px,
+                           double *py);
 
 /* Gaussian seed field of one slice (GaussHermite::loadGauss,
  * src/Util/GaussHermite.cpp:20-63, fundamental TEM00), interleaved complex. */

@@ -250,6 +250,9 @@ extern "C" double orc_faw2(const orc_step *st, double x, double y) {
  * px, py; init at :15-46 with base = {0..5}).  The six sequences restart at
  * round(1e9 * first ran2 draw of seed islice) (:55-65).  Arrays must hold
  * npart entries.  Pinned bit-exactly against oracle/_ref. */
+static void quiet_normalise(double *g, double *th, double *x, double *y, double *px, double *py,
+                            const orc_slice_params *sl, int npart, int nbins, double theta0);
+
 extern "C" void orc_quiet_load(double *g, double *th, double *x, double *y, double *px,
                                double *py, const orc_slice_params *sl, int npart, int nbins,
                                double theta0, int islice) {
@@ -267,6 +270,40 @@ extern "C" void orc_quiet_load(double *g, double *th, double *x, double *y, doub
         px[i] = orc_inverfc(2 * spx.next());
         py[i] = orc_inverfc(2 * spy.next());
     }
+    quiet_normalise(g, th, x, y, px, py, sl, npart, nbins, theta0);
+}
+
+/* one4one loading, QuietLoading::init(true, {seed, rank}) (:15-40) and
+ * loadQuiet (:50-260) for every slice in order: the six sequences are one
+ * ran2 stream (RandomU::set does nothing), seeded with round(1e9 u), u the
+ * (rank+1)-th draw of ran2(seed); nbins = 1; theta0 = 2 pi sample
+ * (LoadBeam.cpp:129-132). */
+extern "C" void orc_quiet_load_one4one(int seed, int rank, int nslice, const long long *off,
+                                       const orc_slice_params *sl, double theta0, double *g, double *th,
+                                       double *x, double *y, double *px, double *py) {
+    Ran2 rseed((unsigned int)seed);
+    double val = 0;
+    for (int i = 0; i <= rank; i++) val = rseed.next();
+    Ran2 st((unsigned int)(int)round(val * 1e9));
+    for (int j = 0; j < nslice; j++) {
+        long long a = off[j];
+        int n = (int)(off[j + 1] - a);
+        for (int i = 0; i < n; i++) {
+            th[a + i] = st.next();
+            g[a + i] = orc_inverfc(2 * st.next());
+            x[a + i] = orc_inverfc(2 * st.next());
+            y[a + i] = orc_inverfc(2 * st.next());
+            px[a + i] = orc_inverfc(2 * st.next());
+            py[a + i] = orc_inverfc(2 * st.next());
+        }
+        quiet_normalise(g + a, th + a, x + a, y + a, px + a, py + a, &sl[j], n, 1, theta0);
+    }
+}
+
+static void quiet_normalise(double *g, double *th, double *x, double *y, double *px, double *py,
+                            const orc_slice_params *sl, int npart, int nbins, double theta0) {
+    int mpart = npart / nbins;
+    double dtheta = 1. / (double)nbins;
     double z = 0, zz = 0, norm = 0;
     if (mpart > 0) norm = 1. / (double)mpart;
     for (int i = 0; i < mpart; i++) {
@@ -1328,6 +1365,72 @@ extern "C" void orc_step_all(orc_state *s, const orc_step *st, double gamma0) {
     orc_beam_track(s, st, gamma0, delz);
     orc_field_advance(s, st, delz);
     orc_slippage(s, st->slip);
+}
+
+/* Beam::sort for one4one beams on one rank (src/Core/Beam.cpp:81-94;
+ * Sorting::globalSort/fillPushVectors :190-262, 367-428 with s0 = 0,
+ * keep window [0, slen ns]; localSort :74-145): particles outside the
+ * window are dropped, every other particle moves to slice a + floor(theta /
+ * slen) with theta -= slen atar, by swap-with-last and push_back in the
+ * reference's order.  The beam is given and returned as per-slice arrays:
+ * in: off[ns+1] and the six coordinate arrays (capacity cap); out: the same
+ * arrays rewritten, off updated, current[ns] = np ce / slicelength.
+ * Returns the new particle count, or -1 if cap is too small. */
+extern "C" long long orc_sort_one4one(int ns, long long *off, long long cap, double slen, double slicelength,
+                                      double *g, double *th, double *x, double *y, double *px, double *py,
+                                      double *current) {
+    struct P { double th, g, x, y, px, py; };
+    vector<vector<P>> rec(ns);
+    for (int a = 0; a < ns; a++)
+        for (long long i = off[a]; i < off[a + 1]; i++) rec[a].push_back({th[i], g[i], x[i], y[i], px[i], py[i]});
+    /* fillPushVectors' deletion pass (one rank: nothing is sent) */
+    double keepmin = 0, keepmax = slen * ns;
+    for (int i = 0; i < ns; i++) {
+        size_t j = 0;
+        while (j < rec[i].size()) {
+            double s = slen * i + rec[i][j].th;
+            if ((s < keepmin) || (s > keepmax)) {
+                rec[i][j] = rec[i].back();
+                rec[i].pop_back();
+            } else {
+                j++;
+            }
+        }
+    }
+    /* localSort */
+    double invslen = 1. / slen;
+    for (int a = 0; a < ns; a++) {
+        size_t b = 0;
+        while (b < rec[a].size()) {
+            double theta = rec[a][b].th;
+            int atar = (int)floor(theta * invslen);
+            if (atar != 0) {
+                P p = rec[a][b];
+                p.th = rec[a][b].th - slen * (atar);
+                /* a particle exactly at the window's upper edge (s = slen ns)
+                 * would index one past the last slice; it is dropped */
+                if (a + atar < ns) rec[a + atar].push_back(p);
+                rec[a][b] = rec[a].back();
+                rec[a].pop_back();
+            } else {
+                b++;
+            }
+        }
+    }
+    long long n = 0;
+    for (int a = 0; a < ns; a++) n += (long long)rec[a].size();
+    if (n > cap) return -1;
+    n = 0;
+    for (int a = 0; a < ns; a++) {
+        off[a] = n;
+        for (const P &p : rec[a]) {
+            th[n] = p.th; g[n] = p.g; x[n] = p.x; y[n] = p.y; px[n] = p.px; py[n] = p.py;
+            n++;
+        }
+        current[a] = (double)rec[a].size() * k_ce / slicelength;
+    }
+    off[ns] = n;
+    return n;
 }
 
 /* Localsort target index, src/Util/Sorting.cpp:96 (atar=floor(theta/slen)). */

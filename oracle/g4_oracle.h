@@ -135,6 +135,12 @@ void orc_field_advance(orc_state *s, const orc_step *st, double delz);
 void *orc_incoherent_new(int seed, int rank, int do_loss, int do_spread);
 void orc_incoherent_free(void *h);
 void orc_incoherent_apply(orc_state *s, const orc_step *st, double gamma0, double delz);
+/* one4one beams: loading of every slice from one ran2 stream, and
+ * Beam::sort's migration between slices (details in g4_oracle.cpp). */
+void orc_quiet_load_one4one(int seed, int rank, int nslice, const long long *off, const orc_slice_params *sl,
+                            double theta0, double *g, double *th, double *x, double *y, double *px, double *py);
+long long orc_sort_one4one(int ns, long long *off, long long cap, double slen, double slicelength, double *g,
+                           double *th, double *x, double *y, double *px, double *py, double *current);
 /* FieldSolverFFT: unscaled 2-D DFT in place (sign -1 forward, +1 backward)
  * and the K2 / sigmoid tables of FieldSolverFFT::init. */
 void orc_dft2(int n, double *data, int sign);

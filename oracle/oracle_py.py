@@ -88,6 +88,10 @@ def lib():
         L.orc_shotnoise_new.argtypes = [C.c_int, C.c_int]
         L.orc_shotnoise_new.restype = C.c_void_p
         L.orc_shotnoise_free.argtypes = [C.c_void_p]
+        L.orc_quiet_load_one4one.argtypes = [C.c_int, C.c_int, C.c_int, _lp, C.POINTER(SliceParams),
+                                             C.c_double] + [_dp] * 6
+        L.orc_sort_one4one.argtypes = [C.c_int, _lp, C.c_longlong, C.c_double, C.c_double] + [_dp] * 7
+        L.orc_sort_one4one.restype = C.c_longlong
         L.orc_incoherent_new.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int]
         L.orc_incoherent_new.restype = C.c_void_p
         L.orc_incoherent_free.argtypes = [C.c_void_p]
@@ -131,6 +135,7 @@ def ref():
         R.ref_besselj.argtypes = [C.c_int, C.c_double]
         R.ref_besselj.restype = C.c_double
         R.ref_quiet_load.argtypes = [_dp] * 7 + [C.c_int, C.c_int, C.c_double, C.c_int]
+        R.ref_quiet_load_one4one.argtypes = [C.c_int, C.c_int, C.c_int, _lp, _dp, C.c_double] + [_dp] * 6
         R.ref_shotnoise_new.argtypes = [C.c_int, C.c_int]
         R.ref_shotnoise_new.restype = C.c_void_p
         R.ref_shotnoise_free.argtypes = [C.c_void_p]
@@ -263,8 +268,30 @@ class Sim:
         out["global_energy"] = out["global_energy"][0]
         return out
 
+    one4one = False  # Beam::sort at sort markers (one4one beams)
+
     def step(self, st, gamma0):
+        if self.one4one and (st.marker & 4):
+            # Gencore::run order: beam, sort, field, slippage (Gencore.cpp:114-147)
+            lib().orc_beam_track(C.byref(self.s), C.byref(st), gamma0, st.dz)
+            self.sort()
+            lib().orc_field_advance(C.byref(self.s), C.byref(st), st.dz)
+            lib().orc_slippage(C.byref(self.s), st.slip)
+            return
         lib().orc_step_all(C.byref(self.s), C.byref(st), gamma0)
+
+    def sort(self):
+        """Beam::sort of a one4one beam (orc_sort_one4one), in place."""
+        slen = 4.0 * np.arcsin(1.0) * self.s.slicelength / self.s.reflength
+        n = lib().orc_sort_one4one(self.ns, self.off.ctypes.data_as(_lp), len(self.gamma), slen,
+                                   self.s.slicelength, *[_ptr(getattr(self, k)) for k in (
+                                       "gamma", "theta", "x", "y", "px", "py", "current")])
+        assert n >= 0
+        cnt = np.diff(self.off)
+        if len(cnt) and cnt.max() > len(self.sc_ez):
+            self.sc_ez = np.zeros(int(cnt.max()))
+            self.s.sc_ez = _ptr(self.sc_ez)
+        return n
 
     def run(self, steps, gamma0, record=("power", "bunching", "energy"), nsteps=None):
         nsteps = len(steps) if nsteps is None else nsteps
@@ -311,6 +338,29 @@ def load_beam(sim, slice_params, npart_per_slice, nbins, shotnoise_seed=None, is
     finally:
         if sn is not None:
             L.orc_shotnoise_free(sn)
+
+
+def one4one_counts(currents, lambda0, sample=1):
+    """Electrons per slice, round(I lambda0 sample / ce) (LoadBeam.cpp:172, 191)."""
+    return [int(round(c * lambda0 * sample / CE)) for c in currents]
+
+
+def load_beam_one4one(sim, slice_params, seed, lambda0, sample=1, rank=0):
+    """one4one loading of every slice (orc_quiet_load_one4one); the Sim's
+    counts must be one4one_counts of the currents."""
+    ns = sim.ns
+    sps = (SliceParams * ns)()
+    for j in range(ns):
+        p = slice_params[j] if isinstance(slice_params, (list, tuple)) else slice_params
+        sps[j] = SliceParams(**{k: float(p.get(k, 0.0)) for k in (
+            "gamma", "delgam", "current", "ex", "ey", "xcen", "ycen", "pxcen", "pycen",
+            "betax", "betay", "alphax", "alphay", "bunch", "bunchphase", "emod", "emodphase")})
+        sim.current[j] = sps[j].current
+    theta0 = 4.0 * np.arcsin(1.0) * sample
+    lib().orc_quiet_load_one4one(seed, rank, ns, sim.off.ctypes.data_as(_lp), sps, theta0,
+                                 *[_ptr(getattr(sim, k)) for k in ("gamma", "theta", "x", "y", "px", "py")])
+    sim.one4one = True
+    sim.s.nbins = 1
 
 
 def load_gauss_all(sim, power, w0, lambda0, z0=0.0):

@@ -62,6 +62,28 @@ void ref_quiet_load(double *g, double *th, double *x, double *y, double *px, dou
     }
 }
 
+/* one4one: QuietLoading::init(true, {seed, rank}) once, then loadQuiet
+ * for every slice in order on the same object (one stream). */
+void ref_quiet_load_one4one(int seed, int rank, int nslice, const long long *off, const double *sp17,
+                            double theta0, double *g, double *th, double *x, double *y, double *px,
+                            double *py) {
+    QuietLoading ql;
+    int base[2] = {seed, rank};
+    ql.init(true, base);
+    for (int j = 0; j < nslice; j++) {
+        BeamSlice s;
+        std::memcpy(&s, sp17 + 17 * j, sizeof(BeamSlice));
+        const long long a = off[j];
+        const int n = (int)(off[j + 1] - a);
+        std::vector<Particle> p(n > 0 ? n : 1);
+        ql.loadQuiet(p.data(), &s, n, 1, theta0, j);
+        for (int i = 0; i < n; i++) {
+            g[a + i] = p[i].gamma; th[a + i] = p[i].theta; x[a + i] = p[i].x;
+            y[a + i] = p[i].y; px[a + i] = p[i].px; py[a + i] = p[i].py;
+        }
+    }
+}
+
 void *ref_shotnoise_new(int seed, int rank) {
     ShotNoise *s = new ShotNoise;
     s->init(seed, rank);

@@ -38,13 +38,17 @@ def _fft_state(counts, ngrid, gridmax=cases.GRIDMAX, filt=None, **kw):
     return sim, ctx
 
 
+@pytest.mark.parametrize("variant", [0, 1, 2, 3])
 @pytest.mark.parametrize("ngrid", [31, 40, 63, 151])
-def test_fft_field_track_parity(ngrid):
+def test_fft_field_track_parity(ngrid, variant):
     """Deposit + spectral step on grids that take each output tile edge
-    (32, 48, 64, 80) of the complex GEMM, three undulator steps."""
-    _log(f"spectral field step parity: 6 slices x 1024, {ngrid}^2, 3 steps")
+    (32, 48, 64, 80) of the complex GEMM, for each GEMM variant (four or
+    three real products per complex one, k chunks of 16 or 32), three
+    undulator steps."""
+    _log(f"spectral field step parity: 6 slices x 1024, {ngrid}^2, GEMM variant {variant}, 3 steps")
     st = H.steps()
     sim, ctx = _fft_state([1024] * 6, ngrid, seed_power=5e7)
+    ctx.set_schedule("fft_variant", variant)
     gst = H.gsteps(st)
     for i in range(3):
         O.lib().orc_field_advance(C.byref(sim.s), C.byref(st[i]), st[i].dz)
@@ -54,7 +58,23 @@ def test_fft_field_track_parity(ngrid):
     err = np.max(np.abs(f - sim.field)) / np.max(np.abs(sim.field))
     _log(f"  field max diff / max |E| = {err:.3e}")
     assert err < 1e-12
+    assert ctx.schedule()["gemm_variant"] == variant
     ctx.close()
+
+
+def test_fft_variant_tuner():
+    """With no variant fixed, the first spectral step times all four and
+    keeps the fastest; the result agrees with the oracle."""
+    st = H.steps()
+    sim, ctx = _fft_state([1024] * 4, 63, gridmax=0.3e-3, seed_power=5e7)
+    O.lib().orc_field_advance(C.byref(sim.s), C.byref(st[0]), st[0].dz)
+    ctx.field_track(H.gsteps(st)[0], st[0].dz)
+    sch = ctx.schedule()
+    _log(f"  tuned: {sch}")
+    ms = sch["gemm_tuned_ms"]
+    assert sch["gemm_variant"] in range(4) and ms[sch["gemm_variant"]] == min(ms)
+    f, _ = ctx.download_field()
+    assert np.max(np.abs(f - sim.field)) < 1e-12 * np.max(np.abs(sim.field))
 
 
 def test_fft_drift_step_is_pure_diffraction():

@@ -1,0 +1,150 @@
+"""one4one beams: loading from one ran2 stream (QuietLoading with
+one4one) and Beam::sort's migration of particles between slices.
+
+CPU: the oracle's loader against the reference's own QuietLoading
+(oracle/_ref, bit for bit), the library's host loader against the oracle
+(bit for bit), the oracle's sort against a numpy brute force.
+GPU: the device sort against the oracle's (the same particles in every
+slice, bit for bit; the order within a slice differs), and a SASE run with
+sort markers against the oracle, P(z) within 1e-10."""
+import ctypes as C
+import math
+import os
+import sys
+
+import numpy as np
+import pytest
+
+import cases
+import g4mi
+import harness as H
+import oracle_py as O
+
+LAM = 1e-10
+SLICE = dict(gamma=11357.82, delgam=2.0, current=50.0, ex=4e-7, ey=4e-7, betax=10.0, betay=12.0,
+             alphax=0.3, alphay=-0.2)
+
+
+def _slices(ns, cur0=50.0):
+    return [dict(SLICE, current=cur0 * (1 + 0.1 * j)) for j in range(ns)]
+
+
+def _oracle_beam(ns, seed=7, sample=1, ngrid=31, gridmax=0.2e-3, timerun=True):
+    sl = _slices(ns)
+    cnt = O.one4one_counts([s["current"] for s in sl], LAM, sample)
+    sim = O.Sim(cnt, ngrid, gridmax, LAM, sample=sample, timerun=timerun)
+    O.load_beam_one4one(sim, sl, seed, LAM, sample)
+    return sim, sl, cnt
+
+
+def test_one4one_counts_and_loader_match_oracle():
+    sim, sl, cnt = _oracle_beam(5)
+    assert list(cnt) == [round(s["current"] * LAM / O.CE) for s in sl]
+    assert list(g4mi.one4one_counts(sl, LAM)) == list(cnt)
+    b = g4mi.load_beam_one4one(sim.off, sl, 7)
+    for k in ("gamma", "theta", "x", "y", "px", "py"):
+        assert np.array_equal(b[k], getattr(sim, k)), k
+
+
+def test_one4one_loader_matches_reference_build():
+    R = O.ref()
+    if R is None:
+        pytest.skip("oracle/_ref not built (needs the reference tree)")
+    sim, sl, cnt = _oracle_beam(4, seed=31, sample=2)
+    sp = np.array([[s.get(k, 0.0) for k in (
+        "gamma", "delgam", "current", "ex", "ey", "xcen", "ycen", "pxcen", "pycen",
+        "betax", "betay", "alphax", "alphay", "bunch", "bunchphase", "emod", "emodphase")] for s in sl])
+    out = {k: np.zeros(len(sim.gamma)) for k in ("gamma", "theta", "x", "y", "px", "py")}
+    R.ref_quiet_load_one4one(31, 0, 4, sim.off.ctypes.data_as(C.POINTER(C.c_longlong)),
+                             np.ascontiguousarray(sp).ctypes.data_as(C.POINTER(C.c_double)),
+                             4 * math.asin(1) * 2, *[out[k].ctypes.data_as(C.POINTER(C.c_double)) for k in
+                                                    ("gamma", "theta", "x", "y", "px", "py")])
+    for k in out:
+        assert np.array_equal(out[k], getattr(sim, k)), k
+
+
+def _brute_sort(sim, slen):
+    """Per slice, the sorted list of particles (theta, gamma, x, y, px, py)
+    after migration, computed particle by particle."""
+    ns = sim.ns
+    dest = [[] for _ in range(ns)]
+    for a in range(ns):
+        for i in range(int(sim.off[a]), int(sim.off[a + 1])):
+            th = sim.theta[i]
+            s = slen * a + th
+            if s < 0 or s > slen * ns:
+                continue
+            atar = math.floor(th * (1.0 / slen))
+            if a + atar >= ns:
+                continue
+            dest[a + atar].append((th - slen * atar, sim.gamma[i], sim.x[i], sim.y[i], sim.px[i], sim.py[i]))
+    return [sorted(d) for d in dest]
+
+
+def _per_slice(off, b):
+    return [sorted(zip(*[b[k][int(off[a]):int(off[a + 1])] for k in ("theta", "gamma", "x", "y", "px", "py")]))
+            for a in range(len(off) - 1)]
+
+
+def _scramble(sim, rng, slen, spread=1.6):
+    """theta moved by up to +-spread slices, so particles leave their slice
+    in both directions and some leave the window."""
+    sim.theta += rng.uniform(-spread, spread, size=len(sim.theta)) * slen
+
+
+def test_oracle_sort_matches_brute_force():
+    sim, sl, cnt = _oracle_beam(6, sample=2)
+    slen = 2 * math.pi * 2
+    _scramble(sim, np.random.default_rng(3), slen)
+    ref = _brute_sort(sim, slen)
+    sim.sort()
+    got = _per_slice(sim.off, {k: getattr(sim, k) for k in ("gamma", "theta", "x", "y", "px", "py")})
+    assert got == ref
+    assert np.array_equal(sim.current, np.diff(sim.off) * O.CE / (2 * LAM))
+    assert all(0 <= p[0] < slen for sl_ in got for p in sl_)
+    assert sum(len(g) for g in got) < int(sum(cnt))  # some left the window
+
+
+@pytest.mark.gpu
+def test_device_sort_matches_oracle():
+    print("one4one sort: 8 slices, scrambled phases, GPU vs oracle", flush=True)
+    sim, sl, cnt = _oracle_beam(8, sample=3, ngrid=15)
+    slen = 2 * math.pi * 3
+    _scramble(sim, np.random.default_rng(11), slen)
+    ctx = H.to_gpu(sim)
+    ctx.set_sorting(True)
+    n = ctx.sort()
+    sim.sort()
+    off, cur = ctx.slices()
+    assert n == int(sim.off[-1]) and np.array_equal(off, sim.off)
+    assert np.array_equal(cur, sim.current)
+    b = ctx.download_beam()
+    assert _per_slice(off, b) == _per_slice(sim.off, {k: getattr(sim, k) for k in b})
+    # a second sort moves nothing
+    assert ctx.sort() == n and np.array_equal(ctx.slices()[0], off)
+    ctx.close()
+
+
+@pytest.mark.gpu
+def test_one4one_sase_with_sort_markers():
+    """A one4one SASE run (8 slices, 1.3-1.6 k electrons each, 31^2) with a
+    sort every 10 steps over 150 steps: P(z) and |b(z)| against the oracle
+    within 1e-10; the particle counts per slice after the run are equal."""
+    print("one4one SASE with sorting, 150 steps, GPU vs oracle", flush=True)
+    st = H.steps()[:150]
+    for i, s in enumerate(st):
+        if i % 10 == 0:
+            s.marker |= 4
+    sim, sl, cnt = _oracle_beam(8, seed=3, ngrid=31, gridmax=0.2e-3)
+    sim.s.ndiag_threads = 16
+    ctx = H.to_gpu(sim)
+    ctx.set_sorting(True)
+    g = ctx.run(H.gsteps(st), cases.GAMMA0, record=("power", "bunching"))
+    r = sim.run(st, cases.GAMMA0, record=("power", "bunching"))
+    ep = H.rel_err(g["power"][1:], r["power"][1:])
+    eb = H.rel_err(g["bunching"], r["bunching"])
+    off, _ = ctx.slices()
+    print(f"  P(z) rel err {ep:.3e}  |b(z)| {eb:.3e}  particles {int(cnt.sum())} -> {int(off[-1])}", flush=True)
+    assert np.array_equal(off, sim.off)
+    assert ep < 1e-10 and eb < 1e-10
+    ctx.close()
