@@ -350,10 +350,6 @@ bool do_setup(Deck &dk, Namelist &nl, std::string &err) {
         err = k.err;
         return false;
     }
-    if (s.one4one) {
-        err = "one4one = true is not supported by this build";
-        return false;
-    }
     std::string path = s.lattice;
     if (!read_file(path, dk.lattice_text)) {
         path = dir_of(dk.input_file) + "/" + s.lattice;
@@ -513,19 +509,34 @@ bool do_beam(Deck &dk, Namelist &nl, std::string &err) {
         return false;
     }
     const SetupNL &st = dk.setup;
-    if (st.npart % st.nbins != 0) {
+    const int nbins = st.one4one ? 1 : st.nbins;  // LoadBeam.cpp:129-132
+    if (st.npart % nbins != 0) {
         err = "NPART is not a multiple of NBINS";
         return false;
     }
     const int ns = time_nslice(dk);
+    std::vector<g4mi_slice_beam> slices(ns, sb);
     dk.off.assign(ns + 1, 0);
-    for (int i = 0; i < ns; i++) dk.off[i + 1] = dk.off[i] + st.npart;
+    if (st.one4one) {  // one electron per macroparticle
+        std::vector<long long> cnt(ns);
+        if (g4mi_one4one_counts(ns, slices.data(), st.lambda0, (double)dk.time.sample, cnt.data()) != G4MI_OK) {
+            err = g4mi_last_error();
+            return false;
+        }
+        for (int i = 0; i < ns; i++) dk.off[i + 1] = dk.off[i] + cnt[i];
+    } else {
+        for (int i = 0; i < ns; i++) dk.off[i + 1] = dk.off[i] + st.npart;
+    }
     const size_t n = (size_t)dk.off[ns];
     for (auto *v : {&dk.gamma, &dk.theta, &dk.x, &dk.y, &dk.px, &dk.py}) v->assign(n, 0.0);
-    std::vector<g4mi_slice_beam> slices(ns, sb);
-    const int rc = g4mi_load_beam(ns, dk.off.data(), slices.data(), st.nbins, st.shotnoise ? 1 : 0, st.seed, 0,
-                                  st.lambda0, (double)dk.time.sample, dk.time.dotime ? 1 : 0, dk.gamma.data(),
-                                  dk.theta.data(), dk.x.data(), dk.y.data(), dk.px.data(), dk.py.data());
+    const int rc = st.one4one
+                       ? g4mi_load_beam_one4one(ns, dk.off.data(), slices.data(), st.seed, (double)dk.time.sample,
+                                                dk.gamma.data(), dk.theta.data(), dk.x.data(), dk.y.data(),
+                                                dk.px.data(), dk.py.data())
+                       : g4mi_load_beam(ns, dk.off.data(), slices.data(), st.nbins, st.shotnoise ? 1 : 0, st.seed,
+                                        0, st.lambda0, (double)dk.time.sample, dk.time.dotime ? 1 : 0,
+                                        dk.gamma.data(), dk.theta.data(), dk.x.data(), dk.y.data(), dk.px.data(),
+                                        dk.py.data());
     if (rc != G4MI_OK) {
         err = "beam loading failed";
         return false;
@@ -605,7 +616,7 @@ bool do_importbeam(Deck &dk, Namelist &nl, std::string &err) {
         err = "mismatch between reference length of run and of input file";
         return false;
     }
-    if ((int)nbinsd != st.nbins) {
+    if ((int)nbinsd != st.nbins && !(st.one4one && (int)nbinsd == 1)) {
         err = "mismatch between beamlet size (NBINS) of run and of input file";
         return false;
     }
@@ -821,7 +832,8 @@ bool do_sponrad(Deck &dk, Namelist &nl, std::string &err) {
         return false;
     }
     sp.set = true;
-    if (dk.ctx && g4mi_set_incoherent(dk.ctx, sp.seed, dk.setup.nbins, sp.loss ? 1 : 0, sp.spread ? 1 : 0) != G4MI_OK) {
+    if (dk.ctx && g4mi_set_incoherent(dk.ctx, sp.seed, dk.setup.one4one ? 1 : dk.setup.nbins, sp.loss ? 1 : 0,
+                                      sp.spread ? 1 : 0) != G4MI_OK) {
         err = g4mi_last_error();
         return false;
     }
@@ -898,6 +910,13 @@ void write_version(H5File &f, bool &ok) {
 bool host_state(Deck &dk, int *first, std::string &err) {
     *first = 0;
     if (!dk.ctx) return true;
+    // one4one sorting moves particles between slices on the device
+    if (g4mi_get_slices(dk.ctx, dk.off.data(), dk.current.data()) != G4MI_OK) {
+        err = g4mi_last_error();
+        return false;
+    }
+    const size_t n = (size_t)dk.off.back();
+    for (auto *v : {&dk.gamma, &dk.theta, &dk.x, &dk.y, &dk.px, &dk.py}) v->resize(n);
     if (g4mi_beam_download(dk.ctx, dk.gamma.data(), dk.theta.data(), dk.x.data(), dk.y.data(), dk.px.data(),
                            dk.py.data()) != G4MI_OK ||
         g4mi_field_download(dk.ctx, dk.field.data(), first) != G4MI_OK) {
@@ -976,9 +995,9 @@ bool write_beam_dump(const Deck &dk, const std::string &basename, int stride, st
     ok &= f.write_single("slicelength", "m", {dk.setup.lambda0});
     ok &= f.write_single("slicespacing", "m", {dk.setup.lambda0 * dk.time.sample});
     ok &= f.write_single("refposition", "m", {dk.time.s0});
-    ok &= f.write_int("beamletsize", {dk.setup.nbins});
+    ok &= f.write_int("beamletsize", {dk.setup.one4one ? 1 : dk.setup.nbins});
     ok &= f.write_int("slicecount", {ns});
-    ok &= f.write_int("one4one", {0});
+    ok &= f.write_int("one4one", {dk.setup.one4one ? 1 : 0});
     if (stride < 1) stride = 1;
     char name[32];
     std::vector<double> w;
@@ -1121,7 +1140,7 @@ bool write_output(const Deck &dk, const std::string &fname, const std::vector<g4
     ok &= f.write_single("Global/lambdaref", "m", {st.lambda0});
     ok &= f.write_single("Global/sample", " ", {slicelength / st.lambda0});
     ok &= f.write_single("Global/slen", "m", {slicelength * ns});
-    ok &= f.write_single("Global/one4one", " ", {0.0});
+    ok &= f.write_single("Global/one4one", " ", {dk.setup.one4one ? 1.0 : 0.0});
     ok &= f.write_single("Global/time", " ", {dk.time.dotime ? 1.0 : 0.0});
     ok &= f.write_single("Global/scan", " ", {0.0});
     {
@@ -1382,22 +1401,29 @@ bool do_track(Deck &dk, Namelist &nl, int device, bool quiet, bool dry, std::str
             err = "&track: precision and periodic cannot change between &track elements";
             return false;
         }
-        // the once-values of the output describe the beam at the start of this run
-        if (g4mi_beam_download(dk.ctx, dk.gamma.data(), dk.theta.data(), dk.x.data(), dk.y.data(), dk.px.data(),
-                               dk.py.data()) != G4MI_OK) {
-            err = g4mi_last_error();
-            return false;
-        }
+        // the once-values of the output describe the beam at the start of
+        // this run (slices as the last sort left them)
+        int first = 0;
+        if (!host_state(dk, &first, err)) return false;
     }
     // the incoherent stream is seeded once (it runs on across &track
     // elements); the wake loss follows the current beam
     if (!dry && dk.spon.set && !dk.spon_applied) {
-        if (g4mi_set_incoherent(dk.ctx, dk.spon.seed, dk.setup.nbins, dk.spon.loss ? 1 : 0,
-                                dk.spon.spread ? 1 : 0) != G4MI_OK) {
+        if (g4mi_set_incoherent(dk.ctx, dk.spon.seed, dk.setup.one4one ? 1 : dk.setup.nbins,
+                                dk.spon.loss ? 1 : 0, dk.spon.spread ? 1 : 0) != G4MI_OK) {
             err = g4mi_last_error();
             return false;
         }
         dk.spon_applied = true;
+    }
+    // Beam::initSorting: one4one beams are sorted at the sort markers
+    if (g4mi_set_sorting(dk.ctx, dk.setup.one4one ? 1 : 0) != G4MI_OK) {
+        err = g4mi_last_error();
+        return false;
+    }
+    if (dk.wake.set && dk.setup.one4one) {
+        err = "&wake with one4one beams is not supported by this build (the loss would follow every sort)";
+        return false;
     }
     if (dk.wake.set) {
         const WakeNL &w = dk.wake;

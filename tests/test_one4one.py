@@ -148,3 +148,94 @@ def test_one4one_sase_with_sort_markers():
     assert np.array_equal(off, sim.off)
     assert ep < 1e-10 and eb < 1e-10
     ctx.close()
+
+
+O4O_IN = """&setup
+rootname={root}
+lattice={lat}
+beamline=FEL
+lambda0=1e-10
+gamma0=11357.82
+delz=0.045000
+one4one=true
+seed=9
+&end
+
+&time
+slen=6e-10
+sample=2
+&end
+
+&field
+power=5e3
+dgrid=2.000000e-04
+ngrid=31
+waist_size=30e-6
+&end
+
+&beam
+current=30
+delgam=1.000000
+ex=4.000000e-07
+ey=4.000000e-07
+betax=8
+betay=16
+&end
+
+&write
+beam = @.init
+&end
+
+&track
+zstop=3
+sort_step=10
+beam_dump_step=40
+{extra}
+&end
+"""
+
+
+def _o4o_deck(tmp_path, root, extra=""):
+    import test_deck as T
+    (tmp_path / "Example1.lat").write_text(T.EX1_LAT.format(extra=""))
+    path = tmp_path / "o4o.in"
+    path.write_text(O4O_IN.format(root=root, lat=str(tmp_path / "Example1.lat"), extra=extra))
+    return path
+
+
+def test_one4one_deck_setup(tmp_path):
+    """&setup one4one = true: electrons per slice round(I lambda0 sample /
+    ce), loaded as the oracle's one4one loader; the .par.h5 dump says
+    one4one = 1 with beamlets of 1."""
+    root = str(tmp_path / "o")
+    s = g4mi.check_deck(_o4o_deck(tmp_path, root))
+    ne = round(30 * 1e-10 * 2 / O.CE)
+    assert s["nslice"] == 3 and s["npart"] == 3 * ne
+    par = root + ".init.par.h5"
+    assert g4mi.h5_read(par, "one4one")[0] == 1 and g4mi.h5_read(par, "beamletsize")[0] == 1
+    sl = [dict(SLICE, current=30.0, delgam=1.0, betax=8.0, betay=16.0, alphax=0.0, alphay=0.0)] * 3
+    cnt = O.one4one_counts([30.0] * 3, LAM, 2)
+    sim = O.Sim(cnt, 31, 2e-4, LAM, sample=2)
+    O.load_beam_one4one(sim, sl, 9, LAM, 2)
+    for i in range(3):
+        a, b = int(sim.off[i]), int(sim.off[i + 1])
+        for k in ("gamma", "theta", "x", "y", "px", "py"):
+            assert np.array_equal(g4mi.h5_read(par, f"slice{i + 1:06d}/{k}"), getattr(sim, k)[a:b]), (i, k)
+
+
+@pytest.mark.gpu
+def test_one4one_deck_run(tmp_path):
+    """The one4one deck on the device: sorts every 10 steps, beam dumps hold
+    the sorted slices (their counts add up to the surviving particles) and
+    the slice currents follow np ce / slicelength."""
+    print("deck runner: one4one run with sorting and beam dumps", flush=True)
+    root = str(tmp_path / "o")
+    s = g4mi.check_deck(_o4o_deck(tmp_path, root))
+    g4mi.run_deck(_o4o_deck(tmp_path, root), device=0, quiet=True)
+    steps = list(g4mi.h5_read(root + ".out.h5", "Meta/Beamdumps/intstep"))
+    assert steps[0] == 0 and len(steps) >= 2
+    last = f"{root}.{int(steps[-1])}.par.h5"
+    n = [len(g4mi.h5_read(last, f"slice{i + 1:06d}/gamma")) for i in range(3)]
+    cur = [g4mi.h5_read(last, f"slice{i + 1:06d}/current")[0] for i in range(3)]
+    assert np.allclose(cur, np.array(n) * O.CE / (2 * LAM), rtol=1e-15)
+    assert sum(n) <= s["npart"]
