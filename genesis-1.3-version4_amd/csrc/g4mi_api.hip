@@ -97,6 +97,17 @@ struct Ctx {
     // wake loss per slice [eV/m] (Collective: wakeext + wakeint) or unset
     double *d_wake = nullptr, *d_wake_dg = nullptr, *d_wake_buf = nullptr;
     std::vector<double> h_wake, wake_applied;
+    // fields at harmonics of the fundamental (FP64 form): own grid and
+    // wavenumber, the fundamental's ring offset, Thomas ADI per field
+    struct HField {
+        int harm = 1, ng = 0;
+        double gridmax = 0, dgrid = 0, xks = 0;
+        double2 *d_field = nullptr, *d_work = nullptr, *d_src = nullptr, *d_fac = nullptr;
+        int4 *d_box = nullptr;
+        double fac_delz = 0, cstep = 0;
+        bool fac_ready = false;
+    };
+    std::vector<HField> hf;
     // one4one beams: Beam::sort at sort markers, with its work buffers
     bool one4one = false;
     SortWork sw{};
@@ -384,6 +395,16 @@ static int beam_track(Ctx *c, const g4mi_step_params &st, double gamma0, double 
     p.first = c->first;
     p.nslice = c->ns;
     p.use_sc = sc ? 1 : 0;
+    p.nh = (int)c->hf.size();
+    for (int k = 0; k < p.nh; k++) {
+        const Ctx::HField &h = c->hf[k];
+        p.hg[k].field = h.d_field;
+        p.hg[k].gridmax = h.gridmax;
+        p.hg[k].dgrid = h.dgrid;
+        p.hg[k].ngrid = h.ng;
+        p.hg[k].harm = (double)h.harm;
+        p.hg[k].rtmp = coupling(st, h.harm) / h.xks;  // BeamSolver.cpp:28
+    }
     launch_push(c->stream, c->nblk, c->d_blk_slice, c->d_blk_begin, c->d_off, c->d_g, c->d_th,
                 c->d_x, c->d_y, c->d_px, c->d_py, c->d_field, c->d_esc, c->d_ez, p);
     if (c->ev) G4_HIP(hipEventRecord(c->ev[3], c->stream));
@@ -560,15 +581,53 @@ static int fft_field_step(Ctx *c, double delz) {
     return G4MI_OK;
 }
 
+// Field::track of every harmonic field (FieldSolverADI per field, Field.cpp
+// :177-185): the source at harmonic h (odd h only, FieldSolverADI.cpp:19-22)
+// and the two-kernel Thomas ADI with the field's own factors.
+static int harmonics_track(Ctx *c, const g4mi_step_params &st, double delz) {
+    for (Ctx::HField &h : c->hf) {
+        if (!h.fac_ready || delz != h.fac_delz) {  // FieldSolverADI::init for this field
+            std::vector<double> f(6 * (size_t)h.ng);
+            h.cstep = adi_factors(h.ng, delz, h.dgrid, h.xks, f.data(), f.data() + 2 * h.ng, f.data() + 4 * h.ng);
+            G4_HIP(hipMemcpyAsync(h.d_fac, f.data(), f.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
+            G4_HIP(hipStreamSynchronize(c->stream));
+            h.fac_delz = delz;
+            h.fac_ready = true;
+        }
+        DepositParams d{};
+        d.enabled = (st.aw > 0 && (h.harm % 2 == 1)) ? 1 : 0;
+        d.harm = (double)h.harm;
+        d.fcoup = coupling(st, h.harm);
+        d.xks = h.xks;
+        d.delz = delz;
+        d.dgrid = h.dgrid;
+        d.gridmax = h.gridmax;
+        d.ax = st.ax;
+        d.ay = st.ay;
+        d.kx = st.kx;
+        d.ky = st.ky;
+        d.gradx = st.gradx;
+        d.grady = st.grady;
+        d.ngrid = h.ng;
+        launch_deposit(c->stream, c->ns, c->d_off, c->d_g, c->d_th, c->d_x, c->d_y, c->d_cur, h.d_src, h.d_box, d);
+        AdiFactors f{h.d_fac, h.d_fac + h.ng, h.d_fac + 2 * h.ng, h.cstep, h.ng};
+        launch_adi(c->stream, c->ns, h.d_field, h.d_work, h.d_src, h.d_box, f, c->first);
+    }
+    G4_HIP(hipGetLastError());
+    return G4MI_OK;
+}
+
 static int field_track(Ctx *c, const g4mi_step_params &st, double delz) {
     if (!c->has_field) {
         set_error("field not uploaded");
         return G4MI_E_STATE;
     }
+    if (!c->hf.empty()) G4_TRY(harmonics_track(c, st, delz));
     if (!c->fft) G4_TRY(ensure_factors(c, delz));
     if (c->ev) G4_HIP(hipEventRecord(c->ev[5], c->stream));
     DepositParams d{};
     d.enabled = st.aw > 0 ? 1 : 0;  // in undulator, fundamental
+    d.harm = 1.0;
     d.fcoup = coupling(st, 1);
     d.xks = c->xks;
     d.delz = delz;
@@ -615,6 +674,8 @@ static int slippage(Ctx *c, double slip) {
         if (!c->periodic) {
             launch_zero(c->stream, c->d_field + (long long)last * cells, cells);
             c->zeroed_since_adi.push_back(last);
+            for (const Ctx::HField &h : c->hf)  // every field slips alike
+                launch_zero(c->stream, h.d_field + (long long)last * h.ng * h.ng, (long long)h.ng * h.ng);
         }
         c->first = last;
         if (direction < 0) c->first = (last + 1) % c->ns;
@@ -625,6 +686,63 @@ static int slippage(Ctx *c, double slip) {
 // Normalisation of the per-slice sums: hb[8*is] beam sums per beam slice,
 // hf[8*is0] field sums per storage slice, mapped to beam slices through the
 // ring offset `first`.
+// DiagField::getValues (Diagnostic.cpp:754-969) from the per-storage-slice
+// field sums hf[8 is0] of a field on a grid of spacing dgrid with
+// wavenumber ks; first = ring offset.
+static void finish_field(const std::vector<double> &hf, int ns, int first, double dgrid, double ks,
+                         double slicelength, g4mi_diag_out *o) {
+    const double scl = dgrid * kEev / ks;
+    double g_pow = 0, fx1 = 0, fx2 = 0, fy1 = 0, fy2 = 0, g_ff = 0, g_in = 0;
+    for (int is0 = 0; is0 < ns; is0++) {
+        const int is = (ns + is0 - first) % ns;
+        const double *v = &hf[8 * is0];
+        double power = v[0], x1 = v[1], x2 = v[2], y1 = v[3], y2 = v[4];
+        g_pow += power;
+        fx1 += x1;
+        fx2 += x2;
+        fy1 += y1;
+        fy2 += y2;
+        if (o->ff_phase) o->ff_phase[is] = atan2(v[6], v[5]);
+        if (power > 0) {
+            x1 /= power;
+            x2 /= power;
+            y1 /= power;
+            y2 /= power;
+        }
+        x2 = sqrt(fabs(x2 - x1 * x1));
+        y2 = sqrt(fabs(y2 - y1 * y1));
+        double farfield = v[5] * v[5] + v[6] * v[6];
+        farfield *= dgrid * dgrid;
+        double inten = v[7];
+        g_ff += farfield;
+        g_in += inten;
+        power *= scl * scl / kVacimp;
+        inten *= kEev * kEev / ks / ks / kVacimp;
+        if (o->power) o->power[is] = power;
+        if (o->f_xposition) o->f_xposition[is] = x1 * dgrid;
+        if (o->f_xsize) o->f_xsize[is] = x2 * dgrid;
+        if (o->f_yposition) o->f_yposition[is] = y1 * dgrid;
+        if (o->f_ysize) o->f_ysize[is] = y2 * dgrid;
+        if (o->nf_intensity) o->nf_intensity[is] = inten;
+        if (o->ff_intensity) o->ff_intensity[is] = farfield;
+    }
+    {
+        const double norm = g_pow > 0 ? 1. / g_pow : 1.;
+        fx1 *= norm, fx2 *= norm, fy1 *= norm, fy2 *= norm;
+        if (o->g_f_xsize) o->g_f_xsize[0] = sqrt(fabs(fx2 - fx1 * fx1)) * dgrid;
+        if (o->g_f_ysize) o->g_f_ysize[0] = sqrt(fabs(fy2 - fy1 * fy1)) * dgrid;
+        if (o->g_f_xposition) o->g_f_xposition[0] = fx1 * dgrid;
+        if (o->g_f_yposition) o->g_f_yposition[0] = fy1 * dgrid;
+        const double ninv = 1. / (double)ns;
+        if (o->g_nf_intensity) o->g_nf_intensity[0] = g_in * ninv * kEev * kEev / ks / ks / kVacimp;
+        if (o->g_ff_intensity) o->g_ff_intensity[0] = g_ff * ninv;
+    }
+    if (o->global_energy) {
+        g_pow *= scl * scl / kVacimp * slicelength / 299792458.0;
+        o->global_energy[0] = g_pow;
+    }
+}
+
 static void finish_diag(const Ctx *c, const std::vector<double> &hb, const std::vector<double> &hf,
                         int first, g4mi_diag_out *o) {
     const int ns = c->ns;
@@ -664,58 +782,7 @@ static void finish_diag(const Ctx *c, const std::vector<double> &hb, const std::
         if (o->g_yposition) o->g_yposition[0] = gy1;
         if (o->g_ysize) o->g_ysize[0] = sqrt(fabs(gy2 - gy1 * gy1));
     }
-    // DiagField::getValues (Diagnostic.cpp:754-969)
-    const double ks = 4. * asin(1) / c->xlambda;
-    const double scl = c->dgrid * kEev / ks;
-    double g_pow = 0, fx1 = 0, fx2 = 0, fy1 = 0, fy2 = 0, g_ff = 0, g_in = 0;
-    for (int is0 = 0; is0 < ns; is0++) {
-        const int is = (ns + is0 - first) % ns;
-        const double *v = &hf[8 * is0];
-        double power = v[0], x1 = v[1], x2 = v[2], y1 = v[3], y2 = v[4];
-        g_pow += power;
-        fx1 += x1;
-        fx2 += x2;
-        fy1 += y1;
-        fy2 += y2;
-        if (o->ff_phase) o->ff_phase[is] = atan2(v[6], v[5]);
-        if (power > 0) {
-            x1 /= power;
-            x2 /= power;
-            y1 /= power;
-            y2 /= power;
-        }
-        x2 = sqrt(fabs(x2 - x1 * x1));
-        y2 = sqrt(fabs(y2 - y1 * y1));
-        double farfield = v[5] * v[5] + v[6] * v[6];
-        farfield *= c->dgrid * c->dgrid;
-        double inten = v[7];
-        g_ff += farfield;
-        g_in += inten;
-        power *= scl * scl / kVacimp;
-        inten *= kEev * kEev / ks / ks / kVacimp;
-        if (o->power) o->power[is] = power;
-        if (o->f_xposition) o->f_xposition[is] = x1 * c->dgrid;
-        if (o->f_xsize) o->f_xsize[is] = x2 * c->dgrid;
-        if (o->f_yposition) o->f_yposition[is] = y1 * c->dgrid;
-        if (o->f_ysize) o->f_ysize[is] = y2 * c->dgrid;
-        if (o->nf_intensity) o->nf_intensity[is] = inten;
-        if (o->ff_intensity) o->ff_intensity[is] = farfield;
-    }
-    {
-        const double norm = g_pow > 0 ? 1. / g_pow : 1.;
-        fx1 *= norm, fx2 *= norm, fy1 *= norm, fy2 *= norm;
-        if (o->g_f_xsize) o->g_f_xsize[0] = sqrt(fabs(fx2 - fx1 * fx1)) * c->dgrid;
-        if (o->g_f_ysize) o->g_f_ysize[0] = sqrt(fabs(fy2 - fy1 * fy1)) * c->dgrid;
-        if (o->g_f_xposition) o->g_f_xposition[0] = fx1 * c->dgrid;
-        if (o->g_f_yposition) o->g_f_yposition[0] = fy1 * c->dgrid;
-        const double ninv = 1. / (double)ns;
-        if (o->g_nf_intensity) o->g_nf_intensity[0] = g_in * ninv * kEev * kEev / ks / ks / kVacimp;
-        if (o->g_ff_intensity) o->g_ff_intensity[0] = g_ff * ninv;
-    }
-    if (o->global_energy) {
-        g_pow *= scl * scl / kVacimp * c->slicelength / 299792458.0;
-        o->global_energy[0] = g_pow;
-    }
+    finish_field(hf, ns, first, c->dgrid, 4. * asin(1) / c->xlambda, c->slicelength, o);
 }
 
 static bool wants_aux(const g4mi_diag_out *o) {
@@ -1733,6 +1800,95 @@ int g4mi_set_incoherent(g4mi_ctx *h, int seed, int nbins, int do_loss, int do_sp
     return G4MI_OK;
 }
 
+int g4mi_add_harmonic(g4mi_ctx *hc, int harm, int ngrid, double gridmax, int *index) {
+    if (!hc || harm < 2 || ngrid < 3 || !(gridmax > 0)) {
+        set_error("g4mi_add_harmonic: invalid argument (harm >= 2, ngrid >= 3, gridmax > 0)");
+        return G4MI_E_ARG;
+    }
+    Ctx *c = &hc->c;
+    if (c->precision != G4MI_FP64) {
+        set_error("g4mi_add_harmonic: harmonic fields run in the FP64 form only");
+        return G4MI_E_ARG;
+    }
+    if (c->fft) {
+        set_error("g4mi_add_harmonic: harmonic fields run with the ADI solver only");
+        return G4MI_E_ARG;
+    }
+    if ((int)c->hf.size() >= kMaxHarm) {
+        set_error("g4mi_add_harmonic: at most 4 harmonic fields besides the fundamental");
+        return G4MI_E_ARG;
+    }
+    for (const Ctx::HField &e : c->hf)
+        if (e.harm == harm) {
+            set_error("g4mi_add_harmonic: a field at this harmonic exists already");
+            return G4MI_E_ARG;
+        }
+    Ctx::HField h;
+    h.harm = harm;
+    h.ng = ngrid;
+    h.gridmax = gridmax;
+    h.dgrid = 2 * gridmax / (double)(ngrid - 1);  // Field::init :45-46
+    h.xks = 4. * asin(1) / (c->lambda0 / (double)harm);
+    const size_t cells = (size_t)c->ns * ngrid * ngrid;
+    G4_TRY(dalloc(c, &h.d_field, cells));
+    G4_TRY(dalloc(c, &h.d_work, cells));
+    G4_TRY(dalloc(c, &h.d_src, cells));
+    G4_TRY(dalloc(c, &h.d_fac, 3 * (size_t)ngrid));
+    G4_TRY(dalloc(c, &h.d_box, (size_t)c->ns));
+    G4_HIP(hipMemsetAsync(h.d_field, 0, cells * sizeof(double2), c->stream));
+    G4_HIP(hipStreamSynchronize(c->stream));
+    c->hf.push_back(h);
+    if (index) *index = (int)c->hf.size() - 1;
+    return G4MI_OK;
+}
+
+static int harm_of(g4mi_ctx *hc, int index, Ctx::HField **h) {
+    if (!hc || index < 0 || index >= (int)hc->c.hf.size()) {
+        set_error("harmonic field index out of range");
+        return G4MI_E_ARG;
+    }
+    *h = &hc->c.hf[index];
+    return G4MI_OK;
+}
+
+int g4mi_harmonic_upload(g4mi_ctx *hc, int index, const double *field) {
+    Ctx::HField *h = nullptr;
+    G4_TRY(harm_of(hc, index, &h));
+    if (!field) return G4MI_E_ARG;
+    G4_HIP(hipMemcpy(h->d_field, field, (size_t)hc->c.ns * h->ng * h->ng * sizeof(double2), hipMemcpyHostToDevice));
+    return G4MI_OK;
+}
+
+int g4mi_harmonic_download(g4mi_ctx *hc, int index, double *field) {
+    Ctx::HField *h = nullptr;
+    G4_TRY(harm_of(hc, index, &h));
+    if (!field) return G4MI_E_ARG;
+    G4_HIP(hipStreamSynchronize(hc->c.stream));
+    G4_HIP(hipMemcpy(field, h->d_field, (size_t)hc->c.ns * h->ng * h->ng * sizeof(double2), hipMemcpyDeviceToHost));
+    return G4MI_OK;
+}
+
+int g4mi_diag_harmonic(g4mi_ctx *hc, int index, g4mi_diag_out *out) {
+    Ctx::HField *h = nullptr;
+    G4_TRY(harm_of(hc, index, &h));
+    if (!out) return G4MI_E_ARG;
+    Ctx *c = &hc->c;
+    const int ns = c->ns;
+    launch_diag_field(c->stream, ns, h->d_field, h->ng, c->d_diag_f);
+    std::vector<double> hf(8 * (size_t)ns);
+    G4_HIP(hipMemcpyAsync(hf.data(), c->d_diag_f, hf.size() * 8, hipMemcpyDeviceToHost, c->stream));
+    G4_HIP(hipStreamSynchronize(c->stream));
+    c->field_diag_fresh = false;  // the fundamental's sums in d_diag_f were overwritten
+    finish_field(hf, ns, c->first, h->dgrid, h->xks, c->slicelength, out);
+    return G4MI_OK;
+}
+
+int g4mi_harmonic_power(g4mi_ctx *hc, int index, double *power) {
+    g4mi_diag_out o{};
+    o.power = power;
+    return g4mi_diag_harmonic(hc, index, &o);
+}
+
 int g4mi_set_sorting(g4mi_ctx *h, int one4one) {
     if (!h) {
         set_error("g4mi_set_sorting: null context");
@@ -1833,6 +1989,10 @@ int g4mi_set_field_solver(g4mi_ctx *h, int fft, int filter, double xcut, double 
     Ctx *c = &h->c;
     if (fft && c->precision != G4MI_FP64) {
         set_error("g4mi_set_field_solver: the spectral solver runs in the FP64 form only");
+        return G4MI_E_ARG;
+    }
+    if (fft && !c->hf.empty()) {
+        set_error("g4mi_set_field_solver: the spectral solver does not run harmonic fields");
         return G4MI_E_ARG;
     }
     c->fft = fft ? 1 : 0;

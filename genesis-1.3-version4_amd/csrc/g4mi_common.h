@@ -45,17 +45,29 @@ struct TransParams {
 };
 
 // Parameters of BeamSolver::advance for the fundamental.
+// A field at a harmonic of the fundamental as the push gathers it
+// (BeamSolver::advance :23-32, 63-79): its grid, rtmp = fc(h) / xks_h and h.
+struct HarmGather {
+    const double2 *field;
+    double gridmax, dgrid, rtmp, harm;
+    int ngrid;
+};
+constexpr int kMaxHarm = 4;  // harmonic fields besides the fundamental
+
 struct PushParams {
     double delz, xks, xku, rtmp, aw, autophase;
     double ax, ay, kx, ky, gradx, grady;   // faw(x,y)
     double gridmax, dgrid;
     int ngrid, first, nslice, use_sc;
+    int nh;                                // harmonic fields, 0..kMaxHarm
+    HarmGather hg[kMaxHarm];
 };
 
 // Parameters of the source deposition (FieldSolverADI::advance :13-51).
 struct DepositParams {
     double fcoup, xks, delz, dgrid, gridmax;
     double ax, ay, kx, ky, gradx, grady;
+    double harm;                           // phase factor h of sin(h theta), cos(h theta)
     int ngrid, enabled;
 };
 

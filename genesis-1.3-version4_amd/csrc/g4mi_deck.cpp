@@ -200,6 +200,12 @@ struct Deck {
     std::vector<long long> off;
     std::vector<double> gamma, theta, x, y, px, py, current;
     std::vector<double> field;  // [ns][ng^2][2]
+    struct HarmField {
+        int harm = 3, ngrid = 0;
+        double gridmax = 0;
+        std::vector<double> field;  // [ns][ng^2][2], storage order of the fundamental's ring
+    };
+    std::vector<HarmField> hfields;  // &field harm > 1, in definition order
     int ngrid = 0;
     double gridmax = 0, flambda = 0;
     bool have_beam = false, have_field = false, spon_applied = false;
@@ -431,11 +437,36 @@ bool do_field(Deck &dk, Namelist &nl, std::string &err) {
         err = k.err;
         return false;
     }
-    if (harm != 1 || nx != 0 || ny != 0) {
-        err = "&field: only the fundamental (harm = 1) TEM00 mode (nx = ny = 0) is supported";
+    if (harm < 1 || nx != 0 || ny != 0) {
+        err = "&field: only TEM00 modes (nx = ny = 0) at harm >= 1 are supported";
         return false;
     }
-    if (add && dk.have_field) {
+    Deck::HarmField *hfp = nullptr;  // LoadField.cpp:99-116: the record of this harmonic
+    if (harm > 1 && fabs(lambda - dk.setup.lambda0) > 1e-12 * dk.setup.lambda0) {
+        err = "&field: harmonic fields at a wavelength other than lambda0 are not supported by this build";
+        return false;
+    }
+    if (harm > 1) {
+        if (!dk.have_field) {
+            err = "&field: define the fundamental (harm = 1) before its harmonics";
+            return false;
+        }
+        for (auto &e : dk.hfields)
+            if (e.harm == harm) hfp = &e;
+        if (!hfp) {
+            if (dk.hfields.size() >= 4) {
+                err = "&field: at most 4 harmonic fields";
+                return false;
+            }
+            dk.hfields.push_back(Deck::HarmField());
+            hfp = &dk.hfields.back();
+            hfp->harm = harm;
+            add = false;
+        } else if (add) {
+            dgrid = hfp->gridmax;
+            ngrid = hfp->ngrid;
+        }
+    } else if (add && dk.have_field) {
         dgrid = dk.gridmax;
         ngrid = dk.ngrid;
     } else {
@@ -450,17 +481,23 @@ bool do_field(Deck &dk, Namelist &nl, std::string &err) {
     const double l0 = dk.setup.lambda0;
     const double dlam = -(lambda - l0) / l0 / l0 * 4 * asin(1);  // LoadField.cpp
     const size_t cells = (size_t)ngrid * ngrid;
-    if (!add) dk.field.assign((size_t)ns * cells * 2, 0.0);
+    std::vector<double> &target = hfp ? hfp->field : dk.field;
+    if (!add) target.assign((size_t)ns * cells * 2, 0.0);
     std::vector<double> slice(cells * 2);
     for (int i = 0; i < ns; i++) {
         const double s = dk.time.s0 + i * ds;
         gauss_field(slice.data(), ngrid, dgrid, lambda, power, w0, z0, fmod(phase + s * dlam, 4 * asin(1.)), xcen,
-                    ycen, xangle, yangle);
-        double *dst = dk.field.data() + (size_t)i * cells * 2;
+                    ycen, xangle, yangle, harm);
+        double *dst = target.data() + (size_t)i * cells * 2;
         if (add)
             for (size_t q = 0; q < cells * 2; q++) dst[q] += slice[q];
         else
             std::memcpy(dst, slice.data(), cells * 2 * sizeof(double));
+    }
+    if (hfp) {
+        hfp->ngrid = ngrid;
+        hfp->gridmax = dgrid;
+        return true;
     }
     dk.ngrid = ngrid;
     dk.gridmax = dgrid;
@@ -923,13 +960,38 @@ bool host_state(Deck &dk, int *first, std::string &err) {
         err = g4mi_last_error();
         return false;
     }
+    for (size_t k = 0; k < dk.hfields.size(); k++)
+        if (g4mi_harmonic_download(dk.ctx, (int)k, dk.hfields[k].field.data()) != G4MI_OK) {
+            err = g4mi_last_error();
+            return false;
+        }
     return true;
 }
 
 // <basename>.fld.h5 (WriteFieldHDF5::writeMain, src/IO/writeFieldHDF5.cpp:42-171):
 // globals, one group per slice in window order with the field scaled to
 // sqrt(W) per cell, and the intensity projections int_xy, int_xz, int_yz.
+bool write_field_dump_of(const Deck &dk, const std::vector<double> &field, int ngrid, double gridmax,
+                         double xlambda, double slicelen, const std::string &basename, int first,
+                         std::string &err);
+
+// every field: the fundamental as <basename>.fld.h5, harmonic h as
+// <basename>.h<h>.fld.h5 (WriteFieldHDF5::write, writeFieldHDF5.cpp:13-39)
 bool write_field_dump(const Deck &dk, const std::string &basename, int first, std::string &err) {
+    if (!write_field_dump_of(dk, dk.field, dk.ngrid, dk.gridmax, dk.flambda, dk.flambda * dk.time.sample,
+                             basename, first, err))
+        return false;
+    for (const auto &h : dk.hfields)  // Field::init: xlambda = lambda / harm
+        if (!write_field_dump_of(dk, h.field, h.ngrid, h.gridmax, dk.setup.lambda0 / h.harm,
+                                 dk.setup.lambda0 * dk.time.sample, basename + ".h" + std::to_string(h.harm), first,
+                                 err))
+            return false;
+    return true;
+}
+
+bool write_field_dump_of(const Deck &dk, const std::vector<double> &field, int ngrid, double gridmax,
+                         double xlambda, double slicelen, const std::string &basename, int first,
+                         std::string &err) {
     const std::string fname = basename + ".fld.h5";
     H5File f(fname);
     if (!f.ok()) {
@@ -938,21 +1000,21 @@ bool write_field_dump(const Deck &dk, const std::string &basename, int first, st
     }
     bool ok = true;
     write_version(f, ok);
-    const int ns = (int)dk.off.size() - 1, ng = dk.ngrid;
+    const int ns = (int)dk.off.size() - 1, ng = ngrid;
     const size_t cells = (size_t)ng * ng;
-    const double dgrid = 2 * dk.gridmax / (ng - 1);
-    ok &= f.write_single("wavelength", "m", {dk.flambda});
-    ok &= f.write_single("slicespacing", "m", {dk.flambda * dk.time.sample});
+    const double dgrid = 2 * gridmax / (ng - 1);
+    ok &= f.write_single("wavelength", "m", {xlambda});
+    ok &= f.write_single("slicespacing", "m", {slicelen});
     ok &= f.write_single("refposition", "m", {dk.time.s0});
     ok &= f.write_single("gridsize", "m", {dgrid});
     ok &= f.write_int("gridpoints", {ng});
     ok &= f.write_int("slicecount", {ns});
-    const double ks = 4. * asin(1) / dk.flambda;
+    const double ks = 4. * asin(1) / xlambda;
     const double scl = dgrid * kEevD / ks / sqrt(kVacimpD);
     std::vector<double> re(cells), im(cells), ixy(cells, 0.0), ixz((size_t)ng * ns, 0.0), iyz((size_t)ng * ns, 0.0);
     char name[32];
     for (int i = 0; i < ns; i++) {
-        const double *fs = dk.field.data() + (size_t)((i + first) % ns) * cells * 2;
+        const double *fs = field.data() + (size_t)((i + first) % ns) * cells * 2;
         for (size_t j = 0; j < cells; j++) {
             re[j] = scl * fs[2 * j];
             im[j] = scl * fs[2 * j + 1];
@@ -1258,6 +1320,48 @@ bool write_output(const Deck &dk, const std::string &fname, const std::vector<g4
     ok &= f.write_single("Field/gridspacing", "m", {dgrid});
     ok &= f.write_single("Field/dgrid", "m", {dk.gridmax});
     ok &= f.write_single("Field/ngrid", " ", {(double)dk.ngrid});
+    // the harmonic fields as groups Field<h> (Diagnostic::writeToOutputFile, Diagnostic.cpp:219-225)
+    for (size_t k = 0; k < dk.hfields.size(); k++) {
+        const auto &h = dk.hfields[k];
+        const std::string g = "Field" + std::to_string(h.harm) + "/", pre = "H" + std::to_string(k) + ":";
+        auto hput = [&](const char *path, const char *unit, const char *key) {
+            auto it = rec.find(pre + key);
+            if (it != rec.end()) ok &= f.write_buffer(g + path, unit, it->second, nrec, (size_t)ns);
+        };
+        auto hput1 = [&](const char *path, const char *unit, const char *key) {
+            auto it = rec.find(pre + key);
+            if (it != rec.end() && nrec > 0) ok &= f.write_single(g + path, unit, it->second);
+        };
+        hput("power", "W", "power");
+        if (!st.excl_spatial) {
+            hput("xposition", "m", "f_xposition");
+            hput("xsize", "m", "f_xsize");
+            hput("yposition", "m", "f_yposition");
+            hput("ysize", "m", "f_ysize");
+        }
+        if (!st.excl_intensity) {
+            hput("intensity-nearfield", "W/m^2", "nf_intensity");
+            hput("intensity-farfield", "W/rad^2", "ff_intensity");
+            hput("phase-farfield", "rad", "ff_phase");
+        }
+        if (st.field_global_stat && ns > 1) {
+            hput1("Global/energy", "J", "global_energy");
+            if (!st.excl_spatial) {
+                hput1("Global/xposition", "m", "g_f_xposition");
+                hput1("Global/xsize", "m", "g_f_xsize");
+                hput1("Global/yposition", "m", "g_f_yposition");
+                hput1("Global/ysize", "m", "g_f_ysize");
+            }
+            if (!st.excl_intensity) {
+                hput1("Global/intensity-nearfield", "W/m^2", "g_nf_intensity");
+                hput1("Global/intensity-farfield", "W/rad^2", "g_ff_intensity");
+            }
+        }
+        const double hd = 2 * h.gridmax / (h.ngrid - 1);
+        ok &= f.write_single(g + "gridspacing", "m", {hd});
+        ok &= f.write_single(g + "dgrid", "m", {h.gridmax});
+        ok &= f.write_single(g + "ngrid", " ", {(double)h.ngrid});
+    }
     f.close();
     if (!ok) err = "writing " + fname + " failed";
     return ok;
@@ -1396,6 +1500,14 @@ bool do_track(Deck &dk, Namelist &nl, int device, bool quiet, bool dry, std::str
             err = g4mi_last_error();
             return false;
         }
+        for (const auto &h : dk.hfields) {  // Field records at harmonics (LoadField with harm)
+            int idx = -1;
+            if (g4mi_add_harmonic(dk.ctx, h.harm, h.ngrid, h.gridmax, &idx) != G4MI_OK ||
+                g4mi_harmonic_upload(dk.ctx, idx, h.field.data()) != G4MI_OK) {
+                err = g4mi_last_error();
+                return false;
+            }
+        }
     } else {
         if (precision != dk.precision || periodic != dk.periodic) {
             err = "&track: precision and periodic cannot change between &track elements";
@@ -1443,6 +1555,15 @@ bool do_track(Deck &dk, Namelist &nl, int device, bool quiet, bool dry, std::str
     }
     for (const char *key : kRecKeys) rec[key].assign(nrec * (size_t)ns, 0.0);
     for (const char *key : kRecScalars) rec[key].assign(nrec, 0.0);
+    for (size_t k = 0; k < dk.hfields.size(); k++) {
+        const std::string pre = "H" + std::to_string(k) + ":";
+        for (const char *key : {"power", "f_xposition", "f_xsize", "f_yposition", "f_ysize", "nf_intensity",
+                                "ff_intensity", "ff_phase"})
+            rec[pre + key].assign(nrec * (size_t)ns, 0.0);
+        for (const char *key : {"global_energy", "g_f_xposition", "g_f_xsize", "g_f_yposition", "g_f_ysize",
+                                "g_nf_intensity", "g_ff_intensity"})
+            rec[pre + key].assign(nrec, 0.0);
+    }
     auto record = [&](size_t r) {
         g4mi_diag_out d{};
         auto ps = [&](const char *k) { return rec[k].data() + r * ns; };
@@ -1505,7 +1626,30 @@ bool do_track(Deck &dk, Namelist &nl, int device, bool quiet, bool dry, std::str
         d.nf_intensity = rec["nf_intensity"].data() + r * ns;
         d.ff_intensity = rec["ff_intensity"].data() + r * ns;
         d.global_energy = rec["global_energy"].data() + r;
-        return g4mi_diag(dk.ctx, &d);
+        int rc = g4mi_diag(dk.ctx, &d);
+        for (size_t k = 0; rc == G4MI_OK && k < dk.hfields.size(); k++) {  // DiagField of every harmonic
+            const std::string pre = "H" + std::to_string(k) + ":";
+            g4mi_diag_out dh{};
+            auto hs = [&](const char *key) { return rec[pre + key].data() + r * ns; };
+            auto h1 = [&](const char *key) { return rec[pre + key].data() + r; };
+            dh.power = hs("power");
+            dh.f_xposition = hs("f_xposition");
+            dh.f_xsize = hs("f_xsize");
+            dh.f_yposition = hs("f_yposition");
+            dh.f_ysize = hs("f_ysize");
+            dh.nf_intensity = hs("nf_intensity");
+            dh.ff_intensity = hs("ff_intensity");
+            dh.ff_phase = hs("ff_phase");
+            dh.global_energy = h1("global_energy");
+            dh.g_f_xposition = h1("g_f_xposition");
+            dh.g_f_xsize = h1("g_f_xsize");
+            dh.g_f_yposition = h1("g_f_yposition");
+            dh.g_f_ysize = h1("g_f_ysize");
+            dh.g_nf_intensity = h1("g_nf_intensity");
+            dh.g_ff_intensity = h1("g_ff_intensity");
+            rc = g4mi_diag_harmonic(dk.ctx, (int)k, &dh);
+        }
+        return rc;
     };
     if (record(0) != G4MI_OK) {
         err = g4mi_last_error();

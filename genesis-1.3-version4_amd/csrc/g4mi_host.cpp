@@ -584,9 +584,9 @@ void shot_noise(Ran2 &stream, double *th, int npart, int nbins, double ne) {
 
 void gauss_field(double *field, int ngrid, double gridmax, double lambda, double power,
                  double w0, double z0in, double phase, double xcen, double ycen, double xangle,
-                 double yangle) {
+                 double yangle, int harm) {
     typedef std::complex<double> cd;
-    const double k = 4. * asin(1) / lambda * (double)1;
+    const double k = 4. * asin(1) / lambda * (double)harm;
     const double z0 = -z0in;
     const double zr = w0 * w0 * k * 0.5;
     const double f0 = sqrt(k * zr / (zr * zr + z0 * z0));

@@ -81,10 +81,11 @@ void shot_noise(Ran2 &stream, double *theta, int npart, int nbins, double ne);
 // Seed stream of ShotNoise::init (:16-27).
 unsigned int shot_noise_seed(int seed, int rank);
 double inverse_erfc(double y);
-// Gaussian seed field (src/Util/GaussHermite.cpp:20-63), fundamental mode.
+// Gaussian seed field (src/Util/GaussHermite.cpp:20-63), TEM00 at harmonic harm
+// (wavenumber harm 2 pi / lambda).
 void gauss_field(double *field, int ngrid, double gridmax, double lambda, double power,
                  double w0, double z0, double phase, double xcen, double ycen, double xangle = 0.0,
-                 double yangle = 0.0);
+                 double yangle = 0.0, int harm = 1);
 
 // Lattice file (LatticeParser.cpp) -> unrolled steps (Lattice.cpp).
 struct LatticeResult {

@@ -382,15 +382,24 @@ __global__ void __launch_bounds__(256) k_sc_solve(
 // RungeKutta (:89-141), ODE (:144-163), field gather in the order of
 // Field::getLLGridpoint (src/Core/Field.cpp:160-174) + :66-79.
 // ---------------------------------------------------------------------------
+template <int NH>
 struct Ode {
     double xks, xku, btpar, ez;
     double2 rpart;
+    double2 rph[NH > 0 ? NH : 1];
+    double hh[NH > 0 ? NH : 1];
     double k2gg, k2pp;
     __device__ __forceinline__ void eval(double tgam, double tthet) {
         double ztemp1 = -2. / xks;
         double s, c;
         sincos(1.0 * tthet, &s, &c);
         double2 ctmp = cmul(rpart, make_double2(c, -s));
+#pragma unroll
+        for (int k = 0; k < NH; k++) {  // the harmonic fields, in field order
+            double sh, ch;
+            sincos(hh[k] * tthet, &sh, &ch);
+            ctmp = cadd(ctmp, cmul(rph[k], make_double2(ch, -sh)));
+        }
         double btper0 = btpar + ztemp1 * ctmp.x;
         double btpar0 = sqrt(1. - btper0 / (tgam * tgam));
         k2pp += xks * (1. - 1. / btpar0) + xku;
@@ -398,6 +407,7 @@ struct Ode {
     }
 };
 
+template <int NH>
 __global__ void __launch_bounds__(256) k_push(const int *__restrict__ blk_slice,
                                               const long long *__restrict__ blk_begin,
                                               const long long *__restrict__ off,
@@ -417,7 +427,7 @@ __global__ void __launch_bounds__(256) k_push(const int *__restrict__ blk_slice,
     const double xi = x[i], yi = y[i], pxi = px[i], pyi = py[i];
     double dx = xi - p.ax, dy = yi - p.ay;
     double awloc = (1 + 0.5 * (p.kx * dx * dx + p.ky * dy * dy) + p.gradx * dx + p.grady * dy);
-    Ode o;
+    Ode<NH> o;
     o.xks = p.xks;
     o.xku = p.xku;
     o.btpar = 1 + pxi * pxi + pyi * pyi + p.aw * p.aw * awloc * awloc;
@@ -445,6 +455,27 @@ __global__ void __launch_bounds__(256) k_push(const int *__restrict__ blk_slice,
         o.rpart = make_double2(r * cpart.x, r * -cpart.y);
     } else {
         o.rpart = make_double2(0, 0);
+    }
+#pragma unroll
+    for (int k = 0; k < NH; k++) {  // BeamSolver::advance :63-79 for each harmonic field
+        const HarmGather &h = p.hg[k];
+        o.hh[k] = h.harm;
+        o.rph[k] = make_double2(0, 0);
+        if ((xi > -h.gridmax) && (xi < h.gridmax) && (yi > -h.gridmax) && (yi < h.gridmax)) {
+            double wx = (xi + h.gridmax) / h.dgrid;
+            double wy = (yi + h.gridmax) / h.dgrid;
+            int ix = min((int)floor(wx), h.ngrid - 2), iy = min((int)floor(wy), h.ngrid - 2);
+            wx = 1 + floor(wx) - wx;
+            wy = 1 + floor(wy) - wy;
+            const int idx = ix + iy * h.ngrid;
+            const double2 *f = h.field + (long long)((is + p.first) % p.nslice) * h.ngrid * h.ngrid;
+            double2 cpart = cscale(cscale(f[idx], wx), wy);
+            cpart = cadd(cpart, cscale(cscale(f[idx + 1], 1 - wx), wy));
+            cpart = cadd(cpart, cscale(cscale(f[idx + h.ngrid], wx), 1 - wy));
+            cpart = cadd(cpart, cscale(cscale(f[idx + h.ngrid + 1], 1 - wx), 1 - wy));
+            const double r = h.rtmp * awloc;
+            o.rph[k] = make_double2(r * cpart.x, r * -cpart.y);
+        }
     }
     const double delz = p.delz;
     // RungeKutta, :93-141
@@ -564,7 +595,7 @@ __global__ void __launch_bounds__(1024) k_deposit(const long long *__restrict__ 
             double faw2 = (1 + p.kx * ddx * ddx + p.ky * ddy * ddy + 2 * (p.gradx * ddx + p.grady * ddy));
             double part = sqrt(faw2) * scl / g[i];
             double s, c;
-            sincos(1.0 * th[i], &s, &c);
+            sincos(p.harm * th[i], &s, &c);
             double2 cp = make_double2(s * part, c * part);
             double wgt[4] = {wx * wy, (1 - wx) * wy, wx * (1 - wy), (1 - wx) * (1 - wy)};
             int cx[4] = {ix, ix + 1, ix, ix + 1};
@@ -1443,9 +1474,18 @@ void launch_push(hipStream_t s, int nblocks, const int *blk_slice, const long lo
                  const long long *off, double *g, double *th, const double *x, const double *y,
                  const double *px, const double *py, const double2 *field, const double *esc,
                  const double *ezsc, const PushParams &p) {
-    if (nblocks > 0)
-        hipLaunchKernelGGL(k_push, dim3(nblocks), dim3(256), 0, s, blk_slice, blk_begin, off, g, th,
-                           x, y, px, py, field, esc, ezsc, p);
+    if (nblocks <= 0) return;
+#define G4_PUSH(NH)                                                                                     \
+    hipLaunchKernelGGL(k_push<NH>, dim3(nblocks), dim3(256), 0, s, blk_slice, blk_begin, off, g, th, x, y, \
+                       px, py, field, esc, ezsc, p)
+    switch (p.nh) {
+        case 1: G4_PUSH(1); break;
+        case 2: G4_PUSH(2); break;
+        case 3: G4_PUSH(3); break;
+        case 4: G4_PUSH(4); break;
+        default: G4_PUSH(0); break;
+    }
+#undef G4_PUSH
 }
 void launch_deposit(hipStream_t s, int ns, const long long *off, const double *g,
                     const double *th, const double *x, const double *y, const double *cur,

@@ -87,6 +87,10 @@ def lib():
         L.g4mi_set_efield.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_double]
         L.g4mi_set_incoherent.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int]
         L.g4mi_set_sorting.argtypes = [C.c_void_p, C.c_int]
+        L.g4mi_add_harmonic.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_double, C.POINTER(C.c_int)]
+        L.g4mi_harmonic_upload.argtypes = [C.c_void_p, C.c_int, _dp]
+        L.g4mi_harmonic_download.argtypes = [C.c_void_p, C.c_int, _dp]
+        L.g4mi_harmonic_power.argtypes = [C.c_void_p, C.c_int, _dp]
         L.g4mi_sort.argtypes = [C.c_void_p, C.POINTER(C.c_longlong)]
         L.g4mi_get_slices.argtypes = [C.c_void_p, _lp, _dp]
         L.g4mi_one4one_counts.argtypes = [C.c_int, C.c_void_p, C.c_double, C.c_double, _lp]
@@ -250,6 +254,30 @@ class Context:
         """Beam::initIncoherent (src/Core/Incoherent.cpp:17-31), rank 0."""
         _chk(lib().g4mi_set_incoherent(self.h, seed, nbins, int(do_loss), int(do_spread)),
              "g4mi_set_incoherent")
+
+    def add_harmonic(self, harm, ngrid, gridmax):
+        """A field at harmonic `harm` with its own grid; returns its index."""
+        idx = C.c_int(-1)
+        _chk(lib().g4mi_add_harmonic(self.h, harm, ngrid, gridmax, C.byref(idx)), "g4mi_add_harmonic")
+        self._hgrid = getattr(self, "_hgrid", {})
+        self._hgrid[idx.value] = ngrid
+        return idx.value
+
+    def upload_harmonic(self, index, field):
+        f = np.ascontiguousarray(field, dtype=np.float64)
+        assert f.size == self.ns * self._hgrid[index] ** 2 * 2
+        _chk(lib().g4mi_harmonic_upload(self.h, index, _p(f)), "g4mi_harmonic_upload")
+
+    def download_harmonic(self, index):
+        ng = self._hgrid[index]
+        f = np.zeros((self.ns, ng * ng, 2))
+        _chk(lib().g4mi_harmonic_download(self.h, index, _p(f)), "g4mi_harmonic_download")
+        return f
+
+    def harmonic_power(self, index):
+        p = np.zeros(self.ns)
+        _chk(lib().g4mi_harmonic_power(self.h, index, _p(p)), "g4mi_harmonic_power")
+        return p
 
     def set_sorting(self, one4one=True):
         """Beam::initSorting: Beam::sort at sort markers for one4one beams."""

@@ -128,6 +128,25 @@ int g4mi_field_track(g4mi_ctx *ctx, const g4mi_step_params *st, double delz);
  * switches it off.  FP64 form only. */
 int g4mi_set_incoherent(g4mi_ctx *ctx, int seed, int nbins, int do_loss, int do_spread);
 
+/* Fields at harmonics of the fundamental (&field harm = h: Field::init with
+ * harm, src/Core/Field.cpp:34-70), FP64 form, ADI solver, at most 4 besides
+ * the fundamental.  Each has its own grid (ngrid, gridmax) and wavenumber
+ * h 2 pi / lambda0 and shares the fundamental's ring offset.  The push sums
+ * their coupling fc(h) / xks_h awloc conj(E_h) exp(-i h theta) into the
+ * ODE after the fundamental's (BeamSolver.cpp:23-32, 63-79, 144-163); each
+ * odd harmonic gets the source of harmonic h (FieldSolverADI.cpp:19-50);
+ * slippage zeroes their inflow slice too.  Fields are [nslice][ngrid^2]
+ * complex in storage order (the fundamental's `first`); power is per beam
+ * slice in W (DiagField). */
+int g4mi_add_harmonic(g4mi_ctx *ctx, int harm, int ngrid, double gridmax, int *index);
+int g4mi_harmonic_upload(g4mi_ctx *ctx, int index, const double *field);
+int g4mi_harmonic_download(g4mi_ctx *ctx, int index, double *field);
+int g4mi_harmonic_power(g4mi_ctx *ctx, int index, double *power);
+/* DiagField's outputs of harmonic field `index` (power, sizes and
+ * positions, intensities, far-field phase, their global values and
+ * global_energy); the beam outputs of `out` are not touched. */
+int g4mi_diag_harmonic(g4mi_ctx *ctx, int index, g4mi_diag_out *out);
+
 /* one4one beams, Beam::initSorting (src/Core/Beam.cpp:75-79): with
  * one4one = 1 every step whose marker has bit 4 set runs Beam::sort
  * (:81-94) after the beam step, as Gencore::run (src/Core/Gencore.cpp:

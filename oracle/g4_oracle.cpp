@@ -901,12 +901,16 @@ static void sc_slice(orc_state *s, int is, const ScSlice &a, double rmax, double
 struct PushCtx {
     double xks, xku, btpar, ez;
     complex<double> rpart;
+    int nh;
+    complex<double> rph[8];
+    double hh[8];
     double k2gg, k2pp, k3gg, k3pp;
 };
 static inline void ode(PushCtx &c, double tgam, double tthet) {   /* :144-163 */
     double ztemp1 = -2. / c.xks;
     complex<double> ctmp = 0;
     ctmp += c.rpart * complex<double>(cos(1.0 * tthet), -sin(1.0 * tthet));
+    for (int k = 0; k < c.nh; k++) ctmp += c.rph[k] * complex<double>(cos(c.hh[k] * tthet), -sin(c.hh[k] * tthet));
     double btper0 = c.btpar + ztemp1 * ctmp.real();
     double btpar0 = sqrt(1. - btper0 / (tgam * tgam));
     c.k2pp += c.xks * (1. - 1. / btpar0) + c.xku;
@@ -978,6 +982,7 @@ extern "C" void orc_advance(orc_state *s, const orc_step *st, double gamma0, dou
             PushCtx c;
             c.xks = xks;
             c.xku = xku;
+            c.nh = s->nh < 8 ? s->nh : 8;
             for (long long ip = s->off[is]; ip < s->off[is + 1]; ip++) {
                 double gamma = s->gamma[ip];
                 double theta = s->theta[ip] + autophase;
@@ -1008,6 +1013,28 @@ extern "C" void orc_advance(orc_state *s, const orc_step *st, double gamma0, dou
                     c.rpart = rtmp * awloc * std::conj(cpart);
                 } else {
                     c.rpart = 0;
+                }
+                for (int k = 0; k < c.nh; k++) {  /* the harmonic fields, :63-79 */
+                    const orc_hfield &h = s->hf[k];
+                    c.hh[k] = (double)h.harm;
+                    c.rph[k] = 0;
+                    if ((x > -h.gridmax) && (x < h.gridmax) && (y > -h.gridmax) && (y < h.gridmax)) {
+                        double wx = (x + h.gridmax) / h.dgrid;
+                        double wy = (y + h.gridmax) / h.dgrid;
+                        int ix = (int)floor(wx), iy = (int)floor(wy);
+                        wx = 1 + floor(wx) - wx;
+                        wy = 1 + floor(wy) - wy;
+                        int idx = ix + iy * h.ngrid;
+                        const double *fh = h.field + 2LL * islice * h.ngrid * h.ngrid;
+                        complex<double> cp = complex<double>(fh[2 * idx], fh[2 * idx + 1]) * wx * wy;
+                        idx++;
+                        cp += complex<double>(fh[2 * idx], fh[2 * idx + 1]) * (1 - wx) * wy;
+                        idx += h.ngrid - 1;
+                        cp += complex<double>(fh[2 * idx], fh[2 * idx + 1]) * wx * (1 - wy);
+                        idx++;
+                        cp += complex<double>(fh[2 * idx], fh[2 * idx + 1]) * (1 - wx) * (1 - wy);
+                        c.rph[k] = (orc_fc(st, h.harm) / h.xks) * awloc * std::conj(cp);
+                    }
                 }
                 rk4(c, delz, gamma, theta);
                 s->gamma[ip] = gamma;
@@ -1063,22 +1090,31 @@ extern "C" void orc_adi_factors(int ngrid, double delz, double dgrid, double xks
 
 /* Source term of beam slice ii, FieldSolverADI.cpp:13-51 (harm=1).
  * src holds ngrid^2 complex values and is cleared here. */
+/* the source of beam slice ii for the field at harmonic `harm` on a grid
+ * (ng, gridmax, dgrid) with wavenumber xks (FieldSolverADI.cpp:13-51) */
+static void deposit_h(const orc_state *s, const orc_step *st, int ii, double delz, int harm, int ng,
+                      double gridmax, double dgrid, double xks, double *src);
+
 extern "C" void orc_deposit_slice(const orc_state *s, const orc_step *st, int ii, double delz,
                                   double *src) {
-    int ng = s->ngrid;
+    deposit_h(s, st, ii, delz, 1, s->ngrid, s->gridmax, s->dgrid, s->xks, src);
+}
+
+static void deposit_h(const orc_state *s, const orc_step *st, int ii, double delz, int harm, int ng,
+                      double gridmax, double dgrid, double xks, double *src) {
     for (int ig = 0; ig < 2 * ng * ng; ig++) src[ig] = 0;
     complex<double> *cs = reinterpret_cast<complex<double> *>(src);
-    if (!(st->aw > 0)) return;                     /* inUndulator, odd harmonic */
+    if (!(st->aw > 0) || (harm % 2 != 1)) return;  /* inUndulator, odd harmonic */
     long long n0 = s->off[ii], n = s->off[ii + 1] - n0;
-    double scl = orc_fc(st, 1) * k_vacimp * s->current[ii] * s->xks * delz;
-    scl /= 4 * k_eev * (double)n * s->dgrid * s->dgrid;
+    double scl = orc_fc(st, harm) * k_vacimp * s->current[ii] * xks * delz;
+    scl /= 4 * k_eev * (double)n * dgrid * dgrid;
     for (long long k = 0; k < n; k++) {
         double x = s->x[n0 + k], y = s->y[n0 + k];
-        double theta = (double)1 * s->theta[n0 + k];
+        double theta = (double)harm * s->theta[n0 + k];
         double gamma = s->gamma[n0 + k];
-        if ((x > -s->gridmax) && (x < s->gridmax) && (y > -s->gridmax) && (y < s->gridmax)) {
-            double wx = (x + s->gridmax) / s->dgrid;
-            double wy = (y + s->gridmax) / s->dgrid;
+        if ((x > -gridmax) && (x < gridmax) && (y > -gridmax) && (y < gridmax)) {
+            double wx = (x + gridmax) / dgrid;
+            double wy = (y + gridmax) / dgrid;
             int ix = (int)floor(wx), iy = (int)floor(wy);
             wx = 1 + floor(wx) - wx;
             wy = 1 + floor(wy) - wy;
@@ -1237,7 +1273,10 @@ static void field_advance_fft(orc_state *s, const orc_step *st, double delz) {
 }
 
 /* Field::track -> FieldSolverADI::init/advance, Field.cpp:177-185. */
+static void field_advance_harmonics(orc_state *s, const orc_step *st, double delz);
+
 extern "C" void orc_field_advance(orc_state *s, const orc_step *st, double delz) {
+    field_advance_harmonics(s, st, delz);
     if (s->fft_solver) {
         field_advance_fft(s, st, delz);
         return;
@@ -1259,6 +1298,44 @@ extern "C" void orc_field_advance(orc_state *s, const orc_step *st, double delz)
     }
 }
 
+/* DiagField's power (Diagnostic.cpp:763-780, 868) of harmonic field k in
+ * beam slice order: sum |E|^2 (dgrid eev / ks)^2 / vacimp. */
+extern "C" void orc_diag_hpower(const orc_state *s, int k, double *power) {
+    const orc_hfield &h = s->hf[k];
+    const int ns = s->nslice, ng = h.ngrid;
+    const double scl = h.dgrid * k_eev / h.xks;
+    for (int is0 = 0; is0 < ns; is0++) {  /* storage slice is0 -> output (ns + is0 - first) % ns */
+        int is = (ns + is0 - s->first) % ns;
+        const double *f = h.field + 2LL * is0 * ng * ng;
+        double p = 0;
+        for (int i = 0; i < ng * ng; i++) p += f[2 * i] * f[2 * i] + f[2 * i + 1] * f[2 * i + 1];
+        power[is] = p * scl * scl / k_vacimp;
+    }
+}
+
+/* Field::track of the harmonic fields (ADI solver): each has its own
+ * factors (its grid and wavenumber) and source. */
+static void field_advance_harmonics(orc_state *s, const orc_step *st, double delz) {
+    for (int k = 0; k < s->nh; k++) {
+        const orc_hfield &h = s->hf[k];
+        int ng = h.ngrid, ns = s->nslice;
+        vector<double> c(2 * ng), cb(2 * ng), cw(2 * ng);
+        double cim;
+        orc_adi_factors(ng, delz, h.dgrid, h.xks, c.data(), cb.data(), cw.data(), &cim);
+#pragma omp parallel num_threads(s->ndiag_threads)
+        {
+            vector<double> src(2 * ng * ng), r(2 * ng * ng);
+#pragma omp for schedule(dynamic, 1)
+            for (int ii = 0; ii < ns; ii++) {
+                deposit_h(s, st, ii, delz, h.harm, ng, h.gridmax, h.dgrid, h.xks, src.data());
+                int i = (ii + s->first) % ns;
+                orc_adi_slice(ng, c.data(), cb.data(), cw.data(), cim, src.data(), h.field + 2LL * i * ng * ng,
+                              r.data());
+            }
+        }
+    }
+}
+
 /* Control::applySlippage, src/Core/Control.cpp:158-276, one rank. */
 extern "C" void orc_slippage(orc_state *s, double slip) {
     if (!s->timerun) return;
@@ -1275,6 +1352,11 @@ extern "C" void orc_slippage(orc_state *s, double slip) {
             if (direction > 0 || direction < 0) {
                 double *f = s->field + 2LL * last * ng * ng;
                 for (int i = 0; i < 2 * ng * ng; i++) f[i] = 0;
+                for (int k = 0; k < s->nh; k++) {  /* every field slips alike */
+                    const int nh = s->hf[k].ngrid;
+                    double *fh = s->hf[k].field + 2LL * last * nh * nh;
+                    for (int i = 0; i < 2 * nh * nh; i++) fh[i] = 0;
+                }
             }
         }
         s->first = last;

@@ -28,6 +28,15 @@ typedef struct {
     int helical, marker;
 } orc_step;
 
+/* A field at a harmonic of the fundamental (Field with harm > 1): its own
+ * grid and wavenumber xks = harm 2 pi / lambda0, the fundamental's ring
+ * offset. */
+typedef struct {
+    int harm, ngrid;
+    double gridmax, dgrid, xks;
+    double *field;                   /* [nslice][ngrid*ngrid] complex        */
+} orc_hfield;
+
 /* Simulation state.  All arrays are owned by the caller. */
 typedef struct {
     int nslice;
@@ -63,6 +72,10 @@ typedef struct {
     /* wake loss per slice [eV/m] (Collective's wakeext + wakeint) or NULL;
      * applied after the push and copied to eloss */
     const double *wake;
+    /* fields at harmonics besides the fundamental, in the reference's field
+     * order after it (BeamSolver::advance loops over them, :23-32) */
+    int nh;
+    orc_hfield *hf;
 } orc_state;
 
 /* Per-slice input of the quiet loader (QuietLoading.h BeamSlice). */
@@ -135,6 +148,8 @@ void orc_field_advance(orc_state *s, const orc_step *st, double delz);
 void *orc_incoherent_new(int seed, int rank, int do_loss, int do_spread);
 void orc_incoherent_free(void *h);
 void orc_incoherent_apply(orc_state *s, const orc_step *st, double gamma0, double delz);
+/* power per beam slice of harmonic field k */
+void orc_diag_hpower(const orc_state *s, int k, double *power);
 /* one4one beams: loading of every slice from one ran2 stream, and
  * Beam::sort's migration between slices (details in g4_oracle.cpp). */
 void orc_quiet_load_one4one(int seed, int rank, int nslice, const long long *off, const orc_slice_params *sl,

@@ -29,6 +29,11 @@ class Step(C.Structure):
         ("helical", C.c_int), ("marker", C.c_int)]
 
 
+class HField(C.Structure):
+    _fields_ = [("harm", C.c_int), ("ngrid", C.c_int), ("gridmax", C.c_double), ("dgrid", C.c_double),
+                ("xks", C.c_double), ("field", C.POINTER(C.c_double))]
+
+
 class State(C.Structure):
     _fields_ = [
         ("nslice", C.c_int), ("off", _lp),
@@ -43,7 +48,8 @@ class State(C.Structure):
         ("periodic", C.c_int), ("ndiag_threads", C.c_int),
         ("fft_solver", C.c_int), ("fft_filter", C.c_int),
         ("fft_xcut", C.c_double), ("fft_ycut", C.c_double), ("fft_sig", C.c_double),
-        ("incoh", C.c_void_p), ("nbins", C.c_int), ("wake", _dp)]
+        ("incoh", C.c_void_p), ("nbins", C.c_int), ("wake", _dp),
+        ("nh", C.c_int), ("hf", C.POINTER(HField))]
 
 
 class SliceParams(C.Structure):
@@ -92,6 +98,7 @@ def lib():
                                              C.c_double] + [_dp] * 6
         L.orc_sort_one4one.argtypes = [C.c_int, _lp, C.c_longlong, C.c_double, C.c_double] + [_dp] * 7
         L.orc_sort_one4one.restype = C.c_longlong
+        L.orc_diag_hpower.argtypes = [C.POINTER(State), C.c_int, _dp]
         L.orc_incoherent_new.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int]
         L.orc_incoherent_new.restype = C.c_void_p
         L.orc_incoherent_free.argtypes = [C.c_void_p]
@@ -269,6 +276,23 @@ class Sim:
         return out
 
     one4one = False  # Beam::sort at sort markers (one4one beams)
+
+    def add_harmonics(self, specs):
+        """Fields at harmonics besides the fundamental: specs = list of
+        (harm, ngrid, gridmax); arrays in self.hfields, zero-filled."""
+        self.hfields = [np.zeros((self.ns, ng * ng, 2)) for (_, ng, _) in specs]
+        arr = (HField * len(specs))()
+        for k, (h, ng, gm) in enumerate(specs):
+            arr[k] = HField(h, ng, gm, 2 * gm / float(ng - 1), h * 4.0 * np.arcsin(1.0) / self.s.xlambda,
+                            _ptr(self.hfields[k]))
+        self._harr = arr
+        self.s.nh = len(specs)
+        self.s.hf = arr
+
+    def hpower(self, k):
+        p = np.zeros(self.ns)
+        lib().orc_diag_hpower(C.byref(self.s), k, _ptr(p))
+        return p
 
     def step(self, st, gamma0):
         if self.one4one and (st.marker & 4):

@@ -175,7 +175,7 @@ def test_time_window_slices(tmp_path):
     ("&setup\nlattice={lat}\nbeamline=FEL\n&end\n&profile_gauss\n&end\n", "not supported"),
     ("&setup\nlattice={lat}\nbeamline=FEL\n&end\n&beam\ncurrent=@prof\n&end\n", "profile"),
     ("&setup\nlattice={lat}\nbeamline=FEL\n&end\n&field\nharm=3\n&end\n", "harm"),
-    ("&setup\nlattice={lat}\nbeamline=FEL\none4one=true\n&end\n", "one4one"),
+    ("&setup\nlattice={lat}\nbeamline=FEL\n&end\n&field\nnx=1\n&end\n", "TEM00"),
     ("&setup\nlattice={lat}\nbeamline=NOPE\n&end\n", "lattice"),
 ])
 def test_deck_errors_are_reported(tmp_path, bad, msg):
@@ -680,3 +680,62 @@ def test_example1_deck_with_wake_and_sponrad(tmp_path):
     print(f"  P_end {p[-1]:.6e} W, <gamma> change {en[-1] - en[0]:.4f}, wake loss {eloss[0]:.4e} eV/m; "
           f"P(z) rel err {ep:.3e}, |b(z)| {eb:.3e}, <gamma> {ee:.3e}", flush=True)
     assert ep < 1e-10 and eb < 1e-10 and ee < 1e-13
+
+
+HARM_IN = EX1_IN.replace("ngrid=255", "ngrid=63").replace(
+    "&beam", "&field\nharm=3\npower=1e2\ndgrid=1.5e-4\nngrid=41\nwaist_size=20e-6\n&end\n\n&beam").replace(
+    "helical= True", "helical= False").replace("&track\n&end", "&track\nzstop=5\n&end")
+
+
+def test_harmonic_field_deck_setup(tmp_path):
+    """&field harm = 3 after the fundamental: a second field on its own grid;
+    the Gaussian at 3 ks (GaussHermite::loadGauss with harm) has the
+    requested power, and the 3rd harmonic's dump is <root>.h3.fld.h5 with
+    wavelength lambda0 / 3."""
+    deck = HARM_IN.replace("&track", "&write\nfield = @.init\n&end\n\n&track")
+    path = _write(tmp_path, deck=deck, lat_extra="", root=str(tmp_path / "h"))
+    g4mi.check_deck(path)
+    fld = str(tmp_path / "h.init.h3.fld.h5")
+    assert g4mi.h5_read(fld, "gridpoints")[0] == 41
+    assert g4mi.h5_read(fld, "wavelength")[0] == pytest.approx(1e-10 / 3, rel=1e-15)
+    re = g4mi.h5_read(fld, "slice000001/field-real")
+    im = g4mi.h5_read(fld, "slice000001/field-imag")
+    assert abs(np.sum(re * re + im * im) / 1e2 - 1) < 1e-6
+
+
+@pytest.mark.gpu
+def test_harmonic_field_deck_run(tmp_path):
+    """A planar-undulator Example1 (63^2) with a seeded 3rd harmonic (41^2):
+    /Field3/power against the oracle's harmonic field on the same beam and
+    fields, every record within 1e-10, and /Field/power likewise."""
+    print("deck runner: Example1 planar with a 3rd-harmonic field", flush=True)
+    path = _write(tmp_path, deck=HARM_IN, root=str(tmp_path / "h"))
+    s = g4mi.check_deck(path)
+    g4mi.run_deck(path, device=0, quiet=True)
+    out = str(tmp_path / "h.out.h5")
+    p = g4mi.h5_read(out, "Field/power")[:, 0]
+    p3 = g4mi.h5_read(out, "Field3/power")[:, 0]
+    els = []
+    for k1 in (2.0, -2.0) * 6:
+        els += [("undulator", dict(aw=0.84853, lambdau=0.015, nwig=266, helical=False)),
+                ("drift", dict(l=0.44)), ("quadrupole", dict(l=0.08, k1=k1)), ("drift", dict(l=0.24))]
+    st = O.unroll(els, 0.045, 1e-10, 11357.82)
+    sim = O.Sim([8192], 63, 2e-4, 1e-10, timerun=False, threads=16)
+    O.load_beam(sim, dict(gamma=11357.82, delgam=1.0, current=3000.0, ex=4e-7, ey=4e-7,
+                          betax=s["betax"], betay=s["betay"], alphax=s["alphax"], alphay=s["alphay"]),
+                None, 8, None, lambda0=1e-10)
+    O.load_gauss_all(sim, 5e3, 30e-6, 1e-10)
+    sim.add_harmonics([(3, 41, 1.5e-4)])
+    lib = g4mi.lib()
+    h = g4mi.load_gauss(41, 1.5e-4, 1e-10 / 3, 1e2, 20e-6)  # k = 3 ks
+    sim.hfields[0][0] = h
+    nrec = len(p)
+    rp, rp3 = [sim.diag()["power"][0]], [sim.hpower(0)[0]]
+    for i in range(nrec - 1):
+        sim.step(st[i], 11357.82)
+        rp.append(sim.diag()["power"][0])
+        rp3.append(sim.hpower(0)[0])
+    ep = float(np.max(np.abs(p / np.array(rp) - 1)))
+    ep3 = float(np.max(np.abs(p3 / np.array(rp3) - 1)))
+    print(f"  {nrec} records: P(z) rel err {ep:.3e}, P3(z) rel err {ep3:.3e}, P3 end {p3[-1]:.4e} W", flush=True)
+    assert ep < 1e-10 and ep3 < 1e-10

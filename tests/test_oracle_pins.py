@@ -592,3 +592,75 @@ def test_wake_kick_is_eloss_times_dz():
         a, b = int(sim.off[i]), int(sim.off[i + 1])
         np.testing.assert_allclose(sim.gamma[a:b] - 11357.82, w[i] * 0.05 / 510998.95069, rtol=1e-9, atol=1e-12)
     assert np.array_equal(sim.eloss, w)
+
+
+# ------------------------------------------------------------ harmonic fields
+def _harm_state(seed_power=5e7, helical=False):
+    sim = O.Sim([1024] * 2, 63, 0.3e-3, cases.LAMBDA_R, timerun=False)
+    O.load_beam(sim, [cases.slice_beam()] * 2, None, 4, None, lambda0=cases.LAMBDA_R)
+    O.load_gauss_all(sim, seed_power, 30e-6, cases.LAMBDA_R)
+    return sim
+
+
+def _harm_und_step(helical=False):
+    ku = 2 * np.pi / cases.LAMBDAU
+    return O.Step(aw=cases.AW, ku=ku, ky=0.0 if helical else ku * ku, kx=ku * ku / 2 if helical else 0.0,
+                  helical=1 if helical else 0, dz=cases.DELZ)
+
+
+def test_harmonics_without_coupling_change_nothing():
+    """A 2nd harmonic in a planar undulator and a 3rd harmonic in a helical
+    one have fc(h) = 0 (Undulator::fc): the beam and the fundamental evolve
+    bit for bit as without them, and the harmonic field only diffracts
+    (equal to the ADI with zero source)."""
+    for h, helical in ((2, False), (3, True)):
+        ref = _harm_state()
+        sim = _harm_state()
+        sim.add_harmonics([(h, 41, 0.25e-3)])
+        rng = np.random.default_rng(h)
+        sim.hfields[0][:] = rng.normal(size=sim.hfields[0].shape) * 1e3
+        h0 = sim.hfields[0].copy()
+        st = _harm_und_step(helical)
+        for _ in range(3):
+            sim.step(st, cases.GAMMA0)
+            ref.step(st, cases.GAMMA0)
+        for k in ("gamma", "theta", "x", "y", "px", "py"):
+            assert np.array_equal(getattr(sim, k), getattr(ref, k)), (h, k)
+        assert np.array_equal(sim.field, ref.field)
+        ng, dgrid = 41, 0.5e-3 / 40
+        xks = h * 2 * np.pi / cases.LAMBDA_R
+        c, cb, cw = (np.zeros(2 * ng) for _ in range(3))
+        cim = C.c_double(0)
+        O.lib().orc_adi_factors(ng, C.c_double(cases.DELZ), C.c_double(dgrid), C.c_double(xks), dp(c), dp(cb),
+                                dp(cw), C.byref(cim))
+        src = np.zeros(2 * ng * ng)
+        r = np.zeros(2 * ng * ng)
+        for i in range(2):
+            u = np.ascontiguousarray(h0[i].reshape(-1))
+            for _ in range(3):
+                O.lib().orc_adi_slice(ng, dp(c), dp(cb), dp(cw), cim, dp(src), dp(u), dp(r))
+            assert np.array_equal(u, sim.hfields[0][i].reshape(-1)), h
+
+
+def test_third_harmonic_source_phase_and_current():
+    """The 3rd-harmonic source of a planar undulator (FieldSolverADI with
+    harm = 3): doubling the current doubles the driven field exactly (the
+    field starts at zero); shifting every phase by 2 pi / 3 leaves it
+    unchanged and by pi / 3 negates it (sin/cos of 3 theta), to rounding."""
+    st = _harm_und_step()
+
+    def drive(cur_scale=1.0, dth=0.0):
+        sim = _harm_state(seed_power=0.0)
+        sim.current[:] *= cur_scale
+        # a quiet beam of 4 beamlets has no 3rd-harmonic bunching; modulate it
+        sim.theta[:] += 0.3 * np.sin(3 * sim.theta) + dth
+        sim.add_harmonics([(3, 41, 0.25e-3)])
+        O.lib().orc_field_advance(C.byref(sim.s), C.byref(st), st.dz)
+        return sim.hfields[0].copy()
+
+    f1 = drive()
+    assert np.max(np.abs(f1)) > 0
+    assert np.array_equal(drive(2.0), 2 * f1)
+    m = np.max(np.abs(f1))
+    assert np.max(np.abs(drive(dth=2 * np.pi / 3) - f1)) < 1e-11 * m
+    assert np.max(np.abs(drive(dth=np.pi / 3) + f1)) < 1e-11 * m
