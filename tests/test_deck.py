@@ -684,7 +684,14 @@ def test_example1_deck_with_wake_and_sponrad(tmp_path):
 
 HARM_IN = EX1_IN.replace("ngrid=255", "ngrid=63").replace(
     "&beam", "&field\nharm=3\npower=1e2\ndgrid=1.5e-4\nngrid=41\nwaist_size=20e-6\n&end\n\n&beam").replace(
-    "helical= True", "helical= False").replace("&track\n&end", "&track\nzstop=5\n&end")
+    "&track\n&end", "&track\nzstop=5\n&end")
+
+
+def _write_planar(tmp_path, deck, root):
+    """The Example1 deck with a planar undulator lattice (odd harmonics couple)."""
+    path = _write(tmp_path, deck=deck, root=root)
+    (tmp_path / "Example1.lat").write_text(EX1_LAT.format(extra="").replace("helical= True", "helical= False"))
+    return path
 
 
 def test_harmonic_field_deck_setup(tmp_path):
@@ -693,7 +700,7 @@ def test_harmonic_field_deck_setup(tmp_path):
     requested power, and the 3rd harmonic's dump is <root>.h3.fld.h5 with
     wavelength lambda0 / 3."""
     deck = HARM_IN.replace("&track", "&write\nfield = @.init\n&end\n\n&track")
-    path = _write(tmp_path, deck=deck, lat_extra="", root=str(tmp_path / "h"))
+    path = _write_planar(tmp_path, deck, str(tmp_path / "h"))
     g4mi.check_deck(path)
     fld = str(tmp_path / "h.init.h3.fld.h5")
     assert g4mi.h5_read(fld, "gridpoints")[0] == 41
@@ -709,7 +716,7 @@ def test_harmonic_field_deck_run(tmp_path):
     /Field3/power against the oracle's harmonic field on the same beam and
     fields, every record within 1e-10, and /Field/power likewise."""
     print("deck runner: Example1 planar with a 3rd-harmonic field", flush=True)
-    path = _write(tmp_path, deck=HARM_IN, root=str(tmp_path / "h"))
+    path = _write_planar(tmp_path, HARM_IN, str(tmp_path / "h"))
     s = g4mi.check_deck(path)
     g4mi.run_deck(path, device=0, quiet=True)
     out = str(tmp_path / "h.out.h5")

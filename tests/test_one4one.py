@@ -20,7 +20,7 @@ import g4mi
 import harness as H
 import oracle_py as O
 
-LAM = 1e-10
+LAM = cases.LAMBDA_R
 SLICE = dict(gamma=11357.82, delgam=2.0, current=50.0, ex=4e-7, ey=4e-7, betax=10.0, betay=12.0,
              alphax=0.3, alphay=-0.2)
 
@@ -144,7 +144,7 @@ def test_one4one_sase_with_sort_markers():
     ep = H.rel_err(g["power"][1:], r["power"][1:])
     eb = H.rel_err(g["bunching"], r["bunching"])
     off, _ = ctx.slices()
-    print(f"  P(z) rel err {ep:.3e}  |b(z)| {eb:.3e}  particles {int(cnt.sum())} -> {int(off[-1])}", flush=True)
+    print(f"  P(z) rel err {ep:.3e}  |b(z)| {eb:.3e}  particles {int(sum(cnt))} -> {int(off[-1])}", flush=True)
     assert np.array_equal(off, sim.off)
     assert ep < 1e-10 and eb < 1e-10
     ctx.close()
@@ -214,9 +214,9 @@ def test_one4one_deck_setup(tmp_path):
     par = root + ".init.par.h5"
     assert g4mi.h5_read(par, "one4one")[0] == 1 and g4mi.h5_read(par, "beamletsize")[0] == 1
     sl = [dict(SLICE, current=30.0, delgam=1.0, betax=8.0, betay=16.0, alphax=0.0, alphay=0.0)] * 3
-    cnt = O.one4one_counts([30.0] * 3, LAM, 2)
-    sim = O.Sim(cnt, 31, 2e-4, LAM, sample=2)
-    O.load_beam_one4one(sim, sl, 9, LAM, 2)
+    cnt = O.one4one_counts([30.0] * 3, 1e-10, 2)
+    sim = O.Sim(cnt, 31, 2e-4, 1e-10, sample=2)
+    O.load_beam_one4one(sim, sl, 9, 1e-10, 2)
     for i in range(3):
         a, b = int(sim.off[i]), int(sim.off[i + 1])
         for k in ("gamma", "theta", "x", "y", "px", "py"):
@@ -237,5 +237,5 @@ def test_one4one_deck_run(tmp_path):
     last = f"{root}.{int(steps[-1])}.par.h5"
     n = [len(g4mi.h5_read(last, f"slice{i + 1:06d}/gamma")) for i in range(3)]
     cur = [g4mi.h5_read(last, f"slice{i + 1:06d}/current")[0] for i in range(3)]
-    assert np.allclose(cur, np.array(n) * O.CE / (2 * LAM), rtol=1e-15)
+    assert np.allclose(cur, np.array(n) * O.CE / (2 * 1e-10), rtol=1e-15)
     assert sum(n) <= s["npart"]
