@@ -108,6 +108,17 @@ struct Ctx {
         bool fac_ready = false;
     };
     std::vector<HField> hf;
+    // records of g4mi_run waiting for the host: their per-slice sums sit in
+    // pinned slots (16 ns doubles each) filled by asynchronous copies
+    struct PendingRecord {
+        g4mi_diag_out out;
+        int first;
+        bool fresh;
+        std::vector<int> zeroed;
+    };
+    double *h_recs = nullptr;
+    int rec_cap = 0;
+    std::vector<PendingRecord> pend;
     // one4one beams: Beam::sort at sort markers, with its work buffers
     bool one4one = false;
     SortWork sw{};
@@ -1751,6 +1762,7 @@ int g4mi_destroy(g4mi_ctx *h) {
     for (auto &e : c->pool)
         if (e) (void)hipEventDestroy(e);
     if (c->incoh_ev) (void)hipEventDestroy(c->incoh_ev);
+    if (c->h_recs) (void)hipHostFree(c->h_recs);
     if (c->h_bdg) (void)hipHostFree(c->h_bdg);
     delete c->incoh;
     if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -2105,6 +2117,50 @@ int g4mi_step(g4mi_ctx *h, const g4mi_step_params *st, double gammaref) {
     return step_end_timing(&h->c);
 }
 
+// Deferred records of the FP64 run loop: each record's beam and field sums
+// are copied asynchronously into a pinned slot, and the host finishes them
+// when the slots are full or the run ends, so a record after every step
+// does not stall the device between steps.  Records that ask for the
+// auxiliary or spectral blocks, and one4one runs (whose slices change at
+// sorts), take the synchronous path.
+static bool record_deferrable(const Ctx *c, const g4mi_diag_out *o) {
+    return !wants_aux(o) && !wants_spectral(o) && !c->one4one;
+}
+
+static int flush_records(Ctx *c) {
+    if (c->pend.empty()) return G4MI_OK;
+    G4_HIP(hipStreamSynchronize(c->stream));
+    const size_t ns = (size_t)c->ns, row = 16 * ns;
+    std::vector<double> hb(8 * ns), hf(8 * ns);
+    for (size_t k = 0; k < c->pend.size(); k++) {
+        Ctx::PendingRecord &r = c->pend[k];
+        std::copy(c->h_recs + k * row, c->h_recs + k * row + 8 * ns, hb.begin());
+        std::copy(c->h_recs + k * row + 8 * ns, c->h_recs + (k + 1) * row, hf.begin());
+        if (r.fresh)  // slices zeroed by slippage after the ADI's sums
+            for (int z : r.zeroed)
+                for (int q = 0; q < 8; q++) hf[8 * (size_t)z + q] = 0;
+        finish_diag(c, hb, hf, r.first, &r.out);
+    }
+    c->pend.clear();
+    return G4MI_OK;
+}
+
+static int enqueue_record(Ctx *c, const g4mi_diag_out *o) {
+    const size_t ns = (size_t)c->ns, row = 16 * ns;
+    if (!c->h_recs) {
+        c->rec_cap = (int)std::max<size_t>(4, std::min<size_t>(256, ((size_t)32 << 20) / (row * sizeof(double))));
+        G4_HIP(hipHostMalloc((void **)&c->h_recs, (size_t)c->rec_cap * row * sizeof(double)));
+    }
+    if ((int)c->pend.size() == c->rec_cap) G4_TRY(flush_records(c));
+    launch_diag_beam(c->stream, c->ns, c->d_off, c->d_g, c->d_th, c->d_x, c->d_y, c->d_diag_b);
+    if (!c->field_diag_fresh) launch_diag_field(c->stream, c->ns, c->d_field, c->ng, c->d_diag_f);
+    double *slot = c->h_recs + c->pend.size() * row;
+    G4_HIP(hipMemcpyAsync(slot, c->d_diag_b, 8 * ns * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    G4_HIP(hipMemcpyAsync(slot + 8 * ns, c->d_diag_f, 8 * ns * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    c->pend.push_back({*o, c->first, c->field_diag_fresh, c->zeroed_since_adi});
+    return G4MI_OK;
+}
+
 int g4mi_run(g4mi_ctx *h, const g4mi_step_params *steps, int nsteps, double gammaref,
              g4mi_diag_out *records) {
     if (!h || (!steps && nsteps > 0) || nsteps < 0) return G4MI_E_ARG;
@@ -2113,14 +2169,26 @@ int g4mi_run(g4mi_ctx *h, const g4mi_step_params *steps, int nsteps, double gamm
         G4_TRY(mix_run(&h->c, steps, nsteps, gammaref, records, &done));
         return done;
     }
-    for (int i = 0; i < nsteps; i++) {
+    Ctx *c = &h->c;
+    int rc = G4MI_OK;
+    for (int i = 0; i < nsteps && rc == G4MI_OK; i++) {
         if (steps[i].marker & 8) break;
-        G4_TRY(step(&h->c, steps[i], gammaref));
-        if (records) G4_TRY(diag(&h->c, &records[i]));
-        G4_TRY(step_end_timing(&h->c));
-        done++;
+        rc = step(c, steps[i], gammaref);
+        if (rc == G4MI_OK && records) {
+            if (record_deferrable(c, &records[i])) {
+                rc = enqueue_record(c, &records[i]);
+            } else {
+                rc = flush_records(c);
+                if (rc == G4MI_OK) rc = diag(c, &records[i]);
+            }
+        }
+        if (rc == G4MI_OK) rc = step_end_timing(c);
+        if (rc == G4MI_OK) done++;
     }
-    G4_HIP(hipStreamSynchronize(h->c.stream));
+    const int rf = flush_records(c);
+    if (rc != G4MI_OK) return rc;
+    if (rf != G4MI_OK) return rf;
+    G4_HIP(hipStreamSynchronize(c->stream));
     return done;
 }
 
