@@ -796,6 +796,15 @@ static void finish_diag(const Ctx *c, const std::vector<double> &hb, const std::
     finish_field(hf, ns, first, c->dgrid, 4. * asin(1) / c->xlambda, c->slicelength, o);
 }
 
+// pinned record slots (at most 32 MiB, 4..256 records), allocated with the
+// context so no run pays for the allocation
+static int alloc_records(Ctx *c) {
+    const size_t row = 16 * (size_t)c->ns;
+    c->rec_cap = (int)std::max<size_t>(4, std::min<size_t>(256, ((size_t)32 << 20) / (row * sizeof(double))));
+    G4_HIP(hipHostMalloc((void **)&c->h_recs, (size_t)c->rec_cap * row * sizeof(double)));
+    return G4MI_OK;
+}
+
 static bool wants_aux(const g4mi_diag_out *o) {
     return o->pxposition || o->pyposition || o->nf_phase || o->efield || o->wakefield || o->lscfield ||
            o->sscfield || o->xmin || o->xmax || o->pxmin || o->pxmax || o->ymin || o->ymax || o->pymin ||
@@ -1728,6 +1737,7 @@ int g4mi_create(g4mi_ctx **out, int device, int nslice, const long long *off, in
         (rc = dalloc(c, &c->d_count, 1)))
         return fail(rc);
     if (!fp64 && (rc = mix_alloc(c))) return fail(rc);
+    if (fp64 && (rc = alloc_records(c))) return fail(rc);
     std::vector<int> bs;
     std::vector<long long> bb;
     build_block_table(c->off, bs, bb);
@@ -2147,10 +2157,7 @@ static int flush_records(Ctx *c) {
 
 static int enqueue_record(Ctx *c, const g4mi_diag_out *o) {
     const size_t ns = (size_t)c->ns, row = 16 * ns;
-    if (!c->h_recs) {
-        c->rec_cap = (int)std::max<size_t>(4, std::min<size_t>(256, ((size_t)32 << 20) / (row * sizeof(double))));
-        G4_HIP(hipHostMalloc((void **)&c->h_recs, (size_t)c->rec_cap * row * sizeof(double)));
-    }
+    if (!c->h_recs) G4_TRY(alloc_records(c));
     if ((int)c->pend.size() == c->rec_cap) G4_TRY(flush_records(c));
     launch_diag_beam(c->stream, c->ns, c->d_off, c->d_g, c->d_th, c->d_x, c->d_y, c->d_diag_b);
     if (!c->field_diag_fresh) launch_diag_field(c->stream, c->ns, c->d_field, c->ng, c->d_diag_f);
