@@ -1822,6 +1822,18 @@ int g4mi_set_incoherent(g4mi_ctx *h, int seed, int nbins, int do_loss, int do_sp
     return G4MI_OK;
 }
 
+int g4mi_incoherent_state(g4mi_ctx *h, int *state, int store) {
+    if (!h || !state) return G4MI_E_ARG;
+    Ctx *c = &h->c;
+    if (!c->incoh) {
+        set_error("g4mi_incoherent_state: no incoherent synchrotron radiation stream in this context");
+        return G4MI_E_ARG;
+    }
+    if (store) c->incoh->load(state);
+    else c->incoh->save(state);
+    return G4MI_OK;
+}
+
 int g4mi_add_harmonic(g4mi_ctx *hc, int harm, int ngrid, double gridmax, int *index) {
     if (!hc || harm < 2 || ngrid < 3 || !(gridmax > 0)) {
         set_error("g4mi_add_harmonic: invalid argument (harm >= 2, ngrid >= 3, gridmax > 0)");

@@ -209,11 +209,20 @@ struct Deck {
     int ngrid = 0;
     double gridmax = 0, flambda = 0;
     bool have_beam = false, have_field = false, spon_applied = false;
-    // device context (created by the first &track)
+    // ring offset of the host fields: window slice w is storage slot
+    // (w + field_first) % nslice (the device's `first` when the host copy
+    // was taken)
+    int field_first = 0;
+    // device context (created by a &track, released when an element changes
+    // the beam or field layout; the host copies are then the state)
     g4mi_ctx *ctx = nullptr;
     int precision = G4MI_FP64;
     bool periodic = false;
-    int runs = 0;
+    // Setup::runcount: &alter_setup without rootname adds .Run<n+1>
+    int runcount = 0;
+    // incoherent stream carried over a released context
+    bool incoh_saved = false;
+    int incoh_state[35] = {0};
     std::string summary;  // JSON of what was set up (check mode)
 };
 
@@ -412,6 +421,9 @@ bool do_time(Deck &dk, Namelist &nl, std::string &err) {
 int time_nslice(const Deck &dk) { return dk.time.dotime || dk.time.ds > 0 ? dk.time.nslice : 1; }
 double time_ds(const Deck &dk) { return dk.time.ds > 0 ? dk.time.ds : dk.setup.lambda0 * dk.time.sample; }
 
+bool release_ctx(Deck &dk, std::string &err);
+void unroll_ring(std::vector<double> &f, size_t cells, int first);
+
 bool do_field(Deck &dk, Namelist &nl, std::string &err) {
     Keys k{nl, ""};
     double lambda = dk.setup.lambda0, power = 0, phase = 0, z0 = 0, w0 = 100e-6;
@@ -441,6 +453,8 @@ bool do_field(Deck &dk, Namelist &nl, std::string &err) {
         err = "&field: only TEM00 modes (nx = ny = 0) at harm >= 1 are supported";
         return false;
     }
+    // a field defined after a &track replaces or adds to the tracked one
+    if (!release_ctx(dk, err)) return false;
     Deck::HarmField *hfp = nullptr;  // LoadField.cpp:99-116: the record of this harmonic
     if (harm > 1 && fabs(lambda - dk.setup.lambda0) > 1e-12 * dk.setup.lambda0) {
         err = "&field: harmonic fields at a wavelength other than lambda0 are not supported by this build";
@@ -475,6 +489,15 @@ bool do_field(Deck &dk, Namelist &nl, std::string &err) {
     if (ngrid < 3 || dgrid <= 0) {
         err = "&field: invalid grid";
         return false;
+    }
+    // Field::init resets the ring offset of the field it (re)defines
+    // (src/Core/Field.cpp:66): an accumulated field takes its storage order
+    // as the new window order, every other field keeps its window order.
+    if (dk.field_first != 0) {
+        if (dk.have_field && !(add && !hfp)) unroll_ring(dk.field, (size_t)dk.ngrid * dk.ngrid, dk.field_first);
+        for (auto &e : dk.hfields)
+            if (!(add && hfp == &e)) unroll_ring(e.field, (size_t)e.ngrid * e.ngrid, dk.field_first);
+        dk.field_first = 0;
     }
     const int ns = time_nslice(dk);
     const double ds = time_ds(dk);
@@ -869,6 +892,8 @@ bool do_sponrad(Deck &dk, Namelist &nl, std::string &err) {
         return false;
     }
     sp.set = true;
+    dk.spon_applied = false;  // a new stream from this seed
+    dk.incoh_saved = false;
     if (dk.ctx && g4mi_set_incoherent(dk.ctx, sp.seed, dk.setup.one4one ? 1 : dk.setup.nbins, sp.loss ? 1 : 0,
                                       sp.spread ? 1 : 0) != G4MI_OK) {
         err = g4mi_last_error();
@@ -945,7 +970,7 @@ void write_version(H5File &f, bool &ok) {
 
 // the state on the host: download from the device when a run has started
 bool host_state(Deck &dk, int *first, std::string &err) {
-    *first = 0;
+    *first = dk.field_first;
     if (!dk.ctx) return true;
     // one4one sorting moves particles between slices on the device
     if (g4mi_get_slices(dk.ctx, dk.off.data(), dk.current.data()) != G4MI_OK) {
@@ -966,6 +991,61 @@ bool host_state(Deck &dk, int *first, std::string &err) {
             return false;
         }
     return true;
+}
+
+// a ring-stored field ([nslice][cells] complex, window slice w in storage
+// slot (w + first) % nslice) in window order
+void unroll_ring(std::vector<double> &f, size_t cells, int first) {
+    const size_t per = cells * 2, ns = per ? f.size() / per : 0;
+    if (first == 0 || ns < 2) return;
+    std::vector<double> t(f.size());
+    for (size_t w = 0; w < ns; w++)
+        std::memcpy(t.data() + w * per, f.data() + ((w + (size_t)first) % ns) * per, per * sizeof(double));
+    f.swap(t);
+}
+
+// Take the state off the device: beam, fields, ring offset and the
+// incoherent stream go to the host copies and the context is destroyed;
+// the next &track builds a new one from them.
+bool release_ctx(Deck &dk, std::string &err) {
+    if (!dk.ctx) return true;
+    int first = 0;
+    if (!host_state(dk, &first, err)) return false;
+    dk.field_first = first;
+    if (dk.spon.set && dk.spon_applied && (dk.spon.loss || dk.spon.spread)) {
+        if (g4mi_incoherent_state(dk.ctx, dk.incoh_state, 0) != G4MI_OK) {
+            err = g4mi_last_error();
+            return false;
+        }
+        dk.incoh_saved = true;
+    }
+    g4mi_destroy(dk.ctx);
+    dk.ctx = nullptr;
+    return true;
+}
+
+// Beam::sort (src/Core/Beam.cpp:81-94) of the host beam on the device, in a
+// context of its own (the field grid is a placeholder); offsets, currents
+// and coordinates come back to the host copies.
+bool device_sort(Deck &dk, int device, std::string &err) {
+    const int ns = (int)dk.off.size() - 1;
+    g4mi_ctx *c = nullptr;
+    long long kept = 0;
+    bool ok = g4mi_create(&c, device, ns, dk.off.data(), 3, 1.0, dk.setup.lambda0, (double)dk.time.sample,
+                          dk.time.dotime ? 1 : 0, 0, G4MI_FP64) == G4MI_OK &&
+              g4mi_beam_upload(c, dk.gamma.data(), dk.theta.data(), dk.x.data(), dk.y.data(), dk.px.data(),
+                               dk.py.data(), dk.current.data()) == G4MI_OK &&
+              g4mi_set_sorting(c, 1) == G4MI_OK && g4mi_sort(c, &kept) == G4MI_OK &&
+              g4mi_get_slices(c, dk.off.data(), dk.current.data()) == G4MI_OK;
+    if (ok) {
+        const size_t n = (size_t)dk.off.back();
+        for (auto *v : {&dk.gamma, &dk.theta, &dk.x, &dk.y, &dk.px, &dk.py}) v->resize(n);
+        ok = g4mi_beam_download(c, dk.gamma.data(), dk.theta.data(), dk.x.data(), dk.y.data(), dk.px.data(),
+                                dk.py.data()) == G4MI_OK;
+    }
+    if (!ok) err = std::string("device: ") + g4mi_last_error();
+    if (c) g4mi_destroy(c);
+    return ok;
 }
 
 // <basename>.fld.h5 (WriteFieldHDF5::writeMain, src/IO/writeFieldHDF5.cpp:42-171):
@@ -1470,19 +1550,19 @@ bool do_track(Deck &dk, Namelist &nl, int device, bool quiet, bool dry, std::str
     dk.summary = tmp;
     std::string root = st.rootname;
     if (!st.outputdir.empty()) root = st.outputdir + "/" + root;
-    if (dk.runs > 0) root += ".Run" + std::to_string(dk.runs + 1);
+    if (dk.runcount > 0) root += ".Run" + std::to_string(dk.runcount + 1);
     const std::string fname = root + ".out.h5";
 
     std::map<std::string, std::vector<double>> rec;
     DumpList dl;
     if (dry) {
         // setup check: the output file without device records
-        dk.runs++;
         if (st.rootname.empty()) return true;
         return write_output(dk, fname, steps, zout, rec, 0, dl, err);
     }
     // device context: created by the first &track, reused by later ones
     const int precision = prec == "fp32" ? G4MI_FP32 : G4MI_FP64;
+    const bool created = !dk.ctx;
     if (!dk.ctx) {
         dk.precision = precision;
         dk.periodic = periodic;
@@ -1496,7 +1576,7 @@ bool do_track(Deck &dk, Namelist &nl, int device, bool quiet, bool dry, std::str
         if ((rc = g4mi_set_efield(dk.ctx, e.longrange ? 1 : 0, e.nz, e.nphi, e.ngrid, e.rmax)) != G4MI_OK ||
             (rc = g4mi_beam_upload(dk.ctx, dk.gamma.data(), dk.theta.data(), dk.x.data(), dk.y.data(),
                                    dk.px.data(), dk.py.data(), dk.current.data())) != G4MI_OK ||
-            (rc = g4mi_field_upload(dk.ctx, dk.field.data(), 0)) != G4MI_OK) {
+            (rc = g4mi_field_upload(dk.ctx, dk.field.data(), dk.field_first)) != G4MI_OK) {
             err = g4mi_last_error();
             return false;
         }
@@ -1520,13 +1600,15 @@ bool do_track(Deck &dk, Namelist &nl, int device, bool quiet, bool dry, std::str
     }
     // the incoherent stream is seeded once (it runs on across &track
     // elements); the wake loss follows the current beam
-    if (!dry && dk.spon.set && !dk.spon_applied) {
+    if (!dry && dk.spon.set && (!dk.spon_applied || created)) {
         if (g4mi_set_incoherent(dk.ctx, dk.spon.seed, dk.setup.one4one ? 1 : dk.setup.nbins,
-                                dk.spon.loss ? 1 : 0, dk.spon.spread ? 1 : 0) != G4MI_OK) {
+                                dk.spon.loss ? 1 : 0, dk.spon.spread ? 1 : 0) != G4MI_OK ||
+            (dk.spon_applied && dk.incoh_saved && g4mi_incoherent_state(dk.ctx, dk.incoh_state, 1) != G4MI_OK)) {
             err = g4mi_last_error();
             return false;
         }
         dk.spon_applied = true;
+        dk.incoh_saved = false;
     }
     // Beam::initSorting: one4one beams are sorted at the sort markers
     if (g4mi_set_sorting(dk.ctx, dk.setup.one4one ? 1 : 0) != G4MI_OK) {
@@ -1701,13 +1783,247 @@ bool do_track(Deck &dk, Namelist &nl, int device, bool quiet, bool dry, std::str
         err = g4mi_last_error();
         return false;
     }
-    dk.runs++;
     if (st.rootname.empty()) {
         err = "no rootname in &setup";
         return false;
     }
     if (!quiet) printf("Writing output file %s...\n", fname.c_str());
     return write_output(dk, fname, steps, zout, rec, nrec, dl, err);
+}
+
+// Time::finishInit (src/Main/GenTime.cpp:64-86) after a change of the
+// reference length or sample rate, on one rank
+void time_refresh(Deck &dk) {
+    TimeNL &t = dk.time;
+    if (!(t.ds > 0)) return;  // no &time: the window is not initialised
+    t.ds = dk.setup.lambda0 * t.sample;
+    t.nslice = std::max(1, (int)std::round(t.slen / t.ds));
+    t.slen = t.ds * t.nslice;
+}
+
+// AlterSetup.cpp:136-170 with Beam::subharmonicConversion (Beam.cpp:243-299)
+// and Field::subharmonicConversion (Field.cpp:188-258): the reference
+// length grows by h, theta shrinks by h; resampling (one4one) merges h
+// slices into one (all particles into slice 0 at theta + i dtheta, then
+// sorted); harmonic fields are deleted and the fundamental becomes
+// harmonic h of the new reference, its slices merged in groups of h with
+// the group's mean power.
+bool subharmonic_conversion(Deck &dk, int h, bool resample, int device, bool dry, std::string &err) {
+    TimeNL &t = dk.time;
+    const double lam_old = dk.setup.lambda0;
+    if (!resample && t.dotime && t.sample % h != 0) {
+        err = "subharmonic conversion without resampling needs a sample rate that is a multiple of " +
+              std::to_string(h);
+        return false;
+    }
+    dk.setup.lambda0 = lam_old * h;
+    if (!resample && t.dotime) t.sample = t.sample / h;
+    time_refresh(dk);
+    const double hd = (double)h;
+    if (dk.have_beam) {
+        const int ns = (int)dk.off.size() - 1;
+        if (resample && !dk.setup.one4one) {
+            err = "Cannot convert beam distribution to lower harmonic (resampling needs one4one)";
+            return false;
+        }
+        if (resample && ns % h != 0) {
+            err = "Cannot convert beam distribution to lower harmonic (" + std::to_string(ns) +
+                  " slices cannot be merged in groups of " + std::to_string(h) + ")";
+            return false;
+        }
+        for (double &v : dk.theta) v /= hd;
+        if (resample) {
+            // the beam's slicelength and reflength after the conversion
+            const double slicelength = lam_old * (double)t.sample * hd, reflength = lam_old * hd;
+            const double dtheta = 4. * asin(1) * slicelength / reflength / hd;
+            for (int i = 1; i < ns; i++)
+                for (long long j = dk.off[i]; j < dk.off[i + 1]; j++) dk.theta[j] += i * dtheta;
+            const long long n = dk.off[ns];
+            dk.off.assign((size_t)ns / h + 1, n);
+            dk.off[0] = 0;
+            dk.current.assign((size_t)ns / h, 0.0);
+            if (!dry && !device_sort(dk, device, err)) return false;
+        }
+    }
+    dk.hfields.clear();  // AlterSetup.cpp:151-156
+    if (dk.have_field) {
+        Deck::HarmField f;
+        f.harm = h;
+        f.ngrid = dk.ngrid;
+        f.gridmax = dk.gridmax;
+        f.field.swap(dk.field);
+        const size_t cells = (size_t)f.ngrid * f.ngrid, per = 2 * cells;
+        const int nsize = (int)(f.field.size() / per);
+        if (resample && h > 1) {
+            if (nsize % h != 0) {
+                err = "Cannot convert field distribution to lower harmonic";
+                return false;
+            }
+            std::vector<double> pw(nsize, 0.0);
+            for (int i = 0; i < nsize; i++)
+                for (size_t k = 0; k < cells; k++) {
+                    const double re = f.field[i * per + 2 * k], im = f.field[i * per + 2 * k + 1];
+                    pw[i] += re * re + im * im;
+                }
+            const int first = dk.field_first;
+            for (int i = 0; i < nsize; i += h) {  // merge each group into its first slice
+                const int is0 = (i + first) % nsize;
+                for (int j = 1; j < h; j++) {
+                    const int is1 = (i + j + first) % nsize;
+                    for (size_t q = 0; q < per; q++) f.field[is0 * per + q] += f.field[is1 * per + q];
+                    pw[is0] += pw[is1];
+                }
+            }
+            const double scl = 1. / hd;
+            const int di = first % h;
+            for (int i = 0; i < nsize / h; i++) {
+                if (i != i * h + di)
+                    std::memcpy(f.field.data() + i * per, f.field.data() + (size_t)(i * h + di) * per,
+                                per * sizeof(double));
+                pw[i] = pw[i * h + di] * scl;
+            }
+            dk.field_first = (first - di) / h;
+            f.field.resize((size_t)(nsize / h) * per);
+            for (int i = 0; i < nsize / h; i++) {
+                double pl = 0;
+                for (size_t k = 0; k < cells; k++) {
+                    const double re = f.field[i * per + 2 * k], im = f.field[i * per + 2 * k + 1];
+                    pl += re * re + im * im;
+                }
+                if (pw[i] > 0) {
+                    const double a = sqrt(pw[i] / pl);
+                    for (size_t q = 0; q < per; q++) f.field[i * per + q] *= a;
+                }
+            }
+        }
+        dk.hfields.push_back(std::move(f));
+        dk.field.clear();
+        dk.have_field = false;  // a fundamental of the new reference needs a &field
+    }
+    if (resample) dk.wake.set = false;  // Beam::subharmonicConversion clears the wakes
+    return true;
+}
+
+// AlterSetup.cpp:172-209 with Beam::harmonicConversion (Beam.cpp:187-241)
+// and Field::harmonicConversion (Field.cpp:261-288): the reference length
+// shrinks by h and theta grows by h; resampling (one4one) moves slice i to
+// slice h i and sorts; the field at harmonic h becomes the fundamental
+// (each slice copied into h slices when resampling) and every other field
+// is deleted.
+bool harmonic_conversion(Deck &dk, int h, bool resample, int device, bool dry, std::string &err) {
+    TimeNL &t = dk.time;
+    const double hd = (double)h;
+    dk.setup.lambda0 = dk.setup.lambda0 / hd;
+    if (!resample && t.dotime) t.sample = t.sample * h;
+    time_refresh(dk);
+    if (dk.have_beam) {
+        if (resample && !dk.setup.one4one) {
+            err = "Cannot convert beam distribution to higher harmonic (resampling needs one4one)";
+            return false;
+        }
+        for (double &v : dk.theta) v *= hd;
+        if (resample) {
+            const int ns = (int)dk.off.size() - 1;
+            std::vector<long long> off((size_t)ns * h + 1);
+            for (int s = 0; s <= ns * h; s++) off[s] = dk.off[(s + h - 1) / h];
+            dk.off.swap(off);
+            dk.current.assign((size_t)ns * h, 0.0);
+            if (!dry && !device_sort(dk, device, err)) return false;
+        }
+    }
+    Deck::HarmField *match = nullptr;
+    for (auto &e : dk.hfields)
+        if (e.harm == h) match = &e;
+    if (match) {
+        dk.field.swap(match->field);
+        dk.ngrid = match->ngrid;
+        dk.gridmax = match->gridmax;
+        dk.flambda = dk.setup.lambda0;  // Field::init: lambda0 / harm of the old reference
+        dk.have_field = true;
+        if (resample) {
+            const size_t per = 2 * (size_t)dk.ngrid * dk.ngrid;
+            const int nloc = (int)(dk.field.size() / per);
+            std::vector<double> f((size_t)nloc * h * per);
+            for (int i = nloc; i > 0; i--)
+                for (int j = 0; j < h; j++)
+                    std::memcpy(f.data() + (size_t)(i * h - 1 - j) * per, dk.field.data() + (size_t)(i - 1) * per,
+                                per * sizeof(double));
+            dk.field.swap(f);
+            dk.field_first *= h;
+        }
+    } else {
+        dk.field.clear();
+        dk.have_field = false;
+        dk.field_first = 0;
+    }
+    dk.hfields.clear();
+    if (resample) dk.wake.set = false;  // Beam::harmonicConversion clears the wakes
+    return true;
+}
+
+// &alter_setup (AlterSetup::init, src/Main/AlterSetup.cpp:37-220): a new
+// beamline, step length or rootname (else .Run<n> on the output names), then
+// the subharmonic and harmonic conversions of beam and fields.
+bool do_alter_setup(Deck &dk, Namelist &nl, int device, bool dry, std::string &err) {
+    Keys k{nl, ""};
+    std::string rootname, beamline;
+    double delz = dk.setup.delz;
+    int harmonic = 1, subharmonic = 1;
+    bool resample = false, disable = false;
+    k.str("rootname", rootname);
+    k.str("beamline", beamline);
+    k.dbl("delz", delz);
+    k.integer("subharmonic", subharmonic);
+    k.integer("harmonic", harmonic);
+    k.boolean("resample", resample);
+    k.boolean("disable", disable);
+    if (!k.finish()) {
+        err = k.err;
+        return false;
+    }
+    if (harmonic < 1 || subharmonic < 1) {
+        err = "harmonic and subharmonic must be at least 1";
+        return false;
+    }
+    if (disable && harmonic > 1) {
+        err = "disable = true (keeping non-matching harmonics as disabled fields) is not supported by this build";
+        return false;
+    }
+    dk.setup.delz = delz;
+    if (!dk.time.dotime) resample = false;  // AlterSetup.cpp:113
+    if (!beamline.empty()) {
+        LatticeResult lr = parse_and_unroll(dk.lattice_text, beamline, delz, dk.setup.lambda0, dk.setup.gamma0);
+        if (!lr.error.empty()) {
+            err = "lattice: " + lr.error;
+            return false;
+        }
+        dk.setup.beamline = beamline;
+    }
+    if (!rootname.empty()) {
+        dk.setup.rootname = rootname;
+        dk.runcount = 0;
+    } else {
+        dk.runcount++;
+    }
+    if (harmonic == 1 && subharmonic == 1) return true;
+    if (!release_ctx(dk, err)) return false;
+    if (subharmonic > 1 && !subharmonic_conversion(dk, subharmonic, resample, device, dry, err)) return false;
+    if (harmonic > 1 && !harmonic_conversion(dk, harmonic, resample, device, dry, err)) return false;
+    return true;
+}
+
+// &sort (GenMain.cpp:428-431): Beam::sort, which only moves one4one beams
+bool do_sort(Deck &dk, int device, bool dry, std::string &err) {
+    if (!dk.have_beam || !dk.setup.one4one || dry) return true;
+    if (dk.ctx) {
+        long long kept = 0;
+        if (g4mi_sort(dk.ctx, &kept) != G4MI_OK) {
+            err = g4mi_last_error();
+            return false;
+        }
+        return true;
+    }
+    return device_sort(dk, device, err);
 }
 
 int run_deck(const char *input_file, int device, int quiet, bool dry, std::string *summary) {
@@ -1736,6 +2052,9 @@ int run_deck(const char *input_file, int device, int quiet, bool dry, std::strin
         else if (e == "&importfield") ok = do_importfield(dk, nl, err);
         else if (e == "&wake") ok = do_wake(dk, nl, err);
         else if (e == "&sponrad") ok = do_sponrad(dk, nl, err);
+        else if (e == "&alter_setup") ok = dk.setup.defined ? do_alter_setup(dk, nl, device, dry, err)
+                                                           : (err = "&alter_setup before &setup", false);
+        else if (e == "&sort") ok = do_sort(dk, device, dry, err);
         else {
             err = "element " + e + " is not supported by this build";
             ok = false;

@@ -322,6 +322,20 @@ Ran2::Ran2(unsigned int seed) {
     iy_ = iv_[0];
 }
 
+void Ran2::save(int *state) const {
+    for (int i = 0; i < 32; i++) state[i] = iv_[i];
+    state[32] = iy_;
+    state[33] = s1_;
+    state[34] = s2_;
+}
+
+void Ran2::load(const int *state) {
+    for (int i = 0; i < 32; i++) iv_[i] = state[i];
+    iy_ = state[32];
+    s1_ = state[33];
+    s2_ = state[34];
+}
+
 double Ran2::next() {
     s1_ = lcg(s1_, kIa1, kIq1, kIr1, kIm1);
     s2_ = lcg(s2_, kIa2, kIq2, kIr2, kIm2);

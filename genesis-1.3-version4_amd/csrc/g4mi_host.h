@@ -62,6 +62,10 @@ class Ran2 {
   public:
     explicit Ran2(unsigned int seed);
     double next();
+    // the generator's whole state (shuffle table, last output, both LCGs)
+    static constexpr int kStateInts = 35;
+    void save(int *state) const;
+    void load(const int *state);
 
   private:
     int iv_[32], iy_, s1_, s2_;

@@ -86,6 +86,7 @@ def lib():
         L.g4mi_destroy.argtypes = [C.c_void_p]
         L.g4mi_set_efield.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_double]
         L.g4mi_set_incoherent.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int]
+        L.g4mi_incoherent_state.argtypes = [C.c_void_p, C.POINTER(C.c_int), C.c_int]
         L.g4mi_set_sorting.argtypes = [C.c_void_p, C.c_int]
         L.g4mi_add_harmonic.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_double, C.POINTER(C.c_int)]
         L.g4mi_harmonic_upload.argtypes = [C.c_void_p, C.c_int, _dp]
@@ -254,6 +255,14 @@ class Context:
         """Beam::initIncoherent (src/Core/Incoherent.cpp:17-31), rank 0."""
         _chk(lib().g4mi_set_incoherent(self.h, seed, nbins, int(do_loss), int(do_spread)),
              "g4mi_set_incoherent")
+
+    def incoherent_state(self, state=None):
+        """Read (state None) or set the incoherent stream's 35-int state."""
+        buf = (C.c_int * 35)()
+        if state is not None:
+            buf[:] = [int(v) for v in state]
+        _chk(lib().g4mi_incoherent_state(self.h, buf, 0 if state is None else 1), "g4mi_incoherent_state")
+        return list(buf)
 
     def add_harmonic(self, harm, ngrid, gridmax):
         """A field at harmonic `harm` with its own grid; returns its index."""

@@ -127,6 +127,13 @@ int g4mi_field_track(g4mi_ctx *ctx, const g4mi_step_params *st, double delz);
  * drawn on the host in slice and particle order.  do_loss = do_spread = 0
  * switches it off.  FP64 form only. */
 int g4mi_set_incoherent(g4mi_ctx *ctx, int seed, int nbins, int do_loss, int do_spread);
+/* The state of the incoherent stream (35 ints: ran2's shuffle table, last
+ * output and both generators): store = 0 reads it into state, store = 1
+ * sets it.  No counterpart in the reference, whose Incoherent object lives
+ * as long as the beam: it lets a host that rebuilds its context between
+ * runs (a changed slice count or reference wavelength) continue the same
+ * stream. */
+int g4mi_incoherent_state(g4mi_ctx *ctx, int *state, int store);
 
 /* Fields at harmonics of the fundamental (&field harm = h: Field::init with
  * harm, src/Core/Field.cpp:34-70), FP64 form, ADI solver, at most 4 besides

@@ -393,3 +393,85 @@ def load_gauss_all(sim, power, w0, lambda0, z0=0.0):
         f = np.zeros(sim.s.ngrid * sim.s.ngrid * 2)
         lib().orc_load_gauss(_ptr(f), sim.s.ngrid, gm, lambda0, power, w0, z0, 0.0, 0.0, 0.0)
         sim.field[j] = f.reshape(-1, 2)
+
+
+# ---------------------------------------------------------------------------
+# &alter_setup conversions (src/Main/AlterSetup.cpp:136-209), restated on
+# numpy arrays: fields are [nslice][cells][2] in storage order with ring
+# offset `first`, beams are CSR (off[nslice+1]) with coordinate arrays.
+# ---------------------------------------------------------------------------
+def field_harmonic_conversion(field, first, h, resample):
+    """Field::harmonicConversion (src/Core/Field.cpp:261-288): each storage
+    slot copied into h slots, first *= h.  Returns (field, first)."""
+    if not resample or h == 1:
+        return field.copy(), first
+    nloc = field.shape[0]
+    out = np.zeros((nloc * h,) + field.shape[1:])
+    for i in range(nloc, 0, -1):
+        for j in range(h):
+            out[i * h - 1 - j] = field[i - 1]
+    return out, first * h
+
+
+def field_subharmonic_conversion(field, first, h, resample):
+    """Field::subharmonicConversion (src/Core/Field.cpp:188-258): groups of h
+    slices in ring order summed into the group's first slot, compacted to
+    slot i from i h + first % h, and scaled to the group's mean power.
+    Returns (field, first)."""
+    if not resample or h == 1:
+        return field.copy(), first
+    f = field.copy()
+    nsize = f.shape[0]
+    pw = [float(np.sum(f[i] * f[i])) for i in range(nsize)]
+    for i in range(0, nsize, h):
+        is0 = (i + first) % nsize
+        for j in range(1, h):
+            is1 = (i + j + first) % nsize
+            f[is0] += f[is1]
+            pw[is0] += pw[is1]
+    scl = 1.0 / h
+    di = first % h
+    for i in range(nsize // h):
+        f[i] = f[i * h + di]
+        pw[i] = pw[i * h + di] * scl
+    first = (first - di) // h
+    f = f[:nsize // h].copy()
+    for i in range(f.shape[0]):
+        pl = float(np.sum(f[i] * f[i]))
+        if pw[i] > 0:
+            f[i] *= np.sqrt(pw[i] / pl)
+    return f, first
+
+
+def beam_harmonic_conversion(off, theta, h, resample):
+    """Beam::harmonicConversion (src/Core/Beam.cpp:187-241) before its sort:
+    theta *= h; resampling moves slice i to slice h i.  Returns (off, theta)."""
+    theta = theta * float(h)
+    if not resample:
+        return off.copy(), theta
+    ns = len(off) - 1
+    cnt = np.zeros(ns * h, np.int64)
+    cnt[::h] = np.diff(off)
+    new = np.zeros(ns * h + 1, np.int64)
+    new[1:] = np.cumsum(cnt)
+    return new, theta
+
+
+def beam_subharmonic_conversion(off, theta, h, resample, lambda0, sample):
+    """Beam::subharmonicConversion (src/Core/Beam.cpp:243-299) before its
+    sort: theta /= h; resampling puts every particle into slice 0 at
+    theta + i dtheta (dtheta the old slice in the new phase units) and
+    keeps ns / h slices.  lambda0 and sample are the values before the
+    conversion.  Returns (off, theta)."""
+    theta = theta / float(h)
+    if not resample:
+        return off.copy(), theta
+    ns = len(off) - 1
+    assert ns % h == 0
+    slicelength, reflength = lambda0 * float(sample) * h, lambda0 * h
+    dtheta = 4.0 * np.arcsin(1.0) * slicelength / reflength / float(h)
+    for i in range(1, ns):
+        theta[off[i]:off[i + 1]] += i * dtheta
+    new = np.full(ns // h + 1, off[-1], np.int64)
+    new[0] = 0
+    return new, theta
