@@ -6,6 +6,7 @@
 // kernels in g4mi_kernels.hip on the context's stream; only per-step scalars
 // are prepared on the host.
 #include <algorithm>
+#include <complex>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -50,6 +51,7 @@ struct Ctx {
     int *d_blk_slice = nullptr;
     long long *d_blk_begin = nullptr;
     double *d_diag_b = nullptr, *d_diag_f = nullptr;
+    double *d_bunch = nullptr;  // harmonic bunching sums, [ns][8][2]
     unsigned long long *d_count = nullptr;
     int *d_atar = nullptr;
     // state
@@ -1899,6 +1901,41 @@ int g4mi_harmonic_download(g4mi_ctx *hc, int index, double *field) {
     if (!field) return G4MI_E_ARG;
     G4_HIP(hipStreamSynchronize(hc->c.stream));
     G4_HIP(hipMemcpy(field, h->d_field, (size_t)hc->c.ns * h->ng * h->ng * sizeof(double2), hipMemcpyDeviceToHost));
+    return G4MI_OK;
+}
+
+int g4mi_diag_bunching(g4mi_ctx *hc, int hmax, double *bunching, double *phase) {
+    if (!hc || hmax < 2 || !bunching || !phase) {
+        set_error("g4mi_diag_bunching: invalid argument (hmax >= 2, outputs)");
+        return G4MI_E_ARG;
+    }
+    Ctx *c = &hc->c;
+    if (c->precision != G4MI_FP64) {
+        set_error("g4mi_diag_bunching: harmonic bunching runs in the FP64 form only");
+        return G4MI_E_ARG;
+    }
+    const int ns = c->ns;
+    if (!c->d_bunch) G4_TRY(dalloc(c, &c->d_bunch, (size_t)ns * 16));
+    std::vector<double> hb((size_t)ns * 16);
+    for (int h0 = 2; h0 <= hmax; h0 += 8) {
+        const int cnt = std::min(8, hmax - h0 + 1);
+        launch_diag_bunch(c->stream, ns, c->d_off, c->d_th, h0, cnt, c->d_bunch);
+        G4_HIP(hipGetLastError());
+        G4_HIP(hipMemcpyAsync(hb.data(), c->d_bunch, (size_t)ns * cnt * 2 * sizeof(double), hipMemcpyDeviceToHost,
+                              c->stream));
+        G4_HIP(hipStreamSynchronize(c->stream));
+        for (int is = 0; is < ns; is++) {
+            const long long n = c->off[is + 1] - c->off[is];
+            const double norm = n > 0 ? 1. / (double)n : 1.;  // DiagBeam: 1 / slice size
+            for (int k = 0; k < cnt; k++) {
+                const std::complex<double> b(hb[2 * ((size_t)is * cnt + k)] * norm,
+                                             hb[2 * ((size_t)is * cnt + k) + 1] * norm);
+                const size_t o = (size_t)(h0 + k - 2) * ns + is;
+                bunching[o] = std::abs(b);
+                phase[o] = atan2(b.imag(), b.real());
+            }
+        }
+    }
     return G4MI_OK;
 }
 

@@ -1316,6 +1316,13 @@ bool write_output(const Deck &dk, const std::string &fname, const std::vector<g4
     }
     put("Beam/bunching", " ", "bunching");
     put("Beam/bunchingphase", "rad", "bunchingphase");
+    for (int h = 2; h < 1000; h++) {  // bunching at harmonics (&track bunchharm)
+        const std::string b = "bunching" + std::to_string(h), ph = "bunchingphase" + std::to_string(h);
+        if (rec.count(b)) {
+            put(("Beam/" + b).c_str(), " ", b.c_str());
+            put(("Beam/" + ph).c_str(), "rad", ph.c_str());
+        }
+    }
     if (!st.excl_aux) {
         put("Beam/efield", "eV/m", "efield");
         put("Beam/wakefield", "eV/m", "wakefield");
@@ -1494,8 +1501,8 @@ bool do_track(Deck &dk, Namelist &nl, int device, bool quiet, bool dry, std::str
         err = "&track: fft_fieldsolver runs in the FP64 form only (precision = fp64)";
         return false;
     }
-    if (bunchharm > 1) {
-        err = "&track: bunching harmonics above 1 are not supported by this build";
+    if (bunchharm > 1 && prec == "fp32") {
+        err = "&track: bunching harmonics (bunchharm > 1) run in the FP64 form only (precision = fp64)";
         return false;
     }
     if (prec != "fp64" && prec != "fp32") {
@@ -1636,6 +1643,16 @@ bool do_track(Deck &dk, Namelist &nl, int device, bool quiet, bool dry, std::str
         return false;
     }
     for (const char *key : kRecKeys) rec[key].assign(nrec * (size_t)ns, 0.0);
+    // DiagBeam (Diagnostic.cpp:486-500): bunching<h> for h = 2..bunchharm, or
+    // only h = bunchharm with exclusive_harmonics
+    std::vector<int> bharm;
+    for (int h = 2; h <= bunchharm; h++)
+        if (!exclharm || h == bunchharm) bharm.push_back(h);
+    for (int h : bharm) {
+        rec["bunching" + std::to_string(h)].assign(nrec * (size_t)ns, 0.0);
+        rec["bunchingphase" + std::to_string(h)].assign(nrec * (size_t)ns, 0.0);
+    }
+    std::vector<double> hbun, hpha;
     for (const char *key : kRecScalars) rec[key].assign(nrec, 0.0);
     for (size_t k = 0; k < dk.hfields.size(); k++) {
         const std::string pre = "H" + std::to_string(k) + ":";
@@ -1709,6 +1726,17 @@ bool do_track(Deck &dk, Namelist &nl, int device, bool quiet, bool dry, std::str
         d.ff_intensity = rec["ff_intensity"].data() + r * ns;
         d.global_energy = rec["global_energy"].data() + r;
         int rc = g4mi_diag(dk.ctx, &d);
+        if (rc == G4MI_OK && !bharm.empty()) {
+            hbun.resize((size_t)(bunchharm - 1) * ns);
+            hpha.resize(hbun.size());
+            rc = g4mi_diag_bunching(dk.ctx, bunchharm, hbun.data(), hpha.data());
+            for (int h : bharm) {
+                std::copy(hbun.begin() + (size_t)(h - 2) * ns, hbun.begin() + (size_t)(h - 1) * ns,
+                          rec["bunching" + std::to_string(h)].begin() + r * ns);
+                std::copy(hpha.begin() + (size_t)(h - 2) * ns, hpha.begin() + (size_t)(h - 1) * ns,
+                          rec["bunchingphase" + std::to_string(h)].begin() + r * ns);
+            }
+        }
         for (size_t k = 0; rc == G4MI_OK && k < dk.hfields.size(); k++) {  // DiagField of every harmonic
             const std::string pre = "H" + std::to_string(k) + ":";
             g4mi_diag_out dh{};

@@ -1293,6 +1293,37 @@ __global__ void __launch_bounds__(256) k_diag_beam(const long long *__restrict__
         for (int k = 0; k < 8; k++) out[8 * is + k] = v[k];
 }
 
+// Harmonic bunching sums of DiagBeam::getValues (Diagnostic.cpp:431-436) per
+// beam slice for harmonics h0 .. h0 + cnt - 1 (cnt <= 8):
+// out[2 (is cnt + k) + {0, 1}] = sum cos((h0 + k) th), sum sin((h0 + k) th).
+__global__ void __launch_bounds__(256) k_diag_bunch(const long long *__restrict__ off,
+                                                    const double *__restrict__ th, int h0, int cnt,
+                                                    double *__restrict__ out) {
+    __shared__ double lds[16 * 256];
+    const int is = blockIdx.x;
+    const long long a = off[is], b = off[is + 1];
+    double v[16];
+#pragma unroll
+    for (int k = 0; k < 16; k++) v[k] = 0;
+    for (long long i = a + threadIdx.x; i < b; i += 256) {
+        const double t = th[i];
+#pragma unroll
+        for (int k = 0; k < 8; k++)
+            if (k < cnt) {
+                double sn, cs;
+                sincos((double)(h0 + k) * t, &sn, &cs);
+                v[2 * k] += cs;
+                v[2 * k + 1] += sn;
+            }
+    }
+    block_sum<256, 16>(v, lds);
+    if (threadIdx.x == 0)
+        for (int k = 0; k < cnt; k++) {
+            out[2 * ((size_t)is * cnt + k)] = v[2 * k];
+            out[2 * ((size_t)is * cnt + k) + 1] = v[2 * k + 1];
+        }
+}
+
 // Field part of DiagField::getValues (:754-861) per storage slice:
 // out[8*is0 + ...] = power x1 x2 y1 y2 ffre ffim inten(on-axis |E|^2).
 // Auxiliary beam diagnostics of DiagBeam::getValues (Diagnostic.cpp:415-445,
@@ -1587,6 +1618,10 @@ void launch_diag_aux(hipStream_t s, int ns, const long long *off, const double *
                      const double *y, const double *px, const double *py, const double2 *field, int ng,
                      int first, double *out) {
     hipLaunchKernelGGL(k_diag_aux, dim3(ns), dim3(256), 0, s, off, g, x, y, px, py, field, ng, ns, first, out);
+}
+void launch_diag_bunch(hipStream_t s, int ns, const long long *off, const double *th, int h0, int cnt,
+                       double *out) {
+    if (ns > 0 && cnt > 0) hipLaunchKernelGGL(k_diag_bunch, dim3(ns), dim3(256), 0, s, off, th, h0, cnt, out);
 }
 void launch_diag_field(hipStream_t s, int ns, const double2 *field, int ng, double *out) {
     hipLaunchKernelGGL(k_diag_field, dim3(ns), dim3(256), 0, s, field, ng, out);

@@ -87,6 +87,7 @@ def lib():
         L.g4mi_set_efield.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_double]
         L.g4mi_set_incoherent.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int]
         L.g4mi_incoherent_state.argtypes = [C.c_void_p, C.POINTER(C.c_int), C.c_int]
+        L.g4mi_diag_bunching.argtypes = [C.c_void_p, C.c_int, _dp, _dp]
         L.g4mi_set_sorting.argtypes = [C.c_void_p, C.c_int]
         L.g4mi_add_harmonic.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_double, C.POINTER(C.c_int)]
         L.g4mi_harmonic_upload.argtypes = [C.c_void_p, C.c_int, _dp]
@@ -364,6 +365,13 @@ class Context:
 
     def sync(self):
         _chk(lib().g4mi_sync(self.h), "g4mi_sync")
+
+    def diag_bunching(self, hmax):
+        """Bunching and its phase at harmonics 2..hmax, arrays [hmax-1][ns]."""
+        b = np.zeros((hmax - 1, self.ns))
+        ph = np.zeros((hmax - 1, self.ns))
+        _chk(lib().g4mi_diag_bunching(self.h, hmax, _p(b), _p(ph)), "g4mi_diag_bunching")
+        return b, ph
 
     def diag(self, spectral=False):
         """One diagnostics record; spectral=True adds the divergence and

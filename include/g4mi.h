@@ -135,6 +135,13 @@ int g4mi_set_incoherent(g4mi_ctx *ctx, int seed, int nbins, int do_loss, int do_
  * stream. */
 int g4mi_incoherent_state(g4mi_ctx *ctx, int *state, int store);
 
+/* Bunching at the harmonics 2..hmax of the beam's phase, DiagBeam::getValues
+ * (src/Core/Diagnostic.cpp:431-436, 486-500: b_h = (1/N) sum exp(i h theta)
+ * per slice, output bunching<h> = |b_h|, bunchingphase<h> = arg b_h).
+ * bunching and phase are [hmax-1][nslice], row h-2 for harmonic h.  FP64
+ * form; synchronous. */
+int g4mi_diag_bunching(g4mi_ctx *ctx, int hmax, double *bunching, double *phase);
+
 /* Fields at harmonics of the fundamental (&field harm = h: Field::init with
  * harm, src/Core/Field.cpp:34-70), FP64 form, ADI solver, at most 4 besides
  * the fundamental.  Each has its own grid (ngrid, gridmax) and wavenumber
