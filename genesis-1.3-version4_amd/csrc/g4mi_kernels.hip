@@ -326,12 +326,13 @@ __global__ void __launch_bounds__(256) k_sc_solve(
             for (long long i = a + t; i < b; i += 256) {
                 double tx = x[i] - xc, ty = y[i] - yc;
                 double radi = sqrt(tx * tx + ty * ty);
-                double2 cw = make_double2(tx / radi, ty / radi);
                 int bin = (int)floor(radi / dr);
                 bin = (bin >= 0 && bin < ngr) ? bin : (bin >= ngr ? ngr - 1 : 0);
                 double s, c;
                 sincos(l * th[i], &s, &c);
-                double2 v = cmul(cpow_int(cw, -m), make_double2(c, -s));
+                double2 v = make_double2(c, -s);
+                if (m != 0)  // cwork^0 is exactly 1: the azimuthal factor only for m != 0
+                    v = cmul(cpow_int(make_double2(tx / radi, ty / radi), -m), v);
                 unsafeAtomicAdd(&hist[w][bin].x, v.x);
                 unsafeAtomicAdd(&hist[w][bin].y, v.y);
             }
@@ -351,26 +352,31 @@ __global__ void __launch_bounds__(256) k_sc_solve(
             }
             __syncthreads();
             if (t == 0) {  // tridiag (:260-272)
-                double2 bet = make_double2(cmid[0], 0);
-                celm[0] = cdiv(rhs[0], bet);
+                // the operator's entries are real (complex with zero
+                // imaginary parts in the reference), so bet and gam are real:
+                // one reciprocal per cell on the sequential chain
+                double bet = cmid[0], inv = 1.0 / bet;
+                celm[0] = cscale(rhs[0], inv);
                 for (int i = 1; i < ngr; i++) {
-                    gam[i] = cdiv(make_double2(cupp[i], 0), bet);
-                    bet = csub(make_double2(cmid[i], 0), cmul(make_double2(clow[i], 0), gam[i]));
-                    celm[i] = cdiv(csub(rhs[i], cmul(make_double2(clow[i], 0), celm[i - 1])), bet);
+                    const double g = cupp[i] * inv;
+                    gam[i].x = g;
+                    bet = cmid[i] - clow[i] * g;
+                    inv = 1.0 / bet;
+                    celm[i] = cscale(csub(rhs[i], cscale(celm[i - 1], clow[i])), inv);
                 }
-                for (int i = ngr - 2; i > -1; i--) celm[i] = csub(celm[i], cmul(gam[i + 1], celm[i + 1]));
+                for (int i = ngr - 2; i > -1; i--) celm[i] = csub(celm[i], cscale(celm[i + 1], gam[i + 1].x));
                 if (l == 1) efield[is] = hypot(celm[0].x, celm[0].y);
             }
             __syncthreads();
             for (long long i = a + t; i < b; i += 256) {
                 double tx = x[i] - xc, ty = y[i] - yc;
                 double radi = sqrt(tx * tx + ty * ty);
-                double2 cw = make_double2(tx / radi, ty / radi);
                 int bin = (int)floor(radi / dr);
                 bin = (bin >= 0 && bin < ngr) ? bin : (bin >= ngr ? ngr - 1 : 0);
                 double s, c;
                 sincos(l * th[i], &s, &c);
-                double2 ct = cmul(cpow_int(cw, m), make_double2(c, s));
+                double2 ct = make_double2(c, s);
+                if (m != 0) ct = cmul(cpow_int(make_double2(tx / radi, ty / radi), m), ct);
                 ez[i] += 2 * cmul(ct, celm[bin]).x;
             }
         }
