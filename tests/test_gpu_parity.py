@@ -275,6 +275,26 @@ def test_config1_full_size_first_steps():
     assert ep < 1e-10 and eb < 1e-10
 
 
+def test_config3_full_size_first_steps():
+    """Config 3 at full size (4096 slices with the Gaussian current, 2220 to
+    16384 macroparticles per slice, 40.1 M in total, 255^2, shot noise, LSC),
+    the largest single-GPU workload: the first two steps against the oracle
+    (threaded CPU run), P and |b| of every slice within 1e-10."""
+    import g4mi_workloads as W
+    _log("config 3 full size (40.1 M particles, 4096 x 255^2), 2 steps GPU vs oracle")
+    cur, cnt = W.config3_profile()
+    st = H.steps()
+    sim = H.make_state(cnt, ngrid=255, currents=cur, longrange=True, sc=(1, 0, 100, 0.0), threads=16)
+    ctx = H.to_gpu(sim)
+    g = ctx.run(H.gsteps(st[:2]), cases.GAMMA0, record=("power", "bunching"))
+    ctx.close()
+    r = sim.run(st, cases.GAMMA0, record=("power", "bunching"), nsteps=2)
+    ep = H.rel_err(g["power"][1:], r["power"][1:])
+    eb = H.rel_err(g["bunching"], r["bunching"])
+    _log(f"  {int(sim.off[-1])} particles: P rel err {ep:.3e}  |b| rel err {eb:.3e}")
+    assert ep < 1e-10 and eb < 1e-10
+
+
 def test_incoherent_and_wake_kicks(st):
     """Incoherent SR (loss and spread, beamlets of 8 over slices of 512,
     508, 512 particles: one ragged beamlet) and a per-slice wake loss after
