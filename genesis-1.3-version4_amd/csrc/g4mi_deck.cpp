@@ -758,13 +758,31 @@ bool do_importfield(Deck &dk, Namelist &nl, std::string &err) {
         err = k.err;
         return false;
     }
-    if (harm != 1) {
-        err = "&importfield: only the fundamental (harmonic = 1) is supported";
+    if (harm < 1) {
+        err = "&importfield: harmonic must be at least 1";
         return false;
     }
-    if (dk.have_field) {
+    bool exists = harm == 1 && dk.have_field;
+    for (const auto &e : dk.hfields) exists |= e.harm == harm;
+    if (exists) {  // ImportField.cpp:73-91
         err = "cannot import field, because field is already defined";
         return false;
+    }
+    if (harm > 1 && !dk.have_field) {
+        err = "&importfield: define or import the fundamental (harmonic = 1) before its harmonics";
+        return false;
+    }
+    if (harm > 1 && dk.hfields.size() >= 4) {
+        err = "&importfield: at most 4 harmonic fields";
+        return false;
+    }
+    // a field imported after a &track: the tracked fields keep their window
+    // order, the new one starts with ring offset 0 (Field::init)
+    if (!release_ctx(dk, err)) return false;
+    if (dk.field_first != 0) {
+        unroll_ring(dk.field, (size_t)dk.ngrid * dk.ngrid, dk.field_first);
+        for (auto &e : dk.hfields) unroll_ring(e.field, (size_t)e.ngrid * e.ngrid, dk.field_first);
+        dk.field_first = 0;
     }
     double s0f = 0, dgrid = 0, reflen = 0, slicelen = 0, countd = 0, ngd = 0;
     if (!read_scalar(file, "refposition", s0f, err) || !read_scalar(file, "gridsize", dgrid, err) ||
@@ -788,7 +806,8 @@ bool do_importfield(Deck &dk, Namelist &nl, std::string &err) {
     const int ns = time_nslice(dk);
     const double ds = time_ds(dk);
     const size_t cells = (size_t)ngrid * ngrid;
-    dk.field.assign((size_t)ns * cells * 2, 0.0);
+    std::vector<double> &target = harm > 1 ? (dk.hfields.emplace_back(), dk.hfields.back().field) : dk.field;
+    target.assign((size_t)ns * cells * 2, 0.0);
     for (int i = 0; i < ns; i++) {
         const double s = dk.time.s0 + i * ds;
         const double rslice = (s - s0f) / slicelen;
@@ -808,11 +827,18 @@ bool do_importfield(Deck &dk, Namelist &nl, std::string &err) {
             err = file + ": slice " + std::to_string(islice) + " does not hold gridpoints^2 values";
             return false;
         }
-        double *f = dk.field.data() + (size_t)i * cells * 2;
+        double *f = target.data() + (size_t)i * cells * 2;
         for (size_t j = 0; j < cells; j++) {
             f[2 * j] = scl * re[j];
             f[2 * j + 1] = scl * im[j];
         }
+    }
+    if (harm > 1) {
+        Deck::HarmField &h = dk.hfields.back();
+        h.harm = harm;
+        h.ngrid = ngrid;
+        h.gridmax = gridmax;
+        return true;
     }
     dk.ngrid = ngrid;
     dk.gridmax = gridmax;
@@ -1080,8 +1106,9 @@ bool write_field_dump_of(const Deck &dk, const std::vector<double> &field, int n
     }
     bool ok = true;
     write_version(f, ok);
-    const int ns = (int)dk.off.size() - 1, ng = ngrid;
+    const int ng = ngrid;
     const size_t cells = (size_t)ng * ng;
+    const int ns = (int)(field.size() / (2 * cells));  // the field's own slices (a beam may not exist yet)
     const double dgrid = 2 * gridmax / (ng - 1);
     ok &= f.write_single("wavelength", "m", {xlambda});
     ok &= f.write_single("slicespacing", "m", {slicelen});

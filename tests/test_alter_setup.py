@@ -284,3 +284,36 @@ def test_alter_setup_one4one_resample_deck(tmp_path):
         assert _slice(up, "current", i + 1)[0] == pytest.approx(cur[i], rel=1e-15)
     assert os.path.exists(root + ".Run2.out.h5")
     print(f"  counts {np.diff(o).tolist()}", flush=True)
+
+
+def test_importfield_harmonic_round_trip(tmp_path):
+    """&importfield harmonic = 3 (ImportField.cpp:42-91, readFieldHDF5.cpp:21-140):
+    the 3rd-harmonic dump of a deck comes back as the 3rd-harmonic field of
+    another (its wavelength lambda0 / 3 checked against lambda0 / harmonic),
+    within one rounding of the sqrt(W) scaling; importing it twice, or before
+    the fundamental, is an error."""
+    import test_deck as T
+    src = str(tmp_path / "s")
+    T._write_planar(tmp_path, T.HARM_IN.replace("&track\nzstop=5\n&end", "&write\nfield = @.init\n&end"), src)
+    g4mi.check_deck(tmp_path / "Example1.in")
+    body = T.EX1_IN.split("&field")[0]  # &setup and &lattice
+    imp = (body + f"&importfield\nfile = {src}.init.fld.h5\n&end\n\n"
+           f"&importfield\nfile = {src}.init.h3.fld.h5\nharmonic = 3\n&end\n\n"
+           "&write\nfield = @.back\n&end\n")
+    root = str(tmp_path / "i")
+    T._write_planar(tmp_path, imp, root)
+    g4mi.check_deck(tmp_path / "Example1.in")
+    a, b = src + ".init.h3.fld.h5", root + ".back.h3.fld.h5"
+    for k in ("gridsize", "wavelength", "gridpoints"):
+        assert np.array_equal(g4mi.h5_read(a, k), g4mi.h5_read(b, k)), k
+    for k in ("field-real", "field-imag"):
+        va, vb = _slice(a, k), _slice(b, k)
+        assert np.max(np.abs(va - vb)) <= 4e-16 * np.max(np.abs(va)), k
+    twice = imp.replace("&write", f"&importfield\nfile = {src}.init.h3.fld.h5\nharmonic = 3\n&end\n\n&write")
+    T._write_planar(tmp_path, twice, root)
+    with pytest.raises(RuntimeError, match="already defined"):
+        g4mi.check_deck(tmp_path / "Example1.in")
+    first = body + f"&importfield\nfile = {src}.init.h3.fld.h5\nharmonic = 3\n&end\n"
+    T._write_planar(tmp_path, first, root)
+    with pytest.raises(RuntimeError, match="fundamental"):
+        g4mi.check_deck(tmp_path / "Example1.in")
