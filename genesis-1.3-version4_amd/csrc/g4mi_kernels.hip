@@ -587,31 +587,65 @@ __global__ void __launch_bounds__(1024) k_deposit(const long long *__restrict__ 
         const int ncell = (r1 - r0 + 1) * ww;
         for (int k = t; k < ncell; k += nt) tile[k] = make_double2(0, 0);
         __syncthreads();
-        for (long long i = a + t; i < b; i += nt) {
-            double xi = x[i], yi = y[i];
-            if (!((xi > -p.gridmax) && (xi < p.gridmax) && (yi > -p.gridmax) && (yi < p.gridmax)))
-                continue;
-            double wx = (xi + p.gridmax) / p.dgrid;
-            double wy = (yi + p.gridmax) / p.dgrid;
-            int ix = min((int)floor(wx), ng - 2), iy = min((int)floor(wy), ng - 2);
-            if (iy + 1 < r0 || iy > r1) continue;
-            wx = 1 + floor(wx) - wx;
-            wy = 1 + floor(wy) - wy;
-            double ddx = xi - p.ax, ddy = yi - p.ay;
-            double faw2 = (1 + p.kx * ddx * ddx + p.ky * ddy * ddy + 2 * (p.gradx * ddx + p.grady * ddy));
-            double part = sqrt(faw2) * scl / g[i];
-            double s, c;
-            sincos(p.harm * th[i], &s, &c);
-            double2 cp = make_double2(s * part, c * part);
-            double wgt[4] = {wx * wy, (1 - wx) * wy, wx * (1 - wy), (1 - wx) * (1 - wy)};
-            int cx[4] = {ix, ix + 1, ix, ix + 1};
-            int cy[4] = {iy, iy, iy + 1, iy + 1};
+        // The trip count is the same for every lane (inactive lanes carry
+        // zeros), so the lanes of a group of four can combine their values.
+        // The quiet loader puts a beamlet's particles in consecutive slots at
+        // one transverse position: when the four lanes of a group deposit
+        // into the same cells, their sums go in with one set of atomics
+        // instead of four (fewer LDS atomic collisions).
+        for (long long base = a; base < b; base += nt) {
+            const long long i = base + t;
+            bool act = i < b;
+            double xi = 0, yi = 0;
+            if (act) {
+                xi = x[i];
+                yi = y[i];
+                act = (xi > -p.gridmax) && (xi < p.gridmax) && (yi > -p.gridmax) && (yi < p.gridmax);
+            }
+            int ix = 0, iy = 0;
+            double v[8] = {0, 0, 0, 0, 0, 0, 0, 0};  // (re, im) for cells (ix,iy) (ix+1,iy) (ix,iy+1) (ix+1,iy+1)
+            if (act) {
+                double wx = (xi + p.gridmax) / p.dgrid;
+                double wy = (yi + p.gridmax) / p.dgrid;
+                ix = min((int)floor(wx), ng - 2);
+                iy = min((int)floor(wy), ng - 2);
+                act = !(iy + 1 < r0 || iy > r1);
+                if (act) {
+                    wx = 1 + floor(wx) - wx;
+                    wy = 1 + floor(wy) - wy;
+                    double ddx = xi - p.ax, ddy = yi - p.ay;
+                    double faw2 = (1 + p.kx * ddx * ddx + p.ky * ddy * ddy + 2 * (p.gradx * ddx + p.grady * ddy));
+                    double part = sqrt(faw2) * scl / g[i];
+                    double s, c;
+                    sincos(p.harm * th[i], &s, &c);
+                    const double2 cp = make_double2(s * part, c * part);
+                    const double wgt[4] = {wx * wy, (1 - wx) * wy, wx * (1 - wy), (1 - wx) * (1 - wy)};
+#pragma unroll
+                    for (int k = 0; k < 4; k++) {
+                        v[2 * k] = wgt[k] * cp.x;
+                        v[2 * k + 1] = wgt[k] * cp.y;
+                    }
+                }
+            }
+            // all four lanes of the group active and in the same cell?
+            const int key = act ? iy * ng + ix : -1 - (t & 3);
+            const bool grp = key == __shfl_xor(key, 1) && key == __shfl_xor(key, 2) && key == __shfl_xor(key, 3);
+            double vs[8];
+#pragma unroll
+            for (int q = 0; q < 8; q++) {
+                vs[q] = v[q] + __shfl_xor(v[q], 1);
+                vs[q] += __shfl_xor(vs[q], 2);
+            }
+            if (!act || (grp && (t & 3) != 0)) continue;
+            const double *val = grp ? vs : v;
+            const int cx[4] = {ix, ix + 1, ix, ix + 1};
+            const int cy[4] = {iy, iy, iy + 1, iy + 1};
 #pragma unroll
             for (int k = 0; k < 4; k++) {
                 if (cy[k] < r0 || cy[k] > r1 || cx[k] > bx1) continue;
                 int lk = (cy[k] - r0) * ww + (cx[k] - bx0);
-                unsafeAtomicAdd(&tile[lk].x, wgt[k] * cp.x);
-                unsafeAtomicAdd(&tile[lk].y, wgt[k] * cp.y);
+                unsafeAtomicAdd(&tile[lk].x, val[2 * k]);
+                unsafeAtomicAdd(&tile[lk].y, val[2 * k + 1]);
             }
         }
         __syncthreads();
