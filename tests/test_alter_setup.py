@@ -317,3 +317,45 @@ def test_importfield_harmonic_round_trip(tmp_path):
     T._write_planar(tmp_path, first, root)
     with pytest.raises(RuntimeError, match="fundamental"):
         g4mi.check_deck(tmp_path / "Example1.in")
+
+
+@pytest.mark.gpu
+def test_restart_with_harmonic_field(tmp_path):
+    """Checkpoint and restart with a 3rd-harmonic field: the planar Example1
+    (63^2, harmonic 41^2) dumps beam and both fields after one FODO cell and
+    runs a second cell; a second deck imports the beam, the fundamental and
+    (&importfield harmonic = 3) the harmonic dump and tracks one cell.  Its
+    /Field/power and /Field3/power are the uninterrupted run's from the
+    dump on, within 1e-10."""
+    import test_deck as T
+    print("deck runner: restart from dumps with a 3rd-harmonic field", flush=True)
+    base = T.HARM_IN.replace("&track\nzstop=5\n&end", "&track\n&end")
+    path = T._write_planar(tmp_path, base, str(tmp_path / "full"))
+    s = g4mi.check_deck(path)
+    k = s["steps"] // 6
+    z = g4mi.h5_read(str(tmp_path / "full.out.h5"), "Lattice/z")
+    deck_a = base.replace("&track\n&end", f"&track\nzstop = {z[2 * k] - 1e-3}\nfield_dump_step = {k}\n"
+                          f"beam_dump_step = {k}\n&end")
+    path = T._write_planar(tmp_path, deck_a, str(tmp_path / "full"))
+    g4mi.run_deck(path, device=0, quiet=True)
+    out_a = str(tmp_path / "full.out.h5")
+    pa, p3a = g4mi.h5_read(out_a, "Field/power")[:, 0], g4mi.h5_read(out_a, "Field3/power")[:, 0]
+    src = str(tmp_path / f"full.{k}")
+    assert os.path.exists(src + ".h3.fld.h5")
+    deck_b = (base.replace("&field\npower=5e3\ndgrid=2.000000e-04\nngrid=63\nwaist_size=30e-6\n&end",
+                           f"&importfield\nfile = {src}.fld.h5\n&end")
+              .replace("&field\nharm=3\npower=1e2\ndgrid=1.5e-4\nngrid=41\nwaist_size=20e-6\n&end",
+                       f"&importfield\nfile = {src}.h3.fld.h5\nharmonic = 3\n&end")
+              .replace("&beam\ncurrent=3000\ndelgam=1.000000\nex=4.000000e-07\ney=4.000000e-07\n&end",
+                       f"&importbeam\nfile = {src}.par.h5\n&end")
+              .replace("&track\n&end", f"&track\nzstop = {z[k] - 1e-3}\n&end"))
+    assert deck_b.count("&importfield") == 2 and "&importbeam" in deck_b
+    path = T._write_planar(tmp_path, deck_b, str(tmp_path / "rest"))
+    g4mi.run_deck(path, device=0, quiet=True)
+    out_b = str(tmp_path / "rest.out.h5")
+    pb, p3b = g4mi.h5_read(out_b, "Field/power")[:, 0], g4mi.h5_read(out_b, "Field3/power")[:, 0]
+    assert len(pb) == k + 1
+    ep = float(np.max(np.abs(pb / pa[k:2 * k + 1] - 1)))
+    e3 = float(np.max(np.abs(p3b / p3a[k:2 * k + 1] - 1)))
+    print(f"  restart after {k} steps: P(z) rel err {ep:.3e}, P3(z) {e3:.3e}", flush=True)
+    assert ep < 1e-10 and e3 < 1e-10
