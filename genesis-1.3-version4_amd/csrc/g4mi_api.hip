@@ -119,6 +119,7 @@ struct Ctx {
         std::vector<int> zeroed;
     };
     double *h_recs = nullptr;
+    double *d_recs = nullptr;  // h_recs' device address
     int rec_cap = 0;
     std::vector<PendingRecord> pend;
     // one4one beams: Beam::sort at sort markers, with its work buffers
@@ -804,6 +805,7 @@ static int alloc_records(Ctx *c) {
     const size_t row = 16 * (size_t)c->ns;
     c->rec_cap = (int)std::max<size_t>(4, std::min<size_t>(256, ((size_t)32 << 20) / (row * sizeof(double))));
     G4_HIP(hipHostMalloc((void **)&c->h_recs, (size_t)c->rec_cap * row * sizeof(double)));
+    G4_HIP(hipHostGetDevicePointer((void **)&c->d_recs, c->h_recs, 0));
     return G4MI_OK;
 }
 
@@ -2177,7 +2179,8 @@ int g4mi_step(g4mi_ctx *h, const g4mi_step_params *st, double gammaref) {
 }
 
 // Deferred records of the FP64 run loop: each record's beam and field sums
-// are copied asynchronously into a pinned slot, and the host finishes them
+// are written into a pinned slot by a copy kernel on the step's stream, and
+// the host finishes them
 // when the slots are full or the run ends, so a record after every step
 // does not stall the device between steps.  Records that ask for the
 // auxiliary or spectral blocks, and one4one runs (whose slices change at
@@ -2210,9 +2213,10 @@ static int enqueue_record(Ctx *c, const g4mi_diag_out *o) {
     if ((int)c->pend.size() == c->rec_cap) G4_TRY(flush_records(c));
     launch_diag_beam(c->stream, c->ns, c->d_off, c->d_g, c->d_th, c->d_x, c->d_y, c->d_diag_b);
     if (!c->field_diag_fresh) launch_diag_field(c->stream, c->ns, c->d_field, c->ng, c->d_diag_f);
-    double *slot = c->h_recs + c->pend.size() * row;
-    G4_HIP(hipMemcpyAsync(slot, c->d_diag_b, 8 * ns * sizeof(double), hipMemcpyDeviceToHost, c->stream));
-    G4_HIP(hipMemcpyAsync(slot + 8 * ns, c->d_diag_f, 8 * ns * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    double *slot = c->d_recs + c->pend.size() * row;
+    launch_copy_out(c->stream, c->d_diag_b, slot, (long long)(8 * ns));
+    launch_copy_out(c->stream, c->d_diag_f, slot + 8 * ns, (long long)(8 * ns));
+    G4_HIP(hipGetLastError());
     c->pend.push_back({*o, c->first, c->field_diag_fresh, c->zeroed_since_adi});
     return G4MI_OK;
 }

@@ -62,6 +62,8 @@ int launch_adi_band(hipStream_t s, int ns, const double2 *fin, double2 *fout, co
                     const int4 *box, const double2 *band, double a, int ng, int first, int m,
                     int klo, int khi, double *tilediag, double *diag);
 void launch_zero(hipStream_t s, double2 *p, long long n);
+// n doubles from device memory to pinned host memory (dst: its device address)
+void launch_copy_out(hipStream_t s, const double *src, double *dst, long long n);
 // harmonic bunching sums for h0 .. h0 + cnt - 1 (cnt <= 8), out[ns][cnt][2]
 void launch_diag_bunch(hipStream_t s, int ns, const long long *off, const double *th, int h0, int cnt,
                        double *out);

@@ -1647,6 +1647,17 @@ int launch_adi_band(hipStream_t s, int ns, const double2 *fin, double2 *fout, co
     return 0;
 }
 
+// Copy n doubles into pinned host memory (its device address) with plain
+// stores on the compute queue: a record then costs no copy-engine transfer
+// and no hand-off between the copy and compute queues.
+__global__ void __launch_bounds__(256) k_copy_out(const double *__restrict__ src, double *__restrict__ dst,
+                                                  long long n) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) dst[i] = src[i];
+}
+void launch_copy_out(hipStream_t s, const double *src, double *dst, long long n) {
+    if (n > 0) hipLaunchKernelGGL(k_copy_out, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, src, dst, n);
+}
 void launch_zero(hipStream_t s, double2 *p, long long n) {
     if (n > 0) hipLaunchKernelGGL(k_zero, dim3(nblk(n, 256)), dim3(256), 0, s, p, n);
 }
