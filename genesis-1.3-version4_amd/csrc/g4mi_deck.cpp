@@ -12,6 +12,7 @@
 #include <pwd.h>
 
 #include <algorithm>
+#include <cctype>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -1343,12 +1344,12 @@ bool write_output(const Deck &dk, const std::string &fname, const std::vector<g4
     }
     put("Beam/bunching", " ", "bunching");
     put("Beam/bunchingphase", "rad", "bunchingphase");
-    for (int h = 2; h < 1000; h++) {  // bunching at harmonics (&track bunchharm)
-        const std::string b = "bunching" + std::to_string(h), ph = "bunchingphase" + std::to_string(h);
-        if (rec.count(b)) {
-            put(("Beam/" + b).c_str(), " ", b.c_str());
-            put(("Beam/" + ph).c_str(), "rad", ph.c_str());
-        }
+    for (const auto &kv : rec) {  // bunching<h>, bunchingphase<h> (&track bunchharm)
+        const std::string &key = kv.first;
+        const bool phase = key.rfind("bunchingphase", 0) == 0;
+        const size_t n = phase ? 13 : 8;  // length of the name before the harmonic number
+        if ((phase || key.rfind("bunching", 0) == 0) && key.size() > n && isdigit((unsigned char)key[n]))
+            put(("Beam/" + key).c_str(), phase ? "rad" : " ", key.c_str());
     }
     if (!st.excl_aux) {
         put("Beam/efield", "eV/m", "efield");
