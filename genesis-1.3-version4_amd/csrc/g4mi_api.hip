@@ -2211,10 +2211,11 @@ static int enqueue_record(Ctx *c, const g4mi_diag_out *o) {
     const size_t ns = (size_t)c->ns, row = 16 * ns;
     if (!c->h_recs) G4_TRY(alloc_records(c));
     if ((int)c->pend.size() == c->rec_cap) G4_TRY(flush_records(c));
-    launch_diag_beam(c->stream, c->ns, c->d_off, c->d_g, c->d_th, c->d_x, c->d_y, c->d_diag_b);
-    if (!c->field_diag_fresh) launch_diag_field(c->stream, c->ns, c->d_field, c->ng, c->d_diag_f);
+    // the beam sums go straight into the slot; the field sums of the ADI
+    // (or of k_diag_field) are copied after them
     double *slot = c->d_recs + c->pend.size() * row;
-    launch_copy_out(c->stream, c->d_diag_b, slot, (long long)(8 * ns));
+    launch_diag_beam(c->stream, c->ns, c->d_off, c->d_g, c->d_th, c->d_x, c->d_y, slot);
+    if (!c->field_diag_fresh) launch_diag_field(c->stream, c->ns, c->d_field, c->ng, c->d_diag_f);
     launch_copy_out(c->stream, c->d_diag_f, slot + 8 * ns, (long long)(8 * ns));
     G4_HIP(hipGetLastError());
     c->pend.push_back({*o, c->first, c->field_diag_fresh, c->zeroed_since_adi});
