@@ -9,8 +9,9 @@
  *
  * The reference has no C-ABI for this path.  Each entry point below names
  * the reference member function (file:line) whose contract it replaces; a
- * maintainer binds them from FieldSolver/BeamSolver subclasses (shown in
- * INTEGRATION.md).
+ * maintainer binds them from FieldSolver/BeamSolver subclasses whose
+ * init/advance/track call the matching entry points (include/g4mi.hpp
+ * wraps them for C++).
  */
 #ifndef G4MI_H
 #define G4MI_H
