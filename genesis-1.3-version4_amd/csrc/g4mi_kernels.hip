@@ -627,9 +627,12 @@ __global__ void __launch_bounds__(1024) k_deposit(const long long *__restrict__ 
                     }
                 }
             }
-            // all four lanes of the group active and in the same cell?
+            // all four lanes of the group active and in the same cell?  Every
+            // lane takes all three shuffles before any compare: a shuffle
+            // from a lane that skipped it would read a stale register.
             const int key = act ? iy * ng + ix : -1 - (t & 3);
-            const bool grp = key == __shfl_xor(key, 1) && key == __shfl_xor(key, 2) && key == __shfl_xor(key, 3);
+            const int k1 = __shfl_xor(key, 1), k2 = __shfl_xor(key, 2), k3 = __shfl_xor(key, 3);
+            const bool grp = (key == k1) & (key == k2) & (key == k3);
             double vs[8];
 #pragma unroll
             for (int q = 0; q < 8; q++) {

@@ -34,7 +34,9 @@ __global__ void __launch_bounds__(256) k_sort_keys(int ns, const long long *__re
         if (!((s < 0.0) || (s > smax))) {
             const int atar = (int)floor(theta * invslen);
             const int dest = a + atar;
-            if (dest < ns) {  // s exactly at the upper edge would index one past the last slice
+            // s exactly at the upper edge would index one past the last slice,
+            // and s = 0 can round to a target one below slice 0: both dropped
+            if (dest >= 0 && dest < ns) {
                 key = dest;
                 tn = theta - slen * (atar);
             }

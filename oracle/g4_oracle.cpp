@@ -1490,8 +1490,10 @@ extern "C" long long orc_sort_one4one(int ns, long long *off, long long cap, dou
                 P p = rec[a][b];
                 p.th = rec[a][b].th - slen * (atar);
                 /* a particle exactly at the window's upper edge (s = slen ns)
-                 * would index one past the last slice; it is dropped */
-                if (a + atar < ns) rec[a + atar].push_back(p);
+                 * would index one past the last slice, and one at s = 0 whose
+                 * floor(theta / slen) rounds below -a one before the first
+                 * (the reference's at() throws there); both are dropped */
+                if (a + atar >= 0 && a + atar < ns) rec[a + atar].push_back(p);
                 rec[a][b] = rec[a].back();
                 rec[a].pop_back();
             } else {

@@ -75,7 +75,7 @@ def _brute_sort(sim, slen):
             if s < 0 or s > slen * ns:
                 continue
             atar = math.floor(th * (1.0 / slen))
-            if a + atar >= ns:
+            if a + atar >= ns or a + atar < 0:
                 continue
             dest[a + atar].append((th - slen * atar, sim.gamma[i], sim.x[i], sim.y[i], sim.px[i], sim.py[i]))
     return [sorted(d) for d in dest]
@@ -90,6 +90,33 @@ def _scramble(sim, rng, slen, spread=1.6):
     """theta moved by up to +-spread slices, so particles leave their slice
     in both directions and some leave the window."""
     sim.theta += rng.uniform(-spread, spread, size=len(sim.theta)) * slen
+
+
+def _plant_lower_edge(sim, slen):
+    """Puts the first particle of a slice a > 0 at s = slen a + theta = 0
+    exactly, in a slice where floor(theta / slen) rounds to -a - 1, one
+    below the first slice (slen is the sort's own, 4 asin(1) slicelength /
+    reflength; at sample 9 this happens in slice 5).  Returns the index."""
+    assert slen == 4.0 * np.arcsin(1.0) * sim.s.slicelength / sim.s.reflength
+    for a in range(1, sim.ns):
+        th = -(slen * a)
+        if slen * a + th == 0.0 and math.floor(th * (1.0 / slen)) == -a - 1 and sim.off[a + 1] > sim.off[a]:
+            i = int(sim.off[a])
+            sim.theta[i] = th
+            return i
+    raise AssertionError("no slice rounds below the first one at this slice length")
+
+
+def test_oracle_sort_drops_lower_edge_particle():
+    sim, sl, cnt = _oracle_beam(8, sample=9, ngrid=15)
+    slen = 4.0 * np.arcsin(1.0) * sim.s.slicelength / sim.s.reflength
+    _plant_lower_edge(sim, slen)
+    ref = _brute_sort(sim, slen)
+    n0 = int(sim.off[-1])
+    sim.sort()
+    got = _per_slice(sim.off, {k: getattr(sim, k) for k in ("gamma", "theta", "x", "y", "px", "py")})
+    assert got == ref
+    assert int(sim.off[-1]) == n0 - 1
 
 
 def test_oracle_sort_matches_brute_force():
@@ -108,9 +135,10 @@ def test_oracle_sort_matches_brute_force():
 @pytest.mark.gpu
 def test_device_sort_matches_oracle():
     print("one4one sort: 8 slices, scrambled phases, GPU vs oracle", flush=True)
-    sim, sl, cnt = _oracle_beam(8, sample=3, ngrid=15)
-    slen = 2 * math.pi * 3
+    sim, sl, cnt = _oracle_beam(8, sample=9, ngrid=15)
+    slen = 4.0 * np.arcsin(1.0) * sim.s.slicelength / sim.s.reflength
     _scramble(sim, np.random.default_rng(11), slen)
+    _plant_lower_edge(sim, slen)
     ctx = H.to_gpu(sim)
     ctx.set_sorting(True)
     n = ctx.sort()
