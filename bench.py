@@ -4,10 +4,15 @@
 A step is one pass of the hot path over the whole time window: transverse
 half step, space charge, longitudinal RK4 push, transverse half step, source
 deposition, ADI field advance, slippage and the per-step P(z)/|b(z)|
-diagnostics -- Gencore::run's loop body with output_step=1.  The workload
-is config 1 (time-dependent SASE, FP64 plain form, 1024 slices x 8192
-macroparticles, 151x151 field, long- and short-range space charge), with
-beam and field already resident in HBM when the timed region starts.
+diagnostics -- Gencore::run's loop body with output_step=1.  The default
+workload is config 3: time-dependent SASE over 4096 slices with a Gaussian
+current (peak 4 kA), 2220..16384 macroparticles per slice (40.1 M), 255x255
+field, long- and short-range space charge, run in the mixed FP32 form and,
+on the same loaded beam, in the FP64 plain form.  Beam and field are
+resident in HBM when the timed region starts.  Both forms run the whole
+lattice (509 steps); the line reports the mixed form's throughput, its
+ratio to the FP64 form's and its P(z), |b(z)|, final-field and spectrum
+errors against the FP64 form.
 
 Prints one JSON line.  The CPU baseline is the FP64 oracle (a clean-room
 C++ restatement of the reference's loops, OpenMP over slices like the
@@ -30,93 +35,49 @@ import g4mi_workloads as W  # noqa: E402
 HBM_PEAK_GBS = 8000.0
 
 
-def build_workload(cfg):
-    ns = cfg["nslice"]
-    counts, currents = W.counts_and_currents(cfg)
-    off = np.zeros(ns + 1, dtype=np.int64)
-    off[1:] = np.cumsum(counts)
-    slices = [W.slice_beam(current=c) for c in currents]
-    beam = g4mi.load_beam(off, slices, cfg["nbins"], cfg["shotnoise"], cfg["seed"], W.LAMBDA_R,
-                          1.0, cfg["timerun"])
-    steps = g4mi.parse_lattice(W.lattice_text(), "FEL", W.DELZ, W.LAMBDA_R, W.GAMMA0)
-    ng = cfg["ngrid"]
-    field = np.zeros((ns, ng * ng, 2))
-    if cfg["seed_power"] > 0:
-        f0 = g4mi.load_gauss(ng, cfg["gridmax"], W.LAMBDA_R, cfg["seed_power"], 30e-6)
-        field[:] = f0
-    current = np.asarray(currents, dtype=np.float64)
-    return off, beam, field, current, steps
+build_workload = W.build_workload
+make_context = W.make_context
 
 
-def make_context(cfg, off, beam, field, current, precision):
-    ctx = g4mi.Context(off, cfg["ngrid"], cfg["gridmax"], W.LAMBDA_R, 1.0, cfg["timerun"],
-                       False, precision)
-    ctx.set_efield(cfg["longrange"], cfg["sc_nz"], cfg["sc_nphi"], cfg["sc_ngrid"], 0.0)
-    ctx.upload_beam(beam, current)
-    ctx.upload_field(field, 0)
-    return ctx
-
-
-def gpu_measure(cfg, steps_k, warmup, adi_kernel=-1, precision=g4mi.FP64, breakdown=20, refine=0,
-                mixed_adi=1, fft=False):
+def gpu_measure(cfg, steps_k, warmup, adi_kernel=-1, precision=g4mi.FP64, breakdown=20, refine=None,
+                mixed_adi=None, fft=False, work=None, whole_lattice=False):
+    """Times `steps_k` steps after `warmup` on a context of `cfg` in the
+    given precision.  With whole_lattice the rest of the lattice runs after
+    the timed steps (untimed) so the records and the final field cover the
+    whole undulator line."""
     t_load = time.perf_counter()
-    off, beam, field, current, steps = build_workload(cfg)
+    off, beam, field, current, steps = work if work is not None else build_workload(cfg)
     ctx = make_context(cfg, off, beam, field, current, precision)
     t_load = time.perf_counter() - t_load
-    del beam, field
     if fft:
         ctx.set_field_solver(fft=True)
     if adi_kernel >= 0 and precision == g4mi.FP64:
         ctx.set_schedule("adi_kernel", adi_kernel)
     if precision == g4mi.FP32:
-        ctx.set_schedule("adi_refine", refine)
-        ctx.set_schedule("mixed_adi", mixed_adi)
+        if refine is not None:
+            ctx.set_schedule("adi_refine", refine)
+        if mixed_adi is not None:
+            ctx.set_schedule("mixed_adi", mixed_adi)
     nst = len(steps)
     extra = breakdown if precision == g4mi.FP32 else 0
     if warmup + steps_k + extra > nst:
         raise SystemExit(f"warmup+steps must be <= {nst - extra} (lattice length)")
-    warm = (g4mi.StepParams * warmup)(*steps[:warmup])
-    timed = (g4mi.StepParams * steps_k)(*steps[warmup:warmup + steps_k])
-    rec_keys = ("power", "bunching")
-    ns = cfg["nslice"]
-    bufs = {k: np.zeros((steps_k, ns)) for k in rec_keys}
-    rows = (g4mi.DiagOut * steps_k)()
-    for i in range(steps_k):
-        rows[i] = g4mi.DiagOut(*[bufs[k][i].ctypes.data_as(g4mi._dp) if k in bufs else None
-                                 for k in g4mi.DIAG_FIELDS])
-    L = g4mi.lib()
-    if warmup:
-        g4mi._chk(L.g4mi_run(ctx.h, warm, warmup, W.GAMMA0, None), "warmup")
-    ctx.sync()
-    # FP64: kernel groups timed by HIP events inside the timed region.  FP32:
-    # the timed region replays the captured step graphs; the kernel groups
-    # are then timed on `breakdown` further steps launched directly.
-    if precision == g4mi.FP64:
-        ctx.set_timing(True)
-    ctx.sync()
-    t0 = time.perf_counter()
-    done = g4mi._chk(L.g4mi_run(ctx.h, timed, steps_k, W.GAMMA0, rows), "timed run")
-    ctx.sync()
-    t1 = time.perf_counter()
-    if precision == g4mi.FP32:
-        more = (g4mi.StepParams * extra)(*steps[warmup + steps_k:warmup + steps_k + extra])
-        ctx.set_timing(True)
-        g4mi._chk(L.g4mi_run(ctx.h, more, extra, W.GAMMA0, None), "breakdown run")
-        ctx.sync()
-    kt = ctx.kernel_times()
-    ctx.set_timing(False)
+    run_steps = steps if whole_lattice else steps[:warmup + steps_k + extra]
+    out = W.run_lattice(ctx, run_steps, warmup=warmup, timed=steps_k, breakdown=extra)
     sched = ctx.schedule()
-    assert done == steps_k
+    timed = steps[warmup:warmup + steps_k]
     dz = sum(s.dz for s in timed)
     ntot = int(off[-1])
+    f, first = ctx.download_field() if whole_lattice else (None, None)
     ctx.close()
-    return dict(seconds=t1 - t0, ntot=ntot, dz=dz, kt=kt, power=bufs["power"],
-                bunching=bufs["bunching"], schedule=sched, load_s=t_load)
+    return dict(seconds=out["seconds"], ntot=ntot, dz=dz, kt=out["kt"], recs=out["recs"],
+                schedule=sched, load_s=t_load, field=f, first=first)
 
 
 # device kernel behind each timed kernel group (the traffic table names kernels)
 KERNEL_OF = {g4mi.FP64: {"adi": "k_adi_band", "push": "k_push", "transverse": "k_transverse",
                          "deposit": "k_deposit"},
+             "fp64_fused": {"adi": "k_adi_band", "push": "k_fp64_main", "transverse": "k_fp64_front"},
              g4mi.FP32: {"adi": "k_mix_adi_stream", "push": "k_mix_main", "transverse": "k_mix_front"}}
 
 
@@ -163,7 +124,12 @@ def roofline(cfg, kt, ntot, precision=g4mi.FP64, schedule=None):
                            frac=round(tf / FP64_MFMA_PEAK_TFLOPS, 4), traffic=None, kernel="field (fft)",
                            device_kernel="k_cgemm", ms_per_launch=round(ms, 5),
                            algorithmic_flops=int(flops))
-    if precision == g4mi.FP64:
+    fused = precision == g4mi.FP64 and bool(schedule and schedule.get("fp64_fused"))
+    if fused:
+        # front: read gamma, x, y, px, py + write x, y, px, py = 72 B/particle;
+        # main: read and write all six coordinates = 96 B/particle
+        per = {"push": 96.0 * ntot, "adi": 32.0 * cells, "transverse": 72.0 * ntot}
+    elif precision == g4mi.FP64:
         per = {"push": 64.0 * ntot, "adi": 32.0 * cells, "deposit": 32.0 * ntot,
                "transverse": 144.0 * ntot}
     else:
@@ -171,9 +137,12 @@ def roofline(cfg, kt, ntot, precision=g4mi.FP64, schedule=None):
     name = max(per, key=lambda k: kt[k])
     ms = kt[name]
     gbs = per[name] / (ms * 1e-3) / 1e9
-    dev = KERNEL_OF[precision].get(name)
-    if name == "adi" and precision == g4mi.FP64 and schedule and schedule.get("adi") != "band":
-        dev = None  # the traffic table holds the banded kernel only
+    dev = KERNEL_OF["fp64_fused" if fused else precision].get(name)
+    if name == "adi" and precision == g4mi.FP64 and schedule:
+        # the ADI kernel the tuner chose for the undulator step (the timed steps)
+        und = (schedule.get("adi_by_delz") or {}).get(f"{W.DELZ:.6g}", schedule.get("adi"))
+        if und != "band":
+            dev = None  # the traffic table holds the banded kernel only
     traffic = measured_traffic(cfg["name"], dev) if dev else None
     return name, dict(bound="hbm", achieved=round(gbs, 1), peak=HBM_PEAK_GBS, unit="GB/s",
                       frac=round(gbs / HBM_PEAK_GBS, 4),
@@ -182,31 +151,71 @@ def roofline(cfg, kt, ntot, precision=g4mi.FP64, schedule=None):
                       algorithmic_bytes=int(per[name]))
 
 
-def cpu_baseline(cfg, nsteps):
-    """The FP64 oracle on this host's cores over the first nsteps steps of
-    the same workload (all slices)."""
+def _cpu_model():
+    """CPU model name and physical core count of this host (/proc/cpuinfo,
+    as lscpu reports them)."""
+    model, cores = "unknown", set()
+    phys = core = None
+    try:
+        for line in open("/proc/cpuinfo"):
+            k, _, v = line.partition(":")
+            k, v = k.strip(), v.strip()
+            if k == "model name" and model == "unknown":
+                model = v
+            elif k == "physical id":
+                phys = v
+            elif k == "core id":
+                core = v
+            elif not k and phys is not None:
+                cores.add((phys, core))
+                phys = core = None
+        if phys is not None:
+            cores.add((phys, core))
+    except OSError:
+        pass
+    return model, len(cores) or None
+
+
+def cpu_baseline(cfg, work, nsteps=2, nslices=(512, 32)):
+    """The FP64 oracle on this host's CPU over the first `nsteps` steps of a
+    block of the workload's central slices (the same loaded particles),
+    once with the threads of this GPU's CPU share and once on one core."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import ctypes as C
     import oracle_py as O
-    threads = min(16, os.cpu_count() or 1)
+    off, beam, _, current, _ = work
     ns = cfg["nslice"]
-    counts, currents = W.counts_and_currents(cfg)
-    sim = O.Sim(counts, cfg["ngrid"], cfg["gridmax"], W.LAMBDA_R, timerun=cfg["timerun"],
-                sc=(cfg["sc_nz"], cfg["sc_nphi"], cfg["sc_ngrid"], 0.0),
-                longrange=cfg["longrange"], threads=threads)
-    O.load_beam(sim, [W.slice_beam(current=c) for c in currents], None, cfg["nbins"],
-                cfg["seed"] if cfg["shotnoise"] and cfg["timerun"] else None, lambda0=W.LAMBDA_R)
+    share = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or min(16, os.cpu_count() or 1)
     steps = O.unroll(W.fodo_elements(), W.DELZ, W.LAMBDA_R, W.GAMMA0)
-    t0 = time.perf_counter()
-    for i in range(nsteps):
-        O.lib().orc_step_all(C.byref(sim.s), C.byref(steps[i]), W.GAMMA0)
-        sim.diag()
-    dt = time.perf_counter() - t0
-    ntot = int(sum(counts))
-    return dict(value=round(ntot * nsteps / dt, 1), unit="macroparticle-steps/s",
-                cores=threads, kind="port",
-                sample=f"{nsteps} steps of {cfg['name']} in FP64 (all {ns} slices, {ntot} "
-                       f"macroparticles), {dt:.2f} s on {threads} threads")
+    model, phys = _cpu_model()
+    res = []
+    for nsl, threads in ((min(ns, nslices[0]), share), (min(ns, nslices[1]), 1)):
+        i0 = (ns - nsl) // 2
+        a, b = int(off[i0]), int(off[i0 + nsl])
+        counts = np.diff(off[i0:i0 + nsl + 1])
+        sim = O.Sim(counts, cfg["ngrid"], cfg["gridmax"], W.LAMBDA_R, timerun=cfg["timerun"],
+                    sc=(cfg["sc_nz"], cfg["sc_nphi"], cfg["sc_ngrid"], 0.0),
+                    longrange=cfg["longrange"], threads=threads)
+        for k in ("gamma", "theta", "x", "y", "px", "py"):
+            getattr(sim, k)[:] = beam[k][a:b]
+        sim.current[:] = current[i0:i0 + nsl]
+        t0 = time.perf_counter()
+        for i in range(nsteps):
+            O.lib().orc_step_all(C.byref(sim.s), C.byref(steps[i]), W.GAMMA0)
+            sim.diag()
+        dt = time.perf_counter() - t0
+        res.append(dict(value=round((b - a) * nsteps / dt, 1), threads=threads, slices=nsl,
+                        macroparticles=b - a, seconds=round(dt, 2)))
+    multi, single = res
+    return dict(value=multi["value"], unit="macroparticle-steps/s", cores=multi["threads"],
+                kind="port", cpu_model=model, host_physical_cores=phys,
+                single_core=dict(value=single["value"], cores=1,
+                                 sample=f"{single['slices']} central slices, {single['macroparticles']} "
+                                        f"macroparticles, {nsteps} steps, {single['seconds']} s"),
+                sample=f"the first {nsteps} steps of {cfg['name']} (FP64 oracle, OpenMP over slices) on "
+                       f"its {multi['slices']} central slices ({multi['macroparticles']} of the workload's "
+                       f"loaded macroparticles, {cfg['ngrid']}^2 field), {multi['seconds']} s on "
+                       f"{multi['threads']} threads: the CPU share of one GPU on this host")
 
 
 def fno_probe(ngrid=151, batch=16, reps=5):
@@ -397,16 +406,30 @@ def copy_probe(mib):
     return 0
 
 
+def _line(cfg, m, steps, warmup, prec):
+    """Rate, per-kernel times and roofline of one measured form."""
+    rate = m["ntot"] * steps / m["seconds"]
+    cells = cfg["nslice"] * cfg["ngrid"] ** 2
+    _, roof = roofline(cfg, m["kt"], m["ntot"], prec, m["schedule"])
+    return dict(value=round(rate, 1), ms_per_step=round(1e3 * m["seconds"] / steps, 4),
+                field_cell_updates_per_s=round(cells * steps / m["seconds"], 1),
+                s_per_undulator_metre=round(m["seconds"] / m["dz"], 5),
+                kernel_ms_per_step={k: round(v, 5) for k, v in m["kt"].items() if k != "steps"},
+                roofline=roof, schedule=m["schedule"])
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=100)
-    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=20,
+                    help="untimed steps first; the mixed form's schedule tuner takes the first 14 steps")
     ap.add_argument("--config", choices=("config1", "config0", "config2", "config3", "config4"),
-                    default="config1",
-                    help="config4 runs the learned field step: capture, training, deployment")
+                    default="config3",
+                    help="config3 (default) and config2 run the mixed form and, on the same beam, "
+                         "the FP64 form over the whole lattice; config4 runs the learned field step")
     ap.add_argument("--precision", choices=("fp64", "fp32"), default=None,
-                    help="override the configuration's precision")
+                    help="override the configuration's precision (one form only)")
     ap.add_argument("--cpu-steps", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--adi-kernel", type=int, default=-1,
@@ -414,10 +437,10 @@ def main():
                          "default: autotuned")
     ap.add_argument("--field-solver", choices=("adi", "fft"), default="adi",
                     help="FP64 form: the ADI (default) or the spectral field solver")
-    ap.add_argument("--adi-refine", type=int, default=0, choices=(0, 1),
-                    help="mixed form: FP64 residual correction in the ADI (default off)")
-    ap.add_argument("--mixed-adi", type=int, default=1, choices=(0, 1),
-                    help="mixed form: 0 tile ADI kernel, 1 streaming ADI kernel")
+    ap.add_argument("--adi-refine", type=int, default=None, choices=(0, 1),
+                    help="mixed form: FP64 residual correction in the ADI (default: the library's)")
+    ap.add_argument("--mixed-adi", type=int, default=None, choices=(0, 1),
+                    help="mixed form: 0 tile ADI kernel, 1 streaming ADI kernel (default: the library's)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB five times (the reference "
                          "point that scales the FETCH_SIZE/WRITE_SIZE counters) and exit")
@@ -431,51 +454,78 @@ def main():
     ap.add_argument("--fno-train-batch", type=int, default=16)
     ap.add_argument("--fno-deploy-steps", type=int, default=0, help="0: the whole lattice")
     a = ap.parse_args()
-    if a.copy_probe:
-        return copy_probe(a.copy_probe)
-    if a.fno_probe:
-        return fno_probe()
-    if a.config == "config4" and a.gpus == 1:
-        return config4(a)
     if a.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; the slice-block sharding "
                                    "across GPUs is a design, not an implementation",
                           "n_gpus": a.gpus}))
         return 2
+    if a.copy_probe:
+        return copy_probe(a.copy_probe)
+    if a.fno_probe:
+        return fno_probe()
+    if a.config == "config4":
+        return config4(a)
     cfg = dict(W.CONFIGS[a.config])
+    compare = a.precision is None and cfg["precision"] == "fp32"
     if a.precision:
         cfg["precision"] = a.precision
     prec = g4mi.FP32 if cfg["precision"] == "fp32" else g4mi.FP64
-    m = gpu_measure(cfg, a.steps, a.warmup, a.adi_kernel, prec, refine=a.adi_refine,
-                    mixed_adi=a.mixed_adi, fft=a.field_solver == "fft")
-    rate = m["ntot"] * a.steps / m["seconds"]
-    cells = cfg["nslice"] * cfg["ngrid"] ** 2
-    kname, roof = roofline(cfg, m["kt"], m["ntot"], prec, m["schedule"])
+    log = lambda msg: print(msg, file=sys.stderr, flush=True)  # noqa: E731
+    t0 = time.perf_counter()
+    work = build_workload(cfg)
+    log(f"workload {cfg['name']}: {int(work[0][-1])} macroparticles loaded in "
+        f"{time.perf_counter() - t0:.1f} s")
+    kw = dict(adi_kernel=a.adi_kernel, refine=a.adi_refine, mixed_adi=a.mixed_adi,
+              fft=a.field_solver == "fft", work=work, whole_lattice=compare)
+    ref = None
+    if compare:
+        ref = gpu_measure(cfg, a.steps, a.warmup, precision=g4mi.FP64, **kw)
+        log(f"FP64 form: {1e3 * ref['seconds'] / a.steps:.3f} ms per step")
+    m = gpu_measure(cfg, a.steps, a.warmup, precision=prec, **kw)
+    log(f"{cfg['precision']} form: {1e3 * m['seconds'] / a.steps:.3f} ms per step")
+    main_line = _line(cfg, m, a.steps, a.warmup, prec)
+    if prec == g4mi.FP64:
+        dtype = "f64"
+    else:
+        refine = m["schedule"].get("adi_refine", 0)
+        dtype = "f32 (f64 slice phase base, f64 sums" + (", f64 ADI residual correction)" if refine else ")")
     out = {
         "metric": "macroparticle-steps/s",
-        "value": round(rate, 1),
+        "value": main_line["value"],
         "unit": "macroparticle-steps/s",
         "n_gpus": 1,
         "steps": a.steps,
         "warmup": a.warmup,
-        "ms_per_step": round(1e3 * m["seconds"] / a.steps, 4),
+        "ms_per_step": main_line["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64" if prec == g4mi.FP64 else "f32 (f64 phase base, ADI correction, sums)",
+        "dtype": dtype,
         "data": "synthetic: Hammersley quiet loading + shot noise (seed 7), as LoadBeam",
-        "config": {"workload": W.describe(cfg), "slices": cfg["nslice"],
-                   "macroparticles": m["ntot"], "ngrid": cfg["ngrid"],
+        "config": {"workload": W.describe(cfg) +
+                   ("" if prec == g4mi.FP64 else
+                    (", FP64 ADI residual correction" if m["schedule"].get("adi_refine", 0)
+                     else ", no FP64 ADI residual correction")),
+                   "slices": cfg["nslice"], "macroparticles": m["ntot"], "ngrid": cfg["ngrid"],
                    "precision": cfg["precision"], "parallelism": "single GPU"},
-        "field_cell_updates_per_s": round(cells * a.steps / m["seconds"], 1),
-        "s_per_undulator_metre": round(m["seconds"] / m["dz"], 5),
-        "kernel_ms_per_step": {k: round(v, 5) for k, v in m["kt"].items() if k != "steps"},
-        "roofline": roof,
-        "schedule": m["schedule"],
+        "field_cell_updates_per_s": main_line["field_cell_updates_per_s"],
+        "s_per_undulator_metre": main_line["s_per_undulator_metre"],
+        "kernel_ms_per_step": main_line["kernel_ms_per_step"],
+        "roofline": main_line["roofline"],
+        "schedule": main_line["schedule"],
         "setup_s": round(m["load_s"], 2),
     }
+    if ref is not None:
+        fl = _line(cfg, ref, a.steps, a.warmup, g4mi.FP64)
+        out["vs_fp64"] = round(main_line["value"] / fl["value"], 3)
+        out["fp64_form"] = fl
+        e = W.form_errors(ref["recs"], m["recs"], ref["field"], ref["first"], m["field"], m["first"])
+        out["errors_vs_fp64"] = dict({k: (round(v, 8) if isinstance(v, float) else v) for k, v in e.items()},
+                                     span=f"all {e['steps']} steps of the lattice (timed steps "
+                                          f"{a.warmup}..{a.warmup + a.steps - 1} inside it)",
+                                     bar=0.01)
     if not a.no_cpu_baseline:
-        out["cpu_baseline"] = cpu_baseline(cfg, a.cpu_steps)
+        out["cpu_baseline"] = cpu_baseline(cfg, work, a.cpu_steps)
     print(json.dumps(out), flush=True)
     return 0
 

@@ -76,6 +76,8 @@ struct Ctx {
     // 0 two-kernel Thomas, 1 fused recurrence, 2 fused recurrence with
     // prefetch, 3 banded inverse
     int adi_kernel = -1;
+    bool adi_fixed = false;               // set by g4mi_set_schedule("adi_kernel", k >= 0)
+    std::map<double, int> adi_choice;     // the tuner's kernel per step size
     float adi_ms[4] = {-1.f, -1.f, -1.f, -1.f};
     // spectral solver (FieldSolverFFT) instead of the ADI, with the source
     // filter's settings; the propagator matrix is cached per delz
@@ -124,6 +126,11 @@ struct Ctx {
     std::vector<PendingRecord> pend;
     // one4one beams: Beam::sort at sort markers, with its work buffers
     bool one4one = false;
+    // the step's particle passes as two fused kernels (schedule "fp64_fused")
+    bool fuse64 = true;
+    double *d_beamsum = nullptr;           // beam sums of the last fused step [8 ns]
+    int4 *d_pbox = nullptr;                // in-grid footprint per slice after the first half step
+    bool beamsum_fresh = false;            // d_beamsum describes the current particles
     SortWork sw{};
     long long *d_off_new = nullptr;
     double wake_dz = 0;
@@ -156,10 +163,20 @@ struct Ctx {
     std::vector<MixBand> h_binfo;
     std::map<double, int> band_of_dz;      // step size -> band index
     std::vector<int> band_tmpl;            // template half width of each band
-    bool m_refine = false;                 // FP64 residual correction in the ADI (schedule "adi_refine")
+    bool m_refine = true;                  // FP64 residual correction in the ADI (schedule "adi_refine")
     bool m_graphs = true;                  // replay captured step graphs
     bool m_stream = true;                  // streaming ADI (schedule "mixed_adi" 1) or the tile kernel (0)
-    std::map<std::tuple<int, int, int, int, int, int>, hipGraphExec_t> m_graph;
+    int m_main_nt = 384;                   // threads per slice of the main kernel ("main_nt")
+    int m_main_lds = 0;                    // its LDS region: 0 = 68 KiB, 1 = 44 KiB ("main_lds")
+    bool m_persistent = false;             // the whole step as one cooperative launch ("persistent")
+    bool m_autotune = true;                // time the schedules on the first steps of a run
+    bool m_tuned = false;
+    std::string m_tune_log;                // mean ms per candidate, for the schedule report
+    int m_tune_i = 0;                      // tuning steps done
+    double m_tune_main[6] = {0, 0, 0, 0, 0, 0}, m_tune_adi[2] = {0, 0}, m_tune_graph = -1;
+    int m_tune_nmain[6] = {0, 0, 0, 0, 0, 0}, m_tune_nadi[2] = {0, 0};
+    hipEvent_t m_tune_ev[9] = {};
+    std::map<std::tuple<int, int, int, int, int, int, int, int>, hipGraphExec_t> m_graph;
     std::vector<double> m_rec_host;
     std::vector<double> h_cur;             // currents [ns] (host copy)
     double *d_aux = nullptr;               // auxiliary diagnostics [16 ns]
@@ -305,7 +322,11 @@ static int ensure_factors(Ctx *c, double delz) {
             break;
         }
     }
-    if (c->adi_kernel == 1 || c->adi_kernel == 2) {
+    if (!c->adi_fixed) {
+        // the tuner's choice for this step size, or tune it at the next field step
+        auto it = c->adi_choice.find(delz);
+        c->adi_kernel = it != c->adi_choice.end() ? it->second : -1;
+    } else if (c->adi_kernel == 1 || c->adi_kernel == 2) {
         if (!c->adi_tile) c->adi_kernel = -1;      // re-tune if no longer eligible
     } else if (c->adi_kernel == 3 && !c->adi_band_m) {
         c->adi_kernel = -1;
@@ -488,6 +509,7 @@ static int tune_adi(Ctx *c, const AdiFactors &f) {
     (void)hipEventDestroy(e1);
     G4_HIP(hipFree(scratch));
     c->adi_kernel = best_kind;
+    c->adi_choice[c->adi_delz] = best_kind;
     return G4MI_OK;
 }
 
@@ -631,7 +653,7 @@ static int harmonics_track(Ctx *c, const g4mi_step_params &st, double delz) {
     return G4MI_OK;
 }
 
-static int field_track(Ctx *c, const g4mi_step_params &st, double delz) {
+static int field_track(Ctx *c, const g4mi_step_params &st, double delz, bool deposit = true) {
     if (!c->has_field) {
         set_error("field not uploaded");
         return G4MI_E_STATE;
@@ -654,8 +676,9 @@ static int field_track(Ctx *c, const g4mi_step_params &st, double delz) {
     d.gradx = st.gradx;
     d.grady = st.grady;
     d.ngrid = c->ng;
-    launch_deposit(c->stream, c->ns, c->d_off, c->d_g, c->d_th, c->d_x, c->d_y, c->d_cur,
-                   c->d_src, c->d_box, d);
+    if (deposit)
+        launch_deposit(c->stream, c->ns, c->d_off, c->d_g, c->d_th, c->d_x, c->d_y, c->d_cur,
+                       c->d_src, c->d_box, d);
     if (c->ev) G4_HIP(hipEventRecord(c->ev[6], c->stream));
     if (c->fft) {
         G4_TRY(fft_field_step(c, delz));
@@ -1054,12 +1077,114 @@ static int sort_beam(Ctx *c) {
     return G4MI_OK;
 }
 
+// Whether a step can run as the two fused particle kernels: fundamental
+// field only, no energy kicks (incoherent SR, wakes) between the push and
+// R56, no sort between the beam and the field track, and every space-charge
+// mode solution within the main kernel's LDS.
+static bool fp64_fusable(const Ctx *c, const g4mi_step_params &st) {
+    if (!c->fuse64 || !c->hf.empty() || c->d_wake) return false;
+    if (c->incoh && st.aw > 0 && (c->incoh_loss || c->incoh_spread)) return false;
+    if ((st.marker & 4) && c->one4one) return false;
+    const bool sc = (c->sc_nz > 0) && (c->sc_ng > 2);
+    if (sc && (c->sc_ng > 256 || (2 * c->sc_nphi + 1) * c->sc_nz * c->sc_ng > fp64_main_modes())) return false;
+    return true;
+}
+
+// Beam::track + Field::track of one step with the fused kernels: front
+// (first half step, moments, radius, footprint), the long-range field and
+// the radial scan, main (short range, push, R56, second half step, beam
+// sums, deposit), then the field solve.
+static int fused_step(Ctx *c, const g4mi_step_params &st, double gamma0, double delz) {
+    if (!c->has_beam) {
+        set_error("beam not uploaded");
+        return G4MI_E_STATE;
+    }
+    const double aw = st.aw;
+    const bool sc = (c->sc_nz > 0) && (c->sc_ng > 2);
+    const bool moments = c->longrange || sc;
+    TransParams t1 = transverse_params(c, st, gamma0, delz * 0.5, false);
+    if (c->ev) G4_HIP(hipEventRecord(c->ev[0], c->stream));
+    launch_fp64_front(c->stream, c->ns, c->d_off, c->d_g, c->d_x, c->d_y, c->d_px, c->d_py, c->d_cur, t1, moments,
+                      sc, c->gridmax, c->dgrid, c->ng, c->d_mom, c->d_fsize, c->d_cen, c->d_rbound, c->d_pbox);
+    if (c->ev) G4_HIP(hipEventRecord(c->ev[1], c->stream));
+    if (c->longrange) {
+        const double gamma = gamma0 / sqrt(1 + aw * aw);
+        const double scl = c->slicelength / 2. / asin(1) / 299792458.0 / 2 / 8.85e-12;
+        launch_lsc_long(c->stream, c->ns, c->d_cur, c->d_fsize, c->slicelength, gamma, scl, c->d_esc);
+    }
+    if (sc) launch_sc_scan(c->stream, c->ns, c->d_off, c->d_rbound, c->d_rmax, c->sc_ng, c->d_dr);
+    if (c->ev) G4_HIP(hipEventRecord(c->ev[2], c->stream));
+    FuseParams f{};
+    PushParams &p = f.p;
+    p.delz = delz;
+    p.xks = c->xks;
+    p.rtmp = coupling(st, 1) / c->xks;
+    p.xku = st.ku;
+    if (p.xku == 0) p.xku = c->xks * 0.5 / gamma0 / gamma0;
+    p.aw = aw;
+    p.autophase = st.phaseshift;
+    p.ax = st.ax;
+    p.ay = st.ay;
+    p.kx = st.kx;
+    p.ky = st.ky;
+    p.gradx = st.gradx;
+    p.grady = st.grady;
+    p.gridmax = c->gridmax;
+    p.dgrid = c->dgrid;
+    p.ngrid = c->ng;
+    p.first = c->first;
+    p.nslice = c->ns;
+    p.use_sc = sc ? 1 : 0;
+    p.nh = 0;
+    f.t2 = transverse_params(c, st, gamma0, delz * 0.5, true);
+    DepositParams &d = f.d;
+    d.enabled = st.aw > 0 ? 1 : 0;  // in undulator, fundamental
+    d.harm = 1.0;
+    d.fcoup = coupling(st, 1);
+    d.xks = c->xks;
+    d.delz = delz;
+    d.dgrid = c->dgrid;
+    d.gridmax = c->gridmax;
+    d.ax = st.ax;
+    d.ay = st.ay;
+    d.kx = st.kx;
+    d.ky = st.ky;
+    d.gradx = st.gradx;
+    d.grady = st.grady;
+    d.ngrid = c->ng;
+    f.chicane = st.chic_angle != 0 ? 1 : 0;
+    if (f.chicane) f.r56 = chicane_r56(st.chic_angle, st.chic_lb, st.chic_ld, c->lambda0, gamma0);
+    f.gamma0 = gamma0;
+    f.use_sc = sc ? 1 : 0;
+    f.ngr = c->sc_ng;
+    f.nz = c->sc_nz;
+    f.nphi = c->sc_nphi;
+    f.ks = 4 * asin(1) / c->lambda0;
+    f.gz2 = gamma0 * gamma0 / (1 + aw * aw);
+    launch_fp64_main(c->stream, c->ns, c->d_off, c->d_g, c->d_th, c->d_x, c->d_y, c->d_px, c->d_py, c->d_field,
+                     c->d_esc, c->d_cur, c->d_cen, c->d_dr, c->d_efield, c->d_pbox, c->d_src, c->d_box,
+                     c->d_beamsum, f);
+    if (c->ev) {
+        G4_HIP(hipEventRecord(c->ev[3], c->stream));
+        G4_HIP(hipEventRecord(c->ev[4], c->stream));
+    }
+    G4_HIP(hipGetLastError());
+    G4_TRY(field_track(c, st, delz, false));
+    c->beamsum_fresh = true;
+    return G4MI_OK;
+}
+
 static int step(Ctx *c, const g4mi_step_params &st, double gamma0) {
     const double delz = st.dz;
+    c->beamsum_fresh = false;
     G4_TRY(step_begin_timing(c));
-    G4_TRY(beam_track(c, st, gamma0, delz));
-    if ((st.marker & 4) && c->one4one) G4_TRY(sort_beam(c));  // Gencore.cpp:124-134
-    G4_TRY(field_track(c, st, delz));
+    if (fp64_fusable(c, st)) {
+        G4_TRY(fused_step(c, st, gamma0, delz));
+    } else {
+        G4_TRY(beam_track(c, st, gamma0, delz));
+        if ((st.marker & 4) && c->one4one) G4_TRY(sort_beam(c));  // Gencore.cpp:124-134
+        G4_TRY(field_track(c, st, delz));
+    }
     G4_TRY(slippage(c, st.slip));
     return G4MI_OK;
 }
@@ -1247,12 +1372,14 @@ static int mix_fill_step(Ctx *c, const g4mi_step_params &st, double gamma0, MixS
     return G4MI_OK;
 }
 
-typedef std::tuple<int, int, int, int, int, int> MixKey;  // band, refine, sc, moments, maxzero, stream
+// band, refine, sc, moments, maxzero, stream, main threads per slice, main LDS region
+typedef std::tuple<int, int, int, int, int, int, int, int> MixKey;
 
 static int mix_launch(Ctx *c, const MixKey &k, hipEvent_t *ev) {
     const MixArrays a = mix_arrays(c);
     if (launch_mix_step(c->stream, a, std::get<0>(k), std::get<1>(k) != 0, std::get<2>(k) != 0,
-                        std::get<3>(k) != 0, std::get<4>(k), ev, std::get<5>(k) != 0) != 0) {
+                        std::get<3>(k) != 0, std::get<4>(k), ev, std::get<5>(k) != 0, std::get<6>(k),
+                        std::get<7>(k)) != 0) {
         set_error("mixed form: no ADI instantiation for this band width");
         return G4MI_E_ARG;
     }
@@ -1311,7 +1438,7 @@ static int mix_step_fno(Ctx *c, const MixKey &k) {
     int ntiles = 0;
     MixState hs{};
     if (c->fno_mode == 1) {
-        launch_mix_beam(c->stream, a, sc, moments, nullptr, false);
+        launch_mix_beam(c->stream, a, sc, moments, nullptr, false, std::get<6>(k), std::get<7>(k));
         if (launch_mix_adi(c->stream, a, tm, refine, stream, &ntiles)) {
             set_error("mixed form: no ADI instantiation for this band width");
             return G4MI_E_ARG;
@@ -1336,7 +1463,7 @@ static int mix_step_fno(Ctx *c, const MixKey &k) {
     std::vector<double> hf0(8 * (size_t)ns), hf1(8 * (size_t)ns), eg(ns), ss(ns);
     launch_mix_field_sums(c->stream, a, 0, c->d_diag_f);
     G4_HIP(hipMemcpyAsync(hf0.data(), c->d_diag_f, hf0.size() * 8, hipMemcpyDeviceToHost, c->stream));
-    launch_mix_beam(c->stream, a, sc, moments, nullptr, true);
+    launch_mix_beam(c->stream, a, sc, moments, nullptr, true, std::get<6>(k), std::get<7>(k));
     G4_HIP(hipMemcpyAsync(&hs, c->m_st, sizeof(MixState), hipMemcpyDeviceToHost, c->stream));
     G4_HIP(hipStreamSynchronize(c->stream));
     G4_TRY(fno_infer_slices(c->fno, c->stream, c->m_field[hs.cur], c->m_field[hs.cur ^ 1], c->m_src, c->d_box, ns,
@@ -1417,6 +1544,98 @@ static int mix_step_fno(Ctx *c, const MixKey &k) {
     return G4MI_OK;
 }
 
+// One step of the mixed-form schedule tuner (see mix_run): tuning step k
+// runs main candidate k % 6 with ADI candidate (k / 6 + k) % 2 (k < 12), the
+// graph of the chosen kernels (k = 12) or the persistent launch (k = 13).
+constexpr int kMixTuneSteps = 14;
+static const int kMainCand[6][2] = {{384, 0}, {256, 0}, {512, 0}, {384, 1}, {256, 1}, {512, 1}};
+
+static int mix_tune_step(Ctx *c, const MixKey &key) {
+    hipEvent_t *tev = c->m_tune_ev;
+    const int k = c->m_tune_i;
+    if (k == 0)
+        for (int e = 0; e < 9; e++) G4_HIP(hipEventCreate(&tev[e]));
+    MixKey kk = key;
+    float ms = 0;
+    if (k < 12) {
+        const int mc = k % 6, ac = (k / 6 + k) % 2;   // every main candidate twice, both ADIs six times
+        const int tm = std::get<0>(kk);
+        const bool tile_ok = !c->m_refine || tm <= 9;
+        const int stream = (ac == 1 && tile_ok) ? 0 : 1;
+        std::get<1>(kk) = c->m_refine ? 1 : 0;
+        std::get<5>(kk) = stream;
+        std::get<6>(kk) = kMainCand[mc][0];
+        std::get<7>(kk) = kMainCand[mc][1];
+        G4_TRY(mix_launch(c, kk, tev));
+        G4_HIP(hipEventSynchronize(tev[8]));
+        float tm_main = 0, tm_adi = 0;
+        G4_HIP(hipEventElapsedTime(&tm_main, tev[2], tev[3]));
+        G4_HIP(hipEventElapsedTime(&tm_adi, tev[6], tev[7]));
+        c->m_tune_main[mc] += tm_main;
+        c->m_tune_nmain[mc]++;
+        c->m_tune_adi[stream ? 0 : 1] += tm_adi;
+        c->m_tune_nadi[stream ? 0 : 1]++;
+        if (k == 11) {
+            int best = 0;
+            for (int q = 1; q < 6; q++)
+                if (c->m_tune_nmain[q] &&
+                    c->m_tune_main[q] / c->m_tune_nmain[q] < c->m_tune_main[best] / c->m_tune_nmain[best])
+                    best = q;
+            c->m_main_nt = kMainCand[best][0];
+            c->m_main_lds = kMainCand[best][1];
+            c->m_stream = !(c->m_tune_nadi[1] &&
+                            c->m_tune_adi[1] / c->m_tune_nadi[1] < c->m_tune_adi[0] / c->m_tune_nadi[0]);
+        }
+    } else {
+        // the chosen kernels for this key
+        const int tm = std::get<0>(kk);
+        std::get<1>(kk) = (c->m_refine && (c->m_stream || tm <= 9)) ? 1 : 0;
+        std::get<5>(kk) = c->m_stream ? 1 : 0;
+        std::get<6>(kk) = c->m_main_nt;
+        std::get<7>(kk) = c->m_main_lds;
+        if (k == 12) {
+            hipGraphExec_t ex = nullptr;
+            G4_TRY(mix_graph(c, kk, &ex));
+            G4_HIP(hipEventRecord(tev[0], c->stream));
+            G4_HIP(hipGraphLaunch(ex, c->stream));
+            G4_HIP(hipEventRecord(tev[8], c->stream));
+            G4_HIP(hipEventSynchronize(tev[8]));
+            G4_HIP(hipEventElapsedTime(&ms, tev[0], tev[8]));
+            c->m_tune_graph = ms;
+        } else {
+            double pers = -1;
+            if (c->m_stream && launch_mix_step_persistent(c->stream, mix_arrays(c), tm, std::get<1>(kk) != 0,
+                                                          std::get<2>(kk) != 0, std::get<3>(kk) != 0,
+                                                          std::get<4>(kk), tev) == 0) {
+                G4_HIP(hipEventSynchronize(tev[8]));
+                G4_HIP(hipEventElapsedTime(&ms, tev[0], tev[8]));
+                pers = ms;
+            } else {
+                (void)hipGetLastError();  // refused launch: run the step as kernels
+                G4_TRY(mix_launch(c, kk, nullptr));
+            }
+            c->m_persistent = pers > 0 && pers < c->m_tune_graph;
+            char buf[768];
+            const double *mm = c->m_tune_main;
+            const int *nm = c->m_tune_nmain;
+            auto avg = [](double s, int n) { return n ? s / n : -1.0; };
+            snprintf(buf, sizeof(buf),
+                     "{\"main_ms\": {\"384/68K\": %.4f, \"256/68K\": %.4f, \"512/68K\": %.4f, "
+                     "\"384/44K\": %.4f, \"256/44K\": %.4f, \"512/44K\": %.4f}, "
+                     "\"adi_ms\": {\"stream\": %.4f, \"tile\": %.4f}, "
+                     "\"step_ms\": {\"graph\": %.4f, \"persistent\": %.4f}}",
+                     avg(mm[0], nm[0]), avg(mm[1], nm[1]), avg(mm[2], nm[2]), avg(mm[3], nm[3]),
+                     avg(mm[4], nm[4]), avg(mm[5], nm[5]), avg(c->m_tune_adi[0], c->m_tune_nadi[0]),
+                     avg(c->m_tune_adi[1], c->m_tune_nadi[1]), c->m_tune_graph, pers);
+            c->m_tune_log = buf;
+            c->m_tuned = true;
+            for (int e = 0; e < 9; e++) (void)hipEventDestroy(tev[e]);
+        }
+    }
+    c->m_tune_i = k + 1;
+    return G4MI_OK;
+}
+
 static int mix_run(Ctx *c, const g4mi_step_params *steps, int nsteps, double gamma0,
                    g4mi_diag_out *records, int *done) {
     *done = 0;
@@ -1447,8 +1666,10 @@ static int mix_run(Ctx *c, const g4mi_step_params *steps, int nsteps, double gam
     for (int i = 0; i < n; i++) {
         G4_TRY(mix_fill_step(c, steps[i], gamma0, tab[i]));
         const int tm = c->band_tmpl[tab[i].band];
-        const int refine = (c->m_refine && tm <= 9) ? 1 : 0;
-        keys[i] = MixKey(tm, refine, sc ? 1 : 0, moments ? 1 : 0, maxzero, c->m_stream ? 1 : 0);
+        // the tile kernel's correction needs more LDS than a CU has beyond a half width of 9
+        const int refine = (c->m_refine && (c->m_stream || tm <= 9)) ? 1 : 0;
+        keys[i] = MixKey(tm, refine, sc ? 1 : 0, moments ? 1 : 0, maxzero, c->m_stream ? 1 : 0, c->m_main_nt,
+                         c->m_main_lds);
     }
     if (n > c->m_steps_cap) {
         if (c->m_steps) (void)hipFree(c->m_steps);
@@ -1465,9 +1686,35 @@ static int mix_run(Ctx *c, const g4mi_step_params *steps, int nsteps, double gam
     G4_HIP(hipStreamSynchronize(c->stream));
     const size_t rowd = (size_t)c->ns * kMixRecord;
     int pending = 0;  // first step whose record is not yet copied back
+    // The schedule autotuner: the first kMixTuneSteps steps of a context
+    // (over as many g4mi_run calls as they take) each run one candidate of the
+    // main kernel's shape (threads per slice x LDS region) and one of the ADI
+    // kernels (streaming strips or 32 x 32 tiles), timed by HIP events; the
+    // fastest of each is kept; two more steps then time the chosen kernels as
+    // one replayed graph and the whole step as one persistent cooperative
+    // launch.  Every candidate computes the same step, so the tuning steps
+    // are steps of the run, not repeats.
     for (int i = 0; i < n; i++) {
-        if (c->fno && c->fno_mode) {
+        const bool tune = c->m_autotune && !c->m_tuned && !(c->fno && c->fno_mode) && !c->timing;
+        if (tune) {
+            G4_TRY(mix_tune_step(c, keys[i]));
+            if (c->m_tuned)
+                for (int j = i + 1; j < n; j++) {
+                    const int tmj = std::get<0>(keys[j]);
+                    std::get<1>(keys[j]) = (c->m_refine && (c->m_stream || tmj <= 9)) ? 1 : 0;
+                    std::get<5>(keys[j]) = c->m_stream ? 1 : 0;
+                    std::get<6>(keys[j]) = c->m_main_nt;
+                    std::get<7>(keys[j]) = c->m_main_lds;
+                }
+        } else if (c->fno && c->fno_mode) {
             G4_TRY(mix_step_fno(c, keys[i]));
+        } else if (c->m_persistent && !c->timing) {
+            const MixKey &k = keys[i];
+            if (launch_mix_step_persistent(c->stream, mix_arrays(c), std::get<0>(k), std::get<1>(k) != 0,
+                                           std::get<2>(k) != 0, std::get<3>(k) != 0, std::get<4>(k), nullptr)) {
+                set_error("mixed form: the persistent step launch was refused");
+                return G4MI_E_HIP;
+            }
         } else if (c->timing) {
             // timed step: direct launches with the kernel groups bracketed by events
             G4_TRY(step_begin_timing(c));
@@ -1738,7 +1985,8 @@ int g4mi_create(g4mi_ctx **out, int device, int nslice, const long long *off, in
         (rc = dalloc(c, &c->d_work, ns * cells)) || (rc = dalloc(c, &c->d_src, ns * cells)) ||
         (rc = dalloc(c, &c->d_box, ns)) || (rc = dalloc(c, &c->d_fac, 3 * (size_t)ngrid)) ||
         (rc = dalloc(c, &c->d_diag_b, 8 * ns)) || (rc = dalloc(c, &c->d_diag_f, 8 * ns)) ||
-        (rc = dalloc(c, &c->d_count, 1)))
+        (rc = dalloc(c, &c->d_count, 1)) || (rc = dalloc(c, &c->d_beamsum, 8 * ns)) ||
+        (rc = dalloc(c, &c->d_pbox, ns)))
         return fail(rc);
     if (!fp64 && (rc = mix_alloc(c))) return fail(rc);
     if (fp64 && (rc = alloc_records(c))) return fail(rc);
@@ -1773,6 +2021,8 @@ int g4mi_destroy(g4mi_ctx *h) {
     if (c->d_tilediag) (void)hipFree(c->d_tilediag);
     if (c->m_steps) (void)hipFree(c->m_steps);
     mix_clear_graphs(c);
+    if (c->m_tune_i > 0 && !c->m_tuned)  // tuning started and not finished
+        for (auto &e : c->m_tune_ev) (void)hipEventDestroy(e);
     for (auto &e : c->pool)
         if (e) (void)hipEventDestroy(e);
     if (c->incoh_ev) (void)hipEventDestroy(c->incoh_ev);
@@ -2214,7 +2464,10 @@ static int enqueue_record(Ctx *c, const g4mi_diag_out *o) {
     // the beam sums go straight into the slot; the field sums of the ADI
     // (or of k_diag_field) are copied after them
     double *slot = c->d_recs + c->pend.size() * row;
-    launch_diag_beam(c->stream, c->ns, c->d_off, c->d_g, c->d_th, c->d_x, c->d_y, slot);
+    if (c->beamsum_fresh)  // the fused main kernel summed the step's particles
+        launch_copy_out(c->stream, c->d_beamsum, slot, (long long)(8 * ns));
+    else
+        launch_diag_beam(c->stream, c->ns, c->d_off, c->d_g, c->d_th, c->d_x, c->d_y, slot);
     if (!c->field_diag_fresh) launch_diag_field(c->stream, c->ns, c->d_field, c->ng, c->d_diag_f);
     launch_copy_out(c->stream, c->d_diag_f, slot + 8 * ns, (long long)(8 * ns));
     G4_HIP(hipGetLastError());
@@ -2282,6 +2535,8 @@ int g4mi_set_schedule(g4mi_ctx *h, const char *name, int value) {
     Ctx *c = &h->c;
     if (std::string(name) == "adi_kernel" && value >= -1 && value <= 3) {
         c->adi_kernel = value;
+        c->adi_fixed = value >= 0;
+        if (value < 0) c->adi_choice.clear();
         return G4MI_OK;
     }
     if (std::string(name) == "fft_variant" && value >= -1 && value < kCgemmVariants) {
@@ -2294,6 +2549,39 @@ int g4mi_set_schedule(g4mi_ctx *h, const char *name, int value) {
     }
     if (std::string(name) == "mixed_adi" && (value == 0 || value == 1)) {
         c->m_stream = value == 1;
+        c->m_autotune = false;
+        return G4MI_OK;
+    }
+    if (std::string(name) == "main_nt" && (value == 256 || value == 384 || value == 512)) {
+        c->m_main_nt = value;
+        c->m_autotune = false;
+        return G4MI_OK;
+    }
+    if (std::string(name) == "main_lds" && (value == 0 || value == 1)) {
+        c->m_main_lds = value;
+        c->m_autotune = false;
+        return G4MI_OK;
+    }
+    if (std::string(name) == "fp64_fused" && (value == 0 || value == 1)) {
+        c->fuse64 = value != 0;
+        return G4MI_OK;
+    }
+    if (std::string(name) == "persistent" && (value == 0 || value == 1)) {
+        c->m_persistent = value != 0;
+        if (value) c->m_stream = true;  // the persistent step runs the streaming ADI
+        c->m_autotune = false;
+        return G4MI_OK;
+    }
+    if (std::string(name) == "autotune" && (value == 0 || value == 1)) {
+        c->m_autotune = value != 0;
+        if (c->m_tune_i > 0 && !c->m_tuned)
+            for (auto &e : c->m_tune_ev) (void)hipEventDestroy(e);
+        c->m_tuned = false;
+        c->m_tune_i = 0;
+        for (int q = 0; q < 6; q++) c->m_tune_main[q] = 0, c->m_tune_nmain[q] = 0;
+        c->m_tune_adi[0] = c->m_tune_adi[1] = 0;
+        c->m_tune_nadi[0] = c->m_tune_nadi[1] = 0;
+        c->m_tune_graph = -1;
         return G4MI_OK;
     }
     if (std::string(name) == "graphs" && (value == 0 || value == 1)) {
@@ -2316,9 +2604,12 @@ int g4mi_get_schedule(g4mi_ctx *h, char *buf, int len) {
         bands += "]";
         snprintf(tmp, sizeof(tmp),
                  "{\"precision\": \"fp32\", \"adi\": \"%s\", \"adi_refine\": %d, "
-                 "\"graphs\": %d, \"bands_m_over_template\": \"%s\", \"graphs_captured\": %d}",
-                 (c->m_stream && !c->m_refine) ? "band32-stream" : "band32-tile", c->m_refine ? 1 : 0,
-                 c->m_graphs ? 1 : 0, bands.c_str(), (int)c->m_graph.size());
+                 "\"main_nt\": %d, \"main_lds_kib\": %d, \"persistent\": %d, \"autotune\": %d, \"tuned\": %d, "
+                 "\"graphs\": %d, \"bands_m_over_template\": \"%s\", \"graphs_captured\": %d, "
+                 "\"tuning\": %s}",
+                 c->m_stream ? "band32-stream" : "band32-tile", c->m_refine ? 1 : 0, c->m_main_nt,
+                 c->m_main_lds ? 44 : 68, c->m_persistent ? 1 : 0, c->m_autotune ? 1 : 0, c->m_tuned ? 1 : 0, c->m_graphs ? 1 : 0,
+                 bands.c_str(), (int)c->m_graph.size(), c->m_tune_log.empty() ? "null" : c->m_tune_log.c_str());
         snprintf(buf, (size_t)len, "%s", tmp);
         return G4MI_OK;
     }
@@ -2331,11 +2622,20 @@ int g4mi_get_schedule(g4mi_ctx *h, char *buf, int len) {
         snprintf(buf, (size_t)len, "%s", tmp);
         return G4MI_OK;
     }
+    std::string bydz = "{";
+    for (const auto &kv : c->adi_choice) {
+        char e[64];
+        snprintf(e, sizeof(e), "%s\"%.6g\": \"%s\"", bydz.size() > 1 ? ", " : "", kv.first, names[kv.second]);
+        bydz += e;
+    }
+    bydz += "}";
     snprintf(tmp, sizeof(tmp),
              "{\"field_solver\": \"adi\", \"adi_kernel\": %d, \"adi\": \"%s\", \"adi_window\": %d, "
-             "\"adi_band_m\": %d, \"adi_tuned_ms\": [%.4f, %.4f, %.4f, %.4f]}",
+             "\"adi_band_m\": %d, \"adi_tuned_ms\": [%.4f, %.4f, %.4f, %.4f], \"adi_by_delz\": %s, "
+             "\"fp64_fused\": %d}",
              c->adi_kernel, c->adi_kernel >= 0 ? names[c->adi_kernel] : "untuned", c->adi_tile,
-             c->adi_band_m, c->adi_ms[0], c->adi_ms[1], c->adi_ms[2], c->adi_ms[3]);
+             c->adi_band_m, c->adi_ms[0], c->adi_ms[1], c->adi_ms[2], c->adi_ms[3], bydz.c_str(),
+             c->fuse64 ? 1 : 0);
     snprintf(buf, (size_t)len, "%s", tmp);
     return G4MI_OK;
 }

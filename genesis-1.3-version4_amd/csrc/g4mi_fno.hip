@@ -16,7 +16,7 @@ namespace g4mi {
 
 namespace {
 
-typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x16 __attribute__((ext_vector_type(16)));
 
 // FP32 -> BF16, round to nearest even (finite inputs)
@@ -44,13 +44,13 @@ __device__ __forceinline__ float gelu_d(float x) {
 // GEMM: (32 WM) x (32 WN) output tile per workgroup of WM x WN wavefronts,
 // each wavefront one 32x32 block; k in chunks of 32 staged through LDS as
 // BF16 (global reads as float4 where the contiguous dimension allows),
-// v_mfma_f32_32x32x8_bf16 with FP32 accumulation.  MFMA operand layout
-// (32x32x8): lane l holds A row l%32 and B column l%32 at k = 4 (l/32) ..
-// 4 (l/32) + 3; accumulator register v of lane l is C row 8 (v/4) +
-// 4 (l/32) + v%4, column l%32.
+// gfx950's v_mfma_f32_32x32x16_bf16 (two per chunk) with FP32 accumulation.
+// MFMA operand layout (32x32x16): lane l holds A row l%32 and B column l%32
+// at k = 8 (l/32) .. 8 (l/32) + 7 (one 16-byte LDS read each); accumulator
+// register v of lane l is C row 8 (v/4) + 4 (l/32) + v%4, column l%32.
 // ---------------------------------------------------------------------------
 constexpr int GK = 32;
-constexpr int GP = GK + 4;  // LDS row pitch (bf16): 8-byte aligned rows
+constexpr int GP = GK + 8;  // LDS row pitch (bf16): 16-byte aligned rows
 
 __device__ __forceinline__ void st_bf4(unsigned short *p, float4 v) {
     // 4 consecutive bf16 in one 8-byte LDS store
@@ -127,11 +127,11 @@ __global__ void __launch_bounds__(64 * WM * WN) k_gemm_bf16(GemmArgs g, int vecA
         __syncthreads();
         const int mr = wm * 32 + (lane & 31), nc = wn * 32 + (lane & 31);
 #pragma unroll
-        for (int kk = 0; kk < GK; kk += 8) {
-            const int kq = kk + 4 * (lane >> 5);
-            const bf16x4 a = *(const bf16x4 *)&As[mr * GP + kq];
-            const bf16x4 b = *(const bf16x4 *)&Bs[nc * GP + kq];
-            acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a, b, acc, 0, 0, 0);
+        for (int kk = 0; kk < GK; kk += 16) {
+            const int kq = kk + 8 * (lane >> 5);
+            const bf16x8 a = *(const bf16x8 *)&As[mr * GP + kq];
+            const bf16x8 b = *(const bf16x8 *)&Bs[nc * GP + kq];
+            acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, acc, 0, 0, 0);
         }
         __syncthreads();
     }
@@ -332,7 +332,10 @@ __global__ void k_adam(float *p, const float *g, float *m, float *v, long long n
 }
 
 // Batch assembly from the pair set: in[b][p][4] = (E, S) / scale_b,
-// tgt[b][p][2] = (E_next - E) / scale_b, scale_b = sqrt(<|E|^2 + |S|^2>) over the slice.
+// tgt[b][p][2] = (E_next - E - 2 S) / scale_b, scale_b = sqrt(<|E|^2 + |S|^2>)
+// over the slice.  The ADI step is E_next = E + 2 S + (diffraction of E and
+// S, first order in the step's i dz / (4 ks dgrid^2)): the network learns
+// the diffraction only, the source's 2 S passes through exactly.
 // pairs: [n][P][6] = Er Ei Sr Si Nr Ni.
 __global__ void __launch_bounds__(256) k_scale(const float *pairs, const int *idx, int P, float *scale) {
     __shared__ double red[256];
@@ -365,8 +368,8 @@ __global__ void k_assemble(const float *pairs, const int *idx, const float *scal
     o[2] = q[2] * inv;
     o[3] = q[3] * inv;
     if (tgt) {
-        tgt[i * 2] = (q[4] - q[0]) * inv;
-        tgt[i * 2 + 1] = (q[5] - q[1]) * inv;
+        tgt[i * 2] = (q[4] - q[0] - 2.f * q[2]) * inv;
+        tgt[i * 2 + 1] = (q[5] - q[1] - 2.f * q[3]) * inv;
     }
 }
 
@@ -435,16 +438,20 @@ __global__ void k_slice_in(const float2 *fin, const float2 *src, const int4 *box
     o[3] = sv.y * inv;
 }
 
-// E_next = E + scale * out for beam slices [s0, s0 + B) into fout
-__global__ void k_slice_out(const float2 *fin, float2 *fout, int ng, int ns, int first, int s0, int B,
-                            const float *scale, const float *out) {
+// E_next = E + 2 S + scale * out for beam slices [s0, s0 + B) into fout
+__global__ void k_slice_out(const float2 *fin, float2 *fout, const float2 *src, const int4 *box, int ng, int ns,
+                            int first, int s0, int B, const float *scale, const float *out) {
     const long long i = (long long)blockIdx.x * 256 + threadIdx.x;
     const int P = ng * ng;
     if (i >= (long long)B * P) return;
     const int b = (int)(i / P), p = (int)(i % P), ii = s0 + b;
+    const int x = p % ng, y = p / ng;
+    const int4 bx = box[ii];
+    const float2 sv = (x >= bx.x && x <= bx.z && y >= bx.y && y <= bx.w) ? src[(long long)ii * P + p]
+                                                                         : make_float2(0, 0);
     const long long fo = (long long)((ii + first) % ns) * P + p;
     const float2 e = fin[fo];
-    fout[fo] = make_float2(e.x + scale[b] * out[i * 2], e.y + scale[b] * out[i * 2 + 1]);
+    fout[fo] = make_float2(e.x + 2.f * sv.x + scale[b] * out[i * 2], e.y + 2.f * sv.y + scale[b] * out[i * 2 + 1]);
 }
 
 // Error of the network's field step against the ADI's on one storage slice
@@ -1000,7 +1007,7 @@ int fno_eval(Fno *f, long long p0, long long p1, int batch, double *loss) {
 }
 
 // Batched inference over beam slices: fout[(ii + first) % ns] = fin[...] +
-// scale * FNO((fin, src) / scale) for every slice ii.
+// 2 src[ii] + scale * FNO((fin, src) / scale) for every slice ii.
 int fno_infer_slices(Fno *f, hipStream_t s, const float2 *fin, float2 *fout, const float2 *src, const int4 *box,
                      int ns, int first) {
     hipStream_t keep = f->s;
@@ -1011,8 +1018,8 @@ int fno_infer_slices(Fno *f, hipStream_t s, const float2 *fin, float2 *fout, con
         hipLaunchKernelGGL(k_slice_in, dim3(nb((long long)B * f->P)), dim3(256), 0, s, fin, src, box, f->N, ns, first,
                            s0, B, f->scale, f->in);
         forward(f, B);
-        hipLaunchKernelGGL(k_slice_out, dim3(nb((long long)B * f->P)), dim3(256), 0, s, fin, fout, f->N, ns, first,
-                           s0, B, f->scale, f->out);
+        hipLaunchKernelGGL(k_slice_out, dim3(nb((long long)B * f->P)), dim3(256), 0, s, fin, fout, src, box, f->N,
+                           ns, first, s0, B, f->scale, f->out);
     }
     f->s = keep;
     G4_HIP(hipGetLastError());

@@ -101,13 +101,8 @@ __device__ __forceinline__ void line_map(int kind, double delz, double q, double
     }
 }
 
-__global__ void __launch_bounds__(256) k_transverse(long long n, const double *__restrict__ g,
-                                                    double *__restrict__ x, double *__restrict__ y,
-                                                    double *__restrict__ px,
-                                                    double *__restrict__ py, TransParams p) {
-    long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    double gi = g[i], xi = x[i], yi = y[i], pxi = px[i], pyi = py[i];
+__device__ __forceinline__ void transverse_one(const TransParams &p, double gi, double &xi, double &yi,
+                                               double &pxi, double &pyi) {
     if (p.corrector) {
         pxi += p.ccx;
         pyi += p.ccy;
@@ -124,6 +119,16 @@ __global__ void __launch_bounds__(256) k_transverse(long long n, const double *_
     double gammaz = sqrt(gi * gi - 1 - p.aw * p.aw - pxi * pxi - pyi * pyi);
     line_map(p.kindx, p.delz, p.qx, xi, pxi, gammaz, p.xoff);
     line_map(p.kindy, p.delz, p.qy, yi, pyi, gammaz, p.yoff);
+}
+
+__global__ void __launch_bounds__(256) k_transverse(long long n, const double *__restrict__ g,
+                                                    double *__restrict__ x, double *__restrict__ y,
+                                                    double *__restrict__ px,
+                                                    double *__restrict__ py, TransParams p) {
+    long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double xi = x[i], yi = y[i], pxi = px[i], pyi = py[i];
+    transverse_one(p, g[i], xi, yi, pxi, pyi);
     x[i] = xi;
     y[i] = yi;
     px[i] = pxi;
@@ -660,6 +665,486 @@ __global__ void __launch_bounds__(1024) k_deposit(const long long *__restrict__ 
     }
 }
 
+// Deterministic block sum of K doubles for any multiple of 64 threads:
+// butterfly within each wavefront, then the wave totals in wave order.
+template <int NT, int K>
+__device__ __forceinline__ void block_sum_waves(double (&v)[K], double *lds) {
+    const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
+#pragma unroll
+    for (int k = 0; k < K; k++) v[k] = wave_sum(v[k]);
+    __syncthreads();
+    if (lane == 0)
+#pragma unroll
+        for (int k = 0; k < K; k++) lds[wv * K + k] = v[k];
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < K; k++) {
+        double acc = lds[k];
+        for (int q = 1; q < NT / 64; q++) acc += lds[q * K + k];
+        v[k] = acc;
+    }
+    __syncthreads();
+}
+
+// Block minimum of K ints, same structure; lds holds K * NT/64 ints.
+template <int NT, int K>
+__device__ __forceinline__ void block_min_waves(int (&v)[K], int *lds) {
+    const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
+#pragma unroll
+    for (int k = 0; k < K; k++)
+        for (int o = 32; o > 0; o >>= 1) v[k] = min(v[k], __shfl_xor(v[k], o));
+    __syncthreads();
+    if (lane == 0)
+#pragma unroll
+        for (int k = 0; k < K; k++) lds[wv * K + k] = v[k];
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < K; k++) {
+        int acc = lds[k];
+        for (int q = 1; q < NT / 64; q++) acc = min(acc, lds[q * K + k]);
+        v[k] = acc;
+    }
+    __syncthreads();
+}
+
+// ---------------------------------------------------------------------------
+// The FP64 step's particle passes in two kernels, one workgroup per slice.
+//
+// front: first transverse half step (k_transverse), the slice moments
+//   (k_slice_moments, same per-lane order and block tree, so the same bits),
+//   sizes and centroid (k_slice_sizes), the largest radius from the
+//   centroid (k_sc_rbound; a second read of the slice, from L2) and the
+//   footprint box of the in-grid particles for the main kernel's tile.
+// main: the short-range solve (k_sc_solve's histograms, operator and
+//   recurrence, one mode solution per (m, l) kept in LDS), then one pass
+//   per particle: ez from those modes, the RK4 push (k_push), R56 (k_r56),
+//   the second transverse half step, the beam sums of DiagBeam (k_diag_beam,
+//   same order) and the deposit (k_deposit) into an LDS tile over the
+//   footprint padded by 2 cells; a slice whose particles leave the tile
+//   takes the exact two-pass deposit over its box instead.
+// Every value is computed by the same expression as in the separate
+// kernels; only LDS-atomic summation orders of the histograms and the
+// deposit differ, as between any two runs of those kernels.
+// ---------------------------------------------------------------------------
+constexpr int kFrontNT = 256;                          // the front's block tree is k_slice_moments'
+constexpr int kFuseNT = 512;
+constexpr int kFuseNW = kFuseNT / 64;
+constexpr int kFuseModes = 512;                        // mode-solution cells (double2)
+constexpr int kFuseDyn = kFuseModes + 3584;            // double2 cells of the dynamic region (64 KiB)
+constexpr int kFuseTile = kFuseDyn - kFuseModes;        // deposit tile cells
+static_assert(kFuseNW * kScMaxNg + 2 * kScMaxNg + (6 * kScMaxNg + 2) / 2 <= kFuseTile,
+              "the short-range solver's scratch must fit in the tile region");
+
+__global__ void __launch_bounds__(kFrontNT) k_fp64_front(const long long *__restrict__ off,
+                                                        const double *__restrict__ g, double *__restrict__ x,
+                                                        double *__restrict__ y, double *__restrict__ px,
+                                                        double *__restrict__ py, const double *__restrict__ cur,
+                                                        TransParams t1, int moments, int sc, double gridmax,
+                                                        double dgrid, int ng, double *__restrict__ mom,
+                                                        double *__restrict__ fsize, double *__restrict__ cen,
+                                                        double *__restrict__ rbound, int4 *__restrict__ pbox) {
+    __shared__ double lds[4 * kFrontNT];
+    __shared__ int red[4][kFrontNT];
+    const int is = blockIdx.x, t = threadIdx.x;
+    const long long a = off[is], b = off[is + 1];
+    double v[4] = {0, 0, 0, 0};
+    int bx0 = 1 << 30, by0 = 1 << 30, bx1 = -1, by1 = -1;
+    for (long long i = a + t; i < b; i += kFrontNT) {
+        double xi = x[i], yi = y[i], pxi = px[i], pyi = py[i];
+        transverse_one(t1, g[i], xi, yi, pxi, pyi);
+        x[i] = xi;
+        y[i] = yi;
+        px[i] = pxi;
+        py[i] = pyi;
+        v[0] += xi;
+        v[1] += xi * xi;
+        v[2] += yi;
+        v[3] += yi * yi;
+        if ((xi > -gridmax) && (xi < gridmax) && (yi > -gridmax) && (yi < gridmax)) {
+            const int ix = min((int)floor((xi + gridmax) / dgrid), ng - 2);
+            const int iy = min((int)floor((yi + gridmax) / dgrid), ng - 2);
+            bx0 = min(bx0, ix);
+            bx1 = max(bx1, ix + 1);
+            by0 = min(by0, iy);
+            by1 = max(by1, iy + 1);
+        }
+    }
+    red[0][t] = bx0;
+    red[1][t] = by0;
+    red[2][t] = -bx1;
+    red[3][t] = -by1;
+    __syncthreads();
+    for (int q = kFrontNT / 2; q > 0; q >>= 1) {
+        if (t < q)
+            for (int k = 0; k < 4; k++) red[k][t] = min(red[k][t], red[k][t + q]);
+        __syncthreads();
+    }
+    if (t == 0) pbox[is] = make_int4(red[0][0], red[1][0], -red[2][0], -red[3][0]);
+    if (!moments) return;  // uniform
+    block_sum<kFrontNT, 4>(v, lds);
+    const long long cnt = b - a;
+    double xc = 0, yc = 0;
+    if (cnt > 0) {
+        xc = v[0] / (double)cnt;
+        yc = v[2] / (double)cnt;
+    }
+    if (t == 0) {
+        for (int k = 0; k < 4; k++) mom[4 * is + k] = v[k];
+        double sz;
+        if (cur[is] <= 0 || cnt == 0) {
+            sz = 1;
+        } else {
+            const double c = (double)cnt;
+            const double x1 = v[0] / c, x2 = v[1] / c, y1 = v[2] / c, y2 = v[3] / c;
+            sz = sqrt(fabs(x2 - x1 * x1)) * sqrt(fabs(y2 - y1 * y1));
+        }
+        if (sz <= 0) sz = 1;
+        fsize[is] = sz;
+        cen[2 * is] = xc;
+        cen[2 * is + 1] = yc;
+    }
+    if (!sc) return;  // uniform
+    double m = 0;
+    for (long long i = a + t; i < b; i += kFrontNT) {
+        const double tx = x[i] - xc, ty = y[i] - yc;
+        const double r2 = tx * tx + ty * ty;
+        if (r2 > m) m = r2;
+    }
+    lds[t] = m;
+    __syncthreads();
+    for (int q = kFrontNT / 2; q > 0; q >>= 1) {
+        if (t < q) lds[t] = fmax(lds[t], lds[t + q]);
+        __syncthreads();
+    }
+    if (t == 0) rbound[is] = sqrt(lds[0]);
+}
+
+
+__global__ void __launch_bounds__(kFuseNT) k_fp64_main(const long long *__restrict__ off, double *__restrict__ g,
+                                                       double *__restrict__ th, double *__restrict__ x,
+                                                       double *__restrict__ y, double *__restrict__ px,
+                                                       double *__restrict__ py, const double2 *__restrict__ field,
+                                                       const double *__restrict__ esc, const double *__restrict__ cur,
+                                                       const double *__restrict__ cen, const double *__restrict__ dr_arr,
+                                                       double *__restrict__ efield, const int4 *__restrict__ pbox,
+                                                       double2 *__restrict__ src, int4 *__restrict__ box,
+                                                       double *__restrict__ beamsum, FuseParams f) {
+    extern __shared__ double2 fdyn[];
+    __shared__ double wred[8 * kFuseNW];
+    __shared__ int ired[5 * kFuseNW];
+    double2 *celm = fdyn;                                      // [modes][ngr]
+    double2 *work = fdyn + kFuseModes;                         // solver scratch, then the deposit tile
+    const int is = blockIdx.x, t = threadIdx.x, w = t >> 6;
+    const long long a = off[is], b = off[is + 1];
+    const int npart = (int)(b - a);
+    const PushParams &p = f.p;
+    const DepositParams &d = f.d;
+    const int ngr = f.ngr, ng = p.ngrid;
+    const bool sc = f.use_sc && npart > 0;
+    double xc = 0, yc = 0, dr = 1;
+    if (t == 0 && f.use_sc) efield[is] = 0;
+    // ---- short-range space charge (k_sc_solve): every mode's solution into celm
+    if (sc) {
+        double2(*hist)[kScMaxNg] = (double2(*)[kScMaxNg])work;   // [waves][ngr]
+        double2 *gam = work + kFuseNW * kScMaxNg, *rhs = gam + kScMaxNg;
+        double *vol = (double *)(rhs + kScMaxNg), *ldig = vol + kScMaxNg, *rlog = ldig + kScMaxNg + 1;
+        double *cupp = rlog + kScMaxNg, *clow = cupp + kScMaxNg, *cmid = clow + kScMaxNg;
+        dr = dr_arr[is];
+        xc = cen[2 * is];
+        yc = cen[2 * is + 1];
+        const double pi = 2 * asin(1.0);
+        for (int j = t; j <= ngr; j += kFuseNT) {
+            if (j == ngr) {
+                ldig[ngr] = 0;
+            } else if (j == 0) {
+                vol[0] = pi * dr * dr;
+                rlog[0] = 0.5;
+                ldig[0] = 0;
+            } else {
+                vol[j] = pi * dr * dr * (2 * j + 1);
+                ldig[j] = 2 * pi * j;
+                rlog[j] = log((double)(j + 1) / (double)j);
+            }
+        }
+        const double coef = -f.gz2 / f.ks / f.ks;
+        const double econst = kVacimp / kEev * cur[is] / f.ks / (double)npart;
+        int mode = 0;
+        for (int m = -f.nphi; m <= f.nphi; m++) {
+            for (int l = 1; l <= f.nz; l++, mode++) {
+                __syncthreads();
+                for (int k = t; k < kFuseNW * kScMaxNg; k += kFuseNT) hist[k / kScMaxNg][k % kScMaxNg] = make_double2(0, 0);
+                __syncthreads();
+                for (long long i = a + t; i < b; i += kFuseNT) {
+                    double tx = x[i] - xc, ty = y[i] - yc;
+                    double radi = sqrt(tx * tx + ty * ty);
+                    int bin = (int)floor(radi / dr);
+                    bin = (bin >= 0 && bin < ngr) ? bin : (bin >= ngr ? ngr - 1 : 0);
+                    double s, c;
+                    sincos(l * th[i], &s, &c);
+                    double2 v = make_double2(c, -s);
+                    if (m != 0) v = cmul(cpow_int(make_double2(tx / radi, ty / radi), -m), v);
+                    unsafeAtomicAdd(&hist[w][bin].x, v.x);
+                    unsafeAtomicAdd(&hist[w][bin].y, v.y);
+                }
+                __syncthreads();
+                for (int i = t; i < ngr; i += kFuseNT) {
+                    double lm = -ldig[i] - ldig[i + 1] - 2. * pi * m * m * rlog[i];
+                    if (i == ngr - 1) lm -= 2 * pi * ngr;
+                    cmid[i] = 1 + coef * lm / l / l / vol[i];
+                    if (i > 0) {
+                        cupp[i] = coef * ldig[i] / l / l / vol[i - 1];
+                        clow[i] = coef * ldig[i] / l / l / vol[i];
+                    }
+                    double2 cs = hist[0][i];
+                    for (int q = 1; q < kFuseNW; q++) cs = cadd(cs, hist[q][i]);
+                    rhs[i] = cmul(cs, make_double2(0, econst / l / vol[i]));
+                }
+                __syncthreads();
+                if (t == 0) {
+                    double2 *ce = celm + mode * ngr;
+                    double bet = cmid[0], inv = 1.0 / bet;
+                    ce[0] = cscale(rhs[0], inv);
+                    for (int i = 1; i < ngr; i++) {
+                        const double gg = cupp[i] * inv;
+                        gam[i].x = gg;
+                        bet = cmid[i] - clow[i] * gg;
+                        inv = 1.0 / bet;
+                        ce[i] = cscale(csub(rhs[i], cscale(ce[i - 1], clow[i])), inv);
+                    }
+                    for (int i = ngr - 2; i > -1; i--) ce[i] = csub(ce[i], cscale(ce[i + 1], gam[i + 1].x));
+                    if (l == 1) efield[is] = hypot(ce[0].x, ce[0].y);
+                }
+            }
+        }
+    }
+    // ---- deposit tile over the footprint (front kernel) padded by 2 cells
+    const bool dep = d.enabled != 0;
+    const int4 pb = pbox[is];
+    int tx0 = 0, ty0 = 0, tw = 0, tht = 0;
+    bool fused = false;
+    if (dep && pb.z >= pb.x) {
+        tx0 = max(0, pb.x - 2);
+        ty0 = max(0, pb.y - 2);
+        tw = min(ng - 1, pb.z + 2) - tx0 + 1;
+        tht = min(ng - 1, pb.w + 2) - ty0 + 1;
+        fused = tw * tht <= kFuseTile;
+    }
+    double scl = 0;
+    if (dep) {
+        const double scl0 = d.fcoup * kVacimp * cur[is] * d.xks * d.delz;
+        scl = scl0 / (4 * kEev * (double)(b - a) * d.dgrid * d.dgrid);
+    }
+    __syncthreads();  // solver scratch dead: the tile takes its place
+    if (fused)
+        for (int k = t; k < tw * tht; k += kFuseNT) work[k] = make_double2(0, 0);
+    __syncthreads();
+    // ---- one pass per particle
+    const double eloss = -esc[is] / kEev;
+    const int islice = (is + p.first) % p.nslice;
+    const double2 *fsl = field + (long long)islice * ng * ng;
+    double v[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    int bx0 = 1 << 30, by0 = 1 << 30, bx1 = -1, by1 = -1, ovf = 0;
+    for (long long i = a + t; i < b; i += kFuseNT) {
+        double gamma = g[i];
+        double theta = th[i];
+        double xi = x[i], yi = y[i], pxi = px[i], pyi = py[i];
+        // short-range field of the particle (k_sc_solve's gather, mode order)
+        double ez = 0;
+        if (sc) {
+            const double tx = xi - xc, ty = yi - yc;
+            const double radi = sqrt(tx * tx + ty * ty);
+            int bin = (int)floor(radi / dr);
+            bin = (bin >= 0 && bin < ngr) ? bin : (bin >= ngr ? ngr - 1 : 0);
+            int mode = 0;
+            for (int m = -f.nphi; m <= f.nphi; m++) {
+                for (int l = 1; l <= f.nz; l++, mode++) {
+                    double s, c;
+                    sincos(l * theta, &s, &c);
+                    double2 ct = make_double2(c, s);
+                    if (m != 0) ct = cmul(cpow_int(make_double2(tx / radi, ty / radi), m), ct);
+                    ez += 2 * cmul(ct, celm[mode * ngr + bin]).x;
+                }
+            }
+        }
+        // push (k_push)
+        theta += p.autophase;
+        const double dx = xi - p.ax, dy = yi - p.ay;
+        const double awloc = (1 + 0.5 * (p.kx * dx * dx + p.ky * dy * dy) + p.gradx * dx + p.grady * dy);
+        Ode<0> o;
+        o.xks = p.xks;
+        o.xku = p.xku;
+        o.btpar = 1 + pxi * pxi + pyi * pyi + p.aw * p.aw * awloc * awloc;
+        o.ez = (p.use_sc ? ez : 0.0) + eloss;
+        if ((xi > -p.gridmax) && (xi < p.gridmax) && (yi > -p.gridmax) && (yi < p.gridmax)) {
+            double wx = (xi + p.gridmax) / p.dgrid;
+            double wy = (yi + p.gridmax) / p.dgrid;
+            int ix = (int)floor(wx), iy = (int)floor(wy);
+            wx = 1 + floor(wx) - wx;
+            wy = 1 + floor(wy) - wy;
+            ix = min(ix, p.ngrid - 2);
+            iy = min(iy, p.ngrid - 2);
+            const int idx = ix + iy * p.ngrid;
+            const double2 f0 = fsl[idx], f1 = fsl[idx + 1], f2 = fsl[idx + p.ngrid], f3 = fsl[idx + p.ngrid + 1];
+            double2 cpart = cscale(cscale(f0, wx), wy);
+            cpart = cadd(cpart, cscale(cscale(f1, 1 - wx), wy));
+            cpart = cadd(cpart, cscale(cscale(f2, wx), 1 - wy));
+            cpart = cadd(cpart, cscale(cscale(f3, 1 - wx), 1 - wy));
+            const double r = p.rtmp * awloc;
+            o.rpart = make_double2(r * cpart.x, r * -cpart.y);
+        } else {
+            o.rpart = make_double2(0, 0);
+        }
+        const double delz = p.delz;
+        double k3gg, k3pp;
+        o.k2gg = 0;
+        o.k2pp = 0;
+        o.eval(gamma, theta);
+        double stpz = 0.5 * delz;
+        gamma += stpz * o.k2gg;
+        theta += stpz * o.k2pp;
+        k3gg = o.k2gg;
+        k3pp = o.k2pp;
+        o.k2gg = 0;
+        o.k2pp = 0;
+        o.eval(gamma, theta);
+        gamma += stpz * (o.k2gg - k3gg);
+        theta += stpz * (o.k2pp - k3pp);
+        k3gg /= 6;
+        k3pp /= 6;
+        o.k2gg *= -0.5;
+        o.k2pp *= -0.5;
+        o.eval(gamma, theta);
+        stpz = delz;
+        gamma += stpz * o.k2gg;
+        theta += stpz * o.k2pp;
+        k3gg -= o.k2gg;
+        k3pp -= o.k2pp;
+        o.k2gg *= 2;
+        o.k2pp *= 2;
+        o.eval(gamma, theta);
+        gamma += stpz * (k3gg + o.k2gg / 6.0);
+        theta += stpz * (k3pp + o.k2pp / 6.0);
+        if (f.chicane) theta += f.r56 * (gamma - f.gamma0);   // k_r56
+        transverse_one(f.t2, gamma, xi, yi, pxi, pyi);
+        g[i] = gamma;
+        th[i] = theta;
+        x[i] = xi;
+        y[i] = yi;
+        px[i] = pxi;
+        py[i] = pyi;
+        // beam sums (k_diag_beam)
+        v[0] += gamma;
+        v[1] += gamma * gamma;
+        v[2] += xi;
+        v[3] += xi * xi;
+        v[4] += yi;
+        v[5] += yi * yi;
+        double s1, c1;
+        sincos((0 + 1) * theta, &s1, &c1);
+        v[6] += c1;
+        v[7] += s1;
+        // deposit (k_deposit)
+        if (dep && (xi > -d.gridmax) && (xi < d.gridmax) && (yi > -d.gridmax) && (yi < d.gridmax)) {
+            double wx = (xi + d.gridmax) / d.dgrid;
+            double wy = (yi + d.gridmax) / d.dgrid;
+            const int ix = min((int)floor(wx), ng - 2), iy = min((int)floor(wy), ng - 2);
+            bx0 = min(bx0, ix);
+            bx1 = max(bx1, ix + 1);
+            by0 = min(by0, iy);
+            by1 = max(by1, iy + 1);
+            if (fused) {
+                if (ix >= tx0 && ix + 1 < tx0 + tw && iy >= ty0 && iy + 1 < ty0 + tht) {
+                    wx = 1 + floor(wx) - wx;
+                    wy = 1 + floor(wy) - wy;
+                    const double ddx = xi - d.ax, ddy = yi - d.ay;
+                    const double faw2 = (1 + d.kx * ddx * ddx + d.ky * ddy * ddy + 2 * (d.gradx * ddx + d.grady * ddy));
+                    const double part = sqrt(faw2) * scl / gamma;
+                    double s, c;
+                    sincos(d.harm * theta, &s, &c);
+                    const double2 cp = make_double2(s * part, c * part);
+                    const int lk = (iy - ty0) * tw + (ix - tx0);
+                    const double w00 = wx * wy, w10 = (1 - wx) * wy, w01 = wx * (1 - wy), w11 = (1 - wx) * (1 - wy);
+                    unsafeAtomicAdd(&work[lk].x, w00 * cp.x);
+                    unsafeAtomicAdd(&work[lk].y, w00 * cp.y);
+                    unsafeAtomicAdd(&work[lk + 1].x, w10 * cp.x);
+                    unsafeAtomicAdd(&work[lk + 1].y, w10 * cp.y);
+                    unsafeAtomicAdd(&work[lk + tw].x, w01 * cp.x);
+                    unsafeAtomicAdd(&work[lk + tw].y, w01 * cp.y);
+                    unsafeAtomicAdd(&work[lk + tw + 1].x, w11 * cp.x);
+                    unsafeAtomicAdd(&work[lk + tw + 1].y, w11 * cp.y);
+                } else {
+                    ovf = 1;
+                }
+            }
+        }
+    }
+    block_sum_waves<kFuseNT, 8>(v, wred);
+    if (t == 0)
+        for (int k = 0; k < 8; k++) beamsum[8 * is + k] = v[k];
+    // ---- footprint box and overflow flag
+    int bv[5] = {bx0, by0, -bx1, -by1, -ovf};
+    block_min_waves<kFuseNT, 5>(bv, ired);
+    bx0 = bv[0];
+    by0 = bv[1];
+    bx1 = -bv[2];
+    by1 = -bv[3];
+    ovf = -bv[4];
+    if (!dep || bx1 < 0) {  // no source this step: an empty box
+        if (t == 0) box[is] = make_int4(1, 1, 0, 0);
+        return;
+    }
+    double2 *plane = src + (long long)is * ng * ng;
+    if (fused && !ovf) {
+        for (int k = t; k < tw * tht; k += kFuseNT) plane[(ty0 + k / tw) * ng + tx0 + k % tw] = work[k];
+        if (t == 0) box[is] = make_int4(tx0, ty0, tx0 + tw - 1, ty0 + tht - 1);
+        return;
+    }
+    // ---- exact two-pass deposit over the footprint box (rows in bands of the tile)
+    bx1 = min(bx1, ng - 1);
+    by1 = min(by1, ng - 1);
+    if (t == 0) box[is] = make_int4(bx0, by0, bx1, by1);
+    const int ww = bx1 - bx0 + 1;
+    const int rows_per_band = max(1, kFuseTile / ww);
+    for (int r0 = by0; r0 <= by1; r0 += rows_per_band) {
+        const int r1 = min(by1, r0 + rows_per_band - 1);
+        const int ncell = (r1 - r0 + 1) * ww;
+        for (int k = t; k < ncell; k += kFuseNT) work[k] = make_double2(0, 0);
+        __syncthreads();
+        for (long long i = a + t; i < b; i += kFuseNT) {
+            const double xi = x[i], yi = y[i];
+            if (!((xi > -d.gridmax) && (xi < d.gridmax) && (yi > -d.gridmax) && (yi < d.gridmax))) continue;
+            double wx = (xi + d.gridmax) / d.dgrid;
+            double wy = (yi + d.gridmax) / d.dgrid;
+            const int ix = min((int)floor(wx), ng - 2), iy = min((int)floor(wy), ng - 2);
+            if (iy + 1 < r0 || iy > r1) continue;
+            wx = 1 + floor(wx) - wx;
+            wy = 1 + floor(wy) - wy;
+            const double ddx = xi - d.ax, ddy = yi - d.ay;
+            const double faw2 = (1 + d.kx * ddx * ddx + d.ky * ddy * ddy + 2 * (d.gradx * ddx + d.grady * ddy));
+            const double part = sqrt(faw2) * scl / g[i];
+            double s, c;
+            sincos(d.harm * th[i], &s, &c);
+            const double2 cp = make_double2(s * part, c * part);
+            const double wgt[4] = {wx * wy, (1 - wx) * wy, wx * (1 - wy), (1 - wx) * (1 - wy)};
+            const int cx[4] = {ix, ix + 1, ix, ix + 1};
+            const int cy[4] = {iy, iy, iy + 1, iy + 1};
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+                if (cy[k] < r0 || cy[k] > r1 || cx[k] < bx0 || cx[k] > bx1) continue;
+                const int lk = (cy[k] - r0) * ww + (cx[k] - bx0);
+                unsafeAtomicAdd(&work[lk].x, wgt[k] * cp.x);
+                unsafeAtomicAdd(&work[lk].y, wgt[k] * cp.y);
+            }
+        }
+        __syncthreads();
+        for (int k = t; k < ncell; k += kFuseNT) {
+            const int ry = r0 + k / ww, rx = bx0 + k % ww;
+            plane[ry * ng + rx] = work[k];
+        }
+        __syncthreads();
+    }
+}
+
 // ---------------------------------------------------------------------------
 // K5/K6: ADI half steps, FieldSolverADI::ADI (:60-91) with tridagx
 // (:94-104) and tridagy (:106-122).  One lane per line; the x sweep writes
@@ -781,26 +1266,6 @@ __device__ __forceinline__ void lds_barrier() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
 }
 
-// Deterministic block sum of K doubles for any multiple of 64 threads:
-// butterfly within each wavefront, then the wave totals in wave order.
-template <int NT, int K>
-__device__ __forceinline__ void block_sum_waves(double (&v)[K], double *lds) {
-    const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
-#pragma unroll
-    for (int k = 0; k < K; k++) v[k] = wave_sum(v[k]);
-    __syncthreads();
-    if (lane == 0)
-#pragma unroll
-        for (int k = 0; k < K; k++) lds[wv * K + k] = v[k];
-    __syncthreads();
-#pragma unroll
-    for (int k = 0; k < K; k++) {
-        double acc = lds[k];
-        for (int q = 1; q < NT / 64; q++) acc += lds[q * K + k];
-        v[k] = acc;
-    }
-    __syncthreads();
-}
 
 template <int T, int G, bool PREF>
 __global__ void __launch_bounds__(G <= 192 ? 192 : 256)
@@ -1068,7 +1533,10 @@ __global__ void __launch_bounds__(G <= 192 ? 192 : 256)
 // of the diagonal (1/128 ulp).  A tile then needs u on a halo of M+1 cells, and the
 // update is a separable stencil: no sequential sweep, HBM traffic of one
 // read and one write of the field (the result goes to the second field
-// buffer; the host swaps them).  Complex multiply-adds use fused FMA here:
+// buffer; the host swaps them).  A is symmetric, so the rows of its inverse
+// that equal the central (Toeplitz) row are symmetric too: interior tiles
+// apply them as b_M E_k + sum_j b_{M+j} (E_{k-j} + E_{k+j}), M + 1 complex
+// multiplies per output instead of 2M + 1.  Complex multiply-adds use fused FMA here:
 // the rounding differs from Thomas' only at the last bits (the banded
 // product is a different, equally accurate, evaluation of A^-1 r).
 // Per-tile field-diagnostic sums go to tilediag[slice][tile][8] and are
@@ -1198,12 +1666,15 @@ __global__ void __launch_bounds__(256) k_adi_band(const double2 *__restrict__ fi
         double2 acc[CX];
 #pragma unroll
         for (int j = 0; j < CX; j++) acc[j] = make_double2(0, 0);
-        if (xin) {  // uniform: the shared band through the scalar cache
+        if (xin) {  // uniform: the shared band through the scalar cache, symmetric form
+            const double2 b0 = bc[M];
 #pragma unroll
-            for (int m = 0; m < W; m++) {
-                const double2 b = bc[m];
+            for (int j = 0; j < CX; j++) acc[j] = cmul(b0, ev[j + M]);
 #pragma unroll
-                for (int j = 0; j < CX; j++) acc[j] = cfma(b, ev[j + m], acc[j]);
+            for (int k = 1; k <= M; k++) {
+                const double2 b = bc[M + k];
+#pragma unroll
+                for (int j = 0; j < CX; j++) acc[j] = cfma(b, cadd(ev[j + M - k], ev[j + M + k]), acc[j]);
             }
         } else {
 #pragma unroll
@@ -1246,11 +1717,14 @@ __global__ void __launch_bounds__(256) k_adi_band(const double2 *__restrict__ fi
 #pragma unroll
         for (int j = 0; j < 4; j++) acc4[j] = make_double2(0, 0);
         if (yin) {
+            const double2 b0 = bc[M];
 #pragma unroll
-            for (int m = 0; m < W; m++) {
-                const double2 b = bc[m];
+            for (int j = 0; j < 4; j++) acc4[j] = cmul(b0, fv[j + M]);
 #pragma unroll
-                for (int j = 0; j < 4; j++) acc4[j] = cfma(b, fv[j + m], acc4[j]);
+            for (int k = 1; k <= M; k++) {
+                const double2 b = bc[M + k];
+#pragma unroll
+                for (int j = 0; j < 4; j++) acc4[j] = cfma(b, cadd(fv[j + M - k], fv[j + M + k]), acc4[j]);
             }
         } else {
 #pragma unroll
@@ -1561,6 +2035,30 @@ void launch_push(hipStream_t s, int nblocks, const int *blk_slice, const long lo
     }
 #undef G4_PUSH
 }
+void launch_fp64_front(hipStream_t s, int ns, const long long *off, const double *g, double *x, double *y,
+                       double *px, double *py, const double *cur, const TransParams &t1, bool moments, bool sc,
+                       double gridmax, double dgrid, int ng, double *mom, double *fsize, double *cen,
+                       double *rbound, int4 *pbox) {
+    hipLaunchKernelGGL(k_fp64_front, dim3(ns), dim3(kFrontNT), 0, s, off, g, x, y, px, py, cur, t1, moments ? 1 : 0,
+                       sc ? 1 : 0, gridmax, dgrid, ng, mom, fsize, cen, rbound, pbox);
+}
+
+int fp64_main_modes() { return kFuseModes; }
+
+void launch_fp64_main(hipStream_t s, int ns, const long long *off, double *g, double *th, double *x, double *y,
+                      double *px, double *py, const double2 *field, const double *esc, const double *cur,
+                      const double *cen, const double *dr, double *efield, const int4 *pbox, double2 *src,
+                      int4 *box, double *beamsum, const FuseParams &f) {
+    static bool attr = false;
+    if (!attr) {  // 64 KiB of dynamic LDS plus the static reduction scratch
+        (void)hipFuncSetAttribute((const void *)k_fp64_main, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                  kFuseDyn * sizeof(double2));
+        attr = true;
+    }
+    hipLaunchKernelGGL(k_fp64_main, dim3(ns), dim3(kFuseNT), kFuseDyn * sizeof(double2), s, off, g, th, x, y, px,
+                       py, field, esc, cur, cen, dr, efield, pbox, src, box, beamsum, f);
+}
+
 void launch_deposit(hipStream_t s, int ns, const long long *off, const double *g,
                     const double *th, const double *x, const double *y, const double *cur,
                     double2 *src, int4 *box, const DepositParams &p) {

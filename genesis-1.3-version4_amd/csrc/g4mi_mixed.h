@@ -108,13 +108,21 @@ size_t mix_band_lds(int tmpl_m, bool refine);
 // [7,8]; the other intervals are empty.
 // stream: the streaming ADI (one wavefront per column strip) instead of
 // the 32x32 tile kernel (no residual correction only).
+// main_nt: threads per slice of the main kernel (256, 384, 512); main_lds: its
+// dynamic LDS region, 0 = 68 KiB, 1 = 44 KiB.
 int launch_mix_step(hipStream_t s, const MixArrays &a, int tmpl_m, bool refine, bool sc,
-                    bool moments, int maxzero, hipEvent_t *ev, bool stream);
+                    bool moments, int maxzero, hipEvent_t *ev, bool stream, int main_nt, int main_lds);
+// The same step as one cooperative launch of co-resident workgroups with
+// grid barriers between the phases (the streaming ADI); ev (optional, 9
+// events): [0] before, [8] after.  Returns -1 if the launch is refused.
+int launch_mix_step_persistent(hipStream_t s, const MixArrays &a, int tmpl_m, bool refine, bool sc, bool moments,
+                               int maxzero, hipEvent_t *ev);
 // The same step in three parts: the beam kernels (front, scan, main; ev
 // slots 0-6), the ADI (ntiles: the field-sum blocks per slice it wrote) and
 // the tail (slippage, zeroing, record of ntiles blocks; ev slots 7-8).
 // egsum: also sum the push's energy increments per slice (a.egsum)
-void launch_mix_beam(hipStream_t s, const MixArrays &a, bool sc, bool moments, hipEvent_t *ev, bool egsum);
+void launch_mix_beam(hipStream_t s, const MixArrays &a, bool sc, bool moments, hipEvent_t *ev, bool egsum,
+                     int main_nt = 384, int main_lds = 0);
 int launch_mix_adi(hipStream_t s, const MixArrays &a, int tmpl_m, bool refine, bool stream, int *ntiles);
 void launch_mix_tail(hipStream_t s, const MixArrays &a, int maxzero, int ntiles, hipEvent_t *ev);
 // Field sums per storage slice ([ns][8], as the ADI's diagnostics with one

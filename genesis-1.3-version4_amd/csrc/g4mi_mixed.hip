@@ -20,9 +20,13 @@
 #include <algorithm>
 #include <type_traits>
 
+#include <hip/hip_cooperative_groups.h>
+
 #include "g4mi_mixed.h"
 
 namespace g4mi {
+
+namespace cg = cooperative_groups;
 
 namespace {
 
@@ -127,19 +131,80 @@ __device__ __forceinline__ void transverse32(const TransParams &p, float gam, fl
 // radius (EFieldSolver::analyseBeam, EFieldSolver.cpp:137-157).  One
 // workgroup per slice.
 // ---------------------------------------------------------------------------
+// Block sum of K doubles: butterfly within each wave, then the per-wave
+// partials in wave order (deterministic).  lds holds K * NT/64 doubles.
+template <int NT, int K>
+__device__ __forceinline__ void bsum_w(double (&v)[K], double *lds) {
+    constexpr int NW = NT / 64;
+    const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+#pragma unroll
+    for (int k = 0; k < K; k++)
+        for (int o = 32; o > 0; o >>= 1) v[k] += __shfl_xor(v[k], o);
+    __syncthreads();
+    if (lane == 0)
+#pragma unroll
+        for (int k = 0; k < K; k++) lds[k * NW + wv] = v[k];
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < K; k++) {
+        double acc = 0;
+        for (int q = 0; q < NW; q++) acc += lds[k * NW + q];
+        v[k] = acc;
+    }
+    __syncthreads();
+}
+
+// Block minimum of K ints, same structure as bsum_w; lds holds K * NT/64 ints.
+template <int NT, int K>
+__device__ __forceinline__ void bmin_w(int (&v)[K], int *lds) {
+    constexpr int NW = NT / 64;
+    const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+#pragma unroll
+    for (int k = 0; k < K; k++)
+        for (int o = 32; o > 0; o >>= 1) v[k] = min(v[k], __shfl_xor(v[k], o));
+    __syncthreads();
+    if (lane == 0)
+#pragma unroll
+        for (int k = 0; k < K; k++) lds[k * NW + wv] = v[k];
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < K; k++) {
+        int acc = lds[k * NW];
+        for (int q = 1; q < NW; q++) acc = min(acc, lds[k * NW + q]);
+        v[k] = acc;
+    }
+    __syncthreads();
+}
+
+// Block maximum of one double, same structure; lds holds NT/64 doubles.
+template <int NT>
+__device__ __forceinline__ double bmax_w(double v, double *lds) {
+    constexpr int NW = NT / 64;
+    const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o));
+    __syncthreads();
+    if (lane == 0) lds[wv] = v;
+    __syncthreads();
+    double acc = lds[0];
+    for (int q = 1; q < NW; q++) acc = fmax(acc, lds[q]);
+    __syncthreads();
+    return acc;
+}
+
 constexpr int kFrontNT = 256;
-__global__ void __launch_bounds__(kFrontNT) k_mix_front(MixArrays a, int moments, int sc) {
-    __shared__ double lds[4 * kFrontNT];
+// front of slice `is` on a workgroup of NT threads (a multiple of 64);
+// lds: 4 NT / 64 doubles of LDS scratch
+template <int NT>
+__device__ void front_body(const MixArrays &a, int is, int moments, int sc, double *lds) {
     const MixStep &S = a.steps[a.st->istep];
-    const int is = blockIdx.x, t = threadIdx.x;
+    const int t = threadIdx.x;
     const long long b0 = a.off[is], b1 = a.off[is + 1];
     const float gref = (float)a.gref;
-    __shared__ int bb[4][kFrontNT];
     double v[4] = {0, 0, 0, 0};
     const float gridmax = (float)S.p.gridmax, inv_dgrid = 1.f / (float)S.p.dgrid;
     const int ng = a.ng;
     int bx0 = 1 << 30, by0 = 1 << 30, bx1 = -1, by1 = -1;
-    for (long long i = b0 + t; i < b1; i += kFrontNT) {
+    for (long long i = b0 + t; i < b1; i += NT) {
         float x = a.x[i], y = a.y[i], px = a.px[i], py = a.py[i];
         const float gam = gref + a.dg[i];
         transverse32(S.t1, gam, x, y, px, py);
@@ -161,19 +226,11 @@ __global__ void __launch_bounds__(kFrontNT) k_mix_front(MixArrays a, int moments
         }
     }
     // footprint of the slice for the main kernel's deposit tile
-    bb[0][t] = bx0;
-    bb[1][t] = by0;
-    bb[2][t] = -bx1;
-    bb[3][t] = -by1;
-    __syncthreads();
-    for (int s = kFrontNT / 2; s > 0; s >>= 1) {
-        if (t < s)
-            for (int k = 0; k < 4; k++) bb[k][t] = min(bb[k][t], bb[k][t + s]);
-        __syncthreads();
-    }
-    if (t == 0) a.pbox[is] = make_int4(bb[0][0], bb[1][0], -bb[2][0], -bb[3][0]);
+    int bv[4] = {bx0, by0, -bx1, -by1};
+    bmin_w<NT, 4>(bv, (int *)lds);
+    if (t == 0) a.pbox[is] = make_int4(bv[0], bv[1], -bv[2], -bv[3]);
     if (!moments) return;  // uniform
-    bsum<kFrontNT, 4>(v, lds);
+    bsum_w<NT, 4>(v, lds);
     const long long cnt = b1 - b0;
     double xc = 0, yc = 0;
     if (cnt > 0) {
@@ -197,23 +254,23 @@ __global__ void __launch_bounds__(kFrontNT) k_mix_front(MixArrays a, int moments
     if (!sc) return;  // uniform
     // largest radius from the centroid; each lane rereads what it wrote
     double m = 0;
-    for (long long i = b0 + t; i < b1; i += kFrontNT) {
+    for (long long i = b0 + t; i < b1; i += NT) {
         const double tx = (double)a.x[i] - xc, ty = (double)a.y[i] - yc;
         m = fmax(m, tx * tx + ty * ty);
     }
-    lds[t] = m;
-    __syncthreads();
-    for (int s = kFrontNT / 2; s > 0; s >>= 1) {
-        if (t < s) lds[t] = fmax(lds[t], lds[t + s]);
-        __syncthreads();
-    }
-    if (t == 0) a.rbound[is] = sqrt(lds[0]);
+    m = bmax_w<NT>(m, lds);
+    if (t == 0) a.rbound[is] = sqrt(m);
+}
+
+__global__ void __launch_bounds__(kFrontNT) k_mix_front(MixArrays a, int moments, int sc) {
+    __shared__ double lds[4 * (kFrontNT / 64)];
+    front_body<kFrontNT>(a, blockIdx.x, moments, sc, lds);
 }
 
 // the growing radial grid of the short-range solver, in slice order
 // (EFieldSolver.cpp:159-165); the same ballot walk as the FP64 form
-__global__ void __launch_bounds__(64) k_mix_scan(MixArrays a) {
-    const int lane = threadIdx.x;
+__device__ void scan_body(const MixArrays &a) {
+    const int lane = threadIdx.x & 63;
     double rmax = a.rmax[0];
     const int ngr = a.sc_ng;
     for (int base = 0; base < a.ns; base += 64) {
@@ -240,6 +297,8 @@ __global__ void __launch_bounds__(64) k_mix_scan(MixArrays a) {
     if (lane == 0) a.rmax[0] = rmax;
 }
 
+__global__ void __launch_bounds__(64) k_mix_scan(MixArrays a) { scan_body(a); }
+
 // ---------------------------------------------------------------------------
 // main: one workgroup per beam slice.
 //   long-range space charge of the slice (EFieldSolver::longRange :103-116)
@@ -254,9 +313,9 @@ __global__ void __launch_bounds__(64) k_mix_scan(MixArrays a) {
 // space-charge part and the particle pass, reduction scratch after it, the
 // deposit tile last.
 // ---------------------------------------------------------------------------
-constexpr int kMainNT = 384;
 constexpr int kScMax = 256;
-constexpr int kMixDyn = 8704;  // float2 cells of the dynamic region (68 KiB)
+constexpr int kMixDyn = 8704;    // float2 cells of the large dynamic region (68 KiB)
+constexpr int kMixDynS = 5632;   // the small one (44 KiB)
 
 struct OdeMix {
     float xks, eps_ref, bref, gref, ztemp1, ez;
@@ -321,69 +380,26 @@ __device__ __forceinline__ float2 cw_pow(float2 cw, int m) {
     return p;
 }
 
-// Block sum of K doubles: butterfly within each wave, then the per-wave
-// partials in wave order (deterministic).  lds holds K * NT/64 doubles.
-template <int NT, int K>
-__device__ __forceinline__ void bsum_w(double (&v)[K], double *lds) {
-    constexpr int NW = NT / 64;
-    const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
-#pragma unroll
-    for (int k = 0; k < K; k++)
-        for (int o = 32; o > 0; o >>= 1) v[k] += __shfl_xor(v[k], o);
-    __syncthreads();
-    if (lane == 0)
-#pragma unroll
-        for (int k = 0; k < K; k++) lds[k * NW + wv] = v[k];
-    __syncthreads();
-#pragma unroll
-    for (int k = 0; k < K; k++) {
-        double acc = 0;
-        for (int q = 0; q < NW; q++) acc += lds[k * NW + q];
-        v[k] = acc;
-    }
-    __syncthreads();
-}
-
-// Block minimum of K ints, same structure as bsum_w; lds holds K * NT/64 ints.
-template <int NT, int K>
-__device__ __forceinline__ void bmin_w(int (&v)[K], int *lds) {
-    constexpr int NW = NT / 64;
-    const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
-#pragma unroll
-    for (int k = 0; k < K; k++)
-        for (int o = 32; o > 0; o >>= 1) v[k] = min(v[k], __shfl_xor(v[k], o));
-    __syncthreads();
-    if (lane == 0)
-#pragma unroll
-        for (int k = 0; k < K; k++) lds[k * NW + wv] = v[k];
-    __syncthreads();
-#pragma unroll
-    for (int k = 0; k < K; k++) {
-        int acc = lds[k * NW];
-        for (int q = 1; q < NW; q++) acc = min(acc, lds[k * NW + q]);
-        v[k] = acc;
-    }
-    __syncthreads();
-}
-
 // LDS accumulation is FP64 (ds_add_f64): on gfx950 FP32 LDS atomics
 // (ds_add_f32) issue far slower than FP64 ones, and the sums gain precision.
 constexpr int kCelmCells = 1024;                          // space-charge mode solutions (float2)
-constexpr int kTileCells = (kMixDyn - kCelmCells) / 2;    // double2 cells: histograms, then the deposit tile
 
 // EG: also sum the push's energy increments per slice (egsum, the energy
 // balance of the learned field step).  A variant of its own because the
 // extra accumulator costs registers the common path does not pay for.
-template <bool EG>
-__global__ void __launch_bounds__(kMainNT)
-k_mix_main(MixArrays a, int sc_on) {
-    extern __shared__ float2 dyn[];
-    __shared__ int bred[5 * (kMainNT / 64)];
-    __shared__ double wred[9 * (kMainNT / 64)];
+// NT: threads per slice (4, 6 or 8 wavefronts); DYN: float2 cells of the
+// dynamic LDS region (68 or 44 KiB: the larger deposit tile, or more
+// workgroups per CU).  The autotuner picks both per context.
+template <bool EG, int NT, int DYN>
+__device__ void main_body(const MixArrays &a, const int is, int sc_on, float2 *dyn) {
+    constexpr int kTileCells = (DYN - kCelmCells) / 2;   // double2 cells: histograms, then the deposit tile
+    static_assert((NT / 64) * kScMax <= kTileCells, "the histograms must fit in the tile region");
+    __shared__ int bred[5 * (NT / 64)];
+    __shared__ double wred[9 * (NT / 64)];
     __shared__ float gsol[kScMax], scupp[kScMax], sclow[kScMax], scmid[kScMax];
     __shared__ float2 scrhs[kScMax];
     const MixStep &S = a.steps[a.st->istep];
-    const int is = blockIdx.x, t = threadIdx.x, w = t >> 6;
+    const int t = threadIdx.x, w = t >> 6;
     const long long b0 = a.off[is], b1 = a.off[is + 1];
     const int npart = (int)(b1 - b0);
     const int ns = a.ns, ng = a.ng;
@@ -391,13 +407,13 @@ k_mix_main(MixArrays a, int sc_on) {
     const float2 *field = a.field[a.st->cur] + (long long)((is + first) % ns) * ng * ng;
     float2 *celm = dyn;                                   // [nmodes][ngr]
     double2 *work = (double2 *)(dyn + kCelmCells);        // histograms / deposit tile
-    double2(*hist)[kScMax] = (double2(*)[kScMax])work;    // [kMainNT/64][kScMax]
+    double2(*hist)[kScMax] = (double2(*)[kScMax])work;    // [NT/64][kScMax]
 
     // ---- long-range space charge of this slice
     double esc = 0;
     if (S.longrange) {
         double e = 0;
-        for (int j = t; j < ns; j += kMainNT) {
+        for (int j = t; j < ns; j += NT) {
             const double sz = a.mom_fsize[j];
             const double ds = (double)(j - is) * S.lsc_ds;
             const double coef = 1 - sqrt(ds * ds / (ds * ds + sz));
@@ -405,7 +421,7 @@ k_mix_main(MixArrays a, int sc_on) {
             e += sgn * coef * S.lsc_scl * a.cur[j] / sz;
         }
         double v[1] = {e};
-        bsum_w<kMainNT, 1>(v, wred);
+        bsum_w<NT, 1>(v, wred);
         esc = v[0];
     }
     if (t == 0) a.esc[is] = esc;  // the slice's LSC field for the diagnostics
@@ -431,7 +447,7 @@ k_mix_main(MixArrays a, int sc_on) {
         for (int m = -a.sc_nphi; m <= a.sc_nphi; m++) {
             for (int l = 1; l <= a.sc_nz; l++, mode++) {
                 __syncthreads();
-                for (int k = t; k < (kMainNT / 64) * kScMax; k += kMainNT)
+                for (int k = t; k < (NT / 64) * kScMax; k += NT)
                     hist[k / kScMax][k % kScMax] = make_double2(0, 0);
                 __syncthreads();
                 float nx = 0, ny = 0, nt = 0;
@@ -440,12 +456,12 @@ k_mix_main(MixArrays a, int sc_on) {
                     ny = a.y[b0 + t];
                     nt = a.dth[b0 + t];
                 }
-                for (long long i = b0 + t; i < b1; i += kMainNT) {
+                for (long long i = b0 + t; i < b1; i += NT) {
                     const float tx = nx - xc, ty = ny - yc, th = nt;
-                    if (i + kMainNT < b1) {
-                        nx = a.x[i + kMainNT];
-                        ny = a.y[i + kMainNT];
-                        nt = a.dth[i + kMainNT];
+                    if (i + NT < b1) {
+                        nx = a.x[i + NT];
+                        ny = a.y[i + NT];
+                        nt = a.dth[i + NT];
                     }
                     const float radi = sqrtf(tx * tx + ty * ty);
                     const int bin = sc_bin(radi, dr, ngr);
@@ -459,7 +475,7 @@ k_mix_main(MixArrays a, int sc_on) {
                 // constructLaplaceOperator (:176-190): coefficients and source of
                 // every radial cell, one cell per lane; tridiag (:260-272) on
                 // lane 0.  The operator is real, the right-hand side complex.
-                for (int i = t; i < ngr; i += kMainNT) {
+                for (int i = t; i < ngr; i += NT) {
                     const double voli = kPi * drd * drd * (i == 0 ? 1.0 : (double)(2 * i + 1));
                     const double ldi = i > 0 ? 2 * kPi * i : 0.0;
                     const double ldn = (i + 1 < ngr) ? 2 * kPi * (i + 1) : 0.0;
@@ -473,7 +489,7 @@ k_mix_main(MixArrays a, int sc_on) {
                         sclow[i] = (float)(coef * ldi / l / l / voli);
                     }
                     double csx = 0, csy = 0;
-                    for (int q = 0; q < kMainNT / 64; q++) {
+                    for (int q = 0; q < NT / 64; q++) {
                         csx += hist[q][i].x;
                         csy += hist[q][i].y;
                     }
@@ -521,7 +537,7 @@ k_mix_main(MixArrays a, int sc_on) {
     }
     __syncthreads();  // histograms dead, tile takes their place
     if (fused)
-        for (int k = t; k < tw * th; k += kMainNT) work[k] = make_double2(0, 0);
+        for (int k = t; k < tw * th; k += NT) work[k] = make_double2(0, 0);
     __syncthreads();
 
     // ---- particle pass
@@ -558,12 +574,12 @@ k_mix_main(MixArrays a, int sc_on) {
         n_px = a.px[i];
         n_py = a.py[i];
     }
-    for (long long i = b0 + t; i < b1; i += kMainNT) {
+    for (long long i = b0 + t; i < b1; i += NT) {
         const float th_in = n_th;
         const float g0 = n_g, t0 = th_in + autoph;
         float x = n_x, y = n_y, px = n_px, py = n_py;
-        if (i + kMainNT < b1) {
-            const long long j = i + kMainNT;
+        if (i + NT < b1) {
+            const long long j = i + NT;
             n_th = a.dth[j];
             n_g = a.dg[j];
             n_x = a.x[j];
@@ -705,7 +721,7 @@ k_mix_main(MixArrays a, int sc_on) {
     double v[NV];
 #pragma unroll
     for (int k = 0; k < NV; k++) v[k] = fv[k];
-    bsum_w<kMainNT, NV>(v, wred);
+    bsum_w<NT, NV>(v, wred);
     if (t == 0) {
         // sum gamma = n gref + sum dg, sum gamma^2 = n gref^2 + 2 gref sum dg + sum dg^2
         const double n = (double)npart, gr = a.gref;
@@ -717,7 +733,7 @@ k_mix_main(MixArrays a, int sc_on) {
     }
     // ---- footprint box and overflow flag
     int bv[5] = {bx0, by0, -bx1, -by1, -ovf};
-    bmin_w<kMainNT, 5>(bv, bred);
+    bmin_w<NT, 5>(bv, bred);
     bx0 = bv[0];
     by0 = bv[1];
     bx1 = -bv[2];
@@ -729,7 +745,7 @@ k_mix_main(MixArrays a, int sc_on) {
     }
     float2 *plane = a.src + (long long)is * ng * ng;
     if (fused && !ovf) {
-        for (int k = t; k < tw * th; k += kMainNT)
+        for (int k = t; k < tw * th; k += NT)
             plane[(ty0 + k / tw) * ng + tx0 + k % tw] = tof(work[k]);
         if (t == 0) a.box[is] = make_int4(tx0, ty0, tx0 + tw - 1, ty0 + th - 1);
         return;
@@ -744,9 +760,9 @@ k_mix_main(MixArrays a, int sc_on) {
     for (int r0 = by0; r0 <= by1; r0 += rows_per_band) {
         const int r1 = min(by1, r0 + rows_per_band - 1);
         const int ncell = (r1 - r0 + 1) * ww;
-        for (int k = t; k < ncell; k += kMainNT) tile[k] = make_double2(0, 0);
+        for (int k = t; k < ncell; k += NT) tile[k] = make_double2(0, 0);
         __syncthreads();
-        for (long long i = b0 + t; i < b1; i += kMainNT) {
+        for (long long i = b0 + t; i < b1; i += NT) {
             const float x = a.x[i], y = a.y[i];
             if (!((x > -gridmax) && (x < gridmax) && (y > -gridmax) && (y < gridmax))) continue;
             float wx = (x + gridmax) * inv_dgrid, wy = (y + gridmax) * inv_dgrid;
@@ -772,12 +788,18 @@ k_mix_main(MixArrays a, int sc_on) {
             }
         }
         __syncthreads();
-        for (int k = t; k < ncell; k += kMainNT) {
+        for (int k = t; k < ncell; k += NT) {
             const int ry = r0 + k / ww, rx = bx0 + k % ww;
             plane[ry * ng + rx] = tof(tile[k]);
         }
         __syncthreads();
     }
+}
+
+template <bool EG, int NT, int DYN>
+__global__ void __launch_bounds__(NT) k_mix_main(MixArrays a, int sc_on) {
+    extern __shared__ float2 dyn[];
+    main_body<EG, NT, DYN>(a, blockIdx.x, sc_on, dyn);
 }
 
 // ---------------------------------------------------------------------------
@@ -1061,50 +1083,82 @@ __global__ void __launch_bounds__(256) k_mix_adi(MixArrays a) {
 }
 
 // ---------------------------------------------------------------------------
-// Streaming ADI of the mixed form (FP32, no residual correction): both half
-// steps of FieldSolverADI::ADI (:60-91) with the same FP32 banded inverse as
-// the tile kernel, organised as one wavefront per strip of columns walking
-// down the rows of a slice.  Lane l owns column x = x0 - M + l; lanes
-// [M, 64-M) produce output.  For each row r:
+// Streaming ADI of the mixed form: both half steps of FieldSolverADI::ADI
+// (:60-91) with the FP32 banded inverse B ~ A^-1, organised as one wavefront
+// per strip of columns walking down the rows of a slice.  Lane l owns column
+// x = x0 - H + l; lanes [H, 64-H) produce output.  For each row r:
 //   E(r) = S + u + i a (u(r+1) - 2 u(r) + u(r-1))     (field rows prefetched)
 //   u'(r) = sum_m B[x][m] E(r, x+m-M)                   (E row through LDS)
 //   F(r) = S + 2 u'(r) - E(r)                            (kept in a register ring)
 //   u''(r-M) = sum_m B[r-M][m] F(r-2M+m)                 (ring of 2M+1 rows)
-// Each field value is read once (plus the 2M-column halo of the strip) and
-// written once.  The x band rows are per lane (registers); the y band rows
-// are uniform per row (scalar loads).
+// Rows and columns whose band equals the central one use it in the
+// symmetric form b_M F_k + sum_j b_{M+j} (F_{k-j} + F_{k+j}) (A, and so its
+// Toeplitz interior inverse, is symmetric), M + 1 complex multiplies
+// instead of 2M + 1.
+//
+// REF: one FP64 residual-correction sweep per half step.  A = tridiag(-i a,
+// 1 + 2 i a, -i a) (FieldSolverADI::init, :137-165) is diagonally dominant,
+// |off-diagonal / diagonal| = a / |1 + 2 i a| ~ 0.01, so one Jacobi step on
+// the FP64 residual,
+//   u = u1 + D^-1 (r - A u1),  u1 = B r (FP32),  D = 1 + 2 i a,
+// with r, A u1 and the update in FP64, shrinks the solve's error (band
+// truncation and FP32 accumulation) by 2 a / |1 + 2 i a| before the result
+// is rounded to the FP32 field.  E is exact in FP64 in its lane; F is kept as
+// an FP32 value plus its FP32 remainder (F = hi + lo to ~2^-48), so the y
+// residual uses F to FP64 accuracy.  The x residual needs u1 of the
+// neighbouring columns (H = M + 1 halo lanes), the y residual u'' of the row
+// below, so the corrected row y leaves the loop one row later.
+//
+// Each field value is read once (plus the strip's 2H-column halo) and
+// written once.  The x band rows of edge strips are per lane (registers);
+// the y band rows are uniform per row (scalar loads).
 // ---------------------------------------------------------------------------
-template <int M>
+template <int M, bool REF>
 struct StreamGeom {
     static constexpr int W = 2 * M + 1;
-    static constexpr int SO = 64 - 2 * M;  // output columns per strip
-    static constexpr int PF = 3;           // field rows prefetched ahead
+    static constexpr int H = REF ? M + 1 : M;  // halo lanes on each side
+    static constexpr int SO = 64 - 2 * H;      // output columns per strip
+    static constexpr int PF = 3;               // field rows prefetched ahead
 };
 
-int stream_strips(int ng, int m) { return (ng + (64 - 2 * m) - 1) / (64 - 2 * m); }
+int stream_strips(int ng, int m, bool ref) {
+    const int so = 64 - 2 * (ref ? m + 1 : m);
+    return (ng + so - 1) / so;
+}
 
 // a row of zeros the streaming kernel reads for source rows outside the box
 __device__ float2 g_zero_row[4096];
 
-template <int M>
-__global__ void __launch_bounds__(64) k_mix_adi_stream(MixArrays a) {
-    using G = StreamGeom<M>;
-    constexpr int W = G::W, SO = G::SO, PF = G::PF;
-    __shared__ float2 lrow[2][64];
-    const int lane = threadIdx.x;
+// A u for u and the sum of its two neighbours, A = tridiag(-i a, 1 + 2 i a, -i a)
+__device__ __forceinline__ double2 adi_apply(double2 u, double2 nb, double a) {
+    return d2(u.x - 2.0 * a * u.y + a * nb.y, u.y + 2.0 * a * u.x - a * nb.x);
+}
+// u + D^-1 rho, D = 1 + 2 i a, with dn = 1 / (1 + 4 a^2)
+__device__ __forceinline__ double2 jacobi_update(double2 u, double2 rho, double a, double dn) {
+    return d2(u.x + (rho.x + 2.0 * a * rho.y) * dn, u.y + (rho.y - 2.0 * a * rho.x) * dn);
+}
+
+// strip `strip` of beam slice ii on one wavefront; lrow, lu1: [2][64] LDS rows of its own
+template <int M, bool REF>
+__device__ void adi_stream_body(const MixArrays &a, const int ii, const int strip, float2 (*lrow)[64],
+                                float2 (*lu1)[64]) {
+    using G = StreamGeom<M, REF>;
+    constexpr int W = G::W, H = G::H, SO = G::SO, PF = G::PF;
+    const int lane = threadIdx.x & 63;
     const MixStep &S = a.steps[a.st->istep];
     const MixBand bi = a.binfo[S.band];
     const float2 *band = a.bands + bi.off;
+    const double ad = bi.a;
     const float af = (float)bi.a;
+    const double dn = 1.0 / (1.0 + 4.0 * ad * ad);
     const int ng = a.ng, ns = a.ns;
     const int nstrip = (ng + SO - 1) / SO;
-    const int b = blockIdx.x;
-    const int ii = (b / (8 * nstrip)) * 8 + (b & 7);   // XCD-aware: a slice's strips on one XCD
-    const int strip = (b % (8 * nstrip)) >> 3;
-    if (ii >= ns) return;
-    const int x = strip * SO - M + lane;
+    const int x = strip * SO - H + lane;
     const bool xin = x >= 0 && x < ng;
-    const bool outl = lane >= M && lane < 64 - M && xin;
+    const bool bandl = lane >= M && lane < 64 - M;    // lanes with the band's whole support
+    const bool outl = lane >= H && lane < 64 - H && xin;
+    // every band lane of the strip on a column that shares the central row (uniform)
+    const bool xsym = strip * SO - H + M >= bi.klo && strip * SO - H + 63 - M <= bi.khi;
     const long long fs = (long long)((ii + a.st->first) % ns);
     const int cur = a.st->cur;
     const float2 *u = a.field[cur] + fs * ng * ng;
@@ -1112,14 +1166,14 @@ __global__ void __launch_bounds__(64) k_mix_adi_stream(MixArrays a) {
     const float2 *Sp = a.src + (long long)ii * ng * ng;
     const int4 bx = a.box[ii];
     const bool sxin = x >= bx.x && x <= bx.z;
-    // this column's band row
+    // this column's band row (edge strips only)
     float2 bxr[W];
 #pragma unroll
-    for (int m = 0; m < W; m++) bxr[m] = xin ? band[(long long)x * W + m] : f2(0, 0);
+    for (int m = 0; m < W; m++) bxr[m] = (xin && !xsym) ? band[(long long)x * W + m] : f2(0, 0);
     const float2 *bc = band + (long long)((bi.klo + bi.khi) / 2) * W;
-    float2 bcr[W];  // the shared interior row of the y band, held in registers
+    float2 bcr[M + 1];  // the central row's right half b_M .. b_2M (symmetric)
 #pragma unroll
-    for (int m = 0; m < W; m++) bcr[m] = bc[m];
+    for (int m = 0; m <= M; m++) bcr[m] = bc[M + m];
     // field rows q[k] = u(r - 1 + k) and source rows sq[k] = S(r + k - 1),
     // k = 0..PF+1, prefetched in the order they are consumed (the memory
     // counter retires loads in issue order)
@@ -1148,24 +1202,58 @@ __global__ void __launch_bounds__(64) k_mix_adi_stream(MixArrays a) {
     float2 ring[W];
 #pragma unroll
     for (int m = 0; m < W; m++) ring[m] = f2(0, 0);
+    // REF: the FP32 remainders of F for rows r-M-1 .. r, and u'' of the two
+    // rows before the current band output
+    constexpr int NL = REF ? M + 2 : 1;
+    float2 lo[NL];
+#pragma unroll
+    for (int m = 0; m < NL; m++) lo[m] = f2(0, 0);
+    float2 p1 = f2(0, 0), p2 = f2(0, 0);
     const double shift = -0.5 * (double)(ng - 1);
     const int icen = (ng / 2) * ng + (ng / 2);
     double dg[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-    const int nrow = ng + M;
+    auto emit = [&](int y, float2 o) {
+        if (outl) {
+            uo[(long long)y * ng + x] = o;
+            const double ox = o.x, oy = o.y;
+            const double wei = ox * ox + oy * oy;
+            const double dx = (double)x + shift, dy = (double)y + shift;
+            dg[0] += wei;
+            dg[1] += dx * wei;
+            dg[2] += dx * dx * wei;
+            dg[3] += dy * wei;
+            dg[4] += dy * dy * wei;
+            dg[5] += ox;
+            dg[6] += oy;
+            if (y * ng + x == icen) dg[7] = wei;
+        }
+    };
+    const int nrow = ng + M + (REF ? 1 : 0);
     for (int r0 = 0; r0 < nrow; r0 += QN) {
 #pragma unroll
         for (int j = 0; j < QN; j++) {
             const int r = r0 + j;
             if (r >= nrow) break;
             // slots: row r-1 -> j, row r -> (j+1)%QN, row r+1 -> (j+2)%QN
-            float2 Fr = f2(0, 0);
+            float2 Fr = f2(0, 0), Flo = f2(0, 0);
             if (r < ng) {
                 const float2 sv = sq[(j + 1) % QN];
                 const float2 uc = q[(j + 1) % QN];
                 const float2 un = (r + 1 < ng) ? q[(j + 2) % QN] : f2(0, 0);
                 const float2 ul = (r > 0) ? q[j] : f2(0, 0);
-                const float sx = un.x - 2.f * uc.x + ul.x, sy = un.y - 2.f * uc.y + ul.y;
-                const float2 E = xin ? f2(sv.x + uc.x - af * sy, sv.y + uc.y + af * sx) : f2(0, 0);
+                double2 E64 = d2(0, 0);
+                float2 E = f2(0, 0);
+                if constexpr (REF) {
+                    if (xin) {
+                        const double sx = (double)un.x - 2.0 * (double)uc.x + (double)ul.x;
+                        const double sy = (double)un.y - 2.0 * (double)uc.y + (double)ul.y;
+                        E64 = d2((double)sv.x + (double)uc.x - ad * sy, (double)sv.y + (double)uc.y + ad * sx);
+                    }
+                    E = tof(E64);
+                } else {
+                    const float sx = un.x - 2.f * uc.x + ul.x, sy = un.y - 2.f * uc.y + ul.y;
+                    E = xin ? f2(sv.x + uc.x - af * sy, sv.y + uc.y + af * sx) : f2(0, 0);
+                }
                 float2 *lr = lrow[j & 1];
                 lr[lane] = E;
                 // one wavefront per workgroup: LDS operations of a wave complete in
@@ -1176,11 +1264,35 @@ __global__ void __launch_bounds__(64) k_mix_adi_stream(MixArrays a) {
                 __builtin_amdgcn_wave_barrier();
                 __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
                 float2 up = f2(0, 0);
-                if (lane >= M && lane < 64 - M) {
+                if (bandl && xin) {
+                    if (xsym) {
+                        up = fmul(bcr[0], lr[lane]);
 #pragma unroll
-                    for (int m = 0; m < W; m++) up = ffma(bxr[m], lr[lane + m - M], up);
+                        for (int k = 1; k <= M; k++) up = ffma(bcr[k], fadd(lr[lane - k], lr[lane + k]), up);
+                    } else {
+#pragma unroll
+                        for (int m = 0; m < W; m++) up = ffma(bxr[m], lr[lane + m - M], up);
+                    }
                 }
-                Fr = xin ? f2(sv.x + 2.f * up.x - E.x, sv.y + 2.f * up.y - E.y) : f2(0, 0);
+                if constexpr (REF) {
+                    float2 *lu = lu1[j & 1];
+                    lu[lane] = up;  // zero off the grid and on the halo lanes' far side
+                    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                    __builtin_amdgcn_wave_barrier();
+                    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+                    const float2 nl = lu[lane > 0 ? lane - 1 : 0], nr = lu[lane < 63 ? lane + 1 : 63];
+                    const double2 u1 = tod(up);
+                    const double2 nb = d2((double)nl.x + (double)nr.x, (double)nl.y + (double)nr.y);
+                    const double2 au = adi_apply(u1, nb, ad);
+                    const double2 upd = jacobi_update(u1, d2(E64.x - au.x, E64.y - au.y), ad, dn);
+                    if (xin) {
+                        const double2 F64 = d2((double)sv.x + 2.0 * upd.x - E64.x, (double)sv.y + 2.0 * upd.y - E64.y);
+                        Fr = tof(F64);
+                        Flo = f2((float)(F64.x - (double)Fr.x), (float)(F64.y - (double)Fr.y));
+                    }
+                } else {
+                    Fr = xin ? f2(sv.x + 2.f * up.x - E.x, sv.y + 2.f * up.y - E.y) : f2(0, 0);
+                }
                 // slot j (row r-1) is free: it takes row r + PF + 1, first used
                 // PF rows later
                 q[j] = uload(r + PF + 1);
@@ -1189,31 +1301,40 @@ __global__ void __launch_bounds__(64) k_mix_adi_stream(MixArrays a) {
 #pragma unroll
             for (int m = 0; m < W - 1; m++) ring[m] = ring[m + 1];
             ring[W - 1] = Fr;
-            const int y = r - M;
-            if (y >= 0) {
-                float2 o = f2(0, 0);
-                if (y >= bi.klo && y <= bi.khi) {
+            if constexpr (REF) {
 #pragma unroll
-                    for (int m = 0; m < W; m++) o = ffma(bcr[m], ring[m], o);
+                for (int m = 0; m < NL - 1; m++) lo[m] = lo[m + 1];
+                lo[NL - 1] = Flo;
+            }
+            const int y = r - M;
+            float2 o = f2(0, 0);
+            if (y >= 0 && y < ng) {
+                if (y >= bi.klo && y <= bi.khi) {
+                    o = fmul(bcr[0], ring[M]);
+#pragma unroll
+                    for (int k = 1; k <= M; k++) o = ffma(bcr[k], fadd(ring[M - k], ring[M + k]), o);
                 } else {
                     const float2 *by = band + (long long)y * W;
 #pragma unroll
                     for (int m = 0; m < W; m++) o = ffma(by[m], ring[m], o);
                 }
-                if (outl) {
-                    uo[(long long)y * ng + x] = o;
-                    const double ox = o.x, oy = o.y;
-                    const double wei = ox * ox + oy * oy;
-                    const double dx = (double)x + shift, dy = (double)y + shift;
-                    dg[0] += wei;
-                    dg[1] += dx * wei;
-                    dg[2] += dx * dx * wei;
-                    dg[3] += dy * wei;
-                    dg[4] += dy * dy * wei;
-                    dg[5] += ox;
-                    dg[6] += oy;
-                    if (y * ng + x == icen) dg[7] = wei;
+            }
+            if constexpr (REF) {
+                // row yf = y - 1 with u''(yf - 1) = p2, u''(yf) = p1, u''(yf + 1) = o;
+                // F(yf) = ring[M - 1] + lo[0]
+                const int yf = y - 1;
+                if (yf >= 0 && yf < ng) {
+                    const float2 fh = ring[M - 1], fl = lo[0];
+                    const double2 F64 = d2((double)fh.x + (double)fl.x, (double)fh.y + (double)fl.y);
+                    const double2 u2 = tod(p1);
+                    const double2 nb = d2((double)p2.x + (double)o.x, (double)p2.y + (double)o.y);
+                    const double2 au = adi_apply(u2, nb, ad);
+                    emit(yf, tof(jacobi_update(u2, d2(F64.x - au.x, F64.y - au.y), ad, dn)));
                 }
+                p2 = p1;
+                p1 = o;
+            } else {
+                if (y >= 0) emit(y, o);
             }
         }
     }
@@ -1227,12 +1348,24 @@ __global__ void __launch_bounds__(64) k_mix_adi_stream(MixArrays a) {
     }
 }
 
+template <int M, bool REF>
+__global__ void __launch_bounds__(64) k_mix_adi_stream(MixArrays a) {
+    __shared__ float2 lrow[2][64];
+    __shared__ float2 lu1[2][64];
+    constexpr int SO = StreamGeom<M, REF>::SO;
+    const int nstrip = (a.ng + SO - 1) / SO;
+    const int b = blockIdx.x;
+    const int ii = (b / (8 * nstrip)) * 8 + (b & 7);   // XCD-aware: a slice's strips on one XCD
+    const int strip = (b % (8 * nstrip)) >> 3;
+    if (ii >= a.ns) return;
+    adi_stream_body<M, REF>(a, ii, strip, lrow, lu1);
+}
+
 // ---------------------------------------------------------------------------
 // slippage on the device (Control::applySlippage, Control.cpp:158-276): the
 // ring offset, the slices to zero, the live field buffer and the step index.
 // ---------------------------------------------------------------------------
-__global__ void k_mix_slip(MixArrays a) {
-    if (threadIdx.x != 0) return;
+__device__ void slip_body(const MixArrays &a) {
     MixState &st = *a.st;
     const MixStep &S = a.steps[st.istep];
     const int ns = a.ns;
@@ -1256,6 +1389,10 @@ __global__ void k_mix_slip(MixArrays a) {
     st.seq = st.seq + 1;
 }
 
+__global__ void k_mix_slip(MixArrays a) {
+    if (threadIdx.x == 0) slip_body(a);
+}
+
 __global__ void __launch_bounds__(256) k_mix_zero(MixArrays a) {
     const int z = blockIdx.y;
     if (z >= a.st->nzero) return;
@@ -1266,9 +1403,7 @@ __global__ void __launch_bounds__(256) k_mix_zero(MixArrays a) {
 
 // Record of the completed step per beam slice: beam sums and the field sums
 // of its storage slice (zero for slices slippage has just cleared).
-__global__ void __launch_bounds__(64) k_mix_record(MixArrays a, int ntiles) {
-    const int is = blockIdx.x * 64 + threadIdx.x;
-    if (is >= a.ns) return;
+__device__ void record_body(const MixArrays &a, const int is, int ntiles) {
     const MixState &st = *a.st;
     const int fs = (is + st.first) % a.ns;
     bool zeroed = false;
@@ -1280,6 +1415,11 @@ __global__ void __launch_bounds__(64) k_mix_record(MixArrays a, int ntiles) {
     double *o = a.rec + ((long long)(st.rec_step % a.nrec) * a.ns + is) * kMixRecord;
     for (int q = 0; q < 8; q++) o[q] = a.beamsum[8 * is + q];
     for (int q = 0; q < 8; q++) o[8 + q] = f[q];
+}
+
+__global__ void __launch_bounds__(64) k_mix_record(MixArrays a, int ntiles) {
+    const int is = blockIdx.x * 64 + threadIdx.x;
+    if (is < a.ns) record_body(a, is, ntiles);
 }
 
 // Standalone sums for a diagnostics call outside a run.
@@ -1379,6 +1519,54 @@ __global__ void __launch_bounds__(256) k_mix_sort_index(MixArrays a, double invs
     }
 }
 
+// ---------------------------------------------------------------------------
+// The whole step in one launch: every phase of launch_mix_step on a grid of
+// co-resident workgroups (a cooperative launch, so every workgroup is
+// resident and reaches every barrier), phases separated by grid barriers:
+//   front (all slices) | radial scan | main (all slices) | ADI (all strips)
+//   | slippage | zeroing + records.
+// Workgroups of kPerNT threads walk the slices; in the ADI phase every
+// wavefront walks (slice, strip) items, the strips of a slice on the
+// workgroups of one XCD (workgroups are dispatched to the XCDs round-robin,
+// so block b runs on XCD b % 8: locality only, not correctness).  The
+// dynamic LDS region is the main kernel's; front and ADI carve their
+// scratch from it.  Every loop is uniform per workgroup and nothing returns
+// early, so each workgroup meets each grid barrier once.
+// ---------------------------------------------------------------------------
+constexpr int kPerNT = 384;
+
+template <int M, bool REF>
+__global__ void __launch_bounds__(kPerNT) k_mix_step_persistent(MixArrays a, int sc, int moments, int maxzero) {
+    extern __shared__ float2 dyn[];
+    cg::grid_group grid = cg::this_grid();
+    const int G = (int)gridDim.x, b = (int)blockIdx.x, t = (int)threadIdx.x, w = t >> 6;
+    const int ns = a.ns;
+    for (int is = b; is < ns; is += G) front_body<kPerNT>(a, is, moments, sc, (double *)dyn);
+    grid.sync();
+    if (sc && b == 0 && t < 64) scan_body(a);
+    grid.sync();
+    for (int is = b; is < ns; is += G) main_body<false, kPerNT, kMixDyn>(a, is, sc, dyn);
+    grid.sync();
+    constexpr int SO = StreamGeom<M, REF>::SO;
+    const int nstrip = (a.ng + SO - 1) / SO;
+    {
+        float2(*lrow)[64] = (float2(*)[64])dyn + 4 * w;  // 4 rows of 64 per wavefront
+        float2(*lu1)[64] = lrow + 2;
+        const int x = b & 7, nbx = (G - x + 7) / 8, j = b >> 3;
+        const int nsx = (ns - x + 7) / 8;                // slices x, x + 8, ... of this XCD
+        for (int L = j * (kPerNT / 64) + w; L < nsx * nstrip; L += nbx * (kPerNT / 64))
+            adi_stream_body<M, REF>(a, x + 8 * (L / nstrip), L % nstrip, lrow, lu1);
+    }
+    grid.sync();
+    if (b == 0 && t == 0) slip_body(a);
+    grid.sync();
+    const long long cells = (long long)a.ng * a.ng;
+    const long long nzc = (long long)min(a.st->nzero, maxzero) * cells;
+    for (long long e = (long long)b * kPerNT + t; e < nzc; e += (long long)G * kPerNT)
+        a.field[a.st->cur][(long long)a.st->zero[e / cells] * cells + e % cells] = f2(0, 0);
+    for (int is = b * kPerNT + t; is < ns; is += G * kPerNT) record_body(a, is, nstrip);
+}
+
 template <int M, bool REF>
 int set_adi_attr() {
     using Gm = MixGeom<M, REF>;
@@ -1412,7 +1600,7 @@ bool mix_band_supported(int m, int *tmpl_m) {
 int mix_band_tiles(int ng) {
     const int ntx = (ng + 31) / 32;
     // the tile-diagnostic scratch also holds the strip sums of the streaming kernel
-    return std::max(ntx * ntx, stream_strips(ng, 12));
+    return std::max(ntx * ntx, stream_strips(ng, 12, true));
 }
 
 size_t mix_band_lds(int tmpl_m, bool refine) {
@@ -1423,12 +1611,76 @@ size_t mix_band_lds(int tmpl_m, bool refine) {
     return 0;
 }
 
+template <bool EG, int NT, int DYN>
+int set_main_attr() {
+    return hipFuncSetAttribute((const void *)k_mix_main<EG, NT, DYN>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                               DYN * (int)sizeof(float2)) == hipSuccess;
+}
+
+template <bool EG, int NT, int DYN>
+void launch_main_t(hipStream_t s, const MixArrays &a, int sc) {
+    hipLaunchKernelGGL((k_mix_main<EG, NT, DYN>), dim3(a.ns), dim3(NT), DYN * sizeof(float2), s, a, sc);
+}
+
+template <bool EG>
+void launch_main(hipStream_t s, const MixArrays &a, int sc, int nt, int lds) {
+    if (lds == 1) {
+        if (nt == 256) launch_main_t<EG, 256, kMixDynS>(s, a, sc);
+        else if (nt == 512) launch_main_t<EG, 512, kMixDynS>(s, a, sc);
+        else launch_main_t<EG, 384, kMixDynS>(s, a, sc);
+    } else {
+        if (nt == 256) launch_main_t<EG, 256, kMixDyn>(s, a, sc);
+        else if (nt == 512) launch_main_t<EG, 512, kMixDyn>(s, a, sc);
+        else launch_main_t<EG, 384, kMixDyn>(s, a, sc);
+    }
+}
+
+template <int M, bool REF>
+int set_persistent_attr() {
+    return hipFuncSetAttribute((const void *)k_mix_step_persistent<M, REF>,
+                               hipFuncAttributeMaxDynamicSharedMemorySize, kMixDyn * (int)sizeof(float2)) ==
+           hipSuccess;
+}
+
+template <int M, bool REF>
+int launch_persistent_t(hipStream_t s, const MixArrays &a, bool sc, bool moments, int maxzero) {
+    const void *f = (const void *)k_mix_step_persistent<M, REF>;
+    const size_t dynb = kMixDyn * sizeof(float2);
+    int nb = 0, dev = 0, ncu = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, f, kPerNT, dynb) != hipSuccess || nb < 1) return -1;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || ncu < 1)
+        return -1;
+    MixArrays aa = a;
+    int sci = sc ? 1 : 0, momi = moments ? 1 : 0, mz = maxzero;
+    void *args[] = {&aa, &sci, &momi, &mz};
+    // co-resident workgroups only: the launch fails (no hang) if they do not fit
+    return hipLaunchCooperativeKernel(f, dim3((unsigned)(nb * ncu)), dim3(kPerNT), args, (unsigned)dynb, s) ==
+                   hipSuccess
+               ? 0
+               : -1;
+}
+
+int launch_mix_step_persistent(hipStream_t s, const MixArrays &a, int tmpl_m, bool refine, bool sc, bool moments,
+                               int maxzero, hipEvent_t *ev) {
+    if (ev) (void)hipEventRecord(ev[0], s);
+    int rc = -1;
+#define G4_Q(MM) \
+    if (tmpl_m == MM) rc = refine ? launch_persistent_t<MM, true>(s, a, sc, moments, maxzero) \
+                                  : launch_persistent_t<MM, false>(s, a, sc, moments, maxzero);
+    G4_Q(3) G4_Q(4) G4_Q(6) G4_Q(9) G4_Q(12)
+#undef G4_Q
+    if (ev) (void)hipEventRecord(ev[8], s);
+    return rc;
+}
+
 int mix_prepare() {
-    int ok = hipFuncSetAttribute((const void *)k_mix_main<true>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                 kMixDyn * (int)sizeof(float2)) == hipSuccess;
-    ok &= hipFuncSetAttribute((const void *)k_mix_main<false>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                              kMixDyn * (int)sizeof(float2)) == hipSuccess;
-#define G4_P(MM) ok &= set_adi_attr<MM, false>(); if (MixGeom<MM, true>::lds_bytes <= 160 * 1024) ok &= set_adi_attr<MM, true>();
+    int ok = 1;
+#define G4_M(EG, NT) ok &= set_main_attr<EG, NT, kMixDyn>(); ok &= set_main_attr<EG, NT, kMixDynS>();
+    G4_M(true, 256) G4_M(true, 384) G4_M(true, 512) G4_M(false, 256) G4_M(false, 384) G4_M(false, 512)
+#undef G4_M
+#define G4_P(MM) ok &= set_adi_attr<MM, false>(); if (MixGeom<MM, true>::lds_bytes <= 160 * 1024) ok &= set_adi_attr<MM, true>(); \
+    ok &= set_persistent_attr<MM, false>(); ok &= set_persistent_attr<MM, true>();
     G4_P(3) G4_P(4) G4_P(6) G4_P(9) G4_P(12)
 #undef G4_P
     return ok ? 0 : -1;
@@ -1436,7 +1688,8 @@ int mix_prepare() {
 
 int mix_sc_mode_cells() { return kCelmCells; }
 
-void launch_mix_beam(hipStream_t s, const MixArrays &a, bool sc, bool moments, hipEvent_t *ev, bool egsum) {
+void launch_mix_beam(hipStream_t s, const MixArrays &a, bool sc, bool moments, hipEvent_t *ev, bool egsum,
+                     int main_nt, int main_lds) {
     auto mark = [&](int k) {
         if (ev) (void)hipEventRecord(ev[k], s);
     };
@@ -1445,12 +1698,8 @@ void launch_mix_beam(hipStream_t s, const MixArrays &a, bool sc, bool moments, h
     if (sc) hipLaunchKernelGGL(k_mix_scan, dim3(1), dim3(64), 0, s, a);
     mark(1);
     mark(2);
-    if (egsum)
-        hipLaunchKernelGGL(k_mix_main<true>, dim3(a.ns), dim3(kMainNT), kMixDyn * sizeof(float2), s, a,
-                           sc ? 1 : 0);
-    else
-        hipLaunchKernelGGL(k_mix_main<false>, dim3(a.ns), dim3(kMainNT), kMixDyn * sizeof(float2), s, a,
-                           sc ? 1 : 0);
+    if (egsum) launch_main<true>(s, a, sc ? 1 : 0, main_nt, main_lds);
+    else launch_main<false>(s, a, sc ? 1 : 0, main_nt, main_lds);
     mark(3);
     mark(4);
     mark(5);
@@ -1458,18 +1707,22 @@ void launch_mix_beam(hipStream_t s, const MixArrays &a, bool sc, bool moments, h
 }
 
 int launch_mix_adi(hipStream_t s, const MixArrays &a, int tmpl_m, bool refine, bool stream, int *ntiles) {
-    if (refine && mix_band_lds(tmpl_m, true) > 160 * 1024) return -1;
+    if (refine && !stream && mix_band_lds(tmpl_m, true) > 160 * 1024) return -1;
 #define G4_A(MM)                                         \
     else if (tmpl_m == MM) {                             \
         if (refine) launch_adi_t<MM, true>(s, a);        \
         else launch_adi_t<MM, false>(s, a);              \
     }
     *ntiles = (a.ng + 31) / 32 * ((a.ng + 31) / 32);
-    if (!refine && stream) {
+    if (stream) {
         // streaming kernel: one wavefront per strip
-        *ntiles = stream_strips(a.ng, tmpl_m);
+        *ntiles = stream_strips(a.ng, tmpl_m, refine);
         const unsigned nblocks = (unsigned)(((a.ns + 7) / 8) * 8) * (unsigned)*ntiles;
-#define G4_S(MM) else if (tmpl_m == MM) hipLaunchKernelGGL(k_mix_adi_stream<MM>, dim3(nblocks), dim3(64), 0, s, a);
+#define G4_S(MM)                                                                                       \
+    else if (tmpl_m == MM) {                                                                           \
+        if (refine) hipLaunchKernelGGL((k_mix_adi_stream<MM, true>), dim3(nblocks), dim3(64), 0, s, a); \
+        else hipLaunchKernelGGL((k_mix_adi_stream<MM, false>), dim3(nblocks), dim3(64), 0, s, a);      \
+    }
         if (false) {
         }
         G4_S(3) G4_S(4) G4_S(6) G4_S(9) G4_S(12)
@@ -1496,8 +1749,8 @@ void launch_mix_tail(hipStream_t s, const MixArrays &a, int maxzero, int ntiles,
 }
 
 int launch_mix_step(hipStream_t s, const MixArrays &a, int tmpl_m, bool refine, bool sc,
-                    bool moments, int maxzero, hipEvent_t *ev, bool stream) {
-    launch_mix_beam(s, a, sc, moments, ev, false);
+                    bool moments, int maxzero, hipEvent_t *ev, bool stream, int main_nt, int main_lds) {
+    launch_mix_beam(s, a, sc, moments, ev, false, main_nt, main_lds);
     int ntiles = 0;
     if (launch_mix_adi(s, a, tmpl_m, refine, stream, &ntiles) != 0) return -1;
     launch_mix_tail(s, a, maxzero, ntiles, ev);

@@ -231,6 +231,7 @@ class Context:
         self.ns = len(self.off) - 1
         self.ngrid = ngrid
         self.n = int(self.off[-1])
+        self.precision = precision
         h = C.c_void_p()
         _chk(lib().g4mi_create(C.byref(h), device, self.ns, self.off.ctypes.data_as(_lp), ngrid,
                                gridmax, lambda0, sample, int(timerun), int(periodic), precision),

@@ -104,7 +104,7 @@ def describe(cfg):
                    f"{cfg['sc_ngrid']} radial cells")
     mode = "time-dependent SASE" if cfg["timerun"] else "steady state"
     prec = "FP64 plain form" if cfg.get("precision", "fp64") == "fp64" else \
-        "mixed FP32 form (FP64 slice phase base, FP64 ADI residual correction)"
+        "mixed FP32 form (FP64 slice phase base, FP64 sums)"
     if cfg.get("uneven"):
         cnt, _ = counts_and_currents(cfg)
         parts = (f"{cfg['nslice']} slices with a Gaussian current (peak 4 kA, sigma 1024 slices), "
@@ -115,3 +115,121 @@ def describe(cfg):
             f"(nbins {cfg['nbins']}), {cfg['ngrid']}x{cfg['ngrid']} field, "
             f"planar aw={AW} lambda_u={LAMBDAU} m, 4x125 periods + FODO, "
             + (", ".join(lsc) if lsc else "no space charge"))
+
+
+# ---------------------------------------------------------------- runs
+def build_workload(cfg):
+    """Host side of a configuration: CSR offsets, the quiet-loaded beam (with
+    shot noise for time-dependent runs), the initial field, the currents and
+    the unrolled lattice."""
+    import numpy as np
+    import g4mi
+    ns = cfg["nslice"]
+    counts, currents = counts_and_currents(cfg)
+    off = np.zeros(ns + 1, dtype=np.int64)
+    off[1:] = np.cumsum(counts)
+    slices = [slice_beam(current=c) for c in currents]
+    beam = g4mi.load_beam(off, slices, cfg["nbins"], cfg["shotnoise"], cfg["seed"], LAMBDA_R,
+                          1.0, cfg["timerun"])
+    steps = g4mi.parse_lattice(lattice_text(), "FEL", DELZ, LAMBDA_R, GAMMA0)
+    ng = cfg["ngrid"]
+    field = np.zeros((ns, ng * ng, 2))
+    if cfg["seed_power"] > 0:
+        field[:] = g4mi.load_gauss(ng, cfg["gridmax"], LAMBDA_R, cfg["seed_power"], 30e-6)
+    current = np.asarray(currents, dtype=np.float64)
+    return off, beam, field, current, steps
+
+
+def make_context(cfg, off, beam, field, current, precision):
+    import g4mi
+    ctx = g4mi.Context(off, cfg["ngrid"], cfg["gridmax"], LAMBDA_R, 1.0, cfg["timerun"], False,
+                       precision)
+    ctx.set_efield(cfg["longrange"], cfg["sc_nz"], cfg["sc_nphi"], cfg["sc_ngrid"], 0.0)
+    ctx.upload_beam(beam, current)
+    ctx.upload_field(field, 0)
+    return ctx
+
+
+def run_lattice(ctx, steps, warmup=0, timed=0, breakdown=0, keys=("power", "bunching")):
+    """Runs every step of `steps` with a record after each: `warmup` steps,
+    then `timed` steps bracketed by stream synchronisation, then the rest.
+    Per-kernel-group times come from HIP events: over the timed steps in the
+    FP64 form; in the mixed form (whose timed steps replay captured graphs)
+    over `breakdown` further steps launched directly.  Returns the seconds of
+    the timed steps, the kernel times and the records [len(steps)][ns]."""
+    import time
+    import numpy as np
+    import g4mi
+    nst = len(steps)
+    recs = {k: np.zeros((nst, ctx.ns)) for k in keys}
+    L = g4mi.lib()
+
+    def seg(a, b):
+        if b <= a:
+            return
+        rows = (g4mi.DiagOut * (b - a))()
+        for i in range(a, b):
+            rows[i - a] = g4mi.DiagOut(*[recs[k][i].ctypes.data_as(g4mi._dp) if k in recs else None
+                                         for k in g4mi.DIAG_FIELDS])
+        arr = (g4mi.StepParams * (b - a))(*steps[a:b])
+        done = g4mi._chk(L.g4mi_run(ctx.h, arr, b - a, GAMMA0, rows), "g4mi_run")
+        assert done == b - a, (done, a, b)
+
+    fp64 = ctx.precision == g4mi.FP64
+    seg(0, warmup)
+    ctx.sync()
+    if fp64:
+        ctx.set_timing(True)
+    ctx.sync()
+    t0 = time.perf_counter()
+    seg(warmup, warmup + timed)
+    ctx.sync()
+    t1 = time.perf_counter()
+    nb = 0
+    if not fp64 and breakdown:
+        nb = min(breakdown, nst - warmup - timed)
+        ctx.set_timing(True)
+        seg(warmup + timed, warmup + timed + nb)
+        ctx.sync()
+    kt = ctx.kernel_times()
+    ctx.set_timing(False)
+    seg(warmup + timed + nb, nst)
+    ctx.sync()
+    return dict(seconds=t1 - t0, kt=kt, recs=recs)
+
+
+def far_field_spectrum(field, first):
+    """|FFT over the time window of the on-axis far-field amplitude|^2: the
+    far field at zero angle is the sum of the near field over the grid
+    (DiagField's farfield, Diagnostic.cpp:859-861), per beam slice i in
+    window order (storage slice (i + first) mod ns)."""
+    import numpy as np
+    ns = field.shape[0]
+    ff = field[..., 0].sum(axis=1) + 1j * field[..., 1].sum(axis=1)
+    ff = np.roll(ff, -first)
+    return np.abs(np.fft.fft(ff)) ** 2
+
+
+def form_errors(r_ref, r, f_ref, first_ref, f, first, floor=1e-6):
+    """Errors of a run against a reference run of the same workload:
+    slice-summed P(z) and slice-averaged |b(z)| (max relative error over the
+    steps where the reference's P is above `floor` of its maximum, and at
+    the end), the final field (relative L2 and max-normalised) and the
+    far-field spectrum (relative L2)."""
+    import numpy as np
+    assert first == first_ref, (first, first_ref)
+    p0, p1 = r_ref["power"].sum(axis=1), r["power"].sum(axis=1)
+    mask = p0 > floor * p0.max()
+    b0, b1 = r_ref["bunching"].mean(axis=1), r["bunching"].mean(axis=1)
+    d = f - f_ref
+    s0, s1 = far_field_spectrum(f_ref, first_ref), far_field_spectrum(f, first)
+    return dict(
+        steps=int(len(p0)),
+        p_rel_max=float(np.max(np.abs(p1[mask] / p0[mask] - 1))),
+        p_rel_end=float(abs(p1[-1] / p0[-1] - 1)),
+        b_rel_max=float(np.max(np.abs(b1 / b0 - 1))),
+        b_rel_end=float(abs(b1[-1] / b0[-1] - 1)),
+        field_rel_l2=float(np.linalg.norm(d) / np.linalg.norm(f_ref)),
+        field_rel_max=float(np.max(np.abs(d)) / np.max(np.abs(f_ref))),
+        spectrum_rel_l2=float(np.linalg.norm(s1 - s0) / np.linalg.norm(s0)),
+        p_end_w=float(p0[-1]))

@@ -166,6 +166,61 @@ def test_mixed_graph_replay_matches_direct_launch(st):
         assert e < 1e-4
 
 
+def test_mixed_persistent_step_matches_kernels(st):
+    """The whole step as one persistent cooperative launch (grid barriers
+    between front, scan, main, ADI, slippage, records) against the same
+    steps as separate kernels from the same state: 60 steps of a 40-slice
+    window with long- and short-range space charge and slippage.  Records
+    and final field within FP32 rounding (the deposit and histograms add
+    with LDS atomics whose order varies)."""
+    _log("mixed persistent step vs kernels, 60 steps, 40 slices x 512, 63^2")
+    out = []
+    for pers in (1, 0):
+        sim = H.make_state([512] * 40, ngrid=63, gridmax=0.3e-3, longrange=True, sc=(1, 0, 100, 0.0))
+        ctx = H.to_gpu(sim, precision=g4mi.FP32)
+        ctx.set_schedule("persistent", pers)
+        r = ctx.run(H.gsteps(st[:60]), cases.GAMMA0, record=("power", "bunching"))
+        f, first = ctx.download_field()
+        sched = ctx.schedule()
+        assert sched["persistent"] == pers, sched
+        out.append((r, f, first))
+        ctx.close()
+    (ra, fa, qa), (rb, fb, qb) = out
+    assert qa == qb
+    for k in ("power", "bunching"):
+        e = float(np.max(np.abs(ra[k] - rb[k])) / np.max(np.abs(rb[k])))
+        _log(f"  {k}: max rel diff persistent vs kernels {e:.3e}")
+        assert e < 1e-4
+    ef = float(np.max(np.abs(fa - fb)) / np.max(np.abs(fb)))
+    _log(f"  final field: max diff / max {ef:.3e}")
+    assert ef < 1e-4
+
+
+def test_mixed_autotuner_picks_a_schedule(st):
+    """The first run of 14 or more steps times every main-kernel shape (4, 6,
+    8 wavefronts per slice x 68 or 44 KiB of LDS), both ADI kernels, the
+    graph of kernels and the persistent launch, and keeps the fastest; the
+    run's records stay within 1 % of the FP64 oracle."""
+    _log("mixed autotuner: 20 steps, 12 slices x 1024, 63^2")
+    sim = H.make_state([1024] * 12, ngrid=63, gridmax=0.3e-3, longrange=True, sc=(1, 0, 100, 0.0))
+    ctx = H.to_gpu(sim, precision=g4mi.FP32)
+    n = 20
+    g = ctx.run(H.gsteps(st[:n]), cases.GAMMA0, record=("power", "bunching"))
+    sched = ctx.schedule()
+    _log(f"  schedule {sched}")
+    assert sched["tuned"] == 1
+    tun = sched["tuning"]
+    assert len(tun["main_ms"]) == 6 and all(v > 0 for v in tun["main_ms"].values())
+    assert tun["adi_ms"]["stream"] > 0 and tun["step_ms"]["graph"] > 0
+    assert sched["main_nt"] in (256, 384, 512) and sched["main_lds_kib"] in (44, 68)
+    r = sim.run(st, cases.GAMMA0, record=("power", "bunching"), nsteps=n)
+    pg, pr = g["power"][1:].sum(axis=1), r["power"][1:].sum(axis=1)
+    e = float(np.max(np.abs(pg / pr - 1)))
+    _log(f"  sum P(z) max rel err vs oracle {e:.3e}")
+    assert e < 1e-2
+    ctx.close()
+
+
 def test_mixed_upload_download_and_sort_index():
     """Offset form round trip: gamma and theta come back to FP32 rounding of
     their offsets; the sort target slice equals floor(theta / slen) of the

@@ -85,6 +85,91 @@ def test_field_track_parity(st):
     assert err < 1e-13
 
 
+def test_fused_step_matches_separate_kernels(st):
+    """The FP64 step's particle passes as two fused kernels (front; space
+    charge + push + R56 + transverse + beam sums + deposit) against the
+    separate kernels from the same state: 60 steps (undulator, drift and
+    quadrupole) of an uneven 10-slice SASE window with long- and short-range
+    space charge.  Both are the reference's arithmetic; they differ only in
+    the LDS-atomic order of the histograms and the deposit, so records,
+    particles and field agree to 1e-12, and each matches the oracle."""
+    _log("fused vs separate FP64 particle kernels: 10 uneven slices, LSC, 60 steps")
+    counts = [512, 1024, 256, 2048, 768, 1024, 512, 256, 1536, 1024]
+    out = []
+    for fused in (1, 0):
+        sim = H.make_state(counts, ngrid=63, gridmax=0.3e-3, longrange=True, sc=(1, 0, 100, 0.0),
+                           currents=[1000.0 + 200 * k for k in range(len(counts))])
+        ctx = H.to_gpu(sim)
+        ctx.set_schedule("fp64_fused", fused)
+        idx = list(range(120, 180))
+        r = ctx.run(H.gsteps([st[i] for i in idx]), cases.GAMMA0, record=("power", "bunching"))
+        sched = ctx.schedule()
+        assert sched["fp64_fused"] == fused
+        b = ctx.download_beam()
+        f, first = ctx.download_field()
+        out.append((r, b, f, first))
+        ctx.close()
+    rs = sim.run([st[i] for i in idx], cases.GAMMA0, record=("power", "bunching"))
+    (ra, ba, fa, qa), (rb, bb, fb, qb) = out
+    assert qa == qb == sim.s.first
+    for k in ("power", "bunching"):
+        e = H.rel_err(ra[k], rb[k])
+        eo = H.rel_err(ra[k], rs[k])
+        _log(f"  {k}: fused vs separate {e:.3e}, fused vs oracle {eo:.3e}")
+        assert e < 1e-12 and eo < 1e-10
+    for k in BEAM:
+        e = float(np.max(np.abs(ba[k] - bb[k])) / (np.max(np.abs(bb[k])) + 1e-300))
+        assert e < 1e-12, (k, e)
+    ef = float(np.max(np.abs(fa - fb)) / np.max(np.abs(fb)))
+    _log(f"  final field fused vs separate {ef:.3e}")
+    assert ef < 1e-12
+
+
+def test_deposit_corner_cell_and_mixed_groups(st):
+    """The deposit combines the four lanes of a group when they share a
+    cell.  Groups here: all four in the corner cell (0, 0); one in the
+    corner and three elsewhere; two and two; four different cells; four in
+    one interior cell; and a tail group cut short by the slice's end."""
+    _log("deposit parity: corner cell (0,0) and mixed lane groups, 3 slices")
+    sim = H.make_state([256, 254, 64], seed_power=5e3)
+    # the loader fills whole beamlets of nbins = 4: the last two slots of
+    # the 254-particle slice take the coordinates of the two before them
+    e1 = int(sim.off[2])
+    for k in ("gamma", "theta", "px", "py"):
+        getattr(sim, k)[e1 - 2:e1] = getattr(sim, k)[e1 - 4:e1 - 2]
+    gm, dg = sim.s.gridmax, sim.s.dgrid
+    corner = -gm + 0.3 * dg                    # inside cell 0
+    rng = np.random.default_rng(21)
+    n = int(sim.off[-1])
+    x = rng.normal(size=n) * 2e-5
+    y = rng.normal(size=n) * 2e-5
+    cells = [(corner, corner)] * 4, [(corner, corner), (0.0, 0.0), (1e-5, 0.0), (0.0, 2e-5)], \
+        [(corner, corner), (corner, corner), (3e-5, 3e-5), (3e-5, 3e-5)], \
+        [(0.0, 0.0), (dg, 0.0), (0.0, dg), (dg, dg)], [(2.1e-6, -1.3e-6)] * 4
+    for a in range(sim.ns):
+        base = int(sim.off[a])
+        for gi, grp in enumerate(cells):
+            for k, (px_, py_) in enumerate(grp):
+                x[base + 4 * gi + k] = px_ + 1e-9 * k
+                y[base + 4 * gi + k] = py_ + 1e-9 * k
+    sim.x[:], sim.y[:] = x, y
+    ctx = H.to_gpu(sim)
+    gst = H.gsteps(st)
+    for i in range(2):
+        O.lib().orc_field_advance(C.byref(sim.s), C.byref(st[i]), st[i].dz)
+        ctx.field_track(gst[i], st[i].dz)
+    f, first = ctx.download_field()
+    err = np.max(np.abs(f - sim.field)) / np.max(np.abs(sim.field))
+    ng = sim.s.ngrid
+    cidx = [iy * ng + ix for iy in range(3) for ix in range(3)]    # the corner's 3x3 cells
+    cref = sim.field[:, cidx]
+    assert np.max(np.abs(cref)) > 0
+    corner_err = np.max(np.abs(f[:, cidx] - cref)) / np.max(np.abs(cref))
+    _log(f"  field max diff / max |E| = {err:.3e}, corner cell {corner_err:.3e}")
+    assert err < 1e-13 and corner_err < 1e-12
+    ctx.close()
+
+
 @pytest.mark.parametrize("kernel", [0, 1, 2, 3])
 @pytest.mark.parametrize("delz", [0.015, 0.15, 5.0])
 def test_adi_kernels_and_regimes(delz, kernel):
