@@ -14,6 +14,7 @@
 #include <vector>
 
 #include <map>
+#include <mutex>
 #include <tuple>
 
 #include "g4mi_host.h"
@@ -1908,6 +1909,42 @@ static int mix_alloc(Ctx *c) {
 
 }  // namespace g4mi
 
+namespace g4mi {
+// ---------------------------------------------------------------------------
+// Scratch of the allocation-free entry point (g4mi_sort_index_noalloc): a
+// file-scope device array, one copy per GPU at its own address, held until
+// the process ends.  Layout: the moved-particle counter (8 bytes) at 0, the
+// target slice of every particle (int) from kScratchAtar on.  Host code
+// never passes `scratch` itself to a kernel (on the host it names a
+// host-side stand-in): it uses the per-device address that
+// hipGetSymbolAddress returns, taken once per device at context creation.
+// ---------------------------------------------------------------------------
+constexpr long long kNoAllocMaxParticles = 1LL << 22;   // 4194304 particles: 16 MiB of targets
+constexpr size_t kScratchAtar = 256;
+constexpr size_t kScratchBytes = kScratchAtar + (size_t)kNoAllocMaxParticles * sizeof(int);
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+constexpr int kMaxDevices = 64;
+static std::mutex g_scratch_mu;
+static void *g_scratch_ptr[kMaxDevices];          // device address of `scratch`, per device
+static hipError_t g_scratch_err[kMaxDevices];     // hipGetSymbolAddress status, per device
+
+// Once per device, with that device current.
+static void scratch_setup(int device) {
+    if (device < 0 || device >= kMaxDevices) return;
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (g_scratch_ptr[device]) return;
+    void *p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    g_scratch_err[device] = e;
+    if (e == hipSuccess) g_scratch_ptr[device] = p;
+}
+
+__global__ void k_clear_u64(unsigned long long *p) {
+    if (threadIdx.x == 0) *p = 0ULL;
+}
+}  // namespace g4mi
+
 using namespace g4mi;
 
 struct g4mi_ctx {
@@ -1944,6 +1981,7 @@ int g4mi_create(g4mi_ctx **out, int device, int nslice, const long long *off, in
         return G4MI_E_NODEV;
     }
     G4_HIP(hipSetDevice(device));
+    scratch_setup(device);  // the allocation-free entry point's scratch address on this device
     g4mi_ctx *h = new g4mi_ctx;
     Ctx *c = &h->c;
     *out = nullptr;
@@ -2525,6 +2563,44 @@ int g4mi_sort_index(g4mi_ctx *h, double slen, int *atar, long long *moved) {
     if (atar && c->ntot)
         G4_HIP(hipMemcpyAsync(atar, c->d_atar, c->ntot * sizeof(int), hipMemcpyDeviceToHost, c->stream));
     G4_HIP(hipMemcpyAsync(&m, c->d_count, sizeof(m), hipMemcpyDeviceToHost, c->stream));
+    G4_HIP(hipStreamSynchronize(c->stream));
+    if (moved) *moved = (long long)m;
+    return G4MI_OK;
+}
+
+// g4mi_sort_index without any allocation: the target indices and the
+// counter live in the library's static device scratch (see `scratch`).
+// Calls that overlap on one GPU share that scratch and race.
+int g4mi_sort_index_noalloc(g4mi_ctx *h, double slen, int *atar, long long *moved) {
+    if (!h || slen == 0) return G4MI_E_ARG;
+    Ctx *c = &h->c;
+    if (c->ntot > kNoAllocMaxParticles) {
+        set_error("g4mi_sort_index_noalloc: " + std::to_string(c->ntot) +
+                  " particles exceed the static scratch's maximum of " +
+                  std::to_string(kNoAllocMaxParticles) + "; use g4mi_sort_index");
+        return G4MI_E_ARG;
+    }
+    if (c->device < 0 || c->device >= kMaxDevices || !g_scratch_ptr[c->device]) {
+        set_error(std::string("g4mi_sort_index_noalloc: no scratch address on this device: ") +
+                  hipGetErrorString(c->device >= 0 && c->device < kMaxDevices ? g_scratch_err[c->device]
+                                                                               : hipErrorInvalidDevice));
+        return G4MI_E_HIP;
+    }
+    char *base = (char *)g_scratch_ptr[c->device];
+    auto *cnt = (unsigned long long *)base;
+    int *d_at = (int *)(base + kScratchAtar);
+    // the counter starts at zero: a fill kernel, not a memset node, so the
+    // sequence stays valid under stream capture
+    hipLaunchKernelGGL(k_clear_u64, dim3(1), dim3(64), 0, c->stream, cnt);
+    if (c->precision == G4MI_FP32)
+        launch_mix_sort_index(c->stream, mix_arrays(c), 1. / slen, d_at, cnt);
+    else
+        launch_sort_index(c->stream, c->ntot, c->d_th, 1. / slen, d_at, cnt);
+    G4_HIP(hipGetLastError());
+    unsigned long long m = 0;
+    if (atar && c->ntot)
+        G4_HIP(hipMemcpyAsync(atar, d_at, (size_t)c->ntot * sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    G4_HIP(hipMemcpyAsync(&m, cnt, sizeof(m), hipMemcpyDeviceToHost, c->stream));
     G4_HIP(hipStreamSynchronize(c->stream));
     if (moved) *moved = (long long)m;
     return G4MI_OK;

@@ -112,6 +112,7 @@ def lib():
                                C.POINTER(DiagOut)]
         L.g4mi_diag.argtypes = [C.c_void_p, C.POINTER(DiagOut)]
         L.g4mi_sort_index.argtypes = [C.c_void_p, C.c_double, _ip, _lp]
+        L.g4mi_sort_index_noalloc.argtypes = [C.c_void_p, C.c_double, _ip, _lp]
         L.g4mi_sync.argtypes = [C.c_void_p]
         L.g4mi_set_timing.argtypes = [C.c_void_p, C.c_int]
         L.g4mi_set_schedule.argtypes = [C.c_void_p, C.c_char_p, C.c_int]
@@ -413,6 +414,15 @@ class Context:
         moved = C.c_longlong(0)
         _chk(lib().g4mi_sort_index(self.h, slen, atar.ctypes.data_as(_ip), C.byref(moved)),
              "g4mi_sort_index")
+        return atar, moved.value
+
+    def sort_index_noalloc(self, slen):
+        """g4mi_sort_index_noalloc: the same targets from the library's static
+        device scratch (at most 4194304 particles; no allocation)."""
+        atar = np.zeros(self.n, dtype=np.int32)
+        moved = C.c_longlong(0)
+        _chk(lib().g4mi_sort_index_noalloc(self.h, slen, atar.ctypes.data_as(_ip), C.byref(moved)),
+             "g4mi_sort_index_noalloc")
         return atar, moved.value
 
     def set_schedule(self, name, value):

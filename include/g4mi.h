@@ -228,6 +228,15 @@ int g4mi_diag(g4mi_ctx *ctx, g4mi_diag_out *out);
  * with atar != 0 in *moved. atar is a host array of total-particle length. */
 int g4mi_sort_index(g4mi_ctx *ctx, double slen, int *atar, long long *moved);
 
+/* No counterpart in the reference: g4mi_sort_index for callers that cannot
+ * allocate device memory (code under stream capture, tight serving loops).
+ * Same inputs, outputs and results; it calls no allocator: the targets are
+ * staged and counted in the library's static device scratch (16 MiB per
+ * GPU, held until the process ends), so the beam may hold at most 4194304
+ * particles (larger beams get G4MI_E_ARG).  Calls that overlap on one GPU
+ * share that scratch and race. */
+int g4mi_sort_index_noalloc(g4mi_ctx *ctx, double slen, int *atar, long long *moved);
+
 /* Kernel schedule.  The first field step of a context times every ADI
  * kernel eligible for its grid and step (two-kernel Thomas, fused
  * recurrence with and without prefetch, banded inverse) on a scratch copy
