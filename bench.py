@@ -40,7 +40,7 @@ make_context = W.make_context
 
 
 def gpu_measure(cfg, steps_k, warmup, adi_kernel=-1, precision=g4mi.FP64, breakdown=20, refine=None,
-                mixed_adi=None, fft=False, work=None, whole_lattice=False):
+                mixed_adi=None, fft=False, work=None, whole_lattice=False, sched=()):
     """Times `steps_k` steps after `warmup` on a context of `cfg` in the
     given precision.  With whole_lattice the rest of the lattice runs after
     the timed steps (untimed) so the records and the final field cover the
@@ -58,6 +58,9 @@ def gpu_measure(cfg, steps_k, warmup, adi_kernel=-1, precision=g4mi.FP64, breakd
             ctx.set_schedule("adi_refine", refine)
         if mixed_adi is not None:
             ctx.set_schedule("mixed_adi", mixed_adi)
+    for key, val, form in sched:
+        if form is None or form == precision:
+            ctx.set_schedule(key, val)
     nst = len(steps)
     extra = breakdown if precision == g4mi.FP32 else 0
     if warmup + steps_k + extra > nst:
@@ -441,6 +444,9 @@ def main():
                     help="mixed form: FP64 residual correction in the ADI (default: the library's)")
     ap.add_argument("--mixed-adi", type=int, default=None, choices=(0, 1),
                     help="mixed form: 0 tile ADI kernel, 1 streaming ADI kernel (default: the library's)")
+    ap.add_argument("--set", action="append", default=[], metavar="[FORM:]KEY=VAL",
+                    help="a schedule entry (g4mi_set_schedule) for the measured contexts, e.g. "
+                         "fp32:main_nt=384 or fp64:adi_kernel=3; repeatable")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB five times (the reference "
                          "point that scales the FETCH_SIZE/WRITE_SIZE counters) and exit")
@@ -475,8 +481,13 @@ def main():
     work = build_workload(cfg)
     log(f"workload {cfg['name']}: {int(work[0][-1])} macroparticles loaded in "
         f"{time.perf_counter() - t0:.1f} s")
+    sched = []
+    for item in a.set:
+        form, _, kv = item.rpartition(":")
+        key, _, val = kv.partition("=")
+        sched.append((key, int(val), {"fp32": g4mi.FP32, "fp64": g4mi.FP64}.get(form)))
     kw = dict(adi_kernel=a.adi_kernel, refine=a.adi_refine, mixed_adi=a.mixed_adi,
-              fft=a.field_solver == "fft", work=work, whole_lattice=compare)
+              fft=a.field_solver == "fft", work=work, whole_lattice=compare, sched=sched)
     ref = None
     if compare:
         ref = gpu_measure(cfg, a.steps, a.warmup, precision=g4mi.FP64, **kw)

@@ -131,6 +131,7 @@ struct Ctx {
     bool fuse64 = true;
     double *d_beamsum = nullptr;           // beam sums of the last fused step [8 ns]
     int4 *d_pbox = nullptr;                // in-grid footprint per slice after the first half step
+    double *d_pmaxf = nullptr;             // per slice: largest sqrt(faw2) / gamma in the grid (deposit scale)
     bool beamsum_fresh = false;            // d_beamsum describes the current particles
     SortWork sw{};
     long long *d_off_new = nullptr;
@@ -192,6 +193,10 @@ struct Ctx {
     float2 *m_judge = nullptr;
     double *d_judge = nullptr;
     long long fno_njudge = 0, fno_nres_all = 0;
+    // replace mode backs off after kFnoRejectRun rejections in a row: the
+    // next kFnoSkip steps run the ADI without inference
+    int fno_reject_run = 0, fno_skip = 0;
+    long long fno_skipped = 0;
     double fno_judge_field = 0, fno_judge_update = 0, fno_res_all_sum = 0;
 };
 
@@ -1104,17 +1109,6 @@ static int fused_step(Ctx *c, const g4mi_step_params &st, double gamma0, double 
     const bool sc = (c->sc_nz > 0) && (c->sc_ng > 2);
     const bool moments = c->longrange || sc;
     TransParams t1 = transverse_params(c, st, gamma0, delz * 0.5, false);
-    if (c->ev) G4_HIP(hipEventRecord(c->ev[0], c->stream));
-    launch_fp64_front(c->stream, c->ns, c->d_off, c->d_g, c->d_x, c->d_y, c->d_px, c->d_py, c->d_cur, t1, moments,
-                      sc, c->gridmax, c->dgrid, c->ng, c->d_mom, c->d_fsize, c->d_cen, c->d_rbound, c->d_pbox);
-    if (c->ev) G4_HIP(hipEventRecord(c->ev[1], c->stream));
-    if (c->longrange) {
-        const double gamma = gamma0 / sqrt(1 + aw * aw);
-        const double scl = c->slicelength / 2. / asin(1) / 299792458.0 / 2 / 8.85e-12;
-        launch_lsc_long(c->stream, c->ns, c->d_cur, c->d_fsize, c->slicelength, gamma, scl, c->d_esc);
-    }
-    if (sc) launch_sc_scan(c->stream, c->ns, c->d_off, c->d_rbound, c->d_rmax, c->sc_ng, c->d_dr);
-    if (c->ev) G4_HIP(hipEventRecord(c->ev[2], c->stream));
     FuseParams f{};
     PushParams &p = f.p;
     p.delz = delz;
@@ -1162,9 +1156,21 @@ static int fused_step(Ctx *c, const g4mi_step_params &st, double gamma0, double 
     f.nphi = c->sc_nphi;
     f.ks = 4 * asin(1) / c->lambda0;
     f.gz2 = gamma0 * gamma0 / (1 + aw * aw);
+    if (c->ev) G4_HIP(hipEventRecord(c->ev[0], c->stream));
+    launch_fp64_front(c->stream, c->ns, c->d_off, c->d_g, c->d_x, c->d_y, c->d_px, c->d_py, c->d_cur, t1, moments,
+                      sc, c->gridmax, c->dgrid, c->ng, c->d_mom, c->d_fsize, c->d_cen, c->d_rbound, c->d_pbox,
+                      f.d, c->d_pmaxf);
+    if (c->ev) G4_HIP(hipEventRecord(c->ev[1], c->stream));
+    if (c->longrange) {
+        const double gamma = gamma0 / sqrt(1 + aw * aw);
+        const double scl = c->slicelength / 2. / asin(1) / 299792458.0 / 2 / 8.85e-12;
+        launch_lsc_long(c->stream, c->ns, c->d_cur, c->d_fsize, c->slicelength, gamma, scl, c->d_esc);
+    }
+    if (sc) launch_sc_scan(c->stream, c->ns, c->d_off, c->d_rbound, c->d_rmax, c->sc_ng, c->d_dr);
+    if (c->ev) G4_HIP(hipEventRecord(c->ev[2], c->stream));
     launch_fp64_main(c->stream, c->ns, c->d_off, c->d_g, c->d_th, c->d_x, c->d_y, c->d_px, c->d_py, c->d_field,
-                     c->d_esc, c->d_cur, c->d_cen, c->d_dr, c->d_efield, c->d_pbox, c->d_src, c->d_box,
-                     c->d_beamsum, f);
+                     c->d_esc, c->d_cur, c->d_cen, c->d_dr, c->d_efield, c->d_pbox, c->d_pmaxf, c->d_src,
+                     c->d_box, c->d_beamsum, f);
     if (c->ev) {
         G4_HIP(hipEventRecord(c->ev[3], c->stream));
         G4_HIP(hipEventRecord(c->ev[4], c->stream));
@@ -1458,7 +1464,26 @@ static int mix_step_fno(Ctx *c, const MixKey &k) {
         G4_HIP(hipGetLastError());
         return G4MI_OK;
     }
-    // replace mode
+    // replace mode; after kFnoRejectRun residual rejections in a row the
+    // next kFnoSkip steps run the ADI alone (no inference to pay for)
+    constexpr int kFnoRejectRun = 4, kFnoSkip = 8;
+    {
+        const long long stepno0 = c->fno_steps + c->fno_adi_steps;
+        const bool judge0 = c->fno_every > 0 && stepno0 % c->fno_every == 0;
+        if (c->fno_skip > 0 && !judge0) {
+            c->fno_skip--;
+            c->fno_skipped++;
+            c->fno_adi_steps++;
+            launch_mix_beam(c->stream, a, sc, moments, nullptr, false, std::get<6>(k), std::get<7>(k));
+            if (launch_mix_adi(c->stream, a, tm, refine, stream, &ntiles)) {
+                set_error("mixed form: no ADI instantiation for this band width");
+                return G4MI_E_ARG;
+            }
+            launch_mix_tail(c->stream, a, maxzero, ntiles, nullptr);
+            G4_HIP(hipGetLastError());
+            return G4MI_OK;
+        }
+    }
     const double ks = 4. * asin(1) / c->xlambda;
     const double scl = c->dgrid * kEev / ks, pw = scl * scl / kVacimp;
     std::vector<double> hf0(8 * (size_t)ns), hf1(8 * (size_t)ns), eg(ns), ss(ns);
@@ -1494,6 +1519,16 @@ static int mix_step_fno(Ctx *c, const MixKey &k) {
     const long long stepno = c->fno_steps + c->fno_adi_steps;
     const bool judge = c->fno_every > 0 && stepno % c->fno_every == 0;
     const bool use_adi = judge || !(res <= c->fno_thr);
+    if (!judge) {
+        if (use_adi) {
+            if (++c->fno_reject_run >= kFnoRejectRun) {
+                c->fno_skip = kFnoSkip;
+                c->fno_reject_run = 0;
+            }
+        } else {
+            c->fno_reject_run = 0;
+        }
+    }
     if (use_adi) {
         if (judge) {
             if (!c->m_judge) {
@@ -2024,7 +2059,7 @@ int g4mi_create(g4mi_ctx **out, int device, int nslice, const long long *off, in
         (rc = dalloc(c, &c->d_box, ns)) || (rc = dalloc(c, &c->d_fac, 3 * (size_t)ngrid)) ||
         (rc = dalloc(c, &c->d_diag_b, 8 * ns)) || (rc = dalloc(c, &c->d_diag_f, 8 * ns)) ||
         (rc = dalloc(c, &c->d_count, 1)) || (rc = dalloc(c, &c->d_beamsum, 8 * ns)) ||
-        (rc = dalloc(c, &c->d_pbox, ns)))
+        (rc = dalloc(c, &c->d_pbox, ns)) || (rc = dalloc(c, &c->d_pmaxf, ns)))
         return fail(rc);
     if (!fp64 && (rc = mix_alloc(c))) return fail(rc);
     if (fp64 && (rc = alloc_records(c))) return fail(rc);
@@ -2671,7 +2706,7 @@ int g4mi_set_schedule(g4mi_ctx *h, const char *name, int value) {
 int g4mi_get_schedule(g4mi_ctx *h, char *buf, int len) {
     if (!h || !buf || len < 1) return G4MI_E_ARG;
     Ctx *c = &h->c;
-    char tmp[1024];
+    char tmp[4096];
     static const char *names[4] = {"thomas", "fused", "fused-prefetch", "band"};
     if (c->precision == G4MI_FP32) {
         std::string bands = "[";
@@ -2682,10 +2717,11 @@ int g4mi_get_schedule(g4mi_ctx *h, char *buf, int len) {
                  "{\"precision\": \"fp32\", \"adi\": \"%s\", \"adi_refine\": %d, "
                  "\"main_nt\": %d, \"main_lds_kib\": %d, \"persistent\": %d, \"autotune\": %d, \"tuned\": %d, "
                  "\"graphs\": %d, \"bands_m_over_template\": \"%s\", \"graphs_captured\": %d, "
-                 "\"tuning\": %s}",
+                 "\"tuning\": %s, \"fno_skipped_steps\": %lld}",
                  c->m_stream ? "band32-stream" : "band32-tile", c->m_refine ? 1 : 0, c->m_main_nt,
                  c->m_main_lds ? 44 : 68, c->m_persistent ? 1 : 0, c->m_autotune ? 1 : 0, c->m_tuned ? 1 : 0, c->m_graphs ? 1 : 0,
-                 bands.c_str(), (int)c->m_graph.size(), c->m_tune_log.empty() ? "null" : c->m_tune_log.c_str());
+                 bands.c_str(), (int)c->m_graph.size(), c->m_tune_log.empty() ? "null" : c->m_tune_log.c_str(),
+                 c->fno_skipped);
         snprintf(buf, (size_t)len, "%s", tmp);
         return G4MI_OK;
     }
@@ -2735,6 +2771,8 @@ int g4mi_attach_fno(g4mi_ctx *h, g4mi_fno *fh, int mode, int every, double thres
     c->fno_every = every;
     c->fno_thr = threshold;
     c->fno_steps = c->fno_adi_steps = c->fno_nres_adi = c->fno_njudge = c->fno_nres_all = 0;
+    c->fno_reject_run = c->fno_skip = 0;
+    c->fno_skipped = 0;
     c->fno_res_sum = c->fno_res_adi_sum = c->fno_judge_field = c->fno_judge_update = c->fno_res_all_sum = 0;
     return G4MI_OK;
 }

@@ -42,12 +42,14 @@ void launch_push(hipStream_t s, int nblocks, const int *blk_slice, const long lo
                  const double *ezsc, const PushParams &p);
 // The step's particle passes in two kernels (one workgroup per slice).
 // front: first transverse half step, moments mom[4 ns] (the same bits as
-// launch_slice_moments), fsize, cen, rbound (when sc) and the in-grid
-// footprint box pbox[ns] of each slice.
+// launch_slice_moments), fsize, cen, rbound (when sc), the in-grid
+// footprint box pbox[ns] of each slice and pmaxf[ns], the largest
+// sqrt(faw2) / gamma of its in-grid particles (the deposit's fixed-point
+// scale).
 void launch_fp64_front(hipStream_t s, int ns, const long long *off, const double *g, double *x, double *y,
                        double *px, double *py, const double *cur, const TransParams &t1, bool moments, bool sc,
                        double gridmax, double dgrid, int ng, double *mom, double *fsize, double *cen,
-                       double *rbound, int4 *pbox);
+                       double *rbound, int4 *pbox, const DepositParams &d, double *pmaxf);
 // Parameters of the main kernel: the push (fundamental only), the second
 // transverse half step, R56, the short-range solver and the deposit.
 struct FuseParams {
@@ -65,8 +67,8 @@ int fp64_main_modes();
 // slice's source (src, box) when f.d.enabled.
 void launch_fp64_main(hipStream_t s, int ns, const long long *off, double *g, double *th, double *x, double *y,
                       double *px, double *py, const double2 *field, const double *esc, const double *cur,
-                      const double *cen, const double *dr, double *efield, const int4 *pbox, double2 *src,
-                      int4 *box, double *beamsum, const FuseParams &f);
+                      const double *cen, const double *dr, double *efield, const int4 *pbox,
+                      const double *pmaxf, double2 *src, int4 *box, double *beamsum, const FuseParams &f);
 void launch_deposit(hipStream_t s, int ns, const long long *off, const double *g,
                     const double *th, const double *x, const double *y, const double *cur,
                     double2 *src, int4 *box, const DepositParams &p);

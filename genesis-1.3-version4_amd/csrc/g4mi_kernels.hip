@@ -148,6 +148,27 @@ __global__ void __launch_bounds__(256) k_r56(long long n, const double *__restri
 // src/Core/Beam.cpp:42-67; centroid of EFieldSolver::analyseBeam :137-145).
 // mom[4*is + {0,1,2,3}] = sum x, sum x^2, sum y, sum y^2.
 // ---------------------------------------------------------------------------
+// Fixed-point accumulation in LDS.  A value v goes in as the integer
+// round(v s) with s a power of two, added with 64-bit integer atomics:
+// integer addition is associative, so a sum does not depend on the order in
+// which the lanes' atomics land, and the FP64 step is bit-reproducible run
+// to run.  s is the largest power of two with bound * s <= 2^61, where
+// bound caps the magnitude of any partial sum; the sum is rounded once,
+// when it is read back as sum / s.  The quantum is bound * 2^-61, below the
+// FP64 rounding of a sum of that size.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double fx_scale(double bound) {
+    if (!(bound > 0) || !(bound < 1e300)) return 1.0;
+    int e;
+    (void)frexp(bound, &e);  // bound < 2^e
+    return ldexp(1.0, 61 - e);
+}
+__device__ __forceinline__ long long fx(double v, double s) { return __double2ll_rn(v * s); }
+__device__ __forceinline__ void fx_add(long long *cell, long long v) {
+    atomicAdd((unsigned long long *)cell, (unsigned long long)v);
+}
+
+// ---------------------------------------------------------------------------
 __global__ void __launch_bounds__(256) k_slice_moments(const long long *__restrict__ off,
                                                        const double *__restrict__ x,
                                                        const double *__restrict__ y,
@@ -288,14 +309,15 @@ __global__ void __launch_bounds__(64) k_sc_scan(int ns, const long long *__restr
 // K3b: short-range space charge of one slice per workgroup,
 // EFieldSolver::shortRange (:197-258) with constructLaplaceOperator
 // (:176-190) and tridiag (:260-272).  Radial charge histograms are private
-// per wavefront and summed in wave order, so the result is deterministic.
+// per wavefront and accumulate in fixed point (fx_add), so the result is
+// bit-reproducible.
 constexpr int kScMaxNg = 256;
 __global__ void __launch_bounds__(256) k_sc_solve(
     const long long *__restrict__ off, const double *__restrict__ x, const double *__restrict__ y,
     const double *__restrict__ th, const double *__restrict__ cur, const double *__restrict__ cen,
     const double *__restrict__ dr_arr, int ngr, int nz, int nphi, double ks, double gz2,
     double *__restrict__ ez, double *__restrict__ efield) {
-    __shared__ double2 hist[4][kScMaxNg];
+    __shared__ long long hist[4][kScMaxNg][2];
     __shared__ double2 celm[kScMaxNg], gam[kScMaxNg], rhs[kScMaxNg];
     __shared__ double vol[kScMaxNg], ldig[kScMaxNg + 1], rlog[kScMaxNg];
     __shared__ double cupp[kScMaxNg], clow[kScMaxNg], cmid[kScMaxNg];
@@ -323,10 +345,11 @@ __global__ void __launch_bounds__(256) k_sc_solve(
     }
     const double coef = -gz2 / ks / ks;
     const double econst = kVacimp / kEev * cur[is] / ks / (double)npart;
+    const double hs = fx_scale((double)npart);  // |cos|, |sin| <= 1: a bin sums at most npart of them
     for (int m = -nphi; m <= nphi; m++) {
         for (int l = 1; l <= nz; l++) {
             __syncthreads();
-            for (int k = t; k < 4 * kScMaxNg; k += 256) hist[k / kScMaxNg][k % kScMaxNg] = make_double2(0, 0);
+            for (int k = t; k < 4 * kScMaxNg; k += 256) hist[k / kScMaxNg][k % kScMaxNg][0] = hist[k / kScMaxNg][k % kScMaxNg][1] = 0;
             __syncthreads();
             for (long long i = a + t; i < b; i += 256) {
                 double tx = x[i] - xc, ty = y[i] - yc;
@@ -338,8 +361,8 @@ __global__ void __launch_bounds__(256) k_sc_solve(
                 double2 v = make_double2(c, -s);
                 if (m != 0)  // cwork^0 is exactly 1: the azimuthal factor only for m != 0
                     v = cmul(cpow_int(make_double2(tx / radi, ty / radi), -m), v);
-                unsafeAtomicAdd(&hist[w][bin].x, v.x);
-                unsafeAtomicAdd(&hist[w][bin].y, v.y);
+                fx_add(&hist[w][bin][0], fx(v.x, hs));
+                fx_add(&hist[w][bin][1], fx(v.y, hs));
             }
             __syncthreads();
             // operator coefficients (:240-245) and source of every radial cell,
@@ -352,7 +375,9 @@ __global__ void __launch_bounds__(256) k_sc_solve(
                     cupp[i] = coef * ldig[i] / l / l / vol[i - 1];
                     clow[i] = coef * ldig[i] / l / l / vol[i];
                 }
-                const double2 cs = cadd(cadd(cadd(hist[0][i], hist[1][i]), hist[2][i]), hist[3][i]);
+                const double2 cs = make_double2(
+                    (double)(hist[0][i][0] + hist[1][i][0] + hist[2][i][0] + hist[3][i][0]) / hs,
+                    (double)(hist[0][i][1] + hist[1][i][1] + hist[2][i][1] + hist[3][i][1]) / hs);
                 rhs[i] = cmul(cs, make_double2(0, econst / l / vol[i]));
             }
             __syncthreads();
@@ -527,8 +552,8 @@ __global__ void __launch_bounds__(256) k_push(const int *__restrict__ blk_slice,
 // K4: source deposition of one beam slice per workgroup,
 // FieldSolverADI::advance (src/Core/FieldSolverADI.cpp:13-51).  The slice's
 // footprint (bounding box of its in-grid cells + 1) is accumulated in an LDS
-// tile with FP64 LDS atomics, in row bands when it exceeds the tile, and
-// written to the slice's source plane.  box[is] = {ix0, iy0, ix1, iy1}
+// tile in fixed point (fx_add: bit-reproducible), in row bands when it
+// exceeds the tile, and written to the slice's source plane.  box[is] = {ix0, iy0, ix1, iy1}
 // (inclusive) tells the ADI where the source is non-zero; the plane outside
 // the box is never read, so it is never cleared.
 // ---------------------------------------------------------------------------
@@ -541,12 +566,15 @@ __global__ void __launch_bounds__(1024) k_deposit(const long long *__restrict__ 
                                                   const double *__restrict__ cur,
                                                   double2 *__restrict__ src,
                                                   int4 *__restrict__ box, DepositParams p) {
-    extern __shared__ double2 tile[];
+    extern __shared__ double2 tile_mem[];
+    long long *tile = (long long *)tile_mem;  // [cells][2] fixed-point sums
     __shared__ int red[4][1024];
+    __shared__ double redd[1024];
     const int is = blockIdx.x, t = threadIdx.x, nt = blockDim.x;
     const long long a = off[is], b = off[is + 1];
     const int ng = p.ngrid;
     int bx0 = 1 << 30, by0 = 1 << 30, bx1 = -1, by1 = -1;
+    double pmax = 0;  // largest sqrt(faw2) / gamma of an in-grid particle
     if (p.enabled) {
         for (long long i = a + t; i < b; i += nt) {
             double xi = x[i], yi = y[i];
@@ -557,6 +585,9 @@ __global__ void __launch_bounds__(1024) k_deposit(const long long *__restrict__ 
                 bx1 = max(bx1, ix + 1);
                 by0 = min(by0, iy);
                 by1 = max(by1, iy + 1);
+                const double ddx = xi - p.ax, ddy = yi - p.ay;
+                const double faw2 = (1 + p.kx * ddx * ddx + p.ky * ddy * ddy + 2 * (p.gradx * ddx + p.grady * ddy));
+                pmax = fmax(pmax, sqrt(faw2) / fabs(g[i]));
             }
         }
     }
@@ -564,16 +595,20 @@ __global__ void __launch_bounds__(1024) k_deposit(const long long *__restrict__ 
     red[1][t] = by0;
     red[2][t] = -bx1;
     red[3][t] = -by1;
+    redd[t] = pmax;
     __syncthreads();
     for (int s = nt / 2; s > 0; s >>= 1) {
-        if (t < s)
+        if (t < s) {
             for (int k = 0; k < 4; k++) red[k][t] = min(red[k][t], red[k][t + s]);
+            redd[t] = fmax(redd[t], redd[t + s]);
+        }
         __syncthreads();
     }
     bx0 = red[0][0];
     by0 = red[1][0];
     bx1 = -red[2][0];
     by1 = -red[3][0];
+    pmax = redd[0];
     __syncthreads();
     if (bx1 < 0) {  // nothing deposited
         if (t == 0) box[is] = make_int4(1, 1, 0, 0);
@@ -586,11 +621,13 @@ __global__ void __launch_bounds__(1024) k_deposit(const long long *__restrict__ 
     const int rows_per_band = max(1, kDepTile / ww);
     const double scl0 = p.fcoup * kVacimp * cur[is] * p.xks * p.delz;
     const double scl = scl0 / (4 * kEev * (double)(b - a) * p.dgrid * p.dgrid);
+    // no cell sum exceeds sum_i |part_i| <= n max|part|
+    const double fs = fx_scale((double)(b - a) * pmax * fabs(scl));
     double2 *plane = src + (long long)is * ng * ng;
     for (int r0 = by0; r0 <= by1; r0 += rows_per_band) {
         const int r1 = min(by1, r0 + rows_per_band - 1);
         const int ncell = (r1 - r0 + 1) * ww;
-        for (int k = t; k < ncell; k += nt) tile[k] = make_double2(0, 0);
+        for (int k = t; k < 2 * ncell; k += nt) tile[k] = 0;
         __syncthreads();
         // The trip count is the same for every lane (inactive lanes carry
         // zeros), so the lanes of a group of four can combine their values.
@@ -638,28 +675,29 @@ __global__ void __launch_bounds__(1024) k_deposit(const long long *__restrict__ 
             const int key = act ? iy * ng + ix : -1 - (t & 3);
             const int k1 = __shfl_xor(key, 1), k2 = __shfl_xor(key, 2), k3 = __shfl_xor(key, 3);
             const bool grp = (key == k1) & (key == k2) & (key == k3);
-            double vs[8];
+            long long iv[8], ivs[8];
 #pragma unroll
             for (int q = 0; q < 8; q++) {
-                vs[q] = v[q] + __shfl_xor(v[q], 1);
-                vs[q] += __shfl_xor(vs[q], 2);
+                iv[q] = fx(v[q], fs);
+                ivs[q] = iv[q] + __shfl_xor(iv[q], 1);
+                ivs[q] += __shfl_xor(ivs[q], 2);
             }
             if (!act || (grp && (t & 3) != 0)) continue;
-            const double *val = grp ? vs : v;
+            const long long *val = grp ? ivs : iv;
             const int cx[4] = {ix, ix + 1, ix, ix + 1};
             const int cy[4] = {iy, iy, iy + 1, iy + 1};
 #pragma unroll
             for (int k = 0; k < 4; k++) {
                 if (cy[k] < r0 || cy[k] > r1 || cx[k] > bx1) continue;
                 int lk = (cy[k] - r0) * ww + (cx[k] - bx0);
-                unsafeAtomicAdd(&tile[lk].x, val[2 * k]);
-                unsafeAtomicAdd(&tile[lk].y, val[2 * k + 1]);
+                fx_add(&tile[2 * lk], val[2 * k]);
+                fx_add(&tile[2 * lk + 1], val[2 * k + 1]);
             }
         }
         __syncthreads();
         for (int k = t; k < ncell; k += nt) {
             int ry = r0 + k / ww, rx = bx0 + k % ww;
-            plane[ry * ng + rx] = tile[k];
+            plane[ry * ng + rx] = make_double2((double)tile[2 * k] / fs, (double)tile[2 * k + 1] / fs);
         }
         __syncthreads();
     }
@@ -742,13 +780,15 @@ __global__ void __launch_bounds__(kFrontNT) k_fp64_front(const long long *__rest
                                                         TransParams t1, int moments, int sc, double gridmax,
                                                         double dgrid, int ng, double *__restrict__ mom,
                                                         double *__restrict__ fsize, double *__restrict__ cen,
-                                                        double *__restrict__ rbound, int4 *__restrict__ pbox) {
+                                                        double *__restrict__ rbound, int4 *__restrict__ pbox,
+                                                        DepositParams d, double *__restrict__ pmaxf) {
     __shared__ double lds[4 * kFrontNT];
     __shared__ int red[4][kFrontNT];
     const int is = blockIdx.x, t = threadIdx.x;
     const long long a = off[is], b = off[is + 1];
     double v[4] = {0, 0, 0, 0};
     int bx0 = 1 << 30, by0 = 1 << 30, bx1 = -1, by1 = -1;
+    double pmax = 0;  // largest sqrt(faw2) / gamma of an in-grid particle (the deposit's scale)
     for (long long i = a + t; i < b; i += kFrontNT) {
         double xi = x[i], yi = y[i], pxi = px[i], pyi = py[i];
         transverse_one(t1, g[i], xi, yi, pxi, pyi);
@@ -767,19 +807,28 @@ __global__ void __launch_bounds__(kFrontNT) k_fp64_front(const long long *__rest
             bx1 = max(bx1, ix + 1);
             by0 = min(by0, iy);
             by1 = max(by1, iy + 1);
+            const double ddx = xi - d.ax, ddy = yi - d.ay;
+            const double faw2 = (1 + d.kx * ddx * ddx + d.ky * ddy * ddy + 2 * (d.gradx * ddx + d.grady * ddy));
+            pmax = fmax(pmax, sqrt(faw2) / fabs(g[i]));
         }
     }
     red[0][t] = bx0;
     red[1][t] = by0;
     red[2][t] = -bx1;
     red[3][t] = -by1;
+    lds[t] = pmax;
     __syncthreads();
     for (int q = kFrontNT / 2; q > 0; q >>= 1) {
-        if (t < q)
+        if (t < q) {
             for (int k = 0; k < 4; k++) red[k][t] = min(red[k][t], red[k][t + q]);
+            lds[t] = fmax(lds[t], lds[t + q]);
+        }
         __syncthreads();
     }
-    if (t == 0) pbox[is] = make_int4(red[0][0], red[1][0], -red[2][0], -red[3][0]);
+    if (t == 0) {
+        pbox[is] = make_int4(red[0][0], red[1][0], -red[2][0], -red[3][0]);
+        pmaxf[is] = lds[0];
+    }
     if (!moments) return;  // uniform
     block_sum<kFrontNT, 4>(v, lds);
     const long long cnt = b - a;
@@ -827,10 +876,11 @@ __global__ void __launch_bounds__(kFuseNT) k_fp64_main(const long long *__restri
                                                        const double *__restrict__ esc, const double *__restrict__ cur,
                                                        const double *__restrict__ cen, const double *__restrict__ dr_arr,
                                                        double *__restrict__ efield, const int4 *__restrict__ pbox,
-                                                       double2 *__restrict__ src, int4 *__restrict__ box,
-                                                       double *__restrict__ beamsum, FuseParams f) {
+                                                       const double *__restrict__ pmaxf, double2 *__restrict__ src,
+                                                       int4 *__restrict__ box, double *__restrict__ beamsum,
+                                                       FuseParams f) {
     extern __shared__ double2 fdyn[];
-    __shared__ double wred[8 * kFuseNW];
+    __shared__ double wred[9 * kFuseNW];
     __shared__ int ired[5 * kFuseNW];
     double2 *celm = fdyn;                                      // [modes][ngr]
     double2 *work = fdyn + kFuseModes;                         // solver scratch, then the deposit tile
@@ -845,7 +895,7 @@ __global__ void __launch_bounds__(kFuseNT) k_fp64_main(const long long *__restri
     if (t == 0 && f.use_sc) efield[is] = 0;
     // ---- short-range space charge (k_sc_solve): every mode's solution into celm
     if (sc) {
-        double2(*hist)[kScMaxNg] = (double2(*)[kScMaxNg])work;   // [waves][ngr]
+        long long(*hist)[kScMaxNg][2] = (long long(*)[kScMaxNg][2])work;   // [waves][ngr][2] fixed point
         double2 *gam = work + kFuseNW * kScMaxNg, *rhs = gam + kScMaxNg;
         double *vol = (double *)(rhs + kScMaxNg), *ldig = vol + kScMaxNg, *rlog = ldig + kScMaxNg + 1;
         double *cupp = rlog + kScMaxNg, *clow = cupp + kScMaxNg, *cmid = clow + kScMaxNg;
@@ -868,11 +918,13 @@ __global__ void __launch_bounds__(kFuseNT) k_fp64_main(const long long *__restri
         }
         const double coef = -f.gz2 / f.ks / f.ks;
         const double econst = kVacimp / kEev * cur[is] / f.ks / (double)npart;
+        const double hs = fx_scale((double)npart);
         int mode = 0;
         for (int m = -f.nphi; m <= f.nphi; m++) {
             for (int l = 1; l <= f.nz; l++, mode++) {
                 __syncthreads();
-                for (int k = t; k < kFuseNW * kScMaxNg; k += kFuseNT) hist[k / kScMaxNg][k % kScMaxNg] = make_double2(0, 0);
+                for (int k = t; k < kFuseNW * kScMaxNg; k += kFuseNT)
+                    hist[k / kScMaxNg][k % kScMaxNg][0] = hist[k / kScMaxNg][k % kScMaxNg][1] = 0;
                 __syncthreads();
                 for (long long i = a + t; i < b; i += kFuseNT) {
                     double tx = x[i] - xc, ty = y[i] - yc;
@@ -883,8 +935,8 @@ __global__ void __launch_bounds__(kFuseNT) k_fp64_main(const long long *__restri
                     sincos(l * th[i], &s, &c);
                     double2 v = make_double2(c, -s);
                     if (m != 0) v = cmul(cpow_int(make_double2(tx / radi, ty / radi), -m), v);
-                    unsafeAtomicAdd(&hist[w][bin].x, v.x);
-                    unsafeAtomicAdd(&hist[w][bin].y, v.y);
+                    fx_add(&hist[w][bin][0], fx(v.x, hs));
+                    fx_add(&hist[w][bin][1], fx(v.y, hs));
                 }
                 __syncthreads();
                 for (int i = t; i < ngr; i += kFuseNT) {
@@ -895,8 +947,12 @@ __global__ void __launch_bounds__(kFuseNT) k_fp64_main(const long long *__restri
                         cupp[i] = coef * ldig[i] / l / l / vol[i - 1];
                         clow[i] = coef * ldig[i] / l / l / vol[i];
                     }
-                    double2 cs = hist[0][i];
-                    for (int q = 1; q < kFuseNW; q++) cs = cadd(cs, hist[q][i]);
+                    long long csx = hist[0][i][0], csy = hist[0][i][1];
+                    for (int q = 1; q < kFuseNW; q++) {
+                        csx += hist[q][i][0];
+                        csy += hist[q][i][1];
+                    }
+                    const double2 cs = make_double2((double)csx / hs, (double)csy / hs);
                     rhs[i] = cmul(cs, make_double2(0, econst / l / vol[i]));
                 }
                 __syncthreads();
@@ -934,15 +990,22 @@ __global__ void __launch_bounds__(kFuseNT) k_fp64_main(const long long *__restri
         const double scl0 = d.fcoup * kVacimp * cur[is] * d.xks * d.delz;
         scl = scl0 / (4 * kEev * (double)(b - a) * d.dgrid * d.dgrid);
     }
+    // fixed-point tile: a cell sums at most sum_i |part_i| <= n max|part|; the
+    // front's max|part| (before the push) is doubled as a margin for the
+    // step's energy change, and the pass checks that its own sum of |part|
+    // (v[8]) stays within that bound, else it deposits again in two passes
+    const double bound1 = 2.0 * (double)npart * pmaxf[is] * fabs(scl);
+    const double fs1 = fx_scale(bound1);
+    long long *itile = (long long *)work;  // [cells][2]
     __syncthreads();  // solver scratch dead: the tile takes its place
     if (fused)
-        for (int k = t; k < tw * tht; k += kFuseNT) work[k] = make_double2(0, 0);
+        for (int k = t; k < 2 * tw * tht; k += kFuseNT) itile[k] = 0;
     __syncthreads();
     // ---- one pass per particle
     const double eloss = -esc[is] / kEev;
     const int islice = (is + p.first) % p.nslice;
     const double2 *fsl = field + (long long)islice * ng * ng;
-    double v[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    double v[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};  // beam sums; [8]: sum of |part| (the tile's bound check)
     int bx0 = 1 << 30, by0 = 1 << 30, bx1 = -1, by1 = -1, ovf = 0;
     for (long long i = a + t; i < b; i += kFuseNT) {
         double gamma = g[i];
@@ -1052,35 +1115,37 @@ __global__ void __launch_bounds__(kFuseNT) k_fp64_main(const long long *__restri
             bx1 = max(bx1, ix + 1);
             by0 = min(by0, iy);
             by1 = max(by1, iy + 1);
+            const double ddx = xi - d.ax, ddy = yi - d.ay;
+            const double faw2 = (1 + d.kx * ddx * ddx + d.ky * ddy * ddy + 2 * (d.gradx * ddx + d.grady * ddy));
+            const double part = sqrt(faw2) * scl / gamma;
+            v[8] += fabs(part);
             if (fused) {
                 if (ix >= tx0 && ix + 1 < tx0 + tw && iy >= ty0 && iy + 1 < ty0 + tht) {
                     wx = 1 + floor(wx) - wx;
                     wy = 1 + floor(wy) - wy;
-                    const double ddx = xi - d.ax, ddy = yi - d.ay;
-                    const double faw2 = (1 + d.kx * ddx * ddx + d.ky * ddy * ddy + 2 * (d.gradx * ddx + d.grady * ddy));
-                    const double part = sqrt(faw2) * scl / gamma;
                     double s, c;
                     sincos(d.harm * theta, &s, &c);
                     const double2 cp = make_double2(s * part, c * part);
                     const int lk = (iy - ty0) * tw + (ix - tx0);
                     const double w00 = wx * wy, w10 = (1 - wx) * wy, w01 = wx * (1 - wy), w11 = (1 - wx) * (1 - wy);
-                    unsafeAtomicAdd(&work[lk].x, w00 * cp.x);
-                    unsafeAtomicAdd(&work[lk].y, w00 * cp.y);
-                    unsafeAtomicAdd(&work[lk + 1].x, w10 * cp.x);
-                    unsafeAtomicAdd(&work[lk + 1].y, w10 * cp.y);
-                    unsafeAtomicAdd(&work[lk + tw].x, w01 * cp.x);
-                    unsafeAtomicAdd(&work[lk + tw].y, w01 * cp.y);
-                    unsafeAtomicAdd(&work[lk + tw + 1].x, w11 * cp.x);
-                    unsafeAtomicAdd(&work[lk + tw + 1].y, w11 * cp.y);
+                    fx_add(&itile[2 * lk], fx(w00 * cp.x, fs1));
+                    fx_add(&itile[2 * lk + 1], fx(w00 * cp.y, fs1));
+                    fx_add(&itile[2 * (lk + 1)], fx(w10 * cp.x, fs1));
+                    fx_add(&itile[2 * (lk + 1) + 1], fx(w10 * cp.y, fs1));
+                    fx_add(&itile[2 * (lk + tw)], fx(w01 * cp.x, fs1));
+                    fx_add(&itile[2 * (lk + tw) + 1], fx(w01 * cp.y, fs1));
+                    fx_add(&itile[2 * (lk + tw + 1)], fx(w11 * cp.x, fs1));
+                    fx_add(&itile[2 * (lk + tw + 1) + 1], fx(w11 * cp.y, fs1));
                 } else {
                     ovf = 1;
                 }
             }
         }
     }
-    block_sum_waves<kFuseNT, 8>(v, wred);
+    block_sum_waves<kFuseNT, 9>(v, wred);
     if (t == 0)
         for (int k = 0; k < 8; k++) beamsum[8 * is + k] = v[k];
+    if (!(v[8] <= bound1)) ovf = 1;  // a particle's coupling grew past the margin: exact two-pass deposit
     // ---- footprint box and overflow flag
     int bv[5] = {bx0, by0, -bx1, -by1, -ovf};
     block_min_waves<kFuseNT, 5>(bv, ired);
@@ -1095,10 +1160,13 @@ __global__ void __launch_bounds__(kFuseNT) k_fp64_main(const long long *__restri
     }
     double2 *plane = src + (long long)is * ng * ng;
     if (fused && !ovf) {
-        for (int k = t; k < tw * tht; k += kFuseNT) plane[(ty0 + k / tw) * ng + tx0 + k % tw] = work[k];
+        for (int k = t; k < tw * tht; k += kFuseNT)
+            plane[(ty0 + k / tw) * ng + tx0 + k % tw] =
+                make_double2((double)itile[2 * k] / fs1, (double)itile[2 * k + 1] / fs1);
         if (t == 0) box[is] = make_int4(tx0, ty0, tx0 + tw - 1, ty0 + tht - 1);
         return;
     }
+    const double fs2 = fx_scale(v[8]);  // the pass's own sum of |part|: exact bound
     // ---- exact two-pass deposit over the footprint box (rows in bands of the tile)
     bx1 = min(bx1, ng - 1);
     by1 = min(by1, ng - 1);
@@ -1108,7 +1176,7 @@ __global__ void __launch_bounds__(kFuseNT) k_fp64_main(const long long *__restri
     for (int r0 = by0; r0 <= by1; r0 += rows_per_band) {
         const int r1 = min(by1, r0 + rows_per_band - 1);
         const int ncell = (r1 - r0 + 1) * ww;
-        for (int k = t; k < ncell; k += kFuseNT) work[k] = make_double2(0, 0);
+        for (int k = t; k < 2 * ncell; k += kFuseNT) itile[k] = 0;
         __syncthreads();
         for (long long i = a + t; i < b; i += kFuseNT) {
             const double xi = x[i], yi = y[i];
@@ -1132,14 +1200,14 @@ __global__ void __launch_bounds__(kFuseNT) k_fp64_main(const long long *__restri
             for (int k = 0; k < 4; k++) {
                 if (cy[k] < r0 || cy[k] > r1 || cx[k] < bx0 || cx[k] > bx1) continue;
                 const int lk = (cy[k] - r0) * ww + (cx[k] - bx0);
-                unsafeAtomicAdd(&work[lk].x, wgt[k] * cp.x);
-                unsafeAtomicAdd(&work[lk].y, wgt[k] * cp.y);
+                fx_add(&itile[2 * lk], fx(wgt[k] * cp.x, fs2));
+                fx_add(&itile[2 * lk + 1], fx(wgt[k] * cp.y, fs2));
             }
         }
         __syncthreads();
         for (int k = t; k < ncell; k += kFuseNT) {
             const int ry = r0 + k / ww, rx = bx0 + k % ww;
-            plane[ry * ng + rx] = work[k];
+            plane[ry * ng + rx] = make_double2((double)itile[2 * k] / fs2, (double)itile[2 * k + 1] / fs2);
         }
         __syncthreads();
     }
@@ -2038,17 +2106,17 @@ void launch_push(hipStream_t s, int nblocks, const int *blk_slice, const long lo
 void launch_fp64_front(hipStream_t s, int ns, const long long *off, const double *g, double *x, double *y,
                        double *px, double *py, const double *cur, const TransParams &t1, bool moments, bool sc,
                        double gridmax, double dgrid, int ng, double *mom, double *fsize, double *cen,
-                       double *rbound, int4 *pbox) {
+                       double *rbound, int4 *pbox, const DepositParams &d, double *pmaxf) {
     hipLaunchKernelGGL(k_fp64_front, dim3(ns), dim3(kFrontNT), 0, s, off, g, x, y, px, py, cur, t1, moments ? 1 : 0,
-                       sc ? 1 : 0, gridmax, dgrid, ng, mom, fsize, cen, rbound, pbox);
+                       sc ? 1 : 0, gridmax, dgrid, ng, mom, fsize, cen, rbound, pbox, d, pmaxf);
 }
 
 int fp64_main_modes() { return kFuseModes; }
 
 void launch_fp64_main(hipStream_t s, int ns, const long long *off, double *g, double *th, double *x, double *y,
                       double *px, double *py, const double2 *field, const double *esc, const double *cur,
-                      const double *cen, const double *dr, double *efield, const int4 *pbox, double2 *src,
-                      int4 *box, double *beamsum, const FuseParams &f) {
+                      const double *cen, const double *dr, double *efield, const int4 *pbox,
+                      const double *pmaxf, double2 *src, int4 *box, double *beamsum, const FuseParams &f) {
     static bool attr = false;
     if (!attr) {  // 64 KiB of dynamic LDS plus the static reduction scratch
         (void)hipFuncSetAttribute((const void *)k_fp64_main, hipFuncAttributeMaxDynamicSharedMemorySize,
@@ -2056,7 +2124,7 @@ void launch_fp64_main(hipStream_t s, int ns, const long long *off, double *g, do
         attr = true;
     }
     hipLaunchKernelGGL(k_fp64_main, dim3(ns), dim3(kFuseNT), kFuseDyn * sizeof(double2), s, off, g, th, x, y, px,
-                       py, field, esc, cur, cen, dr, efield, pbox, src, box, beamsum, f);
+                       py, field, esc, cur, cen, dr, efield, pbox, pmaxf, src, box, beamsum, f);
 }
 
 void launch_deposit(hipStream_t s, int ns, const long long *off, const double *g,

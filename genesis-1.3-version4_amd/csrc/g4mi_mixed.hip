@@ -1118,7 +1118,15 @@ struct StreamGeom {
     static constexpr int W = 2 * M + 1;
     static constexpr int H = REF ? M + 1 : M;  // halo lanes on each side
     static constexpr int SO = 64 - 2 * H;      // output columns per strip
-    static constexpr int PF = 3;               // field rows prefetched ahead
+    // FIX (narrow bands): the row loop is unrolled by W and the field queues,
+    // the F ring and its remainders all have W slots, so every slot of row r
+    // is the fixed register r mod W and no value moves between registers;
+    // the queues then prefetch W - 2 rows ahead.  Wider bands keep queues
+    // of 5 rows and shift their rings (register budget).
+    static constexpr bool FIX = M <= 4;
+    static constexpr int QN = FIX ? W : 5;     // queue slots
+    static constexpr int PF = QN - 2;          // field rows prefetched ahead
+    static constexpr int RS = FIX ? W : W;     // F ring slots
 };
 
 int stream_strips(int ng, int m, bool ref) {
@@ -1144,6 +1152,7 @@ __device__ void adi_stream_body(const MixArrays &a, const int ii, const int stri
                                 float2 (*lu1)[64]) {
     using G = StreamGeom<M, REF>;
     constexpr int W = G::W, H = G::H, SO = G::SO, PF = G::PF;
+    constexpr bool FIX = G::FIX;
     const int lane = threadIdx.x & 63;
     const MixStep &S = a.steps[a.st->istep];
     const MixBand bi = a.binfo[S.band];
@@ -1190,7 +1199,7 @@ __device__ void adi_stream_body(const MixArrays &a, const int ii, const int stri
     // queues of QN = PF+2 rows, indexed modulo QN: the row loop is unrolled
     // by QN so every slot is a fixed register (no register moves of loads in
     // flight).  Slot (r + 1 + k) % QN holds row r + k, k = -1..PF.
-    constexpr int QN = PF + 2;
+    constexpr int QN = G::QN;
     float2 q[QN], sq[QN];
     q[0] = f2(0, 0);
     sq[0] = f2(0, 0);
@@ -1202,9 +1211,9 @@ __device__ void adi_stream_body(const MixArrays &a, const int ii, const int stri
     float2 ring[W];
 #pragma unroll
     for (int m = 0; m < W; m++) ring[m] = f2(0, 0);
-    // REF: the FP32 remainders of F for rows r-M-1 .. r, and u'' of the two
-    // rows before the current band output
-    constexpr int NL = REF ? M + 2 : 1;
+    // REF: the FP32 remainders of F for rows r-M-1 .. r (FIX: W slots, row
+    // r in slot r mod W), and u'' of the two rows before the band output
+    constexpr int NL = REF ? (FIX ? W : M + 2) : 1;
     float2 lo[NL];
 #pragma unroll
     for (int m = 0; m < NL; m++) lo[m] = f2(0, 0);
@@ -1298,33 +1307,50 @@ __device__ void adi_stream_body(const MixArrays &a, const int ii, const int stri
                 q[j] = uload(r + PF + 1);
                 sq[j] = sload(r + PF + 1);
             }
+            // F of rows r - W + 1 .. r: window index k = 0..W-1 (k = W-1 the
+            // newest row).  FIX: row r sits in slot r mod W = j, index k in
+            // slot (j + 1 + k) mod W; otherwise the ring shifts down by one.
+            if constexpr (FIX) {
+                ring[j % W] = Fr;
+                if constexpr (REF) lo[j % W] = Flo;
+            } else {
 #pragma unroll
-            for (int m = 0; m < W - 1; m++) ring[m] = ring[m + 1];
-            ring[W - 1] = Fr;
-            if constexpr (REF) {
+                for (int m = 0; m < W - 1; m++) ring[m] = ring[m + 1];
+                ring[W - 1] = Fr;
+                if constexpr (REF) {
 #pragma unroll
-                for (int m = 0; m < NL - 1; m++) lo[m] = lo[m + 1];
-                lo[NL - 1] = Flo;
+                    for (int m = 0; m < NL - 1; m++) lo[m] = lo[m + 1];
+                    lo[NL - 1] = Flo;
+                }
+            }
+            float2 rw[W];  // the window in index order (register renaming once unrolled)
+#pragma unroll
+            for (int k = 0; k < W; k++) {
+                if constexpr (FIX) rw[k] = ring[(j + 1 + k) % W];
+                else rw[k] = ring[k];
             }
             const int y = r - M;
             float2 o = f2(0, 0);
             if (y >= 0 && y < ng) {
                 if (y >= bi.klo && y <= bi.khi) {
-                    o = fmul(bcr[0], ring[M]);
+                    o = fmul(bcr[0], rw[M]);
 #pragma unroll
-                    for (int k = 1; k <= M; k++) o = ffma(bcr[k], fadd(ring[M - k], ring[M + k]), o);
+                    for (int k = 1; k <= M; k++) o = ffma(bcr[k], fadd(rw[M - k], rw[M + k]), o);
                 } else {
                     const float2 *by = band + (long long)y * W;
 #pragma unroll
-                    for (int m = 0; m < W; m++) o = ffma(by[m], ring[m], o);
+                    for (int m = 0; m < W; m++) o = ffma(by[m], rw[m], o);
                 }
             }
             if constexpr (REF) {
                 // row yf = y - 1 with u''(yf - 1) = p2, u''(yf) = p1, u''(yf + 1) = o;
-                // F(yf) = ring[M - 1] + lo[0]
+                // F(yf) = window index M - 1 plus its remainder
                 const int yf = y - 1;
                 if (yf >= 0 && yf < ng) {
-                    const float2 fh = ring[M - 1], fl = lo[0];
+                    const float2 fh = rw[M - 1];
+                    float2 fl;
+                    if constexpr (FIX) fl = lo[(j + W - M - 1) % W];
+                    else fl = lo[0];
                     const double2 F64 = d2((double)fh.x + (double)fl.x, (double)fh.y + (double)fl.y);
                     const double2 u2 = tod(p1);
                     const double2 nb = d2((double)p2.x + (double)o.x, (double)p2.y + (double)o.y);

@@ -430,8 +430,8 @@ class Context:
 
     def schedule(self):
         import json
-        buf = C.create_string_buffer(1024)
-        _chk(lib().g4mi_get_schedule(self.h, buf, 1024), "g4mi_get_schedule")
+        buf = C.create_string_buffer(4096)
+        _chk(lib().g4mi_get_schedule(self.h, buf, 4096), "g4mi_get_schedule")
         return json.loads(buf.value.decode())
 
     def attach_fno(self, fno, mode, every=8, threshold=1e-3):

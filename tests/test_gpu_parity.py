@@ -125,6 +125,34 @@ def test_fused_step_matches_separate_kernels(st):
     assert ef < 1e-12
 
 
+@pytest.mark.parametrize("fused", [1, 0])
+def test_fp64_runs_are_bit_reproducible(st, fused):
+    """The space-charge histograms and the deposit sum in LDS in 64-bit
+    fixed point (integer atomics commute), so two runs of the FP64 form from
+    the same state give the same bits: records, particles and field, over
+    40 steps with long- and short-range space charge."""
+    _log(f"FP64 bit reproducibility, fused={fused}: 6 uneven slices, LSC, 40 steps")
+    counts = [2048, 512, 4096, 1024, 256, 3072]
+    out = []
+    for _ in range(2):
+        sim = H.make_state(counts, ngrid=63, gridmax=0.3e-3, longrange=True, sc=(1, 0, 100, 0.0),
+                           currents=[1500.0 + 300 * k for k in range(len(counts))], seed_power=1e4)
+        ctx = H.to_gpu(sim)
+        ctx.set_schedule("fp64_fused", fused)
+        r = ctx.run(H.gsteps([st[i] for i in range(120, 160)]), cases.GAMMA0, record=("power", "bunching"))
+        b = ctx.download_beam()
+        f, first = ctx.download_field()
+        out.append((r, b, f, first))
+        ctx.close()
+    (ra, ba, fa, qa), (rb, bb, fb, qb) = out
+    assert qa == qb
+    for k in ("power", "bunching"):
+        assert np.array_equal(ra[k], rb[k]), k
+    for k in BEAM:
+        assert np.array_equal(ba[k], bb[k]), k
+    assert np.array_equal(fa, fb)
+
+
 def test_deposit_corner_cell_and_mixed_groups(st):
     """The deposit combines the four lanes of a group when they share a
     cell.  Groups here: all four in the corner cell (0, 0); one in the
