@@ -141,6 +141,9 @@ def roofline(cfg, kt, ntot, precision=g4mi.FP64, schedule=None):
     ms = kt[name]
     gbs = per[name] / (ms * 1e-3) / 1e9
     dev = KERNEL_OF["fp64_fused" if fused else precision].get(name)
+    if name == "adi" and precision == g4mi.FP32 and schedule:
+        dev = {"band32-wide": "k_mix_adi_wide", "band32-stream": "k_mix_adi_stream",
+               "band32-tile": "k_mix_adi<"}.get(schedule.get("adi"), dev)
     if name == "adi" and precision == g4mi.FP64 and schedule:
         # the ADI kernel the tuner chose for the undulator step (the timed steps)
         und = (schedule.get("adi_by_delz") or {}).get(f"{W.DELZ:.6g}", schedule.get("adi"))
@@ -442,8 +445,9 @@ def main():
                     help="FP64 form: the ADI (default) or the spectral field solver")
     ap.add_argument("--adi-refine", type=int, default=None, choices=(0, 1),
                     help="mixed form: FP64 residual correction in the ADI (default: the library's)")
-    ap.add_argument("--mixed-adi", type=int, default=None, choices=(0, 1),
-                    help="mixed form: 0 tile ADI kernel, 1 streaming ADI kernel (default: the library's)")
+    ap.add_argument("--mixed-adi", type=int, default=None, choices=(0, 1, 2),
+                    help="mixed form: 0 tile ADI kernel, 1 streaming strips, 2 whole-width streaming "
+                         "(default: the library's)")
     ap.add_argument("--set", action="append", default=[], metavar="[FORM:]KEY=VAL",
                     help="a schedule entry (g4mi_set_schedule) for the measured contexts, e.g. "
                          "fp32:main_nt=384 or fp64:adi_kernel=3; repeatable")

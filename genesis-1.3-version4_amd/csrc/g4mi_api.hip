@@ -167,7 +167,7 @@ struct Ctx {
     std::vector<int> band_tmpl;            // template half width of each band
     bool m_refine = true;                  // FP64 residual correction in the ADI (schedule "adi_refine")
     bool m_graphs = true;                  // replay captured step graphs
-    bool m_stream = true;                  // streaming ADI (schedule "mixed_adi" 1) or the tile kernel (0)
+    int m_adi_mode = kAdiWide;             // mixed ADI kernel (schedule "mixed_adi"): tile 0, strips 1, wide 2
     int m_main_nt = 384;                   // threads per slice of the main kernel ("main_nt")
     int m_main_lds = 0;                    // its LDS region: 0 = 68 KiB, 1 = 44 KiB ("main_lds")
     bool m_persistent = false;             // the whole step as one cooperative launch ("persistent")
@@ -175,8 +175,8 @@ struct Ctx {
     bool m_tuned = false;
     std::string m_tune_log;                // mean ms per candidate, for the schedule report
     int m_tune_i = 0;                      // tuning steps done
-    double m_tune_main[6] = {0, 0, 0, 0, 0, 0}, m_tune_adi[2] = {0, 0}, m_tune_graph = -1;
-    int m_tune_nmain[6] = {0, 0, 0, 0, 0, 0}, m_tune_nadi[2] = {0, 0};
+    double m_tune_main[6] = {0, 0, 0, 0, 0, 0}, m_tune_adi[3] = {0, 0, 0}, m_tune_graph = -1;
+    int m_tune_nmain[6] = {0, 0, 0, 0, 0, 0}, m_tune_nadi[3] = {0, 0, 0};
     hipEvent_t m_tune_ev[9] = {};
     std::map<std::tuple<int, int, int, int, int, int, int, int>, hipGraphExec_t> m_graph;
     std::vector<double> m_rec_host;
@@ -1379,13 +1379,13 @@ static int mix_fill_step(Ctx *c, const g4mi_step_params &st, double gamma0, MixS
     return G4MI_OK;
 }
 
-// band, refine, sc, moments, maxzero, stream, main threads per slice, main LDS region
+// band, refine, sc, moments, maxzero, ADI kernel (kAdi*), main threads per slice, main LDS region
 typedef std::tuple<int, int, int, int, int, int, int, int> MixKey;
 
 static int mix_launch(Ctx *c, const MixKey &k, hipEvent_t *ev) {
     const MixArrays a = mix_arrays(c);
     if (launch_mix_step(c->stream, a, std::get<0>(k), std::get<1>(k) != 0, std::get<2>(k) != 0,
-                        std::get<3>(k) != 0, std::get<4>(k), ev, std::get<5>(k) != 0, std::get<6>(k),
+                        std::get<3>(k) != 0, std::get<4>(k), ev, std::get<5>(k), std::get<6>(k),
                         std::get<7>(k)) != 0) {
         set_error("mixed form: no ADI instantiation for this band width");
         return G4MI_E_ARG;
@@ -1439,14 +1439,14 @@ static int mix_step_fno(Ctx *c, const MixKey &k) {
     const MixArrays a = mix_arrays(c);
     const int tm = std::get<0>(k), maxzero = std::get<4>(k);
     const bool refine = std::get<1>(k) != 0, sc = std::get<2>(k) != 0, moments = std::get<3>(k) != 0;
-    const bool stream = std::get<5>(k) != 0;
+    const int adi_mode = std::get<5>(k);
     const int ns = c->ns;
     const long long P = (long long)c->ng * c->ng;
     int ntiles = 0;
     MixState hs{};
     if (c->fno_mode == 1) {
         launch_mix_beam(c->stream, a, sc, moments, nullptr, false, std::get<6>(k), std::get<7>(k));
-        if (launch_mix_adi(c->stream, a, tm, refine, stream, &ntiles)) {
+        if (launch_mix_adi(c->stream, a, tm, refine, adi_mode, &ntiles)) {
             set_error("mixed form: no ADI instantiation for this band width");
             return G4MI_E_ARG;
         }
@@ -1475,7 +1475,7 @@ static int mix_step_fno(Ctx *c, const MixKey &k) {
             c->fno_skipped++;
             c->fno_adi_steps++;
             launch_mix_beam(c->stream, a, sc, moments, nullptr, false, std::get<6>(k), std::get<7>(k));
-            if (launch_mix_adi(c->stream, a, tm, refine, stream, &ntiles)) {
+            if (launch_mix_adi(c->stream, a, tm, refine, adi_mode, &ntiles)) {
                 set_error("mixed form: no ADI instantiation for this band width");
                 return G4MI_E_ARG;
             }
@@ -1538,7 +1538,7 @@ static int mix_step_fno(Ctx *c, const MixKey &k) {
             G4_HIP(hipMemcpyAsync(c->m_judge, c->m_field[hs.cur ^ 1], (size_t)ns * P * sizeof(float2),
                                   hipMemcpyDeviceToDevice, c->stream));
         }
-        if (launch_mix_adi(c->stream, a, tm, refine, stream, &ntiles)) {
+        if (launch_mix_adi(c->stream, a, tm, refine, adi_mode, &ntiles)) {
             set_error("mixed form: no ADI instantiation for this band width");
             return G4MI_E_ARG;
         }
@@ -1581,8 +1581,9 @@ static int mix_step_fno(Ctx *c, const MixKey &k) {
 }
 
 // One step of the mixed-form schedule tuner (see mix_run): tuning step k
-// runs main candidate k % 6 with ADI candidate (k / 6 + k) % 2 (k < 12), the
-// graph of the chosen kernels (k = 12) or the persistent launch (k = 13).
+// runs main candidate k % 6 with ADI candidate (k / 6 + k) % 3 (k < 12: the
+// whole-width streaming kernel, the strips, the tiles), the graph of the
+// chosen kernels (k = 12) or the persistent launch (k = 13).
 constexpr int kMixTuneSteps = 14;
 static const int kMainCand[6][2] = {{384, 0}, {256, 0}, {512, 0}, {384, 1}, {256, 1}, {512, 1}};
 
@@ -1594,12 +1595,12 @@ static int mix_tune_step(Ctx *c, const MixKey &key) {
     MixKey kk = key;
     float ms = 0;
     if (k < 12) {
-        const int mc = k % 6, ac = (k / 6 + k) % 2;   // every main candidate twice, both ADIs six times
+        const int mc = k % 6, ac = (k / 6 + k) % 3;   // every main candidate twice, each ADI four times
         const int tm = std::get<0>(kk);
         const bool tile_ok = !c->m_refine || tm <= 9;
-        const int stream = (ac == 1 && tile_ok) ? 0 : 1;
+        const int mode = ac == 0 ? kAdiWide : ac == 1 ? kAdiStrips : (tile_ok ? kAdiTile : kAdiWide);
         std::get<1>(kk) = c->m_refine ? 1 : 0;
-        std::get<5>(kk) = stream;
+        std::get<5>(kk) = mode;
         std::get<6>(kk) = kMainCand[mc][0];
         std::get<7>(kk) = kMainCand[mc][1];
         G4_TRY(mix_launch(c, kk, tev));
@@ -1609,8 +1610,8 @@ static int mix_tune_step(Ctx *c, const MixKey &key) {
         G4_HIP(hipEventElapsedTime(&tm_adi, tev[6], tev[7]));
         c->m_tune_main[mc] += tm_main;
         c->m_tune_nmain[mc]++;
-        c->m_tune_adi[stream ? 0 : 1] += tm_adi;
-        c->m_tune_nadi[stream ? 0 : 1]++;
+        c->m_tune_adi[mode] += tm_adi;
+        c->m_tune_nadi[mode]++;
         if (k == 11) {
             int best = 0;
             for (int q = 1; q < 6; q++)
@@ -1619,14 +1620,18 @@ static int mix_tune_step(Ctx *c, const MixKey &key) {
                     best = q;
             c->m_main_nt = kMainCand[best][0];
             c->m_main_lds = kMainCand[best][1];
-            c->m_stream = !(c->m_tune_nadi[1] &&
-                            c->m_tune_adi[1] / c->m_tune_nadi[1] < c->m_tune_adi[0] / c->m_tune_nadi[0]);
+            int ba = -1;
+            for (int q = 0; q < 3; q++)
+                if (c->m_tune_nadi[q] &&
+                    (ba < 0 || c->m_tune_adi[q] / c->m_tune_nadi[q] < c->m_tune_adi[ba] / c->m_tune_nadi[ba]))
+                    ba = q;
+            c->m_adi_mode = ba < 0 ? kAdiWide : ba;
         }
     } else {
         // the chosen kernels for this key
         const int tm = std::get<0>(kk);
-        std::get<1>(kk) = (c->m_refine && (c->m_stream || tm <= 9)) ? 1 : 0;
-        std::get<5>(kk) = c->m_stream ? 1 : 0;
+        std::get<1>(kk) = (c->m_refine && (c->m_adi_mode != kAdiTile || tm <= 9)) ? 1 : 0;
+        std::get<5>(kk) = c->m_adi_mode;
         std::get<6>(kk) = c->m_main_nt;
         std::get<7>(kk) = c->m_main_lds;
         if (k == 12) {
@@ -1640,7 +1645,7 @@ static int mix_tune_step(Ctx *c, const MixKey &key) {
             c->m_tune_graph = ms;
         } else {
             double pers = -1;
-            if (c->m_stream && launch_mix_step_persistent(c->stream, mix_arrays(c), tm, std::get<1>(kk) != 0,
+            if (c->m_adi_mode != kAdiTile && launch_mix_step_persistent(c->stream, mix_arrays(c), tm, std::get<1>(kk) != 0,
                                                           std::get<2>(kk) != 0, std::get<3>(kk) != 0,
                                                           std::get<4>(kk), tev) == 0) {
                 G4_HIP(hipEventSynchronize(tev[8]));
@@ -1658,11 +1663,12 @@ static int mix_tune_step(Ctx *c, const MixKey &key) {
             snprintf(buf, sizeof(buf),
                      "{\"main_ms\": {\"384/68K\": %.4f, \"256/68K\": %.4f, \"512/68K\": %.4f, "
                      "\"384/44K\": %.4f, \"256/44K\": %.4f, \"512/44K\": %.4f}, "
-                     "\"adi_ms\": {\"stream\": %.4f, \"tile\": %.4f}, "
+                     "\"adi_ms\": {\"wide\": %.4f, \"strips\": %.4f, \"tile\": %.4f}, "
                      "\"step_ms\": {\"graph\": %.4f, \"persistent\": %.4f}}",
                      avg(mm[0], nm[0]), avg(mm[1], nm[1]), avg(mm[2], nm[2]), avg(mm[3], nm[3]),
-                     avg(mm[4], nm[4]), avg(mm[5], nm[5]), avg(c->m_tune_adi[0], c->m_tune_nadi[0]),
-                     avg(c->m_tune_adi[1], c->m_tune_nadi[1]), c->m_tune_graph, pers);
+                     avg(mm[4], nm[4]), avg(mm[5], nm[5]), avg(c->m_tune_adi[kAdiWide], c->m_tune_nadi[kAdiWide]),
+                     avg(c->m_tune_adi[kAdiStrips], c->m_tune_nadi[kAdiStrips]),
+                     avg(c->m_tune_adi[kAdiTile], c->m_tune_nadi[kAdiTile]), c->m_tune_graph, pers);
             c->m_tune_log = buf;
             c->m_tuned = true;
             for (int e = 0; e < 9; e++) (void)hipEventDestroy(tev[e]);
@@ -1703,8 +1709,8 @@ static int mix_run(Ctx *c, const g4mi_step_params *steps, int nsteps, double gam
         G4_TRY(mix_fill_step(c, steps[i], gamma0, tab[i]));
         const int tm = c->band_tmpl[tab[i].band];
         // the tile kernel's correction needs more LDS than a CU has beyond a half width of 9
-        const int refine = (c->m_refine && (c->m_stream || tm <= 9)) ? 1 : 0;
-        keys[i] = MixKey(tm, refine, sc ? 1 : 0, moments ? 1 : 0, maxzero, c->m_stream ? 1 : 0, c->m_main_nt,
+        const int refine = (c->m_refine && (c->m_adi_mode != kAdiTile || tm <= 9)) ? 1 : 0;
+        keys[i] = MixKey(tm, refine, sc ? 1 : 0, moments ? 1 : 0, maxzero, c->m_adi_mode, c->m_main_nt,
                          c->m_main_lds);
     }
     if (n > c->m_steps_cap) {
@@ -1737,8 +1743,8 @@ static int mix_run(Ctx *c, const g4mi_step_params *steps, int nsteps, double gam
             if (c->m_tuned)
                 for (int j = i + 1; j < n; j++) {
                     const int tmj = std::get<0>(keys[j]);
-                    std::get<1>(keys[j]) = (c->m_refine && (c->m_stream || tmj <= 9)) ? 1 : 0;
-                    std::get<5>(keys[j]) = c->m_stream ? 1 : 0;
+                    std::get<1>(keys[j]) = (c->m_refine && (c->m_adi_mode != kAdiTile || tmj <= 9)) ? 1 : 0;
+                    std::get<5>(keys[j]) = c->m_adi_mode;
                     std::get<6>(keys[j]) = c->m_main_nt;
                     std::get<7>(keys[j]) = c->m_main_lds;
                 }
@@ -2658,8 +2664,8 @@ int g4mi_set_schedule(g4mi_ctx *h, const char *name, int value) {
         c->m_refine = value != 0;
         return G4MI_OK;
     }
-    if (std::string(name) == "mixed_adi" && (value == 0 || value == 1)) {
-        c->m_stream = value == 1;
+    if (std::string(name) == "mixed_adi" && value >= kAdiTile && value <= kAdiWide) {
+        c->m_adi_mode = value;
         c->m_autotune = false;
         return G4MI_OK;
     }
@@ -2679,7 +2685,7 @@ int g4mi_set_schedule(g4mi_ctx *h, const char *name, int value) {
     }
     if (std::string(name) == "persistent" && (value == 0 || value == 1)) {
         c->m_persistent = value != 0;
-        if (value) c->m_stream = true;  // the persistent step runs the streaming ADI
+        if (value && c->m_adi_mode == kAdiTile) c->m_adi_mode = kAdiWide;  // the persistent step streams strips
         c->m_autotune = false;
         return G4MI_OK;
     }
@@ -2690,8 +2696,7 @@ int g4mi_set_schedule(g4mi_ctx *h, const char *name, int value) {
         c->m_tuned = false;
         c->m_tune_i = 0;
         for (int q = 0; q < 6; q++) c->m_tune_main[q] = 0, c->m_tune_nmain[q] = 0;
-        c->m_tune_adi[0] = c->m_tune_adi[1] = 0;
-        c->m_tune_nadi[0] = c->m_tune_nadi[1] = 0;
+        for (int q = 0; q < 3; q++) c->m_tune_adi[q] = 0, c->m_tune_nadi[q] = 0;
         c->m_tune_graph = -1;
         return G4MI_OK;
     }
@@ -2718,7 +2723,8 @@ int g4mi_get_schedule(g4mi_ctx *h, char *buf, int len) {
                  "\"main_nt\": %d, \"main_lds_kib\": %d, \"persistent\": %d, \"autotune\": %d, \"tuned\": %d, "
                  "\"graphs\": %d, \"bands_m_over_template\": \"%s\", \"graphs_captured\": %d, "
                  "\"tuning\": %s, \"fno_skipped_steps\": %lld}",
-                 c->m_stream ? "band32-stream" : "band32-tile", c->m_refine ? 1 : 0, c->m_main_nt,
+                 c->m_adi_mode == kAdiWide ? "band32-wide" : c->m_adi_mode == kAdiStrips ? "band32-stream" : "band32-tile",
+                 c->m_refine ? 1 : 0, c->m_main_nt,
                  c->m_main_lds ? 44 : 68, c->m_persistent ? 1 : 0, c->m_autotune ? 1 : 0, c->m_tuned ? 1 : 0, c->m_graphs ? 1 : 0,
                  bands.c_str(), (int)c->m_graph.size(), c->m_tune_log.empty() ? "null" : c->m_tune_log.c_str(),
                  c->fno_skipped);

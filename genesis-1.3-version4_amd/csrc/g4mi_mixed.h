@@ -106,12 +106,15 @@ size_t mix_band_lds(int tmpl_m, bool refine);
 // slippage, zeroing and the step record.  ev (optional, 9 events) brackets
 // the kernel groups: front [0,1], main [3 - 2], ADI [6,7], slippage + record
 // [7,8]; the other intervals are empty.
-// stream: the streaming ADI (one wavefront per column strip) instead of
-// the 32x32 tile kernel (no residual correction only).
+// adi_mode: kAdiTile (32x32 tiles, correction up to a half width of 9),
+// kAdiStrips (streaming, one wavefront per column strip) or kAdiWide
+// (streaming, one workgroup per slice covering all columns; grids up to 512
+// columns, wider ones run the strips).
 // main_nt: threads per slice of the main kernel (256, 384, 512); main_lds: its
 // dynamic LDS region, 0 = 68 KiB, 1 = 44 KiB.
+constexpr int kAdiTile = 0, kAdiStrips = 1, kAdiWide = 2;
 int launch_mix_step(hipStream_t s, const MixArrays &a, int tmpl_m, bool refine, bool sc,
-                    bool moments, int maxzero, hipEvent_t *ev, bool stream, int main_nt, int main_lds);
+                    bool moments, int maxzero, hipEvent_t *ev, int adi_mode, int main_nt, int main_lds);
 // The same step as one cooperative launch of co-resident workgroups with
 // grid barriers between the phases (the streaming ADI); ev (optional, 9
 // events): [0] before, [8] after.  Returns -1 if the launch is refused.
@@ -123,7 +126,7 @@ int launch_mix_step_persistent(hipStream_t s, const MixArrays &a, int tmpl_m, bo
 // egsum: also sum the push's energy increments per slice (a.egsum)
 void launch_mix_beam(hipStream_t s, const MixArrays &a, bool sc, bool moments, hipEvent_t *ev, bool egsum,
                      int main_nt = 384, int main_lds = 0);
-int launch_mix_adi(hipStream_t s, const MixArrays &a, int tmpl_m, bool refine, bool stream, int *ntiles);
+int launch_mix_adi(hipStream_t s, const MixArrays &a, int tmpl_m, bool refine, int adi_mode, int *ntiles);
 void launch_mix_tail(hipStream_t s, const MixArrays &a, int maxzero, int ntiles, hipEvent_t *ev);
 // Field sums per storage slice ([ns][8], as the ADI's diagnostics with one
 // block per slice) of the live buffer (next = 0) or the other one (1).

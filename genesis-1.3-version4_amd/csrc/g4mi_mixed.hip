@@ -46,6 +46,13 @@ __device__ __forceinline__ float2 ffma(float2 a, float2 b, float2 acc) {
     return acc;
 }
 __device__ __forceinline__ double2 d2(double a, double b) { return make_double2(a, b); }
+// The hardware's FP32 reciprocal, square root and reciprocal square root
+// (v_rcp_f32, v_sqrt_f32, v_rsq_f32, about 1 ulp on normal inputs) in place
+// of the correctly rounded sequences (a division is ten instructions): the
+// mixed form's error is set by its FP32 storage, not by the last ulp here.
+__device__ __forceinline__ float frcp(float x) { return __builtin_amdgcn_rcpf(x); }
+__device__ __forceinline__ float fsqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
+__device__ __forceinline__ float frsq(float x) { return __builtin_amdgcn_rsqf(x); }
 __device__ __forceinline__ double2 tod(float2 a) { return d2((double)a.x, (double)a.y); }
 __device__ __forceinline__ float2 tof(double2 a) { return f2((float)a.x, (float)a.y); }
 __device__ __forceinline__ float2 tof(float2 a) { return a; }
@@ -83,23 +90,25 @@ __device__ __forceinline__ void bsum(double (&v)[K], double *lds) {
 // focusing term's full relative precision.
 __device__ __forceinline__ void line_map32(int kind, float delz, float q, float &x, float &px,
                                            float gz, float d) {
+    // igz = 1 / gz
+    const float igz = frcp(gz);
     if (kind == 0) {
-        x += px * delz / gz;
+        x += px * delz * igz;
     } else if (kind > 0) {
-        const float foc = sqrtf(q / gz), omg = foc * delz;
+        const float foc = fsqrt(q * igz), omg = foc * delz;
         float s, c, sh, ch;
         __sincosf(omg, &s, &c);
         __sincosf(0.5f * omg, &sh, &ch);
-        const float a1m = -2.f * sh * sh, a2 = s / foc, a3 = -a2 * foc * foc;
+        const float a1m = -2.f * sh * sh, a2 = s * frcp(foc), a3 = -a2 * foc * foc;
         const float xt = x - d;
-        x = xt + (a1m * xt + a2 * px / gz) + d;
+        x = xt + (a1m * xt + a2 * px * igz) + d;
         px = px + (a3 * xt * gz + a1m * px);
     } else {
-        const float foc = sqrtf(-q / gz), omg = foc * delz;
+        const float foc = fsqrt(-q * igz), omg = foc * delz;
         const float sh = sinhf(0.5f * omg);
-        const float a1m = 2.f * sh * sh, a2 = sinhf(omg) / foc, a3 = a2 * foc * foc;
+        const float a1m = 2.f * sh * sh, a2 = sinhf(omg) * frcp(foc), a3 = a2 * foc * foc;
         const float xt = x - d;
-        x = xt + (a1m * xt + a2 * px / gz) + d;
+        x = xt + (a1m * xt + a2 * px * igz) + d;
         px = px + (a3 * xt * gz + a1m * px);
     }
 }
@@ -111,16 +120,16 @@ __device__ __forceinline__ void transverse32(const TransParams &p, float gam, fl
         py += (float)p.ccy;
     }
     if (p.chicane) {
-        const float gz = sqrtf(gam * gam - 1.f - px * px - py * py);
+        const float gz = fsqrt(gam * gam - 1.f - px * px - py * py), igz = frcp(gz);
         float t = x;
-        x = (float)p.m00 * t + (float)p.m01 * px / gz;
+        x = (float)p.m00 * t + (float)p.m01 * px * igz;
         px = (float)p.m10 * t * gz + (float)p.m11 * px;
         t = y;
-        y = (float)p.m22 * t + (float)p.m23 * py / gz;
+        y = (float)p.m22 * t + (float)p.m23 * py * igz;
         py = (float)p.m32 * t * gz + (float)p.m33 * py;
     }
     const float aw = (float)p.aw;
-    const float gz = sqrtf(gam * gam - 1.f - aw * aw - px * px - py * py);
+    const float gz = fsqrt(gam * gam - 1.f - aw * aw - px * px - py * py);
     line_map32(p.kindx, (float)p.delz, (float)p.qx, x, px, gz, (float)p.xoff);
     line_map32(p.kindy, (float)p.delz, (float)p.qy, y, py, gz, (float)p.yoff);
 }
@@ -328,14 +337,14 @@ struct OdeMix {
         // rpart * (cos, -sin)
         const float cre = rpart.x * c + rpart.y * s, cim = rpart.y * c - rpart.x * s;
         const float gam = gref + dgam;
-        const float ig2 = 1.f / (gam * gam);
+        const float ig = frcp(gam), ig2 = ig * ig;
         const float dbt = btpar_m1 + ztemp1 * cre;   // btper0 - bref
         const float eps = (bref + dbt) * ig2;
-        const float beta = sqrtf(1.f - eps);
+        const float ibeta = frsq(1.f - eps);          // 1 / beta
         // eps_ref - eps = bref (gam^2 - gref^2) / (gref^2 gam^2) - (btper0 - bref) / gam^2
         const float deps = (eps_ref * dgam * (2.f * gref + dgam) - dbt) * ig2;
         kpp += xks * (0.5f + 0.375f * (eps + eps_ref)) * deps;
-        kgg += cim / beta / gam - ez;
+        kgg += cim * ibeta * ig - ez;
     }
 };
 
@@ -366,8 +375,9 @@ __device__ __forceinline__ unsigned mix_hash(unsigned long long i, unsigned step
     return x;
 }
 
-__device__ __forceinline__ int sc_bin(float radi, float dr, int ngr) {
-    int bin = (int)floorf(radi / dr);
+// radial cell of a particle: rdr = radius / cell width
+__device__ __forceinline__ int sc_bin(float rdr, int ngr) {
+    int bin = (int)floorf(rdr);
     return (bin >= 0 && bin < ngr) ? bin : (bin >= ngr ? ngr - 1 : 0);
 }
 
@@ -435,11 +445,11 @@ __device__ void main_body(const MixArrays &a, const int is, int sc_on, float2 *d
     const float phh = wrap_phase(basep + S.kappa * 0.5 * P.delz);
     const float ph1 = wrap_phase(basep + S.kappa * P.delz);
     const int ngr = a.sc_ng;
-    float xc = 0, yc = 0, dr = 1;
+    float xc = 0, yc = 0, idr = 1;  // idr: 1 / radial cell width
     if (sc) {
         xc = (float)a.cen[2 * is];
         yc = (float)a.cen[2 * is + 1];
-        dr = (float)a.dr[is];
+        idr = (float)(1.0 / a.dr[is]);
         const double econst = kVacimp / kEev * a.cur[is] / S.sc_ks / (double)npart;
         const double coef = -S.sc_gz2 / S.sc_ks / S.sc_ks;
         const double drd = a.dr[is];
@@ -463,11 +473,11 @@ __device__ void main_body(const MixArrays &a, const int is, int sc_on, float2 *d
                         ny = a.y[i + NT];
                         nt = a.dth[i + NT];
                     }
-                    const float radi = sqrtf(tx * tx + ty * ty);
-                    const int bin = sc_bin(radi, dr, ngr);
+                    const float radi = fsqrt(tx * tx + ty * ty), ir = frcp(radi);
+                    const int bin = sc_bin(radi * idr, ngr);
                     float s, c;
                     __sincosf((float)l * (ph0 + th), &s, &c);
-                    const float2 v = fmul(cw_pow(f2(tx / radi, ty / radi), -m), f2(c, -s));
+                    const float2 v = fmul(cw_pow(f2(tx * ir, ty * ir), -m), f2(c, -s));
                     unsafeAtomicAdd(&hist[w][bin].x, (double)v.x);
                     unsafeAtomicAdd(&hist[w][bin].y, (double)v.y);
                 }
@@ -553,6 +563,13 @@ __device__ void main_body(const MixArrays &a, const int is, int sc_on, float2 *d
     const float autoph = (float)P.autophase;
     const float dax = (float)D.ax, day = (float)D.ay, dkx = (float)D.kx, dky = (float)D.ky;
     const float dgx = (float)D.gradx, dgy = (float)D.grady;
+    // step constants in registers: the particle stores below may alias the
+    // step table as far as the compiler can tell, which would reload them
+    const float pax = (float)P.ax, pay = (float)P.ay, pkx = (float)P.kx, pky = (float)P.ky;
+    const float pgx = (float)P.gradx, pgy = (float)P.grady;
+    const float eps_ref = (float)S.eps_ref;
+    const bool chic = S.chicane != 0;
+    const TransParams t2 = S.t2;
     // per-lane beam sums of at most a few dozen particles in FP32, energy as
     // the offset dg; widened to FP64 for the block sum
     constexpr int NV = EG ? 9 : 8;
@@ -587,12 +604,11 @@ __device__ void main_body(const MixArrays &a, const int is, int sc_on, float2 *d
             n_px = a.px[j];
             n_py = a.py[j];
         }
-        const float dx = x - (float)P.ax, dy = y - (float)P.ay;
-        const float awm1 = 0.5f * ((float)P.kx * dx * dx + (float)P.ky * dy * dy) +
-                           (float)P.gradx * dx + (float)P.grady * dy;  // awloc - 1
+        const float dx = x - pax, dy = y - pay;
+        const float awm1 = 0.5f * (pkx * dx * dx + pky * dy * dy) + pgx * dx + pgy * dy;  // awloc - 1
         OdeMix o;
         o.xks = xks;
-        o.eps_ref = (float)S.eps_ref;
+        o.eps_ref = eps_ref;
         o.bref = bref;
         o.gref = gref;
         o.ztemp1 = -2.f / xks;
@@ -600,9 +616,9 @@ __device__ void main_body(const MixArrays &a, const int is, int sc_on, float2 *d
         float ez = ez_long;
         if (sc) {
             const float tx = x - xc, ty = y - yc;
-            const float radi = sqrtf(tx * tx + ty * ty);
-            const float2 cw = f2(tx / radi, ty / radi);
-            const int bin = sc_bin(radi, dr, ngr);
+            const float radi = fsqrt(tx * tx + ty * ty), ir = frcp(radi);
+            const float2 cw = f2(tx * ir, ty * ir);
+            const int bin = sc_bin(radi * idr, ngr);
             int mode = 0;
             for (int m = -a.sc_nphi; m <= a.sc_nphi; m++) {
                 for (int l = 1; l <= a.sc_nz; l++, mode++) {
@@ -650,8 +666,8 @@ __device__ void main_body(const MixArrays &a, const int is, int sc_on, float2 *d
         o.eval(g0 + eg, phh + (t0 + et));
         eg += stpz * (o.kgg - k3gg);
         et += stpz * (o.kpp - k3pp);
-        k3gg /= 6;
-        k3pp /= 6;
+        k3gg *= 1.f / 6.f;
+        k3pp *= 1.f / 6.f;
         o.kgg *= -0.5f;
         o.kpp *= -0.5f;
         o.eval(g0 + eg, phh + (t0 + et));
@@ -663,14 +679,14 @@ __device__ void main_body(const MixArrays &a, const int is, int sc_on, float2 *d
         o.kgg *= 2;
         o.kpp *= 2;
         o.eval(g0 + eg, ph1 + (t0 + et));
-        eg += stpz * (k3gg + o.kgg / 6.0f);
-        et += stpz * (k3pp + o.kpp / 6.0f);
+        eg += stpz * (k3gg + o.kgg * (1.f / 6.f));
+        et += stpz * (k3pp + o.kpp * (1.f / 6.f));
         const unsigned hs = mix_hash((unsigned long long)i, stepno, 0x51u);
         const float dg = add_sr(g0, eg, hs);
         if constexpr (EG) fv[NV - 1] += eg;
-        if (S.chicane) et += r56 * (dg + dgam0);  // TrackBeam::applyR56
+        if (chic) et += r56 * (dg + dgam0);  // TrackBeam::applyR56
         const float dth = add_sr(t0, et, mix_hash((unsigned long long)i, stepno, 0xA7u));
-        transverse32(S.t2, gref + dg, x, y, px, py);
+        transverse32(t2, gref + dg, x, y, px, py);
         a.dg[i] = dg;
         a.dth[i] = dth;
         a.x[i] = x;
@@ -701,7 +717,7 @@ __device__ void main_body(const MixArrays &a, const int is, int sc_on, float2 *d
                     wy = 1.f + floorf(wy) - wy;
                     const float ddx = x - dax, ddy = y - day;
                     const float faw2 = 1.f + dkx * ddx * ddx + dky * ddy * ddy + 2.f * (dgx * ddx + dgy * ddy);
-                    const float part = sqrtf(faw2) * scl / (gref + dg);
+                    const float part = fsqrt(faw2) * scl * frcp(gref + dg);
                     const float2 cp = f2(s * part, c * part);
                     const int lk = (iy - ty0) * tw + (ix - tx0);
                     unsafeAtomicAdd(&work[lk].x, (double)(wx * wy * cp.x));
@@ -772,7 +788,7 @@ __device__ void main_body(const MixArrays &a, const int is, int sc_on, float2 *d
             wy = 1.f + floorf(wy) - wy;
             const float ddx = x - dax, ddy = y - day;
             const float faw2 = 1.f + dkx * ddx * ddx + dky * ddy * ddy + 2.f * (dgx * ddx + dgy * ddy);
-            const float part = sqrtf(faw2) * scl / (gref + a.dg[i]);
+            const float part = fsqrt(faw2) * scl * frcp(gref + a.dg[i]);
             float s, c;
             __sincosf(ph1 + a.dth[i], &s, &c);
             const float2 cp = f2(s * part, c * part);
@@ -1388,6 +1404,268 @@ __global__ void __launch_bounds__(64) k_mix_adi_stream(MixArrays a) {
 }
 
 // ---------------------------------------------------------------------------
+// Whole-width streaming ADI: the row recurrence of adi_stream_body with one
+// workgroup of NW = blockDim / 64 wavefronts covering every column of a
+// slice (one lane per column), so no column is computed twice; the strips
+// above recompute a 2H-column halo (320 lanes for 255 columns at M = 4 with
+// the correction).  The E and u1 rows pass between the wavefronts through
+// LDS with one workgroup barrier per exchange whose fences cover LDS only,
+// so the field rows prefetched from HBM stay in flight across it.  Cells off
+// the grid hold zeros in the LDS rows: the Dirichlet boundary of A.
+//
+// Column map: wavefront 0 takes the first and the last 32 columns, whose
+// band rows differ from the central one; wavefronts 1.. take the columns in
+// between and use the symmetric central row when all of theirs lie in
+// [klo, khi].  Lanes left over sit on column ng (off the grid, zero).
+//
+// Buffers: the E row is double-buffered by row parity.  Without the
+// correction a wavefront may write row r+1 while another still reads row r
+// (one barrier per row).  With it, a second barrier between the E reads and
+// the u1 reads separates every write of a row from the reads of the row
+// before, so u1 needs one buffer.
+//
+// Dynamic LDS: E rows [2][NC + 2M] float2, u1 row [NC + 2] double2 (with the
+// correction), per-column band rows [W][NC] float2, wavefront sums [NW][8].
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void lds_barrier() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+    __builtin_amdgcn_s_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
+template <int M, bool REF>
+size_t wide_lds_bytes(int nw) {
+    const int nc = 64 * nw;
+    return (size_t)2 * (nc + 2 * M) * sizeof(float2) + (REF ? (size_t)(nc + 2) * sizeof(double2) : 0) +
+           (size_t)(2 * M + 1) * nc * sizeof(float2) + (size_t)nw * 8 * sizeof(double);
+}
+
+template <int M, bool REF>
+__global__ void __launch_bounds__(512) k_mix_adi_wide(MixArrays a) {
+    using G = StreamGeom<M, REF>;
+    constexpr int W = G::W, QN = G::QN, PF = G::PF;
+    constexpr bool FIX = G::FIX;
+    extern __shared__ double2 wide_lds[];
+    const int nt = blockDim.x, nw = nt >> 6, nc = nt;
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+    float2 *lrow0 = (float2 *)wide_lds;                    // [2][nc + 2M]
+    const int lp = nc + 2 * M;
+    double2 *lu = (double2 *)(lrow0 + 2 * lp);             // [nc + 2]
+    float2 *lband = (float2 *)(lu + (REF ? nc + 2 : 0));    // [W][nc]
+    double *wsum = (double *)(lband + W * nc);              // [nw][8]
+    const int ii = blockIdx.x;
+    if (ii >= a.ns) return;
+    const MixStep &S = a.steps[a.st->istep];
+    const MixBand bi = a.binfo[S.band];
+    const float2 *band = a.bands + bi.off;
+    const double ad = bi.a;
+    const float af = (float)bi.a;
+    const double dn = 1.0 / (1.0 + 4.0 * ad * ad);
+    const int ng = a.ng, ns = a.ns;
+    // this lane's column
+    int x;
+    if (ng <= 64) x = lane < ng ? lane : ng;
+    else if (w == 0) x = lane < 32 ? lane : ng - 64 + lane;
+    else {
+        x = 32 + 64 * (w - 1) + lane;
+        if (x >= ng - 32) x = ng;
+    }
+    const bool xin = x < ng;
+    // every grid column of this wavefront shares the central band row (uniform)
+    bool wsym;
+    if (ng <= 64 || w == 0) wsym = bi.klo <= 0 && bi.khi >= ng - 1;
+    else {
+        const int xlo = 32 + 64 * (w - 1), xhi = min(xlo + 63, ng - 33);
+        wsym = xlo > xhi || (xlo >= bi.klo && xhi <= bi.khi);
+    }
+    // zero rows (the pads stay zero: only lanes on the grid write non-zero values)
+    for (int k = t; k < 2 * lp; k += nt) lrow0[k] = f2(0, 0);
+    if constexpr (REF)
+        for (int k = t; k < nc + 2; k += nt) lu[k] = d2(0, 0);
+    if (!wsym)
+        for (int m = 0; m < W; m++) lband[m * nc + t] = xin ? band[(long long)x * W + m] : f2(0, 0);
+    const float2 *bc = band + (long long)((bi.klo + bi.khi) / 2) * W;
+    float2 bcr[M + 1];  // the central row's right half b_M .. b_2M (symmetric)
+#pragma unroll
+    for (int m = 0; m <= M; m++) bcr[m] = bc[M + m];
+    const long long fs = (long long)((ii + a.st->first) % ns);
+    const int cur = a.st->cur;
+    const float2 *u = a.field[cur] + fs * ng * ng;
+    float2 *uo = a.field[cur ^ 1] + fs * ng * ng;
+    const float2 *Sp = a.src + (long long)ii * ng * ng;
+    const int4 bx = a.box[ii];
+    const bool sxin = xin && x >= bx.x && x <= bx.z;
+    const int xcl = min(x, ng - 1);
+    const float2 *ucol = u + xcl, *scol = Sp + xcl;
+    auto uload = [&](int rr) -> float2 { return ucol[(long long)min(rr, ng - 1) * ng]; };
+    auto sload = [&](int rr) -> float2 {
+        const bool in = sxin && rr >= bx.y && rr <= bx.w;
+        return *(in ? scol + (long long)rr * ng : g_zero_row + xcl);
+    };
+    float2 q[QN], sq[QN];
+    q[0] = f2(0, 0);
+    sq[0] = f2(0, 0);
+#pragma unroll
+    for (int k = 1; k < QN; k++) {
+        q[k] = uload(k - 1);
+        sq[k] = sload(k - 1);
+    }
+    float2 ring[W];
+#pragma unroll
+    for (int m = 0; m < W; m++) ring[m] = f2(0, 0);
+    constexpr int NL = REF ? (FIX ? W : M + 2) : 1;
+    float2 lo[NL];
+#pragma unroll
+    for (int m = 0; m < NL; m++) lo[m] = f2(0, 0);
+    float2 p1 = f2(0, 0), p2 = f2(0, 0);
+    // field sums of this column: x is fixed, so the x moments are dx, dx^2
+    // times the column's power, applied at the end
+    double s_w = 0, s_yw = 0, s_yyw = 0, s_re = 0, s_im = 0, s_cen = 0;
+    const bool xcen = x == ng / 2;
+    const double shift = -0.5 * (double)(ng - 1);
+    auto emit = [&](int y, float2 o) {
+        if (xin) {
+            uo[(long long)y * ng + x] = o;
+            const double ox = o.x, oy = o.y;
+            const double wei = ox * ox + oy * oy;
+            const double dy = (double)y + shift;
+            s_w += wei;
+            s_yw += dy * wei;
+            s_yyw += dy * dy * wei;
+            s_re += ox;
+            s_im += oy;
+            if (xcen && y == ng / 2) s_cen = wei;
+        }
+    };
+    lds_barrier();
+    float2 *lx = lrow0 + x + M;  // this column in E row 0; row 1 is lp further
+    const int nrow = ng + M + (REF ? 1 : 0);
+    for (int r0 = 0; r0 < nrow; r0 += QN) {
+#pragma unroll
+        for (int j = 0; j < QN; j++) {
+            const int r = r0 + j;
+            if (r >= nrow) break;
+            float2 Fr = f2(0, 0), Flo = f2(0, 0);
+            if (r < ng) {
+                const float2 sv = sq[(j + 1) % QN];
+                const float2 uc = q[(j + 1) % QN];
+                const float2 un = (r + 1 < ng) ? q[(j + 2) % QN] : f2(0, 0);
+                const float2 ul = (r > 0) ? q[j] : f2(0, 0);
+                double2 E64 = d2(0, 0);
+                float2 E = f2(0, 0);
+                if constexpr (REF) {
+                    if (xin) {
+                        const double sx = (double)un.x - 2.0 * (double)uc.x + (double)ul.x;
+                        const double sy = (double)un.y - 2.0 * (double)uc.y + (double)ul.y;
+                        E64 = d2((double)sv.x + (double)uc.x - ad * sy, (double)sv.y + (double)uc.y + ad * sx);
+                    }
+                    E = tof(E64);
+                } else {
+                    const float sx = un.x - 2.f * uc.x + ul.x, sy = un.y - 2.f * uc.y + ul.y;
+                    E = xin ? f2(sv.x + uc.x - af * sy, sv.y + uc.y + af * sx) : f2(0, 0);
+                }
+                float2 *lr = lx + (r & 1) * lp;
+                lr[0] = E;
+                lds_barrier();
+                float2 up = f2(0, 0);
+                if (xin) {
+                    if (wsym) {
+                        up = fmul(bcr[0], lr[0]);
+#pragma unroll
+                        for (int k = 1; k <= M; k++) up = ffma(bcr[k], fadd(lr[-k], lr[k]), up);
+                    } else {
+#pragma unroll
+                        for (int m = 0; m < W; m++) up = ffma(lband[m * nc + t], lr[m - M], up);
+                    }
+                }
+                if constexpr (REF) {
+                    const double2 u1 = tod(up);
+                    lu[x + 1] = u1;  // zero off the grid
+                    lds_barrier();
+                    const double2 nl = lu[x], nr = lu[x + 2];
+                    const double2 au = adi_apply(u1, d2(nl.x + nr.x, nl.y + nr.y), ad);
+                    const double2 upd = jacobi_update(u1, d2(E64.x - au.x, E64.y - au.y), ad, dn);
+                    if (xin) {
+                        const double2 F64 = d2((double)sv.x + 2.0 * upd.x - E64.x, (double)sv.y + 2.0 * upd.y - E64.y);
+                        Fr = tof(F64);
+                        Flo = f2((float)(F64.x - (double)Fr.x), (float)(F64.y - (double)Fr.y));
+                    }
+                } else {
+                    Fr = xin ? f2(sv.x + 2.f * up.x - E.x, sv.y + 2.f * up.y - E.y) : f2(0, 0);
+                }
+                q[j] = uload(r + PF + 1);
+                sq[j] = sload(r + PF + 1);
+            }
+            if constexpr (FIX) {
+                ring[j % W] = Fr;
+                if constexpr (REF) lo[j % W] = Flo;
+            } else {
+#pragma unroll
+                for (int m = 0; m < W - 1; m++) ring[m] = ring[m + 1];
+                ring[W - 1] = Fr;
+                if constexpr (REF) {
+#pragma unroll
+                    for (int m = 0; m < NL - 1; m++) lo[m] = lo[m + 1];
+                    lo[NL - 1] = Flo;
+                }
+            }
+            float2 rw[W];
+#pragma unroll
+            for (int k = 0; k < W; k++) {
+                if constexpr (FIX) rw[k] = ring[(j + 1 + k) % W];
+                else rw[k] = ring[k];
+            }
+            const int y = r - M;
+            float2 o = f2(0, 0);
+            if (y >= 0 && y < ng) {
+                if (y >= bi.klo && y <= bi.khi) {
+                    o = fmul(bcr[0], rw[M]);
+#pragma unroll
+                    for (int k = 1; k <= M; k++) o = ffma(bcr[k], fadd(rw[M - k], rw[M + k]), o);
+                } else {
+                    const float2 *by = band + (long long)y * W;
+#pragma unroll
+                    for (int m = 0; m < W; m++) o = ffma(by[m], rw[m], o);
+                }
+            }
+            if constexpr (REF) {
+                const int yf = y - 1;
+                if (yf >= 0 && yf < ng) {
+                    const float2 fh = rw[M - 1];
+                    float2 fl;
+                    if constexpr (FIX) fl = lo[(j + W - M - 1) % W];
+                    else fl = lo[0];
+                    const double2 F64 = d2((double)fh.x + (double)fl.x, (double)fh.y + (double)fl.y);
+                    const double2 u2 = tod(p1);
+                    const double2 nb = d2((double)p2.x + (double)o.x, (double)p2.y + (double)o.y);
+                    const double2 au = adi_apply(u2, nb, ad);
+                    emit(yf, tof(jacobi_update(u2, d2(F64.x - au.x, F64.y - au.y), ad, dn)));
+                }
+                p2 = p1;
+                p1 = o;
+            } else {
+                if (y >= 0) emit(y, o);
+            }
+        }
+    }
+    const double dx = (double)x + shift;
+    double dg[8] = {s_w, dx * s_w, dx * dx * s_w, s_yw, s_yyw, s_re, s_im, s_cen};
+    if (!xin)
+        for (int k = 0; k < 8; k++) dg[k] = 0;
+#pragma unroll
+    for (int k = 0; k < 8; k++)
+        for (int o = 32; o > 0; o >>= 1) dg[k] += __shfl_xor(dg[k], o);
+    if (lane == 0)
+        for (int k = 0; k < 8; k++) wsum[w * 8 + k] = dg[k];
+    __syncthreads();
+    if (t < 8) {
+        double v = wsum[t];
+        for (int q2 = 1; q2 < nw; q2++) v += wsum[q2 * 8 + t];
+        a.tilediag[fs * 8 + t] = v;
+    }
+}
+
+// ---------------------------------------------------------------------------
 // slippage on the device (Control::applySlippage, Control.cpp:158-276): the
 // ring offset, the slices to zero, the live field buffer and the step index.
 // ---------------------------------------------------------------------------
@@ -1662,6 +1940,12 @@ void launch_main(hipStream_t s, const MixArrays &a, int sc, int nt, int lds) {
 }
 
 template <int M, bool REF>
+int set_wide_attr() {
+    return hipFuncSetAttribute((const void *)k_mix_adi_wide<M, REF>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                               (int)wide_lds_bytes<M, REF>(8)) == hipSuccess;
+}
+
+template <int M, bool REF>
 int set_persistent_attr() {
     return hipFuncSetAttribute((const void *)k_mix_step_persistent<M, REF>,
                                hipFuncAttributeMaxDynamicSharedMemorySize, kMixDyn * (int)sizeof(float2)) ==
@@ -1706,7 +1990,8 @@ int mix_prepare() {
     G4_M(true, 256) G4_M(true, 384) G4_M(true, 512) G4_M(false, 256) G4_M(false, 384) G4_M(false, 512)
 #undef G4_M
 #define G4_P(MM) ok &= set_adi_attr<MM, false>(); if (MixGeom<MM, true>::lds_bytes <= 160 * 1024) ok &= set_adi_attr<MM, true>(); \
-    ok &= set_persistent_attr<MM, false>(); ok &= set_persistent_attr<MM, true>();
+    ok &= set_persistent_attr<MM, false>(); ok &= set_persistent_attr<MM, true>(); \
+    ok &= set_wide_attr<MM, false>(); ok &= set_wide_attr<MM, true>();
     G4_P(3) G4_P(4) G4_P(6) G4_P(9) G4_P(12)
 #undef G4_P
     return ok ? 0 : -1;
@@ -1732,8 +2017,31 @@ void launch_mix_beam(hipStream_t s, const MixArrays &a, bool sc, bool moments, h
     mark(6);
 }
 
-int launch_mix_adi(hipStream_t s, const MixArrays &a, int tmpl_m, bool refine, bool stream, int *ntiles) {
+int launch_mix_adi(hipStream_t s, const MixArrays &a, int tmpl_m, bool refine, int mode, int *ntiles) {
+    if (mode == kAdiWide && a.ng > 512) mode = kAdiStrips;  // eight wavefronts cover 512 columns
+    const bool stream = mode != kAdiTile;
     if (refine && !stream && mix_band_lds(tmpl_m, true) > 160 * 1024) return -1;
+    if (mode == kAdiWide) {
+        // whole-width kernel: one workgroup of ceil(ng / 64) wavefronts per slice
+        *ntiles = 1;
+        const int nw = std::max(1, (a.ng + 63) / 64);
+#define G4_W(MM)                                                                                          \
+    else if (tmpl_m == MM) {                                                                              \
+        if (refine) {                                                                                     \
+            const size_t lds = wide_lds_bytes<MM, true>(nw);                                              \
+            hipLaunchKernelGGL((k_mix_adi_wide<MM, true>), dim3(a.ns), dim3(64 * nw), lds, s, a);         \
+        } else {                                                                                          \
+            const size_t lds = wide_lds_bytes<MM, false>(nw);                                             \
+            hipLaunchKernelGGL((k_mix_adi_wide<MM, false>), dim3(a.ns), dim3(64 * nw), lds, s, a);        \
+        }                                                                                                 \
+    }
+        if (false) {
+        }
+        G4_W(3) G4_W(4) G4_W(6) G4_W(9) G4_W(12)
+        else return -1;
+#undef G4_W
+        return 0;
+    }
 #define G4_A(MM)                                         \
     else if (tmpl_m == MM) {                             \
         if (refine) launch_adi_t<MM, true>(s, a);        \
@@ -1775,10 +2083,10 @@ void launch_mix_tail(hipStream_t s, const MixArrays &a, int maxzero, int ntiles,
 }
 
 int launch_mix_step(hipStream_t s, const MixArrays &a, int tmpl_m, bool refine, bool sc,
-                    bool moments, int maxzero, hipEvent_t *ev, bool stream, int main_nt, int main_lds) {
+                    bool moments, int maxzero, hipEvent_t *ev, int adi_mode, int main_nt, int main_lds) {
     launch_mix_beam(s, a, sc, moments, ev, false, main_nt, main_lds);
     int ntiles = 0;
-    if (launch_mix_adi(s, a, tmpl_m, refine, stream, &ntiles) != 0) return -1;
+    if (launch_mix_adi(s, a, tmpl_m, refine, adi_mode, &ntiles) != 0) return -1;
     launch_mix_tail(s, a, maxzero, ntiles, ev);
     return 0;
 }

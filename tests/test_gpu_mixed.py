@@ -35,19 +35,23 @@ def _field_err(ctx, sim):
     return float(np.max(np.abs(f - sim.field)) / np.max(np.abs(sim.field)))
 
 
+@pytest.mark.parametrize("mode", [2, 1, 0])
 @pytest.mark.parametrize("refine", [1, 0])
-def test_mixed_adi_against_oracle(st, refine):
+def test_mixed_adi_against_oracle(st, refine, mode):
     """Field-only steps (zero current: no source) over undulator, quadrupole
     and drift step sizes (band half widths 3, 4, 6 on 151^2): the FP32 band
-    ADI, with and without the FP64 residual correction, against the FP64
-    Thomas ADI of the oracle.  Bound: the FP32 rounding of the stored field
-    per step (2^-24 relative) times the step count, with margin."""
-    _log(f"mixed ADI vs oracle, refine={refine}: 4 slices x 256, 151^2, steps 120..130")
+    ADI, with and without the FP64 residual correction, in each kernel
+    (2: whole-width streaming, 1: streaming strips, 0: 32x32 tiles), against
+    the FP64 Thomas ADI of the oracle.  Bound: the FP32 rounding of the
+    stored field per step (2^-24 relative) times the step count, with
+    margin."""
+    _log(f"mixed ADI vs oracle, refine={refine}, kernel {mode}: 4 slices x 256, 151^2, steps 120..130")
     # steady state: no slippage, so the seed stays on the grid for every step
     sim = H.make_state([256] * 4, seed_power=5e3, timerun=False)
     sim.current[:] = 0.0
     ctx = H.to_gpu(sim, precision=g4mi.FP32)
     ctx.set_schedule("adi_refine", refine)
+    ctx.set_schedule("mixed_adi", mode)
     idx = list(range(120, 131))
     errs = []
     for i in idx:
@@ -55,12 +59,42 @@ def test_mixed_adi_against_oracle(st, refine):
         ctx.step(H.gsteps(st)[i], cases.GAMMA0)
         errs.append(_field_err(ctx, sim))
     sched = ctx.schedule()
+    assert sched["adi"] == {2: "band32-wide", 1: "band32-stream", 0: "band32-tile"}[mode]
     _log(f"  dz {[round(st[i].dz, 3) for i in idx]}\n  field rel err per step {['%.2e' % e for e in errs]}"
          f"\n  schedule {sched}")
     bound = (2.0 if refine else 8.0) * 2.0 ** -24 * len(idx)
     assert np.all(np.isfinite(errs)), errs
     assert max(errs) < bound, (max(errs), bound)
     ctx.close()
+
+
+def test_mixed_adi_kernels_agree_with_source(st):
+    """The three ADI kernels of the mixed form on a SASE window with a
+    source, long-range space charge and slippage (10 slices x 1024, 131^2:
+    the whole-width kernel then has idle lanes and a partly filled last
+    wavefront) over 40 steps: records and final field of the whole-width
+    kernel and of the strips against the tile kernel.  All three round the
+    same FP32 band products in a different order only, so they agree to
+    a few FP32 ulps per step."""
+    _log("mixed ADI kernels agree: 10 slices x 1024, 131^2, 40 steps, modes 2, 1, 0")
+    out = {}
+    for mode in (2, 1, 0):
+        sim = H.make_state([1024] * 10, ngrid=131, gridmax=0.4e-3, longrange=True, seed_power=1e4)
+        ctx = H.to_gpu(sim, precision=g4mi.FP32)
+        ctx.set_schedule("mixed_adi", mode)
+        r = ctx.run(H.gsteps(st[120:160]), cases.GAMMA0, record=("power", "bunching"))
+        f, first = ctx.download_field()
+        out[mode] = (r, f, first)
+        ctx.close()
+    r0, f0, q0 = out[0]
+    for mode in (2, 1):
+        r, f, q = out[mode]
+        assert q == q0
+        ep = float(np.max(np.abs(r["power"] - r0["power"])) / np.max(np.abs(r0["power"])))
+        eb = float(np.max(np.abs(r["bunching"] - r0["bunching"])) / np.max(np.abs(r0["bunching"])))
+        ef = float(np.max(np.abs(f - f0)) / np.max(np.abs(f0)))
+        _log(f"  kernel {mode} vs tiles: power {ep:.3e}, bunching {eb:.3e}, field {ef:.3e}")
+        assert ep < 1e-4 and eb < 1e-4 and ef < 1e-4
 
 
 def test_mixed_free_push_offset_form(st):
@@ -198,7 +232,7 @@ def test_mixed_persistent_step_matches_kernels(st):
 
 def test_mixed_autotuner_picks_a_schedule(st):
     """The first run of 14 or more steps times every main-kernel shape (4, 6,
-    8 wavefronts per slice x 68 or 44 KiB of LDS), both ADI kernels, the
+    8 wavefronts per slice x 68 or 44 KiB of LDS), the three ADI kernels, the
     graph of kernels and the persistent launch, and keeps the fastest; the
     run's records stay within 1 % of the FP64 oracle."""
     _log("mixed autotuner: 20 steps, 12 slices x 1024, 63^2")
@@ -211,7 +245,8 @@ def test_mixed_autotuner_picks_a_schedule(st):
     assert sched["tuned"] == 1
     tun = sched["tuning"]
     assert len(tun["main_ms"]) == 6 and all(v > 0 for v in tun["main_ms"].values())
-    assert tun["adi_ms"]["stream"] > 0 and tun["step_ms"]["graph"] > 0
+    assert tun["adi_ms"]["wide"] > 0 and tun["adi_ms"]["strips"] > 0 and tun["step_ms"]["graph"] > 0
+    assert sched["adi"] in ("band32-wide", "band32-stream", "band32-tile")
     assert sched["main_nt"] in (256, 384, 512) and sched["main_lds_kib"] in (44, 68)
     r = sim.run(st, cases.GAMMA0, record=("power", "bunching"), nsteps=n)
     pg, pr = g["power"][1:].sum(axis=1), r["power"][1:].sum(axis=1)
