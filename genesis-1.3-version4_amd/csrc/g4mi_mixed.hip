@@ -1134,14 +1134,20 @@ struct StreamGeom {
     static constexpr int W = 2 * M + 1;
     static constexpr int H = REF ? M + 1 : M;  // halo lanes on each side
     static constexpr int SO = 64 - 2 * H;      // output columns per strip
-    // FIX (narrow bands): the row loop is unrolled by W and the field queues,
-    // the F ring and its remainders all have W slots, so every slot of row r
-    // is the fixed register r mod W and no value moves between registers;
-    // the queues then prefetch W - 2 rows ahead.  Wider bands keep queues
-    // of 5 rows and shift their rings (register budget).
+    // FIX (narrow bands): the row loop is unrolled by W; the F ring and its
+    // remainders have W slots and the field queues QN slots (QN divides W),
+    // so every slot of row r is the fixed register r mod W (r mod QN) and no
+    // value moves between registers.  Wider bands keep queues of 5 rows and
+    // shift their rings (register budget).
     static constexpr bool FIX = M <= 4;
-    static constexpr int QN = FIX ? W : 5;     // queue slots
+    // queue slots: with FIX a divisor of the unroll W.  Three slots (rows
+    // r-1, r, r+1) prefetch one row ahead, which covers the HBM latency at
+    // a row's duration (thousands of cycles) and keeps the registers for
+    // occupancy.
+    static constexpr int QN = FIX ? (W % 3 == 0 ? 3 : W) : 5;
     static constexpr int PF = QN - 2;          // field rows prefetched ahead
+    static constexpr int UN = FIX ? W : QN;    // row-loop unroll (a multiple of QN)
+    static_assert(UN % QN == 0, "queue slots must divide the unroll");
     static constexpr int RS = FIX ? W : W;     // F ring slots
 };
 
@@ -1167,7 +1173,7 @@ template <int M, bool REF>
 __device__ void adi_stream_body(const MixArrays &a, const int ii, const int strip, float2 (*lrow)[64],
                                 float2 (*lu1)[64]) {
     using G = StreamGeom<M, REF>;
-    constexpr int W = G::W, H = G::H, SO = G::SO, PF = G::PF;
+    constexpr int W = G::W, H = G::H, SO = G::SO, PF = G::PF, UN = G::UN;
     constexpr bool FIX = G::FIX;
     const int lane = threadIdx.x & 63;
     const MixStep &S = a.steps[a.st->istep];
@@ -1213,8 +1219,9 @@ __device__ void adi_stream_body(const MixArrays &a, const int ii, const int stri
         return *(in ? scol + (long long)rr * ng : g_zero_row + xcl);
     };
     // queues of QN = PF+2 rows, indexed modulo QN: the row loop is unrolled
-    // by QN so every slot is a fixed register (no register moves of loads in
-    // flight).  Slot (r + 1 + k) % QN holds row r + k, k = -1..PF.
+    // by UN, a multiple of QN, so every slot is a fixed register (no register
+    // moves of loads in flight).  Slot (r + 1 + k) % QN holds row r + k,
+    // k = -1..PF.
     constexpr int QN = G::QN;
     float2 q[QN], sq[QN];
     q[0] = f2(0, 0);
@@ -1254,18 +1261,24 @@ __device__ void adi_stream_body(const MixArrays &a, const int ii, const int stri
         }
     };
     const int nrow = ng + M + (REF ? 1 : 0);
-    for (int r0 = 0; r0 < nrow; r0 += QN) {
+    for (int r0 = 0; r0 < nrow; r0 += UN) {
+        // j = jj QN + jq: the queue slot of row r0 + j is a function of jq
+        // alone (a remainder of the flat index here keeps the loop from
+        // unrolling)
 #pragma unroll
-        for (int j = 0; j < QN; j++) {
+        for (int jj = 0; jj < UN / QN; jj++)
+#pragma unroll
+        for (int jq = 0; jq < QN; jq++) {
+            const int j = jj * QN + jq;
             const int r = r0 + j;
             if (r >= nrow) break;
             // slots: row r-1 -> j, row r -> (j+1)%QN, row r+1 -> (j+2)%QN
             float2 Fr = f2(0, 0), Flo = f2(0, 0);
             if (r < ng) {
-                const float2 sv = sq[(j + 1) % QN];
-                const float2 uc = q[(j + 1) % QN];
-                const float2 un = (r + 1 < ng) ? q[(j + 2) % QN] : f2(0, 0);
-                const float2 ul = (r > 0) ? q[j] : f2(0, 0);
+                const float2 sv = sq[(jq + 1) % QN];
+                const float2 uc = q[(jq + 1) % QN];
+                const float2 un = (r + 1 < ng) ? q[(jq + 2) % QN] : f2(0, 0);
+                const float2 ul = (r > 0) ? q[jq] : f2(0, 0);
                 double2 E64 = d2(0, 0);
                 float2 E = f2(0, 0);
                 if constexpr (REF) {
@@ -1320,8 +1333,8 @@ __device__ void adi_stream_body(const MixArrays &a, const int ii, const int stri
                 }
                 // slot j (row r-1) is free: it takes row r + PF + 1, first used
                 // PF rows later
-                q[j] = uload(r + PF + 1);
-                sq[j] = sload(r + PF + 1);
+                q[jq] = uload(r + PF + 1);
+                sq[jq] = sload(r + PF + 1);
             }
             // F of rows r - W + 1 .. r: window index k = 0..W-1 (k = W-1 the
             // newest row).  FIX: row r sits in slot r mod W = j, index k in
@@ -1440,10 +1453,11 @@ size_t wide_lds_bytes(int nw) {
            (size_t)(2 * M + 1) * nc * sizeof(float2) + (size_t)nw * 8 * sizeof(double);
 }
 
+// three wavefronts per SIMD where the registers allow it without spilling
 template <int M, bool REF>
-__global__ void __launch_bounds__(512) k_mix_adi_wide(MixArrays a) {
+__global__ void __launch_bounds__(512, (REF && M == 9) ? 2 : 3) k_mix_adi_wide(MixArrays a) {
     using G = StreamGeom<M, REF>;
-    constexpr int W = G::W, QN = G::QN, PF = G::PF;
+    constexpr int W = G::W, QN = G::QN, PF = G::PF, UN = G::UN;
     constexpr bool FIX = G::FIX;
     extern __shared__ double2 wide_lds[];
     const int nt = blockDim.x, nw = nt >> 6, nc = nt;
@@ -1518,15 +1532,20 @@ __global__ void __launch_bounds__(512) k_mix_adi_wide(MixArrays a) {
 #pragma unroll
     for (int m = 0; m < NL; m++) lo[m] = f2(0, 0);
     float2 p1 = f2(0, 0), p2 = f2(0, 0);
+    // REF: the field rows r-1, r (and r+1 while row r is formed) widened to
+    // FP64 once each, as they enter the three-row window
+    double2 d_l = d2(0, 0), d_c = tod(q[1]);
     // field sums of this column: x is fixed, so the x moments are dx, dx^2
     // times the column's power, applied at the end
     double s_w = 0, s_yw = 0, s_yyw = 0, s_re = 0, s_im = 0, s_cen = 0;
     const bool xcen = x == ng / 2;
     const double shift = -0.5 * (double)(ng - 1);
-    auto emit = [&](int y, float2 o) {
+    // output row y: the stored FP32 value o, the sums from o64 (the value
+    // before rounding with the correction)
+    auto emit = [&](int y, float2 o, double2 o64) {
         if (xin) {
             uo[(long long)y * ng + x] = o;
-            const double ox = o.x, oy = o.y;
+            const double ox = o64.x, oy = o64.y;
             const double wei = ox * ox + oy * oy;
             const double dy = (double)y + shift;
             s_w += wei;
@@ -1540,25 +1559,36 @@ __global__ void __launch_bounds__(512) k_mix_adi_wide(MixArrays a) {
     lds_barrier();
     float2 *lx = lrow0 + x + M;  // this column in E row 0; row 1 is lp further
     const int nrow = ng + M + (REF ? 1 : 0);
-    for (int r0 = 0; r0 < nrow; r0 += QN) {
+    for (int r0 = 0; r0 < nrow; r0 += UN) {
+        // j = jj QN + jq: the queue slot of row r0 + j is a function of jq
+        // alone (a remainder of the flat index here keeps the loop from
+        // unrolling)
 #pragma unroll
-        for (int j = 0; j < QN; j++) {
+        for (int jj = 0; jj < UN / QN; jj++)
+#pragma unroll
+        for (int jq = 0; jq < QN; jq++) {
+            const int j = jj * QN + jq;
             const int r = r0 + j;
             if (r >= nrow) break;
             float2 Fr = f2(0, 0), Flo = f2(0, 0);
             if (r < ng) {
-                const float2 sv = sq[(j + 1) % QN];
-                const float2 uc = q[(j + 1) % QN];
-                const float2 un = (r + 1 < ng) ? q[(j + 2) % QN] : f2(0, 0);
-                const float2 ul = (r > 0) ? q[j] : f2(0, 0);
+                const float2 sv = sq[(jq + 1) % QN];
+                const float2 uc = q[(jq + 1) % QN];
+                const float2 un = (r + 1 < ng) ? q[(jq + 2) % QN] : f2(0, 0);
+                const float2 ul = (r > 0) ? q[jq] : f2(0, 0);
                 double2 E64 = d2(0, 0);
                 float2 E = f2(0, 0);
+                double2 sv64 = d2(0, 0);
                 if constexpr (REF) {
+                    const double2 d_n = (r + 1 < ng) ? tod(un) : d2(0, 0);
                     if (xin) {
-                        const double sx = (double)un.x - 2.0 * (double)uc.x + (double)ul.x;
-                        const double sy = (double)un.y - 2.0 * (double)uc.y + (double)ul.y;
-                        E64 = d2((double)sv.x + (double)uc.x - ad * sy, (double)sv.y + (double)uc.y + ad * sx);
+                        sv64 = tod(sv);
+                        const double sx = fma(-2.0, d_c.x, d_n.x) + d_l.x;
+                        const double sy = fma(-2.0, d_c.y, d_n.y) + d_l.y;
+                        E64 = d2(fma(-ad, sy, sv64.x + d_c.x), fma(ad, sx, sv64.y + d_c.y));
                     }
+                    d_l = d_c;
+                    d_c = d_n;
                     E = tof(E64);
                 } else {
                     const float sx = un.x - 2.f * uc.x + ul.x, sy = un.y - 2.f * uc.y + ul.y;
@@ -1586,15 +1616,15 @@ __global__ void __launch_bounds__(512) k_mix_adi_wide(MixArrays a) {
                     const double2 au = adi_apply(u1, d2(nl.x + nr.x, nl.y + nr.y), ad);
                     const double2 upd = jacobi_update(u1, d2(E64.x - au.x, E64.y - au.y), ad, dn);
                     if (xin) {
-                        const double2 F64 = d2((double)sv.x + 2.0 * upd.x - E64.x, (double)sv.y + 2.0 * upd.y - E64.y);
+                        const double2 F64 = d2(fma(2.0, upd.x, sv64.x) - E64.x, fma(2.0, upd.y, sv64.y) - E64.y);
                         Fr = tof(F64);
                         Flo = f2((float)(F64.x - (double)Fr.x), (float)(F64.y - (double)Fr.y));
                     }
                 } else {
                     Fr = xin ? f2(sv.x + 2.f * up.x - E.x, sv.y + 2.f * up.y - E.y) : f2(0, 0);
                 }
-                q[j] = uload(r + PF + 1);
-                sq[j] = sload(r + PF + 1);
+                q[jq] = uload(r + PF + 1);
+                sq[jq] = sload(r + PF + 1);
             }
             if constexpr (FIX) {
                 ring[j % W] = Fr;
@@ -1639,12 +1669,13 @@ __global__ void __launch_bounds__(512) k_mix_adi_wide(MixArrays a) {
                     const double2 u2 = tod(p1);
                     const double2 nb = d2((double)p2.x + (double)o.x, (double)p2.y + (double)o.y);
                     const double2 au = adi_apply(u2, nb, ad);
-                    emit(yf, tof(jacobi_update(u2, d2(F64.x - au.x, F64.y - au.y), ad, dn)));
+                    const double2 res = jacobi_update(u2, d2(F64.x - au.x, F64.y - au.y), ad, dn);
+                    emit(yf, tof(res), res);
                 }
                 p2 = p1;
                 p1 = o;
             } else {
-                if (y >= 0) emit(y, o);
+                if (y >= 0) emit(y, o, tod(o));
             }
         }
     }
