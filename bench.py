@@ -147,8 +147,7 @@ def roofline(cfg, kt, ntot, precision=g4mi.FP64, schedule=None):
     if name == "adi" and precision == g4mi.FP64 and schedule:
         # the ADI kernel the tuner chose for the undulator step (the timed steps)
         und = (schedule.get("adi_by_delz") or {}).get(f"{W.DELZ:.6g}", schedule.get("adi"))
-        if und != "band":
-            dev = None  # the traffic table holds the banded kernel only
+        dev = {"band": "k_adi_band", "band-wide": "k_adi_wide64"}.get(und)  # traffic rows: banded kernels
     traffic = measured_traffic(cfg["name"], dev) if dev else None
     return name, dict(bound="hbm", achieved=round(gbs, 1), peak=HBM_PEAK_GBS, unit="GB/s",
                       frac=round(gbs / HBM_PEAK_GBS, 4),

@@ -79,7 +79,7 @@ struct Ctx {
     int adi_kernel = -1;
     bool adi_fixed = false;               // set by g4mi_set_schedule("adi_kernel", k >= 0)
     std::map<double, int> adi_choice;     // the tuner's kernel per step size
-    float adi_ms[4] = {-1.f, -1.f, -1.f, -1.f};
+    float adi_ms[5] = {-1.f, -1.f, -1.f, -1.f, -1.f};
     // spectral solver (FieldSolverFFT) instead of the ADI, with the source
     // filter's settings; the propagator matrix is cached per delz
     int fft = 0, fft_filter = 0;
@@ -336,6 +336,8 @@ static int ensure_factors(Ctx *c, double delz) {
         if (!c->adi_tile) c->adi_kernel = -1;      // re-tune if no longer eligible
     } else if (c->adi_kernel == 3 && !c->adi_band_m) {
         c->adi_kernel = -1;
+    } else if (c->adi_kernel == 4 && !adi_wide64_ok(c->ng, c->adi_band_m, c->adi_klo, c->adi_khi)) {
+        c->adi_kernel = -1;
     }
     G4_HIP(hipMemcpyAsync(c->d_fac, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice,
                           c->stream));
@@ -464,7 +466,10 @@ static int beam_track(Ctx *c, const g4mi_step_params &st, double gamma0, double 
 // One ADI launch of kernel `kind` on field buffer `fin` (the banded kernel
 // writes `fout`, the others work in place using `fout` as scratch).
 static void adi_launch(Ctx *c, int kind, const AdiFactors &f, double2 *fin, double2 *fout) {
-    if (kind == 3)
+    if (kind == 4)
+        (void)launch_adi_wide64(c->stream, c->ns, fin, fout, c->d_src, c->d_box, c->d_band, c->cstep, c->ng,
+                                c->first, c->adi_band_m, c->adi_klo, c->adi_khi, c->d_diag_f);
+    else if (kind == 3)
         launch_adi_band(c->stream, c->ns, fin, fout, c->d_src, c->d_box, c->d_band, c->cstep,
                         c->ng, c->first, c->adi_band_m, c->adi_klo, c->adi_khi, c->d_tilediag,
                         c->d_diag_f);
@@ -489,10 +494,11 @@ static int tune_adi(Ctx *c, const AdiFactors &f) {
     G4_HIP(hipMemcpyAsync(scratch, c->d_field, bytes, hipMemcpyDeviceToDevice, c->stream));
     int best_kind = 0;
     float best_ms = 1e30f;
-    for (int kind = 0; kind < 4; kind++) {
+    for (int kind = 0; kind < 5; kind++) {
         c->adi_ms[kind] = -1.f;
         if ((kind == 1 || kind == 2) && !c->adi_tile) continue;
         if (kind == 3 && !c->adi_band_m) continue;
+        if (kind == 4 && !adi_wide64_ok(c->ng, c->adi_band_m, c->adi_klo, c->adi_khi)) continue;
         adi_launch(c, kind, f, scratch, c->d_work);  // warm-up
         float best = 1e30f;
         for (int rep = 0; rep < 3; rep++) {
@@ -696,7 +702,7 @@ static int field_track(Ctx *c, const g4mi_step_params &st, double delz, bool dep
     AdiFactors f{c->d_fac, c->d_fac + c->ng, c->d_fac + 2 * c->ng, c->cstep, c->ng};
     if (c->adi_kernel < 0) G4_TRY(tune_adi(c, f));
     adi_launch(c, c->adi_kernel, f, c->d_field, c->d_work);
-    if (c->adi_kernel == 3) std::swap(c->d_field, c->d_work);  // result is in the other buffer
+    if (c->adi_kernel >= 3) std::swap(c->d_field, c->d_work);  // result is in the other buffer
     c->field_diag_fresh = c->adi_kernel != 0;
     c->zeroed_since_adi.clear();
     G4_HIP(hipGetLastError());
@@ -2650,7 +2656,7 @@ int g4mi_sort_index_noalloc(g4mi_ctx *h, double slen, int *atar, long long *move
 int g4mi_set_schedule(g4mi_ctx *h, const char *name, int value) {
     if (!h || !name) return G4MI_E_ARG;
     Ctx *c = &h->c;
-    if (std::string(name) == "adi_kernel" && value >= -1 && value <= 3) {
+    if (std::string(name) == "adi_kernel" && value >= -1 && value <= 4) {
         c->adi_kernel = value;
         c->adi_fixed = value >= 0;
         if (value < 0) c->adi_choice.clear();
@@ -2712,7 +2718,7 @@ int g4mi_get_schedule(g4mi_ctx *h, char *buf, int len) {
     if (!h || !buf || len < 1) return G4MI_E_ARG;
     Ctx *c = &h->c;
     char tmp[4096];
-    static const char *names[4] = {"thomas", "fused", "fused-prefetch", "band"};
+    static const char *names[5] = {"thomas", "fused", "fused-prefetch", "band", "band-wide"};
     if (c->precision == G4MI_FP32) {
         std::string bands = "[";
         for (size_t k = 0; k < c->h_binfo.size(); k++)
@@ -2749,10 +2755,10 @@ int g4mi_get_schedule(g4mi_ctx *h, char *buf, int len) {
     bydz += "}";
     snprintf(tmp, sizeof(tmp),
              "{\"field_solver\": \"adi\", \"adi_kernel\": %d, \"adi\": \"%s\", \"adi_window\": %d, "
-             "\"adi_band_m\": %d, \"adi_tuned_ms\": [%.4f, %.4f, %.4f, %.4f], \"adi_by_delz\": %s, "
+             "\"adi_band_m\": %d, \"adi_tuned_ms\": [%.4f, %.4f, %.4f, %.4f, %.4f], \"adi_by_delz\": %s, "
              "\"fp64_fused\": %d}",
              c->adi_kernel, c->adi_kernel >= 0 ? names[c->adi_kernel] : "untuned", c->adi_tile,
-             c->adi_band_m, c->adi_ms[0], c->adi_ms[1], c->adi_ms[2], c->adi_ms[3], bydz.c_str(),
+             c->adi_band_m, c->adi_ms[0], c->adi_ms[1], c->adi_ms[2], c->adi_ms[3], c->adi_ms[4], bydz.c_str(),
              c->fuse64 ? 1 : 0);
     snprintf(buf, (size_t)len, "%s", tmp);
     return G4MI_OK;

@@ -90,6 +90,15 @@ int adi_band_tiles(int ng);
 int launch_adi_band(hipStream_t s, int ns, const double2 *fin, double2 *fout, const double2 *src,
                     const int4 *box, const double2 *band, double a, int ng, int first, int m,
                     int klo, int khi, double *tilediag, double *diag);
+// The banded operator as a whole-width stream (one workgroup per slice, one
+// lane per column; the products of launch_adi_band as four partial sums,
+// diagnostics straight into diag[8 ns]).  Eligible for band half widths 7 and 8, grids up to 512
+// columns whose rows [klo, khi] share the central band except in the first
+// and last 32 columns; returns -1 otherwise.
+bool adi_wide64_ok(int ng, int m, int klo, int khi);
+int launch_adi_wide64(hipStream_t s, int ns, const double2 *fin, double2 *fout, const double2 *src,
+                      const int4 *box, const double2 *band, double a, int ng, int first, int m, int klo, int khi,
+                      double *diag);
 void launch_zero(hipStream_t s, double2 *p, long long n);
 // n doubles from device memory to pinned host memory (dst: its device address)
 void launch_copy_out(hipStream_t s, const double *src, double *dst, long long n);

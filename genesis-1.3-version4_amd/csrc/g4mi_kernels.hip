@@ -1839,6 +1839,199 @@ __global__ void __launch_bounds__(64) k_adi_tile_diag(const double *__restrict__
     for (int q = 0; q < 8; q++) diag[8 * fs + q] = v[q];
 }
 
+// ---------------------------------------------------------------------------
+// K5+K6 as a whole-width stream: the banded operator of k_adi_band, both
+// half steps of one slice per workgroup of NW = blockDim / 64 wavefronts,
+// one lane per column, walking down the rows.  Row r:
+//   E(r)     = S + u + i a (u(r+1) - 2 u(r) + u(r-1))       (own column)
+//   u'(r)    = sum_m band[x][m] E(r, x+m-M)                  (E row via LDS)
+//   F(r)     = S + 2 u'(r) - E(r)                             (register ring)
+//   u''(r-M) = sum_m band[r-M][m] F(r-2M+m)                   (ring of W rows)
+// The products of k_adi_band, each summed as four partial sums (band terms
+// k mod 4: independent FMA chains); no cell is computed twice (a
+// 32 x 32 tile of k_adi_band evaluates its x stage on 32 + 2M rows and
+// loads a halo of M + 1 cells) and F never leaves the registers.
+//
+// Column map: wavefront 0 takes the first and the last 32 columns, where
+// the band rows differ from the central one (their rows sit in LDS); the
+// other wavefronts take the columns in between and use the symmetric
+// central row (the host offers this kernel when [klo, khi] covers them).
+// The E row is double-buffered by row parity with one LDS-only barrier per
+// row; the field and source rows are loaded two rows ahead; the F ring has
+// W slots with row r in slot r mod W (the row loop is unrolled by W).
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void adi_lds_barrier() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+    __builtin_amdgcn_s_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
+// a zero cell the streaming kernel reads for source rows outside the box
+// (a select between two global addresses: a local zero would be addressed
+// in scratch memory)
+__device__ double2 g_zero_cell64;
+
+template <int M>
+size_t wide64_lds_bytes(int nw) {
+    return (size_t)2 * (64 * nw + 2 * M) * sizeof(double2) + (size_t)(2 * M + 1) * 64 * sizeof(double2) +
+           (size_t)nw * 8 * sizeof(double);
+}
+
+template <int M>
+__global__ void __launch_bounds__(512, 3) k_adi_wide64(const double2 *__restrict__ fin,
+                                                     double2 *__restrict__ fout,
+                                                     const double2 *__restrict__ src,
+                                                     const int4 *__restrict__ box,
+                                                     const double2 *__restrict__ band, double a, int ng,
+                                                     int first, int ns, int klo, int khi,
+                                                     double *__restrict__ diag) {
+    constexpr int W = 2 * M + 1;
+    extern __shared__ double2 wsm[];
+    const int nt = blockDim.x, nw = nt >> 6;
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+    const int lp = nt + 2 * M;
+    double2 *lrow = wsm;                           // [2][lp] E rows, cell x at x + M
+    double2 *lband = wsm + 2 * lp;                 // [W][64] band rows of wavefront 0's columns
+    double *wsum = (double *)(lband + W * 64);     // [nw][8]
+    const int ii = blockIdx.x;
+    if (ii >= ns) return;
+    // this lane's column (lanes left over sit on column ng: zero)
+    int x;
+    if (ng <= 64) x = lane < ng ? lane : ng;
+    else if (w == 0) x = lane < 32 ? lane : ng - 64 + lane;
+    else {
+        x = 32 + 64 * (w - 1) + lane;
+        if (x >= ng - 32) x = ng;
+    }
+    const bool xin = x < ng;
+    bool wsym;  // every grid column of this wavefront uses the central band row (uniform)
+    if (ng <= 64 || w == 0) wsym = klo <= 0 && khi >= ng - 1;
+    else {
+        const int xlo = 32 + 64 * (w - 1), xhi = min(xlo + 63, ng - 33);
+        wsym = xlo > xhi || (xlo >= klo && xhi <= khi);
+    }
+    for (int k = t; k < 2 * lp; k += nt) lrow[k] = make_double2(0, 0);
+    if (w == 0)
+        for (int m = 0; m < W; m++)
+            lband[m * 64 + lane] = xin ? band[(long long)x * W + m] : make_double2(0, 0);
+    const double2 *bc = band + (long long)((klo + khi) / 2) * W;
+    const long long fs = (long long)((ii + first) % ns);
+    const double2 *u = fin + fs * ng * ng;
+    double2 *uo = fout + fs * ng * ng;
+    const int4 bx = box[ii];
+    const bool sxin = xin && x >= bx.x && x <= bx.z;
+    const int xcl = min(x, ng - 1);
+    const double2 *ucol = u + xcl;
+    const double2 *scol = src + (long long)ii * ng * ng + xcl;
+    const double2 zero = make_double2(0, 0);
+    auto uload = [&](int rr) -> double2 { return ucol[(long long)min(rr, ng - 1) * ng]; };
+    auto sload = [&](int rr) -> double2 {
+        return *((sxin && rr >= bx.y && rr <= bx.w) ? scol + (long long)rr * ng : &g_zero_cell64);
+    };
+    // u rows r-1 .. r+3 and S rows r .. r+2 at the start of row r
+    double2 ul = zero, uc = uload(0), un = uload(1), up1 = uload(2), up2 = uload(3);
+    double2 sv = sload(0), sp1 = sload(1), sp2 = sload(2);
+    double2 ring[W];
+#pragma unroll
+    for (int m = 0; m < W; m++) ring[m] = zero;
+    // field sums of this column (x fixed: the x moments are dx, dx^2 times
+    // the column's power)
+    double s_w = 0, s_yw = 0, s_yyw = 0, s_re = 0, s_im = 0, s_cen = 0;
+    const bool xcen = x == ng / 2;
+    const double shift = -0.5 * (double)(ng - 1);
+    adi_lds_barrier();
+    double2 *lx = lrow + x + M;
+    const int nrow = ng + M;
+    for (int r0 = 0; r0 < nrow; r0 += W) {
+#pragma unroll
+        for (int j = 0; j < W; j++) {
+            const int r = r0 + j;
+            double2 F = zero;
+            if (r < ng) {
+                // explicit y stencil of the x half step (FieldSolverADI.cpp:64-72)
+                double2 E = zero;
+                if (xin) {
+                    const double2 a1 = (r + 1 < ng) ? un : zero, a0 = (r > 0) ? ul : zero;
+                    const double2 st = cadd(csub(a1, cscale(uc, 2.0)), a0);
+                    E = cadd(cadd(sv, uc), make_double2(-a * st.y, a * st.x));
+                }
+                double2 *lr = lx + (r & 1) * lp;
+                lr[0] = E;
+                adi_lds_barrier();
+                // implicit x and the y half step's right-hand side
+                if (xin) {
+                    // four partial sums (band terms k mod 4): independent FMA
+                    // chains for the latency of the FP64 pipeline
+                    double2 pa[4] = {zero, zero, zero, zero};
+                    if (wsym) {
+                        pa[0] = cmul(bc[M], lr[0]);
+#pragma unroll
+                        for (int k = 1; k <= M; k++) pa[k & 3] = cfma(bc[M + k], cadd(lr[-k], lr[k]), pa[k & 3]);
+                    } else {
+#pragma unroll
+                        for (int m = 0; m < W; m++) pa[m & 3] = cfma(lband[m * 64 + lane], lr[m - M], pa[m & 3]);
+                    }
+                    const double2 acc = cadd(cadd(pa[0], pa[1]), cadd(pa[2], pa[3]));
+                    F = csub(cadd(sv, cscale(acc, 2.0)), E);
+                }
+                ul = uc;
+                uc = un;
+                un = up1;
+                up1 = up2;
+                up2 = uload(r + 4);
+                sv = sp1;
+                sp1 = sp2;
+                sp2 = sload(r + 3);
+            }
+            ring[j] = F;  // row r in slot r mod W
+            // F rows r-2M .. r (window index k in slot (j+1+k) mod W): a copy
+            // in index order, renamed away once the loop is unrolled
+            double2 rw[W];
+#pragma unroll
+            for (int k = 0; k < W; k++) rw[k] = ring[(j + 1 + k) % W];
+            const int y = r - M;
+            if (y >= 0 && y < ng && xin) {
+                // implicit y on F rows y-M .. y+M
+                double2 pa[4] = {zero, zero, zero, zero};
+                if (y >= klo && y <= khi) {
+                    pa[0] = cmul(bc[M], rw[M]);
+#pragma unroll
+                    for (int k = 1; k <= M; k++) pa[k & 3] = cfma(bc[M + k], cadd(rw[M - k], rw[M + k]), pa[k & 3]);
+                } else {
+                    const double2 *by = band + (long long)y * W;
+#pragma unroll
+                    for (int m = 0; m < W; m++) pa[m & 3] = cfma(by[m], rw[m], pa[m & 3]);
+                }
+                const double2 acc = cadd(cadd(pa[0], pa[1]), cadd(pa[2], pa[3]));
+                uo[(long long)y * ng + x] = acc;
+                const double wei = acc.x * acc.x + acc.y * acc.y;
+                const double dy = (double)y + shift;
+                s_w += wei;
+                s_yw += dy * wei;
+                s_yyw += dy * dy * wei;
+                s_re += acc.x;
+                s_im += acc.y;
+                if (xcen && y == ng / 2) s_cen = wei;
+            }
+        }
+    }
+    const double dx = (double)x + shift;
+    double dg[8] = {s_w, dx * s_w, dx * dx * s_w, s_yw, s_yyw, s_re, s_im, s_cen};
+    if (!xin)
+        for (int k = 0; k < 8; k++) dg[k] = 0;
+#pragma unroll
+    for (int k = 0; k < 8; k++)
+        for (int o = 32; o > 0; o >>= 1) dg[k] += __shfl_xor(dg[k], o);
+    if (lane == 0)
+        for (int k = 0; k < 8; k++) wsum[w * 8 + k] = dg[k];
+    __syncthreads();
+    if (t < 8) {
+        double v = wsum[t];
+        for (int q = 1; q < nw; q++) v += wsum[q * 8 + t];
+        diag[8 * fs + t] = v;
+    }
+}
+
 // Zero one field slice (slippage inflow, Control.cpp:255-267).  A plain
 // store kernel rather than a memset so the step can be graph-captured.
 __global__ void __launch_bounds__(256) k_zero(double2 *__restrict__ p, long long n) {
@@ -2202,6 +2395,34 @@ static void launch_adi_band_t(hipStream_t s, int ns, const double2 *fin, double2
 int adi_band_tiles(int ng) {
     const int ntx = (ng + 31) / 32;
     return ntx * ntx;
+}
+
+bool adi_wide64_ok(int ng, int m, int klo, int khi) {
+    return (m == 7 || m == 8) && ng <= 512 && (ng <= 64 || (klo <= 32 && khi >= ng - 33));
+}
+
+template <int M>
+static void launch_adi_wide64_t(hipStream_t s, int ns, const double2 *fin, double2 *fout, const double2 *src,
+                                const int4 *box, const double2 *band, double a, int ng, int first, int klo,
+                                int khi, double *diag) {
+    static bool attr = false;
+    if (!attr) {
+        (void)hipFuncSetAttribute((const void *)k_adi_wide64<M>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                  (int)wide64_lds_bytes<M>(8));
+        attr = true;
+    }
+    const int nw = std::max(1, (ng + 63) / 64);
+    hipLaunchKernelGGL(k_adi_wide64<M>, dim3(ns), dim3(64 * nw), wide64_lds_bytes<M>(nw), s, fin, fout, src, box,
+                       band, a, ng, first, ns, klo, khi, diag);
+}
+
+int launch_adi_wide64(hipStream_t s, int ns, const double2 *fin, double2 *fout, const double2 *src,
+                      const int4 *box, const double2 *band, double a, int ng, int first, int m, int klo, int khi,
+                      double *diag) {
+    if (!adi_wide64_ok(ng, m, klo, khi)) return -1;
+    if (m == 7) launch_adi_wide64_t<7>(s, ns, fin, fout, src, box, band, a, ng, first, klo, khi, diag);
+    else launch_adi_wide64_t<8>(s, ns, fin, fout, src, box, band, a, ng, first, klo, khi, diag);
+    return 0;
 }
 
 int launch_adi_band(hipStream_t s, int ns, const double2 *fin, double2 *fout, const double2 *src,

@@ -1597,16 +1597,23 @@ __global__ void __launch_bounds__(512, (REF && M == 9) ? 2 : 3) k_mix_adi_wide(M
                 float2 *lr = lx + (r & 1) * lp;
                 lr[0] = E;
                 lds_barrier();
-                float2 up = f2(0, 0);
+                float2 up = f2(0, 0), up1 = f2(0, 0);  // two partial sums (even, odd terms)
                 if (xin) {
                     if (wsym) {
                         up = fmul(bcr[0], lr[0]);
 #pragma unroll
-                        for (int k = 1; k <= M; k++) up = ffma(bcr[k], fadd(lr[-k], lr[k]), up);
+                        for (int k = 1; k <= M; k++) {
+                            if (k & 1) up1 = ffma(bcr[k], fadd(lr[-k], lr[k]), up1);
+                            else up = ffma(bcr[k], fadd(lr[-k], lr[k]), up);
+                        }
                     } else {
 #pragma unroll
-                        for (int m = 0; m < W; m++) up = ffma(lband[m * nc + t], lr[m - M], up);
+                        for (int m = 0; m < W; m++) {
+                            if (m & 1) up1 = ffma(lband[m * nc + t], lr[m - M], up1);
+                            else up = ffma(lband[m * nc + t], lr[m - M], up);
+                        }
                     }
+                    up = fadd(up, up1);
                 }
                 if constexpr (REF) {
                     const double2 u1 = tod(up);
@@ -1646,17 +1653,24 @@ __global__ void __launch_bounds__(512, (REF && M == 9) ? 2 : 3) k_mix_adi_wide(M
                 else rw[k] = ring[k];
             }
             const int y = r - M;
-            float2 o = f2(0, 0);
+            float2 o = f2(0, 0), o1 = f2(0, 0);
             if (y >= 0 && y < ng) {
                 if (y >= bi.klo && y <= bi.khi) {
                     o = fmul(bcr[0], rw[M]);
 #pragma unroll
-                    for (int k = 1; k <= M; k++) o = ffma(bcr[k], fadd(rw[M - k], rw[M + k]), o);
+                    for (int k = 1; k <= M; k++) {
+                        if (k & 1) o1 = ffma(bcr[k], fadd(rw[M - k], rw[M + k]), o1);
+                        else o = ffma(bcr[k], fadd(rw[M - k], rw[M + k]), o);
+                    }
                 } else {
                     const float2 *by = band + (long long)y * W;
 #pragma unroll
-                    for (int m = 0; m < W; m++) o = ffma(by[m], rw[m], o);
+                    for (int m = 0; m < W; m++) {
+                        if (m & 1) o1 = ffma(by[m], rw[m], o1);
+                        else o = ffma(by[m], rw[m], o);
+                    }
                 }
+                o = fadd(o, o1);
             }
             if constexpr (REF) {
                 const int yf = y - 1;

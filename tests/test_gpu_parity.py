@@ -198,14 +198,15 @@ def test_deposit_corner_cell_and_mixed_groups(st):
     ctx.close()
 
 
-@pytest.mark.parametrize("kernel", [0, 1, 2, 3])
+@pytest.mark.parametrize("kernel", [0, 1, 2, 3, 4])
 @pytest.mark.parametrize("delz", [0.015, 0.15, 5.0])
 def test_adi_kernels_and_regimes(delz, kernel):
     """rtmp = dz/(4 ks dx^2) = 3e-3, 3e-2 and 1 on the 151^2 grid: the fused
     recurrence kernels use an 8-cell window, a 16-cell window, or are not
-    eligible; the banded-inverse kernel a half width 7, 12, or is not
-    eligible.  Every eligible kernel agrees with the oracle's Thomas; an
-    ineligible request falls back to the autotuner's choice."""
+    eligible; the banded-inverse kernels a half width 7 (tiles and the
+    whole-width stream), 12 (tiles only), or are not eligible.  Every
+    eligible kernel agrees with the oracle's Thomas; an ineligible request
+    falls back to the autotuner's choice."""
     _log(f"ADI regime delz={delz} kernel={kernel}")
     sim = H.make_state([1024] * 4, seed_power=5e7)
     ctx = H.to_gpu(sim)
@@ -222,6 +223,43 @@ def test_adi_kernels_and_regimes(delz, kernel):
     assert err < 1e-13
     assert sch["adi_window"] == {0.015: 8, 0.15: 16, 5.0: 0}[delz]
     assert sch["adi_band_m"] == {0.015: 7, 0.15: 12, 5.0: 0}[delz]
+    if kernel == 4 and delz == 0.015:
+        assert sch["adi"] == "band-wide"
+
+
+@pytest.mark.parametrize("ngrid", [151, 255, 63])
+def test_adi_wide_stream_matches_band_tiles(ngrid):
+    """The whole-width banded ADI evaluates the products of the 32x32-tile
+    banded kernel; a column or row takes the symmetric form of the central
+    band where its own wavefront (stream) or its whole tile (tiles) lies in
+    [klo, khi], so the two differ by the rounding of that choice only: over
+    6 undulator steps with a source the fields agree to 1e-14 and both
+    match the oracle's Thomas."""
+    _log(f"banded ADI: whole-width stream vs tiles, {ngrid}^2, 6 slices, 6 steps")
+    st = H.steps()
+    out = []
+    for kernel in (3, 4):
+        sim = H.make_state([512, 256, 1024, 768, 512, 256], ngrid=ngrid, gridmax=0.3e-3 * ngrid / 151,
+                           seed_power=2e4)
+        ctx = H.to_gpu(sim)
+        ctx.set_schedule("adi_kernel", kernel)
+        r = ctx.run(H.gsteps(st[118:124]), cases.GAMMA0, record=("power",))
+        sch = ctx.schedule()
+        assert sch["adi"] == ("band" if kernel == 3 else "band-wide"), sch
+        f, first = ctx.download_field()
+        out.append((r, f, first))
+        ctx.close()
+    rs = sim.run(st[118:124], cases.GAMMA0, record=("power",))
+    (ra, fa, qa), (rb, fb, qb) = out
+    assert qa == qb == sim.s.first
+    ef = float(np.max(np.abs(fa - fb)) / np.max(np.abs(fa)))
+    eo = float(np.max(np.abs(fb - sim.field)) / np.max(np.abs(fa)))
+    _log(f"  field stream vs tiles {ef:.3e}, stream vs oracle {eo:.3e}")
+    assert ef < 1e-14 and eo < 1e-12
+    ep = H.rel_err(rb["power"], ra["power"])
+    eo = H.rel_err(rb["power"], rs["power"])
+    _log(f"  power stream vs tiles {ep:.3e}, stream vs oracle {eo:.3e}")
+    assert ep < 1e-13 and eo < 1e-10
 
 
 def test_adi_autotuner_picks_a_measured_variant():
@@ -232,7 +270,7 @@ def test_adi_autotuner_picks_a_measured_variant():
     sch = ctx.schedule()
     _log(f"  autotuned schedule {sch}")
     ms = sch["adi_tuned_ms"]
-    assert sch["adi_kernel"] in (0, 1, 2, 3) and ms[sch["adi_kernel"]] == min(m for m in ms if m > 0)
+    assert sch["adi_kernel"] in (0, 1, 2, 3, 4) and ms[sch["adi_kernel"]] == min(m for m in ms if m > 0)
     O.lib().orc_field_advance(C.byref(sim.s), C.byref(st[0]), st[0].dz)
     f, _ = ctx.download_field()
     assert np.max(np.abs(f - sim.field)) < 1e-13 * np.max(np.abs(sim.field))
