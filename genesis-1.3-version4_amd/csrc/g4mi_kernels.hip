@@ -789,9 +789,27 @@ __global__ void __launch_bounds__(kFrontNT) k_fp64_front(const long long *__rest
     double v[4] = {0, 0, 0, 0};
     int bx0 = 1 << 30, by0 = 1 << 30, bx1 = -1, by1 = -1;
     double pmax = 0;  // largest sqrt(faw2) / gamma of an in-grid particle (the deposit's scale)
+    // the next particle's coordinates are loaded before this one's stores,
+    // so two particles' loads are in flight per lane
+    double nx = 0, ny = 0, npx = 0, npy = 0, ng_ = 0;
+    if (a + t < b) {
+        nx = x[a + t];
+        ny = y[a + t];
+        npx = px[a + t];
+        npy = py[a + t];
+        ng_ = g[a + t];
+    }
     for (long long i = a + t; i < b; i += kFrontNT) {
-        double xi = x[i], yi = y[i], pxi = px[i], pyi = py[i];
-        transverse_one(t1, g[i], xi, yi, pxi, pyi);
+        double xi = nx, yi = ny, pxi = npx, pyi = npy;
+        const double gi = ng_;
+        if (i + kFrontNT < b) {
+            nx = x[i + kFrontNT];
+            ny = y[i + kFrontNT];
+            npx = px[i + kFrontNT];
+            npy = py[i + kFrontNT];
+            ng_ = g[i + kFrontNT];
+        }
+        transverse_one(t1, gi, xi, yi, pxi, pyi);
         x[i] = xi;
         y[i] = yi;
         px[i] = pxi;
@@ -809,7 +827,7 @@ __global__ void __launch_bounds__(kFrontNT) k_fp64_front(const long long *__rest
             by1 = max(by1, iy + 1);
             const double ddx = xi - d.ax, ddy = yi - d.ay;
             const double faw2 = (1 + d.kx * ddx * ddx + d.ky * ddy * ddy + 2 * (d.gradx * ddx + d.grady * ddy));
-            pmax = fmax(pmax, sqrt(faw2) / fabs(g[i]));
+            pmax = fmax(pmax, sqrt(faw2) / fabs(gi));
         }
     }
     red[0][t] = bx0;

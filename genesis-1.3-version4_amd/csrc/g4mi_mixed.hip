@@ -88,6 +88,18 @@ __device__ __forceinline__ void bsum(double (&v)[K], double *lds) {
 // within a few FP32 ulp; the map is applied as x + (a1 - 1) x + ... with
 // a1 - 1 = -2 sin^2(omg/2) (cosh: +2 sinh^2(omg/2)), which keeps the
 // focusing term's full relative precision.
+// sinh for the defocusing map: its Taylor series through x^9 below 0.5
+// (truncation below 1e-10 relative), exp above
+__device__ __forceinline__ float sinh32(float v) {
+    const float av = fabsf(v);
+    if (av < 0.5f) {
+        const float v2 = v * v;
+        return v * (1.f + v2 * (1.f / 6.f + v2 * (1.f / 120.f + v2 * (1.f / 5040.f + v2 * (1.f / 362880.f)))));
+    }
+    const float e = __expf(av), r = 0.5f * (e - frcp(e));
+    return v < 0.f ? -r : r;
+}
+
 __device__ __forceinline__ void line_map32(int kind, float delz, float q, float &x, float &px,
                                            float gz, float d) {
     // igz = 1 / gz
@@ -105,8 +117,8 @@ __device__ __forceinline__ void line_map32(int kind, float delz, float q, float 
         px = px + (a3 * xt * gz + a1m * px);
     } else {
         const float foc = fsqrt(-q * igz), omg = foc * delz;
-        const float sh = sinhf(0.5f * omg);
-        const float a1m = 2.f * sh * sh, a2 = sinhf(omg) * frcp(foc), a3 = a2 * foc * foc;
+        const float sh = sinh32(0.5f * omg);
+        const float a1m = 2.f * sh * sh, a2 = sinh32(omg) * frcp(foc), a3 = a2 * foc * foc;
         const float xt = x - d;
         x = xt + (a1m * xt + a2 * px * igz) + d;
         px = px + (a3 * xt * gz + a1m * px);
@@ -213,10 +225,28 @@ __device__ void front_body(const MixArrays &a, int is, int moments, int sc, doub
     const float gridmax = (float)S.p.gridmax, inv_dgrid = 1.f / (float)S.p.dgrid;
     const int ng = a.ng;
     int bx0 = 1 << 30, by0 = 1 << 30, bx1 = -1, by1 = -1;
+    const TransParams t1 = S.t1;  // in registers: the stores below may alias the step table
+    // the next particle's coordinates are loaded before this one's stores,
+    // so two particles' loads are in flight per lane
+    float nx = 0, ny = 0, npx = 0, npy = 0, ndg = 0;
+    if (b0 + t < b1) {
+        nx = a.x[b0 + t];
+        ny = a.y[b0 + t];
+        npx = a.px[b0 + t];
+        npy = a.py[b0 + t];
+        ndg = a.dg[b0 + t];
+    }
     for (long long i = b0 + t; i < b1; i += NT) {
-        float x = a.x[i], y = a.y[i], px = a.px[i], py = a.py[i];
-        const float gam = gref + a.dg[i];
-        transverse32(S.t1, gam, x, y, px, py);
+        float x = nx, y = ny, px = npx, py = npy;
+        const float gam = gref + ndg;
+        if (i + NT < b1) {
+            nx = a.x[i + NT];
+            ny = a.y[i + NT];
+            npx = a.px[i + NT];
+            npy = a.py[i + NT];
+            ndg = a.dg[i + NT];
+        }
+        transverse32(t1, gam, x, y, px, py);
         a.x[i] = x;
         a.y[i] = y;
         a.px[i] = px;
