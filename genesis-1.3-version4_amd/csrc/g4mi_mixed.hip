@@ -1467,7 +1467,7 @@ __global__ void __launch_bounds__(64) k_mix_adi_stream(MixArrays a) {
 // the u1 reads separates every write of a row from the reads of the row
 // before, so u1 needs one buffer.
 //
-// Dynamic LDS: E rows [2][NC + 2M] float2, u1 row [NC + 2] double2 (with the
+// Dynamic LDS: E rows [2][NC + 2M] float2, u1 row [NC + 2] float2 (with the
 // correction), per-column band rows [W][NC] float2, wavefront sums [NW][8].
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ void lds_barrier() {
@@ -1479,7 +1479,7 @@ __device__ __forceinline__ void lds_barrier() {
 template <int M, bool REF>
 size_t wide_lds_bytes(int nw) {
     const int nc = 64 * nw;
-    return (size_t)2 * (nc + 2 * M) * sizeof(float2) + (REF ? (size_t)(nc + 2) * sizeof(double2) : 0) +
+    return (size_t)2 * (nc + 2 * M) * sizeof(float2) + (REF ? (size_t)(nc + 2) * sizeof(float2) : 0) +
            (size_t)(2 * M + 1) * nc * sizeof(float2) + (size_t)nw * 8 * sizeof(double);
 }
 
@@ -1494,8 +1494,8 @@ __global__ void __launch_bounds__(512, (REF && M == 9) ? 2 : 3) k_mix_adi_wide(M
     const int t = threadIdx.x, lane = t & 63, w = t >> 6;
     float2 *lrow0 = (float2 *)wide_lds;                    // [2][nc + 2M]
     const int lp = nc + 2 * M;
-    double2 *lu = (double2 *)(lrow0 + 2 * lp);             // [nc + 2]
-    float2 *lband = (float2 *)(lu + (REF ? nc + 2 : 0));    // [W][nc]
+    float2 *lu = lrow0 + 2 * lp;                            // [nc + 2] u1 row (FP32 values)
+    float2 *lband = lu + (REF ? nc + 2 : 0);                // [W][nc]
     double *wsum = (double *)(lband + W * nc);              // [nw][8]
     const int ii = blockIdx.x;
     if (ii >= a.ns) return;
@@ -1525,7 +1525,7 @@ __global__ void __launch_bounds__(512, (REF && M == 9) ? 2 : 3) k_mix_adi_wide(M
     // zero rows (the pads stay zero: only lanes on the grid write non-zero values)
     for (int k = t; k < 2 * lp; k += nt) lrow0[k] = f2(0, 0);
     if constexpr (REF)
-        for (int k = t; k < nc + 2; k += nt) lu[k] = d2(0, 0);
+        for (int k = t; k < nc + 2; k += nt) lu[k] = f2(0, 0);
     if (!wsym)
         for (int m = 0; m < W; m++) lband[m * nc + t] = xin ? band[(long long)x * W + m] : f2(0, 0);
     const float2 *bc = band + (long long)((bi.klo + bi.khi) / 2) * W;
@@ -1646,11 +1646,12 @@ __global__ void __launch_bounds__(512, (REF && M == 9) ? 2 : 3) k_mix_adi_wide(M
                     up = fadd(up, up1);
                 }
                 if constexpr (REF) {
-                    const double2 u1 = tod(up);
-                    lu[x + 1] = u1;  // zero off the grid
+                    lu[x + 1] = up;  // zero off the grid
                     lds_barrier();
-                    const double2 nl = lu[x], nr = lu[x + 2];
-                    const double2 au = adi_apply(u1, d2(nl.x + nr.x, nl.y + nr.y), ad);
+                    const double2 u1 = tod(up);
+                    const float2 nl = lu[x], nr = lu[x + 2];
+                    const double2 au =
+                        adi_apply(u1, d2((double)nl.x + (double)nr.x, (double)nl.y + (double)nr.y), ad);
                     const double2 upd = jacobi_update(u1, d2(E64.x - au.x, E64.y - au.y), ad, dn);
                     if (xin) {
                         const double2 F64 = d2(fma(2.0, upd.x, sv64.x) - E64.x, fma(2.0, upd.y, sv64.y) - E64.y);
@@ -2014,10 +2015,22 @@ void launch_main(hipStream_t s, const MixArrays &a, int sc, int nt, int lds) {
     }
 }
 
+constexpr size_t kLdsMax = 160 * 1024;
+
+// the whole-width kernel's LDS limit: what eight wavefronts need, at most
+// the CU's 160 KiB (wider grids with wide bands then run the strips)
 template <int M, bool REF>
 int set_wide_attr() {
     return hipFuncSetAttribute((const void *)k_mix_adi_wide<M, REF>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                               (int)wide_lds_bytes<M, REF>(8)) == hipSuccess;
+                               (int)std::min(wide_lds_bytes<M, REF>(8), kLdsMax)) == hipSuccess;
+}
+
+size_t wide_lds_need(int tmpl_m, bool refine, int nw) {
+#define G4_L(MM) \
+    if (tmpl_m == MM) return refine ? wide_lds_bytes<MM, true>(nw) : wide_lds_bytes<MM, false>(nw);
+    G4_L(3) G4_L(4) G4_L(6) G4_L(9) G4_L(12)
+#undef G4_L
+    return ~(size_t)0;
 }
 
 template <int M, bool REF>
@@ -2093,7 +2106,10 @@ void launch_mix_beam(hipStream_t s, const MixArrays &a, bool sc, bool moments, h
 }
 
 int launch_mix_adi(hipStream_t s, const MixArrays &a, int tmpl_m, bool refine, int mode, int *ntiles) {
-    if (mode == kAdiWide && a.ng > 512) mode = kAdiStrips;  // eight wavefronts cover 512 columns
+    // eight wavefronts cover 512 columns; the LDS of one workgroup at most 160 KiB
+    if (mode == kAdiWide &&
+        (a.ng > 512 || wide_lds_need(tmpl_m, refine, std::max(1, (a.ng + 63) / 64)) > kLdsMax))
+        mode = kAdiStrips;
     const bool stream = mode != kAdiTile;
     if (refine && !stream && mix_band_lds(tmpl_m, true) > 160 * 1024) return -1;
     if (mode == kAdiWide) {
