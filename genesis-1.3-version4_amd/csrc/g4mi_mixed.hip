@@ -447,7 +447,6 @@ __device__ void main_body(const MixArrays &a, const int is, int sc_on, float2 *d
     const float2 *field = a.field[a.st->cur] + (long long)((is + first) % ns) * ng * ng;
     float2 *celm = dyn;                                   // [nmodes][ngr]
     double2 *work = (double2 *)(dyn + kCelmCells);        // histograms / deposit tile
-    double2(*hist)[kScMax] = (double2(*)[kScMax])work;    // [NT/64][kScMax]
 
     // ---- long-range space charge of this slice
     double esc = 0;
@@ -483,12 +482,17 @@ __device__ void main_body(const MixArrays &a, const int is, int sc_on, float2 *d
         const double econst = kVacimp / kEev * a.cur[is] / S.sc_ks / (double)npart;
         const double coef = -S.sc_gz2 / S.sc_ks / S.sc_ks;
         const double drd = a.dr[is];
+        // sub-histograms per lane group (lane mod nsub) within each wavefront:
+        // a beam's particles crowd into the inner radial cells, and lanes
+        // adding into one cell serialise in the LDS
+        const int nsub = (NT / 64) * 4 * ngr <= kTileCells ? 4 : 1;
+        double2 *hs = work;  // [NT/64][nsub][ngr]
+        double2 *myh = hs + (w * nsub + (t & (nsub - 1))) * ngr;
         int mode = 0;
         for (int m = -a.sc_nphi; m <= a.sc_nphi; m++) {
             for (int l = 1; l <= a.sc_nz; l++, mode++) {
                 __syncthreads();
-                for (int k = t; k < (NT / 64) * kScMax; k += NT)
-                    hist[k / kScMax][k % kScMax] = make_double2(0, 0);
+                for (int k = t; k < (NT / 64) * nsub * ngr; k += NT) hs[k] = make_double2(0, 0);
                 __syncthreads();
                 float nx = 0, ny = 0, nt = 0;
                 if (b0 + t < b1) {
@@ -508,8 +512,8 @@ __device__ void main_body(const MixArrays &a, const int is, int sc_on, float2 *d
                     float s, c;
                     __sincosf((float)l * (ph0 + th), &s, &c);
                     const float2 v = fmul(cw_pow(f2(tx * ir, ty * ir), -m), f2(c, -s));
-                    unsafeAtomicAdd(&hist[w][bin].x, (double)v.x);
-                    unsafeAtomicAdd(&hist[w][bin].y, (double)v.y);
+                    unsafeAtomicAdd(&myh[bin].x, (double)v.x);
+                    unsafeAtomicAdd(&myh[bin].y, (double)v.y);
                 }
                 __syncthreads();
                 // constructLaplaceOperator (:176-190): coefficients and source of
@@ -529,9 +533,9 @@ __device__ void main_body(const MixArrays &a, const int is, int sc_on, float2 *d
                         sclow[i] = (float)(coef * ldi / l / l / voli);
                     }
                     double csx = 0, csy = 0;
-                    for (int q = 0; q < NT / 64; q++) {
-                        csx += hist[q][i].x;
-                        csy += hist[q][i].y;
+                    for (int q = 0; q < (NT / 64) * nsub; q++) {
+                        csx += hs[q * ngr + i].x;
+                        csy += hs[q * ngr + i].y;
                     }
                     const double e = econst / l / voli;
                     scrhs[i] = f2((float)(-csy * e), (float)(csx * e));
@@ -578,6 +582,10 @@ __device__ void main_body(const MixArrays &a, const int is, int sc_on, float2 *d
     __syncthreads();  // histograms dead, tile takes their place
     if (fused)
         for (int k = t; k < tw * th; k += NT) work[k] = make_double2(0, 0);
+    // the tile as two planes (real parts, then imaginary parts): a lane's
+    // 8-byte atomic then spans two LDS banks, not four, and the random cells
+    // of a wavefront's particles collide in fewer banks
+    double *tre = (double *)work, *tim = tre + tw * th;
     __syncthreads();
 
     // ---- particle pass
@@ -750,14 +758,14 @@ __device__ void main_body(const MixArrays &a, const int is, int sc_on, float2 *d
                     const float part = fsqrt(faw2) * scl * frcp(gref + dg);
                     const float2 cp = f2(s * part, c * part);
                     const int lk = (iy - ty0) * tw + (ix - tx0);
-                    unsafeAtomicAdd(&work[lk].x, (double)(wx * wy * cp.x));
-                    unsafeAtomicAdd(&work[lk].y, (double)(wx * wy * cp.y));
-                    unsafeAtomicAdd(&work[lk + 1].x, (double)((1 - wx) * wy * cp.x));
-                    unsafeAtomicAdd(&work[lk + 1].y, (double)((1 - wx) * wy * cp.y));
-                    unsafeAtomicAdd(&work[lk + tw].x, (double)(wx * (1 - wy) * cp.x));
-                    unsafeAtomicAdd(&work[lk + tw].y, (double)(wx * (1 - wy) * cp.y));
-                    unsafeAtomicAdd(&work[lk + tw + 1].x, (double)((1 - wx) * (1 - wy) * cp.x));
-                    unsafeAtomicAdd(&work[lk + tw + 1].y, (double)((1 - wx) * (1 - wy) * cp.y));
+                    unsafeAtomicAdd(&tre[lk], (double)(wx * wy * cp.x));
+                    unsafeAtomicAdd(&tim[lk], (double)(wx * wy * cp.y));
+                    unsafeAtomicAdd(&tre[lk + 1], (double)((1 - wx) * wy * cp.x));
+                    unsafeAtomicAdd(&tim[lk + 1], (double)((1 - wx) * wy * cp.y));
+                    unsafeAtomicAdd(&tre[lk + tw], (double)(wx * (1 - wy) * cp.x));
+                    unsafeAtomicAdd(&tim[lk + tw], (double)(wx * (1 - wy) * cp.y));
+                    unsafeAtomicAdd(&tre[lk + tw + 1], (double)((1 - wx) * (1 - wy) * cp.x));
+                    unsafeAtomicAdd(&tim[lk + tw + 1], (double)((1 - wx) * (1 - wy) * cp.y));
                 } else {
                     ovf = 1;
                 }
@@ -792,7 +800,7 @@ __device__ void main_body(const MixArrays &a, const int is, int sc_on, float2 *d
     float2 *plane = a.src + (long long)is * ng * ng;
     if (fused && !ovf) {
         for (int k = t; k < tw * th; k += NT)
-            plane[(ty0 + k / tw) * ng + tx0 + k % tw] = tof(work[k]);
+            plane[(ty0 + k / tw) * ng + tx0 + k % tw] = f2((float)tre[k], (float)tim[k]);
         if (t == 0) a.box[is] = make_int4(tx0, ty0, tx0 + tw - 1, ty0 + th - 1);
         return;
     }
@@ -807,6 +815,7 @@ __device__ void main_body(const MixArrays &a, const int is, int sc_on, float2 *d
         const int r1 = min(by1, r0 + rows_per_band - 1);
         const int ncell = (r1 - r0 + 1) * ww;
         for (int k = t; k < ncell; k += NT) tile[k] = make_double2(0, 0);
+        double *pre = (double *)tile, *pim = pre + ncell;  // two planes, as the fused tile
         __syncthreads();
         for (long long i = b0 + t; i < b1; i += NT) {
             const float x = a.x[i], y = a.y[i];
@@ -829,14 +838,14 @@ __device__ void main_body(const MixArrays &a, const int is, int sc_on, float2 *d
             for (int k = 0; k < 4; k++) {
                 if (cy[k] < r0 || cy[k] > r1 || cx[k] < bx0 || cx[k] > bx1) continue;
                 const int lk = (cy[k] - r0) * ww + (cx[k] - bx0);
-                unsafeAtomicAdd(&tile[lk].x, (double)(wgt[k] * cp.x));
-                unsafeAtomicAdd(&tile[lk].y, (double)(wgt[k] * cp.y));
+                unsafeAtomicAdd(&pre[lk], (double)(wgt[k] * cp.x));
+                unsafeAtomicAdd(&pim[lk], (double)(wgt[k] * cp.y));
             }
         }
         __syncthreads();
         for (int k = t; k < ncell; k += NT) {
             const int ry = r0 + k / ww, rx = bx0 + k % ww;
-            plane[ry * ng + rx] = tof(tile[k]);
+            plane[ry * ng + rx] = f2((float)pre[k], (float)pim[k]);
         }
         __syncthreads();
     }
