@@ -1014,10 +1014,13 @@ __global__ void __launch_bounds__(kFuseNT) k_fp64_main(const long long *__restri
     // (v[8]) stays within that bound, else it deposits again in two passes
     const double bound1 = 2.0 * (double)npart * pmaxf[is] * fabs(scl);
     const double fs1 = fx_scale(bound1);
-    long long *itile = (long long *)work;  // [cells][2]
+    // the tile as two planes (real parts, then imaginary parts): a lane's
+    // 8-byte atomic then spans two LDS banks, not four
+    long long *itile = (long long *)work;  // [2][cells]
     __syncthreads();  // solver scratch dead: the tile takes its place
     if (fused)
         for (int k = t; k < 2 * tw * tht; k += kFuseNT) itile[k] = 0;
+    long long *tre = itile, *tim = itile + tw * tht;
     __syncthreads();
     // ---- one pass per particle
     const double eloss = -esc[is] / kEev;
@@ -1146,14 +1149,14 @@ __global__ void __launch_bounds__(kFuseNT) k_fp64_main(const long long *__restri
                     const double2 cp = make_double2(s * part, c * part);
                     const int lk = (iy - ty0) * tw + (ix - tx0);
                     const double w00 = wx * wy, w10 = (1 - wx) * wy, w01 = wx * (1 - wy), w11 = (1 - wx) * (1 - wy);
-                    fx_add(&itile[2 * lk], fx(w00 * cp.x, fs1));
-                    fx_add(&itile[2 * lk + 1], fx(w00 * cp.y, fs1));
-                    fx_add(&itile[2 * (lk + 1)], fx(w10 * cp.x, fs1));
-                    fx_add(&itile[2 * (lk + 1) + 1], fx(w10 * cp.y, fs1));
-                    fx_add(&itile[2 * (lk + tw)], fx(w01 * cp.x, fs1));
-                    fx_add(&itile[2 * (lk + tw) + 1], fx(w01 * cp.y, fs1));
-                    fx_add(&itile[2 * (lk + tw + 1)], fx(w11 * cp.x, fs1));
-                    fx_add(&itile[2 * (lk + tw + 1) + 1], fx(w11 * cp.y, fs1));
+                    fx_add(&tre[lk], fx(w00 * cp.x, fs1));
+                    fx_add(&tim[lk], fx(w00 * cp.y, fs1));
+                    fx_add(&tre[lk + 1], fx(w10 * cp.x, fs1));
+                    fx_add(&tim[lk + 1], fx(w10 * cp.y, fs1));
+                    fx_add(&tre[lk + tw], fx(w01 * cp.x, fs1));
+                    fx_add(&tim[lk + tw], fx(w01 * cp.y, fs1));
+                    fx_add(&tre[lk + tw + 1], fx(w11 * cp.x, fs1));
+                    fx_add(&tim[lk + tw + 1], fx(w11 * cp.y, fs1));
                 } else {
                     ovf = 1;
                 }
@@ -1180,7 +1183,7 @@ __global__ void __launch_bounds__(kFuseNT) k_fp64_main(const long long *__restri
     if (fused && !ovf) {
         for (int k = t; k < tw * tht; k += kFuseNT)
             plane[(ty0 + k / tw) * ng + tx0 + k % tw] =
-                make_double2((double)itile[2 * k] / fs1, (double)itile[2 * k + 1] / fs1);
+                make_double2((double)tre[k] / fs1, (double)tim[k] / fs1);
         if (t == 0) box[is] = make_int4(tx0, ty0, tx0 + tw - 1, ty0 + tht - 1);
         return;
     }
@@ -1195,6 +1198,7 @@ __global__ void __launch_bounds__(kFuseNT) k_fp64_main(const long long *__restri
         const int r1 = min(by1, r0 + rows_per_band - 1);
         const int ncell = (r1 - r0 + 1) * ww;
         for (int k = t; k < 2 * ncell; k += kFuseNT) itile[k] = 0;
+        long long *pre = itile, *pim = itile + ncell;  // two planes, as the fused tile
         __syncthreads();
         for (long long i = a + t; i < b; i += kFuseNT) {
             const double xi = x[i], yi = y[i];
@@ -1218,14 +1222,14 @@ __global__ void __launch_bounds__(kFuseNT) k_fp64_main(const long long *__restri
             for (int k = 0; k < 4; k++) {
                 if (cy[k] < r0 || cy[k] > r1 || cx[k] < bx0 || cx[k] > bx1) continue;
                 const int lk = (cy[k] - r0) * ww + (cx[k] - bx0);
-                fx_add(&itile[2 * lk], fx(wgt[k] * cp.x, fs2));
-                fx_add(&itile[2 * lk + 1], fx(wgt[k] * cp.y, fs2));
+                fx_add(&pre[lk], fx(wgt[k] * cp.x, fs2));
+                fx_add(&pim[lk], fx(wgt[k] * cp.y, fs2));
             }
         }
         __syncthreads();
         for (int k = t; k < ncell; k += kFuseNT) {
             const int ry = r0 + k / ww, rx = bx0 + k % ww;
-            plane[ry * ng + rx] = make_double2((double)itile[2 * k] / fs2, (double)itile[2 * k + 1] / fs2);
+            plane[ry * ng + rx] = make_double2((double)pre[k] / fs2, (double)pim[k] / fs2);
         }
         __syncthreads();
     }
