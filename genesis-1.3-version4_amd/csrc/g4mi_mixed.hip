@@ -423,6 +423,9 @@ __device__ __forceinline__ float2 cw_pow(float2 cw, int m) {
 // LDS accumulation is FP64 (ds_add_f64): on gfx950 FP32 LDS atomics
 // (ds_add_f32) issue far slower than FP64 ones, and the sums gain precision.
 constexpr int kCelmCells = 1024;                          // space-charge mode solutions (float2)
+// accumulation type of the main kernel's deposit tile (FP64: an FP32 tile
+// holds twice the footprint but its atomics made the kernel 2.1x slower)
+using TileT = double;
 
 // EG: also sum the push's energy increments per slice (egsum, the energy
 // balance of the learned field step).  A variant of its own because the
@@ -433,6 +436,8 @@ constexpr int kCelmCells = 1024;                          // space-charge mode s
 template <bool EG, int NT, int DYN>
 __device__ void main_body(const MixArrays &a, const int is, int sc_on, float2 *dyn) {
     constexpr int kTileCells = (DYN - kCelmCells) / 2;   // double2 cells: histograms, then the deposit tile
+    // deposit tile cells: two planes of TileT
+    constexpr int kDepCells = (DYN - kCelmCells) * (int)sizeof(float2) / (2 * (int)sizeof(TileT));
     static_assert((NT / 64) * kScMax <= kTileCells, "the histograms must fit in the tile region");
     __shared__ int bred[5 * (NT / 64)];
     __shared__ double wred[9 * (NT / 64)];
@@ -572,7 +577,7 @@ __device__ void main_body(const MixArrays &a, const int is, int sc_on, float2 *d
         ty0 = max(0, pb.y - 2);
         tw = min(ng - 1, pb.z + 2) - tx0 + 1;
         th = min(ng - 1, pb.w + 2) - ty0 + 1;
-        fused = tw * th <= kTileCells;
+        fused = tw * th <= kDepCells;
     }
     float scl = 0;
     if (dep) {
@@ -580,12 +585,12 @@ __device__ void main_body(const MixArrays &a, const int is, int sc_on, float2 *d
         scl = npart > 0 ? (float)(scl0 / (4 * kEev * (double)npart * D.dgrid * D.dgrid)) : 0.f;
     }
     __syncthreads();  // histograms dead, tile takes their place
-    if (fused)
-        for (int k = t; k < tw * th; k += NT) work[k] = make_double2(0, 0);
     // the tile as two planes (real parts, then imaginary parts): a lane's
-    // 8-byte atomic then spans two LDS banks, not four, and the random cells
-    // of a wavefront's particles collide in fewer banks
-    double *tre = (double *)work, *tim = tre + tw * th;
+    // atomic then spans fewer LDS banks, and the random cells of a
+    // wavefront's particles collide in fewer banks
+    TileT *tre = (TileT *)work, *tim = tre + tw * th;
+    if (fused)
+        for (int k = t; k < 2 * tw * th; k += NT) tre[k] = (TileT)0;
     __syncthreads();
 
     // ---- particle pass
@@ -758,14 +763,14 @@ __device__ void main_body(const MixArrays &a, const int is, int sc_on, float2 *d
                     const float part = fsqrt(faw2) * scl * frcp(gref + dg);
                     const float2 cp = f2(s * part, c * part);
                     const int lk = (iy - ty0) * tw + (ix - tx0);
-                    unsafeAtomicAdd(&tre[lk], (double)(wx * wy * cp.x));
-                    unsafeAtomicAdd(&tim[lk], (double)(wx * wy * cp.y));
-                    unsafeAtomicAdd(&tre[lk + 1], (double)((1 - wx) * wy * cp.x));
-                    unsafeAtomicAdd(&tim[lk + 1], (double)((1 - wx) * wy * cp.y));
-                    unsafeAtomicAdd(&tre[lk + tw], (double)(wx * (1 - wy) * cp.x));
-                    unsafeAtomicAdd(&tim[lk + tw], (double)(wx * (1 - wy) * cp.y));
-                    unsafeAtomicAdd(&tre[lk + tw + 1], (double)((1 - wx) * (1 - wy) * cp.x));
-                    unsafeAtomicAdd(&tim[lk + tw + 1], (double)((1 - wx) * (1 - wy) * cp.y));
+                    unsafeAtomicAdd(&tre[lk], (TileT)(wx * wy * cp.x));
+                    unsafeAtomicAdd(&tim[lk], (TileT)(wx * wy * cp.y));
+                    unsafeAtomicAdd(&tre[lk + 1], (TileT)((1 - wx) * wy * cp.x));
+                    unsafeAtomicAdd(&tim[lk + 1], (TileT)((1 - wx) * wy * cp.y));
+                    unsafeAtomicAdd(&tre[lk + tw], (TileT)(wx * (1 - wy) * cp.x));
+                    unsafeAtomicAdd(&tim[lk + tw], (TileT)(wx * (1 - wy) * cp.y));
+                    unsafeAtomicAdd(&tre[lk + tw + 1], (TileT)((1 - wx) * (1 - wy) * cp.x));
+                    unsafeAtomicAdd(&tim[lk + tw + 1], (TileT)((1 - wx) * (1 - wy) * cp.y));
                 } else {
                     ovf = 1;
                 }
@@ -809,13 +814,12 @@ __device__ void main_body(const MixArrays &a, const int is, int sc_on, float2 *d
     by1 = min(by1, ng - 1);
     if (t == 0) a.box[is] = make_int4(bx0, by0, bx1, by1);
     const int ww = bx1 - bx0 + 1;
-    const int rows_per_band = max(1, kTileCells / ww);
-    double2 *tile = work;
+    const int rows_per_band = max(1, kDepCells / ww);
     for (int r0 = by0; r0 <= by1; r0 += rows_per_band) {
         const int r1 = min(by1, r0 + rows_per_band - 1);
         const int ncell = (r1 - r0 + 1) * ww;
-        for (int k = t; k < ncell; k += NT) tile[k] = make_double2(0, 0);
-        double *pre = (double *)tile, *pim = pre + ncell;  // two planes, as the fused tile
+        TileT *pre = (TileT *)work, *pim = pre + ncell;  // two planes, as the fused tile
+        for (int k = t; k < 2 * ncell; k += NT) pre[k] = (TileT)0;
         __syncthreads();
         for (long long i = b0 + t; i < b1; i += NT) {
             const float x = a.x[i], y = a.y[i];
@@ -838,8 +842,8 @@ __device__ void main_body(const MixArrays &a, const int is, int sc_on, float2 *d
             for (int k = 0; k < 4; k++) {
                 if (cy[k] < r0 || cy[k] > r1 || cx[k] < bx0 || cx[k] > bx1) continue;
                 const int lk = (cy[k] - r0) * ww + (cx[k] - bx0);
-                unsafeAtomicAdd(&pre[lk], (double)(wgt[k] * cp.x));
-                unsafeAtomicAdd(&pim[lk], (double)(wgt[k] * cp.y));
+                unsafeAtomicAdd(&pre[lk], (TileT)(wgt[k] * cp.x));
+                unsafeAtomicAdd(&pim[lk], (TileT)(wgt[k] * cp.y));
             }
         }
         __syncthreads();

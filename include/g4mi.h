@@ -239,10 +239,12 @@ int g4mi_sort_index_noalloc(g4mi_ctx *ctx, double slen, int *atar, long long *mo
 
 /* Kernel schedule.  The first field step of each step size times every
  * ADI kernel eligible for its grid and step (two-kernel Thomas, fused
- * recurrence with and without prefetch, banded inverse) on a scratch copy
- * of its field and keeps the fastest for that step size (the library's
- * autotuner).  g4mi_set_schedule fixes an entry instead ("adi_kernel":
- * 0..3, -1 re-enables tuning); g4mi_get_schedule writes the current choice
+ * recurrence with and without prefetch, banded inverse in 32x32 tiles and
+ * as a whole-width stream) on a scratch copy of its field and keeps the
+ * fastest for that step size (the library's autotuner).  g4mi_set_schedule
+ * fixes an entry instead ("adi_kernel": 0..4, -1 re-enables tuning; a
+ * kernel not eligible for a step size falls back to the tuner's choice);
+ * g4mi_get_schedule writes the current choice
  * and the measured timings as a JSON object into buf.  "fp64_fused" (0/1,
  * default 1): the step's particle passes as two fused kernels where the
  * step allows it.  The spectral solver's first step likewise times its
@@ -250,10 +252,11 @@ int g4mi_sort_index_noalloc(g4mi_ctx *ctx, double slen, int *atar, long long *mo
  * one and k chunks of 16, 1 three products, 2 four products with chunks of
  * 32, 3 three with 32; -1 tunes again).
  * Mixed form: its first 14 steps time the main kernel's shapes (4, 6 or 8
- * wavefronts per slice x 68 or 44 KiB of LDS), the streaming and the tile
- * ADI, and the step as a graph of kernels against one persistent
+ * wavefronts per slice x 68 or 44 KiB of LDS), the three ADI kernels,
+ * and the step as a graph of kernels against one persistent
  * cooperative launch, and keep the fastest.  Entries: "adi_refine" (FP64
- * residual correction, default 1), "mixed_adi" (1 streaming, 0 tile),
+ * residual correction, default 1), "mixed_adi" (2 whole-width streaming,
+ * 1 streaming strips, 0 tiles),
  * "main_nt" (256, 384, 512), "main_lds" (0: 68 KiB, 1: 44 KiB),
  * "persistent", "graphs" (0/1); setting any of the kernel entries turns the
  * tuner off, "autotune" 1 turns it on again. */
