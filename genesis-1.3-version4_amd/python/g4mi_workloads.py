@@ -155,8 +155,9 @@ def run_lattice(ctx, steps, warmup=0, timed=0, breakdown=0, keys=("power", "bunc
     then `timed` steps bracketed by stream synchronisation, then the rest.
     Per-kernel-group times come from HIP events: over the timed steps in the
     FP64 form; in the mixed form (whose timed steps replay captured graphs)
-    over `breakdown` further steps launched directly.  Returns the seconds of
-    the timed steps, the kernel times and the records [len(steps)][ns]."""
+    over `breakdown` steps launched directly between the warm-up and the
+    timed steps.  Returns the seconds of the timed steps, the kernel times
+    and the records [len(steps)][ns]."""
     import time
     import numpy as np
     import g4mi
@@ -178,22 +179,25 @@ def run_lattice(ctx, steps, warmup=0, timed=0, breakdown=0, keys=("power", "bunc
     fp64 = ctx.precision == g4mi.FP64
     seg(0, warmup)
     ctx.sync()
-    if fp64:
-        ctx.set_timing(True)
-    ctx.sync()
-    t0 = time.perf_counter()
-    seg(warmup, warmup + timed)
-    ctx.sync()
-    t1 = time.perf_counter()
     nb = 0
     if not fp64 and breakdown:
         nb = min(breakdown, nst - warmup - timed)
         ctx.set_timing(True)
-        seg(warmup + timed, warmup + timed + nb)
+        seg(warmup, warmup + nb)
         ctx.sync()
-    kt = ctx.kernel_times()
-    ctx.set_timing(False)
-    seg(warmup + timed + nb, nst)
+        kt = ctx.kernel_times()
+        ctx.set_timing(False)
+    elif fp64:
+        ctx.set_timing(True)
+    ctx.sync()
+    t0 = time.perf_counter()
+    seg(warmup + nb, warmup + nb + timed)
+    ctx.sync()
+    t1 = time.perf_counter()
+    if fp64 or not nb:
+        kt = ctx.kernel_times()
+        ctx.set_timing(False)
+    seg(warmup + nb + timed, nst)
     ctx.sync()
     return dict(seconds=t1 - t0, kt=kt, recs=recs)
 
