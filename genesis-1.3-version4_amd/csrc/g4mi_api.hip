@@ -23,6 +23,7 @@
 #include "g4mi_sort.h"
 #include "g4mi_fno_api.h"
 #include "g4mi_mixed.h"
+#include "g4mi_load.h"
 
 namespace g4mi {
 
@@ -2438,6 +2439,112 @@ int g4mi_beam_upload(g4mi_ctx *h, const double *g, const double *th, const doubl
     G4_HIP(hipMemcpyAsync(c->d_cur, cur, c->ns * sizeof(double), hipMemcpyHostToDevice, c->stream));
     G4_HIP(hipStreamSynchronize(c->stream));
     c->h_cur.assign(cur, cur + c->ns);
+    c->has_beam = true;
+    return G4MI_OK;
+}
+
+int g4mi_load_beam_device(g4mi_ctx *h, const g4mi_slice_beam *slices, int nbins, int shotnoise, int seed,
+                          int islice0, double lambda0, double sample, int timerun) {
+    if (!h || !slices || nbins < 1) {
+        set_error("g4mi_load_beam_device: invalid argument");
+        return G4MI_E_ARG;
+    }
+    Ctx *c = &h->c;
+    const int ns = c->ns;
+    const long long n = c->ntot;
+    int maxpart = 0;
+    for (int j = 0; j < ns; j++) {
+        const long long m = c->off[j + 1] - c->off[j];
+        if (m % nbins != 0) {
+            set_error("g4mi_load_beam_device: NPART is not a multiple of NBINS");
+            return G4MI_E_ARG;
+        }
+        maxpart = std::max(maxpart, (int)m);
+    }
+    // host: the per-slice sequence starts and the sequential shot-noise
+    // stream (LoadBeam.cpp:127-212), in the order g4mi_load_beam draws them
+    std::vector<unsigned int> starts(ns);
+    for (int j = 0; j < ns; j++) starts[j] = quiet_start(j + islice0);
+    const int nh = (nbins - 1) / 2;
+    std::vector<double> draws;
+    std::vector<long long> dbase(ns, -1);
+    if (shotnoise && timerun && nh > 0) {
+        draws.reserve(2 * (size_t)nh * (size_t)(n / nbins));
+        Ran2 stream(shot_noise_seed(seed, 0));
+        for (int j = 0; j < ns; j++) {
+            const long long before = (long long)draws.size() / 2;
+            const double ne = slices[j].current * lambda0 * sample / kCeH();
+            if (shot_noise_draws(stream, (int)(c->off[j + 1] - c->off[j]), nbins, ne, draws) > 0)
+                dbase[j] = before;
+        }
+    }
+    // one temporary block: starts, slices, dbase, draws, and in the mixed
+    // form the FP64 coordinates and the energy partial sums
+    const bool mixed = c->precision == G4MI_FP32;
+    constexpr int kSumBlocks = 1024;
+    auto up = [](size_t b) { return (b + 255) & ~(size_t)255; };
+    const size_t o_slices = up(ns * sizeof(unsigned int));
+    const size_t o_dbase = o_slices + up(ns * sizeof(g4mi_slice_beam));
+    const size_t o_draws = o_dbase + up(ns * sizeof(long long));
+    const size_t o_coord = o_draws + up(std::max<size_t>(draws.size(), 1) * sizeof(double));
+    const size_t o_sums = o_coord + (mixed ? up(6 * (size_t)n * sizeof(double)) : 0);
+    const size_t total = o_sums + (mixed ? up(kSumBlocks * sizeof(double)) : 0);
+    char *tmp = nullptr;
+    G4_HIP(hipMalloc(&tmp, total));
+    struct Free {
+        char *p;
+        ~Free() { (void)hipFree(p); }
+    } free_tmp{tmp};
+    unsigned int *d_starts = (unsigned int *)tmp;
+    g4mi_slice_beam *d_slices = (g4mi_slice_beam *)(tmp + o_slices);
+    long long *d_dbase = (long long *)(tmp + o_dbase);
+    double *d_draws = (double *)(tmp + o_draws);
+    G4_HIP(hipMemcpyAsync(d_starts, starts.data(), ns * sizeof(unsigned int), hipMemcpyHostToDevice, c->stream));
+    G4_HIP(hipMemcpyAsync(d_slices, slices, ns * sizeof(g4mi_slice_beam), hipMemcpyHostToDevice, c->stream));
+    G4_HIP(hipMemcpyAsync(d_dbase, dbase.data(), ns * sizeof(long long), hipMemcpyHostToDevice, c->stream));
+    if (!draws.empty())
+        G4_HIP(hipMemcpyAsync(d_draws, draws.data(), draws.size() * sizeof(double), hipMemcpyHostToDevice,
+                              c->stream));
+    double *g = c->d_g, *th = c->d_th, *x = c->d_x, *y = c->d_y, *px = c->d_px, *py = c->d_py;
+    if (mixed) {
+        g = (double *)(tmp + o_coord);
+        th = g + n;
+        x = th + n;
+        y = x + n;
+        px = y + n;
+        py = px + n;
+    }
+    const double theta0 = 4. * asin(1.);  // LoadBeam.cpp:127
+    launch_quiet_load(c->stream, ns, c->d_off, d_starts, d_slices, nbins, theta0, g, th, x, y, px, py);
+    G4_HIP(hipGetLastError());
+    if (!draws.empty()) {
+        launch_shot_noise(c->stream, ns, c->d_off, d_dbase, nbins, nh, d_draws, th, maxpart);
+        G4_HIP(hipGetLastError());
+    }
+    std::vector<double> cur(ns);
+    for (int j = 0; j < ns; j++) cur[j] = slices[j].current;
+    G4_HIP(hipMemcpyAsync(c->d_cur, cur.data(), ns * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    if (mixed) {
+        // offset form as mix_beam_upload: gref = the mean energy rounded to
+        // FP32, a phase base per slice
+        double *d_sums = (double *)(tmp + o_sums);
+        std::vector<double> sums(kSumBlocks);
+        launch_sum_gamma(c->stream, g, n, d_sums, kSumBlocks);
+        G4_HIP(hipGetLastError());
+        G4_HIP(hipMemcpyAsync(sums.data(), d_sums, kSumBlocks * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+        G4_HIP(hipStreamSynchronize(c->stream));
+        double gsum = 0;
+        for (double v : sums) gsum += v;
+        c->gref = n > 0 ? (double)(float)(gsum / (double)n) : 1.0;
+        launch_theta_base(c->stream, ns, c->d_off, th, c->m_base);
+        G4_HIP(hipGetLastError());
+        launch_to_offsets(c->stream, ns, c->d_off, n, c->m_base, c->gref, g, th, x, y, px, py, c->m_dg, c->m_dth,
+                          c->m_x, c->m_y, c->m_px, c->m_py, maxpart);
+        G4_HIP(hipGetLastError());
+        mix_clear_graphs(c);  // gref is part of every captured launch
+    }
+    G4_HIP(hipStreamSynchronize(c->stream));
+    c->h_cur = cur;
     c->has_beam = true;
     return G4MI_OK;
 }

@@ -436,10 +436,14 @@ void quiet_finish(double *g, double *th, double *x, double *y, double *px, doubl
                   int npart, int nbins, double theta0);
 }  // namespace
 
+unsigned int quiet_start(int islice) {
+    Ran2 r((unsigned int)islice);
+    return (unsigned int)(int)round(r.next() * 1e9);
+}
+
 void quiet_load(double *g, double *th, double *x, double *y, double *px, double *py,
                 const SliceBeam &sl, int npart, int nbins, double theta0, int islice) {
-    Ran2 r((unsigned int)islice);
-    const unsigned int start = (unsigned int)(int)round(r.next() * 1e9);
+    const unsigned int start = quiet_start(islice);
     Halton h[6] = {Halton(2, start), Halton(3, start), Halton(5, start),
                    Halton(7, start), Halton(11, start), Halton(13, start)};
     const int mpart = npart / nbins;
@@ -575,18 +579,34 @@ unsigned int shot_noise_seed(int seed, int rank) {
     return (unsigned int)(int)round(v * 1e9);
 }
 
-void shot_noise(Ran2 &stream, double *th, int npart, int nbins, double ne) {
-    if (ne <= 0) return;
+long long shot_noise_draws(Ran2 &stream, int npart, int nbins, double ne, std::vector<double> &out) {
+    if (ne <= 0) return 0;
     const int mpart = npart / nbins;
     double nbl = ne / (double)mpart;
     if (nbl < 1) nbl = 1;
-    std::vector<double> dth(npart, 0.0);
     const double twopi = 4 * asin(1.);
+    const long long nd = (long long)((nbins - 1) / 2) * mpart;
     for (int ih = 0; ih < (nbins - 1) / 2; ih++) {
         for (int i1 = 0; i1 < mpart; i1++) {
             const double phi = 4 * asin(1) * stream.next();
             double an = sqrt(-log(stream.next()) / nbl) * 2 / (double)(ih + 1);
             if (an > twopi) an = fmod(an, twopi);
+            out.push_back(phi);
+            out.push_back(an);
+        }
+    }
+    return nd;
+}
+
+void shot_noise(Ran2 &stream, double *th, int npart, int nbins, double ne) {
+    std::vector<double> d;
+    d.reserve(2 * (size_t)((nbins - 1) / 2) * (size_t)(npart / nbins));
+    if (shot_noise_draws(stream, npart, nbins, ne, d) == 0) return;
+    const int mpart = npart / nbins;
+    std::vector<double> dth(npart, 0.0);
+    for (int ih = 0; ih < (nbins - 1) / 2; ih++) {
+        for (int i1 = 0; i1 < mpart; i1++) {
+            const double phi = d[2 * ((size_t)ih * mpart + i1)], an = d[2 * ((size_t)ih * mpart + i1) + 1];
             for (int i2 = 0; i2 < nbins; i2++) {
                 const int k = i1 * nbins + i2;
                 dth[k] -= an * sin(th[k] * (ih + 1) + phi);

@@ -82,6 +82,13 @@ void quiet_load_one4one(Ran2 &stream, double *gamma, double *theta, double *x, d
                         double *py, const SliceBeam &sl, int npart, double theta0);
 // Shot noise (src/Loading/ShotNoise.cpp:29-68) consuming `stream`.
 void shot_noise(Ran2 &stream, double *theta, int npart, int nbins, double ne);
+// The draws of shot_noise for one slice in stream order, appended to out as
+// (phase, amplitude) pairs, harmonic-major then beamlet; returns the pair
+// count (0 when ne <= 0: the stream is then not advanced).
+long long shot_noise_draws(Ran2 &stream, int npart, int nbins, double ne, std::vector<double> &out);
+// First Hammersley index of slice islice's sequences: round(1e9 u), u the
+// first ran2 draw seeded with the slice index (QuietLoading.cpp:54-65).
+unsigned int quiet_start(int islice);
 // Seed stream of ShotNoise::init (:16-27).
 unsigned int shot_noise_seed(int seed, int rank);
 double inverse_erfc(double y);

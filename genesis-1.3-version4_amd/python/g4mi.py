@@ -123,6 +123,8 @@ def lib():
         L.g4mi_load_beam.argtypes = [C.c_int, _lp, C.POINTER(SliceBeam), C.c_int, C.c_int,
                                      C.c_int, C.c_int, C.c_double, C.c_double, C.c_int] + [_dp] * 6
         L.g4mi_load_gauss.argtypes = [_dp, C.c_int] + [C.c_double] * 6
+        L.g4mi_load_beam_device.argtypes = [C.c_void_p, C.POINTER(SliceBeam), C.c_int, C.c_int, C.c_int,
+                                            C.c_int, C.c_double, C.c_double, C.c_int]
         L.g4mi_run_deck.argtypes = [C.c_char_p, C.c_int, C.c_int]
         _fp = C.POINTER(C.c_float)
         L.g4mi_fno_create.argtypes = [C.POINTER(C.c_void_p), C.c_int, C.c_int, C.c_int, C.c_int,
@@ -330,6 +332,17 @@ class Context:
         a = [np.ascontiguousarray(beam[k], dtype=np.float64)
              for k in ("gamma", "theta", "x", "y", "px", "py")]
         _chk(lib().g4mi_beam_upload(self.h, *[_p(v) for v in a], _p(cur)), "g4mi_beam_upload")
+
+    def load_beam(self, slices, nbins, shotnoise, seed, lambda0, sample=1.0, timerun=True, islice0=0):
+        """LoadBeam::init into the device arrays (g4mi_load_beam_device): the
+        beam of load_beam() with the context's slice counts, uploaded with
+        the slices' currents."""
+        if len(slices) != self.ns:
+            raise ValueError(f"load_beam: {len(slices)} slice records for {self.ns} slices")
+        sb = (SliceBeam * self.ns)(*[SliceBeam(**{k: float(s.get(k, 0.0)) for k in SLICE_FIELDS})
+                                     for s in slices])
+        _chk(lib().g4mi_load_beam_device(self.h, sb, nbins, int(shotnoise), seed, islice0, lambda0, sample,
+                                         int(timerun)), "g4mi_load_beam_device")
 
     def download_beam(self):
         """The particles in their current slice order (after one4one sorting

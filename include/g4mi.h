@@ -308,6 +308,18 @@ int g4mi_load_beam(int nslice, const long long *off, const g4mi_slice_beam *slic
                    int timerun, double *gamma, double *theta, double *x, double *y, double *px,
                    double *py);
 
+/* LoadBeam::init (src/Loading/LoadBeam.cpp:127-212) straight into the
+ * context's device arrays: the same beam as g4mi_load_beam followed by
+ * g4mi_beam_upload with current[j] = slices[j].current, for the slice
+ * counts the context was created with.  Quiet loading
+ * (QuietLoading::loadQuiet, src/Loading/QuietLoading.cpp:50-260) runs on
+ * the device, one workgroup per slice; the shot-noise draws
+ * (ShotNoise::applyShotNoise, src/Loading/ShotNoise.cpp:29-68) come from the
+ * host's sequential ran2 stream and are applied on the device.  Agrees with
+ * the host loader to rounding (device log/exp/sin and block sums). */
+int g4mi_load_beam_device(g4mi_ctx *ctx, const g4mi_slice_beam *slices, int nbins, int shotnoise,
+                          int seed, int islice0, double lambda0, double sample, int timerun);
+
 /* one4one beams (one macroparticle per electron), LoadBeam::init with
  * one4one (src/Loading/LoadBeam.cpp:129-132, 157-199): counts[j] =
  * round(current lambda0 sample / ce) electrons per slice; then every
