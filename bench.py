@@ -118,6 +118,16 @@ def roofline(cfg, kt, ntot, precision=g4mi.FP64, schedule=None):
       front (transverse + moments): read 5 + write 4 floats = 36 B/particle."""
     cells = cfg["nslice"] * cfg["ngrid"] ** 2
     top = max((k for k in kt if k != "steps"), key=kt.get)
+    if schedule and schedule.get("field_solver") == "fft" and top == "adi" and \
+            schedule.get("propagation") == "fft-conv":
+        # spectral step as FFT convolutions of every row, then every column:
+        # the field read + written once = 32 B/cell, as the ADI
+        ms = kt["adi"]
+        gbs = 32.0 * cells / (ms * 1e-3) / 1e9
+        return "adi", dict(bound="hbm", achieved=round(gbs, 1), peak=HBM_PEAK_GBS, unit="GB/s",
+                           frac=round(gbs / HBM_PEAK_GBS, 4), traffic=None, kernel="field (fft)",
+                           device_kernel="k_fft_conv", ms_per_launch=round(ms, 5),
+                           algorithmic_bytes=int(32.0 * cells))
     if schedule and schedule.get("field_solver") == "fft" and top == "adi":
         # spectral step: two complex n x n x n products per slice (E M^T, then
         # M T), 8 real flops per complex multiply-add, on FP64 MFMA

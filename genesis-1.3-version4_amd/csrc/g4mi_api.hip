@@ -86,6 +86,11 @@ struct Ctx {
     int fft = 0, fft_filter = 0;
     int fft_variant = -1;  // complex GEMM variant chosen by the tuner (-1: not tuned yet)
     float fft_ms[kCgemmVariants] = {-1.f, -1.f, -1.f, -1.f};
+    // propagation: 0 the two GEMMs with M, 1 FFT convolution (-1: the tuner
+    // picks the faster); the convolution's length and tables per delz
+    int fft_prop = -1, fft_conv_L = 0;
+    float fft_conv_ms = -1.f;
+    double2 *d_fft_hk = nullptr, *d_fft_tw = nullptr;
     double fft_xc = 1, fft_yc = 1, fft_sig = 1;
     bool fft_ready = false;
     double fft_delz = 0;
@@ -544,9 +549,10 @@ static int tune_fft(Ctx *c) {
     g.a = c->d_field;
     g.mat = c->d_fft_m;
     g.c = c->d_work;
-    int best = 0;
+    int best = c->fft_variant;
     float best_ms = 1e30f;
     for (int v = 0; v < kCgemmVariants; v++) {
+        if (c->fft_variant >= 0 && v != c->fft_variant) continue;  // a fixed variant is only timed
         g.variant = v;
         G4_TRY(launch_cgemm_rows(c->stream, g));  // warm-up
         float b = 1e30f;
@@ -560,14 +566,42 @@ static int tune_fft(Ctx *c) {
             b = std::min(b, ms);
         }
         c->fft_ms[v] = b;
-        if (b < best_ms) {
+        if (b < best_ms || best < 0) {
             best_ms = b;
             best = v;
         }
     }
+    c->fft_variant = best;
+    if (c->fft_prop < 0) {  // the convolution's rows pass against the best GEMM's
+        c->fft_prop = 0;
+        c->fft_conv_ms = -1.f;
+        if (c->fft_conv_L) {
+            ConvArgs a{};
+            a.in = c->d_field;
+            a.out = c->d_work;
+            a.hk = c->d_fft_hk;
+            a.tw = c->d_fft_tw;
+            a.nrows = (long long)c->ns * c->ng;
+            a.ng = c->ng;
+            a.ns = c->ns;
+            a.L = c->fft_conv_L;
+            G4_TRY(launch_fft_conv_rows(c->stream, a));  // warm-up
+            float b = 1e30f;
+            for (int rep = 0; rep < 3; rep++) {
+                G4_HIP(hipEventRecord(e0, c->stream));
+                G4_TRY(launch_fft_conv_rows(c->stream, a));
+                G4_HIP(hipEventRecord(e1, c->stream));
+                G4_HIP(hipEventSynchronize(e1));
+                float ms = 0;
+                G4_HIP(hipEventElapsedTime(&ms, e0, e1));
+                b = std::min(b, ms);
+            }
+            c->fft_conv_ms = b;
+            if (b < c->fft_ms[best]) c->fft_prop = 1;
+        }
+    }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
-    c->fft_variant = best;
     return G4MI_OK;
 }
 
@@ -581,10 +615,28 @@ static int fft_field_step(Ctx *c, double delz) {
         G4_HIP(hipMemcpyAsync(c->d_fft_m, m.data(), m.size() * sizeof(double), hipMemcpyHostToDevice,
                               c->stream));
         G4_HIP(hipStreamSynchronize(c->stream));
+        c->fft_conv_L = fft_conv_length(ng);
+        if (c->fft_conv_L) {
+            if (!c->d_fft_hk) {
+                G4_TRY(dalloc(c, &c->d_fft_hk, 512));
+                G4_TRY(dalloc(c, &c->d_fft_tw, 512));
+            }
+            std::vector<double> hk(2 * (size_t)c->fft_conv_L), tw(2 * (size_t)c->fft_conv_L);
+            fft_conv_tables(ng, c->dgrid, c->xks, delz, hk.data(), tw.data());
+            G4_HIP(hipMemcpyAsync(c->d_fft_hk, hk.data(), hk.size() * sizeof(double), hipMemcpyHostToDevice,
+                                  c->stream));
+            G4_HIP(hipMemcpyAsync(c->d_fft_tw, tw.data(), tw.size() * sizeof(double), hipMemcpyHostToDevice,
+                                  c->stream));
+            G4_HIP(hipStreamSynchronize(c->stream));
+        }
         c->fft_delz = delz;
         c->fft_ready = true;
     }
-    if (c->fft_variant < 0) G4_TRY(tune_fft(c));
+    if (c->fft_prop == 1 && !c->fft_conv_L) {
+        set_error("fft_propagation 1: the FFT convolution supports ngrid <= 256");
+        return G4MI_E_ARG;
+    }
+    if (c->fft_variant < 0 || c->fft_prop < 0) G4_TRY(tune_fft(c));
     CgemmArgs g{};
     g.ng = ng;
     g.ns = c->ns;
@@ -613,6 +665,28 @@ static int fft_field_step(Ctx *c, double delz) {
         g.scale = 1.0 / (double)nn;
         G4_TRY(launch_cgemm_cols(c->stream, g));
         g.scale = 1.0;
+    }
+    if (c->fft_prop == 1) {  // the same two products as FFT convolutions of every line
+        ConvArgs a{};
+        a.hk = c->d_fft_hk;
+        a.tw = c->d_fft_tw;
+        a.nrows = (long long)c->ns * ng;
+        a.ng = ng;
+        a.ns = c->ns;
+        a.L = c->fft_conv_L;
+        a.in = c->d_field;
+        a.out = c->d_work;
+        G4_TRY(launch_fft_conv_rows(c->stream, a));
+        a.in = c->d_work;
+        a.out = c->d_field;
+        a.box = c->d_box;
+        a.ring = 1;
+        a.first = c->first;
+        if (c->fft_filter)
+            a.add = c->d_fft_v;
+        else
+            a.src = c->d_src;
+        return launch_fft_conv_cols(c->stream, a);
     }
     g.a = c->d_field;  // T = E M^T
     g.mat = c->d_fft_m;
@@ -2773,6 +2847,14 @@ int g4mi_set_schedule(g4mi_ctx *h, const char *name, int value) {
         c->fft_variant = value;  // -1: tune again at the next spectral step
         return G4MI_OK;
     }
+    if (std::string(name) == "fft_propagation" && value >= -1 && value <= 1) {
+        if (value == 1 && !fft_conv_length(c->ng)) {
+            set_error("g4mi_set_schedule: fft_propagation 1 supports ngrid <= 256");
+            return G4MI_E_ARG;
+        }
+        c->fft_prop = value;  // -1: tune again at the next spectral step
+        return G4MI_OK;
+    }
     if (std::string(name) == "adi_refine" && (value == 0 || value == 1)) {
         c->m_refine = value != 0;
         return G4MI_OK;
@@ -2847,9 +2929,11 @@ int g4mi_get_schedule(g4mi_ctx *h, char *buf, int len) {
     if (c->fft) {
         snprintf(tmp, sizeof(tmp),
                  "{\"field_solver\": \"fft\", \"source_filter\": %d, \"gemm_tile\": %d, \"gemm_variant\": %d, "
-                 "\"gemm_tuned_ms\": [%.4f, %.4f, %.4f, %.4f]}",
+                 "\"gemm_tuned_ms\": [%.4f, %.4f, %.4f, %.4f], \"propagation\": \"%s\", "
+                 "\"conv_length\": %d, \"conv_tuned_ms\": %.4f, \"fp64_fused\": %d}",
                  c->fft_filter, cgemm_tile(c->ng), c->fft_variant, c->fft_ms[0], c->fft_ms[1], c->fft_ms[2],
-                 c->fft_ms[3]);
+                 c->fft_ms[3], c->fft_prop == 1 ? "fft-conv" : (c->fft_prop == 0 ? "gemm" : "untuned"),
+                 fft_conv_length(c->ng), c->fft_conv_ms, c->fuse64 ? 1 : 0);
         snprintf(buf, (size_t)len, "%s", tmp);
         return G4MI_OK;
     }

@@ -40,6 +40,26 @@ int launch_cgemm_cols(hipStream_t s, const CgemmArgs &g);
 // output tile edge used for ngrid (32, 48, 64 or 80)
 int cgemm_tile(int ng);
 
+// The propagation E' = M E M^T (+ 2 S or 2 Y) as circular convolutions of
+// every line with the propagator kernel, each by a zero-padded FFT of length
+// L (Stockham radix-R passes in LDS, one wavefront per line, 8 lines per
+// workgroup): rows pass out = in M^T over all ns*ng rows; columns pass
+// out = M in per slice with the column epilogue of the GEMM (add, src in
+// box, ring).  hk/tw: the tables of fft_conv_tables, L entries each.
+struct ConvArgs {
+    const double2 *in;
+    double2 *out;
+    const double2 *hk, *tw;
+    long long nrows;  // rows pass: ns * ng
+    int ng, ns, L;
+    const double2 *add;  // columns pass epilogue, as CgemmArgs
+    const double2 *src;
+    const int4 *box;
+    int first, ring;
+};
+int launch_fft_conv_rows(hipStream_t s, const ConvArgs &a);
+int launch_fft_conv_cols(hipStream_t s, const ConvArgs &a);
+
 // Moments of |F|^2 over the FFT index of each of ns slices F[ns][ng][ng]
 // (DiagField::getValues, src/Core/Diagnostic.cpp:789-807): out[5 s + 0..4] =
 // sum w, sum dx w, sum dx^2 w, sum dy w, sum dy^2 w, with w = |F|^2 and

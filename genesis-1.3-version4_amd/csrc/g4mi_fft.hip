@@ -286,4 +286,241 @@ void launch_widen_field(hipStream_t s, const float2 *in, double2 *out, long long
 int launch_cgemm_rows(hipStream_t s, const CgemmArgs &g) { return launch_cgemm<true>(s, g); }
 int launch_cgemm_cols(hipStream_t s, const CgemmArgs &g) { return launch_cgemm<false>(s, g); }
 
+// ---- propagation by FFT convolution ---------------------------------------
+//
+// Every line (a row, or a column of one slice) of n values is convolved
+// circularly with the propagator kernel m: zero-padded to L >= 2n - 1, the
+// cyclic convolution of length L with the kernel placed at d mod L for
+// |d| < n equals the circular one of length n on the first n outputs.  The
+// FFTs are Stockham autosort passes (Govindaraju et al., "High Performance
+// Discrete Fourier Transforms on Graphics Processors", SC08): L = R^NST with
+// R = L / 64, so each of a wavefront's 64 lanes does one radix-R butterfly
+// per pass on R values held in registers, reading them at stride 64 and
+// writing them at stride ns in natural order for the next pass.
+//
+// The last forward pass leaves lane j holding X[j + 64 r], which is what the
+// first inverse pass reads: the product with the kernel's transform (hk,
+// already divided by L) happens in registers, with no exchange.  As n <= L/2,
+// the first pass's inputs r >= R/2 are zero and the last pass's outputs
+// r >= R/2 are not needed.  Rows are read and written by the lanes directly
+// (64 consecutive values per access); columns go through a tile of 8
+// adjacent columns in LDS so global accesses stay row-contiguous, with the
+// column groups of a slice numbered onto one XCD so the 128-byte lines they
+// share are fetched into one L2.  Within an LDS line, entry p is stored at
+// p ^ ((p >> 3) & 31): with it the stride-ns writes and stride-64 reads of
+// the radix-8 exchanges touch every bank once per pass whether the LDS
+// serves 8, 16 or 32 lanes of 16 bytes per cycle.
+
+__device__ __forceinline__ double2 conv_cmul(double2 a, double2 b) {
+    return make_double2(fma(a.x, b.x, -a.y * b.y), fma(a.x, b.y, a.y * b.x));
+}
+__device__ __forceinline__ double2 conv_add(double2 a, double2 b) { return make_double2(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ double2 conv_sub(double2 a, double2 b) { return make_double2(a.x - b.x, a.y - b.y); }
+// -i z (forward) or +i z (inverse)
+template <bool INV>
+__device__ __forceinline__ double2 conv_rot(double2 z) {
+    return INV ? make_double2(-z.y, z.x) : make_double2(z.y, -z.x);
+}
+__device__ __forceinline__ int conv_swz(int p) { return p ^ ((p >> 3) & 31); }
+
+// in-register DFT of length R (2, 4 or 8), sign - (forward) or + (INV)
+template <int R, bool INV>
+__device__ __forceinline__ void conv_dft(double2 (&v)[R]) {
+    if constexpr (R == 2) {
+        const double2 a = v[0], b = v[1];
+        v[0] = conv_add(a, b);
+        v[1] = conv_sub(a, b);
+    } else if constexpr (R == 4) {
+        const double2 t0 = conv_add(v[0], v[2]), t1 = conv_sub(v[0], v[2]);
+        const double2 t2 = conv_add(v[1], v[3]), t3 = conv_rot<INV>(conv_sub(v[1], v[3]));
+        v[0] = conv_add(t0, t2);
+        v[2] = conv_sub(t0, t2);
+        v[1] = conv_add(t1, t3);
+        v[3] = conv_sub(t1, t3);
+    } else {
+        double2 e[4] = {v[0], v[2], v[4], v[6]}, o[4] = {v[1], v[3], v[5], v[7]};
+        conv_dft<4, INV>(e);
+        conv_dft<4, INV>(o);
+        const double c = 0.70710678118654752440;  // sqrt(1/2)
+        // o[k] *= exp(-+ 2 pi i k / 8)
+        o[1] = INV ? make_double2(c * (o[1].x - o[1].y), c * (o[1].x + o[1].y))
+                   : make_double2(c * (o[1].x + o[1].y), c * (o[1].y - o[1].x));
+        o[2] = conv_rot<INV>(o[2]);
+        o[3] = INV ? make_double2(-c * (o[3].x + o[3].y), c * (o[3].x - o[3].y))
+                   : make_double2(c * (o[3].y - o[3].x), -c * (o[3].x + o[3].y));
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            v[k] = conv_add(e[k], o[k]);
+            v[k + 4] = conv_sub(e[k], o[k]);
+        }
+    }
+}
+
+// one Stockham pass of span ns on lane j's registers: twiddles
+// w^(r (j mod ns) L / (ns R)), w = exp(-+ 2 pi i / L), from tw[] for r = 1
+// and by products for higher r; then the radix-R butterfly
+template <int R, bool INV>
+__device__ __forceinline__ void conv_pass(double2 (&v)[R], int ns, int j, const double2 *__restrict__ tw) {
+    constexpr int L = 64 * R;
+    if (ns > 1) {
+        double2 w1 = tw[(j % ns) * (L / (ns * R))];
+        if (INV) w1.y = -w1.y;
+        double2 wr = w1;
+        v[1] = conv_cmul(v[1], wr);
+#pragma unroll
+        for (int r = 2; r < R; r++) {
+            wr = conv_cmul(wr, w1);
+            v[r] = conv_cmul(v[r], wr);
+        }
+    }
+    conv_dft<R, INV>(v);
+}
+
+// workgroup barrier that orders LDS only: global loads issued before it (the
+// next line's prefetch) stay in flight
+__device__ __forceinline__ void conv_lds_barrier() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+    __builtin_amdgcn_s_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
+// a whole transform on registers, exchanging through this wavefront's LDS
+// line d between passes; in: v[r] = x[j + 64 r]; out: v[r] = X[j + 64 r]
+template <int R, int NST, bool INV>
+__device__ __forceinline__ void conv_fft(double2 (&v)[R], double2 *d, int j, const double2 *__restrict__ tw) {
+    int ns = 1;
+#pragma unroll
+    for (int st = 0; st < NST; st++) {
+        conv_pass<R, INV>(v, ns, j, tw);
+        if (st < NST - 1) {
+            const int dst = (j / ns) * ns * R + j % ns;
+#pragma unroll
+            for (int r = 0; r < R; r++) d[conv_swz(dst + r * ns)] = v[r];
+            conv_lds_barrier();
+#pragma unroll
+            for (int r = 0; r < R; r++) v[r] = d[conv_swz(j + 64 * r)];
+            conv_lds_barrier();
+        }
+        ns *= R;
+    }
+}
+
+__device__ __forceinline__ bool conv_in_box(const int4 &bx, int x, int y) {
+    return x >= bx.x && x <= bx.z && y >= bx.y && y <= bx.w;
+}
+
+template <bool COLS>
+struct ConvLines {
+    static constexpr int value = 4;  // lines (wavefronts) per workgroup
+};
+
+// One chunk of NL lines per workgroup (rows: NL consecutive rows; columns:
+// NL adjacent columns of one slice).  Workgroups go to the 8 XCDs
+// round-robin; the chunks are numbered so that XCD b % 8 takes a contiguous
+// range of them, and the column groups of a slice, which share 128-byte
+// lines, are read through one L2.
+template <int R, int NST, bool COLS>
+__global__ void __launch_bounds__(64 * ConvLines<COLS>::value) k_fft_conv(ConvArgs a, int nchunks) {
+    constexpr int L = 64 * R, NL = ConvLines<COLS>::value, NT = 64 * NL, LP = L + 2, NP = R / 2;
+    __shared__ double2 buf[NL * LP];
+    const int t = threadIdx.x, j = t & 63, w = t >> 6;
+    const int ng = a.ng;
+    const long long nn = (long long)ng * ng;
+    const int c = (blockIdx.x % 8) * (gridDim.x / 8) + blockIdx.x / 8;
+    if (c >= nchunks) return;  // padding of the grid to a multiple of 8
+    double2 *d = buf + w * LP;
+    double2 v[R];
+#pragma unroll
+    for (int r = NP; r < R; r++) v[r] = make_double2(0., 0.);
+    const int cpb = (ng + NL - 1) / NL;
+    const int s = COLS ? c / cpb : 0, x0 = COLS ? (c % cpb) * NL : 0;
+    const long long g = COLS ? 0 : (long long)c * NL + w;
+    if (COLS) {
+        // tile of NL adjacent columns, positions [0, L/2) (zero from ng on)
+#pragma unroll
+        for (int q = 0; q < NP; q++) {
+            const int e = t + NT * q, line = e % NL, pos = e / NL, x = x0 + line;
+            double2 u = make_double2(0., 0.);
+            if (pos < ng && x < ng) u = a.in[(long long)s * nn + (long long)pos * ng + x];
+            buf[line * LP + conv_swz(pos)] = u;
+        }
+        conv_lds_barrier();
+#pragma unroll
+        for (int r = 0; r < NP; r++) v[r] = d[conv_swz(j + 64 * r)];
+        conv_lds_barrier();
+    } else {
+#pragma unroll
+        for (int r = 0; r < NP; r++) {
+            const int p = j + 64 * r;
+            v[r] = (g < a.nrows && p < ng) ? a.in[g * ng + p] : make_double2(0., 0.);
+        }
+    }
+    conv_fft<R, NST, false>(v, d, j, a.tw);
+#pragma unroll
+    for (int r = 0; r < R; r++) v[r] = conv_cmul(v[r], a.hk[j + 64 * r]);
+    conv_fft<R, NST, true>(v, d, j, a.tw);
+    if (!COLS) {
+        if (g < a.nrows) {
+#pragma unroll
+            for (int r = 0; r < NP; r++) {
+                const int p = j + 64 * r;
+                if (p < ng) a.out[g * ng + p] = v[r];
+            }
+        }
+        return;
+    }
+#pragma unroll
+    for (int r = 0; r < NP; r++) d[conv_swz(j + 64 * r)] = v[r];
+    conv_lds_barrier();
+    const int ii = a.ring ? (s - a.first + a.ns) % a.ns : s;
+    int4 bx = make_int4(1, 1, 0, 0);
+    if (a.src) bx = a.box[ii];
+    for (int e = t; e < NL * ng; e += NT) {
+        const int line = e % NL, y = e / NL, x = x0 + line;
+        if (x >= ng) continue;
+        double2 u = buf[line * LP + conv_swz(y)];
+        const long long o = (long long)y * ng + x;
+        if (a.add) {
+            const double2 a2 = a.add[(long long)ii * nn + o];
+            u.x += 2. * a2.x;
+            u.y += 2. * a2.y;
+        }
+        if (a.src && conv_in_box(bx, x, y)) {
+            const double2 s2 = a.src[(long long)ii * nn + o];
+            u.x += 2. * s2.x;
+            u.y += 2. * s2.y;
+        }
+        a.out[(long long)s * nn + o] = u;
+    }
+}
+
+template <bool COLS>
+static int launch_fft_conv(hipStream_t st, const ConvArgs &a) {
+    if (a.ng < 2 || !a.in || !a.out || !a.hk || !a.tw || a.in == a.out || a.L < 2 * a.ng - 1 ||
+        (a.L != 128 && a.L != 256 && a.L != 512)) {
+        set_error("fft_conv: bad arguments");
+        return G4MI_E_ARG;
+    }
+    constexpr int NL = ConvLines<COLS>::value;
+    const long long chunks = COLS ? (long long)a.ns * ((a.ng + NL - 1) / NL) : (a.nrows + NL - 1) / NL;
+    if (chunks < 1) return G4MI_OK;
+    if (chunks > 0x7ffffff0LL) {
+        set_error("fft_conv: problem too large for one launch");
+        return G4MI_E_ARG;
+    }
+    const long long blocks = (chunks + 7) / 8 * 8;  // one chunk per workgroup, every XCD the same count
+    const dim3 grid((unsigned)blocks), block(64 * NL);
+    const int nc = (int)chunks;
+    switch (a.L) {
+        case 128: hipLaunchKernelGGL((k_fft_conv<2, 7, COLS>), grid, block, 0, st, a, nc); break;
+        case 256: hipLaunchKernelGGL((k_fft_conv<4, 4, COLS>), grid, block, 0, st, a, nc); break;
+        default: hipLaunchKernelGGL((k_fft_conv<8, 3, COLS>), grid, block, 0, st, a, nc); break;
+    }
+    G4_HIP(hipGetLastError());
+    return G4MI_OK;
+}
+
+int launch_fft_conv_rows(hipStream_t s, const ConvArgs &a) { return launch_fft_conv<false>(s, a); }
+int launch_fft_conv_cols(hipStream_t s, const ConvArgs &a) { return launch_fft_conv<true>(s, a); }
+
 }  // namespace g4mi

@@ -198,7 +198,10 @@ double fft_offset(int n, int k) {
 }
 }  // namespace
 
-void fft_propagator(int n, double dgrid, double xks, double delz, double *m) {
+namespace {
+// m[d] = (1/n) sum_k p_k exp(2 pi i k d / n), p_k = exp(i K2_1(k) delz), in
+// long double with the DFT exponents reduced mod n exactly
+std::vector<std::complex<long double>> propagator_kernel(int n, double dgrid, double xks, double delz) {
     typedef std::complex<long double> cl;
     const double dk = 4. * asin(1.) / ((double)n * dgrid);
     std::vector<cl> p(n), md(n);
@@ -216,12 +219,51 @@ void fft_propagator(int n, double dgrid, double xks, double delz, double *m) {
         }
         md[d] = acc / (long double)n;
     }
+    return md;
+}
+}  // namespace
+
+void fft_propagator(int n, double dgrid, double xks, double delz, double *m) {
+    typedef std::complex<long double> cl;
+    const std::vector<cl> md = propagator_kernel(n, dgrid, xks, delz);
     for (int j = 0; j < n; j++)
         for (int l = 0; l < n; l++) {
             const cl v = md[((j - l) % n + n) % n];
             m[2 * ((size_t)j * n + l)] = (double)v.real();
             m[2 * ((size_t)j * n + l) + 1] = (double)v.imag();
         }
+}
+
+int fft_conv_length(int n) {
+    int L = 128;
+    while (L < 2 * n - 1) L *= 2;
+    return (n >= 2 && L <= 512) ? L : 0;
+}
+
+int fft_conv_tables(int n, double dgrid, double xks, double delz, double *hk, double *tw) {
+    typedef std::complex<long double> cl;
+    const int L = fft_conv_length(n);
+    if (!L) return 0;
+    const std::vector<cl> md = propagator_kernel(n, dgrid, xks, delz);
+    const long double twopi = 8.0L * atanl(1.0L);
+    std::vector<cl> w(L);
+    for (int k = 0; k < L; k++) {
+        const long double a = twopi * (long double)k / (long double)L;
+        w[k] = cl(cosl(a), -sinl(a));
+        tw[2 * k] = (double)w[k].real();
+        tw[2 * k + 1] = (double)w[k].imag();
+    }
+    for (int k = 0; k < L; k++) {  // (1/L) sum over |d| < n of m[d mod n] exp(-2 pi i k d / L)
+        cl acc(0, 0);
+        for (int d = -(n - 1); d < n; d++) {
+            const int dl = (d % L + L) % L;
+            acc += md[(d % n + n) % n] * w[(int)(((long long)k * dl) % L)];
+        }
+        acc /= (long double)L;
+        hk[2 * k] = (double)acc.real();
+        hk[2 * k + 1] = (double)acc.imag();
+    }
+    return L;
 }
 
 void dft_matrix(int n, int sign, double *w) {

@@ -42,6 +42,13 @@ double adi_band(int ngrid, const double *c, const double *cbet, const double *cw
 // at FFT index k.  exp(K2 delz) = p_ky p_kx, so the 2-D step
 // IDFT(exp(K2 delz) DFT(E)) / n^2 equals M E M^T.
 void fft_propagator(int n, double dgrid, double xks, double delz, double *m);
+// The same step as a circular convolution of every line with m, run as a
+// zero-padded FFT of length L = fft_conv_length(n) (the power of two >= 2n-1,
+// at least 128; 0 when n > 256): hk[k] = (1/L) sum_{|d|<n} m[d mod n]
+// exp(-2 pi i k d / L) and tw[k] = exp(-2 pi i k / L), L interleaved complex
+// entries each.  Returns L (0: not supported for this n).
+int fft_conv_length(int n);
+int fft_conv_tables(int n, double dgrid, double xks, double delz, double *hk, double *tw);
 // W[j][l] = exp(sign 2 pi i j l / n), the exponent reduced mod n exactly.
 void dft_matrix(int n, int sign, double *w);
 // The source filter's sigmoid at FFT index [ky][kx] (:141-147), n*n reals.

@@ -250,7 +250,10 @@ int g4mi_sort_index_noalloc(g4mi_ctx *ctx, double slen, int *atar, long long *mo
  * step allows it.  The spectral solver's first step likewise times its
  * complex-GEMM variants ("fft_variant": 0 four real products per complex
  * one and k chunks of 16, 1 three products, 2 four products with chunks of
- * 32, 3 three with 32; -1 tunes again).
+ * 32, 3 three with 32; -1 tunes again) and its propagation by those GEMMs
+ * against FFT convolutions of every row and column (zero-padded
+ * power-of-two FFTs, ngrid <= 256), and keeps the faster
+ * ("fft_propagation": 0 GEMMs, 1 FFT convolution, -1 tunes again).
  * Mixed form: its first 14 steps time the main kernel's shapes (4, 6 or 8
  * wavefronts per slice x 68 or 44 KiB of LDS), the three ADI kernels,
  * and the step as a graph of kernels against one persistent

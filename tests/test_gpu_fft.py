@@ -2,7 +2,8 @@
 src/Core/FieldSolverFFT.cpp) against the oracle's restatement (a direct
 2-D DFT per slice, pinned against numpy.fft and the analytic Gaussian
 beam).  The device runs the step as E' = M E M^T + 2 S with M the circulant
-1-D propagator, FP64 MFMA products; the oracle transforms, multiplies by
+1-D propagator, either as FP64 MFMA products or as FFT convolutions of
+every row and column; the oracle transforms, multiplies by
 exp(K2 delz) and transforms back.  The two differ only by rounding: field
 within 1e-12 of max |E| per step, P(z) and |b(z)| within 1e-10 over a run."""
 import ctypes as C
@@ -49,6 +50,7 @@ def test_fft_field_track_parity(ngrid, variant):
     st = H.steps()
     sim, ctx = _fft_state([1024] * 6, ngrid, seed_power=5e7)
     ctx.set_schedule("fft_variant", variant)
+    ctx.set_schedule("fft_propagation", 0)
     gst = H.gsteps(st)
     for i in range(3):
         O.lib().orc_field_advance(C.byref(sim.s), C.byref(st[i]), st[i].dz)
@@ -59,6 +61,41 @@ def test_fft_field_track_parity(ngrid, variant):
     _log(f"  field max diff / max |E| = {err:.3e}")
     assert err < 1e-12
     assert ctx.schedule()["gemm_variant"] == variant
+    assert ctx.schedule()["propagation"] == "gemm"
+    ctx.close()
+
+
+@pytest.mark.parametrize("ngrid", [31, 40, 64, 65, 151, 255])
+def test_fft_conv_field_track_parity(ngrid):
+    """The propagation as FFT convolutions (zero-padded Stockham FFTs of
+    length 128, 256 or 512: radix 2, 4 and 8 passes), three undulator
+    steps against the oracle's 2-D DFT step; 64 and 65 straddle the switch
+    from length 128 to 256."""
+    _log(f"spectral field step parity, FFT convolution: 6 slices x 1024, {ngrid}^2, 3 steps")
+    st = H.steps()
+    sim, ctx = _fft_state([1024] * 6, ngrid, seed_power=5e7)
+    ctx.set_schedule("fft_propagation", 1)
+    gst = H.gsteps(st)
+    for i in range(3):
+        O.lib().orc_field_advance(C.byref(sim.s), C.byref(st[i]), st[i].dz)
+        ctx.field_track(gst[i], st[i].dz)
+    f, first = ctx.download_field()
+    assert first == sim.s.first
+    err = np.max(np.abs(f - sim.field)) / np.max(np.abs(sim.field))
+    sch = ctx.schedule()
+    _log(f"  field max diff / max |E| = {err:.3e}, length {sch['conv_length']}")
+    assert err < 1e-12
+    assert sch["propagation"] == "fft-conv"
+    assert sch["conv_length"] == max(128, 1 << (2 * ngrid - 2).bit_length())
+    ctx.close()
+
+
+def test_fft_conv_rejects_large_grids():
+    sim = H.make_state([256] * 2, ngrid=301, gridmax=0.3e-3)
+    ctx = H.to_gpu(sim)
+    ctx.set_field_solver(fft=True)
+    with pytest.raises(g4mi.G4miError, match="256"):
+        ctx.set_schedule("fft_propagation", 1)
     ctx.close()
 
 
@@ -73,6 +110,8 @@ def test_fft_variant_tuner():
     _log(f"  tuned: {sch}")
     ms = sch["gemm_tuned_ms"]
     assert sch["gemm_variant"] in range(4) and ms[sch["gemm_variant"]] == min(ms)
+    conv_faster = 0 <= sch["conv_tuned_ms"] < min(ms)
+    assert sch["propagation"] == ("fft-conv" if conv_faster else "gemm")
     f, _ = ctx.download_field()
     assert np.max(np.abs(f - sim.field)) < 1e-12 * np.max(np.abs(sim.field))
 
@@ -89,12 +128,15 @@ def test_fft_drift_step_is_pure_diffraction():
     assert err < 1e-13
 
 
-def test_fft_source_filter_parity():
+@pytest.mark.parametrize("prop", [0, 1])
+def test_fft_source_filter_parity(prop):
     """Source filtered by a sigmoid at a third of the band (xcut = ycut =
-    0.3, width 0.05), two undulator steps."""
-    _log("spectral step with source filter: 4 slices x 1024, 63^2")
+    0.3, width 0.05), two undulator steps, propagation by the GEMMs (0) or
+    by FFT convolution (1)."""
+    _log(f"spectral step with source filter: 4 slices x 1024, 63^2, propagation {prop}")
     st = H.steps()
     sim, ctx = _fft_state([1024] * 4, 63, gridmax=0.3e-3, seed_power=5e7, filt=(0.3, 0.3, 0.05))
+    ctx.set_schedule("fft_propagation", prop)
     gst = H.gsteps(st)
     for i in range(2):
         O.lib().orc_field_advance(C.byref(sim.s), C.byref(st[i]), st[i].dz)
@@ -105,13 +147,16 @@ def test_fft_source_filter_parity():
     assert err < 1e-12
 
 
-def test_fft_sase_end_to_end():
+@pytest.mark.parametrize("prop", [0, 1])
+def test_fft_sase_end_to_end(prop):
     """Time-dependent SASE with slippage on the spectral solver: 12 slices
     x 1024, 63^2, the first 200 steps (two undulator modules and a FODO
-    drift), P(z) and |b(z)| within 1e-10."""
-    _log("spectral solver SASE, 12 slices, 200 steps, GPU vs oracle")
+    drift), P(z) and |b(z)| within 1e-10; propagation by the GEMMs (0) or
+    by FFT convolution (1)."""
+    _log(f"spectral solver SASE, 12 slices, 200 steps, GPU vs oracle, propagation {prop}")
     st = H.steps()
     sim, ctx = _fft_state([1024] * 12, 63, gridmax=0.3e-3, longrange=True, threads=16)
+    ctx.set_schedule("fft_propagation", prop)
     g = ctx.run(H.gsteps(st)[:200], cases.GAMMA0, record=("power", "bunching"))
     r = sim.run(st, cases.GAMMA0, record=("power", "bunching"), nsteps=200)
     ep = H.rel_err(g["power"][1:], r["power"][1:])
