@@ -6,7 +6,7 @@
 namespace g4mi {
 
 // Work buffers of one sort of n particles: keys/values in and out, the
-// corrected phases, the radix sort's temporary storage and the six
+// corrected phases, the radix sort's digit counts and scan totals, and the six
 // destination coordinate arrays (swapped with the beam's after the sort).
 struct SortWork {
     int *keys, *keys_out, *vals, *vals_out;
@@ -16,8 +16,9 @@ struct SortWork {
     double *g, *th, *x, *y, *px, *py;
 };
 
-// temporary storage the radix sort needs for n keys in 0..ns (the storage
-// for the initial particle count serves every later, smaller sort)
+// storage the radix sort needs for n keys in 0..ns: 16 digit counts per
+// tile of 2048 keys and their scan's chunk totals (the storage for the
+// initial particle count serves every later, smaller sort)
 int sort_work_bytes(int ns, long long n, size_t *bytes);
 // Sort the beam (off[ns+1] on the device, n particles) into w's coordinate
 // arrays; the new offsets go to off_new[ns+1] (device).

@@ -153,6 +153,49 @@ def test_device_sort_matches_oracle():
     ctx.close()
 
 
+def _stable_order(sim, slen):
+    """The device sort's documented result: particles keyed by destination
+    slice (dropped ones last), each slice in source order, theta shifted by
+    slen atar; as arrays and offsets."""
+    ns = sim.ns
+    a = np.repeat(np.arange(ns), np.diff(sim.off))
+    th = sim.theta
+    s = slen * a + th
+    atar = np.floor(th * (1.0 / slen))
+    dest = a + atar
+    ok = ~((s < 0) | (s > slen * ns)) & (dest >= 0) & (dest < ns)
+    key = np.where(ok, dest, ns).astype(np.int64)
+    th_new = np.where(ok, th - slen * atar, th)
+    order = np.argsort(key, kind="stable")
+    keep = order[key[order] < ns]
+    out = {k: getattr(sim, k)[keep] for k in ("gamma", "x", "y", "px", "py")}
+    out["theta"] = th_new[keep]
+    off = np.concatenate([[0], np.cumsum(np.bincount(key[keep], minlength=ns))]).astype(np.int64)
+    return out, off
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("ns,sample", [(40, 9), (300, 1)])
+def test_device_sort_is_stable(ns, sample):
+    """The hand-written radix sort over many tiles of 2048 keys and two
+    (ns = 40) or three (ns = 300) 4-bit digit passes: every slice holds its
+    particles in source order, bit for bit as a stable sort by destination."""
+    print(f"one4one sort order: {ns} slices, scrambled phases", flush=True)
+    sim, sl, cnt = _oracle_beam(ns, sample=sample, ngrid=15)
+    slen = 4.0 * np.arcsin(1.0) * sim.s.slicelength / sim.s.reflength
+    _scramble(sim, np.random.default_rng(5), slen, spread=3.5)
+    ref, roff = _stable_order(sim, slen)
+    ctx = H.to_gpu(sim)
+    ctx.set_sorting(True)
+    n = ctx.sort()
+    off, _ = ctx.slices()
+    assert n == int(roff[-1]) and np.array_equal(off, roff)
+    b = ctx.download_beam()
+    for k in ref:
+        assert np.array_equal(b[k], ref[k]), k
+    ctx.close()
+
+
 @pytest.mark.gpu
 def test_one4one_sase_with_sort_markers():
     """A one4one SASE run (8 slices, 1.3-1.6 k electrons each, 31^2) with a
