@@ -101,6 +101,23 @@ def measured_traffic(workload, kernel):
     return None
 
 
+def measured_traffic_sum(workload, kernel):
+    """Sum of the HBM bytes per launch of every kernel whose name contains
+    `kernel` on `workload` (profiles/pmc_traffic.md), or None."""
+    path = os.path.join(ROOT, "profiles", "pmc_traffic.md")
+    if not os.path.exists(path):
+        return None
+    tot = 0.0
+    for line in open(path):
+        cols = [c.strip() for c in line.strip().strip("|").split("|")]
+        if len(cols) == 6 and cols[0] == workload and kernel in cols[1]:
+            try:
+                tot += float(cols[5])
+            except ValueError:
+                return None
+    return tot or None
+
+
 FP64_MFMA_PEAK_TFLOPS = 78.6
 
 
@@ -124,8 +141,10 @@ def roofline(cfg, kt, ntot, precision=g4mi.FP64, schedule=None):
         # the field read + written once = 32 B/cell, as the ADI
         ms = kt["adi"]
         gbs = 32.0 * cells / (ms * 1e-3) / 1e9
+        traffic = measured_traffic_sum(cfg["name"] + "-fft", "k_fft_conv")  # both passes of a step
         return "adi", dict(bound="hbm", achieved=round(gbs, 1), peak=HBM_PEAK_GBS, unit="GB/s",
-                           frac=round(gbs / HBM_PEAK_GBS, 4), traffic=None, kernel="field (fft)",
+                           frac=round(gbs / HBM_PEAK_GBS, 4), traffic=int(traffic) if traffic else None,
+                           kernel="field (fft)",
                            device_kernel="k_fft_conv", ms_per_launch=round(ms, 5),
                            algorithmic_bytes=int(32.0 * cells))
     if schedule and schedule.get("field_solver") == "fft" and top == "adi":
