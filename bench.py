@@ -459,7 +459,7 @@ def main():
     ap.add_argument("--steps", type=int, default=80,
                     help="timed steps; the default keeps them inside the first undulator module")
     ap.add_argument("--warmup", type=int, default=20,
-                    help="untimed steps first; the mixed form's schedule tuner takes the first 14 steps")
+                    help="untimed steps first; the mixed form's schedule tuner takes the first 18 steps")
     ap.add_argument("--config", choices=("config1", "config0", "config2", "config3", "config4"),
                     default="config3",
                     help="config3 (default) and config2 run the mixed form and, on the same beam, "

@@ -175,14 +175,14 @@ struct Ctx {
     bool m_graphs = true;                  // replay captured step graphs
     int m_adi_mode = kAdiWide;             // mixed ADI kernel (schedule "mixed_adi"): tile 0, strips 1, wide 2
     int m_main_nt = 384;                   // threads per slice of the main kernel ("main_nt")
-    int m_main_lds = 0;                    // its LDS region: 0 = 68 KiB, 1 = 44 KiB ("main_lds")
+    int m_main_lds = 0;                    // its LDS region: 0 = 68 KiB, 1 = 44 KiB, 2 = 32 KiB ("main_lds")
     bool m_persistent = false;             // the whole step as one cooperative launch ("persistent")
     bool m_autotune = true;                // time the schedules on the first steps of a run
     bool m_tuned = false;
     std::string m_tune_log;                // mean ms per candidate, for the schedule report
     int m_tune_i = 0;                      // tuning steps done
-    double m_tune_main[6] = {0, 0, 0, 0, 0, 0}, m_tune_adi[3] = {0, 0, 0}, m_tune_graph = -1;
-    int m_tune_nmain[6] = {0, 0, 0, 0, 0, 0}, m_tune_nadi[3] = {0, 0, 0};
+    double m_tune_main[8] = {0, 0, 0, 0, 0, 0, 0, 0}, m_tune_adi[3] = {0, 0, 0}, m_tune_graph = -1;
+    int m_tune_nmain[8] = {0, 0, 0, 0, 0, 0, 0, 0}, m_tune_nadi[3] = {0, 0, 0};
     hipEvent_t m_tune_ev[9] = {};
     std::map<std::tuple<int, int, int, int, int, int, int, int>, hipGraphExec_t> m_graph;
     std::vector<double> m_rec_host;
@@ -1662,11 +1662,16 @@ static int mix_step_fno(Ctx *c, const MixKey &k) {
 }
 
 // One step of the mixed-form schedule tuner (see mix_run): tuning step k
-// runs main candidate k % 6 with ADI candidate (k / 6 + k) % 3 (k < 12: the
+// runs main candidate k % 8 with ADI candidate (k / 8 + k) % 3 (k < 16: the
 // whole-width streaming kernel, the strips, the tiles), the graph of the
-// chosen kernels (k = 12) or the persistent launch (k = 13).
-constexpr int kMixTuneSteps = 14;
-static const int kMainCand[6][2] = {{384, 0}, {256, 0}, {512, 0}, {384, 1}, {256, 1}, {512, 1}};
+// chosen kernels (k = 16) or the persistent launch (k = 17).  The main
+// candidates: 6, 4 or 8 wavefronts per slice with 68 or 44 KiB of LDS, and
+// 4 or 6 wavefronts with 32 KiB (more workgroups per CU, a smaller
+// deposit tile).
+constexpr int kMixTuneSteps = 18;
+constexpr int kMainCands = 8;
+static const int kMainCand[kMainCands][2] = {{384, 0}, {256, 0}, {512, 0}, {384, 1},
+                                              {256, 1}, {512, 1}, {256, 2}, {384, 2}};
 
 static int mix_tune_step(Ctx *c, const MixKey &key) {
     hipEvent_t *tev = c->m_tune_ev;
@@ -1675,8 +1680,9 @@ static int mix_tune_step(Ctx *c, const MixKey &key) {
         for (int e = 0; e < 9; e++) G4_HIP(hipEventCreate(&tev[e]));
     MixKey kk = key;
     float ms = 0;
-    if (k < 12) {
-        const int mc = k % 6, ac = (k / 6 + k) % 3;   // every main candidate twice, each ADI four times
+    if (k < 2 * kMainCands) {
+        // every main candidate twice, each ADI kernel four to six times
+        const int mc = k % kMainCands, ac = (k / kMainCands + k) % 3;
         const int tm = std::get<0>(kk);
         const bool tile_ok = !c->m_refine || tm <= 9;
         const int mode = ac == 0 ? kAdiWide : ac == 1 ? kAdiStrips : (tile_ok ? kAdiTile : kAdiWide);
@@ -1693,9 +1699,9 @@ static int mix_tune_step(Ctx *c, const MixKey &key) {
         c->m_tune_nmain[mc]++;
         c->m_tune_adi[mode] += tm_adi;
         c->m_tune_nadi[mode]++;
-        if (k == 11) {
+        if (k == 2 * kMainCands - 1) {
             int best = 0;
-            for (int q = 1; q < 6; q++)
+            for (int q = 1; q < kMainCands; q++)
                 if (c->m_tune_nmain[q] &&
                     c->m_tune_main[q] / c->m_tune_nmain[q] < c->m_tune_main[best] / c->m_tune_nmain[best])
                     best = q;
@@ -1715,7 +1721,7 @@ static int mix_tune_step(Ctx *c, const MixKey &key) {
         std::get<5>(kk) = c->m_adi_mode;
         std::get<6>(kk) = c->m_main_nt;
         std::get<7>(kk) = c->m_main_lds;
-        if (k == 12) {
+        if (k == 2 * kMainCands) {
             hipGraphExec_t ex = nullptr;
             G4_TRY(mix_graph(c, kk, &ex));
             G4_HIP(hipEventRecord(tev[0], c->stream));
@@ -1743,11 +1749,13 @@ static int mix_tune_step(Ctx *c, const MixKey &key) {
             auto avg = [](double s, int n) { return n ? s / n : -1.0; };
             snprintf(buf, sizeof(buf),
                      "{\"main_ms\": {\"384/68K\": %.4f, \"256/68K\": %.4f, \"512/68K\": %.4f, "
-                     "\"384/44K\": %.4f, \"256/44K\": %.4f, \"512/44K\": %.4f}, "
+                     "\"384/44K\": %.4f, \"256/44K\": %.4f, \"512/44K\": %.4f, \"256/32K\": %.4f, "
+                     "\"384/32K\": %.4f}, "
                      "\"adi_ms\": {\"wide\": %.4f, \"strips\": %.4f, \"tile\": %.4f}, "
                      "\"step_ms\": {\"graph\": %.4f, \"persistent\": %.4f}}",
                      avg(mm[0], nm[0]), avg(mm[1], nm[1]), avg(mm[2], nm[2]), avg(mm[3], nm[3]),
-                     avg(mm[4], nm[4]), avg(mm[5], nm[5]), avg(c->m_tune_adi[kAdiWide], c->m_tune_nadi[kAdiWide]),
+                     avg(mm[4], nm[4]), avg(mm[5], nm[5]), avg(mm[6], nm[6]), avg(mm[7], nm[7]),
+                     avg(c->m_tune_adi[kAdiWide], c->m_tune_nadi[kAdiWide]),
                      avg(c->m_tune_adi[kAdiStrips], c->m_tune_nadi[kAdiStrips]),
                      avg(c->m_tune_adi[kAdiTile], c->m_tune_nadi[kAdiTile]), c->m_tune_graph, pers);
             c->m_tune_log = buf;
@@ -2869,7 +2877,7 @@ int g4mi_set_schedule(g4mi_ctx *h, const char *name, int value) {
         c->m_autotune = false;
         return G4MI_OK;
     }
-    if (std::string(name) == "main_lds" && (value == 0 || value == 1)) {
+    if (std::string(name) == "main_lds" && value >= 0 && value <= 2) {
         c->m_main_lds = value;
         c->m_autotune = false;
         return G4MI_OK;
@@ -2890,7 +2898,7 @@ int g4mi_set_schedule(g4mi_ctx *h, const char *name, int value) {
             for (auto &e : c->m_tune_ev) (void)hipEventDestroy(e);
         c->m_tuned = false;
         c->m_tune_i = 0;
-        for (int q = 0; q < 6; q++) c->m_tune_main[q] = 0, c->m_tune_nmain[q] = 0;
+        for (int q = 0; q < kMainCands; q++) c->m_tune_main[q] = 0, c->m_tune_nmain[q] = 0;
         for (int q = 0; q < 3; q++) c->m_tune_adi[q] = 0, c->m_tune_nadi[q] = 0;
         c->m_tune_graph = -1;
         return G4MI_OK;
@@ -2920,7 +2928,7 @@ int g4mi_get_schedule(g4mi_ctx *h, char *buf, int len) {
                  "\"tuning\": %s, \"fno_skipped_steps\": %lld}",
                  c->m_adi_mode == kAdiWide ? "band32-wide" : c->m_adi_mode == kAdiStrips ? "band32-stream" : "band32-tile",
                  c->m_refine ? 1 : 0, c->m_main_nt,
-                 c->m_main_lds ? 44 : 68, c->m_persistent ? 1 : 0, c->m_autotune ? 1 : 0, c->m_tuned ? 1 : 0, c->m_graphs ? 1 : 0,
+                 c->m_main_lds == 2 ? 32 : c->m_main_lds ? 44 : 68, c->m_persistent ? 1 : 0, c->m_autotune ? 1 : 0, c->m_tuned ? 1 : 0, c->m_graphs ? 1 : 0,
                  bands.c_str(), (int)c->m_graph.size(), c->m_tune_log.empty() ? "null" : c->m_tune_log.c_str(),
                  c->fno_skipped);
         snprintf(buf, (size_t)len, "%s", tmp);

@@ -355,6 +355,7 @@ __global__ void __launch_bounds__(64) k_mix_scan(MixArrays a) { scan_body(a); }
 constexpr int kScMax = 256;
 constexpr int kMixDyn = 8704;    // float2 cells of the large dynamic region (68 KiB)
 constexpr int kMixDynS = 5632;   // the small one (44 KiB)
+constexpr int kMixDynT = 4096;   // the smallest (32 KiB)
 
 struct OdeMix {
     float xks, eps_ref, bref, gref, ztemp1, ez;
@@ -2017,7 +2018,10 @@ void launch_main_t(hipStream_t s, const MixArrays &a, int sc) {
 
 template <bool EG>
 void launch_main(hipStream_t s, const MixArrays &a, int sc, int nt, int lds) {
-    if (lds == 1) {
+    if (lds == 2 && nt != 512) {  // eight wavefronts' histograms need more than 32 KiB
+        if (nt == 256) launch_main_t<EG, 256, kMixDynT>(s, a, sc);
+        else launch_main_t<EG, 384, kMixDynT>(s, a, sc);
+    } else if (lds >= 1) {
         if (nt == 256) launch_main_t<EG, 256, kMixDynS>(s, a, sc);
         else if (nt == 512) launch_main_t<EG, 512, kMixDynS>(s, a, sc);
         else launch_main_t<EG, 384, kMixDynS>(s, a, sc);
@@ -2089,6 +2093,8 @@ int mix_prepare() {
     int ok = 1;
 #define G4_M(EG, NT) ok &= set_main_attr<EG, NT, kMixDyn>(); ok &= set_main_attr<EG, NT, kMixDynS>();
     G4_M(true, 256) G4_M(true, 384) G4_M(true, 512) G4_M(false, 256) G4_M(false, 384) G4_M(false, 512)
+    ok &= set_main_attr<true, 256, kMixDynT>() && set_main_attr<true, 384, kMixDynT>() &&
+          set_main_attr<false, 256, kMixDynT>() && set_main_attr<false, 384, kMixDynT>();
 #undef G4_M
 #define G4_P(MM) ok &= set_adi_attr<MM, false>(); if (MixGeom<MM, true>::lds_bytes <= 160 * 1024) ok &= set_adi_attr<MM, true>(); \
     ok &= set_persistent_attr<MM, false>(); ok &= set_persistent_attr<MM, true>(); \

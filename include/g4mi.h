@@ -254,13 +254,14 @@ int g4mi_sort_index_noalloc(g4mi_ctx *ctx, double slen, int *atar, long long *mo
  * against FFT convolutions of every row and column (zero-padded
  * power-of-two FFTs, ngrid <= 256), and keeps the faster
  * ("fft_propagation": 0 GEMMs, 1 FFT convolution, -1 tunes again).
- * Mixed form: its first 14 steps time the main kernel's shapes (4, 6 or 8
- * wavefronts per slice x 68 or 44 KiB of LDS), the three ADI kernels,
+ * Mixed form: its first 18 steps time the main kernel's shapes (4, 6 or 8
+ * wavefronts per slice x 68 or 44 KiB of LDS, 4 or 6 x 32 KiB), the three ADI kernels,
  * and the step as a graph of kernels against one persistent
  * cooperative launch, and keep the fastest.  Entries: "adi_refine" (FP64
  * residual correction, default 1), "mixed_adi" (2 whole-width streaming,
  * 1 streaming strips, 0 tiles),
- * "main_nt" (256, 384, 512), "main_lds" (0: 68 KiB, 1: 44 KiB),
+ * "main_nt" (256, 384, 512), "main_lds" (0: 68 KiB, 1: 44 KiB, 2: 32 KiB;
+ * 512 threads take 44 KiB for 2),
  * "persistent", "graphs" (0/1); setting any of the kernel entries turns the
  * tuner off, "autotune" 1 turns it on again. */
 int g4mi_set_schedule(g4mi_ctx *ctx, const char *name, int value);

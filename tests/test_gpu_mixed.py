@@ -231,29 +231,59 @@ def test_mixed_persistent_step_matches_kernels(st):
 
 
 def test_mixed_autotuner_picks_a_schedule(st):
-    """The first run of 14 or more steps times every main-kernel shape (4, 6,
-    8 wavefronts per slice x 68 or 44 KiB of LDS), the three ADI kernels, the
+    """The first run of 18 or more steps times every main-kernel shape (4, 6,
+    8 wavefronts per slice x 68 or 44 KiB of LDS; 4, 6 x 32 KiB), the three ADI kernels, the
     graph of kernels and the persistent launch, and keeps the fastest; the
     run's records stay within 1 % of the FP64 oracle."""
-    _log("mixed autotuner: 20 steps, 12 slices x 1024, 63^2")
+    _log("mixed autotuner: 24 steps, 12 slices x 1024, 63^2")
     sim = H.make_state([1024] * 12, ngrid=63, gridmax=0.3e-3, longrange=True, sc=(1, 0, 100, 0.0))
     ctx = H.to_gpu(sim, precision=g4mi.FP32)
-    n = 20
+    n = 24
     g = ctx.run(H.gsteps(st[:n]), cases.GAMMA0, record=("power", "bunching"))
     sched = ctx.schedule()
     _log(f"  schedule {sched}")
     assert sched["tuned"] == 1
     tun = sched["tuning"]
-    assert len(tun["main_ms"]) == 6 and all(v > 0 for v in tun["main_ms"].values())
+    assert len(tun["main_ms"]) == 8 and all(v > 0 for v in tun["main_ms"].values())
     assert tun["adi_ms"]["wide"] > 0 and tun["adi_ms"]["strips"] > 0 and tun["step_ms"]["graph"] > 0
     assert sched["adi"] in ("band32-wide", "band32-stream", "band32-tile")
-    assert sched["main_nt"] in (256, 384, 512) and sched["main_lds_kib"] in (44, 68)
+    assert sched["main_nt"] in (256, 384, 512) and sched["main_lds_kib"] in (32, 44, 68)
     r = sim.run(st, cases.GAMMA0, record=("power", "bunching"), nsteps=n)
     pg, pr = g["power"][1:].sum(axis=1), r["power"][1:].sum(axis=1)
     e = float(np.max(np.abs(pg / pr - 1)))
     _log(f"  sum P(z) max rel err vs oracle {e:.3e}")
     assert e < 1e-2
     ctx.close()
+
+
+def test_mixed_main_kernel_shapes_agree(st):
+    """Every main-kernel shape the tuner chooses from (4, 6, 8 wavefronts per
+    slice x 68 or 44 KiB of LDS; 4, 6 x 32 KiB) on 6 slices x 4096 at 255^2
+    with space charge, 30 steps: the slices' deposit footprints (~45 x 45
+    cells) fit the 44 and 68 KiB tiles (48 x 48, 62 x 62) but not the 32 KiB
+    one (39 x 39), whose shapes take the banded two-pass deposit.  Records
+    and field agree to FP32 rounding (the LDS atomics add in varying order)."""
+    _log("mixed main-kernel shapes: 6 slices x 4096, 255^2, 30 steps, 8 shapes")
+    ref = None
+    for nt, lds in ((384, 0), (256, 0), (512, 0), (384, 1), (256, 1), (512, 1), (256, 2), (384, 2)):
+        sim = H.make_state([4096] * 6, ngrid=255, gridmax=0.5e-3, longrange=True, sc=(1, 0, 100, 0.0),
+                           seed_power=1e4)
+        ctx = H.to_gpu(sim, precision=g4mi.FP32)
+        ctx.set_schedule("main_nt", nt)
+        ctx.set_schedule("main_lds", lds)
+        r = ctx.run(H.gsteps(st[:30]), cases.GAMMA0, record=("power", "bunching"))
+        f, _ = ctx.download_field()
+        sched = ctx.schedule()
+        ctx.close()
+        assert sched["main_nt"] == nt and sched["main_lds_kib"] == (68, 44, 32)[lds]
+        if ref is None:
+            ref = (r, f)
+            continue
+        ep = float(np.max(np.abs(r["power"] - ref[0]["power"])) / np.max(np.abs(ref[0]["power"])))
+        eb = float(np.max(np.abs(r["bunching"] - ref[0]["bunching"])) / np.max(np.abs(ref[0]["bunching"])))
+        ef = float(np.max(np.abs(f - ref[1])) / np.max(np.abs(ref[1])))
+        _log(f"  {nt} threads, {sched['main_lds_kib']} KiB: power {ep:.2e}, bunching {eb:.2e}, field {ef:.2e}")
+        assert ep < 1e-4 and eb < 1e-4 and ef < 1e-4
 
 
 def test_mixed_upload_download_and_sort_index():
