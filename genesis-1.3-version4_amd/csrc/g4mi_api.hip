@@ -2928,7 +2928,7 @@ int g4mi_get_schedule(g4mi_ctx *h, char *buf, int len) {
                  "\"tuning\": %s, \"fno_skipped_steps\": %lld}",
                  c->m_adi_mode == kAdiWide ? "band32-wide" : c->m_adi_mode == kAdiStrips ? "band32-stream" : "band32-tile",
                  c->m_refine ? 1 : 0, c->m_main_nt,
-                 c->m_main_lds == 2 ? 32 : c->m_main_lds ? 44 : 68, c->m_persistent ? 1 : 0, c->m_autotune ? 1 : 0, c->m_tuned ? 1 : 0, c->m_graphs ? 1 : 0,
+                 (c->m_main_lds == 2 && c->m_main_nt != 512) ? 32 : c->m_main_lds ? 44 : 68, c->m_persistent ? 1 : 0, c->m_autotune ? 1 : 0, c->m_tuned ? 1 : 0, c->m_graphs ? 1 : 0,
                  bands.c_str(), (int)c->m_graph.size(), c->m_tune_log.empty() ? "null" : c->m_tune_log.c_str(),
                  c->fno_skipped);
         snprintf(buf, (size_t)len, "%s", tmp);
