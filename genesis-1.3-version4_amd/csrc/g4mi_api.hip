@@ -91,6 +91,8 @@ struct Ctx {
     int fft_prop = -1, fft_conv_L = 0;
     float fft_conv_ms = -1.f;
     double2 *d_fft_hk = nullptr, *d_fft_tw = nullptr;
+    // Bluestein DFT tables (forward, inverse): [L] kernel transform, then [ng] chirp
+    double2 *d_fft_bf = nullptr, *d_fft_bi = nullptr;
     double fft_xc = 1, fft_yc = 1, fft_sig = 1;
     bool fft_ready = false;
     double fft_delz = 0;
@@ -605,6 +607,9 @@ static int tune_fft(Ctx *c) {
     return G4MI_OK;
 }
 
+static int ensure_bluestein(Ctx *c);
+static ConvArgs bluestein_pass(const Ctx *c, int sign, const double2 *in, double2 *out);
+
 static int fft_field_step(Ctx *c, double delz) {
     const int ng = c->ng;
     const size_t nn = (size_t)ng * ng;
@@ -644,7 +649,20 @@ static int fft_field_step(Ctx *c, double delz) {
     g.box = c->d_box;
     g.scale = 1.0;
     g.variant = c->fft_variant;
-    if (c->fft_filter) {
+    if (c->fft_filter && c->fft_prop == 1) {  // the same four DFT passes by Bluestein
+        G4_TRY(ensure_bluestein(c));
+        ConvArgs a = bluestein_pass(c, -1, c->d_src, c->d_fft_u);
+        a.mask = 1;
+        a.box = c->d_box;
+        G4_TRY(launch_fft_conv_rows(c->stream, a));
+        a = bluestein_pass(c, -1, c->d_fft_u, c->d_fft_v);
+        a.ptw = c->d_fft_sig;
+        G4_TRY(launch_fft_conv_cols(c->stream, a));
+        G4_TRY(launch_fft_conv_rows(c->stream, bluestein_pass(c, +1, c->d_fft_v, c->d_fft_u)));
+        a = bluestein_pass(c, +1, c->d_fft_u, c->d_fft_v);
+        a.scale = 1.0 / (double)nn;
+        G4_TRY(launch_fft_conv_cols(c->stream, a));
+    } else if (c->fft_filter) {
         g.a = c->d_src;  // DFT of the source planes: U = S W^T, V = sigmoid (W U)
         g.mask_a = 1;
         g.mat = c->d_fft_w;
@@ -994,6 +1012,51 @@ static bool wants_spectral(const g4mi_diag_out *o) {
            o->g_ydivergence || o->g_xpointing || o->g_ypointing;
 }
 
+// The twiddles and the Bluestein tables of the DFTs by FFT convolution
+// (ngrid <= 256), computed once per context.
+static int ensure_bluestein(Ctx *c) {
+    const int L = fft_conv_length(c->ng);
+    if (!L) return G4MI_E_ARG;
+    if (c->d_fft_bf) return G4MI_OK;
+    if (!c->d_fft_tw) {
+        G4_TRY(dalloc(c, &c->d_fft_hk, 512));
+        G4_TRY(dalloc(c, &c->d_fft_tw, 512));
+        std::vector<double> hk(2 * (size_t)L), tw(2 * (size_t)L);
+        (void)fft_conv_tables(c->ng, c->dgrid, c->xks, 1.0, hk.data(), tw.data());  // twiddles only
+        G4_HIP(hipMemcpy(c->d_fft_tw, tw.data(), tw.size() * sizeof(double), hipMemcpyHostToDevice));
+    }
+    double2 *bf = nullptr, *bi = nullptr;
+    G4_TRY(dalloc(c, &bf, 512 + 256));
+    G4_TRY(dalloc(c, &bi, 512 + 256));
+    std::vector<double> hk(2 * (size_t)L), ch(2 * (size_t)c->ng);
+    for (int sign : {-1, +1}) {
+        (void)fft_chirp_tables(c->ng, sign, hk.data(), ch.data());
+        double2 *dst = sign < 0 ? bf : bi;
+        G4_HIP(hipMemcpy(dst, hk.data(), hk.size() * sizeof(double), hipMemcpyHostToDevice));
+        G4_HIP(hipMemcpy(dst + 512, ch.data(), ch.size() * sizeof(double), hipMemcpyHostToDevice));
+    }
+    c->d_fft_bf = bf;
+    c->d_fft_bi = bi;
+    return G4MI_OK;
+}
+
+// One DFT pass (rows or columns) of every slice by Bluestein: sign -1
+// forward, +1 inverse (unnormalised).
+static ConvArgs bluestein_pass(const Ctx *c, int sign, const double2 *in, double2 *out) {
+    ConvArgs a{};
+    const double2 *t = sign < 0 ? c->d_fft_bf : c->d_fft_bi;
+    a.in = in;
+    a.out = out;
+    a.hk = t;
+    a.pre = a.post = t + 512;
+    a.tw = c->d_fft_tw;
+    a.nrows = (long long)c->ns * c->ng;
+    a.ng = c->ng;
+    a.ns = c->ns;
+    a.L = fft_conv_length(c->ng);
+    return a;
+}
+
 // DiagField's spectral block (src/Core/Diagnostic.cpp:789-807, 839-846 and
 // the global values :952-963): F = W E W^T per slice as two GEMMs, moments
 // of |F|^2 per slice, normalised on the host.  `wide` is the FP64 field
@@ -1010,13 +1073,19 @@ static int spectral_diag(Ctx *c, const double2 *wide, int first, g4mi_diag_out *
     // that does not hold the input, the result goes to the other one
     double2 *inter = wide == c->d_fft_u ? c->d_fft_v : c->d_fft_u;
     double2 *res = wide == c->d_fft_u ? c->d_fft_u : c->d_fft_v;
-    g.a = wide;
-    g.mat = c->d_fft_w;
-    g.c = inter;
-    G4_TRY(launch_cgemm_rows(c->stream, g));
-    g.b = inter;
-    g.c = res;
-    G4_TRY(launch_cgemm_cols(c->stream, g));
+    if (fft_conv_length(ng) && c->fft_prop != 0) {  // DFTs by Bluestein unless the GEMMs are asked for
+        G4_TRY(ensure_bluestein(c));
+        G4_TRY(launch_fft_conv_rows(c->stream, bluestein_pass(c, -1, wide, inter)));
+        G4_TRY(launch_fft_conv_cols(c->stream, bluestein_pass(c, -1, inter, res)));
+    } else {
+        g.a = wide;
+        g.mat = c->d_fft_w;
+        g.c = inter;
+        G4_TRY(launch_cgemm_rows(c->stream, g));
+        g.b = inter;
+        g.c = res;
+        G4_TRY(launch_cgemm_cols(c->stream, g));
+    }
     launch_fft_moments(c->stream, ns, res, ng, c->d_fft_mom);
     std::vector<double> m(5 * (size_t)ns);
     G4_HIP(hipMemcpyAsync(m.data(), c->d_fft_mom, m.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));

@@ -46,13 +46,21 @@ int cgemm_tile(int ng);
 // workgroup): rows pass out = in M^T over all ns*ng rows; columns pass
 // out = M in per slice with the column epilogue of the GEMM (add, src in
 // box, ring).  hk/tw: the tables of fft_conv_tables, L entries each.
+// With pre/post chirps (fft_chirp_tables) a pass is a length-n DFT of every
+// line instead (Bluestein): each line is multiplied by pre before and by
+// post after the convolution.
 struct ConvArgs {
     const double2 *in;
     double2 *out;
     const double2 *hk, *tw;
     long long nrows;  // rows pass: ns * ng
     int ng, ns, L;
-    const double2 *add;  // columns pass epilogue, as CgemmArgs
+    const double2 *pre, *post;  // [ng] per-position factors, or null
+    int mask;                   // rows pass: rows are source planes, zero outside box[row / ng]
+    // columns pass epilogue, as CgemmArgs: out = scale * ptw * result + 2 add + 2 src (in box)
+    double scale = 1.0;
+    const double *ptw;
+    const double2 *add;
     const double2 *src;
     const int4 *box;
     int first, ring;

@@ -449,16 +449,31 @@ __global__ void __launch_bounds__(64 * ConvLines<COLS>::value) k_fft_conv(ConvAr
         for (int r = 0; r < NP; r++) v[r] = d[conv_swz(j + 64 * r)];
         conv_lds_barrier();
     } else {
+        int4 mb = make_int4(0, 0, ng - 1, ng - 1);
+        if (a.mask && g < a.nrows) mb = a.box[g / ng];
+        const int yrow = (int)(g % ng);
+        const bool rin = yrow >= mb.y && yrow <= mb.w;
 #pragma unroll
         for (int r = 0; r < NP; r++) {
             const int p = j + 64 * r;
-            v[r] = (g < a.nrows && p < ng) ? a.in[g * ng + p] : make_double2(0., 0.);
+            const bool ok = g < a.nrows && p < ng && rin && p >= mb.x && p <= mb.z;
+            v[r] = ok ? a.in[g * ng + p] : make_double2(0., 0.);
         }
+    }
+    if (a.pre) {
+#pragma unroll
+        for (int r = 0; r < NP; r++)
+            if (j + 64 * r < ng) v[r] = conv_cmul(v[r], a.pre[j + 64 * r]);
     }
     conv_fft<R, NST, false>(v, d, j, a.tw);
 #pragma unroll
     for (int r = 0; r < R; r++) v[r] = conv_cmul(v[r], a.hk[j + 64 * r]);
     conv_fft<R, NST, true>(v, d, j, a.tw);
+    if (a.post) {
+#pragma unroll
+        for (int r = 0; r < NP; r++)
+            if (j + 64 * r < ng) v[r] = conv_cmul(v[r], a.post[j + 64 * r]);
+    }
     if (!COLS) {
         if (g < a.nrows) {
 #pragma unroll
@@ -480,6 +495,15 @@ __global__ void __launch_bounds__(64 * ConvLines<COLS>::value) k_fft_conv(ConvAr
         if (x >= ng) continue;
         double2 u = buf[line * LP + conv_swz(y)];
         const long long o = (long long)y * ng + x;
+        if (a.scale != 1.0) {
+            u.x *= a.scale;
+            u.y *= a.scale;
+        }
+        if (a.ptw) {
+            const double f = a.ptw[o];
+            u.x *= f;
+            u.y *= f;
+        }
         if (a.add) {
             const double2 a2 = a.add[(long long)ii * nn + o];
             u.x += 2. * a2.x;

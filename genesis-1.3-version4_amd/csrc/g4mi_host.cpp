@@ -266,6 +266,43 @@ int fft_conv_tables(int n, double dgrid, double xks, double delz, double *hk, do
     return L;
 }
 
+int fft_chirp_tables(int n, int sign, double *hk, double *chirp) {
+    // Bluestein: sum_j x_j exp(sign 2 pi i j k / n) = w_k sum_j (x_j w_j) conj(w_(k-j)),
+    // w_m = exp(sign i pi m^2 / n), as jk = (j^2 + k^2 - (k - j)^2) / 2; the
+    // exponent m^2 is reduced mod 2n exactly
+    typedef std::complex<long double> cl;
+    const int L = fft_conv_length(n);
+    if (!L) return 0;
+    const long double pi = 4.0L * atanl(1.0L);
+    auto w = [&](long long m) {
+        const long long r = (m * m) % (2LL * n);
+        const long double a = (long double)sign * pi * (long double)r / (long double)n;
+        return cl(cosl(a), sinl(a));
+    };
+    for (int j = 0; j < n; j++) {
+        const cl c = w(j);
+        chirp[2 * j] = (double)c.real();
+        chirp[2 * j + 1] = (double)c.imag();
+    }
+    const long double twopi = 8.0L * atanl(1.0L);
+    std::vector<cl> tw(L);
+    for (int k = 0; k < L; k++) {
+        const long double a = twopi * (long double)k / (long double)L;
+        tw[k] = cl(cosl(a), -sinl(a));
+    }
+    for (int k = 0; k < L; k++) {  // (1/L) DFT_L of conj(w_d) placed at d mod L, |d| < n
+        cl acc(0, 0);
+        for (int d = -(n - 1); d < n; d++) {
+            const int dl = (d % L + L) % L;
+            acc += std::conj(w(d < 0 ? -d : d)) * tw[(int)(((long long)k * dl) % L)];
+        }
+        acc /= (long double)L;
+        hk[2 * k] = (double)acc.real();
+        hk[2 * k + 1] = (double)acc.imag();
+    }
+    return L;
+}
+
 void dft_matrix(int n, int sign, double *w) {
     const long double twopi = 8.0L * atanl(1.0L);
     for (int j = 0; j < n; j++)

@@ -49,6 +49,11 @@ void fft_propagator(int n, double dgrid, double xks, double delz, double *m);
 // entries each.  Returns L (0: not supported for this n).
 int fft_conv_length(int n);
 int fft_conv_tables(int n, double dgrid, double xks, double delz, double *hk, double *tw);
+// A length-n DFT with exp(sign 2 pi i j k / n) as the same convolution
+// (Bluestein): chirp[j] = exp(sign i pi j^2 / n) multiplies the line before
+// and after, hk = (1/L) DFT_L of conj(chirp) placed at d mod L for |d| < n.
+// n entries of chirp, L of hk (interleaved complex); returns L.
+int fft_chirp_tables(int n, int sign, double *hk, double *chirp);
 // W[j][l] = exp(sign 2 pi i j l / n), the exponent reduced mod n exactly.
 void dft_matrix(int n, int sign, double *w);
 // The source filter's sigmoid at FFT index [ky][kx] (:141-147), n*n reals.

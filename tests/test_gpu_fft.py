@@ -215,22 +215,27 @@ def _numpy_spectral(field, ng, first, lam, dgrid):
     return res
 
 
-@pytest.mark.parametrize("prec", [g4mi.FP64, g4mi.FP32])
-def test_spectral_diagnostics(prec):
+@pytest.mark.parametrize("prec,ng,dft", [(g4mi.FP64, 121, "bluestein"), (g4mi.FP32, 121, "bluestein"),
+                                         (g4mi.FP64, 255, "bluestein"), (g4mi.FP64, 121, "gemm")])
+def test_spectral_diagnostics(prec, ng, dft):
     """Divergence and pointing (Diagnostic.cpp:789-846, globals :952-963)
-    of three tilted Gaussian beams (w0 = 30 um on 121^2 cells of 5 um, ring
-    offset first = 1): against the same moments from numpy.fft (1e-12 in
+    of three tilted Gaussian beams (w0 = 30 um on 121^2 cells of 5 um or
+    255^2 of 2.4 um, ring offset first = 1), the 2-D DFT by Bluestein's
+    chirp convolution (zero-padded FFTs of length 256 or 512) or by the
+    dense DFT products: against the same moments from numpy.fft (1e-12 in
     the FP64 form, 1e-5 in the FP32 form) and against optics: the pointing
     is the tilt and the rms divergence lambda / (2 pi w0), to 1e-6."""
-    ng, gm, w0 = 121, 0.3e-3, 30e-6
+    gm, w0 = 0.3e-3, 30e-6
     lam = cases.LAMBDA_R
     sig = lam / (2 * np.pi * w0)
     tilts = [(3 * sig, -2 * sig, 1e7), (0.0, 0.0, 2e7), (-1.5 * sig, 4 * sig, 5e6)]
-    _log(f"spectral diagnostics ({'FP64' if prec == g4mi.FP64 else 'FP32'} form), 3 tilted beams, {ng}^2")
+    _log(f"spectral diagnostics ({'FP64' if prec == g4mi.FP64 else 'FP32'} form), 3 tilted beams, {ng}^2, {dft}")
     sim = H.make_state([64] * 3, ngrid=ng, gridmax=gm)
     sim.field[:] = _tilted_fields(ng, gm, lam, w0, tilts)
     sim.s.first = 1
     ctx = H.to_gpu(sim, precision=prec)
+    if dft == "gemm":
+        ctx.set_schedule("fft_propagation", 0)
     d = ctx.diag(spectral=True)
     ref = _numpy_spectral(sim.field, ng, 1, lam, 2 * gm / (ng - 1))
     tol = 1e-12 if prec == g4mi.FP64 else 1e-5
