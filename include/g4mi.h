@@ -253,7 +253,10 @@ int g4mi_sort_index_noalloc(g4mi_ctx *ctx, double slen, int *atar, long long *mo
  * 32, 3 three with 32; -1 tunes again) and its propagation by those GEMMs
  * against FFT convolutions of every row and column (zero-padded
  * power-of-two FFTs, ngrid <= 256), and keeps the faster
- * ("fft_propagation": 0 GEMMs, 1 FFT convolution, -1 tunes again).
+ * ("fft_propagation": 0 GEMMs, 1 FFT convolution, -1 tunes again).  The
+ * source filter (with the FFT convolution) and the spectral diagnostics
+ * (unless "fft_propagation" is 0) compute their DFTs as Bluestein chirp
+ * convolutions on the same FFTs for ngrid <= 256.
  * Mixed form: its first 18 steps time the main kernel's shapes (4, 6 or 8
  * wavefronts per slice x 68 or 44 KiB of LDS, 4 or 6 x 32 KiB), the three ADI kernels,
  * and the step as a graph of kernels against one persistent
