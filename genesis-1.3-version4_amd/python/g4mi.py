@@ -390,7 +390,7 @@ class Context:
 
     def diag(self, spectral=False):
         """One diagnostics record; spectral=True adds the divergence and
-        pointing block (two DFT products per slice)."""
+        pointing block (a 2-D DFT per slice)."""
         keys = [k for k in DIAG_FIELDS if spectral or k not in SPECTRAL_FIELDS]
         out = {k: np.zeros(1 if k in DIAG_SCALARS else self.ns) for k in keys}
         d = DiagOut(*[_p(out.get(k)) for k in DIAG_FIELDS])
