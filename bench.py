@@ -280,6 +280,39 @@ def fno_probe(ngrid=151, batch=16, reps=5):
     return 0
 
 
+def deck_example1(reps=2):
+    """Wall time of the reference's Example1 deck through the deck runner
+    (parsing, lattice matching, beam and field generation, 1104 steps with a
+    record after each including the spectral block, the .out.h5 file), the
+    whole-program figure the reference publishes for it
+    (examples/Example1-SteadyState/README.md: 29.5481 s on one MPI rank)."""
+    import tempfile
+    tmp = tempfile.mkdtemp(prefix="g4mi_ex1_")
+    lat = os.path.join(tmp, "Example1.lat")
+    with open(lat, "w") as f:
+        f.write(W.EXAMPLE1_LAT.format(extra=""))
+    path = os.path.join(tmp, "Example1.in")
+    with open(path, "w") as f:
+        f.write(W.EXAMPLE1_IN.format(root=os.path.join(tmp, "ex1"), lat=lat))
+    times = []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        g4mi.run_deck(path, device=0, quiet=True)
+        times.append(time.perf_counter() - t0)
+    out = os.path.join(tmp, "ex1.out.h5")
+    pend = None
+    try:
+        pend = float(g4mi.h5_read(out, "Field/power")[-1, 0])
+    except Exception:
+        pass
+    print(json.dumps(dict(example1_deck=dict(
+        seconds=round(min(times), 3), seconds_each=[round(t, 3) for t in times], p_end_w=pend,
+        output_h5=os.path.exists(out), reference_published_s=29.5481,
+        reference_source="examples/Example1-SteadyState/README.md:142,180 (whole program, one MPI rank)"))),
+        flush=True)
+    return 0
+
+
 def _run_steps(ctx, steps):
     """Advance ctx over `steps` without records."""
     if not steps:
@@ -485,6 +518,8 @@ def main():
                     help="only run a plain device copy of MIB MiB five times (the reference "
                          "point that scales the FETCH_SIZE/WRITE_SIZE counters) and exit")
     ap.add_argument("--fno-probe", action="store_true", help="only time the learned field step")
+    ap.add_argument("--deck-example1", action="store_true",
+                    help="only time the reference's Example1 deck through the deck runner (whole program)")
     ap.add_argument("--fno-pairs", type=int, default=200000)
     ap.add_argument("--fno-seeds", type=int, default=100)
     ap.add_argument("--fno-seed0", type=int, default=100)
@@ -503,6 +538,8 @@ def main():
         return copy_probe(a.copy_probe)
     if a.fno_probe:
         return fno_probe()
+    if a.deck_example1:
+        return deck_example1()
     if a.config == "config4":
         return config4(a)
     cfg = dict(W.CONFIGS[a.config])

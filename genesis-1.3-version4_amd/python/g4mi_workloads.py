@@ -52,6 +52,54 @@ def lattice_text(nund=4, nwig=125):
     return "\n".join(s) + "\n"
 
 
+# The reference's Example1 deck (examples/Example1-SteadyState/Example1.in
+# and Example1.lat) as data: steady state, helical aw = 0.84853, 255^2,
+# 8192 macroparticles (nbins 8), 1104 steps with a record after each.
+EXAMPLE1_IN = """&setup
+rootname={root}
+lattice={lat}
+beamline=FEL
+lambda0=1e-10
+gamma0=11357.82
+delz=0.045000
+shotnoise=0
+nbins = 8
+&end
+
+&lattice
+zmatch=9.5
+&end
+
+&field
+power=5e3
+dgrid=2.000000e-04
+ngrid=255
+waist_size=30e-6
+&end
+
+&beam
+current=3000
+delgam=1.000000
+ex=4.000000e-07
+ey=4.000000e-07
+&end
+
+&track
+&end
+"""
+
+EXAMPLE1_LAT = """D1: DRIFT = {{ l = 0.44}};
+D2: DRIFT = {{ l = 0.24}};
+QF: QUADRUPOLE = {{ l = 0.080000, k1= 2.000000 }};
+QD: QUADRUPOLE = {{ l = 0.080000, k1= -2.000000 }};
+UND: UNDULATOR = {{ lambdau=0.015000, nwig=266, aw=0.84853, helical= True{extra}}};
+
+FODO: LINE={{UND,D1,QF,D2,UND,D1,QD,D2}};
+
+FEL: LINE={{6*FODO}};
+"""
+
+
 def gaussian_current(ns, peak=4000.0, sigma_frac=0.25):
     """Gaussian current profile over ns slices (sigma = sigma_frac*ns)."""
     c = (ns - 1) / 2.0

@@ -12,51 +12,12 @@ import numpy as np
 import pytest
 
 import g4mi
+import g4mi_workloads as W
 import oracle_py as O
 
-EX1_IN = """&setup
-rootname={root}
-lattice={lat}
-beamline=FEL
-lambda0=1e-10
-gamma0=11357.82
-delz=0.045000
-shotnoise=0
-nbins = 8
-&end
-
-&lattice
-zmatch=9.5
-&end
-
-&field
-power=5e3
-dgrid=2.000000e-04
-ngrid=255
-waist_size=30e-6
-&end
-
-&beam
-current=3000
-delgam=1.000000
-ex=4.000000e-07
-ey=4.000000e-07
-&end
-
-&track
-&end
-"""
-
-EX1_LAT = """D1: DRIFT = {{ l = 0.44}};
-D2: DRIFT = {{ l = 0.24}};
-QF: QUADRUPOLE = {{ l = 0.080000, k1= 2.000000 }};
-QD: QUADRUPOLE = {{ l = 0.080000, k1= -2.000000 }};
-UND: UNDULATOR = {{ lambdau=0.015000, nwig=266, aw=0.84853, helical= True{extra}}};
-
-FODO: LINE={{UND,D1,QF,D2,UND,D1,QD,D2}};
-
-FEL: LINE={{6*FODO}};
-"""
+# the reference's Example1 deck, kept with the benchmark workloads
+EX1_IN = W.EXAMPLE1_IN
+EX1_LAT = W.EXAMPLE1_LAT
 
 
 def _write(tmp_path, deck=EX1_IN, lat_extra="", root="Example1", **fmt):
