@@ -66,7 +66,7 @@ struct Ctx {
     // ADI factor cache and the kernels eligible for it
     double adi_delz = 0, cstep = 0;
     int adi_tile = 0;  // 8 or 16: window of the fused recurrence kernel; 0: not eligible
-    int adi_band_m = 0;  // 7, 8 or 12: half width of the banded-inverse kernel; 0: not eligible
+    int adi_band_m = 0;  // 7, 8, 12 or 16: half width of the banded-inverse kernel; 0: not eligible
     int adi_klo = 0, adi_khi = -1;  // rows sharing the central band row
     double2 *d_band = nullptr;
     double *d_tilediag = nullptr;
@@ -303,7 +303,7 @@ static int ensure_factors(Ctx *c, double delz) {
     // banded inverse: accept the narrowest band whose dropped entries are
     // all below 2^-60 of the diagonal (1/128 ulp of the result)
     c->adi_band_m = 0;
-    for (int m : {7, 8, 12}) {
+    for (int m : {7, 8, 12, 16}) {
         std::vector<double> band(2 * (size_t)c->ng * (2 * m + 1));
         const double worst = adi_band(c->ng, h.data(), h.data() + 2 * c->ng,
                                       h.data() + 4 * c->ng, m, band.data());

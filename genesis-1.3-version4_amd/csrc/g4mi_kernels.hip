@@ -1655,6 +1655,7 @@ struct BandGeom {
     // the x band rows of edge tiles in a region of their own
     static constexpr size_t lds_cells = (size_t)RE * PE + (size_t)TS * W;
     static_assert(PG <= PE, "F must fit in E's rows");
+    static_assert(RE * (TS / CX) <= 256, "the x stage maps one lane per (row, CX columns): M <= 16");
     static_assert((size_t)RE * PG + (size_t)TS * W <= (size_t)RE * PE, "F and the y band must fit in E");
 };
 
@@ -2454,6 +2455,7 @@ int launch_adi_band(hipStream_t s, int ns, const double2 *fin, double2 *fout, co
     if (m == 7) G4_BAND(7);
     else if (m == 8) G4_BAND(8);
     else if (m == 12) G4_BAND(12);
+    else if (m == 16) G4_BAND(16);
     else return -1;
 #undef G4_BAND
     return 0;

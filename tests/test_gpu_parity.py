@@ -227,6 +227,32 @@ def test_adi_kernels_and_regimes(delz, kernel):
         assert sch["adi"] == "band-wide"
 
 
+@pytest.mark.parametrize("delz,m", [(0.35, 16), (0.8, 0)])
+def test_adi_band_tiles_wide_halfwidths(delz, m):
+    """Longer steps: rtmp = 0.07 on 151^2 needs a half width of 16 for the
+    banded inverse to drop only entries below 2^-60 of the diagonal, the
+    widest the 32x32-tile kernel takes (the parallel form for a single
+    steady-state slice, whose Thomas sweeps run 151 lines deep); rtmp = 0.16
+    would need 24, so the band is not eligible there and the request falls
+    back to the tuner's choice.  Both against the oracle's Thomas over 3
+    steps."""
+    _log(f"banded ADI tiles, half width {m}: delz={delz}, 151^2, 4 slices")
+    sim = H.make_state([1024] * 4, seed_power=5e7)
+    ctx = H.to_gpu(sim)
+    ctx.set_schedule("adi_kernel", 3)
+    st = O.Step(aw=cases.AW, ku=2 * np.pi / cases.LAMBDAU, ky=(2 * np.pi / cases.LAMBDAU) ** 2, dz=delz)
+    for _ in range(3):
+        O.lib().orc_field_advance(C.byref(sim.s), C.byref(st), delz)
+        ctx.field_track(g4mi.steps_from([st])[0], delz)
+    f, _ = ctx.download_field()
+    err = np.max(np.abs(f - sim.field)) / np.max(np.abs(sim.field))
+    sch = ctx.schedule()
+    _log(f"  field max diff / max |E| = {err:.3e}  schedule {sch}")
+    assert sch["adi_band_m"] == m and (sch["adi"] == "band") == (m > 0)
+    assert err < 1e-13
+    ctx.close()
+
+
 @pytest.mark.parametrize("ngrid", [151, 255, 63])
 def test_adi_wide_stream_matches_band_tiles(ngrid):
     """The whole-width banded ADI evaluates the products of the 32x32-tile
