@@ -135,8 +135,11 @@ struct Ctx {
     std::vector<PendingRecord> pend;
     // one4one beams: Beam::sort at sort markers, with its work buffers
     bool one4one = false;
-    // the step's particle passes as two fused kernels (schedule "fp64_fused")
-    bool fuse64 = true;
+    // the step's particle passes as two fused kernels (schedule "fp64_fused":
+    // 1 on, 0 off, -1 on when the window has at least kFuseMinSlices slices,
+    // since the fused kernels run one workgroup per slice while the separate
+    // ones spread a slice's particles over blocks of 256)
+    int fuse64 = -1;
     double *d_beamsum = nullptr;           // beam sums of the last fused step [8 ns]
     int4 *d_pbox = nullptr;                // in-grid footprint per slice after the first half step
     double *d_pmaxf = nullptr;             // per slice: largest sqrt(faw2) / gamma in the grid (deposit scale)
@@ -1237,8 +1240,10 @@ static int sort_beam(Ctx *c) {
 // field only, no energy kicks (incoherent SR, wakes) between the push and
 // R56, no sort between the beam and the field track, and every space-charge
 // mode solution within the main kernel's LDS.
+constexpr int kFuseMinSlices = 256;  // one workgroup per CU at least
+static bool fp64_fuse_on(const Ctx *c) { return c->fuse64 > 0 || (c->fuse64 < 0 && c->ns >= kFuseMinSlices); }
 static bool fp64_fusable(const Ctx *c, const g4mi_step_params &st) {
-    if (!c->fuse64 || !c->hf.empty() || c->d_wake) return false;
+    if (!fp64_fuse_on(c) || !c->hf.empty() || c->d_wake) return false;
     if (c->incoh && st.aw > 0 && (c->incoh_loss || c->incoh_spread)) return false;
     if ((st.marker & 4) && c->one4one) return false;
     const bool sc = (c->sc_nz > 0) && (c->sc_ng > 2);
@@ -2951,8 +2956,8 @@ int g4mi_set_schedule(g4mi_ctx *h, const char *name, int value) {
         c->m_autotune = false;
         return G4MI_OK;
     }
-    if (std::string(name) == "fp64_fused" && (value == 0 || value == 1)) {
-        c->fuse64 = value != 0;
+    if (std::string(name) == "fp64_fused" && value >= -1 && value <= 1) {
+        c->fuse64 = value;
         return G4MI_OK;
     }
     if (std::string(name) == "persistent" && (value == 0 || value == 1)) {
@@ -3010,7 +3015,7 @@ int g4mi_get_schedule(g4mi_ctx *h, char *buf, int len) {
                  "\"conv_length\": %d, \"conv_tuned_ms\": %.4f, \"fp64_fused\": %d}",
                  c->fft_filter, cgemm_tile(c->ng), c->fft_variant, c->fft_ms[0], c->fft_ms[1], c->fft_ms[2],
                  c->fft_ms[3], c->fft_prop == 1 ? "fft-conv" : (c->fft_prop == 0 ? "gemm" : "untuned"),
-                 fft_conv_length(c->ng), c->fft_conv_ms, c->fuse64 ? 1 : 0);
+                 fft_conv_length(c->ng), c->fft_conv_ms, fp64_fuse_on(c) ? 1 : 0);
         snprintf(buf, (size_t)len, "%s", tmp);
         return G4MI_OK;
     }
@@ -3027,7 +3032,7 @@ int g4mi_get_schedule(g4mi_ctx *h, char *buf, int len) {
              "\"fp64_fused\": %d}",
              c->adi_kernel, c->adi_kernel >= 0 ? names[c->adi_kernel] : "untuned", c->adi_tile,
              c->adi_band_m, c->adi_ms[0], c->adi_ms[1], c->adi_ms[2], c->adi_ms[3], c->adi_ms[4], bydz.c_str(),
-             c->fuse64 ? 1 : 0);
+             fp64_fuse_on(c) ? 1 : 0);
     snprintf(buf, (size_t)len, "%s", tmp);
     return G4MI_OK;
 }

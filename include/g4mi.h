@@ -245,9 +245,11 @@ int g4mi_sort_index_noalloc(g4mi_ctx *ctx, double slen, int *atar, long long *mo
  * fixes an entry instead ("adi_kernel": 0..4, -1 re-enables tuning; a
  * kernel not eligible for a step size falls back to the tuner's choice);
  * g4mi_get_schedule writes the current choice
- * and the measured timings as a JSON object into buf.  "fp64_fused" (0/1,
- * default 1): the step's particle passes as two fused kernels where the
- * step allows it.  The spectral solver's first step likewise times its
+ * and the measured timings as a JSON object into buf.  "fp64_fused" (1 on,
+ * 0 off, -1 the default: on for windows of 256 slices or more): the step's
+ * particle passes as two fused kernels (one workgroup per slice) where the
+ * step allows it; smaller windows spread each slice over blocks of 256
+ * particles in the separate kernels.  The spectral solver's first step likewise times its
  * complex-GEMM variants ("fft_variant": 0 four real products per complex
  * one and k chunks of 16, 1 three products, 2 four products with chunks of
  * 32, 3 three with 32; -1 tunes again) and its propagation by those GEMMs
