@@ -98,7 +98,7 @@ struct Ctx {
     double fft_delz = 0;
     double2 *d_fft_m = nullptr, *d_fft_w = nullptr, *d_fft_wc = nullptr, *d_fft_u = nullptr,
             *d_fft_v = nullptr;
-    double *d_fft_sig = nullptr, *d_fft_mom = nullptr;
+    double *d_fft_sig = nullptr, *d_fft_mom = nullptr, *d_fft_mompart = nullptr;
     // incoherent synchrotron radiation (Incoherent): the host's uniform
     // stream, one kick per beamlet staged through pinned memory
     Ran2 *incoh = nullptr;
@@ -1002,6 +1002,7 @@ static int ensure_dft(Ctx *c) {
     G4_TRY(dalloc(c, &c->d_fft_u, cells));
     G4_TRY(dalloc(c, &c->d_fft_v, cells));
     G4_TRY(dalloc(c, &c->d_fft_mom, 5 * (size_t)c->ns));
+    G4_TRY(dalloc(c, &c->d_fft_mompart, 5 * (size_t)c->ns * fft_moments_parts(c->ns)));
     std::vector<double> w(2 * nn), wc(2 * nn);
     dft_matrix(ng, -1, w.data());
     dft_matrix(ng, +1, wc.data());
@@ -1089,7 +1090,7 @@ static int spectral_diag(Ctx *c, const double2 *wide, int first, g4mi_diag_out *
         g.c = res;
         G4_TRY(launch_cgemm_cols(c->stream, g));
     }
-    launch_fft_moments(c->stream, ns, res, ng, c->d_fft_mom);
+    launch_fft_moments(c->stream, ns, res, ng, c->d_fft_mompart, c->d_fft_mom);
     std::vector<double> m(5 * (size_t)ns);
     G4_HIP(hipMemcpyAsync(m.data(), c->d_fft_mom, m.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     G4_HIP(hipStreamSynchronize(c->stream));

@@ -72,7 +72,12 @@ int launch_fft_conv_cols(hipStream_t s, const ConvArgs &a);
 // (DiagField::getValues, src/Core/Diagnostic.cpp:789-807): out[5 s + 0..4] =
 // sum w, sum dx w, sum dx^2 w, sum dy w, sum dy^2 w, with w = |F|^2 and
 // (dx, dy) the centred offsets of the grid point whose FFT index it is.
-void launch_fft_moments(hipStream_t s, int ns, const double2 *F, int ng, double *out);
+// part: fft_moments_parts(ns) * 5 ns doubles of scratch (at most
+// kFftMomentParts blocks per slice, so that a window of few slices still
+// fills the device).
+constexpr int kFftMomentParts = 64;
+int fft_moments_parts(int ns);
+void launch_fft_moments(hipStream_t s, int ns, const double2 *F, int ng, double *part, double *out);
 // FP32 field -> FP64 copy (n complex values)
 void launch_widen_field(hipStream_t s, const float2 *in, double2 *out, long long n);
 

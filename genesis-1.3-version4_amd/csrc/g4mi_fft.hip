@@ -238,14 +238,18 @@ static int launch_cgemm(hipStream_t st, const CgemmArgs &g) {
     return G4MI_OK;
 }
 
-__global__ void __launch_bounds__(256) k_fft_moments(const double2 *__restrict__ F, int ng,
-                                                     double *__restrict__ out) {
+// Partial moments of |F|^2 over the FFT index: block (p, s) of a slice's
+// np blocks covers cells [p P, (p + 1) P), P = ceil(ng^2 / np), so that a
+// window of few slices still spreads over many workgroups.
+__global__ void __launch_bounds__(256) k_fft_moments(const double2 *F, int ng, int np, double *part) {
     __shared__ double red[5][256];
-    const int s = blockIdx.x, t = threadIdx.x;
+    const int s = blockIdx.y, p = blockIdx.x, t = threadIdx.x;
     const double2 *f = F + (long long)s * ng * ng;
     const double shift = -0.5 * (double)(ng - 1);
+    const int nn = ng * ng, chunk = (nn + np - 1) / np;
+    const int k0 = p * chunk, k1 = min(nn, k0 + chunk);
     double acc[5] = {0, 0, 0, 0, 0};
-    for (int k = t; k < ng * ng; k += 256) {
+    for (int k = k0 + t; k < k1; k += 256) {
         const int ky = k / ng, kx = k % ng;
         // grid point of FFT index k: (k - (n+1)/2) mod n
         const double dx = (double)((kx - (ng + 1) / 2 + ng) % ng) + shift;
@@ -265,11 +269,26 @@ __global__ void __launch_bounds__(256) k_fft_moments(const double2 *__restrict__
             for (int q = 0; q < 5; q++) red[q][t] += red[q][t + h];
         __syncthreads();
     }
-    if (t < 5) out[5 * s + t] = red[t][0];
+    if (t < 5) part[((long long)s * np + p) * 5 + t] = red[t][0];
 }
 
-void launch_fft_moments(hipStream_t s, int ns, const double2 *F, int ng, double *out) {
-    if (ns > 0) hipLaunchKernelGGL(k_fft_moments, dim3(ns), dim3(256), 0, s, F, ng, out);
+// out[5 s + q] = the slice's partial sums in block order
+__global__ void k_fft_moments_fold(const double *part, int ns, int np, double *out) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= 5 * ns) return;
+    const int s = i / 5, q = i % 5;
+    double v = 0;
+    for (int p = 0; p < np; p++) v += part[((long long)s * np + p) * 5 + q];
+    out[i] = v;
+}
+
+int fft_moments_parts(int ns) { return std::max(1, std::min(kFftMomentParts, (512 + ns - 1) / ns)); }
+
+void launch_fft_moments(hipStream_t s, int ns, const double2 *F, int ng, double *part, double *out) {
+    if (ns <= 0) return;
+    const int np = fft_moments_parts(ns);
+    hipLaunchKernelGGL(k_fft_moments, dim3(np, ns), dim3(256), 0, s, F, ng, np, part);
+    hipLaunchKernelGGL(k_fft_moments_fold, dim3((5 * ns + 255) / 256), dim3(256), 0, s, part, ns, np, out);
 }
 
 __global__ void k_widen_field(const float2 *__restrict__ in, double2 *__restrict__ out, long long n) {
