@@ -128,6 +128,7 @@ struct Ctx {
         int first;
         bool fresh;
         std::vector<int> zeroed;
+        std::vector<double> wake;  // Beam::eloss at the record (auxiliary block)
     };
     double *h_recs = nullptr;
     double *d_recs = nullptr;  // h_recs' device address
@@ -932,8 +933,13 @@ static void finish_diag(const Ctx *c, const std::vector<double> &hb, const std::
 
 // pinned record slots (at most 32 MiB, 4..256 records), allocated with the
 // context so no run pays for the allocation
+// A deferred record's slot: beam and field sums (16 per slice), the
+// auxiliary sums (16), the long- and short-range space-charge fields (1 + 1)
+// and the spectral moments (5).
+constexpr int kRecRow = 39;
+
 static int alloc_records(Ctx *c) {
-    const size_t row = 16 * (size_t)c->ns;
+    const size_t row = kRecRow * (size_t)c->ns;
     c->rec_cap = (int)std::max<size_t>(4, std::min<size_t>(256, ((size_t)32 << 20) / (row * sizeof(double))));
     G4_HIP(hipHostMalloc((void **)&c->h_recs, (size_t)c->rec_cap * row * sizeof(double)));
     G4_HIP(hipHostGetDevicePointer((void **)&c->d_recs, c->h_recs, 0));
@@ -949,8 +955,8 @@ static bool wants_aux(const g4mi_diag_out *o) {
 // The auxiliary outputs (DiagBeam :415-445, :505-519; DiagField :855-858)
 // from aux[16 ns] of launch_diag_aux / launch_mix_diag_aux and the per-slice
 // long-range (esc) and short-range (ssc) space-charge fields.
-static void finish_aux(const Ctx *c, const std::vector<double> &aux, const std::vector<double> &esc,
-                       const std::vector<double> &ssc, g4mi_diag_out *o) {
+static void finish_aux(const Ctx *c, const double *aux, const double *esc, const double *ssc,
+                       const std::vector<double> &wake, g4mi_diag_out *o) {
     double *dst[10] = {o->xmin, o->xmax, o->pxmin, o->pxmax, o->ymin, o->ymax, o->pymin, o->pymax, o->emin, o->emax};
     for (int is = 0; is < c->ns; is++) {
         const long long n = c->off[is + 1] - c->off[is];
@@ -962,7 +968,7 @@ static void finish_aux(const Ctx *c, const std::vector<double> &aux, const std::
             if (dst[k]) dst[k][is] = a[2 + k];
         if (o->nf_phase) o->nf_phase[is] = atan2(a[13], a[12]);
         // Beam::eloss: the wake loss of the last step that applied it
-        const double eloss = is < (int)c->wake_applied.size() ? c->wake_applied[is] : 0.0;
+        const double eloss = is < (int)wake.size() ? wake[is] : 0.0;
         if (o->efield) o->efield[is] = eloss + esc[is];
         if (o->wakefield) o->wakefield[is] = eloss;
         if (o->lscfield) o->lscfield[is] = esc[is];
@@ -972,8 +978,8 @@ static void finish_aux(const Ctx *c, const std::vector<double> &aux, const std::
 
 static MixArrays mix_arrays(const Ctx *c);
 
-// aux sums plus the space-charge fields, copied to the host
-static int fetch_aux(Ctx *c, std::vector<double> &aux, std::vector<double> &esc, std::vector<double> &ssc) {
+// the aux sums of this step into d_aux (device)
+static int launch_aux(Ctx *c) {
     const int ns = c->ns;
     if (!c->d_aux) G4_TRY(dalloc(c, &c->d_aux, 16 * (size_t)ns));
     if (c->precision == G4MI_FP32)
@@ -981,6 +987,13 @@ static int fetch_aux(Ctx *c, std::vector<double> &aux, std::vector<double> &esc,
     else
         launch_diag_aux(c->stream, ns, c->d_off, c->d_g, c->d_x, c->d_y, c->d_px, c->d_py, c->d_field, c->ng,
                         c->first, c->d_aux);
+    return G4MI_OK;
+}
+
+// aux sums plus the space-charge fields, copied to the host
+static int fetch_aux(Ctx *c, std::vector<double> &aux, std::vector<double> &esc, std::vector<double> &ssc) {
+    const int ns = c->ns;
+    G4_TRY(launch_aux(c));
     aux.resize(16 * (size_t)ns);
     esc.resize(ns);
     ssc.resize(ns);
@@ -1061,18 +1074,10 @@ static ConvArgs bluestein_pass(const Ctx *c, int sign, const double2 *in, double
     return a;
 }
 
-// DiagField's spectral block (src/Core/Diagnostic.cpp:789-807, 839-846 and
-// the global values :952-963): F = W E W^T per slice as two GEMMs, moments
-// of |F|^2 per slice, normalised on the host.  `wide` is the FP64 field
-// ([ns][ng][ng], storage order), `first` the ring offset.
-static int spectral_diag(Ctx *c, const double2 *wide, int first, g4mi_diag_out *o) {
+// The moments of |DFT(E)|^2 per slice into d_fft_mom (device, 5 ns).
+static int spectral_moments(Ctx *c, const double2 *wide) {
     G4_TRY(ensure_dft(c));
     const int ns = c->ns, ng = c->ng;
-    CgemmArgs g{};
-    g.ng = ng;
-    g.ns = ns;
-    g.nrows = (long long)ns * ng;
-    g.scale = 1.0;
     // the products cannot run in place: the intermediate is the work buffer
     // that does not hold the input, the result goes to the other one
     double2 *inter = wide == c->d_fft_u ? c->d_fft_v : c->d_fft_u;
@@ -1082,6 +1087,11 @@ static int spectral_diag(Ctx *c, const double2 *wide, int first, g4mi_diag_out *
         G4_TRY(launch_fft_conv_rows(c->stream, bluestein_pass(c, -1, wide, inter)));
         G4_TRY(launch_fft_conv_cols(c->stream, bluestein_pass(c, -1, inter, res)));
     } else {
+        CgemmArgs g{};
+        g.ng = ng;
+        g.ns = ns;
+        g.nrows = (long long)ns * ng;
+        g.scale = 1.0;
         g.a = wide;
         g.mat = c->d_fft_w;
         g.c = inter;
@@ -1091,9 +1101,13 @@ static int spectral_diag(Ctx *c, const double2 *wide, int first, g4mi_diag_out *
         G4_TRY(launch_cgemm_cols(c->stream, g));
     }
     launch_fft_moments(c->stream, ns, res, ng, c->d_fft_mompart, c->d_fft_mom);
-    std::vector<double> m(5 * (size_t)ns);
-    G4_HIP(hipMemcpyAsync(m.data(), c->d_fft_mom, m.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
-    G4_HIP(hipStreamSynchronize(c->stream));
+    return G4MI_OK;
+}
+
+// DiagField's spectral outputs from the moments m[5 ns] (storage order),
+// normalised on the host.
+static void finish_spectral(const Ctx *c, const double *m, int first, g4mi_diag_out *o) {
+    const int ns = c->ns, ng = c->ng;
     const double scltheta = c->xlambda / ng / c->dgrid;
     double f[5] = {0, 0, 0, 0, 0};
     for (int k = 0; k < ns; k++) {  // storage slice k is output slice (ns + k - first) mod ns
@@ -1116,6 +1130,18 @@ static int spectral_diag(Ctx *c, const double2 *wide, int first, g4mi_diag_out *
     if (o->g_ydivergence) o->g_ydivergence[0] = sqrt(fabs(f[4] - f[3] * f[3])) * scltheta;
     if (o->g_xpointing) o->g_xpointing[0] = f[1] * scltheta;
     if (o->g_ypointing) o->g_ypointing[0] = f[3] * scltheta;
+}
+
+// DiagField's spectral block (src/Core/Diagnostic.cpp:789-807, 839-846 and
+// the global values :952-963): F = W E W^T per slice, moments of |F|^2 per
+// slice, normalised on the host.  `wide` is the FP64 field ([ns][ng][ng],
+// storage order), `first` the ring offset.
+static int spectral_diag(Ctx *c, const double2 *wide, int first, g4mi_diag_out *o) {
+    G4_TRY(spectral_moments(c, wide));
+    std::vector<double> m(5 * (size_t)c->ns);
+    G4_HIP(hipMemcpyAsync(m.data(), c->d_fft_mom, m.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    G4_HIP(hipStreamSynchronize(c->stream));
+    finish_spectral(c, m.data(), first, o);
     return G4MI_OK;
 }
 
@@ -1135,7 +1161,7 @@ static int diag(Ctx *c, g4mi_diag_out *o) {
     if (wants_aux(o)) {
         std::vector<double> aux, esc, ssc;
         G4_TRY(fetch_aux(c, aux, esc, ssc));
-        finish_aux(c, aux, esc, ssc, o);
+        finish_aux(c, aux.data(), esc.data(), ssc.data(), c->wake_applied, o);
     }
     if (wants_spectral(o)) G4_TRY(spectral_diag(c, c->d_field, c->first, o));
     return G4MI_OK;
@@ -1970,7 +1996,7 @@ static int mix_diag(Ctx *c, g4mi_diag_out *o) {
     if (wants_aux(o)) {
         std::vector<double> aux, esc, ssc;
         G4_TRY(fetch_aux(c, aux, esc, ssc));
-        finish_aux(c, aux, esc, ssc, o);
+        finish_aux(c, aux.data(), esc.data(), ssc.data(), c->wake_applied, o);
     }
     if (wants_spectral(o)) {  // on an FP64 copy of the FP32 field
         G4_TRY(ensure_dft(c));
@@ -2778,36 +2804,40 @@ int g4mi_step(g4mi_ctx *h, const g4mi_step_params *st, double gammaref) {
 }
 
 // Deferred records of the FP64 run loop: each record's beam and field sums
-// are written into a pinned slot by a copy kernel on the step's stream, and
-// the host finishes them
-// when the slots are full or the run ends, so a record after every step
-// does not stall the device between steps.  Records that ask for the
-// auxiliary or spectral blocks, and one4one runs (whose slices change at
-// sorts), take the synchronous path.
+// (and, when asked for, the auxiliary sums with the space-charge fields and
+// the spectral moments) are written into a pinned slot by copy kernels on
+// the step's stream, and the host finishes them when the slots are full or
+// the run ends, so a record after every step does not stall the device
+// between steps.  one4one runs (whose slices change at sorts) take the
+// synchronous path.
 static bool record_deferrable(const Ctx *c, const g4mi_diag_out *o) {
-    return !wants_aux(o) && !wants_spectral(o) && !c->one4one;
+    (void)o;
+    return !c->one4one;
 }
 
 static int flush_records(Ctx *c) {
     if (c->pend.empty()) return G4MI_OK;
     G4_HIP(hipStreamSynchronize(c->stream));
-    const size_t ns = (size_t)c->ns, row = 16 * ns;
+    const size_t ns = (size_t)c->ns, row = kRecRow * ns;
     std::vector<double> hb(8 * ns), hf(8 * ns);
     for (size_t k = 0; k < c->pend.size(); k++) {
         Ctx::PendingRecord &r = c->pend[k];
-        std::copy(c->h_recs + k * row, c->h_recs + k * row + 8 * ns, hb.begin());
-        std::copy(c->h_recs + k * row + 8 * ns, c->h_recs + (k + 1) * row, hf.begin());
+        const double *slot = c->h_recs + k * row;
+        std::copy(slot, slot + 8 * ns, hb.begin());
+        std::copy(slot + 8 * ns, slot + 16 * ns, hf.begin());
         if (r.fresh)  // slices zeroed by slippage after the ADI's sums
             for (int z : r.zeroed)
                 for (int q = 0; q < 8; q++) hf[8 * (size_t)z + q] = 0;
         finish_diag(c, hb, hf, r.first, &r.out);
+        if (wants_aux(&r.out)) finish_aux(c, slot + 16 * ns, slot + 32 * ns, slot + 33 * ns, r.wake, &r.out);
+        if (wants_spectral(&r.out)) finish_spectral(c, slot + 34 * ns, r.first, &r.out);
     }
     c->pend.clear();
     return G4MI_OK;
 }
 
 static int enqueue_record(Ctx *c, const g4mi_diag_out *o) {
-    const size_t ns = (size_t)c->ns, row = 16 * ns;
+    const size_t ns = (size_t)c->ns, row = kRecRow * ns;
     if (!c->h_recs) G4_TRY(alloc_records(c));
     if ((int)c->pend.size() == c->rec_cap) G4_TRY(flush_records(c));
     // the beam sums go straight into the slot; the field sums of the ADI
@@ -2819,8 +2849,20 @@ static int enqueue_record(Ctx *c, const g4mi_diag_out *o) {
         launch_diag_beam(c->stream, c->ns, c->d_off, c->d_g, c->d_th, c->d_x, c->d_y, slot);
     if (!c->field_diag_fresh) launch_diag_field(c->stream, c->ns, c->d_field, c->ng, c->d_diag_f);
     launch_copy_out(c->stream, c->d_diag_f, slot + 8 * ns, (long long)(8 * ns));
+    std::vector<double> wake;
+    if (wants_aux(o)) {
+        G4_TRY(launch_aux(c));
+        launch_copy_out(c->stream, c->d_aux, slot + 16 * ns, (long long)(16 * ns));
+        launch_copy_out(c->stream, c->d_esc, slot + 32 * ns, (long long)ns);
+        launch_copy_out(c->stream, c->d_efield, slot + 33 * ns, (long long)ns);
+        wake = c->wake_applied;
+    }
+    if (wants_spectral(o)) {
+        G4_TRY(spectral_moments(c, c->d_field));
+        launch_copy_out(c->stream, c->d_fft_mom, slot + 34 * ns, (long long)(5 * ns));
+    }
     G4_HIP(hipGetLastError());
-    c->pend.push_back({*o, c->first, c->field_diag_fresh, c->zeroed_since_adi});
+    c->pend.push_back({*o, c->first, c->field_diag_fresh, c->zeroed_since_adi, std::move(wake)});
     return G4MI_OK;
 }
 

@@ -1691,7 +1691,7 @@ bool do_track(Deck &dk, Namelist &nl, int device, bool quiet, bool dry, std::str
                                 "g_nf_intensity", "g_ff_intensity"})
             rec[pre + key].assign(nrec, 0.0);
     }
-    auto record = [&](size_t r) {
+    auto make_diag = [&](size_t r) {
         g4mi_diag_out d{};
         auto ps = [&](const char *k) { return rec[k].data() + r * ns; };
         auto p1 = [&](const char *k) { return rec[k].data() + r; };
@@ -1753,6 +1753,10 @@ bool do_track(Deck &dk, Namelist &nl, int device, bool quiet, bool dry, std::str
         d.nf_intensity = rec["nf_intensity"].data() + r * ns;
         d.ff_intensity = rec["ff_intensity"].data() + r * ns;
         d.global_energy = rec["global_energy"].data() + r;
+        return d;
+    };
+    auto record = [&](size_t r) {
+        g4mi_diag_out d = make_diag(r);
         int rc = g4mi_diag(dk.ctx, &d);
         if (rc == G4MI_OK && !bharm.empty()) {
             hbun.resize((size_t)(bunchharm - 1) * ns);
@@ -1816,9 +1820,32 @@ bool do_track(Deck &dk, Namelist &nl, int device, bool quiet, bool dry, std::str
         }
         return true;
     };
+    // FP64 runs whose records need nothing beyond g4mi_diag's blocks hand a
+    // stretch of steps that each end in a record to one g4mi_run call, so the
+    // records are deferred on the device instead of taken one by one
+    const bool batch = precision == G4MI_FP64 && bharm.empty() && dk.hfields.empty() && !dk.setup.one4one;
+    std::vector<g4mi_diag_out> dbatch;
     size_t done = 0, r = 0;
     for (size_t p = 0; p <= nrun; p++) {
         const bool out_here = r < outs.size() && outs[r] == p;
+        if (batch && out_here && p == done + 1 && !dflag[p]) {
+            // the longest stretch p, p+1, ... of record steps without dumps
+            size_t q = p;
+            dbatch.clear();
+            while (q <= nrun && r + dbatch.size() < outs.size() && outs[r + dbatch.size()] == q && !dflag[q]) {
+                dbatch.push_back(make_diag(r + dbatch.size() + 1));
+                q++;
+            }
+            const int n = (int)dbatch.size();
+            if (g4mi_run(dk.ctx, steps.data() + done, n, st.gamma0, dbatch.data()) != n) {
+                err = g4mi_last_error();
+                return false;
+            }
+            done += (size_t)n;
+            r += (size_t)n;
+            p = done;
+            continue;
+        }
         if (!out_here && !dflag[p]) continue;
         if (p > done) {
             if (g4mi_run(dk.ctx, steps.data() + done, (int)(p - done), st.gamma0, nullptr) < 0) {
