@@ -166,6 +166,7 @@ struct Ctx {
     int4 *m_pbox = nullptr;                // [ns] footprint after the first half step
     float2 *m_field[2] = {nullptr, nullptr}, *m_src = nullptr;
     double *m_beamsum = nullptr, *m_egsum = nullptr, *m_tilediag = nullptr, *m_rec = nullptr;
+    double *m_mom_next = nullptr;          // [ns] second slice-size buffer (steps alternate with d_fsize)
     int m_nrec = 0;
     MixStep *m_steps = nullptr;            // device step table
     int m_steps_cap = 0;
@@ -183,6 +184,7 @@ struct Ctx {
     int m_main_nt = 384;                   // threads per slice of the main kernel ("main_nt")
     int m_main_lds = 0;                    // its LDS region: 0 = 68 KiB, 1 = 44 KiB, 2 = 32 KiB ("main_lds")
     bool m_persistent = false;             // the whole step as one cooperative launch ("persistent")
+    bool m_fuse_front = true;              // main kernels run the next step's front ("mixed_fuse_front")
     bool m_autotune = true;                // time the schedules on the first steps of a run
     bool m_tuned = false;
     std::string m_tune_log;                // mean ms per candidate, for the schedule report
@@ -190,7 +192,7 @@ struct Ctx {
     double m_tune_main[8] = {0, 0, 0, 0, 0, 0, 0, 0}, m_tune_adi[3] = {0, 0, 0}, m_tune_graph = -1;
     int m_tune_nmain[8] = {0, 0, 0, 0, 0, 0, 0, 0}, m_tune_nadi[3] = {0, 0, 0};
     hipEvent_t m_tune_ev[9] = {};
-    std::map<std::tuple<int, int, int, int, int, int, int, int>, hipGraphExec_t> m_graph;
+    std::map<std::tuple<int, int, int, int, int, int, int, int, int>, hipGraphExec_t> m_graph;
     std::vector<double> m_rec_host;
     std::vector<double> h_cur;             // currents [ns] (host copy)
     double *d_aux = nullptr;               // auxiliary diagnostics [16 ns]
@@ -1402,7 +1404,8 @@ static MixArrays mix_arrays(const Ctx *c) {
     a.src = c->m_src;
     a.box = c->d_box;
     a.pbox = c->m_pbox;
-    a.mom_fsize = c->d_fsize;
+    a.mom_fsize[0] = c->d_fsize;
+    a.mom_fsize[1] = c->m_mom_next;
     a.cen = c->d_cen;
     a.rbound = c->d_rbound;
     a.dr = c->d_dr;
@@ -1561,14 +1564,15 @@ static int mix_fill_step(Ctx *c, const g4mi_step_params &st, double gamma0, MixS
     return G4MI_OK;
 }
 
-// band, refine, sc, moments, maxzero, ADI kernel (kAdi*), main threads per slice, main LDS region
-typedef std::tuple<int, int, int, int, int, int, int, int> MixKey;
+// band, refine, sc, moments, maxzero, ADI kernel (kAdi*), main threads per slice, main LDS region,
+// front kernel launched (0 when the previous step's main kernel ran the front)
+typedef std::tuple<int, int, int, int, int, int, int, int, int> MixKey;
 
 static int mix_launch(Ctx *c, const MixKey &k, hipEvent_t *ev) {
     const MixArrays a = mix_arrays(c);
     if (launch_mix_step(c->stream, a, std::get<0>(k), std::get<1>(k) != 0, std::get<2>(k) != 0,
                         std::get<3>(k) != 0, std::get<4>(k), ev, std::get<5>(k), std::get<6>(k),
-                        std::get<7>(k)) != 0) {
+                        std::get<7>(k), std::get<8>(k) != 0) != 0) {
         set_error("mixed form: no ADI instantiation for this band width");
         return G4MI_E_ARG;
     }
@@ -1627,7 +1631,8 @@ static int mix_step_fno(Ctx *c, const MixKey &k) {
     int ntiles = 0;
     MixState hs{};
     if (c->fno_mode == 1) {
-        launch_mix_beam(c->stream, a, sc, moments, nullptr, false, std::get<6>(k), std::get<7>(k));
+        launch_mix_beam(c->stream, a, sc, moments, nullptr, false, std::get<6>(k), std::get<7>(k),
+                        std::get<8>(k) != 0);
         if (launch_mix_adi(c->stream, a, tm, refine, adi_mode, &ntiles)) {
             set_error("mixed form: no ADI instantiation for this band width");
             return G4MI_E_ARG;
@@ -1656,7 +1661,8 @@ static int mix_step_fno(Ctx *c, const MixKey &k) {
             c->fno_skip--;
             c->fno_skipped++;
             c->fno_adi_steps++;
-            launch_mix_beam(c->stream, a, sc, moments, nullptr, false, std::get<6>(k), std::get<7>(k));
+            launch_mix_beam(c->stream, a, sc, moments, nullptr, false, std::get<6>(k), std::get<7>(k),
+                        std::get<8>(k) != 0);
             if (launch_mix_adi(c->stream, a, tm, refine, adi_mode, &ntiles)) {
                 set_error("mixed form: no ADI instantiation for this band width");
                 return G4MI_E_ARG;
@@ -1671,7 +1677,8 @@ static int mix_step_fno(Ctx *c, const MixKey &k) {
     std::vector<double> hf0(8 * (size_t)ns), hf1(8 * (size_t)ns), eg(ns), ss(ns);
     launch_mix_field_sums(c->stream, a, 0, c->d_diag_f);
     G4_HIP(hipMemcpyAsync(hf0.data(), c->d_diag_f, hf0.size() * 8, hipMemcpyDeviceToHost, c->stream));
-    launch_mix_beam(c->stream, a, sc, moments, nullptr, true, std::get<6>(k), std::get<7>(k));
+    launch_mix_beam(c->stream, a, sc, moments, nullptr, true, std::get<6>(k), std::get<7>(k),
+                        std::get<8>(k) != 0);
     G4_HIP(hipMemcpyAsync(&hs, c->m_st, sizeof(MixState), hipMemcpyDeviceToHost, c->stream));
     G4_HIP(hipStreamSynchronize(c->stream));
     G4_TRY(fno_infer_slices(c->fno, c->stream, c->m_field[hs.cur], c->m_field[hs.cur ^ 1], c->m_src, c->d_box, ns,
@@ -1901,8 +1908,17 @@ static int mix_run(Ctx *c, const g4mi_step_params *steps, int nsteps, double gam
         // the tile kernel's correction needs more LDS than a CU has beyond a half width of 9
         const int refine = (c->m_refine && (c->m_adi_mode != kAdiTile || tm <= 9)) ? 1 : 0;
         keys[i] = MixKey(tm, refine, sc ? 1 : 0, moments ? 1 : 0, maxzero, c->m_adi_mode, c->m_main_nt,
-                         c->m_main_lds);
+                         c->m_main_lds, 1);
     }
+    // the main kernel of every step but the run's last also runs the next
+    // step's front, while its slice's particles are fresh in the caches; the
+    // particles are back in the reference's order between runs
+    if (c->m_fuse_front)
+        for (int i = 0; i + 1 < n; i++) {
+            tab[i].fnext = tab[i + 1].front_done = 1;
+            tab[i + 1].fbuf = tab[i].fbuf ^ 1;
+        }
+    for (int i = 0; i < n; i++) std::get<8>(keys[i]) = tab[i].front_done ? 0 : 1;
     if (n > c->m_steps_cap) {
         if (c->m_steps) (void)hipFree(c->m_steps);
         c->m_steps = nullptr;
@@ -2122,6 +2138,7 @@ static int mix_alloc(Ctx *c) {
         (rc = dalloc(c, &c->m_field[0], ns * cells)) ||
         (rc = dalloc(c, &c->m_field[1], ns * cells)) || (rc = dalloc(c, &c->m_src, ns * cells)) ||
         (rc = dalloc(c, &c->m_beamsum, 8 * ns)) || (rc = dalloc(c, &c->m_egsum, ns)) ||
+        (rc = dalloc(c, &c->m_mom_next, ns)) ||
         (rc = dalloc(c, &c->m_tilediag, ns * mix_band_tiles(c->ng) * 8)) ||
         (rc = dalloc(c, &c->m_rec, (size_t)kMixRecRows * ns * kMixRecord)) ||
         (rc = dalloc(c, &c->m_st, 1)) ||
@@ -3024,6 +3041,10 @@ int g4mi_set_schedule(g4mi_ctx *h, const char *name, int value) {
         c->m_graphs = value != 0;
         return G4MI_OK;
     }
+    if (std::string(name) == "mixed_fuse_front" && (value == 0 || value == 1)) {
+        c->m_fuse_front = value != 0;
+        return G4MI_OK;
+    }
     set_error(std::string("g4mi_set_schedule: unknown entry or value: ") + name);
     return G4MI_E_ARG;
 }
@@ -3041,12 +3062,12 @@ int g4mi_get_schedule(g4mi_ctx *h, char *buf, int len) {
         snprintf(tmp, sizeof(tmp),
                  "{\"precision\": \"fp32\", \"adi\": \"%s\", \"adi_refine\": %d, "
                  "\"main_nt\": %d, \"main_lds_kib\": %d, \"persistent\": %d, \"autotune\": %d, \"tuned\": %d, "
-                 "\"graphs\": %d, \"bands_m_over_template\": \"%s\", \"graphs_captured\": %d, "
+                 "\"graphs\": %d, \"fuse_front\": %d, \"bands_m_over_template\": \"%s\", \"graphs_captured\": %d, "
                  "\"tuning\": %s, \"fno_skipped_steps\": %lld}",
                  c->m_adi_mode == kAdiWide ? "band32-wide" : c->m_adi_mode == kAdiStrips ? "band32-stream" : "band32-tile",
                  c->m_refine ? 1 : 0, c->m_main_nt,
                  (c->m_main_lds == 2 && c->m_main_nt != 512) ? 32 : c->m_main_lds ? 44 : 68, c->m_persistent ? 1 : 0, c->m_autotune ? 1 : 0, c->m_tuned ? 1 : 0, c->m_graphs ? 1 : 0,
-                 bands.c_str(), (int)c->m_graph.size(), c->m_tune_log.empty() ? "null" : c->m_tune_log.c_str(),
+                 c->m_fuse_front ? 1 : 0, bands.c_str(), (int)c->m_graph.size(), c->m_tune_log.empty() ? "null" : c->m_tune_log.c_str(),
                  c->fno_skipped);
         snprintf(buf, (size_t)len, "%s", tmp);
         return G4MI_OK;

@@ -6,6 +6,7 @@
 // may differ from glibc by an ulp.  Each kernel cites the reference loop it
 // restates; expressions keep the reference's association order.
 #include "g4mi_kernels.h"
+#include "g4mi_scan.h"
 
 #include <algorithm>
 
@@ -271,39 +272,14 @@ __global__ void __launch_bounds__(256) k_sc_rbound(const long long *__restrict__
     if (threadIdx.x == 0) rbound[is] = sqrt(lds[0]);
 }
 
-// The growing radial grid of the short-range solver: rmax is solver state
-// that grows to 1.5 rbound whenever a slice exceeds it, visited in slice
-// order (EFieldSolver.cpp:159-165).  One wavefront walks 64 slices at a
-// time; a ballot finds the next slice that raises rmax, so the serial part
-// is one step per growth event, not per slice.
-__global__ void __launch_bounds__(64) k_sc_scan(int ns, const long long *__restrict__ off,
-                                                const double *__restrict__ rbound,
-                                                double *__restrict__ rmax_state, int ngr,
-                                                double *__restrict__ dr) {
-    const int lane = threadIdx.x;
-    double rmax = rmax_state[0];
-    for (int base = 0; base < ns; base += 64) {
-        const int i = base + lane;
-        const bool valid = i < ns;
-        const bool nonempty = valid && (off[i + 1] - off[i] >= 1);
-        const double b = nonempty ? rbound[i] : -1.0;
-        double mine = rmax;
-        int start = 0;
-        for (;;) {
-            const unsigned long long m = __ballot(lane >= start && nonempty && b > rmax);
-            if (m == 0) {
-                if (lane >= start) mine = rmax;
-                break;
-            }
-            const int jl = __ffsll((long long)m) - 1;
-            if (lane >= start && lane < jl) mine = rmax;
-            rmax = __shfl(b, jl) * 1.5;
-            if (lane == jl) mine = rmax;
-            start = jl + 1;
-        }
-        if (valid) dr[i] = mine / (double)(ngr - 1);
-    }
-    if (lane == 0) rmax_state[0] = rmax;
+// The growing radial grid of the short-range solver (the walk of
+// g4mi_scan.h), one workgroup.
+__global__ void __launch_bounds__(kScanNT) k_sc_scan(int ns, const long long *__restrict__ off,
+                                                     const double *__restrict__ rbound,
+                                                     double *__restrict__ rmax_state, int ngr,
+                                                     double *__restrict__ dr) {
+    __shared__ double lds[kScanChunk];
+    scan_staged(ns, off, rbound, rmax_state, ngr, dr, lds);
 }
 
 // K3b: short-range space charge of one slice per workgroup,
@@ -2293,7 +2269,7 @@ void launch_sc_rbound(hipStream_t s, int ns, const long long *off, const double 
 }
 void launch_sc_scan(hipStream_t s, int ns, const long long *off, const double *rbound,
                     double *rmax_state, int ngr, double *dr) {
-    hipLaunchKernelGGL(k_sc_scan, dim3(1), dim3(64), 0, s, ns, off, rbound, rmax_state, ngr, dr);
+    hipLaunchKernelGGL(k_sc_scan, dim3(1), dim3(kScanNT), 0, s, ns, off, rbound, rmax_state, ngr, dr);
 }
 void launch_sc_solve(hipStream_t s, int ns, const long long *off, const double *x,
                      const double *y, const double *th, const double *cur, const double *cen,

@@ -36,6 +36,9 @@ struct MixStep {
     double sc_gz2, sc_ks;           // short-range space charge (EFieldSolver::shortRange)
     double slip;                    // slippage of the step in slices
     int longrange, use_sc, chicane, band;
+    int fnext;                      // the main kernel also runs the next step's front for its slice
+    int front_done;                 // the previous step's main kernel ran this step's front
+    int fbuf;                       // the slice-size buffer of this step (mom_fsize[fbuf])
 };
 
 // Band of the FP32 inverse for one step size.
@@ -71,7 +74,10 @@ struct MixArrays {
     int4 *box;                      // [ns] box of the source plane the ADI reads
     int4 *pbox;                     // [ns] cell footprint after the first half step
     // per-slice scratch
-    double *mom_fsize, *cen, *rbound, *dr, *rmax, *esc, *efield;
+    // slice sizes, two buffers: step S reads mom_fsize[S.fbuf] while its main
+    // kernels write the next step's into the other one
+    double *mom_fsize[2];
+    double *cen, *rbound, *dr, *rmax, *esc, *efield;
     double *beamsum;                // [ns][8]
     double *egsum;                  // [ns] sum of the push's gamma increments (before rounding)
     double *tilediag;               // [ns][tiles][8]
@@ -114,7 +120,8 @@ size_t mix_band_lds(int tmpl_m, bool refine);
 // dynamic LDS region, 0 = 68 KiB, 1 = 44 KiB.
 constexpr int kAdiTile = 0, kAdiStrips = 1, kAdiWide = 2;
 int launch_mix_step(hipStream_t s, const MixArrays &a, int tmpl_m, bool refine, bool sc,
-                    bool moments, int maxzero, hipEvent_t *ev, int adi_mode, int main_nt, int main_lds);
+                    bool moments, int maxzero, hipEvent_t *ev, int adi_mode, int main_nt, int main_lds,
+                    bool front = true);
 // The same step as one cooperative launch of co-resident workgroups with
 // grid barriers between the phases (the streaming ADI); ev (optional, 9
 // events): [0] before, [8] after.  Returns -1 if the launch is refused.
@@ -124,8 +131,9 @@ int launch_mix_step_persistent(hipStream_t s, const MixArrays &a, int tmpl_m, bo
 // slots 0-6), the ADI (ntiles: the field-sum blocks per slice it wrote) and
 // the tail (slippage, zeroing, record of ntiles blocks; ev slots 7-8).
 // egsum: also sum the push's energy increments per slice (a.egsum)
+// front: launch the front kernel (false when the previous step's main kernel ran it)
 void launch_mix_beam(hipStream_t s, const MixArrays &a, bool sc, bool moments, hipEvent_t *ev, bool egsum,
-                     int main_nt = 384, int main_lds = 0);
+                     int main_nt = 384, int main_lds = 0, bool front = true);
 int launch_mix_adi(hipStream_t s, const MixArrays &a, int tmpl_m, bool refine, int adi_mode, int *ntiles);
 void launch_mix_tail(hipStream_t s, const MixArrays &a, int maxzero, int ntiles, hipEvent_t *ev);
 // Field sums per storage slice ([ns][8], as the ADI's diagnostics with one

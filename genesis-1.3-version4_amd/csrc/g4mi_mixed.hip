@@ -23,6 +23,7 @@
 #include <hip/hip_cooperative_groups.h>
 
 #include "g4mi_mixed.h"
+#include "g4mi_scan.h"
 
 namespace g4mi {
 
@@ -213,11 +214,12 @@ __device__ __forceinline__ double bmax_w(double v, double *lds) {
 }
 
 constexpr int kFrontNT = 256;
-// front of slice `is` on a workgroup of NT threads (a multiple of 64);
-// lds: 4 NT / 64 doubles of LDS scratch
+// front of slice `is` for step S on a workgroup of NT threads (a multiple of
+// 64); the slice size goes to fsize_out[is]; lds: 4 NT / 64 doubles of LDS
+// scratch
 template <int NT>
-__device__ void front_body(const MixArrays &a, int is, int moments, int sc, double *lds) {
-    const MixStep &S = a.steps[a.st->istep];
+__device__ void front_body(const MixArrays &a, int is, int moments, int sc, double *lds, const MixStep &S,
+                           double *fsize_out) {
     const int t = threadIdx.x;
     const long long b0 = a.off[is], b1 = a.off[is + 1];
     const float gref = (float)a.gref;
@@ -286,7 +288,7 @@ __device__ void front_body(const MixArrays &a, int is, int moments, int sc, doub
             sz = sqrt(fabs(x2 - x1 * x1)) * sqrt(fabs(y2 - y1 * y1));
         }
         if (sz <= 0) sz = 1;
-        a.mom_fsize[is] = sz;
+        fsize_out[is] = sz;
         a.cen[2 * is] = xc;
         a.cen[2 * is + 1] = yc;
     }
@@ -301,42 +303,29 @@ __device__ void front_body(const MixArrays &a, int is, int moments, int sc, doub
     if (t == 0) a.rbound[is] = sqrt(m);
 }
 
+// the front of step istep unless the previous step's main kernel ran it
+// (uniform per workgroup)
+template <int NT>
+__device__ __forceinline__ void front_step(const MixArrays &a, int is, int moments, int sc, double *lds) {
+    const MixStep &S = a.steps[a.st->istep];
+    if (!S.front_done) front_body<NT>(a, is, moments, sc, lds, S, a.mom_fsize[S.fbuf]);
+}
+
 __global__ void __launch_bounds__(kFrontNT) k_mix_front(MixArrays a, int moments, int sc) {
     __shared__ double lds[4 * (kFrontNT / 64)];
-    front_body<kFrontNT>(a, blockIdx.x, moments, sc, lds);
+    front_step<kFrontNT>(a, blockIdx.x, moments, sc, lds);
 }
 
 // the growing radial grid of the short-range solver, in slice order
-// (EFieldSolver.cpp:159-165); the same ballot walk as the FP64 form
-__device__ void scan_body(const MixArrays &a) {
-    const int lane = threadIdx.x & 63;
-    double rmax = a.rmax[0];
-    const int ngr = a.sc_ng;
-    for (int base = 0; base < a.ns; base += 64) {
-        const int i = base + lane;
-        const bool valid = i < a.ns;
-        const bool nonempty = valid && (a.off[i + 1] - a.off[i] >= 1);
-        const double b = nonempty ? a.rbound[i] : -1.0;
-        double mine = rmax;
-        int start = 0;
-        for (;;) {
-            const unsigned long long m = __ballot(lane >= start && nonempty && b > rmax);
-            if (m == 0) {
-                if (lane >= start) mine = rmax;
-                break;
-            }
-            const int jl = __ffsll((long long)m) - 1;
-            if (lane >= start && lane < jl) mine = rmax;
-            rmax = __shfl(b, jl) * 1.5;
-            if (lane == jl) mine = rmax;
-            start = jl + 1;
-        }
-        if (valid) a.dr[i] = mine / (double)(ngr - 1);
-    }
-    if (lane == 0) a.rmax[0] = rmax;
+// (EFieldSolver.cpp:159-165); the same staged ballot walk as the FP64 form
+__device__ __forceinline__ void scan_body(const MixArrays &a, double *lds) {
+    scan_staged(a.ns, a.off, a.rbound, a.rmax, a.sc_ng, a.dr, lds);
 }
 
-__global__ void __launch_bounds__(64) k_mix_scan(MixArrays a) { scan_body(a); }
+__global__ void __launch_bounds__(kScanNT) k_mix_scan(MixArrays a) {
+    __shared__ double lds[kScanChunk];
+    scan_body(a, lds);
+}
 
 // ---------------------------------------------------------------------------
 // main: one workgroup per beam slice.
@@ -435,7 +424,7 @@ using TileT = double;
 // dynamic LDS region (68 or 44 KiB: the larger deposit tile, or more
 // workgroups per CU).  The autotuner picks both per context.
 template <bool EG, int NT, int DYN>
-__device__ void main_body(const MixArrays &a, const int is, int sc_on, float2 *dyn) {
+__device__ void main_step(const MixArrays &a, const int is, int sc_on, float2 *dyn) {
     constexpr int kTileCells = (DYN - kCelmCells) / 2;   // double2 cells: histograms, then the deposit tile
     // deposit tile cells: two planes of TileT
     constexpr int kDepCells = (DYN - kCelmCells) * (int)sizeof(float2) / (2 * (int)sizeof(TileT));
@@ -459,7 +448,7 @@ __device__ void main_body(const MixArrays &a, const int is, int sc_on, float2 *d
     if (S.longrange) {
         double e = 0;
         for (int j = t; j < ns; j += NT) {
-            const double sz = a.mom_fsize[j];
+            const double sz = a.mom_fsize[S.fbuf][j];
             const double ds = (double)(j - is) * S.lsc_ds;
             const double coef = 1 - sqrt(ds * ds / (ds * ds + sz));
             const double sgn = (j > is) ? -1. : ((j < is) ? 1. : 0.);
@@ -853,6 +842,23 @@ __device__ void main_body(const MixArrays &a, const int is, int sc_on, float2 *d
             plane[ry * ng + rx] = f2((float)pre[k], (float)pim[k]);
         }
         __syncthreads();
+    }
+}
+
+// main of slice is; with fnext, then the next step's front of the slice on
+// the particles this workgroup has just stored (each thread rereads its own:
+// the front partitions a slice as the main loop does).  The slice size goes
+// to the next step's size buffer: other workgroups of this launch still read
+// this step's.
+template <bool EG, int NT, int DYN>
+__device__ void main_body(const MixArrays &a, const int is, int sc_on, float2 *dyn) {
+    main_step<EG, NT, DYN>(a, is, sc_on, dyn);
+    const int k = a.st->istep;
+    if (a.steps[k].fnext) {
+        __syncthreads();  // the deposit tile's last reads before its LDS is reused
+        const int moments = (sc_on || a.steps[k].longrange) ? 1 : 0;
+        const MixStep &N = a.steps[k + 1];
+        front_body<NT>(a, is, moments, sc_on, (double *)dyn, N, a.mom_fsize[N.fbuf]);
     }
 }
 
@@ -1935,9 +1941,9 @@ __global__ void __launch_bounds__(kPerNT) k_mix_step_persistent(MixArrays a, int
     cg::grid_group grid = cg::this_grid();
     const int G = (int)gridDim.x, b = (int)blockIdx.x, t = (int)threadIdx.x, w = t >> 6;
     const int ns = a.ns;
-    for (int is = b; is < ns; is += G) front_body<kPerNT>(a, is, moments, sc, (double *)dyn);
+    for (int is = b; is < ns; is += G) front_step<kPerNT>(a, is, moments, sc, (double *)dyn);
     grid.sync();
-    if (sc && b == 0 && t < 64) scan_body(a);
+    if (sc && b == 0) scan_body(a, (double *)dyn);  // uniform per workgroup
     grid.sync();
     for (int is = b; is < ns; is += G) main_body<false, kPerNT, kMixDyn>(a, is, sc, dyn);
     grid.sync();
@@ -2107,13 +2113,13 @@ int mix_prepare() {
 int mix_sc_mode_cells() { return kCelmCells; }
 
 void launch_mix_beam(hipStream_t s, const MixArrays &a, bool sc, bool moments, hipEvent_t *ev, bool egsum,
-                     int main_nt, int main_lds) {
+                     int main_nt, int main_lds, bool front) {
     auto mark = [&](int k) {
         if (ev) (void)hipEventRecord(ev[k], s);
     };
     mark(0);
-    hipLaunchKernelGGL(k_mix_front, dim3(a.ns), dim3(kFrontNT), 0, s, a, moments ? 1 : 0, sc ? 1 : 0);
-    if (sc) hipLaunchKernelGGL(k_mix_scan, dim3(1), dim3(64), 0, s, a);
+    if (front) hipLaunchKernelGGL(k_mix_front, dim3(a.ns), dim3(kFrontNT), 0, s, a, moments ? 1 : 0, sc ? 1 : 0);
+    if (sc) hipLaunchKernelGGL(k_mix_scan, dim3(1), dim3(kScanNT), 0, s, a);
     mark(1);
     mark(2);
     if (egsum) launch_main<true>(s, a, sc ? 1 : 0, main_nt, main_lds);
@@ -2193,8 +2199,9 @@ void launch_mix_tail(hipStream_t s, const MixArrays &a, int maxzero, int ntiles,
 }
 
 int launch_mix_step(hipStream_t s, const MixArrays &a, int tmpl_m, bool refine, bool sc,
-                    bool moments, int maxzero, hipEvent_t *ev, int adi_mode, int main_nt, int main_lds) {
-    launch_mix_beam(s, a, sc, moments, ev, false, main_nt, main_lds);
+                    bool moments, int maxzero, hipEvent_t *ev, int adi_mode, int main_nt, int main_lds,
+                    bool front) {
+    launch_mix_beam(s, a, sc, moments, ev, false, main_nt, main_lds, front);
     int ntiles = 0;
     if (launch_mix_adi(s, a, tmpl_m, refine, adi_mode, &ntiles) != 0) return -1;
     launch_mix_tail(s, a, maxzero, ntiles, ev);

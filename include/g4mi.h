@@ -268,7 +268,12 @@ int g4mi_sort_index_noalloc(g4mi_ctx *ctx, double slen, int *atar, long long *mo
  * "main_nt" (256, 384, 512), "main_lds" (0: 68 KiB, 1: 44 KiB, 2: 32 KiB;
  * 512 threads take 44 KiB for 2),
  * "persistent", "graphs" (0/1); setting any of the kernel entries turns the
- * tuner off, "autotune" 1 turns it on again. */
+ * tuner off, "autotune" 1 turns it on again.  "mixed_fuse_front" (default
+ * 1): within one g4mi_run the main kernel of each step but the last also
+ * runs the next step's front (first transverse half step, slice sizes,
+ * centroid, largest radius, footprint) for its slice, so the front kernel
+ * of that step only takes over the slice sizes; 0 runs every front as its
+ * own kernel. */
 int g4mi_set_schedule(g4mi_ctx *ctx, const char *name, int value);
 int g4mi_get_schedule(g4mi_ctx *ctx, char *buf, int len);
 

@@ -230,6 +230,50 @@ def test_mixed_persistent_step_matches_kernels(st):
     assert ef < 1e-4
 
 
+def test_mixed_fused_front_matches_separate_kernels(st):
+    """The main kernel running the next step's front for its slice (the
+    default) against every front as its own kernel: 60 steps of a 40-slice
+    window with long- and short-range space charge and slippage, as runs of
+    1, 20 and 39 steps (a run's last step leaves the particles in the
+    reference's order, the next run starts with its own front), with 256
+    threads per slice (the front's partition) and with 384.  Records, final
+    field and the downloaded beam within FP32 rounding (the deposit and
+    histograms add with LDS atomics whose order varies)."""
+    for nt in (256, 384):
+        _log(f"mixed fused front vs separate kernels, {nt} threads, 60 steps, 40 slices x 512, 63^2")
+        out = []
+        for fuse in (1, 0):
+            sim = H.make_state([512] * 40, ngrid=63, gridmax=0.3e-3, longrange=True, sc=(1, 0, 100, 0.0))
+            ctx = H.to_gpu(sim, precision=g4mi.FP32)
+            ctx.set_schedule("mixed_fuse_front", fuse)
+            ctx.set_schedule("main_nt", nt)
+            rec = {"power": [], "bunching": []}
+            for a, b in ((0, 1), (1, 21), (21, 60)):
+                r = ctx.run(H.gsteps(st[a:b]), cases.GAMMA0, record=("power", "bunching"))
+                for k in rec:
+                    rec[k].append(r[k])
+            f, first = ctx.download_field()
+            beam = ctx.download_beam()
+            sched = ctx.schedule()
+            assert sched["fuse_front"] == fuse, sched
+            out.append(({k: np.concatenate(v) for k, v in rec.items()}, f, first, beam))
+            ctx.close()
+        (ra, fa, qa, ba), (rb, fb, qb, bb) = out
+        assert qa == qb
+        for k in ("power", "bunching"):
+            e = float(np.max(np.abs(ra[k] - rb[k])) / np.max(np.abs(rb[k])))
+            _log(f"  {k}: max rel diff fused vs separate {e:.3e}")
+            assert e < 1e-4
+        ef = float(np.max(np.abs(fa - fb)) / np.max(np.abs(fb)))
+        _log(f"  final field: max diff / max {ef:.3e}")
+        assert ef < 1e-4
+        for k in ("x", "y", "px", "py", "gamma", "theta"):
+            sc = max(float(np.max(np.abs(bb[k] - (np.mean(bb[k]) if k == "gamma" else 0.0)))), 1e-300)
+            e = float(np.max(np.abs(ba[k] - bb[k]))) / sc
+            _log(f"  beam {k}: max diff / scale {e:.3e}")
+            assert e < 1e-4, k
+
+
 def test_mixed_autotuner_picks_a_schedule(st):
     """The first run of 18 or more steps times every main-kernel shape (4, 6,
     8 wavefronts per slice x 68 or 44 KiB of LDS; 4, 6 x 32 KiB), the three ADI kernels, the
