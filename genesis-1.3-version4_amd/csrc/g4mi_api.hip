@@ -193,7 +193,8 @@ struct Ctx {
     int m_tune_nmain[8] = {0, 0, 0, 0, 0, 0, 0, 0}, m_tune_nadi[3] = {0, 0, 0};
     hipEvent_t m_tune_ev[9] = {};
     std::map<std::tuple<int, int, int, int, int, int, int, int, int>, hipGraphExec_t> m_graph;
-    std::vector<double> m_rec_host;
+    double *m_rec_pin = nullptr;           // pinned host mirror of the record ring
+    hipEvent_t m_rec_ev[2] = {nullptr, nullptr};  // copy of each half of the ring done
     std::vector<double> h_cur;             // currents [ns] (host copy)
     double *d_aux = nullptr;               // auxiliary diagnostics [16 ns]
     // learned field step (mixed form): 0 detached, 1 capture, 2 replace the ADI
@@ -1933,7 +1934,24 @@ static int mix_run(Ctx *c, const g4mi_step_params *steps, int nsteps, double gam
     G4_HIP(hipMemcpyAsync(&c->m_st->istep, &zero, sizeof(int), hipMemcpyHostToDevice, c->stream));
     G4_HIP(hipStreamSynchronize(c->stream));
     const size_t rowd = (size_t)c->ns * kMixRecord;
-    int pending = 0;  // first step whose record is not yet copied back
+    // Records: the device ring of m_nrec rows (row = step mod m_nrec) goes to
+    // its pinned host mirror half by half, each half by an async copy queued
+    // behind its steps; the host finishes a half's rows just before the next
+    // copy into that half (its own copy ended a half's worth of steps
+    // earlier) or at the end of the run, so the stream never waits on the host.
+    const int half = c->m_nrec / 2;
+    int pending = 0;                       // first step whose record copy is not queued
+    int hfirst[2] = {0, 0}, hcnt[2] = {0, 0};
+    auto finish_half = [&](int h) -> int {
+        if (hcnt[h] == 0) return G4MI_OK;
+        G4_HIP(hipEventSynchronize(c->m_rec_ev[h]));
+        for (int k = 0; k < hcnt[h]; k++) {
+            const int stp = hfirst[h] + k;
+            mix_finish_row(c, c->m_rec_pin + (size_t)(stp % c->m_nrec) * rowd, &records[stp]);
+        }
+        hcnt[h] = 0;
+        return G4MI_OK;
+    };
     // The schedule autotuner: the first kMixTuneSteps steps of a context
     // (over as many g4mi_run calls as they take) each run one candidate of the
     // main kernel's shape (threads per slice x LDS region) and one of the ADI
@@ -1976,19 +1994,22 @@ static int mix_run(Ctx *c, const g4mi_step_params *steps, int nsteps, double gam
         } else {
             G4_TRY(mix_launch(c, keys[i], nullptr));
         }
-        if (records && ((i + 1 - pending) == c->m_nrec || i == n - 1)) {
-            const int cnt = i + 1 - pending;
-            c->m_rec_host.resize((size_t)c->m_nrec * rowd);
-            G4_HIP(hipMemcpyAsync(c->m_rec_host.data(), c->m_rec, (size_t)c->m_nrec * rowd * sizeof(double),
+        if (records && ((i + 1 - pending) == half || i == n - 1)) {
+            // steps pending..i: whole halves but the last, so their rows are
+            // contiguous within one half of the ring
+            const int cnt = i + 1 - pending, h = (pending / half) & 1;
+            G4_TRY(finish_half(h));
+            const size_t r0 = (size_t)(pending % c->m_nrec) * rowd;
+            G4_HIP(hipMemcpyAsync(c->m_rec_pin + r0, c->m_rec + r0, (size_t)cnt * rowd * sizeof(double),
                                   hipMemcpyDeviceToHost, c->stream));
-            G4_HIP(hipStreamSynchronize(c->stream));
-            for (int k = 0; k < cnt; k++) {
-                const int stp = pending + k;
-                mix_finish_row(c, c->m_rec_host.data() + (size_t)(stp % c->m_nrec) * rowd, &records[stp]);
-            }
+            G4_HIP(hipEventRecord(c->m_rec_ev[h], c->stream));
+            hfirst[h] = pending;
+            hcnt[h] = cnt;
             pending = i + 1;
         }
     }
+    G4_TRY(finish_half(0));
+    G4_TRY(finish_half(1));
     MixState hs{};
     G4_HIP(hipMemcpyAsync(&hs, c->m_st, sizeof(MixState), hipMemcpyDeviceToHost, c->stream));
     G4_HIP(hipStreamSynchronize(c->stream));
@@ -2146,6 +2167,8 @@ static int mix_alloc(Ctx *c) {
         (rc = dalloc(c, &c->m_binfo, (size_t)kMixMaxBands)))
         return rc;
     c->m_nrec = kMixRecRows;
+    G4_HIP(hipHostMalloc((void **)&c->m_rec_pin, (size_t)kMixRecRows * ns * kMixRecord * sizeof(double)));
+    for (auto &e : c->m_rec_ev) G4_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
     if (mix_prepare() != 0) {
         set_error("mixed form: setting the kernels' LDS limits failed");
         return G4MI_E_HIP;
@@ -2314,6 +2337,9 @@ int g4mi_destroy(g4mi_ctx *h) {
     if (c->incoh_ev) (void)hipEventDestroy(c->incoh_ev);
     if (c->h_recs) (void)hipHostFree(c->h_recs);
     if (c->h_bdg) (void)hipHostFree(c->h_bdg);
+    if (c->m_rec_pin) (void)hipHostFree(c->m_rec_pin);
+    for (auto &e : c->m_rec_ev)
+        if (e) (void)hipEventDestroy(e);
     delete c->incoh;
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete h;
