@@ -412,7 +412,9 @@ __device__ __forceinline__ float2 cw_pow(float2 cw, int m) {
 
 // LDS accumulation is FP64 (ds_add_f64): on gfx950 FP32 LDS atomics
 // (ds_add_f32) issue far slower than FP64 ones, and the sums gain precision.
-constexpr int kCelmCells = 1024;                          // space-charge mode solutions (float2)
+// largest space-charge mode table (float2 cells); a run takes (2 nphi + 1)
+// nz modes x radial cells of the main kernel's region, the deposit tile the rest
+constexpr int kCelmCells = 1024;
 // accumulation type of the main kernel's deposit tile (FP64: an FP32 tile
 // holds twice the footprint but its atomics made the kernel 2.1x slower)
 using TileT = double;
@@ -425,10 +427,15 @@ using TileT = double;
 // workgroups per CU).  The autotuner picks both per context.
 template <bool EG, int NT, int DYN>
 __device__ void main_step(const MixArrays &a, const int is, int sc_on, float2 *dyn) {
-    constexpr int kTileCells = (DYN - kCelmCells) / 2;   // double2 cells: histograms, then the deposit tile
+    // the region: the space-charge mode solutions ((2 nphi + 1) nz modes x
+    // radial cells when the short-range solver runs, at most kCelmCells,
+    // rounded up to a double2 boundary), then the histograms and later the
+    // deposit tile in the rest
+    static_assert((NT / 64) * kScMax <= (DYN - kCelmCells) / 2, "the histograms must fit in the tile region");
+    const int ncelm = sc_on ? (((2 * a.sc_nphi + 1) * a.sc_nz * a.sc_ng + 1) & ~1) : 0;
+    const int tile_cells = (DYN - ncelm) / 2;             // double2 cells
     // deposit tile cells: two planes of TileT
-    constexpr int kDepCells = (DYN - kCelmCells) * (int)sizeof(float2) / (2 * (int)sizeof(TileT));
-    static_assert((NT / 64) * kScMax <= kTileCells, "the histograms must fit in the tile region");
+    const int dep_cells = (DYN - ncelm) * (int)sizeof(float2) / (2 * (int)sizeof(TileT));
     __shared__ int bred[5 * (NT / 64)];
     __shared__ double wred[9 * (NT / 64)];
     __shared__ float gsol[kScMax], scupp[kScMax], sclow[kScMax], scmid[kScMax];
@@ -441,7 +448,7 @@ __device__ void main_step(const MixArrays &a, const int is, int sc_on, float2 *d
     const int first = a.st->first;
     const float2 *field = a.field[a.st->cur] + (long long)((is + first) % ns) * ng * ng;
     float2 *celm = dyn;                                   // [nmodes][ngr]
-    double2 *work = (double2 *)(dyn + kCelmCells);        // histograms / deposit tile
+    double2 *work = (double2 *)(dyn + ncelm);             // histograms / deposit tile
 
     // ---- long-range space charge of this slice
     double esc = 0;
@@ -480,7 +487,7 @@ __device__ void main_step(const MixArrays &a, const int is, int sc_on, float2 *d
         // sub-histograms per lane group (lane mod nsub) within each wavefront:
         // a beam's particles crowd into the inner radial cells, and lanes
         // adding into one cell serialise in the LDS
-        const int nsub = (NT / 64) * 4 * ngr <= kTileCells ? 4 : 1;
+        const int nsub = (NT / 64) * 4 * ngr <= tile_cells ? 4 : 1;
         double2 *hs = work;  // [NT/64][nsub][ngr]
         double2 *myh = hs + (w * nsub + (t & (nsub - 1))) * ngr;
         int mode = 0;
@@ -567,7 +574,7 @@ __device__ void main_step(const MixArrays &a, const int is, int sc_on, float2 *d
         ty0 = max(0, pb.y - 2);
         tw = min(ng - 1, pb.z + 2) - tx0 + 1;
         th = min(ng - 1, pb.w + 2) - ty0 + 1;
-        fused = tw * th <= kDepCells;
+        fused = tw * th <= dep_cells;
     }
     float scl = 0;
     if (dep) {
@@ -804,7 +811,7 @@ __device__ void main_step(const MixArrays &a, const int is, int sc_on, float2 *d
     by1 = min(by1, ng - 1);
     if (t == 0) a.box[is] = make_int4(bx0, by0, bx1, by1);
     const int ww = bx1 - bx0 + 1;
-    const int rows_per_band = max(1, kDepCells / ww);
+    const int rows_per_band = max(1, dep_cells / ww);
     for (int r0 = by0; r0 <= by1; r0 += rows_per_band) {
         const int r1 = min(by1, r0 + rows_per_band - 1);
         const int ncell = (r1 - r0 + 1) * ww;

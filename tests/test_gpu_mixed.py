@@ -303,10 +303,11 @@ def test_mixed_autotuner_picks_a_schedule(st):
 def test_mixed_main_kernel_shapes_agree(st):
     """Every main-kernel shape the tuner chooses from (4, 6, 8 wavefronts per
     slice x 68 or 44 KiB of LDS; 4, 6 x 32 KiB) on 6 slices x 4096 at 255^2
-    with space charge, 30 steps: the slices' deposit footprints (~45 x 45
-    cells) fit the 44 and 68 KiB tiles (48 x 48, 62 x 62) but not the 32 KiB
-    one (39 x 39), whose shapes take the banded two-pass deposit.  Records
-    and field agree to FP32 rounding (the LDS atomics add in varying order)."""
+    with space charge (one mode of 100 radial cells), 30 steps: the slices'
+    padded deposit footprints (~47 x 48 cells) fit the 44 and 68 KiB tiles
+    (2766 and 4302 cells) but not the 32 KiB one (1998 cells), whose shapes
+    take the banded two-pass deposit.  Records and field agree to FP32
+    rounding (the LDS atomics add in varying order)."""
     _log("mixed main-kernel shapes: 6 slices x 4096, 255^2, 30 steps, 8 shapes")
     ref = None
     for nt, lds in ((384, 0), (256, 0), (512, 0), (384, 1), (256, 1), (512, 1), (256, 2), (384, 2)):
