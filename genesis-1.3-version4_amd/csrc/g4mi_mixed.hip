@@ -1506,7 +1506,7 @@ __device__ __forceinline__ void lds_barrier() {
 template <int M, bool REF>
 size_t wide_lds_bytes(int nw) {
     const int nc = 64 * nw;
-    return (size_t)2 * (nc + 2 * M) * sizeof(float2) + (REF ? (size_t)(nc + 2) * sizeof(float2) : 0) +
+    return (size_t)2 * (nc + 2 * M) * sizeof(float2) + (REF ? (size_t)(nc + 2) * sizeof(double2) : 0) +
            (size_t)(2 * M + 1) * nc * sizeof(float2) + (size_t)nw * 8 * sizeof(double);
 }
 
@@ -1521,8 +1521,8 @@ __global__ void __launch_bounds__(512, (REF && M == 9) ? 2 : 3) k_mix_adi_wide(M
     const int t = threadIdx.x, lane = t & 63, w = t >> 6;
     float2 *lrow0 = (float2 *)wide_lds;                    // [2][nc + 2M]
     const int lp = nc + 2 * M;
-    float2 *lu = lrow0 + 2 * lp;                            // [nc + 2] u1 row (FP32 values)
-    float2 *lband = lu + (REF ? nc + 2 : 0);                // [W][nc]
+    double2 *lu = (double2 *)(lrow0 + 2 * lp);              // [nc + 2] u1 row (FP32 values, widened)
+    float2 *lband = (float2 *)(lu + (REF ? nc + 2 : 0));    // [W][nc]
     double *wsum = (double *)(lband + W * nc);              // [nw][8]
     const int ii = blockIdx.x;
     if (ii >= a.ns) return;
@@ -1552,7 +1552,7 @@ __global__ void __launch_bounds__(512, (REF && M == 9) ? 2 : 3) k_mix_adi_wide(M
     // zero rows (the pads stay zero: only lanes on the grid write non-zero values)
     for (int k = t; k < 2 * lp; k += nt) lrow0[k] = f2(0, 0);
     if constexpr (REF)
-        for (int k = t; k < nc + 2; k += nt) lu[k] = f2(0, 0);
+        for (int k = t; k < nc + 2; k += nt) lu[k] = d2(0, 0);
     if (!wsym)
         for (int m = 0; m < W; m++) lband[m * nc + t] = xin ? band[(long long)x * W + m] : f2(0, 0);
     const float2 *bc = band + (long long)((bi.klo + bi.khi) / 2) * W;
@@ -1673,12 +1673,12 @@ __global__ void __launch_bounds__(512, (REF && M == 9) ? 2 : 3) k_mix_adi_wide(M
                     up = fadd(up, up1);
                 }
                 if constexpr (REF) {
-                    lu[x + 1] = up;  // zero off the grid
-                    lds_barrier();
+                    // u1 goes to the row widened once, not by each neighbour
                     const double2 u1 = tod(up);
-                    const float2 nl = lu[x], nr = lu[x + 2];
-                    const double2 au =
-                        adi_apply(u1, d2((double)nl.x + (double)nr.x, (double)nl.y + (double)nr.y), ad);
+                    lu[x + 1] = u1;  // zero off the grid
+                    lds_barrier();
+                    const double2 nl = lu[x], nr = lu[x + 2];
+                    const double2 au = adi_apply(u1, d2(nl.x + nr.x, nl.y + nr.y), ad);
                     const double2 upd = jacobi_update(u1, d2(E64.x - au.x, E64.y - au.y), ad, dn);
                     if (xin) {
                         const double2 F64 = d2(fma(2.0, upd.x, sv64.x) - E64.x, fma(2.0, upd.y, sv64.y) - E64.y);
