@@ -162,6 +162,7 @@ struct Ctx {
     double gref = 0;                       // reference energy of the FP32 offsets
     float *m_dg = nullptr, *m_dth = nullptr, *m_x = nullptr, *m_y = nullptr, *m_px = nullptr,
           *m_py = nullptr;
+    float *m_xs = nullptr, *m_ys = nullptr;  // post-t2 positions (the main loop's next front)
     double *m_base = nullptr;              // [ns] FP64 phase base
     int4 *m_pbox = nullptr;                // [ns] footprint after the first half step
     float2 *m_field[2] = {nullptr, nullptr}, *m_src = nullptr;
@@ -1397,6 +1398,8 @@ static MixArrays mix_arrays(const Ctx *c) {
     a.y = c->m_y;
     a.px = c->m_px;
     a.py = c->m_py;
+    a.xs = c->m_xs;
+    a.ys = c->m_ys;
     a.base = c->m_base;
     a.off = c->d_off;
     a.cur = c->d_cur;
@@ -2155,6 +2158,7 @@ static int mix_alloc(Ctx *c) {
     int rc;
     if ((rc = dalloc(c, &c->m_dg, n)) || (rc = dalloc(c, &c->m_dth, n)) || (rc = dalloc(c, &c->m_x, n)) ||
         (rc = dalloc(c, &c->m_y, n)) || (rc = dalloc(c, &c->m_px, n)) || (rc = dalloc(c, &c->m_py, n)) ||
+        (rc = dalloc(c, &c->m_xs, n)) || (rc = dalloc(c, &c->m_ys, n)) ||
         (rc = dalloc(c, &c->m_base, ns)) || (rc = dalloc(c, &c->m_pbox, ns)) ||
         (rc = dalloc(c, &c->m_field[0], ns * cells)) ||
         (rc = dalloc(c, &c->m_field[1], ns * cells)) || (rc = dalloc(c, &c->m_src, ns * cells)) ||

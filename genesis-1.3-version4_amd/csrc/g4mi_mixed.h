@@ -65,6 +65,7 @@ struct MixState {
 struct MixArrays {
     // particles
     float *dg, *dth, *x, *y, *px, *py;
+    float *xs, *ys;                 // post-t2 positions when the next front rides in the main loop
     double *base;                   // [ns]
     const long long *off;           // [ns+1]
     const double *cur;              // [ns] current

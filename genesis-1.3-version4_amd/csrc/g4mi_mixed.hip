@@ -214,6 +214,9 @@ __device__ __forceinline__ double bmax_w(double v, double *lds) {
 }
 
 constexpr int kFrontNT = 256;
+template <int NT>
+__device__ void front_finish(const MixArrays &a, int is, int moments, int sc, double *lds, double (&v)[4], int bx0,
+                             int by0, int bx1, int by1, double *fsize_out);
 // front of slice `is` for step S on a workgroup of NT threads (a multiple of
 // 64); the slice size goes to fsize_out[is]; lds: 4 NT / 64 doubles of LDS
 // scratch
@@ -266,6 +269,18 @@ __device__ void front_body(const MixArrays &a, int is, int moments, int sc, doub
             by1 = max(by1, iy + 1);
         }
     }
+    front_finish<NT>(a, is, moments, sc, lds, v, bx0, by0, bx1, by1, fsize_out);
+}
+
+// The rest of a front once its particles are pushed and stored: footprint,
+// slice size and centroid from the per-lane sums v (x, x^2, y, y^2) and
+// cells, then the largest radius (each lane rereads the particles it wrote;
+// the partition is the particle loop's, i = b0 + t + k NT).
+template <int NT>
+__device__ void front_finish(const MixArrays &a, int is, int moments, int sc, double *lds, double (&v)[4], int bx0,
+                             int by0, int bx1, int by1, double *fsize_out) {
+    const int t = threadIdx.x;
+    const long long b0 = a.off[is], b1 = a.off[is + 1];
     // footprint of the slice for the main kernel's deposit tile
     int bv[4] = {bx0, by0, -bx1, -by1};
     bmin_w<NT, 4>(bv, (int *)lds);
@@ -426,7 +441,7 @@ using TileT = double;
 // dynamic LDS region (68 or 44 KiB: the larger deposit tile, or more
 // workgroups per CU).  The autotuner picks both per context.
 template <bool EG, int NT, int DYN>
-__device__ void main_step(const MixArrays &a, const int is, int sc_on, float2 *dyn) {
+__device__ bool main_step(const MixArrays &a, const int is, int sc_on, float2 *dyn) {
     // the region: the space-charge mode solutions ((2 nphi + 1) nz modes x
     // radial cells when the short-range solver runs, at most kCelmCells,
     // rounded up to a double2 boundary), then the histograms and later the
@@ -610,6 +625,16 @@ __device__ void main_step(const MixArrays &a, const int is, int sc_on, float2 *d
     const float eps_ref = (float)S.eps_ref;
     const bool chic = S.chicane != 0;
     const TransParams t2 = S.t2;
+    // With fnext and the fused tile the next step's front rides in this loop:
+    // after the deposit, the first transverse half step of step k+1 on the
+    // registers, its slice sums and footprint, one store.  The post-t2
+    // positions also go to xs/ys for the two-pass deposit, should a particle
+    // leave the tile.  (Without the fused tile the front runs after the
+    // deposit, as a second pass.)
+    const bool nextf = S.fnext && fused;
+    const TransParams t1n = a.steps[a.st->istep + (nextf ? 1 : 0)].t1;
+    double nv[4] = {0, 0, 0, 0};
+    int nbx0 = 1 << 30, nby0 = 1 << 30, nbx1 = -1, nby1 = -1;
     // per-lane beam sums of at most a few dozen particles in FP32, energy as
     // the offset dg; widened to FP64 for the block sum
     constexpr int NV = EG ? 9 : 8;
@@ -729,10 +754,6 @@ __device__ void main_step(const MixArrays &a, const int is, int sc_on, float2 *d
         transverse32(t2, gref + dg, x, y, px, py);
         a.dg[i] = dg;
         a.dth[i] = dth;
-        a.x[i] = x;
-        a.y[i] = y;
-        a.px[i] = px;
-        a.py[i] = py;
         // beam sums of the updated state
         fv[0] += dg;
         fv[1] += dg * dg;
@@ -773,6 +794,32 @@ __device__ void main_step(const MixArrays &a, const int is, int sc_on, float2 *d
                 }
             }
         }
+        if (nextf) {
+            a.xs[i] = x;
+            a.ys[i] = y;
+            transverse32(t1n, gref + dg, x, y, px, py);
+            nv[0] += x;
+            nv[1] += (double)x * x;
+            nv[2] += y;
+            nv[3] += (double)y * y;
+            if ((x > -gridmax) && (x < gridmax) && (y > -gridmax) && (y < gridmax)) {
+                const int ix = min((int)floorf((x + gridmax) * inv_dgrid), ng - 2);
+                const int iy = min((int)floorf((y + gridmax) * inv_dgrid), ng - 2);
+                nbx0 = min(nbx0, ix);
+                nbx1 = max(nbx1, ix + 1);
+                nby0 = min(nby0, iy);
+                nby1 = max(nby1, iy + 1);
+            }
+        }
+        a.x[i] = x;
+        a.y[i] = y;
+        a.px[i] = px;
+        a.py[i] = py;
+    }
+    if (nextf) {
+        const MixStep &N = a.steps[a.st->istep + 1];
+        front_finish<NT>(a, is, (sc_on || S.longrange) ? 1 : 0, sc_on, wred, nv, nbx0, nby0, nbx1, nby1,
+                         a.mom_fsize[N.fbuf]);
     }
     double v[NV];
 #pragma unroll
@@ -797,14 +844,14 @@ __device__ void main_step(const MixArrays &a, const int is, int sc_on, float2 *d
     ovf = -bv[4];
     if (!dep || bx1 < 0) {
         if (t == 0) a.box[is] = make_int4(1, 1, 0, 0);
-        return;
+        return nextf;
     }
     float2 *plane = a.src + (long long)is * ng * ng;
     if (fused && !ovf) {
         for (int k = t; k < tw * th; k += NT)
             plane[(ty0 + k / tw) * ng + tx0 + k % tw] = f2((float)tre[k], (float)tim[k]);
         if (t == 0) a.box[is] = make_int4(tx0, ty0, tx0 + tw - 1, ty0 + th - 1);
-        return;
+        return nextf;
     }
     // ---- exact two-pass deposit over the footprint box (rows in bands of the tile)
     bx1 = min(bx1, ng - 1);
@@ -819,7 +866,7 @@ __device__ void main_step(const MixArrays &a, const int is, int sc_on, float2 *d
         for (int k = t; k < 2 * ncell; k += NT) pre[k] = (TileT)0;
         __syncthreads();
         for (long long i = b0 + t; i < b1; i += NT) {
-            const float x = a.x[i], y = a.y[i];
+            const float x = nextf ? a.xs[i] : a.x[i], y = nextf ? a.ys[i] : a.y[i];  // post-t2
             if (!((x > -gridmax) && (x < gridmax) && (y > -gridmax) && (y < gridmax))) continue;
             float wx = (x + gridmax) * inv_dgrid, wy = (y + gridmax) * inv_dgrid;
             const int ix = min((int)floorf(wx), ng - 2), iy = min((int)floorf(wy), ng - 2);
@@ -850,18 +897,20 @@ __device__ void main_step(const MixArrays &a, const int is, int sc_on, float2 *d
         }
         __syncthreads();
     }
+    return nextf;
 }
 
-// main of slice is; with fnext, then the next step's front of the slice on
+// main of slice is; with fnext, the next step's front of the slice either
+// rides in the particle loop (main_step returns true) or runs after it on
 // the particles this workgroup has just stored (each thread rereads its own:
 // the front partitions a slice as the main loop does).  The slice size goes
 // to the next step's size buffer: other workgroups of this launch still read
 // this step's.
 template <bool EG, int NT, int DYN>
 __device__ void main_body(const MixArrays &a, const int is, int sc_on, float2 *dyn) {
-    main_step<EG, NT, DYN>(a, is, sc_on, dyn);
+    const bool done = main_step<EG, NT, DYN>(a, is, sc_on, dyn);
     const int k = a.st->istep;
-    if (a.steps[k].fnext) {
+    if (a.steps[k].fnext && !done) {
         __syncthreads();  // the deposit tile's last reads before its LDS is reused
         const int moments = (sc_on || a.steps[k].longrange) ? 1 : 0;
         const MixStep &N = a.steps[k + 1];

@@ -274,6 +274,49 @@ def test_mixed_fused_front_matches_separate_kernels(st):
             assert e < 1e-4, k
 
 
+def test_mixed_fused_front_with_tile_overflow(st):
+    """A beam with a transverse momentum offset (px + 50, about 3.4 cells
+    per half step at 63^2): the second half step carries particles out of
+    the deposit tile (the first half step's footprint padded by 2 cells), so
+    every slice takes the two-pass deposit, which with the next front in the
+    main loop reads the post-t2 positions kept aside.  Three undulator
+    steps as one run, next front in the loop against separate front kernels:
+    records, field and beam within FP32 rounding."""
+    _log("mixed fused front, deposit tile overflow: 6 slices x 512, 63^2, 3 steps")
+    out = []
+    for fuse in (1, 0):
+        sim = H.make_state([512] * 6, ngrid=63, gridmax=0.3e-3, longrange=True, sc=(1, 0, 100, 0.0),
+                           seed_power=1e4)
+        ctx = H.to_gpu(sim, precision=g4mi.FP32)
+        beam = {k: np.array(getattr(sim, k)) for k in ("gamma", "theta", "x", "y", "px", "py")}
+        beam["px"] += 50.0
+        ctx.upload_beam(beam, sim.current)
+        ctx.set_schedule("mixed_fuse_front", fuse)
+        ctx.set_schedule("main_nt", 256)
+        ctx.set_schedule("main_lds", 1)
+        r = ctx.run(H.gsteps(st[:3]), cases.GAMMA0, record=("power", "bunching"))
+        f, first = ctx.download_field()
+        out.append((r, f, first, ctx.download_beam()))
+        ctx.close()
+    (ra, fa, qa, ba), (rb, fb, qb, bb) = out
+    assert qa == qb
+    x0 = np.array(bb["x"])
+    moved = float(np.mean(x0)) / (2 * 0.3e-3 / 62)
+    _log(f"  beam centroid moved {moved:.1f} cells in 3 steps")
+    assert moved > 15
+    for k in ("power", "bunching"):
+        e = float(np.max(np.abs(ra[k] - rb[k])) / np.max(np.abs(rb[k])))
+        _log(f"  {k}: max rel diff fused vs separate {e:.3e}")
+        assert e < 1e-4
+    ef = float(np.max(np.abs(fa - fb)) / np.max(np.abs(fb)))
+    _log(f"  final field: max diff / max {ef:.3e}")
+    assert ef < 1e-4
+    for k in ("x", "y", "px", "py", "gamma", "theta"):
+        sc = max(float(np.max(np.abs(bb[k] - (np.mean(bb[k]) if k == "gamma" else 0.0)))), 1e-300)
+        e = float(np.max(np.abs(ba[k] - bb[k]))) / sc
+        assert e < 1e-4, (k, e)
+
+
 def test_mixed_autotuner_picks_a_schedule(st):
     """The first run of 18 or more steps times every main-kernel shape (4, 6,
     8 wavefronts per slice x 68 or 44 KiB of LDS; 4, 6 x 32 KiB), the three ADI kernels, the
