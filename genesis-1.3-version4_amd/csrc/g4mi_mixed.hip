@@ -385,8 +385,9 @@ struct OdeMix {
 
 // a + b rounded stochastically to one of the two neighbouring FP32 values,
 // with probability proportional to the distance (unbiased: E[result] = a + b
-// exactly).  The residual of round-to-nearest comes from TwoSum; u is a
-// uniform 24-bit draw.  Accumulating a small per-step increment into an
+// exactly, up to the draw's resolution).  The residual of round-to-nearest
+// comes from TwoSum; the top 24 bits of u are the uniform draw (the push
+// passes 16 random bits and zeros).  Accumulating a small per-step increment into an
 // O(1) offset with round-to-nearest repeats the same rounding error every
 // step; stochastic rounding makes it a zero-mean random walk instead.
 __device__ __forceinline__ float add_sr(float a, float b, unsigned u) {
@@ -746,11 +747,13 @@ __device__ bool main_step(const MixArrays &a, const int is, int sc_on, float2 *d
         o.eval(g0 + eg, ph1 + (t0 + et));
         eg += stpz * (k3gg + o.kgg * (1.f / 6.f));
         et += stpz * (k3pp + o.kpp * (1.f / 6.f));
+        // one hash for both roundings, 16 bits each (the rounding
+        // probability then resolves 2^-16 of an ulp)
         const unsigned hs = mix_hash((unsigned long long)i, stepno, 0x51u);
-        const float dg = add_sr(g0, eg, hs);
+        const float dg = add_sr(g0, eg, hs << 16);
         if constexpr (EG) fv[NV - 1] += eg;
         if (chic) et += r56 * (dg + dgam0);  // TrackBeam::applyR56
-        const float dth = add_sr(t0, et, mix_hash((unsigned long long)i, stepno, 0xA7u));
+        const float dth = add_sr(t0, et, hs & 0xFFFF0000u);
         transverse32(t2, gref + dg, x, y, px, py);
         a.dg[i] = dg;
         a.dth[i] = dth;
