@@ -308,11 +308,24 @@ __device__ void front_finish(const MixArrays &a, int is, int moments, int sc, do
         a.cen[2 * is + 1] = yc;
     }
     if (!sc) return;  // uniform
-    // largest radius from the centroid; each lane rereads what it wrote
+    // largest radius from the centroid; each lane rereads what it wrote, four
+    // particles per pass with the loads first (a max, so the order is free)
     double m = 0;
-    for (long long i = b0 + t; i < b1; i += NT) {
-        const double tx = (double)a.x[i] - xc, ty = (double)a.y[i] - yc;
-        m = fmax(m, tx * tx + ty * ty);
+    for (long long i0 = b0 + t; i0 < b1; i0 += 4 * NT) {
+        float x4[4], y4[4];
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            const long long i = i0 + (long long)u * NT;
+            const bool in = i < b1;
+            x4[u] = in ? a.x[i] : (float)xc;
+            y4[u] = in ? a.y[i] : (float)yc;
+        }
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            if (i0 + (long long)u * NT >= b1) break;
+            const double tx = (double)x4[u] - xc, ty = (double)y4[u] - yc;
+            m = fmax(m, tx * tx + ty * ty);
+        }
     }
     m = bmax_w<NT>(m, lds);
     if (t == 0) a.rbound[is] = sqrt(m);
@@ -512,26 +525,31 @@ __device__ bool main_step(const MixArrays &a, const int is, int sc_on, float2 *d
                 __syncthreads();
                 for (int k = t; k < (NT / 64) * nsub * ngr; k += NT) hs[k] = make_double2(0, 0);
                 __syncthreads();
-                float nx = 0, ny = 0, nt = 0;
-                if (b0 + t < b1) {
-                    nx = a.x[b0 + t];
-                    ny = a.y[b0 + t];
-                    nt = a.dth[b0 + t];
-                }
-                for (long long i = b0 + t; i < b1; i += NT) {
-                    const float tx = nx - xc, ty = ny - yc, th = nt;
-                    if (i + NT < b1) {
-                        nx = a.x[i + NT];
-                        ny = a.y[i + NT];
-                        nt = a.dth[i + NT];
+                // four particles per lane and pass, loads first: the body is
+                // short, so one particle in flight per lane would leave the
+                // pass waiting on HBM latency
+                for (long long i0 = b0 + t; i0 < b1; i0 += 4 * NT) {
+                    float px4[4], py4[4], pt4[4];
+#pragma unroll
+                    for (int u = 0; u < 4; u++) {
+                        const long long i = i0 + (long long)u * NT;
+                        const bool in = i < b1;
+                        px4[u] = in ? a.x[i] : 0.f;
+                        py4[u] = in ? a.y[i] : 0.f;
+                        pt4[u] = in ? a.dth[i] : 0.f;
                     }
-                    const float radi = fsqrt(tx * tx + ty * ty), ir = frcp(radi);
-                    const int bin = sc_bin(radi * idr, ngr);
-                    float s, c;
-                    __sincosf((float)l * (ph0 + th), &s, &c);
-                    const float2 v = fmul(cw_pow(f2(tx * ir, ty * ir), -m), f2(c, -s));
-                    unsafeAtomicAdd(&myh[bin].x, (double)v.x);
-                    unsafeAtomicAdd(&myh[bin].y, (double)v.y);
+#pragma unroll
+                    for (int u = 0; u < 4; u++) {
+                        if (i0 + (long long)u * NT >= b1) break;
+                        const float tx = px4[u] - xc, ty = py4[u] - yc, th = pt4[u];
+                        const float radi = fsqrt(tx * tx + ty * ty), ir = frcp(radi);
+                        const int bin = sc_bin(radi * idr, ngr);
+                        float s, c;
+                        __sincosf((float)l * (ph0 + th), &s, &c);
+                        const float2 v = fmul(cw_pow(f2(tx * ir, ty * ir), -m), f2(c, -s));
+                        unsafeAtomicAdd(&myh[bin].x, (double)v.x);
+                        unsafeAtomicAdd(&myh[bin].y, (double)v.y);
+                    }
                 }
                 __syncthreads();
                 // constructLaplaceOperator (:176-190): coefficients and source of
