@@ -483,12 +483,27 @@ __device__ bool main_step(const MixArrays &a, const int is, int sc_on, float2 *d
     double esc = 0;
     if (S.longrange) {
         double e = 0;
-        for (int j = t; j < ns; j += NT) {
-            const double sz = a.mom_fsize[S.fbuf][j];
-            const double ds = (double)(j - is) * S.lsc_ds;
-            const double coef = 1 - sqrt(ds * ds / (ds * ds + sz));
-            const double sgn = (j > is) ? -1. : ((j < is) ? 1. : 0.);
-            e += sgn * coef * S.lsc_scl * a.cur[j] / sz;
+        const double *fsz = a.mom_fsize[S.fbuf];
+        // four slices per lane and pass, loads first (the same order of sums)
+        for (int j0 = t; j0 < ns; j0 += 4 * NT) {
+            double sz4[4], cu4[4];
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+                const int j = j0 + u * NT;
+                const bool in = j < ns;
+                sz4[u] = in ? fsz[j] : 1.0;
+                cu4[u] = in ? a.cur[j] : 0.0;
+            }
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+                const int j = j0 + u * NT;
+                if (j >= ns) break;
+                const double sz = sz4[u];
+                const double ds = (double)(j - is) * S.lsc_ds;
+                const double coef = 1 - sqrt(ds * ds / (ds * ds + sz));
+                const double sgn = (j > is) ? -1. : ((j < is) ? 1. : 0.);
+                e += sgn * coef * S.lsc_scl * cu4[u] / sz;
+            }
         }
         double v[1] = {e};
         bsum_w<NT, 1>(v, wred);
