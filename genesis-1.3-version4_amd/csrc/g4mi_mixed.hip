@@ -214,6 +214,8 @@ __device__ __forceinline__ double bmax_w(double v, double *lds) {
 }
 
 constexpr int kFrontNT = 256;
+// loads in flight per lane in the short particle and slice passes
+constexpr int kPassU = 8;
 template <int NT>
 __device__ void front_finish(const MixArrays &a, int is, int moments, int sc, double *lds, double (&v)[4], int bx0,
                              int by0, int bx1, int by1, double *fsize_out);
@@ -308,20 +310,20 @@ __device__ void front_finish(const MixArrays &a, int is, int moments, int sc, do
         a.cen[2 * is + 1] = yc;
     }
     if (!sc) return;  // uniform
-    // largest radius from the centroid; each lane rereads what it wrote, four
-    // particles per pass with the loads first (a max, so the order is free)
+    // largest radius from the centroid; each lane rereads what it wrote,
+    // kPassU particles per pass with the loads first (a max, so the order is free)
     double m = 0;
-    for (long long i0 = b0 + t; i0 < b1; i0 += 4 * NT) {
-        float x4[4], y4[4];
+    for (long long i0 = b0 + t; i0 < b1; i0 += kPassU * NT) {
+        float x4[kPassU], y4[kPassU];
 #pragma unroll
-        for (int u = 0; u < 4; u++) {
+        for (int u = 0; u < kPassU; u++) {
             const long long i = i0 + (long long)u * NT;
             const bool in = i < b1;
             x4[u] = in ? a.x[i] : (float)xc;
             y4[u] = in ? a.y[i] : (float)yc;
         }
 #pragma unroll
-        for (int u = 0; u < 4; u++) {
+        for (int u = 0; u < kPassU; u++) {
             if (i0 + (long long)u * NT >= b1) break;
             const double tx = (double)x4[u] - xc, ty = (double)y4[u] - yc;
             m = fmax(m, tx * tx + ty * ty);
@@ -484,18 +486,18 @@ __device__ bool main_step(const MixArrays &a, const int is, int sc_on, float2 *d
     if (S.longrange) {
         double e = 0;
         const double *fsz = a.mom_fsize[S.fbuf];
-        // four slices per lane and pass, loads first (the same order of sums)
-        for (int j0 = t; j0 < ns; j0 += 4 * NT) {
-            double sz4[4], cu4[4];
+        // kPassU slices per lane and pass, loads first (the same order of sums)
+        for (int j0 = t; j0 < ns; j0 += kPassU * NT) {
+            double sz4[kPassU], cu4[kPassU];
 #pragma unroll
-            for (int u = 0; u < 4; u++) {
+            for (int u = 0; u < kPassU; u++) {
                 const int j = j0 + u * NT;
                 const bool in = j < ns;
                 sz4[u] = in ? fsz[j] : 1.0;
                 cu4[u] = in ? a.cur[j] : 0.0;
             }
 #pragma unroll
-            for (int u = 0; u < 4; u++) {
+            for (int u = 0; u < kPassU; u++) {
                 const int j = j0 + u * NT;
                 if (j >= ns) break;
                 const double sz = sz4[u];
@@ -540,13 +542,13 @@ __device__ bool main_step(const MixArrays &a, const int is, int sc_on, float2 *d
                 __syncthreads();
                 for (int k = t; k < (NT / 64) * nsub * ngr; k += NT) hs[k] = make_double2(0, 0);
                 __syncthreads();
-                // four particles per lane and pass, loads first: the body is
+                // kPassU particles per lane and pass, loads first: the body is
                 // short, so one particle in flight per lane would leave the
                 // pass waiting on HBM latency
-                for (long long i0 = b0 + t; i0 < b1; i0 += 4 * NT) {
-                    float px4[4], py4[4], pt4[4];
+                for (long long i0 = b0 + t; i0 < b1; i0 += kPassU * NT) {
+                    float px4[kPassU], py4[kPassU], pt4[kPassU];
 #pragma unroll
-                    for (int u = 0; u < 4; u++) {
+                    for (int u = 0; u < kPassU; u++) {
                         const long long i = i0 + (long long)u * NT;
                         const bool in = i < b1;
                         px4[u] = in ? a.x[i] : 0.f;
@@ -554,7 +556,7 @@ __device__ bool main_step(const MixArrays &a, const int is, int sc_on, float2 *d
                         pt4[u] = in ? a.dth[i] : 0.f;
                     }
 #pragma unroll
-                    for (int u = 0; u < 4; u++) {
+                    for (int u = 0; u < kPassU; u++) {
                         if (i0 + (long long)u * NT >= b1) break;
                         const float tx = px4[u] - xc, ty = py4[u] - yc, th = pt4[u];
                         const float radi = fsqrt(tx * tx + ty * ty), ir = frcp(radi);
