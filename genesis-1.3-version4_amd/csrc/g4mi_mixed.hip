@@ -233,42 +233,44 @@ __device__ void front_body(const MixArrays &a, int is, int moments, int sc, doub
     const int ng = a.ng;
     int bx0 = 1 << 30, by0 = 1 << 30, bx1 = -1, by1 = -1;
     const TransParams t1 = S.t1;  // in registers: the stores below may alias the step table
-    // the next particle's coordinates are loaded before this one's stores,
-    // so two particles' loads are in flight per lane
-    float nx = 0, ny = 0, npx = 0, npy = 0, ndg = 0;
-    if (b0 + t < b1) {
-        nx = a.x[b0 + t];
-        ny = a.y[b0 + t];
-        npx = a.px[b0 + t];
-        npy = a.py[b0 + t];
-        ndg = a.dg[b0 + t];
-    }
-    for (long long i = b0 + t; i < b1; i += NT) {
-        float x = nx, y = ny, px = npx, py = npy;
-        const float gam = gref + ndg;
-        if (i + NT < b1) {
-            nx = a.x[i + NT];
-            ny = a.y[i + NT];
-            npx = a.px[i + NT];
-            npy = a.py[i + NT];
-            ndg = a.dg[i + NT];
+    // kPassU particles per lane and pass, all their loads before the first
+    // store: the body is short, so fewer loads in flight would leave the
+    // pass waiting on HBM latency (the sums keep the particle order)
+    for (long long i0 = b0 + t; i0 < b1; i0 += kPassU * NT) {
+        float x4[kPassU], y4[kPassU], px4[kPassU], py4[kPassU], g4[kPassU];
+#pragma unroll
+        for (int u = 0; u < kPassU; u++) {
+            const long long i = i0 + (long long)u * NT;
+            const bool in = i < b1;
+            x4[u] = in ? a.x[i] : 0.f;
+            y4[u] = in ? a.y[i] : 0.f;
+            px4[u] = in ? a.px[i] : 0.f;
+            py4[u] = in ? a.py[i] : 0.f;
+            g4[u] = in ? a.dg[i] : 0.f;
         }
-        transverse32(t1, gam, x, y, px, py);
-        a.x[i] = x;
-        a.y[i] = y;
-        a.px[i] = px;
-        a.py[i] = py;
-        v[0] += x;
-        v[1] += (double)x * x;
-        v[2] += y;
-        v[3] += (double)y * y;
-        if ((x > -gridmax) && (x < gridmax) && (y > -gridmax) && (y < gridmax)) {
-            const int ix = min((int)floorf((x + gridmax) * inv_dgrid), ng - 2);
-            const int iy = min((int)floorf((y + gridmax) * inv_dgrid), ng - 2);
-            bx0 = min(bx0, ix);
-            bx1 = max(bx1, ix + 1);
-            by0 = min(by0, iy);
-            by1 = max(by1, iy + 1);
+#pragma unroll
+        for (int u = 0; u < kPassU; u++) {
+            const long long i = i0 + (long long)u * NT;
+            if (i >= b1) break;
+            float x = x4[u], y = y4[u], px = px4[u], py = py4[u];
+            const float gam = gref + g4[u];
+            transverse32(t1, gam, x, y, px, py);
+            a.x[i] = x;
+            a.y[i] = y;
+            a.px[i] = px;
+            a.py[i] = py;
+            v[0] += x;
+            v[1] += (double)x * x;
+            v[2] += y;
+            v[3] += (double)y * y;
+            if ((x > -gridmax) && (x < gridmax) && (y > -gridmax) && (y < gridmax)) {
+                const int ix = min((int)floorf((x + gridmax) * inv_dgrid), ng - 2);
+                const int iy = min((int)floorf((y + gridmax) * inv_dgrid), ng - 2);
+                bx0 = min(bx0, ix);
+                bx1 = max(bx1, ix + 1);
+                by0 = min(by0, iy);
+                by1 = max(by1, iy + 1);
+            }
         }
     }
     front_finish<NT>(a, is, moments, sc, lds, v, bx0, by0, bx1, by1, fsize_out);
